@@ -1,0 +1,193 @@
+"""Device operator handles over the C ABI (include/hymor.h).
+
+``DeviceOperator`` replaces the reference's ``CuSparseMatrixCSC(A*dt)``
+uploads (utils/Stability_analysis/Eigenvalues/EIGENVALUES.jl:139,
+Transient_growth_downstream/TRANSIENT_GROWTH_DOWNSTREAM.jl:189-191) and the
+Taylor loops that run on them.  Vectors are in the reference's variable-major
+order; complex vectors are numpy complex128 (interleaved re/im, as Julia's
+ComplexF64).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import C128, F64, FORM_NORMALISED, FORM_POWERS, PREC_F32, PREC_F64, check, ptr
+
+_FORMS = {"normalised": FORM_NORMALISED, "powers": FORM_POWERS}
+_PRECS = {"f64": PREC_F64, "f32": PREC_F32}
+
+
+def _vec(x, complex_op: bool):
+    x = np.asarray(x)
+    if complex_op or np.iscomplexobj(x):
+        return np.ascontiguousarray(x, dtype=np.complex128), C128
+    return np.ascontiguousarray(x, dtype=np.float64), F64
+
+
+class DeviceOperator:
+    """A linearised operator resident on the GPU (forward and adjoint)."""
+
+    def __init__(self, handle: C.c_void_p, keep=None):
+        self._h = handle
+        info = (C.c_int64 * 10)()
+        check(_lib.load().hymor_op_info(self._h, info, 10))
+        (self.n, self.b, self.ncell, self.nnzb, self.nnzb_padded, self.nnz_extra,
+         self.extra_rows, cplx, prec, self.device_bytes) = [int(v) for v in info]
+        self.is_complex = bool(cplx)
+        self.value_prec = "f32" if prec == PREC_F32 else "f64"
+
+    # ------------------------------------------------------------ creation
+    @classmethod
+    def from_csc(cls, n, colptr, rowval, nzval, *, index_base=1, scale=1.0, vars_per_cell=4,
+                 nchi, neta, value_prec="f64"):
+        lib = _lib.load()
+        colptr = np.ascontiguousarray(colptr, dtype=np.int64)
+        rowval = np.ascontiguousarray(rowval, dtype=np.int64)
+        cplx = np.iscomplexobj(nzval)
+        nzval = np.ascontiguousarray(nzval, dtype=np.complex128 if cplx else np.float64)
+        h = C.c_void_p()
+        check(lib.hymor_op_create_csc(n, ptr(colptr), ptr(rowval), ptr(nzval), index_base,
+                                      C128 if cplx else F64, float(scale), vars_per_cell, nchi,
+                                      neta, _PRECS[value_prec], C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_scipy(cls, A, *, nchi, neta, vars_per_cell=4, scale=1.0, value_prec="f64"):
+        """From a scipy sparse matrix in the reference's var-major order."""
+        A = A.tocsc()
+        A.sort_indices()
+        return cls.from_csc(A.shape[0], A.indptr, A.indices, A.data, index_base=0, scale=scale,
+                            vars_per_cell=vars_per_cell, nchi=nchi, neta=neta,
+                            value_prec=value_prec)
+
+    @classmethod
+    def from_blocks(cls, op, *, scale=1.0, value_prec="f64", with_adjoint=True):
+        """From a hymor_amd.operators.BlockOperator (cell-major blocks)."""
+        lib = _lib.load()
+        brow_ptr = np.ascontiguousarray(op.brow_ptr, dtype=np.int64)
+        bcol = np.ascontiguousarray(op.bcol, dtype=np.int32)
+        if np.iscomplexobj(op.bval) and np.any(op.bval.imag != 0):
+            raise ValueError("block values must be real; put imaginary parts in the extras")
+        bval = np.ascontiguousarray(np.real(op.bval), dtype=np.float64)
+        cplx = op.is_complex
+        xrow = np.ascontiguousarray(op.xrow, dtype=np.int64)
+        xcol = np.ascontiguousarray(op.xcol, dtype=np.int64)
+        xval = np.ascontiguousarray(op.xval, dtype=np.complex128 if cplx else np.float64)
+        h = C.c_void_p()
+        check(lib.hymor_op_create_bsr(op.nchi, op.neta, op.b, op.n_extra, ptr(brow_ptr), ptr(bcol),
+                                      ptr(bval), xrow.shape[0], ptr(xrow), ptr(xcol), ptr(xval),
+                                      C128 if cplx else F64, float(scale), _PRECS[value_prec],
+                                      1 if with_adjoint else 0, C.byref(h)))
+        return cls(h)
+
+    # ------------------------------------------------------------ lifetime
+    def close(self):
+        if self._h is not None and self._h.value:
+            check(_lib.load().hymor_op_destroy(self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ------------------------------------------------------------ actions
+    def spmv(self, x, transpose=False):
+        x, dt = _vec(x, self.is_complex)
+        y = np.empty_like(x)
+        check(_lib.load().hymor_spmv(self._h, int(bool(transpose)), ptr(x), ptr(y), dt))
+        return y
+
+    def expm_taylor(self, x, *, order=5, nsub=1, dt=1.0, form="powers", transpose=False):
+        """T_order(A*dt)^nsub x (transpose: the adjoint operator)."""
+        x, dty = _vec(x, self.is_complex)
+        y = np.empty_like(x)
+        check(_lib.load().hymor_expm_taylor(self._h, int(bool(transpose)), int(order), int(nsub),
+                                            float(dt), _FORMS[form], ptr(x), ptr(y), dty))
+        return y
+
+    def chu_sandwich(self, weight: "DeviceOperator", x, *, order=5, nsub=1, dt=1.0, n_set0=0):
+        """T(A'dt)^n * C * T(A dt)^n * x with CONSTRUCT_C's zeroing."""
+        x, dty = _vec(x, self.is_complex)
+        y = np.empty_like(x)
+        check(_lib.load().hymor_chu_sandwich(self._h, weight._h, int(order), int(nsub), float(dt),
+                                             int(n_set0), ptr(x), ptr(y), dty))
+        return y
+
+    def synchronize(self):
+        check(_lib.load().hymor_op_synchronize(self._h))
+
+    def bench_term_kernel(self, reps=20, transpose=False, nv=None):
+        nv = nv or (2 if self.is_complex else 1)
+        ms = C.c_double()
+        by = C.c_double()
+        check(_lib.load().hymor_bench_term_kernel(self._h, int(bool(transpose)), nv, reps,
+                                                  C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+
+class DeviceVector:
+    """A var-major vector in device memory, for the *_dev entry points."""
+
+    def __init__(self, n: int, dtype=np.float64):
+        self.n = n
+        self.dtype = np.dtype(dtype)
+        self.nbytes = n * self.dtype.itemsize
+        self._p = C.c_void_p()
+        check(_lib.load().hymor_dev_alloc(self.nbytes, C.byref(self._p)))
+
+    @classmethod
+    def from_host(cls, x):
+        x = np.ascontiguousarray(x)
+        d = cls(x.shape[0], x.dtype)
+        d.upload(x)
+        return d
+
+    def upload(self, x):
+        x = np.ascontiguousarray(x, dtype=self.dtype)
+        check(_lib.load().hymor_memcpy_h2d(self._p, ptr(x), self.nbytes))
+
+    def download(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        check(_lib.load().hymor_memcpy_d2h(ptr(out), self._p, self.nbytes))
+        return out
+
+    @property
+    def ptr(self):
+        return self._p
+
+    @property
+    def abi_dtype(self):
+        return C128 if self.dtype == np.complex128 else F64
+
+    def free(self):
+        if self._p is not None and self._p.value:
+            check(_lib.load().hymor_dev_free(self._p))
+        self._p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def expm_taylor_dev(op: DeviceOperator, x: DeviceVector, y: DeviceVector, *, order=5, nsub=1,
+                    dt=1.0, form="powers", transpose=False):
+    """Asynchronous device-resident action; call op.synchronize() to wait."""
+    check(_lib.load().hymor_expm_taylor_dev(op.handle, int(bool(transpose)), int(order), int(nsub),
+                                            float(dt), _FORMS[form], x.ptr, y.ptr, x.abi_dtype))

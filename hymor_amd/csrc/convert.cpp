@@ -1,0 +1,239 @@
+// Host-side layout conversion: the reference's variable-major CSC
+// (a Julia SparseMatrixCSC from LINEARIZE_EQUATIONS_NO_DISCONTINUITY.jl:276-285)
+// into cell-major block storage for the HIP kernels.
+//
+// Index map: var-major global index vm = k*ncell + c (variable k of cell c)
+// for vm < b*ncell; cell-major index cm = c*b + k.  Extra DOFs keep their
+// index.  Entries whose row and column are both flow unknowns become b x b
+// blocks (real part); everything else goes to the "extra" CSR.
+#include "hymor_internal.hpp"
+
+#include <algorithm>
+#include <numeric>
+#include <stdexcept>
+
+namespace hymor {
+
+namespace {
+
+struct BEnt {
+    int64_t cell_r;
+    int32_t i, j;
+    double v;
+};
+
+inline int64_t vm_to_cm(int64_t vm, int64_t ncell, int b, int64_t nflow) {
+    return vm < nflow ? (vm % ncell) * b + vm / ncell : vm;
+}
+
+}  // namespace
+
+void csc_to_blocks(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                   const double *nzval, int index_base, bool is_complex, double scale,
+                   int b, int64_t ncell, HostBSR &fwd, HostBSR &adj,
+                   HostExtra &xf, HostExtra &xa) {
+    const int64_t nflow = (int64_t)b * ncell;
+    const int bb = b * b;
+    const int64_t base = index_base;
+    // basic validation of the CSC arrays
+    if (colptr[0] != base) throw std::invalid_argument("colptr[0] must equal the index base");
+    for (int64_t j = 0; j < n; ++j)
+        if (colptr[j + 1] < colptr[j]) throw std::invalid_argument("colptr is not monotone");
+    const int64_t nnz = colptr[n] - base;
+    for (int64_t k = 0; k < nnz; ++k)
+        if (rowval[k] < base || rowval[k] >= n + base)
+            throw std::invalid_argument("rowval out of range");
+
+    std::vector<int64_t> xr, xc;
+    std::vector<double> xv;
+    const int vstride = is_complex ? 2 : 1;
+
+    // block-compressed columns (BSC) over cell columns
+    std::vector<int64_t> bsc_ptr(ncell + 1, 0);
+    std::vector<int64_t> bsc_row;
+    std::vector<double> bsc_val;
+    std::vector<BEnt> tmp;
+    for (int64_t cc = 0; cc < ncell; ++cc) {
+        tmp.clear();
+        for (int k = 0; k < b; ++k) {
+            const int64_t vmc = (int64_t)k * ncell + cc;
+            const int64_t cmc = cc * b + k;
+            for (int64_t e = colptr[vmc] - base; e < colptr[vmc + 1] - base; ++e) {
+                const int64_t vmr = rowval[e] - base;
+                const int64_t cmr = vm_to_cm(vmr, ncell, b, nflow);
+                const double re = nzval[e * vstride] * scale;
+                const double im = is_complex ? nzval[e * vstride + 1] * scale : 0.0;
+                if (cmr < nflow) {
+                    tmp.push_back({cmr / b, (int32_t)(cmr % b), k, re});
+                    if (is_complex && im != 0.0) {
+                        xr.push_back(cmr); xc.push_back(cmc);
+                        xv.push_back(0.0); xv.push_back(im);
+                    }
+                } else {
+                    xr.push_back(cmr); xc.push_back(cmc);
+                    xv.push_back(re);
+                    if (is_complex) xv.push_back(im);
+                }
+            }
+        }
+        std::stable_sort(tmp.begin(), tmp.end(),
+                         [](const BEnt &a, const BEnt &c) { return a.cell_r < c.cell_r; });
+        int64_t last = -1;
+        for (const BEnt &e : tmp) {
+            if (e.cell_r != last) {
+                bsc_row.push_back(e.cell_r);
+                bsc_val.resize(bsc_val.size() + bb, 0.0);
+                last = e.cell_r;
+            }
+            // duplicates are summed, as sparse(I,J,V) does
+            bsc_val[bsc_val.size() - bb + e.i * b + e.j] += e.v;
+        }
+        bsc_ptr[cc + 1] = (int64_t)bsc_row.size();
+    }
+    // columns of extra DOFs: every entry is extra
+    for (int64_t vmc = nflow; vmc < n; ++vmc) {
+        for (int64_t e = colptr[vmc] - base; e < colptr[vmc + 1] - base; ++e) {
+            const int64_t vmr = rowval[e] - base;
+            xr.push_back(vm_to_cm(vmr, ncell, b, nflow));
+            xc.push_back(vmc);
+            xv.push_back(nzval[e * vstride] * scale);
+            if (is_complex) xv.push_back(nzval[e * vstride + 1] * scale);
+        }
+    }
+
+    // adjoint BSR = BSC with each block transposed (real parts: no conjugate)
+    const int64_t nnzb = (int64_t)bsc_row.size();
+    adj.b = b;
+    adj.ncell = ncell;
+    adj.ptr = bsc_ptr;
+    adj.col.resize(nnzb);
+    adj.val.resize((size_t)nnzb * bb);
+    for (int64_t k = 0; k < nnzb; ++k) {
+        adj.col[k] = (int32_t)bsc_row[k];
+        for (int i = 0; i < b; ++i)
+            for (int j = 0; j < b; ++j)
+                adj.val[(size_t)k * bb + j * b + i] = bsc_val[(size_t)k * bb + i * b + j];
+    }
+    // forward BSR = BSC regrouped by block row (stable => columns ascending)
+    fwd.b = b;
+    fwd.ncell = ncell;
+    fwd.ptr.assign(ncell + 1, 0);
+    for (int64_t k = 0; k < nnzb; ++k) fwd.ptr[bsc_row[k] + 1]++;
+    for (int64_t c = 0; c < ncell; ++c) fwd.ptr[c + 1] += fwd.ptr[c];
+    fwd.col.resize(nnzb);
+    fwd.val.resize((size_t)nnzb * bb);
+    std::vector<int64_t> pos(fwd.ptr.begin(), fwd.ptr.end() - 1);
+    for (int64_t cc = 0; cc < ncell; ++cc) {
+        for (int64_t k = bsc_ptr[cc]; k < bsc_ptr[cc + 1]; ++k) {
+            const int64_t d = pos[bsc_row[k]]++;
+            fwd.col[d] = (int32_t)cc;
+            std::copy(bsc_val.begin() + (size_t)k * bb, bsc_val.begin() + (size_t)(k + 1) * bb,
+                      fwd.val.begin() + (size_t)d * bb);
+        }
+    }
+    extra_from_coo(n, nflow, xr, xc, xv, is_complex, false, xf);
+    extra_from_coo(n, nflow, xr, xc, xv, is_complex, true, xa);
+}
+
+void bsr_transpose(const HostBSR &a, HostBSR &t) {
+    const int b = a.b, bb = b * b;
+    const int64_t ncell = a.ncell, nnzb = a.nnzb();
+    t.b = b;
+    t.ncell = ncell;
+    t.ptr.assign(ncell + 1, 0);
+    for (int64_t k = 0; k < nnzb; ++k) t.ptr[(int64_t)a.col[k] + 1]++;
+    for (int64_t c = 0; c < ncell; ++c) t.ptr[c + 1] += t.ptr[c];
+    t.col.resize(nnzb);
+    t.val.resize((size_t)nnzb * bb);
+    std::vector<int64_t> pos(t.ptr.begin(), t.ptr.end() - 1);
+    for (int64_t r = 0; r < ncell; ++r) {
+        for (int64_t k = a.ptr[r]; k < a.ptr[r + 1]; ++k) {
+            const int64_t d = pos[a.col[k]]++;
+            t.col[d] = (int32_t)r;
+            for (int i = 0; i < b; ++i)
+                for (int j = 0; j < b; ++j)
+                    t.val[(size_t)d * bb + j * b + i] = a.val[(size_t)k * bb + i * b + j];
+        }
+    }
+}
+
+void extra_from_coo(int64_t n, int64_t nflow, const std::vector<int64_t> &r,
+                    const std::vector<int64_t> &c, const std::vector<double> &v,
+                    bool is_complex, bool adjoint, HostExtra &out) {
+    const int vs = is_complex ? 2 : 1;
+    const int64_t m = (int64_t)r.size();
+    const std::vector<int64_t> &rr = adjoint ? c : r;
+    const std::vector<int64_t> &cc = adjoint ? r : c;
+    std::vector<int64_t> idx(m);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b2) {
+        return rr[a] != rr[b2] ? rr[a] < rr[b2] : cc[a] < cc[b2];
+    });
+    out.is_complex = is_complex;
+    out.rows.clear();
+    out.ptr.assign(1, 0);
+    out.col.clear();
+    out.val.clear();
+    out.col.reserve(m);
+    out.val.reserve((size_t)m * vs);
+    // merge entry rows with every extra-DOF row (those always need a result)
+    int64_t e = 0;
+    int64_t next_extra = nflow;
+    while (e < m || next_extra < n) {
+        const int64_t re = e < m ? rr[idx[e]] : INT64_MAX;
+        const int64_t row = std::min(re, next_extra < n ? next_extra : INT64_MAX);
+        if (row == next_extra) ++next_extra;
+        while (e < m && rr[idx[e]] == row) {
+            const int64_t k = idx[e];
+            out.col.push_back((int32_t)cc[k]);
+            out.val.push_back(v[(size_t)k * vs]);
+            if (is_complex) out.val.push_back(adjoint ? -v[(size_t)k * vs + 1] : v[(size_t)k * vs + 1]);
+            ++e;
+        }
+        out.rows.push_back(row);
+        out.ptr.push_back((int64_t)out.col.size());
+    }
+}
+
+void bsr_to_sliced(const HostBSR &a, HostSliced &s) {
+    const int b = a.b, bb = b * b;
+    const int cps = 64 / b;
+    s.b = b;
+    s.cps = cps;
+    s.ncell = a.ncell;
+    s.nslice = (a.ncell + cps - 1) / cps;
+    s.nnzb = a.nnzb();
+    s.slice_ptr.assign(s.nslice + 1, 0);
+    for (int64_t sl = 0; sl < s.nslice; ++sl) {
+        int64_t w = 0;
+        for (int q = 0; q < cps; ++q) {
+            const int64_t cell = sl * cps + q;
+            if (cell < a.ncell) w = std::max(w, a.ptr[cell + 1] - a.ptr[cell]);
+        }
+        s.slice_ptr[sl + 1] = s.slice_ptr[sl] + w * cps;
+    }
+    const int64_t tot = s.slice_ptr[s.nslice];
+    s.col.assign(tot, 0);
+    s.val.assign((size_t)tot * bb, 0.0);
+    for (int64_t sl = 0; sl < s.nslice; ++sl) {
+        const int64_t w = (s.slice_ptr[sl + 1] - s.slice_ptr[sl]) / cps;
+        for (int q = 0; q < cps; ++q) {
+            const int64_t cell = sl * cps + q;
+            const int64_t len = cell < a.ncell ? a.ptr[cell + 1] - a.ptr[cell] : 0;
+            const int32_t self = (int32_t)(cell < a.ncell ? cell : 0);
+            for (int64_t t = 0; t < w; ++t) {
+                const int64_t d = s.slice_ptr[sl] + t * cps + q;
+                if (t < len) {
+                    const int64_t k = a.ptr[cell] + t;
+                    s.col[d] = a.col[k];
+                    std::copy(a.val.begin() + (size_t)k * bb, a.val.begin() + (size_t)(k + 1) * bb,
+                              s.val.begin() + (size_t)d * bb);
+                } else {
+                    s.col[d] = self;
+                }
+            }
+        }
+    }
+}
+
+}  // namespace hymor

@@ -1,0 +1,655 @@
+// libhymor_hip: C ABI, operator handles and launch logic.
+//
+// Reference path being replaced (all in utils/Stability_analysis/):
+//   Eigenvalues/ARNOLDI_POWER_GPU.jl:36-82        POWER_METHOD! + adapter
+//   Transient_growth_downstream/CONSTRUCT_C.jl:33-111  EXPONENTIAL! + zeroing
+//   cuSPARSE SpMV on CuSparseMatrixCSC (mul!, ARNOLDI_POWER_GPU.jl:73)
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hymor.h"
+#include "hymor_internal.hpp"
+#include "kernels.hpp"
+
+using namespace hymor;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct HipError : std::runtime_error {
+    explicit HipError(const std::string &m) : std::runtime_error(m) {}
+};
+struct ArgError : std::invalid_argument {
+    explicit ArgError(const std::string &m) : std::invalid_argument(m) {}
+};
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_) + " (line " + \
+                           std::to_string(__LINE__) + ")");                            \
+    } while (0)
+
+#define ARG_CHECK(cond, msg)              \
+    do {                                  \
+        if (!(cond)) throw ArgError(msg); \
+    } while (0)
+
+struct DevSliced {
+    int b = 0, cps = 0, prec = HYMOR_PREC_F64;
+    int64_t ncell = 0, nslice = 0, nnzb = 0, nnzb_pad = 0;
+    int64_t *slice_ptr = nullptr;
+    int32_t *col = nullptr;
+    void *val = nullptr;
+};
+
+struct DevExtra {
+    bool is_complex = false;
+    int64_t nrows = 0, nnz = 0;
+    int64_t *rows = nullptr, *ptr = nullptr;
+    int32_t *col = nullptr;
+    double *val = nullptr;
+};
+
+}  // namespace
+
+struct hymor_op {
+    int64_t n = 0, nchi = 0, neta = 0, ncell = 0, nflow = 0;
+    int b = 0, prec = HYMOR_PREC_F64;
+    bool is_complex = false, has_adj = false;
+    DevSliced fwd, adj;
+    DevExtra xf, xa;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // workspace: [0] stage (var-major), [1] in0, [2] va, [3] vb, [4] t0, [5] t1, [6] spare
+    int ws_nv = 0;
+    double *ws[7] = {};
+    double *yx = nullptr;
+    int64_t dev_bytes = 0;
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+void *dalloc(hymor_op *op, size_t bytes) {
+    void *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw std::bad_alloc();
+    }
+    op->allocs.push_back(p);
+    op->dev_bytes += (int64_t)bytes;
+    return p;
+}
+
+template <typename T>
+T *upload(hymor_op *op, const std::vector<T> &h) {
+    T *d = (T *)dalloc(op, h.size() * sizeof(T));
+    if (!h.empty()) HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+void upload_sliced(hymor_op *op, const HostBSR &a, int prec, DevSliced &d) {
+    HostSliced s;
+    bsr_to_sliced(a, s);
+    d.b = s.b;
+    d.cps = s.cps;
+    d.prec = prec;
+    d.ncell = s.ncell;
+    d.nslice = s.nslice;
+    d.nnzb = s.nnzb;
+    d.nnzb_pad = s.slice_ptr[s.nslice];
+    d.slice_ptr = upload(op, s.slice_ptr);
+    d.col = upload(op, s.col);
+    if (prec == HYMOR_PREC_F32) {
+        std::vector<float> f(s.val.size());
+        for (size_t i = 0; i < f.size(); ++i) f[i] = (float)s.val[i];
+        d.val = upload(op, f);
+    } else {
+        d.val = upload(op, s.val);
+    }
+}
+
+void upload_extra(hymor_op *op, const HostExtra &h, DevExtra &d) {
+    d.is_complex = h.is_complex;
+    d.nrows = (int64_t)h.rows.size();
+    d.nnz = (int64_t)h.col.size();
+    d.rows = upload(op, h.rows);
+    d.ptr = upload(op, h.ptr);
+    d.col = upload(op, h.col);
+    d.val = upload(op, h.val);
+}
+
+void release(hymor_op *op) {
+    if (!op) return;
+    for (void *p : op->allocs) (void)hipFree(p);
+    op->allocs.clear();
+    if (op->stream) (void)hipStreamDestroy(op->stream);
+    op->stream = nullptr;
+}
+
+void ensure_ws(hymor_op *op, int nv) {
+    if (op->ws_nv >= nv) return;
+    // free the old set (tracked in allocs): rebuild the list without them
+    for (int i = 0; i < 7; ++i) {
+        if (!op->ws[i]) continue;
+        for (size_t k = 0; k < op->allocs.size(); ++k)
+            if (op->allocs[k] == op->ws[i]) {
+                (void)hipFree(op->ws[i]);
+                op->allocs.erase(op->allocs.begin() + (long)k);
+                break;
+            }
+        op->ws[i] = nullptr;
+    }
+    if (op->yx) {
+        for (size_t k = 0; k < op->allocs.size(); ++k)
+            if (op->allocs[k] == op->yx) {
+                (void)hipFree(op->yx);
+                op->allocs.erase(op->allocs.begin() + (long)k);
+                break;
+            }
+        op->yx = nullptr;
+    }
+    const size_t vb = (size_t)op->n * nv * sizeof(double);
+    for (int i = 0; i < 7; ++i) {
+        op->ws[i] = (double *)dalloc(op, vb);
+        HIP_TRY(hipMemsetAsync(op->ws[i], 0, vb, op->stream));
+    }
+    const size_t yb = (size_t)op->nflow * nv * sizeof(double);
+    op->yx = (double *)dalloc(op, yb);
+    // rows without extra entries are never written: they must read as zero
+    HIP_TRY(hipMemsetAsync(op->yx, 0, yb, op->stream));
+    HIP_TRY(hipStreamSynchronize(op->stream));
+    op->ws_nv = nv;
+}
+
+// ---------------------------------------------------------------- launches
+
+template <int B, typename VT, int NV>
+void launch_term_t(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+                   double *tout, const double *vin, double *vout, double c1, double c2) {
+    const int wpb = s.nslice >= 8192 ? 4 : 1;
+    const int64_t grid = (s.nslice + wpb - 1) / wpb;
+    hipLaunchKernelGGL((k_term<B, VT, NV>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
+                       s.slice_ptr, s.col, (const VT *)s.val, s.nslice, s.ncell, x, yx, tout, vin,
+                       vout, c1, c2);
+}
+
+template <int NV>
+void launch_term_nv(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+                    double *tout, const double *vin, double *vout, double c1, double c2) {
+    const bool f32 = s.prec == HYMOR_PREC_F32;
+    if (s.b == 4) {
+        if (f32) launch_term_t<4, float, NV>(s, st, x, yx, tout, vin, vout, c1, c2);
+        else launch_term_t<4, double, NV>(s, st, x, yx, tout, vin, vout, c1, c2);
+    } else if (s.b == 8) {
+        if (f32) launch_term_t<8, float, NV>(s, st, x, yx, tout, vin, vout, c1, c2);
+        else launch_term_t<8, double, NV>(s, st, x, yx, tout, vin, vout, c1, c2);
+    } else {
+        throw ArgError("vars_per_cell must be 4 or 8");
+    }
+}
+
+template <int NV>
+void launch_extra_nv(const DevExtra &xe, int64_t nflow, hipStream_t st, const double *x,
+                     double *yx, double *tout, const double *vin, double *vout, double c1,
+                     double c2) {
+    const int wpb = 4;
+    const int64_t grid = (xe.nrows + wpb - 1) / wpb;
+    if (xe.is_complex)
+        hipLaunchKernelGGL((k_extra<NV, true>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
+                           xe.rows, xe.ptr, xe.col, xe.val, xe.nrows, nflow, x, yx, tout, vin,
+                           vout, c1, c2);
+    else
+        hipLaunchKernelGGL((k_extra<NV, false>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
+                           xe.rows, xe.ptr, xe.col, xe.val, xe.nrows, nflow, x, yx, tout, vin,
+                           vout, c1, c2);
+}
+
+// One Taylor term (or a plain SpMV when vout == nullptr) on `st`.
+// yx is the partial-sum buffer of the operator whose extras are applied.
+void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, double *yx, const double *x,
+                 double *tout, const double *vin, double *vout, double c1, double c2) {
+    const DevSliced &s = adj ? op->adj : op->fwd;
+    const DevExtra &xe = adj ? op->xa : op->xf;
+    const bool has_x = xe.nrows > 0;
+    if (has_x) {
+        if (nv == 1) launch_extra_nv<1>(xe, op->nflow, st, x, yx, tout, vin, vout, c1, c2);
+        else if (nv == 2) launch_extra_nv<2>(xe, op->nflow, st, x, yx, tout, vin, vout, c1, c2);
+        else throw ArgError("unsupported vector width");
+        HIP_TRY(hipGetLastError());
+    }
+    const double *yxp = has_x ? yx : nullptr;
+    if (op->ncell > 0) {
+        if (nv == 1) launch_term_nv<1>(s, st, x, yxp, tout, vin, vout, c1, c2);
+        else if (nv == 2) launch_term_nv<2>(s, st, x, yxp, tout, vin, vout, c1, c2);
+        else throw ArgError("unsupported vector width");
+        HIP_TRY(hipGetLastError());
+    }
+}
+
+void launch_perm_in(hymor_op *op, hipStream_t st, const double *x_vm, int64_t ld, double *out,
+                    int nrhs, int ncomp, int64_t nzero) {
+    const int64_t total = op->n * nrhs * ncomp;
+    const int64_t grid = std::min<int64_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_perm_in, dim3((unsigned)std::max<int64_t>(grid, 1)), dim3(256), 0, st,
+                       x_vm, ld, out, op->n, op->ncell, op->b, nrhs, ncomp, nzero);
+    HIP_TRY(hipGetLastError());
+}
+
+void launch_perm_out(hymor_op *op, hipStream_t st, const double *in, double *y_vm, int64_t ld,
+                     int nrhs, int ncomp, int64_t nzero) {
+    const int64_t total = op->n * nrhs * ncomp;
+    const int64_t grid = std::min<int64_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_perm_out, dim3((unsigned)std::max<int64_t>(grid, 1)), dim3(256), 0, st,
+                       in, y_vm, ld, op->n, op->ncell, op->b, nrhs, ncomp, nzero);
+    HIP_TRY(hipGetLastError());
+}
+
+// T_order(A*dt)^nsub applied to `in` (cell-major, internal buffer).  Returns
+// the buffer that holds the result (ws[2] or ws[3], or `in` if nsub == 0).
+const double *run_taylor(hymor_op *op, bool adj, int order, int64_t nsub, double dt, int form,
+                         int nv, const double *in) {
+    double *va = op->ws[2], *vb = op->ws[3], *t0 = op->ws[4], *t1 = op->ws[5];
+    const double *cur = in;
+    for (int64_t s = 0; s < nsub; ++s) {
+        double *nxt = (cur == va) ? vb : va;
+        const double *x = cur;
+        int64_t fact = 1;
+        for (int j = 1; j <= order; ++j) {
+            double c1, c2;
+            if (form == HYMOR_FORM_POWERS) {
+                fact *= j;
+                c1 = dt;
+                c2 = 1.0 / (double)fact;
+            } else {
+                c1 = dt / (double)j;
+                c2 = 1.0;
+            }
+            double *tout = (j < order) ? ((j & 1) ? t0 : t1) : nullptr;
+            const double *vin = (j == 1) ? cur : nxt;
+            launch_term(op, adj, nv, op->stream, op->yx, x, tout, vin, nxt, c1, c2);
+            x = tout;
+        }
+        cur = nxt;
+    }
+    return cur;
+}
+
+int nv_of(hymor_op *op, int dtype) {
+    ARG_CHECK(dtype == HYMOR_F64 || dtype == HYMOR_C128, "dtype must be HYMOR_F64 or HYMOR_C128");
+    ARG_CHECK(!(op->is_complex && dtype == HYMOR_F64),
+              "complex operator needs complex (HYMOR_C128) vectors");
+    return dtype == HYMOR_C128 ? 2 : 1;
+}
+
+void check_taylor_args(hymor_op *op, int transpose, int order, int64_t nsub, int form) {
+    ARG_CHECK(op != nullptr, "null operator");
+    ARG_CHECK(order >= 1 && order <= 20, "order must be in [1, 20]");
+    ARG_CHECK(nsub >= 0, "nsub must be >= 0");
+    ARG_CHECK(form == HYMOR_FORM_NORMALISED || form == HYMOR_FORM_POWERS, "unknown Taylor form");
+    ARG_CHECK(!transpose || op->has_adj, "operator was built without its adjoint");
+}
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        g_err.clear();
+        return HYMOR_OK;
+    } catch (const ArgError &e) {
+        g_err = e.what();
+        return HYMOR_ERR_ARG;
+    } catch (const std::invalid_argument &e) {
+        g_err = e.what();
+        return HYMOR_ERR_ARG;
+    } catch (const HipError &e) {
+        g_err = e.what();
+        return HYMOR_ERR_HIP;
+    } catch (const std::bad_alloc &) {
+        g_err = "allocation failed";
+        return HYMOR_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return HYMOR_ERR_ARG;
+    }
+}
+
+hymor_op *new_op(int64_t nchi, int64_t neta, int b) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        (void)hipGetLastError();
+        throw HipError("no HIP device available");
+    }
+    hymor_op *op = new hymor_op();
+    op->nchi = nchi;
+    op->neta = neta;
+    op->ncell = nchi * neta;
+    op->b = b;
+    op->nflow = (int64_t)b * op->ncell;
+    HIP_TRY(hipGetDevice(&op->device));
+    HIP_TRY(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+    return op;
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+
+extern "C" {
+
+int hymor_device_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n > 0 ? 1 : 0;
+}
+
+const char *hymor_last_error(void) { return g_err.c_str(); }
+
+int hymor_op_create_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                        const void *nzval, int index_base, int dtype, double scale,
+                        int vars_per_cell, int64_t nchi, int64_t neta, int value_prec,
+                        hymor_op **out) {
+    hymor_op *op = nullptr;
+    int rc = guarded([&] {
+        ARG_CHECK(out != nullptr, "out is null");
+        *out = nullptr;
+        ARG_CHECK(colptr && rowval && nzval, "null CSC array");
+        ARG_CHECK(index_base == 0 || index_base == 1, "index_base must be 0 or 1");
+        ARG_CHECK(dtype == HYMOR_F64 || dtype == HYMOR_C128, "bad dtype");
+        ARG_CHECK(vars_per_cell == 4 || vars_per_cell == 8, "vars_per_cell must be 4 or 8");
+        ARG_CHECK(value_prec == HYMOR_PREC_F64 || value_prec == HYMOR_PREC_F32, "bad value_prec");
+        ARG_CHECK(nchi > 0 && neta > 0, "grid must be non-empty");
+        ARG_CHECK(n >= vars_per_cell * nchi * neta, "n smaller than vars_per_cell*nchi*neta");
+        ARG_CHECK(n < (int64_t)INT32_MAX, "n too large for int32 column indices");
+        HostBSR fwd, adj;
+        HostExtra xf, xa;
+        csc_to_blocks(n, colptr, rowval, (const double *)nzval, index_base, dtype == HYMOR_C128,
+                      scale, vars_per_cell, nchi * neta, fwd, adj, xf, xa);
+        op = new_op(nchi, neta, vars_per_cell);
+        op->n = n;
+        op->prec = value_prec;
+        op->is_complex = dtype == HYMOR_C128;
+        upload_sliced(op, fwd, value_prec, op->fwd);
+        upload_sliced(op, adj, value_prec, op->adj);
+        upload_extra(op, xf, op->xf);
+        upload_extra(op, xa, op->xa);
+        op->has_adj = true;
+        *out = op;
+    });
+    if (rc != HYMOR_OK && op) {
+        release(op);
+        delete op;
+    }
+    return rc;
+}
+
+int hymor_op_create_bsr(int64_t nchi, int64_t neta, int vars_per_cell, int64_t n_extra,
+                        const int64_t *brow_ptr, const int32_t *bcol, const double *bval,
+                        int64_t nx, const int64_t *xrow, const int64_t *xcol, const void *xval,
+                        int dtype, double scale, int value_prec, int with_adjoint,
+                        hymor_op **out) {
+    hymor_op *op = nullptr;
+    int rc = guarded([&] {
+        ARG_CHECK(out != nullptr, "out is null");
+        *out = nullptr;
+        ARG_CHECK(vars_per_cell == 4 || vars_per_cell == 8, "vars_per_cell must be 4 or 8");
+        ARG_CHECK(nchi > 0 && neta > 0 && n_extra >= 0, "bad sizes");
+        ARG_CHECK(dtype == HYMOR_F64 || dtype == HYMOR_C128, "bad dtype");
+        ARG_CHECK(value_prec == HYMOR_PREC_F64 || value_prec == HYMOR_PREC_F32, "bad value_prec");
+        ARG_CHECK(brow_ptr && bcol && bval, "null BSR array");
+        ARG_CHECK(nx == 0 || (xrow && xcol && xval), "null extra array");
+        const int b = vars_per_cell, bb = b * b;
+        const int64_t ncell = nchi * neta;
+        const int64_t n = (int64_t)b * ncell + n_extra;
+        ARG_CHECK(n < (int64_t)INT32_MAX, "n too large for int32 column indices");
+        ARG_CHECK(brow_ptr[0] == 0, "brow_ptr[0] must be 0");
+        HostBSR fwd;
+        fwd.b = b;
+        fwd.ncell = ncell;
+        fwd.ptr.assign(brow_ptr, brow_ptr + ncell + 1);
+        for (int64_t c = 0; c < ncell; ++c)
+            ARG_CHECK(fwd.ptr[c + 1] >= fwd.ptr[c], "brow_ptr not monotone");
+        const int64_t nnzb = fwd.ptr[ncell];
+        fwd.col.assign(bcol, bcol + nnzb);
+        for (int64_t k = 0; k < nnzb; ++k)
+            ARG_CHECK(fwd.col[k] >= 0 && fwd.col[k] < ncell, "bcol out of range");
+        fwd.val.resize((size_t)nnzb * bb);
+        for (size_t i = 0; i < fwd.val.size(); ++i) fwd.val[i] = bval[i] * scale;
+        const bool cplx = dtype == HYMOR_C128;
+        const int vs = cplx ? 2 : 1;
+        std::vector<int64_t> r(xrow, xrow + nx), c(xcol, xcol + nx);
+        std::vector<double> v((size_t)nx * vs);
+        for (int64_t k = 0; k < nx; ++k) {
+            ARG_CHECK(r[k] >= 0 && r[k] < n && c[k] >= 0 && c[k] < n, "extra index out of range");
+            for (int q = 0; q < vs; ++q) v[(size_t)k * vs + q] = ((const double *)xval)[k * vs + q] * scale;
+        }
+        const int64_t nflow = (int64_t)b * ncell;
+        HostExtra xf, xa;
+        extra_from_coo(n, nflow, r, c, v, cplx, false, xf);
+        op = new_op(nchi, neta, b);
+        op->n = n;
+        op->prec = value_prec;
+        op->is_complex = cplx;
+        upload_sliced(op, fwd, value_prec, op->fwd);
+        upload_extra(op, xf, op->xf);
+        if (with_adjoint) {
+            HostBSR adj;
+            bsr_transpose(fwd, adj);
+            fwd = HostBSR();
+            upload_sliced(op, adj, value_prec, op->adj);
+            extra_from_coo(n, nflow, r, c, v, cplx, true, xa);
+            upload_extra(op, xa, op->xa);
+            op->has_adj = true;
+        }
+        *out = op;
+    });
+    if (rc != HYMOR_OK && op) {
+        release(op);
+        delete op;
+    }
+    return rc;
+}
+
+int hymor_op_destroy(hymor_op *op) {
+    return guarded([&] {
+        if (!op) return;
+        if (op->stream) HIP_TRY(hipStreamSynchronize(op->stream));
+        release(op);
+        delete op;
+    });
+}
+
+int hymor_op_info(const hymor_op *op, int64_t *out, int nout) {
+    return guarded([&] {
+        ARG_CHECK(op && out, "null argument");
+        const int64_t v[10] = {op->n, op->b, op->ncell, op->fwd.nnzb, op->fwd.nnzb_pad,
+                               op->xf.nnz, op->xf.nrows, op->is_complex ? 1 : 0, op->prec,
+                               op->dev_bytes};
+        for (int i = 0; i < nout && i < 10; ++i) out[i] = v[i];
+    });
+}
+
+int hymor_spmv(hymor_op *op, int transpose, const void *x, void *y, int dtype) {
+    return guarded([&] {
+        ARG_CHECK(op && x && y, "null argument");
+        ARG_CHECK(!transpose || op->has_adj, "operator was built without its adjoint");
+        const int nv = nv_of(op, dtype);
+        ensure_ws(op, nv);
+        const size_t bytes = (size_t)op->n * nv * sizeof(double);
+        HIP_TRY(hipMemcpyAsync(op->ws[0], x, bytes, hipMemcpyHostToDevice, op->stream));
+        launch_perm_in(op, op->stream, op->ws[0], op->n, op->ws[1], 1, nv, 0);
+        launch_term(op, transpose != 0, nv, op->stream, op->yx, op->ws[1], op->ws[2], nullptr,
+                    nullptr, 1.0, 0.0);
+        launch_perm_out(op, op->stream, op->ws[2], op->ws[0], op->n, 1, nv, 0);
+        HIP_TRY(hipMemcpyAsync(y, op->ws[0], bytes, hipMemcpyDeviceToHost, op->stream));
+        HIP_TRY(hipStreamSynchronize(op->stream));
+    });
+}
+
+int hymor_expm_taylor_dev(hymor_op *op, int transpose, int order, int64_t nsub, double dt,
+                          int form, const void *d_x, void *d_y, int dtype) {
+    return guarded([&] {
+        check_taylor_args(op, transpose, order, nsub, form);
+        ARG_CHECK(d_x && d_y, "null vector");
+        const int nv = nv_of(op, dtype);
+        ensure_ws(op, nv);
+        launch_perm_in(op, op->stream, (const double *)d_x, op->n, op->ws[1], 1, nv, 0);
+        const double *res = run_taylor(op, transpose != 0, order, nsub, dt, form, nv, op->ws[1]);
+        launch_perm_out(op, op->stream, res, (double *)d_y, op->n, 1, nv, 0);
+    });
+}
+
+int hymor_expm_taylor(hymor_op *op, int transpose, int order, int64_t nsub, double dt, int form,
+                      const void *x, void *y, int dtype) {
+    return guarded([&] {
+        check_taylor_args(op, transpose, order, nsub, form);
+        ARG_CHECK(x && y, "null vector");
+        const int nv = nv_of(op, dtype);
+        ensure_ws(op, nv);
+        const size_t bytes = (size_t)op->n * nv * sizeof(double);
+        HIP_TRY(hipMemcpyAsync(op->ws[0], x, bytes, hipMemcpyHostToDevice, op->stream));
+        launch_perm_in(op, op->stream, op->ws[0], op->n, op->ws[1], 1, nv, 0);
+        const double *res = run_taylor(op, transpose != 0, order, nsub, dt, form, nv, op->ws[1]);
+        launch_perm_out(op, op->stream, res, op->ws[0], op->n, 1, nv, 0);
+        HIP_TRY(hipMemcpyAsync(y, op->ws[0], bytes, hipMemcpyDeviceToHost, op->stream));
+        HIP_TRY(hipStreamSynchronize(op->stream));
+    });
+}
+
+static void sandwich_core(hymor_op *A, hymor_op *C, int order, int64_t nsub, double dt,
+                          int64_t n_set0, const double *x_vm, double *y_vm, int nv) {
+    launch_perm_in(A, A->stream, x_vm, A->n, A->ws[1], 1, nv, n_set0);
+    const double *r1 = run_taylor(A, false, order, nsub, dt, HYMOR_FORM_NORMALISED, nv, A->ws[1]);
+    // weight: ws[6] = C * r1 (C's extras use C's own partial-sum buffer)
+    launch_term(C, false, nv, A->stream, C->yx, r1, A->ws[6], nullptr, nullptr, 1.0, 0.0);
+    const double *r2 = run_taylor(A, true, order, nsub, dt, HYMOR_FORM_NORMALISED, nv, A->ws[6]);
+    launch_perm_out(A, A->stream, r2, y_vm, A->n, 1, nv, n_set0);
+}
+
+static void check_sandwich(hymor_op *A, hymor_op *C, int order, int64_t nsub, int64_t n_set0) {
+    check_taylor_args(A, 1, order, nsub, HYMOR_FORM_NORMALISED);
+    ARG_CHECK(C != nullptr, "null weight operator");
+    ARG_CHECK(C->n == A->n && C->b == A->b && C->ncell == A->ncell,
+              "weight operator shape differs from A");
+    ARG_CHECK(n_set0 >= 0 && n_set0 <= A->n, "n_set0 out of range");
+}
+
+int hymor_chu_sandwich_dev(hymor_op *A, hymor_op *C, int order, int64_t nsub, double dt,
+                           int64_t n_set0, const void *d_x, void *d_y, int dtype) {
+    return guarded([&] {
+        check_sandwich(A, C, order, nsub, n_set0);
+        ARG_CHECK(d_x && d_y, "null vector");
+        const int nv = nv_of(A, dtype);
+        ARG_CHECK(!(C->is_complex && nv == 1), "complex weight needs complex vectors");
+        ensure_ws(A, nv);
+        ensure_ws(C, nv);
+        sandwich_core(A, C, order, nsub, dt, n_set0, (const double *)d_x, (double *)d_y, nv);
+    });
+}
+
+int hymor_chu_sandwich(hymor_op *A, hymor_op *C, int order, int64_t nsub, double dt,
+                       int64_t n_set0, const void *x, void *y, int dtype) {
+    return guarded([&] {
+        check_sandwich(A, C, order, nsub, n_set0);
+        ARG_CHECK(x && y, "null vector");
+        const int nv = nv_of(A, dtype);
+        ARG_CHECK(!(C->is_complex && nv == 1), "complex weight needs complex vectors");
+        ensure_ws(A, nv);
+        ensure_ws(C, nv);
+        const size_t bytes = (size_t)A->n * nv * sizeof(double);
+        HIP_TRY(hipMemcpyAsync(A->ws[0], x, bytes, hipMemcpyHostToDevice, A->stream));
+        sandwich_core(A, C, order, nsub, dt, n_set0, A->ws[0], A->ws[0], nv);
+        HIP_TRY(hipMemcpyAsync(y, A->ws[0], bytes, hipMemcpyDeviceToHost, A->stream));
+        HIP_TRY(hipStreamSynchronize(A->stream));
+    });
+}
+
+int hymor_op_synchronize(hymor_op *op) {
+    return guarded([&] {
+        ARG_CHECK(op != nullptr, "null operator");
+        HIP_TRY(hipStreamSynchronize(op->stream));
+        HIP_TRY(hipGetLastError());
+    });
+}
+
+int hymor_dev_alloc(int64_t bytes, void **d_ptr) {
+    return guarded([&] {
+        ARG_CHECK(d_ptr && bytes >= 0, "bad argument");
+        *d_ptr = nullptr;
+        HIP_TRY(hipMalloc(d_ptr, bytes > 0 ? (size_t)bytes : 256));
+    });
+}
+
+int hymor_dev_free(void *d_ptr) {
+    return guarded([&] {
+        if (d_ptr) HIP_TRY(hipFree(d_ptr));
+    });
+}
+
+int hymor_memcpy_h2d(void *d_dst, const void *h_src, int64_t bytes) {
+    return guarded([&] {
+        ARG_CHECK(d_dst && h_src && bytes >= 0, "bad argument");
+        HIP_TRY(hipMemcpy(d_dst, h_src, (size_t)bytes, hipMemcpyHostToDevice));
+    });
+}
+
+int hymor_memcpy_d2h(void *h_dst, const void *d_src, int64_t bytes) {
+    return guarded([&] {
+        ARG_CHECK(h_dst && d_src && bytes >= 0, "bad argument");
+        HIP_TRY(hipMemcpy(h_dst, d_src, (size_t)bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int reps, double *ms_per_launch,
+                            double *bytes_per_launch) {
+    return guarded([&] {
+        ARG_CHECK(op && ms_per_launch && bytes_per_launch, "null argument");
+        ARG_CHECK(nv == 1 || nv == 2, "nv must be 1 or 2");
+        ARG_CHECK(!(op->is_complex && nv == 1), "complex operator needs nv = 2");
+        ARG_CHECK(reps >= 1, "reps must be >= 1");
+        ARG_CHECK(!transpose || op->has_adj, "operator was built without its adjoint");
+        ensure_ws(op, nv);
+        const DevSliced &s = transpose ? op->adj : op->fwd;
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        // one warm launch, then the timed ones
+        launch_term(op, transpose != 0, nv, op->stream, op->yx, op->ws[4], op->ws[5], op->ws[2],
+                    op->ws[2], 0.5, 1.0);
+        HIP_TRY(hipEventRecord(e0, op->stream));
+        for (int i = 0; i < reps; ++i)
+            launch_term(op, transpose != 0, nv, op->stream, op->yx, op->ws[4], op->ws[5],
+                        op->ws[2], op->ws[2], 0.5, 1.0);
+        HIP_TRY(hipEventRecord(e1, op->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms_per_launch = (double)ms / reps;
+        const double sval = op->prec == HYMOR_PREC_F32 ? 4.0 : 8.0;
+        const double bb = (double)s.b * s.b;
+        // blocks (values + int32 column) + block-row pointers + x read,
+        // term written, vout read and written
+        double bytes = (double)s.nnzb * (bb * sval + 4.0) + (double)(s.ncell + 1) * 4.0 +
+                       4.0 * (double)op->n * nv * 8.0;
+        const DevExtra &xe = transpose ? op->xa : op->xf;
+        bytes += (double)xe.nnz * ((xe.is_complex ? 16.0 : 8.0) + 4.0);
+        *bytes_per_launch = bytes;
+    });
+}
+
+}  // extern "C"

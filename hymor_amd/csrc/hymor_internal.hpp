@@ -1,0 +1,64 @@
+// Internal structures of libhymor_hip.  Host-side only; kernels see raw
+// pointers.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace hymor {
+
+// Host block-CSR over cells (0-based, cell-major).  Block k couples cell
+// row(k) to cell col[k]; val holds b*b doubles per block, row-major.
+struct HostBSR {
+    int b = 0;
+    int64_t ncell = 0;
+    std::vector<int64_t> ptr;   // ncell+1
+    std::vector<int32_t> col;   // nnzb
+    std::vector<double> val;    // nnzb*b*b
+    int64_t nnzb() const { return (int64_t)col.size(); }
+};
+
+// Host CSR of the "extra" entries: every entry that touches an extra DOF
+// (index >= b*ncell) and, for complex operators, every imaginary part.
+// Rows are cell-major global indices; only rows holding entries, plus every
+// extra-DOF row, are listed.
+struct HostExtra {
+    bool is_complex = false;
+    std::vector<int64_t> rows;  // listed rows, ascending
+    std::vector<int64_t> ptr;   // rows.size()+1
+    std::vector<int32_t> col;   // cell-major global column
+    std::vector<double> val;    // 1 or 2 doubles per entry
+};
+
+// Sliced-ELL device layout of a BSR: slices of CPS = 64/b consecutive cells,
+// one wavefront per slice, each lane one (cell, block-row) pair.  Inside a
+// slice the blocks are slot-major: slot t of cell q sits at
+// slice_ptr[s] + t*CPS + q, so one slot of a slice is a contiguous
+// CPS*b*b-value segment.  Short rows are padded with zero blocks that point
+// at the row's own cell.
+struct HostSliced {
+    int b = 0;
+    int cps = 0;
+    int64_t ncell = 0;
+    int64_t nslice = 0;
+    std::vector<int64_t> slice_ptr;  // nslice+1, in blocks
+    std::vector<int32_t> col;        // padded blocks
+    std::vector<double> val;         // padded blocks * b*b
+    int64_t nnzb = 0;                // unpadded
+};
+
+void csc_to_blocks(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                   const double *nzval, int index_base, bool is_complex, double scale,
+                   int b, int64_t ncell, HostBSR &fwd, HostBSR &adj,
+                   HostExtra &xf, HostExtra &xa);
+
+void bsr_transpose(const HostBSR &a, HostBSR &t);
+
+void extra_from_coo(int64_t n, int64_t nflow, const std::vector<int64_t> &r,
+                    const std::vector<int64_t> &c, const std::vector<double> &v,
+                    bool is_complex, bool adjoint, HostExtra &out);
+
+void bsr_to_sliced(const HostBSR &a, HostSliced &s);
+
+}  // namespace hymor

@@ -1,0 +1,201 @@
+// Device kernels of libhymor_hip (gfx950, wave64).
+//
+// Layout recap (host side builds it, see convert.cpp):
+//   - vectors are cell-major: row = cell*b + var for the b*ncell flow
+//     unknowns, then the extra DOFs; NV doubles per row (1 real, 2 complex
+//     (re,im), 2*nrhs for batches).
+//   - A's flow-flow part is sliced ELL over cells: a slice is CPS = 64/b
+//     cells, one wavefront per slice, lane = (cell q, block row r).  Slot t
+//     of cell q is block slice_ptr[s] + t*CPS + q, so each slot of a slice is
+//     one contiguous CPS*b*b-value run: lanes read their block row with
+//     16-byte loads and the wavefront covers the run without gaps.
+//   - everything else (shock bands, freestream coupling, imaginary parts) is
+//     a short CSR over the listed rows, one wavefront per row.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace hymor {
+
+typedef double d2 __attribute__((ext_vector_type(2)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Logical tile of a workgroup such that each of the 8 XCDs (which receive
+// workgroups round-robin by blockIdx) works on one contiguous range of
+// tiles: neighbouring cells, and the x entries they gather, stay in one
+// XCD's L2.
+__device__ __forceinline__ int64_t xcd_tile(int64_t bid, int64_t nwg) {
+    const int64_t per = nwg >> 3, rem = nwg & 7;
+    const int64_t xcd = bid & 7, idx = bid >> 3;
+    return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
+template <int N>
+__device__ __forceinline__ void load_f64(const double *__restrict__ p, double *a) {
+    static_assert(N % 2 == 0, "pairs");
+#pragma unroll
+    for (int k = 0; k < N; k += 2) {
+        const d2 v = *reinterpret_cast<const d2 *>(p + k);
+        a[k] = v.x;
+        a[k + 1] = v.y;
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void load_vals(const double *__restrict__ p, double *a) {
+    load_f64<N>(p, a);
+}
+
+template <int N>
+__device__ __forceinline__ void load_vals(const float *__restrict__ p, double *a) {
+    static_assert(N % 4 == 0, "quads");
+#pragma unroll
+    for (int k = 0; k < N; k += 4) {
+        const f4 v = *reinterpret_cast<const f4 *>(p + k);
+        a[k] = (double)v.x;
+        a[k + 1] = (double)v.y;
+        a[k + 2] = (double)v.z;
+        a[k + 3] = (double)v.w;
+    }
+}
+
+// One Taylor term over the block part:  y = A*x (+ yx from the extra CSR),
+// t = c1*y, tout = t (if tout), vout = vin + c2*t (if vout).
+// c1/c2 select the recursion: normalised (c1 = dt/j, c2 = 1) or powers
+// (c1 = dt, c2 = 1/j!).  With tout = y and vout = null this is a plain SpMV.
+// vin may equal vout (in-place accumulation of the own row) or x (first term
+// of a substep); x is never written by this kernel.
+template <int B, typename VT, int NV>
+__global__ __launch_bounds__(256) void k_term(
+    const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
+    const VT *__restrict__ val, int64_t nslice, int64_t ncell,
+    const double *__restrict__ x, const double *__restrict__ yx,
+    double *__restrict__ tout, const double *vin, double *vout, double c1, double c2) {
+    constexpr int CPS = 64 / B;
+    constexpr int BB = B * B;
+    const int lane = threadIdx.x & 63;
+    const int64_t wpb = blockDim.x >> 6;
+    const int64_t slice = xcd_tile(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6);
+    if (slice >= nslice) return;
+    const int q = lane / B, r = lane % B;
+    const int64_t p0 = slice_ptr[slice];
+    const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
+    const int32_t *cp = col + p0 + q;
+    const VT *vp = val + (p0 + q) * BB + r * B;
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+#pragma unroll 3
+    for (int t = 0; t < width; ++t) {
+        const int64_t c = cp[(int64_t)t * CPS];
+        double a[B];
+        load_vals<B>(vp + (int64_t)t * CPS * BB, a);
+        double xv[B * NV];
+        load_f64<B * NV>(x + c * (B * NV), xv);
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] = fma(a[k], xv[k * NV + v], acc[v]);
+    }
+    const int64_t cell = slice * CPS + q;
+    if (cell >= ncell) return;
+    const int64_t row = cell * B + r;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double y = acc[v];
+        if (yx) y += yx[row * NV + v];
+        const double t = c1 * y;
+        if (tout) tout[row * NV + v] = t;
+        if (vout) vout[row * NV + v] = fma(c2, t, vin[row * NV + v]);
+    }
+}
+
+// Extra CSR rows, one wavefront per listed row.  Flow rows (row < nflow)
+// leave their partial sum in yx for k_term; extra-DOF rows finish the term
+// here.  CPLX: values are complex (re,im) and NV counts doubles of complex
+// vectors ((re,im) pairs); otherwise values are real and every one of the
+// NV components is an independent right-hand side.
+template <int NV, bool CPLX>
+__global__ __launch_bounds__(256) void k_extra(
+    const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
+    const int32_t *__restrict__ col, const double *__restrict__ val, int64_t nrows,
+    int64_t nflow, const double *__restrict__ x, double *__restrict__ yx,
+    double *__restrict__ tout, const double *vin, double *vout, double c1, double c2) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= nrows) return;
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+    const int64_t e1 = ptr[w + 1];
+    for (int64_t e = ptr[w] + lane; e < e1; e += 64) {
+        const int64_t c = col[e];
+        if constexpr (CPLX) {
+            const double ar = val[2 * e], ai = val[2 * e + 1];
+#pragma unroll
+            for (int v = 0; v < NV; v += 2) {
+                const double xr = x[c * NV + v], xi = x[c * NV + v + 1];
+                acc[v] = fma(ar, xr, fma(-ai, xi, acc[v]));
+                acc[v + 1] = fma(ar, xi, fma(ai, xr, acc[v + 1]));
+            }
+        } else {
+            const double a = val[e];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] = fma(a, x[c * NV + v], acc[v]);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc[v] += __shfl_xor(acc[v], off, 64);
+    if (lane == 0) {
+        const int64_t row = rows[w];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            if (row < nflow) {
+                yx[row * NV + v] = acc[v];
+            } else {
+                const double t = c1 * acc[v];
+                if (tout) tout[row * NV + v] = t;
+                if (vout) vout[row * NV + v] = fma(c2, t, vin[row * NV + v]);
+            }
+        }
+    }
+}
+
+// var-major (ABI) -> cell-major (internal).  x_vm holds nrhs columns of n
+// elements with leading dimension ld (elements); an element is ncomp
+// doubles.  Internal rows hold nrhs*ncomp doubles.  Var-major indices below
+// nzero read as 0 (CONSTRUCT_C's x[1:N_set_0] .= 0).
+__global__ void k_perm_in(const double *__restrict__ x_vm, int64_t ld, double *__restrict__ out,
+                          int64_t n, int64_t ncell, int b, int nrhs, int ncomp, int64_t nzero) {
+    const int64_t per = (int64_t)nrhs * ncomp;
+    const int64_t total = n * per;
+    const int64_t nflow = ncell * b;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = i / per;
+        const int64_t rem = i - row * per;
+        const int64_t rhs = rem / ncomp, comp = rem - rhs * ncomp;
+        const int64_t vm = row < nflow ? (row % b) * ncell + row / b : row;
+        out[i] = vm < nzero ? 0.0 : x_vm[(rhs * ld + vm) * ncomp + comp];
+    }
+}
+
+__global__ void k_perm_out(const double *__restrict__ in, double *__restrict__ y_vm, int64_t ld,
+                           int64_t n, int64_t ncell, int b, int nrhs, int ncomp, int64_t nzero) {
+    const int64_t per = (int64_t)nrhs * ncomp;
+    const int64_t total = n * per;
+    const int64_t nflow = ncell * b;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t rhs = i / (n * ncomp);
+        const int64_t rem = i - rhs * n * ncomp;
+        const int64_t vm = rem / ncomp, comp = rem - vm * ncomp;
+        const int64_t row = vm < nflow ? (vm % ncell) * b + vm / ncell : vm;
+        y_vm[(rhs * ld + vm) * ncomp + comp] = vm < nzero ? 0.0 : in[(row * nrhs + rhs) * ncomp + comp];
+    }
+}
+
+}  // namespace hymor

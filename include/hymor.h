@@ -1,0 +1,150 @@
+/*
+ * hymor.h -- C ABI of libhymor_hip.so, the MI355X (gfx950) engine for HYMOR's
+ * linear-stability hot path: the Taylor action w = T_p(A*dt)^n * v of the
+ * linearised compressible Navier-Stokes Jacobian, its adjoint, the Chu-norm
+ * sandwich and the energy-tracked integration.
+ *
+ * Plain C types only (pointers, sizes, scalars): callable from Julia
+ * (ccall), Python (ctypes) or C.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative HYMOR_ERR_* code on
+ *     failure; hymor_last_error() then holds a message (thread-local).
+ *   - Vectors cross the ABI in the reference's variable-major order
+ *     [rho(all cells); rho_u; rho_v; rho_E; extra DOFs], length n.  Complex
+ *     vectors are interleaved (re, im) pairs, the memory layout of Julia's
+ *     Vector{ComplexF64}.  Batches are column-major n x nrhs with a leading
+ *     dimension, the layout of a Julia Matrix.
+ *   - The library owns all device memory of an operator handle and one HIP
+ *     stream per handle.  Host-pointer entry points return after the result
+ *     is in host memory.  *_dev entry points take device pointers, enqueue on
+ *     the handle's stream and return at once; hymor_op_synchronize() waits
+ *     and reports asynchronous errors (the reference's CUDA.synchronize(),
+ *     utils/Stability_analysis/Eigenvalues/EIGENVALUES.jl:174).
+ *   - A handle is used by one host thread at a time (ARPACK calls its matvec
+ *     sequentially, EIGENVALUES.jl:153-159).
+ */
+#ifndef HYMOR_H
+#define HYMOR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HYMOR_OK 0
+#define HYMOR_ERR_ARG (-1)     /* invalid argument or shape */
+#define HYMOR_ERR_HIP (-2)     /* HIP runtime error */
+#define HYMOR_ERR_NOMEM (-3)   /* host or device allocation failed */
+#define HYMOR_ERR_NODEV (-4)   /* no usable GPU */
+
+/* element type of vectors and operator values */
+#define HYMOR_F64 0            /* Float64 */
+#define HYMOR_C128 1           /* ComplexF64, interleaved (re, im) */
+
+/* storage precision of the block values; accumulation is always FP64 */
+#define HYMOR_PREC_F64 0
+#define HYMOR_PREC_F32 1
+
+/* Taylor recursion form */
+#define HYMOR_FORM_NORMALISED 0 /* term = (A*term)/j; vout += term
+                                   (CONSTRUCT_C.jl:84-91, EXPONENTIAL!) */
+#define HYMOR_FORM_POWERS 1     /* v = A*v; vout += v/j!
+                                   (ARNOLDI_POWER_GPU.jl:70-79, POWER_METHOD!) */
+
+typedef struct hymor_op hymor_op;
+
+/* 1 if a GPU the library can use is present, else 0.  Replaces the
+ * reference's `use_gpu = CUDA.functional()` (EIGENVALUES.jl:119). */
+int hymor_device_available(void);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char *hymor_last_error(void);
+
+/* Build a device operator from a CSC matrix in the reference's layout.
+ * Replaces `CuSparseMatrixCSC(A * dt)` (EIGENVALUES.jl:139,
+ * TRANSIENT_GROWTH_DOWNSTREAM.jl:189-191, FREESTREAM_RECEPTIVITY.jl:83-85).
+ *   n, colptr[n+1], rowval[nnz], nzval[nnz] : SparseMatrixCSC fields;
+ *       index_base 1 for Julia arrays as-is, 0 for 0-based arrays.
+ *   dtype        : HYMOR_F64 (nzval double) or HYMOR_C128 (interleaved).
+ *   scale        : values are multiplied by it at upload (the reference's
+ *                  A*dt); pass 1.0 and use the per-call dt otherwise.
+ *   vars_per_cell: b (4 in the reference); nchi*neta cells; unknowns past
+ *                  b*nchi*neta (shock, freestream DOFs) are "extra".
+ *   value_prec   : HYMOR_PREC_F64 or HYMOR_PREC_F32 storage of block values.
+ * The transpose (adjoint) operator is built at the same time, replacing
+ * `sparse(A_dt')` (TRANSIENT_GROWTH_DOWNSTREAM.jl:186). */
+int hymor_op_create_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                        const void *nzval, int index_base, int dtype, double scale,
+                        int vars_per_cell, int64_t nchi, int64_t neta,
+                        int value_prec, hymor_op **out);
+
+/* Build a device operator from cell-major block-CSR arrays (0-based), the
+ * library's native layout: brow_ptr[ncell+1], bcol[nnzb] (int32),
+ * bval[nnzb*b*b] (row-major blocks, real), plus nx extra entries in
+ * cell-major global indices (xval real or complex per dtype).  No reference
+ * counterpart: a faster upload for callers that already hold blocks.
+ * with_adjoint = 0 skips building the transpose. */
+int hymor_op_create_bsr(int64_t nchi, int64_t neta, int vars_per_cell, int64_t n_extra,
+                        const int64_t *brow_ptr, const int32_t *bcol, const double *bval,
+                        int64_t nx, const int64_t *xrow, const int64_t *xcol,
+                        const void *xval, int dtype, double scale, int value_prec,
+                        int with_adjoint, hymor_op **out);
+
+/* Release all device memory of a handle (CUDA.unsafe_free! + reclaim,
+ * EIGENVALUES.jl:175-180). */
+int hymor_op_destroy(hymor_op *op);
+
+/* Sizes of a handle: out[0]=n, [1]=b, [2]=ncell, [3]=nnzb, [4]=padded nnzb,
+ * [5]=extra nnz, [6]=extra rows, [7]=is_complex, [8]=value_prec,
+ * [9]=device bytes held.  nout entries are written (at most 10). */
+int hymor_op_info(const hymor_op *op, int64_t *out, int nout);
+
+/* y = A*x (transpose != 0: y = A'*x, the adjoint).  The reference's
+ * mul!(y, A, x, 1.0, 0.0) on CuSparseMatrixCSC (ARNOLDI_POWER_GPU.jl:73). */
+int hymor_spmv(hymor_op *op, int transpose, const void *x, void *y, int dtype);
+
+/* y = T_order(A*dt)^nsub * x, where T_p is the degree-p Taylor polynomial
+ * of exp.  order=5, form=HYMOR_FORM_POWERS is POWER_METHOD!
+ * (ARNOLDI_POWER_GPU.jl:64-82) behind ARNOLDI_POWER_GPU (:36-52);
+ * transpose != 0 applies the adjoint operator.  x and y are host vectors. */
+int hymor_expm_taylor(hymor_op *op, int transpose, int order, int64_t nsub, double dt,
+                      int form, const void *x, void *y, int dtype);
+
+/* Device-pointer form of hymor_expm_taylor (asynchronous). */
+int hymor_expm_taylor_dev(hymor_op *op, int transpose, int order, int64_t nsub, double dt,
+                          int form, const void *d_x, void *d_y, int dtype);
+
+/* Chu-norm sandwich y = T_p(A'dt)^n * C * T_p(A dt)^n * x with the zeroing
+ * of CONSTRUCT_C (CONSTRUCT_C.jl:33-58 around EXPONENTIAL! :78-111):
+ * x[0:n_set0) is zeroed before and y[0:n_set0) after (var-major indices).
+ * C is a handle built from R'*M*R (TRANSIENT_GROWTH_DOWNSTREAM.jl:162). */
+int hymor_chu_sandwich(hymor_op *A, hymor_op *C, int order, int64_t nsub, double dt,
+                       int64_t n_set0, const void *x, void *y, int dtype);
+
+int hymor_chu_sandwich_dev(hymor_op *A, hymor_op *C, int order, int64_t nsub, double dt,
+                           int64_t n_set0, const void *d_x, void *d_y, int dtype);
+
+/* Wait for all work on the handle's stream; reports asynchronous errors. */
+int hymor_op_synchronize(hymor_op *op);
+
+/* Device memory helpers for the *_dev entry points. */
+int hymor_dev_alloc(int64_t bytes, void **d_ptr);
+int hymor_dev_free(void *d_ptr);
+int hymor_memcpy_h2d(void *d_dst, const void *h_src, int64_t bytes);
+int hymor_memcpy_d2h(void *h_dst, const void *d_src, int64_t bytes);
+
+/* Timing probe: launches the fused Taylor-term kernel `reps` times on the
+ * handle's stream (a middle term: gathers x, writes the new term, updates
+ * vout) between two HIP events; returns the mean duration per launch in ms
+ * and the kernel's algorithmic bytes per launch.  nv = doubles per row
+ * (1 real, 2 complex). */
+int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int reps,
+                            double *ms_per_launch, double *bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYMOR_H */

@@ -1,0 +1,106 @@
+// CPU emulation of the device layout and the kernels' index arithmetic.
+//
+// Built by tests/test_layout_cpu.py with g++ together with
+// hymor_amd/csrc/convert.cpp (the library's own host conversion).  It runs
+// csc_to_blocks -> bsr_to_sliced and then replays k_extra / k_term's exact
+// index math on the CPU, checking every access against the buffer it reads,
+// so that a layout or indexing mistake shows up here rather than as a GPU
+// memory fault.
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "hymor_internal.hpp"
+
+using namespace hymor;
+
+namespace {
+int g_oob = 0;
+template <typename T>
+const T &at(const std::vector<T> &v, int64_t i) {
+    if (i < 0 || i >= (int64_t)v.size()) {
+        ++g_oob;
+        static T zero{};
+        return zero;
+    }
+    return v[(size_t)i];
+}
+}  // namespace
+
+extern "C" int emu_spmv_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                            const double *nzval, int base, int is_complex, int b, int64_t ncell,
+                            int transpose, const double *x_vm, double *y_vm) {
+    g_oob = 0;
+    HostBSR fwd, adj;
+    HostExtra xf, xa;
+    csc_to_blocks(n, colptr, rowval, nzval, base, is_complex != 0, 1.0, b, ncell, fwd, adj, xf, xa);
+    const HostBSR &A = transpose ? adj : fwd;
+    const HostExtra &X = transpose ? xa : xf;
+    HostSliced s;
+    bsr_to_sliced(A, s);
+    const int NV = is_complex ? 2 : 1;
+    const int64_t nflow = (int64_t)b * ncell;
+    // perm_in
+    std::vector<double> x((size_t)n * NV), y((size_t)n * NV, 0.0), yx((size_t)nflow * NV, 0.0);
+    for (int64_t row = 0; row < n; ++row) {
+        const int64_t vm = row < nflow ? (row % b) * ncell + row / b : row;
+        for (int v = 0; v < NV; ++v) x[(size_t)(row * NV + v)] = x_vm[vm * NV + v];
+    }
+    // k_extra
+    for (int64_t w = 0; w < (int64_t)X.rows.size(); ++w) {
+        double acc[2] = {0, 0};
+        for (int64_t e = at(X.ptr, w); e < at(X.ptr, w + 1); ++e) {
+            const int64_t c = at(X.col, e);
+            if (is_complex) {
+                const double ar = at(X.val, 2 * e), ai = at(X.val, 2 * e + 1);
+                const double xr = at(x, c * NV), xi = at(x, c * NV + 1);
+                acc[0] += ar * xr - ai * xi;
+                acc[1] += ar * xi + ai * xr;
+            } else {
+                acc[0] += at(X.val, e) * at(x, c);
+            }
+        }
+        const int64_t row = at(X.rows, w);
+        for (int v = 0; v < NV; ++v) {
+            if (row < nflow) {
+                if (row * NV + v >= (int64_t)yx.size()) ++g_oob;
+                else yx[(size_t)(row * NV + v)] = acc[v];
+            } else {
+                if (row * NV + v >= (int64_t)y.size()) ++g_oob;
+                else y[(size_t)(row * NV + v)] = acc[v];
+            }
+        }
+    }
+    // k_term: one "wave" per slice, lane = (q, r)
+    const int CPS = 64 / b, BB = b * b;
+    for (int64_t slice = 0; slice < s.nslice; ++slice) {
+        const int64_t p0 = at(s.slice_ptr, slice);
+        const int64_t width = (at(s.slice_ptr, slice + 1) - p0) / CPS;
+        if ((at(s.slice_ptr, slice + 1) - p0) % CPS) ++g_oob;
+        for (int lane = 0; lane < 64; ++lane) {
+            const int q = lane / b, r = lane % b;
+            double acc[2] = {0, 0};
+            for (int64_t t = 0; t < width; ++t) {
+                const int64_t blk = p0 + q + t * CPS;
+                const int64_t c = at(s.col, blk);
+                if (c < 0 || c >= ncell) ++g_oob;
+                for (int k = 0; k < b; ++k) {
+                    const double a = at(s.val, blk * BB + r * b + k);
+                    for (int v = 0; v < NV; ++v) acc[v] += a * at(x, (c * b + k) * NV + v);
+                }
+            }
+            const int64_t cell = slice * CPS + q;
+            if (cell >= ncell) continue;
+            const int64_t row = cell * b + r;
+            for (int v = 0; v < NV; ++v)
+                y[(size_t)(row * NV + v)] = acc[v] + at(yx, row * NV + v);
+        }
+    }
+    // perm_out
+    for (int64_t vm = 0; vm < n; ++vm) {
+        const int64_t row = vm < nflow ? (vm % ncell) * b + vm / ncell : vm;
+        for (int v = 0; v < NV; ++v) y_vm[vm * NV + v] = y[(size_t)(row * NV + v)];
+    }
+    return g_oob;
+}

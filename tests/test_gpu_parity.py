@@ -1,0 +1,270 @@
+"""GPU parity: libhymor_hip.so (through the C ABI) against the CPU oracle.
+
+Tolerances (relative 2-norm unless stated):
+  * FP64 storage vs the FP64 oracle: 1e-12 for a few substeps, 1e-11 for the
+    768-SpMV config-1 action (summation order and FMA differ from the
+    reference's column scatter, so agreement is to rounding, not bits);
+  * FP32 storage (FP64 accumulation): 2e-6 -- the values themselves carry a
+    2^-24 relative rounding;
+  * true exponential of a block-circulant operator vs its FFT closed form:
+    1e-10 (the north-star bound for exp(A*T)v).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from hymor_amd import operators as ops
+from hymor_amd.core import DeviceOperator, DeviceVector, expm_taylor_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+CASES = {
+    "grid20x16_b4": dict(nchi=20, neta=16, b=4, seed=1),
+    "shock12x10_b4": dict(nchi=12, neta=10, b=4, seed=2, n_shock=12),
+    "grid10x8_b8": dict(nchi=10, neta=8, b=8, seed=3),
+    "periodic9x6_b4": dict(nchi=9, neta=6, b=4, seed=4, periodic_chi=True),
+    "shock7x5_b8": dict(nchi=7, neta=5, b=8, seed=5, n_shock=7),
+}
+
+
+def make_case(name, row_sum=2.0):
+    c = dict(CASES[name])
+    return ops.stencil_operator(c.pop("nchi"), c.pop("neta"), c.pop("b"), c.pop("seed"),
+                                row_sum=row_sum, **c)
+
+
+@pytest.fixture(scope="module", params=sorted(CASES))
+def case(request, orc):
+    say(f"[case {request.param}] building operator")
+    op = make_case(request.param)
+    A = orc.CSC.from_arrays(*op.to_csc_varmajor())
+    dev = DeviceOperator.from_csc(op.n, A.colptr, A.rowval, A.nzval, index_base=1,
+                                  vars_per_cell=op.b, nchi=op.nchi, neta=op.neta)
+    yield request.param, op, A, dev
+    dev.close()
+
+
+def test_spmv_forward_and_adjoint(case, orc):
+    name, op, A, dev = case
+    x = np.random.default_rng(0).standard_normal(op.n)
+    say(f"[{name}] spmv")
+    assert relerr(dev.spmv(x), orc.csc_mul(A, x)) <= 1e-14
+    At = orc.csc_adjoint(A)
+    assert relerr(dev.spmv(x, transpose=True), orc.csc_mul(At, x)) <= 1e-14
+
+
+def test_power_method_parity(case, orc):
+    name, op, A, dev = case
+    x = np.random.default_rng(1).standard_normal(op.n)
+    dt = 0.1
+    say(f"[{name}] POWER_METHOD! order 5 x 7 substeps")
+    ref = orc.power_method(A.scaled(dt), x, 7, order=5)
+    got = dev.expm_taylor(x, order=5, nsub=7, dt=dt, form="powers")
+    assert relerr(got, ref) <= 1e-12
+
+
+def test_adjoint_taylor_normalised_form(case, orc):
+    name, op, A, dev = case
+    x = np.random.default_rng(2).standard_normal(op.n)
+    dt = 0.1
+    say(f"[{name}] adjoint Taylor, normalised recursion")
+    ref = orc.power_method(orc.csc_adjoint(A).scaled(dt), x, 4, order=6)
+    got = dev.expm_taylor(x, order=6, nsub=4, dt=dt, form="normalised", transpose=True)
+    assert relerr(got, ref) <= 1e-12
+
+
+def test_bsr_route_equals_csc_route(case):
+    name, op, A, dev = case
+    say(f"[{name}] create_bsr vs create_csc")
+    x = np.random.default_rng(3).standard_normal(op.n)
+    with DeviceOperator.from_blocks(op) as d2:
+        assert np.array_equal(d2.expm_taylor(x, order=5, nsub=3, dt=0.1),
+                              dev.expm_taylor(x, order=5, nsub=3, dt=0.1))
+        assert np.array_equal(d2.spmv(x, transpose=True), dev.spmv(x, transpose=True))
+
+
+def test_device_resident_path_matches_host_path(case):
+    name, op, A, dev = case
+    say(f"[{name}] device-resident action")
+    x = np.random.default_rng(4).standard_normal(op.n)
+    dx = DeviceVector.from_host(x)
+    dy = DeviceVector(op.n)
+    expm_taylor_dev(dev, dx, dy, order=5, nsub=5, dt=0.1)
+    dev.synchronize()
+    assert np.array_equal(dy.download(), dev.expm_taylor(x, order=5, nsub=5, dt=0.1))
+
+
+def _chu_weight(op, seed):
+    """A 4x4 (b x b) block-diagonal SPD weight shaped like R'MR, plus a
+    diagonal on the extra DOFs."""
+    rng = np.random.default_rng(seed)
+    b = op.b
+    G = rng.standard_normal((op.ncell, b, b))
+    blocks = np.einsum("cij,ckj->cik", G, G) + b * np.eye(b)
+    brow_ptr = np.arange(op.ncell + 1, dtype=np.int64)
+    bcol = np.arange(op.ncell, dtype=np.int32)
+    xr = op.b * op.ncell + np.arange(op.n_extra)
+    return ops.BlockOperator(op.nchi, op.neta, b, op.n_extra, brow_ptr, bcol, blocks,
+                             xr.astype(np.int64), xr.astype(np.int64), rng.uniform(0.5, 1.5, op.n_extra))
+
+
+def test_chu_sandwich_parity(case, orc):
+    name, op, A, dev = case
+    say(f"[{name}] EXPONENTIAL! sandwich")
+    W = _chu_weight(op, 5)
+    Cw = orc.CSC.from_arrays(*W.to_csc_varmajor())
+    dt, nt = 0.1, 3
+    x = np.random.default_rng(6).standard_normal(op.n)
+    At = orc.csc_adjoint(A)
+    ref = orc.construct_c(x, A.scaled(dt), At.scaled(dt), Cw, nt, False, 0)
+    with DeviceOperator.from_csc(op.n, Cw.colptr, Cw.rowval, Cw.nzval, vars_per_cell=op.b,
+                                 nchi=op.nchi, neta=op.neta) as dW:
+        got = dev.chu_sandwich(dW, x, order=5, nsub=nt, dt=dt)
+        assert relerr(got, ref) <= 1e-12
+        n0 = op.b * op.ncell // 2
+        ref0 = orc.construct_c(x, A.scaled(dt), At.scaled(dt), Cw, nt, True, n0)
+        got0 = dev.chu_sandwich(dW, x, order=5, nsub=nt, dt=dt, n_set0=n0)
+        assert not np.any(got0[:n0])
+        assert relerr(got0, ref0) <= 1e-12
+
+
+def test_f32_storage_error_bound(case, orc):
+    name, op, A, dev = case
+    say(f"[{name}] FP32-stored values")
+    x = np.random.default_rng(7).standard_normal(op.n)
+    with DeviceOperator.from_blocks(op, value_prec="f32") as d32:
+        ref = orc.power_method(A.scaled(0.1), x, 5)
+        err = relerr(d32.expm_taylor(x, order=5, nsub=5, dt=0.1), ref)
+        assert 1e-12 < err <= 2e-6       # really FP32 values, bounded error
+
+
+# ------------------------------------------------------------------ complex (receptivity)
+
+@pytest.fixture(scope="module")
+def recept(orc):
+    say("[receptivity] building extended complex operator")
+    base = ops.stencil_operator(10, 8, 4, seed=8, n_shock=10, row_sum=2.0)
+    L_ = ops.receptivity_extension(base, [0.0, 0.7, 2.1, 5.0])
+    A = orc.CSC.from_arrays(*L_.to_csc_varmajor())
+    dev = DeviceOperator.from_csc(L_.n, A.colptr, A.rowval, A.nzval, vars_per_cell=4,
+                                  nchi=10, neta=8)
+    yield base, L_, A, dev
+    dev.close()
+
+
+def _cvec(n, seed):
+    r = np.random.default_rng(seed)
+    return r.standard_normal(n) + 1j * r.standard_normal(n)
+
+
+def test_complex_spmv_and_taylor(recept, orc):
+    base, L_, A, dev = recept
+    x = _cvec(L_.n, 9)
+    say("[receptivity] complex spmv / Taylor")
+    assert relerr(dev.spmv(x), orc.csc_mul(A, x)) <= 1e-14
+    assert relerr(dev.spmv(x, transpose=True), orc.csc_mul(orc.csc_adjoint(A), x)) <= 1e-14
+    ref = orc.power_method(A.scaled(0.05), x, 6)
+    assert relerr(dev.expm_taylor(x, order=5, nsub=6, dt=0.05), ref) <= 1e-12
+
+
+def test_complex_sandwich_with_freestream_zeroing(recept, orc):
+    base, L_, A, dev = recept
+    say("[receptivity] CONSTRUCT_C with N_set_0 zeroing")
+    W = _chu_weight(ops.BlockOperator(L_.nchi, L_.neta, 4, L_.n_extra, base.brow_ptr, base.bcol,
+                                      base.bval), 10)
+    Cw = orc.CSC.from_arrays(*W.to_csc_varmajor())
+    n0 = 4 * base.ncell
+    x = _cvec(L_.n, 11)
+    At = orc.csc_adjoint(A)
+    ref = orc.construct_c(x, A.scaled(0.05), At.scaled(0.05), Cw, 4, True, n0)
+    with DeviceOperator.from_csc(L_.n, Cw.colptr, Cw.rowval, Cw.nzval, vars_per_cell=4,
+                                 nchi=L_.nchi, neta=L_.neta) as dW:
+        got = dev.chu_sandwich(dW, x, order=5, nsub=4, dt=0.05, n_set0=n0)
+    assert not np.any(got[:n0])
+    assert relerr(got, ref) <= 1e-12
+
+
+# ------------------------------------------------------------------ closed forms
+
+def test_true_exponential_block_circulant(orc):
+    say("[circulant] T_12(A T/64)^64 vs exact exp(A T)")
+    for b in (4, 8):
+        op, blocks = ops.circulant_operator(12, 9, b, seed=3, row_sum=8.0)
+        perm = ops.varmajor_to_cellmajor_index(op.ncell, b, op.n)
+        x = np.random.default_rng(12).standard_normal(op.n)
+        x_cm = np.empty_like(x)
+        x_cm[perm] = x
+        ref = orc.circulant_expm(blocks, op.nchi, op.neta, x_cm, 1.0)[perm]
+        with DeviceOperator.from_blocks(op) as dev:
+            got = dev.expm_taylor(x, order=12, nsub=64, dt=1.0 / 64, form="normalised")
+        assert relerr(got, ref) <= 1e-10
+
+
+# ------------------------------------------------------------------ full config sizes
+
+@pytest.fixture(scope="module")
+def c1(orc):
+    say("[C1 100x80 b4] building")
+    op = ops.config_operator("C1")
+    A = orc.CSC.from_arrays(*op.to_csc_varmajor())
+    dev = DeviceOperator.from_csc(op.n, A.colptr, A.rowval, A.nzval, vars_per_cell=4, nchi=100,
+                                  neta=80)
+    yield op, A, dev
+    dev.close()
+
+
+def test_c1_action_vs_oracle(c1, orc):
+    op, A, dev = c1
+    say("[C1] T_12(A/64)^64 v vs oracle")
+    x = np.random.default_rng(1).standard_normal(op.n)
+    ref = orc.power_method(A.scaled(1.0 / 64), x, 64, order=12)
+    got = dev.expm_taylor(x, order=12, nsub=64, dt=1.0 / 64, form="powers")
+    assert relerr(got, ref) <= 1e-11
+
+
+def test_c1_linearity_composition_adjoint(c1):
+    op, A, dev = c1
+    say("[C1] linearity / composition / adjoint dot product")
+    r = np.random.default_rng(2)
+    x, y = r.standard_normal(op.n), r.standard_normal(op.n)
+    T = lambda v, n=8, tr=False: dev.expm_taylor(v, order=12, nsub=n, dt=1 / 64, transpose=tr)
+    assert relerr(T(2.0 * x - 3.0 * y), 2.0 * T(x) - 3.0 * T(y)) <= 1e-13
+    assert relerr(T(T(x, 3), 5), T(x, 8)) <= 1e-13
+    lhs, rhs = T(x) @ y, x @ T(y, tr=True)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_c2_shock_config_properties(orc):
+    say("[C2 200x300 b4 + 200 shock DOFs] properties at full size")
+    op = ops.config_operator("C2")
+    with DeviceOperator.from_blocks(op) as dev:
+        r = np.random.default_rng(3)
+        x, y = r.standard_normal(op.n), r.standard_normal(op.n)
+        T = lambda v, tr=False: dev.expm_taylor(v, order=5, nsub=4, dt=1 / 64, transpose=tr)
+        lhs, rhs = T(x) @ y, x @ T(y, tr=True)
+        assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+        # one SpMV against the oracle at full size
+        A = orc.CSC.from_arrays(*op.to_csc_varmajor())
+        assert relerr(dev.spmv(x), orc.csc_mul(A, x)) <= 1e-14
+
+
+def test_argument_errors_on_gpu():
+    op = make_case("grid20x16_b4")
+    say("[errors] adjoint missing / dtype mismatch")
+    from hymor_amd._lib import HymorError
+    with DeviceOperator.from_blocks(op, with_adjoint=False) as dev:
+        with pytest.raises(HymorError, match="adjoint"):
+            dev.expm_taylor(np.zeros(op.n), transpose=True)
+        with pytest.raises(HymorError, match="order"):
+            dev.expm_taylor(np.zeros(op.n), order=0)
