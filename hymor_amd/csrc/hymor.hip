@@ -71,12 +71,41 @@ struct hymor_op {
     // workspace: [0] stage (var-major), [1] in0, [2] va, [3] vb, [4] t0, [5] t1, [6] spare
     int ws_nv = 0;
     double *ws[7] = {};
-    double *yx = nullptr;
+    // partial sums of the extra rows feeding the block kernel, one buffer per
+    // direction: the forward and adjoint operators list different rows, and a
+    // row the current direction never writes must read as zero
+    double *yx = nullptr, *yxa = nullptr;
     int64_t dev_bytes = 0;
     std::vector<void *> allocs;
 };
 
+// Block-diagonal weights (the Chu matrices R'MR): one b x b block per cell
+// and weight, a diagonal on the extra DOFs.
+struct hymor_weights {
+    int b = 0, nmat = 0;
+    int64_t nchi = 0, neta = 0, ncell = 0, n = 0, n_extra = 0;
+    double *blocks = nullptr;  // nmat * ncell * b * b
+    double *xdiag = nullptr;   // nmat * n_extra
+    double *partial = nullptr; // nmat * nblocks of the quadratic-form kernel
+    int64_t nblocks = 0;
+    std::vector<void *> allocs;
+};
+
 namespace {
+
+constexpr int kMaxWeights = 8;
+
+void *dalloc_raw(std::vector<void *> &allocs, size_t bytes) {
+    void *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw std::bad_alloc();
+    }
+    allocs.push_back(p);
+    return p;
+}
 
 void *dalloc(hymor_op *op, size_t bytes) {
     void *p = nullptr;
@@ -150,14 +179,15 @@ void ensure_ws(hymor_op *op, int nv) {
             }
         op->ws[i] = nullptr;
     }
-    if (op->yx) {
+    for (double **py : {&op->yx, &op->yxa}) {
+        if (!*py) continue;
         for (size_t k = 0; k < op->allocs.size(); ++k)
-            if (op->allocs[k] == op->yx) {
-                (void)hipFree(op->yx);
+            if (op->allocs[k] == *py) {
+                (void)hipFree(*py);
                 op->allocs.erase(op->allocs.begin() + (long)k);
                 break;
             }
-        op->yx = nullptr;
+        *py = nullptr;
     }
     const size_t vb = (size_t)op->n * nv * sizeof(double);
     for (int i = 0; i < 7; ++i) {
@@ -165,9 +195,11 @@ void ensure_ws(hymor_op *op, int nv) {
         HIP_TRY(hipMemsetAsync(op->ws[i], 0, vb, op->stream));
     }
     const size_t yb = (size_t)op->nflow * nv * sizeof(double);
-    op->yx = (double *)dalloc(op, yb);
     // rows without extra entries are never written: they must read as zero
+    op->yx = (double *)dalloc(op, yb);
     HIP_TRY(hipMemsetAsync(op->yx, 0, yb, op->stream));
+    op->yxa = (double *)dalloc(op, yb);
+    HIP_TRY(hipMemsetAsync(op->yxa, 0, yb, op->stream));
     HIP_TRY(hipStreamSynchronize(op->stream));
     op->ws_nv = nv;
 }
@@ -215,12 +247,12 @@ void launch_extra_nv(const DevExtra &xe, int64_t nflow, hipStream_t st, const do
                            vout, c1, c2);
 }
 
-// One Taylor term (or a plain SpMV when vout == nullptr) on `st`.
-// yx is the partial-sum buffer of the operator whose extras are applied.
-void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, double *yx, const double *x,
-                 double *tout, const double *vin, double *vout, double c1, double c2) {
+// One Taylor term (or a plain SpMV when vout == nullptr) of `op` on `st`.
+void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, const double *x, double *tout,
+                 const double *vin, double *vout, double c1, double c2) {
     const DevSliced &s = adj ? op->adj : op->fwd;
     const DevExtra &xe = adj ? op->xa : op->xf;
+    double *yx = adj ? op->yxa : op->yx;
     const bool has_x = xe.nrows > 0;
     if (has_x) {
         if (nv == 1) launch_extra_nv<1>(xe, op->nflow, st, x, yx, tout, vin, vout, c1, c2);
@@ -277,7 +309,7 @@ const double *run_taylor(hymor_op *op, bool adj, int order, int64_t nsub, double
             }
             double *tout = (j < order) ? ((j & 1) ? t0 : t1) : nullptr;
             const double *vin = (j == 1) ? cur : nxt;
-            launch_term(op, adj, nv, op->stream, op->yx, x, tout, vin, nxt, c1, c2);
+            launch_term(op, adj, nv, op->stream, x, tout, vin, nxt, c1, c2);
             x = tout;
         }
         cur = nxt;
@@ -491,7 +523,7 @@ int hymor_spmv(hymor_op *op, int transpose, const void *x, void *y, int dtype) {
         const size_t bytes = (size_t)op->n * nv * sizeof(double);
         HIP_TRY(hipMemcpyAsync(op->ws[0], x, bytes, hipMemcpyHostToDevice, op->stream));
         launch_perm_in(op, op->stream, op->ws[0], op->n, op->ws[1], 1, nv, 0);
-        launch_term(op, transpose != 0, nv, op->stream, op->yx, op->ws[1], op->ws[2], nullptr,
+        launch_term(op, transpose != 0, nv, op->stream, op->ws[1], op->ws[2], nullptr,
                     nullptr, 1.0, 0.0);
         launch_perm_out(op, op->stream, op->ws[2], op->ws[0], op->n, 1, nv, 0);
         HIP_TRY(hipMemcpyAsync(y, op->ws[0], bytes, hipMemcpyDeviceToHost, op->stream));
@@ -534,7 +566,7 @@ static void sandwich_core(hymor_op *A, hymor_op *C, int order, int64_t nsub, dou
     launch_perm_in(A, A->stream, x_vm, A->n, A->ws[1], 1, nv, n_set0);
     const double *r1 = run_taylor(A, false, order, nsub, dt, HYMOR_FORM_NORMALISED, nv, A->ws[1]);
     // weight: ws[6] = C * r1 (C's extras use C's own partial-sum buffer)
-    launch_term(C, false, nv, A->stream, C->yx, r1, A->ws[6], nullptr, nullptr, 1.0, 0.0);
+    launch_term(C, false, nv, A->stream, r1, A->ws[6], nullptr, nullptr, 1.0, 0.0);
     const double *r2 = run_taylor(A, true, order, nsub, dt, HYMOR_FORM_NORMALISED, nv, A->ws[6]);
     launch_perm_out(A, A->stream, r2, y_vm, A->n, 1, nv, n_set0);
 }
@@ -627,12 +659,12 @@ int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int reps, doubl
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         // one warm launch, then the timed ones
-        launch_term(op, transpose != 0, nv, op->stream, op->yx, op->ws[4], op->ws[5], op->ws[2],
-                    op->ws[2], 0.5, 1.0);
+        launch_term(op, transpose != 0, nv, op->stream, op->ws[4], op->ws[5], op->ws[2], op->ws[2],
+                    0.5, 1.0);
         HIP_TRY(hipEventRecord(e0, op->stream));
         for (int i = 0; i < reps; ++i)
-            launch_term(op, transpose != 0, nv, op->stream, op->yx, op->ws[4], op->ws[5],
-                        op->ws[2], op->ws[2], 0.5, 1.0);
+            launch_term(op, transpose != 0, nv, op->stream, op->ws[4], op->ws[5], op->ws[2],
+                        op->ws[2], 0.5, 1.0);
         HIP_TRY(hipEventRecord(e1, op->stream));
         HIP_TRY(hipEventSynchronize(e1));
         float ms = 0.f;

@@ -199,3 +199,80 @@ __global__ void k_perm_out(const double *__restrict__ in, double *__restrict__ y
 }
 
 }  // namespace hymor
+
+namespace hymor {
+
+// Quadratic forms q_m = re(v)' W_m re(v) for nmat block-diagonal weights.
+// Items 0..ncell-1 are cells (b x b block per cell and weight, row-major,
+// layout W[m][cell][i][k]); items ncell.. are extra-DOF rows with a
+// diagonal weight Wx[m][row - nflow].  Only the real part of v is used
+// (component 0 of each NV-double row), as the reference forms real(v).
+// Each workgroup writes one partial per weight; k_reduce_partials sums them
+// in a fixed order, so results do not depend on scheduling.
+template <int B, int NV, int NMAX>
+__global__ __launch_bounds__(256) void k_quad_blockdiag(
+    const double *__restrict__ v, const double *__restrict__ W, const double *__restrict__ Wx,
+    int nmat, int64_t ncell, int64_t n_extra, double *__restrict__ partial) {
+    __shared__ double red[NMAX][4];
+    const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double q[NMAX];
+#pragma unroll
+    for (int m = 0; m < NMAX; ++m) q[m] = 0.0;
+    if (item < ncell) {
+        double vc[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) vc[k] = v[(item * B + k) * NV];
+#pragma unroll
+        for (int m = 0; m < NMAX; ++m) {
+            if (m >= nmat) break;
+            const double *w = W + ((int64_t)m * ncell + item) * (B * B);
+            double s = 0.0;
+#pragma unroll
+            for (int i = 0; i < B; ++i) {
+                double wi[B];
+                load_f64<B>(w + i * B, wi);
+                double r = 0.0;
+#pragma unroll
+                for (int k = 0; k < B; ++k) r = fma(wi[k], vc[k], r);
+                s = fma(vc[i], r, s);
+            }
+            q[m] = s;
+        }
+    } else if (item < ncell + n_extra) {
+        const int64_t e = item - ncell;
+        const double ve = v[(ncell * B + e) * NV];
+#pragma unroll
+        for (int m = 0; m < NMAX; ++m) {
+            if (m >= nmat) break;
+            q[m] = Wx[(int64_t)m * n_extra + e] * ve * ve;
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int m = 0; m < NMAX; ++m) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) q[m] += __shfl_xor(q[m], off, 64);
+        if (lane == 0) red[m][wave] = q[m];
+    }
+    __syncthreads();
+    if (threadIdx.x < nmat) {
+        const int m = threadIdx.x;
+        partial[(int64_t)m * gridDim.x + blockIdx.x] = (red[m][0] + red[m][1]) + (red[m][2] + red[m][3]);
+    }
+}
+
+// out[m*ostride] = sum of partial[m][0..np) in a fixed order; one block per m.
+__global__ __launch_bounds__(256) void k_reduce_partials(const double *__restrict__ partial,
+                                                         int64_t np, double *__restrict__ out,
+                                                         int64_t ostride) {
+    __shared__ double red[4];
+    const int m = blockIdx.x;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < np; i += blockDim.x) s += partial[(int64_t)m * np + i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[(int64_t)m * ostride] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+}  // namespace hymor

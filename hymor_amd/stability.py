@@ -1,0 +1,458 @@
+"""Host-side mirror of HYMOR's stability entry points over libhymor_hip.
+
+Same names, argument meaning and error behaviour as the Julia functions in
+utils/Stability_analysis/ (Julia's ``!`` suffix becomes ``_``).  The Taylor
+actions run on the GPU through the C ABI; the one-time host constructions
+(Chu weights, CFL time step) are numpy restatements of the reference's own
+code; the Krylov driver is ARPACK, as in the reference (Arpack.jl there,
+scipy's bundled ARPACK-ng here), with the same which/tol/maxiter settings.
+
+Matrices passed in are scipy sparse matrices in the reference's layout
+(variable-major state, chi-fastest cells, 4 variables per cell, optional
+Nchi shock DOFs, optional freestream blocks).  Base-flow dictionaries follow
+the reference's ``s`` (see hymor_amd.baseflow for the keys used).  Array
+fields use Julia's column-major flattening (chi fastest).
+"""
+from __future__ import annotations
+
+import time
+import warnings
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from . import _lib
+from .core import DeviceOperator
+
+ORDER = 5            # Taylor order hard-coded by the reference
+B = 4                # conserved variables per cell in the reference
+
+
+def _vec(a):
+    """Julia vec(): column-major flattening (chi fastest)."""
+    return np.asarray(a).reshape(-1, order="F")
+
+
+def _interior(a):
+    return np.asarray(a)[1:-1, 1:-1]
+
+
+def _use_gpu():
+    # CUDA.functional() in the reference; here the engine has no CPU path,
+    # so a missing GPU is an error at the first device call.
+    return _lib.device_available()
+
+
+# ====================================================================== dt
+
+def CFL_TIMESTEP(s):
+    """Time step from the CFL condition (utils/Time_marching/CFL_TIMESTEP.jl:41-138).
+
+    Sets ``s["time_integration"]["dt"]`` and returns ``s``.  If ``s`` holds
+    no base-flow fields, an existing ``dt`` is kept (callers that already
+    know dt pass it that way).
+    """
+    ti = s["time_integration"]
+    if "var" not in s:
+        if "dt" not in ti:
+            raise KeyError("s has neither base-flow fields nor time_integration.dt")
+        return s
+    var, mesh, shock, flux = s["var"], s["mesh"], s["shock"], s["flux"]
+    rho, rho_u, rho_v, rho_E = var["rho"], var["rho_u"], var["rho_v"], var["rho_E"]
+    a, gam, Re, Pr = var["a"], var["gamma_star"], var["Re_flow"], var["Pr_flow"]
+    lr_xn, lr_yn, lr_a = mesh["lr_x_normal"], mesh["lr_y_normal"], mesh["lr_area"]
+    bt_xn, bt_yn, bt_a = mesh["bt_x_normal"], mesh["bt_y_normal"], mesh["bt_area"]
+    vol = mesh["volume"]
+    fc = shock["flow_cells"]
+    c = np.max(np.abs(a))                                                  # :80
+    rs = rho[:-1, 1:-1] + rho[1:, 1:-1]                                    # :84-88
+    ul = (rho_u[:-1, 1:-1] + rho_u[1:, 1:-1]) / rs
+    vl = (rho_v[:-1, 1:-1] + rho_v[1:, 1:-1]) / rs
+    lam_lr = (np.abs(ul * lr_xn + vl * lr_yn) + c) * lr_a
+    rs = rho[1:-1, :-1] + rho[1:-1, 1:]                                    # :91-95
+    ub = (rho_u[1:-1, :-1] + rho_u[1:-1, 1:]) / rs
+    vb = (rho_v[1:-1, :-1] + rho_v[1:-1, 1:]) / rs
+    lam_bt = (np.abs(ub * bt_xn + vb * bt_yn) + c) * bt_a
+    lam_c = np.maximum(lam_lr[:-1, :], lam_lr[1:, :]) + np.maximum(lam_bt[:, :-1], lam_bt[:, 1:])
+    ri, gi, Rei, Pri = _interior(rho), _interior(gam), _interior(Re), _interior(Pr)
+    A_xi = 0.5 * (lr_a[:-1, :] + lr_a[1:, :])                              # :108-112
+    A_eta = 0.5 * (bt_a[:, :-1] + bt_a[:, 1:])
+    nu = np.maximum(4.0 / 3.0, gi / Pri) / (ri * Rei)
+    lam_v = 2 * nu * (A_xi ** 2 + A_eta ** 2) / vol
+    dt1 = ti["CFL"] / np.max((lam_c + lam_v) * fc / vol)                   # :115-116
+    rEi = _interior(rho_E)
+    if shock["enabled"]:                                                   # :121-129
+        mu = np.max((np.abs(flux["rho_v"]) + np.abs(flux["rho_u"])) / ri * fc)
+        mr = np.max(np.abs(flux["rho"] / ri * fc))
+        mE = np.max(np.abs(flux["rho_E"] / rEi * fc))
+    else:
+        mu = np.max((np.abs(flux["rho_v"]) + np.abs(flux["rho_u"])) / ri)
+        mr = np.max(np.abs(flux["rho"] / ri))
+        mE = np.max(np.abs(flux["rho_E"] / rEi))
+    dt2 = ti["CFL"] / max(mu, mr, mE)
+    ti["dt"] = min(dt1, dt2, ti["max_dt"])                                 # :133
+    return s
+
+
+# ====================================================================== Chu weights
+
+# structural pattern of R: the 12 entries CONSTRUCT_R.jl:51-101 assigns
+_R_PATTERN = np.array([[1, 1, 1, 1], [1, 1, 0, 0], [1, 0, 1, 0], [1, 1, 1, 1]], dtype=bool)
+
+
+def _r_blocks(gam, p, rho, u, v):
+    """Per-point 4x4 conservative -> (p, u, v, S) Jacobian, rows/cols
+    (p,u,v,S) x (rho, rho_u, rho_v, rho_E) (CONSTRUCT_R.jl:51-101)."""
+    m = gam.shape[0]
+    R = np.zeros((m, 4, 4))
+    R[:, 0, 0] = (gam - 1) * (u ** 2 + v ** 2) / 2
+    R[:, 0, 1] = -u * (gam - 1)
+    R[:, 0, 2] = -v * (gam - 1)
+    R[:, 0, 3] = gam - 1
+    R[:, 1, 0] = -u / rho
+    R[:, 1, 1] = 1.0 / rho
+    R[:, 2, 0] = -v / rho
+    R[:, 2, 2] = 1.0 / rho
+    R[:, 3, 0] = (u ** 2 + v ** 2) / (2 * p) - gam / (gam - 1) / rho
+    R[:, 3, 1] = -u / p
+    R[:, 3, 2] = -v / p
+    R[:, 3, 3] = 1.0 / p
+    return R
+
+
+def _blocks_varmajor(R, npts, n_total, diag_tail=None):
+    """Assemble per-point 4x4 blocks into a var-major sparse matrix over
+    4*npts unknowns (+ an optional diagonal on the tail)."""
+    pts = np.arange(npts)
+    rows = (np.arange(4)[None, :, None] * npts + pts[:, None, None]) + np.zeros((1, 1, 4), int)
+    cols = (np.arange(4)[None, None, :] * npts + pts[:, None, None]) + np.zeros((1, 4, 1), int)
+    r, c, v = rows.reshape(-1), cols.reshape(-1), R.reshape(-1)
+    keep = np.broadcast_to(_R_PATTERN, R.shape).reshape(-1)   # entries the reference sets
+    r, c, v = r[keep], c[keep], v[keep]
+    if diag_tail is not None:
+        t = np.arange(diag_tail.shape[0]) + 4 * npts
+        r, c, v = np.concatenate([r, t]), np.concatenate([c, t]), np.concatenate([v, diag_tail])
+    return sp.csc_matrix((v, (r, c)), shape=(n_total, n_total))
+
+
+def _flow(s):
+    var = s["var"]
+    gam = _vec(_interior(var["gamma_star"]))
+    p = _vec(_interior(var["p"]))
+    rho = _vec(_interior(var["rho"]))
+    u = _vec(_interior(var["rho_u"])) / rho
+    v = _vec(_interior(var["rho_v"])) / rho
+    a = _vec(_interior(var["a"]))
+    return gam, p, rho, u, v, a
+
+
+def _n_state(s):
+    nx, ny = s["mesh"]["Nchi"], s["mesh"]["Neta"]
+    return 4 * nx * ny + (nx if s["stability_analysis"]["perturb_shock"] else 0)
+
+
+def CONSTRUCT_R(s):
+    """Conservative -> Chu-variable transform (CONSTRUCT_R.jl:24-133)."""
+    nx, ny = s["mesh"]["Nchi"], s["mesh"]["Neta"]
+    gam, p, rho, u, v, _ = _flow(s)
+    R = _r_blocks(gam, p, rho, u, v)
+    tail = np.ones(nx) if s["stability_analysis"]["perturb_shock"] else None   # :114-121
+    return _blocks_varmajor(R, nx * ny, _n_state(s), tail)
+
+
+def _m_diag(rho, a, gam, p, vol, norms):
+    """Diagonal Chu weights per point, (npts, 4) (CONSTRUCT_M.jl:50-75)."""
+    w = np.zeros((rho.shape[0], 4))
+    if norms[0]:
+        w[:, 0] = rho * a ** 2 / (2 * (gam * p) ** 2) * vol
+    if norms[1]:
+        w[:, 1] = rho / 2 * vol
+    if norms[2]:
+        w[:, 2] = rho / 2 * vol
+    if norms[3]:
+        w[:, 3] = (gam - 1) * p / (2 * gam) * vol
+    return w
+
+
+def _diag_varmajor(w, n_total, tail=None, present=None):
+    npts = w.shape[0]
+    idx = (np.arange(4)[None, :] * npts + np.arange(npts)[:, None])
+    cols = [k for k in range(4) if present is None or present[k]]
+    r = idx[:, cols].reshape(-1)
+    v = w[:, cols].reshape(-1)
+    if tail is not None:
+        t = np.arange(tail.shape[0]) + 4 * npts
+        r, v = np.concatenate([r, t]), np.concatenate([v, tail])
+    return sp.csc_matrix((v, (r, r)), shape=(n_total, n_total))
+
+
+def CONSTRUCT_M(s, norms):
+    """Chu energy weights, masked by flow_cells (CONSTRUCT_M.jl:27-94)."""
+    gam, p, rho, u, v, a = _flow(s)
+    vol = _vec(s["mesh"]["volume"]) * _vec(s["shock"]["flow_cells"])
+    present = [bool(x) for x in norms]
+    return _diag_varmajor(_m_diag(rho, a, gam, p, vol, present), _n_state(s), present=present)
+
+
+def CONSTRUCT_M_HAT(s, norms):
+    """Unmasked Chu weights plus shock weights bt_area/2 (CONSTRUCT_M_HAT.jl:25-105)."""
+    gam, p, rho, u, v, a = _flow(s)
+    vol = _vec(s["mesh"]["volume"])
+    present = [bool(x) for x in norms]
+    tail = None
+    if s["stability_analysis"]["perturb_shock"]:
+        nx = s["mesh"]["Nchi"]
+        ci = np.asarray(s["shock"]["cell_indices"]).reshape(nx, -1)[:, 0].astype(int) - 1
+        tail = s["mesh"]["bt_area"][np.arange(nx), ci] / 2                     # :79-82
+    return _diag_varmajor(_m_diag(rho, a, gam, p, vol, present), _n_state(s), tail, present)
+
+
+def _freestream_pts(s):
+    fs = s["freestream"]
+    nx = s["mesh"]["Nchi"]
+    one = np.ones(nx)
+    return (fs["gamma_star"] * one, fs["p_0"] * one, fs["rho_0"] * one,
+            fs["rho_u_0"] / fs["rho_0"] * one, fs["rho_v_0"] / fs["rho_0"] * one,
+            fs["a_0"] * one)
+
+
+def CONSTRUCT_R_INFTY(s):
+    """Freestream transform, 4*Nchi square (CONSTRUCT_R_.jl:55-145)."""
+    gam, p, rho, u, v, _ = _freestream_pts(s)
+    nx = s["mesh"]["Nchi"]
+    return _blocks_varmajor(_r_blocks(gam, p, rho, u, v), nx, 4 * nx)
+
+
+def CONSTRUCT_R_(s, w_infty):
+    """blockdiag(R, R_infty x N) (CONSTRUCT_R_.jl:27-37)."""
+    Rinf = CONSTRUCT_R_INFTY(s)
+    return sp.block_diag([CONSTRUCT_R(s)] + [Rinf] * len(w_infty), format="csc")
+
+
+def CONSTRUCT_M_(s, norms, w_infty):
+    """blockdiag(M, 0) (CONSTRUCT_M_.jl:29-38)."""
+    nx = s["mesh"]["Nchi"]
+    M = CONSTRUCT_M(s, norms)
+    z = 4 * nx * len(w_infty)
+    return sp.block_diag([M, sp.csc_matrix((z, z))], format="csc")
+
+
+def GET_T_REF(s):
+    """Reference advection time: post-shock mass / inflow mass flux (GET_T_REF.jl:27-48)."""
+    mass = np.sum(_interior(s["var"]["rho"]) * s["shock"]["flow_cells"] * s["mesh"]["volume"])
+    fs = s["freestream"]
+    nx = s["mesh"]["Nchi"]
+    ci = np.asarray(s["shock"]["cell_indices"]).reshape(nx, -1)[:, 0].astype(int) - 1
+    i = np.arange(nx)
+    flux = 0.0
+    for k in range(nx):           # sequential sum, as the reference's loop
+        nd = fs["rho_u_0"] * s["mesh"]["bt_x_normal"][i[k], ci[k]] + \
+            fs["rho_v_0"] * s["mesh"]["bt_y_normal"][i[k], ci[k]]
+        flux += s["mesh"]["bt_area"][i[k], ci[k]] * abs(nd)
+    return mass / flux
+
+
+def CONSTRUCT_M_INFTY(s, norms, T, scaling_non_temporal):
+    """Freestream inflow weights (CONSTRUCT_M_INFTY_.jl:72-164)."""
+    gam, p, rho, u, v, a = _freestream_pts(s)
+    nx = s["mesh"]["Nchi"]
+    fs = s["freestream"]
+    vmag = np.sqrt(fs["rho_u_0"] ** 2 + fs["rho_v_0"] ** 2)
+    uh, vh = fs["rho_u_0"] / vmag, fs["rho_v_0"] / vmag
+    ci = np.asarray(s["shock"]["cell_indices"]).reshape(nx, -1)[:, 0].astype(int) - 1
+    i = np.arange(nx)
+    proj = s["mesh"]["bt_x_normal"][i, ci] * uh + s["mesh"]["bt_y_normal"][i, ci] * vh
+    arc = s["mesh"]["bt_area"][i, ci] * np.abs(proj)
+    scaling = GET_T_REF(s) if scaling_non_temporal else T
+    present = [bool(x) for x in norms]
+    return _diag_varmajor(_m_diag(rho, a, gam, p, arc * scaling, present), 4 * nx,
+                          present=present)
+
+
+def CONSTRUCT_M_INFTY_(s, norms, T, w_infty, scaling_non_temporal):
+    """blockdiag(M_hat, M_infty (halved for w != 0) x N) (CONSTRUCT_M_INFTY_.jl:32-50)."""
+    Mh = CONSTRUCT_M_HAT(s, norms)
+    Mi = CONSTRUCT_M_INFTY(s, norms, T, scaling_non_temporal)
+    blocks = [Mh] + [Mi if w == 0 else Mi / 2 for w in w_infty]
+    return sp.block_diag(blocks, format="csc")
+
+
+# ====================================================================== Taylor actions
+
+def _grid(s):
+    return s["mesh"]["Nchi"], s["mesh"]["Neta"]
+
+
+def upload(A, s, scale=1.0, value_prec="f64"):
+    """CuSparseMatrixCSC(A*scale) -> a device operator (forward + adjoint)."""
+    nx, ny = _grid(s)
+    return DeviceOperator.from_scipy(A, nchi=nx, neta=ny, vars_per_cell=B, scale=scale,
+                                     value_prec=value_prec)
+
+
+def POWER_METHOD_(v, A: DeviceOperator, timesteps, vout=None, term=None):
+    """POWER_METHOD! (Eigenvalues/ARNOLDI_POWER_GPU.jl:64-82): v <- T_5(A)^timesteps v
+    in place; A is the device operator of A*dt.  vout/term are accepted for
+    signature parity (the library owns its buffers)."""
+    v[...] = A.expm_taylor(v, order=ORDER, nsub=int(timesteps), dt=1.0, form="powers")
+
+
+def ARNOLDI_POWER_GPU(x, Adt: DeviceOperator, timesteps, use_gpu=True, x_buf=None, vout_buf=None,
+                      term_buf=None, x_cpu_buf=None):
+    """ARPACK callback adapter (ARNOLDI_POWER_GPU.jl:36-52): returns a new vector."""
+    xc = np.ascontiguousarray(x)
+    return Adt.expm_taylor(xc, order=ORDER, nsub=int(timesteps), dt=1.0, form="powers")
+
+
+def LINEAR_INTEGRATION_GPU(x, n_t, order, Adt: DeviceOperator, use_gpu=True, v_buf=None,
+                           vout_buf=None, term_buf=None):
+    """PLOT_MODES.jl:330-351: T_order(A dt)^n_t x."""
+    return Adt.expm_taylor(np.ascontiguousarray(x), order=int(order), nsub=int(n_t), dt=1.0,
+                           form="powers")
+
+
+def EXPONENTIAL_(v, A: DeviceOperator, At, C_no_exp: DeviceOperator, timesteps, vout=None,
+                 term=None):
+    """EXPONENTIAL! (CONSTRUCT_C.jl:78-111): v <- T5(A'dt)^n C T5(A dt)^n v in
+    place.  ``At`` is accepted for signature parity: the handle of A*dt
+    holds its own adjoint."""
+    v[...] = A.chu_sandwich(C_no_exp, v, order=ORDER, nsub=int(timesteps), dt=1.0)
+
+
+def CONSTRUCT_C(x, A_dt: DeviceOperator, At_dt, C_no_exp: DeviceOperator, n_t,
+                freestream_disturbances, N_set_0, use_gpu=True, x_buf=None, vout_buf=None,
+                term_buf=None, x_cpu_buf=None):
+    """CONSTRUCT_C.jl:33-58, with the zeroing of x[1:N_set_0] before and after."""
+    n0 = int(N_set_0) if freestream_disturbances else 0
+    return A_dt.chu_sandwich(C_no_exp, np.ascontiguousarray(x), order=ORDER, nsub=int(n_t), dt=1.0,
+                             n_set0=n0)
+
+
+# ====================================================================== eigen drivers
+
+def _sort_lm(vals, vecs):
+    order = np.argsort(-np.abs(vals), kind="stable")
+    return vals[order], vecs[:, order]
+
+
+def EIGENVALUES(A, s, n_modes):
+    """Dispatcher (EIGENVALUES.jl:37-49)."""
+    solver = s["stability_analysis"]["eigenvalue_solver"]
+    if solver == "CPU_LU":
+        raise NotImplementedError("CPU_LU (shift-invert LU) is not part of the GPU engine; "
+                                  "use GPU_TIMESTEPPER_ARNOLDI")
+    if solver == "GPU_TIMESTEPPER_ARNOLDI":
+        return EIGENVALUES_GPU_ARNOLDI(A, s, n_modes)
+    raise ValueError(f"Unknown eigenvalue solver: {solver}")
+
+
+def EIGENVALUES_GPU_ARNOLDI(A, s, n_modes, *, T_ref=0.5, tol=1e-4, maxiter=None, ncv=None,
+                            v0=None):
+    """Arnoldi on the exponential transform (EIGENVALUES.jl:112-205).
+
+    dt from CFL_TIMESTEP, timesteps = round(T_ref/dt), A*dt uploaded once,
+    ARPACK which=LM, tol=1e-4, maxiter=40k; discrete eigenvalues mapped by
+    lambda = log(mu)/(dt*timesteps).  On any failure returns NaN V, D with a
+    warning, as the reference does.
+    """
+    s = CFL_TIMESTEP(s)
+    dt = s["time_integration"]["dt"]
+    timesteps = int(round(T_ref / dt))            # Julia round(Int, .) ties to even, as Python
+    n = A.shape[0]
+    k = n_modes
+    t0 = time.time()
+    dev = None
+    try:
+        dev = upload(A, s, scale=dt)
+        op = spla.LinearOperator((n, n), dtype=np.float64,
+                                 matvec=lambda x: ARNOLDI_POWER_GPU(x, dev, timesteps))
+        vals, vecs = spla.eigs(op, k=k, which="LM", tol=tol, maxiter=maxiter or 40 * k, ncv=ncv,
+                               v0=v0)
+        vals, vecs = _sort_lm(vals, vecs)
+        D = np.log(vals.astype(np.complex128)) / dt / timesteps          # :164-168
+        return vecs, D
+    except Exception as e:   # the reference returns NaN on any failure (:184-203)
+        warnings.warn(f"eigs did not converge: {e} -- returning NaN.")
+        return np.full((n, k), np.nan), np.full(k, np.nan + 1j * np.nan)
+    finally:
+        if dev is not None:
+            dev.close()
+        s.setdefault("timings", {})["eigenvalues"] = time.time() - t0
+
+
+def TRANSIENT_GROWTH_DOWNSTREAM(A, s, n_modes, T, *, tol=1e-4, maxiter=None, ncv=None, v0=None):
+    """Optimal transient growth (TRANSIENT_GROWTH_DOWNSTREAM.jl:34-153).
+
+    Returns (V, D, T_opt): D are the gains, eigenvalues of
+    exp(A'T) R'MR exp(AT) x = lambda R'M_hat R x (ARPACK, symmetric
+    generalized mode, which=LM, tol=1e-4, maxiter=10k).
+    """
+    k = n_modes
+    s = CFL_TIMESTEP(s)
+    n_t = int(round(T / s["time_integration"]["dt"]))
+    dt = T / n_t
+    T_opt = n_t * dt
+    R = CONSTRUCT_R(s)
+    norms = [1, 1, 1, 1]
+    C_no_exp = (R.T @ CONSTRUCT_M(s, norms) @ R).tocsc()
+    D_mat = (R.T @ CONSTRUCT_M_HAT(s, norms) @ R).tocsc()
+    n = A.shape[0]
+    devA = devC = None
+    try:
+        devA = upload(A, s, scale=dt)
+        devC = upload(C_no_exp, s)
+        op = spla.LinearOperator((n, n), dtype=np.float64,
+                                 matvec=lambda x: CONSTRUCT_C(x, devA, None, devC, n_t, False, 0))
+        vals, vecs = spla.eigsh(op, k=k, M=D_mat, which="LM", tol=tol, maxiter=maxiter or 10 * k,
+                                ncv=ncv, v0=v0)
+        order = np.argsort(-np.abs(vals), kind="stable")
+        return vecs[:, order], vals[order], T_opt
+    except Exception as e:
+        warnings.warn(f"eigs did not converge: {e} -- returning NaN.")
+        return np.full((n, k), np.nan), np.full(k, np.nan), T_opt
+    finally:
+        for d in (devA, devC):
+            if d is not None:
+                d.close()
+
+
+def FREESTREAM_RECEPTIVITY(A_, s, n_modes, T, w_infty, *, tol=1e-4, maxiter=None, ncv=None,
+                           v0=None):
+    """Freestream receptivity (FREESTREAM_RECEPTIVITY.jl:37-162).
+
+    A_ is the complex extended operator [L B..B; 0 kron(diag(-i w), I)].
+    Hermitian generalized ARPACK on CONSTRUCT_C with the first N_set_0 =
+    4*Nchi*Neta entries zeroed; gains are real(lambda)/2.
+    """
+    k = n_modes
+    s = CFL_TIMESTEP(s)
+    n_t = int(round(T / s["time_integration"]["dt"]))
+    dt = T / n_t
+    T_opt = n_t * dt
+    R_ = CONSTRUCT_R_(s, w_infty)
+    norms = [1, 1, 1, 1]
+    C_ = (R_.T @ CONSTRUCT_M_(s, norms, w_infty) @ R_).tocsc()
+    D_ = (R_.T @ CONSTRUCT_M_INFTY_(s, norms, T, w_infty, True) @ R_).tocsc()
+    nx, ny = _grid(s)
+    n_set_0 = 4 * nx * ny
+    n = A_.shape[0]
+    devA = devC = None
+    try:
+        devA = upload(sp.csc_matrix(A_, dtype=np.complex128), s, scale=dt)
+        devC = upload(C_, s)
+        op = spla.LinearOperator((n, n), dtype=np.complex128,
+                                 matvec=lambda x: CONSTRUCT_C(x, devA, None, devC, n_t, True,
+                                                              n_set_0))
+        vals, vecs = spla.eigsh(op, k=k, M=D_.astype(np.complex128), which="LM", tol=tol,
+                                maxiter=maxiter or 10 * k, ncv=ncv, v0=v0)
+        D = np.real(vals) / 2                                            # :115-119
+        order = np.argsort(-np.abs(D), kind="stable")
+        return vecs[:, order], D[order], T_opt
+    except Exception as e:
+        warnings.warn(f"eigs did not converge: {e} -- returning NaN.")
+        return np.full((n, k), np.nan), np.full(k, np.nan), T_opt
+    finally:
+        for d in (devA, devC):
+            if d is not None:
+                d.close()

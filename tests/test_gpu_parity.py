@@ -268,3 +268,20 @@ def test_argument_errors_on_gpu():
             dev.expm_taylor(np.zeros(op.n), transpose=True)
         with pytest.raises(HymorError, match="order"):
             dev.expm_taylor(np.zeros(op.n), order=0)
+
+
+def test_forward_after_adjoint_with_extras(orc):
+    """Regression: the forward and adjoint operators list different extra
+    rows (A12 vs A21'); interleaving them must not leak partial sums."""
+    op = make_case("shock12x10_b4")
+    say("[shock12x10_b4] adjoint spmv, then forward Taylor, then adjoint Taylor")
+    A = orc.CSC.from_arrays(*op.to_csc_varmajor())
+    At = orc.csc_adjoint(A)
+    x = np.random.default_rng(13).standard_normal(op.n)
+    with DeviceOperator.from_blocks(op) as dev:
+        for _ in range(2):
+            dev.spmv(x, transpose=True)
+            got = dev.expm_taylor(x, order=5, nsub=3, dt=0.1)
+            assert relerr(got, orc.power_method(A.scaled(0.1), x, 3)) <= 1e-13
+            got = dev.expm_taylor(x, order=5, nsub=3, dt=0.1, transpose=True)
+            assert relerr(got, orc.power_method(At.scaled(0.1), x, 3)) <= 1e-13
