@@ -37,6 +37,10 @@ SIGNATURES = {
                                     C.c_int]),
     "hymor_expm_taylor_dev": (C.c_int, [_p, C.c_int, C.c_int, _i64, C.c_double, C.c_int, _p, _p,
                                         C.c_int]),
+    "hymor_expm_taylor_batched": (C.c_int, [_p, C.c_int, C.c_int, _i64, C.c_double, C.c_int, _i64,
+                                            _p, _i64, _p, _i64, C.c_int]),
+    "hymor_expm_taylor_batched_dev": (C.c_int, [_p, C.c_int, C.c_int, _i64, C.c_double, C.c_int,
+                                                _i64, _p, _i64, _p, _i64, C.c_int]),
     "hymor_chu_sandwich": (C.c_int, [_p, _p, C.c_int, _i64, C.c_double, _i64, _p, _p, C.c_int]),
     "hymor_chu_sandwich_dev": (C.c_int, [_p, _p, C.c_int, _i64, C.c_double, _i64, _p, _p,
                                          C.c_int]),
@@ -46,6 +50,12 @@ SIGNATURES = {
     "hymor_memcpy_h2d": (C.c_int, [_p, _p, _i64]),
     "hymor_memcpy_d2h": (C.c_int, [_p, _p, _i64]),
     "hymor_bench_term_kernel": (C.c_int, [_p, C.c_int, C.c_int, C.c_int, _pd, _pd]),
+    "hymor_bench_copy": (C.c_int, [_i64, C.c_int, _pd]),
+    "hymor_weights_create_csc": (C.c_int, [_i64, C.c_int, _p, _p, _p, C.c_int, C.c_int, _i64,
+                                           _i64, _pp]),
+    "hymor_weights_destroy": (C.c_int, [_p]),
+    "hymor_energy_track": (C.c_int, [_p, _p, C.c_int, _i64, C.c_double, C.c_int, _p, C.c_int, _p,
+                                     _p, _i64, _i64, _p]),
 }
 
 _lib = None

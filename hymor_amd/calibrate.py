@@ -1,0 +1,27 @@
+"""Streaming-copy calibration: achievable HBM bandwidth on this GPU, and the
+reference kernel for scaling the FETCH_SIZE / WRITE_SIZE profiler counters.
+
+``python -m hymor_amd.calibrate [MiB] [reps]`` prints one JSON line: a copy
+of S bytes reads S and writes S, so GB/s = 2*S / time.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import sys
+
+from . import _lib
+
+
+def copy_bandwidth(mib: int = 1024, reps: int = 20):
+    nbytes = mib * 1024 * 1024
+    ms = C.c_double()
+    _lib.check(_lib.load().hymor_bench_copy(nbytes, reps, C.byref(ms)))
+    return {"kernel": "k_copy", "bytes_each_way": nbytes, "ms_per_launch": ms.value,
+            "GBps": 2.0 * nbytes / (ms.value * 1e-3) / 1e9, "reps": reps}
+
+
+if __name__ == "__main__":
+    mib = int(sys.argv[1]) if len(sys.argv) > 1 else 1024
+    reps = int(sys.argv[2]) if len(sys.argv) > 2 else 20
+    print(json.dumps(copy_bandwidth(mib, reps)), flush=True)

@@ -120,6 +120,20 @@ class DeviceOperator:
                                             float(dt), _FORMS[form], ptr(x), ptr(y), dty))
         return y
 
+    def expm_taylor_batched(self, X, *, order=5, nsub=1, dt=1.0, form="powers", transpose=False):
+        """T_order(A*dt)^nsub applied to every column of X (n x nrhs) in one
+        SpMM sweep per Taylor term."""
+        X = np.asarray(X)
+        cplx = self.is_complex or np.iscomplexobj(X)
+        X = np.asfortranarray(X, dtype=np.complex128 if cplx else np.float64)
+        if X.ndim != 2 or X.shape[0] != self.n:
+            raise ValueError("X must be n x nrhs")
+        Y = np.empty_like(X, order="F")
+        check(_lib.load().hymor_expm_taylor_batched(
+            self._h, int(bool(transpose)), int(order), int(nsub), float(dt), _FORMS[form],
+            X.shape[1], ptr(X), self.n, ptr(Y), self.n, C128 if cplx else F64))
+        return Y
+
     def chu_sandwich(self, weight: "DeviceOperator", x, *, order=5, nsub=1, dt=1.0, n_set0=0):
         """T(A'dt)^n * C * T(A dt)^n * x with CONSTRUCT_C's zeroing."""
         x, dty = _vec(x, self.is_complex)
@@ -186,8 +200,131 @@ class DeviceVector:
             pass
 
 
+class DeviceMatrix:
+    """A column-major n x nrhs block of var-major vectors in device memory."""
+
+    def __init__(self, n: int, nrhs: int, dtype=np.float64):
+        self.n, self.nrhs = n, nrhs
+        self.dtype = np.dtype(dtype)
+        self.nbytes = n * nrhs * self.dtype.itemsize
+        self._p = C.c_void_p()
+        check(_lib.load().hymor_dev_alloc(self.nbytes, C.byref(self._p)))
+
+    @classmethod
+    def from_host(cls, X):
+        X = np.asfortranarray(X)
+        d = cls(X.shape[0], X.shape[1], X.dtype)
+        check(_lib.load().hymor_memcpy_h2d(d._p, ptr(X), d.nbytes))
+        return d
+
+    def download(self):
+        out = np.empty((self.n, self.nrhs), dtype=self.dtype, order="F")
+        check(_lib.load().hymor_memcpy_d2h(ptr(out), self._p, self.nbytes))
+        return out
+
+    @property
+    def ptr(self):
+        return self._p
+
+    @property
+    def abi_dtype(self):
+        return C128 if self.dtype == np.complex128 else F64
+
+    def free(self):
+        if self._p is not None and self._p.value:
+            check(_lib.load().hymor_dev_free(self._p))
+        self._p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def expm_taylor_batched_dev(op: DeviceOperator, X: DeviceMatrix, Y: DeviceMatrix, *, order=5,
+                            nsub=1, dt=1.0, form="powers", transpose=False):
+    """Asynchronous device-resident batched action; op.synchronize() waits."""
+    check(_lib.load().hymor_expm_taylor_batched_dev(
+        op.handle, int(bool(transpose)), int(order), int(nsub), float(dt), _FORMS[form], X.nrhs,
+        X.ptr, X.n, Y.ptr, Y.n, X.abi_dtype))
+
+
 def expm_taylor_dev(op: DeviceOperator, x: DeviceVector, y: DeviceVector, *, order=5, nsub=1,
                     dt=1.0, form="powers", transpose=False):
     """Asynchronous device-resident action; call op.synchronize() to wait."""
     check(_lib.load().hymor_expm_taylor_dev(op.handle, int(bool(transpose)), int(order), int(nsub),
                                             float(dt), _FORMS[form], x.ptr, y.ptr, x.abi_dtype))
+
+
+class DeviceWeights:
+    """Block-diagonal Chu weights C_w = R'M_wR resident on the GPU (1..8 of
+    them), for energy tracking.  Built from CSC matrices in the reference's
+    layout (scipy sparse, var-major)."""
+
+    def __init__(self, mats, *, nchi, neta, vars_per_cell=4):
+        lib = _lib.load()
+        self._keep = []
+        cps, rvs, nzs = [], [], []
+        n = mats[0].shape[0]
+        for M in mats:
+            M = M.tocsc()
+            M.sort_indices()
+            if M.shape != (n, n):
+                raise ValueError("all weights must have the same shape")
+            a = (np.ascontiguousarray(M.indptr, dtype=np.int64),
+                 np.ascontiguousarray(M.indices, dtype=np.int64),
+                 np.ascontiguousarray(M.data, dtype=np.float64))
+            self._keep.append(a)
+            cps.append(a[0].ctypes.data)
+            rvs.append(a[1].ctypes.data)
+            nzs.append(a[2].ctypes.data)
+        k = len(mats)
+        self.n, self.nmat = n, k
+        self._h = C.c_void_p()
+        check(lib.hymor_weights_create_csc(n, k, (C.c_void_p * k)(*cps), (C.c_void_p * k)(*rvs),
+                                           (C.c_void_p * k)(*nzs), 0, vars_per_cell, nchi, neta,
+                                           C.byref(self._h)))
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            check(_lib.load().hymor_weights_destroy(self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+
+def energy_track(op: DeviceOperator, weights: DeviceWeights, x, *, n_t, order=5, dt=1.0,
+                 form="powers", n_skip=0, n_samples=0):
+    """n_t Taylor steps from x with the weights' energies after every step.
+
+    Returns (E, x_final, samples): E has shape (nmat, n_t+1), E[:, 0] the
+    energies of x; samples (n, count) holds the states after every n_skip-th
+    step (at most n_samples of them), or None.
+    """
+    x, dty = _vec(x, op.is_complex)
+    E = np.zeros((weights.nmat, n_t + 1))
+    xf = np.empty_like(x)
+    samp = None
+    sp_ptr = None
+    if n_samples > 0:
+        samp = np.zeros((n_samples, x.shape[0]), dtype=x.dtype)
+        sp_ptr = ptr(samp)
+    check(_lib.load().hymor_energy_track(op.handle, weights.handle, int(order), int(n_t),
+                                         float(dt), _FORMS[form], ptr(x), dty, ptr(E), ptr(xf),
+                                         int(max(n_skip, 1)), int(n_samples), sp_ptr))
+    return E, xf, (samp.T if samp is not None else None)

@@ -93,7 +93,7 @@ struct hymor_weights {
 
 namespace {
 
-constexpr int kMaxWeights = 8;
+constexpr int kMaxWeights = 8;   // matches k_quad_blockdiag's NMAX
 
 void *dalloc_raw(std::vector<void *> &allocs, size_t bytes) {
     void *p = nullptr;
@@ -216,16 +216,46 @@ void launch_term_t(const DevSliced &s, hipStream_t st, const double *x, const do
                        vout, c1, c2);
 }
 
-template <int NV>
-void launch_term_nv(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
-                    double *tout, const double *vin, double *vout, double c1, double c2) {
+template <int B, typename VT, int R>
+void launch_spmm_t(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+                   double *tout, const double *vin, double *vout, double c1, double c2) {
+    const int wpb = 4;
+    const int64_t grid = (s.ncell + wpb - 1) / wpb;
+    hipLaunchKernelGGL((k_spmm<B, VT, R>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
+                       s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, yx, tout, vin, vout, c1,
+                       c2);
+}
+
+// Kernel choice per (block size, row width): lane-per-row k_term up to 4
+// doubles per row (8 for b = 4), the rhs-parallel k_spmm above.
+template <int B, typename VT>
+void launch_block_part(const DevSliced &s, int nv, hipStream_t st, const double *x,
+                       const double *yx, double *tout, const double *vin, double *vout, double c1,
+                       double c2) {
+    switch (nv) {
+    case 1: launch_term_t<B, VT, 1>(s, st, x, yx, tout, vin, vout, c1, c2); break;
+    case 2: launch_term_t<B, VT, 2>(s, st, x, yx, tout, vin, vout, c1, c2); break;
+    case 4: launch_term_t<B, VT, 4>(s, st, x, yx, tout, vin, vout, c1, c2); break;
+    case 8:
+        if constexpr (B == 8) launch_spmm_t<B, VT, 8>(s, st, x, yx, tout, vin, vout, c1, c2);
+        else launch_term_t<B, VT, 8>(s, st, x, yx, tout, vin, vout, c1, c2);
+        break;
+    case 16: launch_spmm_t<B, VT, 16>(s, st, x, yx, tout, vin, vout, c1, c2); break;
+    case 32: launch_spmm_t<B, VT, 32>(s, st, x, yx, tout, vin, vout, c1, c2); break;
+    case 64: launch_spmm_t<B, VT, 64>(s, st, x, yx, tout, vin, vout, c1, c2); break;
+    default: throw ArgError("unsupported vector width");
+    }
+}
+
+void launch_block(const DevSliced &s, int nv, hipStream_t st, const double *x, const double *yx,
+                  double *tout, const double *vin, double *vout, double c1, double c2) {
     const bool f32 = s.prec == HYMOR_PREC_F32;
     if (s.b == 4) {
-        if (f32) launch_term_t<4, float, NV>(s, st, x, yx, tout, vin, vout, c1, c2);
-        else launch_term_t<4, double, NV>(s, st, x, yx, tout, vin, vout, c1, c2);
+        if (f32) launch_block_part<4, float>(s, nv, st, x, yx, tout, vin, vout, c1, c2);
+        else launch_block_part<4, double>(s, nv, st, x, yx, tout, vin, vout, c1, c2);
     } else if (s.b == 8) {
-        if (f32) launch_term_t<8, float, NV>(s, st, x, yx, tout, vin, vout, c1, c2);
-        else launch_term_t<8, double, NV>(s, st, x, yx, tout, vin, vout, c1, c2);
+        if (f32) launch_block_part<8, float>(s, nv, st, x, yx, tout, vin, vout, c1, c2);
+        else launch_block_part<8, double>(s, nv, st, x, yx, tout, vin, vout, c1, c2);
     } else {
         throw ArgError("vars_per_cell must be 4 or 8");
     }
@@ -247,7 +277,23 @@ void launch_extra_nv(const DevExtra &xe, int64_t nflow, hipStream_t st, const do
                            vout, c1, c2);
 }
 
-// One Taylor term (or a plain SpMV when vout == nullptr) of `op` on `st`.
+void launch_extra(const DevExtra &xe, int nv, int64_t nflow, hipStream_t st, const double *x,
+                  double *yx, double *tout, const double *vin, double *vout, double c1,
+                  double c2) {
+    switch (nv) {
+    case 1: launch_extra_nv<1>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2); break;
+    case 2: launch_extra_nv<2>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2); break;
+    case 4: launch_extra_nv<4>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2); break;
+    case 8: launch_extra_nv<8>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2); break;
+    case 16: launch_extra_nv<16>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2); break;
+    case 32: launch_extra_nv<32>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2); break;
+    case 64: launch_extra_nv<64>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2); break;
+    default: throw ArgError("unsupported vector width");
+    }
+}
+
+// One Taylor term (or a plain SpMV when vout == nullptr) of `op` on `st`;
+// nv doubles per row.
 void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, const double *x, double *tout,
                  const double *vin, double *vout, double c1, double c2) {
     const DevSliced &s = adj ? op->adj : op->fwd;
@@ -255,36 +301,39 @@ void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, const double *x
     double *yx = adj ? op->yxa : op->yx;
     const bool has_x = xe.nrows > 0;
     if (has_x) {
-        if (nv == 1) launch_extra_nv<1>(xe, op->nflow, st, x, yx, tout, vin, vout, c1, c2);
-        else if (nv == 2) launch_extra_nv<2>(xe, op->nflow, st, x, yx, tout, vin, vout, c1, c2);
-        else throw ArgError("unsupported vector width");
+        launch_extra(xe, nv, op->nflow, st, x, yx, tout, vin, vout, c1, c2);
         HIP_TRY(hipGetLastError());
     }
-    const double *yxp = has_x ? yx : nullptr;
     if (op->ncell > 0) {
-        if (nv == 1) launch_term_nv<1>(s, st, x, yxp, tout, vin, vout, c1, c2);
-        else if (nv == 2) launch_term_nv<2>(s, st, x, yxp, tout, vin, vout, c1, c2);
-        else throw ArgError("unsupported vector width");
+        launch_block(s, nv, st, x, has_x ? yx : nullptr, tout, vin, vout, c1, c2);
         HIP_TRY(hipGetLastError());
     }
 }
 
+// width: doubles per internal row (>= nrhs*ncomp; the rest is zero padding)
 void launch_perm_in(hymor_op *op, hipStream_t st, const double *x_vm, int64_t ld, double *out,
-                    int nrhs, int ncomp, int64_t nzero) {
-    const int64_t total = op->n * nrhs * ncomp;
+                    int nrhs, int ncomp, int64_t nzero, int width) {
+    const int64_t total = op->n * width;
     const int64_t grid = std::min<int64_t>((total + 255) / 256, 65536);
     hipLaunchKernelGGL(k_perm_in, dim3((unsigned)std::max<int64_t>(grid, 1)), dim3(256), 0, st,
-                       x_vm, ld, out, op->n, op->ncell, op->b, nrhs, ncomp, nzero);
+                       x_vm, ld, out, op->n, op->ncell, op->b, nrhs, ncomp, nzero, width);
     HIP_TRY(hipGetLastError());
 }
 
 void launch_perm_out(hymor_op *op, hipStream_t st, const double *in, double *y_vm, int64_t ld,
-                     int nrhs, int ncomp, int64_t nzero) {
+                     int nrhs, int ncomp, int64_t nzero, int width) {
     const int64_t total = op->n * nrhs * ncomp;
     const int64_t grid = std::min<int64_t>((total + 255) / 256, 65536);
     hipLaunchKernelGGL(k_perm_out, dim3((unsigned)std::max<int64_t>(grid, 1)), dim3(256), 0, st,
-                       in, y_vm, ld, op->n, op->ncell, op->b, nrhs, ncomp, nzero);
+                       in, y_vm, ld, op->n, op->ncell, op->b, nrhs, ncomp, nzero, width);
     HIP_TRY(hipGetLastError());
+}
+
+// Supported internal row widths (doubles per row): powers of two up to 64.
+int pad_width(int64_t valid) {
+    for (int w = 1; w <= 64; w *= 2)
+        if (valid <= w) return w;
+    throw ArgError("nrhs too large: at most 64 doubles per row (64 real or 32 complex rhs)");
 }
 
 // T_order(A*dt)^nsub applied to `in` (cell-major, internal buffer).  Returns
@@ -522,11 +571,59 @@ int hymor_spmv(hymor_op *op, int transpose, const void *x, void *y, int dtype) {
         ensure_ws(op, nv);
         const size_t bytes = (size_t)op->n * nv * sizeof(double);
         HIP_TRY(hipMemcpyAsync(op->ws[0], x, bytes, hipMemcpyHostToDevice, op->stream));
-        launch_perm_in(op, op->stream, op->ws[0], op->n, op->ws[1], 1, nv, 0);
+        launch_perm_in(op, op->stream, op->ws[0], op->n, op->ws[1], 1, nv, 0, nv);
         launch_term(op, transpose != 0, nv, op->stream, op->ws[1], op->ws[2], nullptr,
                     nullptr, 1.0, 0.0);
-        launch_perm_out(op, op->stream, op->ws[2], op->ws[0], op->n, 1, nv, 0);
+        launch_perm_out(op, op->stream, op->ws[2], op->ws[0], op->n, 1, nv, 0, nv);
         HIP_TRY(hipMemcpyAsync(y, op->ws[0], bytes, hipMemcpyDeviceToHost, op->stream));
+        HIP_TRY(hipStreamSynchronize(op->stream));
+    });
+}
+
+static void check_batch(hymor_op *op, int64_t nrhs, int64_t ldx, int64_t ldy) {
+    ARG_CHECK(nrhs >= 1, "nrhs must be >= 1");
+    ARG_CHECK(ldx >= op->n && ldy >= op->n, "leading dimension smaller than n");
+}
+
+int hymor_expm_taylor_batched_dev(hymor_op *op, int transpose, int order, int64_t nsub, double dt,
+                                  int form, int64_t nrhs, const void *d_X, int64_t ldx, void *d_Y,
+                                  int64_t ldy, int dtype) {
+    return guarded([&] {
+        check_taylor_args(op, transpose, order, nsub, form);
+        ARG_CHECK(d_X && d_Y, "null matrix");
+        check_batch(op, nrhs, ldx, ldy);
+        const int ncomp = nv_of(op, dtype);
+        const int w = pad_width(nrhs * ncomp);
+        ensure_ws(op, w);
+        launch_perm_in(op, op->stream, (const double *)d_X, ldx, op->ws[1], (int)nrhs, ncomp, 0, w);
+        const double *res = run_taylor(op, transpose != 0, order, nsub, dt, form, w, op->ws[1]);
+        launch_perm_out(op, op->stream, res, (double *)d_Y, ldy, (int)nrhs, ncomp, 0, w);
+    });
+}
+
+int hymor_expm_taylor_batched(hymor_op *op, int transpose, int order, int64_t nsub, double dt,
+                              int form, int64_t nrhs, const void *X, int64_t ldx, void *Y,
+                              int64_t ldy, int dtype) {
+    return guarded([&] {
+        check_taylor_args(op, transpose, order, nsub, form);
+        ARG_CHECK(X && Y, "null matrix");
+        check_batch(op, nrhs, ldx, ldy);
+        const int ncomp = nv_of(op, dtype);
+        const int w = pad_width(nrhs * ncomp);
+        ensure_ws(op, w);
+        // ws[0] holds n*w doubles >= n*nrhs*ncomp: stage the columns packed (ld = n)
+        const size_t esz = (size_t)ncomp * sizeof(double);
+        for (int64_t j = 0; j < nrhs; ++j)
+            HIP_TRY(hipMemcpyAsync((char *)op->ws[0] + (size_t)j * op->n * esz,
+                                   (const char *)X + (size_t)j * ldx * esz, (size_t)op->n * esz,
+                                   hipMemcpyHostToDevice, op->stream));
+        launch_perm_in(op, op->stream, op->ws[0], op->n, op->ws[1], (int)nrhs, ncomp, 0, w);
+        const double *res = run_taylor(op, transpose != 0, order, nsub, dt, form, w, op->ws[1]);
+        launch_perm_out(op, op->stream, res, op->ws[0], op->n, (int)nrhs, ncomp, 0, w);
+        for (int64_t j = 0; j < nrhs; ++j)
+            HIP_TRY(hipMemcpyAsync((char *)Y + (size_t)j * ldy * esz,
+                                   (const char *)op->ws[0] + (size_t)j * op->n * esz,
+                                   (size_t)op->n * esz, hipMemcpyDeviceToHost, op->stream));
         HIP_TRY(hipStreamSynchronize(op->stream));
     });
 }
@@ -538,9 +635,9 @@ int hymor_expm_taylor_dev(hymor_op *op, int transpose, int order, int64_t nsub, 
         ARG_CHECK(d_x && d_y, "null vector");
         const int nv = nv_of(op, dtype);
         ensure_ws(op, nv);
-        launch_perm_in(op, op->stream, (const double *)d_x, op->n, op->ws[1], 1, nv, 0);
+        launch_perm_in(op, op->stream, (const double *)d_x, op->n, op->ws[1], 1, nv, 0, nv);
         const double *res = run_taylor(op, transpose != 0, order, nsub, dt, form, nv, op->ws[1]);
-        launch_perm_out(op, op->stream, res, (double *)d_y, op->n, 1, nv, 0);
+        launch_perm_out(op, op->stream, res, (double *)d_y, op->n, 1, nv, 0, nv);
     });
 }
 
@@ -553,9 +650,9 @@ int hymor_expm_taylor(hymor_op *op, int transpose, int order, int64_t nsub, doub
         ensure_ws(op, nv);
         const size_t bytes = (size_t)op->n * nv * sizeof(double);
         HIP_TRY(hipMemcpyAsync(op->ws[0], x, bytes, hipMemcpyHostToDevice, op->stream));
-        launch_perm_in(op, op->stream, op->ws[0], op->n, op->ws[1], 1, nv, 0);
+        launch_perm_in(op, op->stream, op->ws[0], op->n, op->ws[1], 1, nv, 0, nv);
         const double *res = run_taylor(op, transpose != 0, order, nsub, dt, form, nv, op->ws[1]);
-        launch_perm_out(op, op->stream, res, op->ws[0], op->n, 1, nv, 0);
+        launch_perm_out(op, op->stream, res, op->ws[0], op->n, 1, nv, 0, nv);
         HIP_TRY(hipMemcpyAsync(y, op->ws[0], bytes, hipMemcpyDeviceToHost, op->stream));
         HIP_TRY(hipStreamSynchronize(op->stream));
     });
@@ -563,12 +660,12 @@ int hymor_expm_taylor(hymor_op *op, int transpose, int order, int64_t nsub, doub
 
 static void sandwich_core(hymor_op *A, hymor_op *C, int order, int64_t nsub, double dt,
                           int64_t n_set0, const double *x_vm, double *y_vm, int nv) {
-    launch_perm_in(A, A->stream, x_vm, A->n, A->ws[1], 1, nv, n_set0);
+    launch_perm_in(A, A->stream, x_vm, A->n, A->ws[1], 1, nv, n_set0, nv);
     const double *r1 = run_taylor(A, false, order, nsub, dt, HYMOR_FORM_NORMALISED, nv, A->ws[1]);
     // weight: ws[6] = C * r1 (C's extras use C's own partial-sum buffer)
     launch_term(C, false, nv, A->stream, r1, A->ws[6], nullptr, nullptr, 1.0, 0.0);
     const double *r2 = run_taylor(A, true, order, nsub, dt, HYMOR_FORM_NORMALISED, nv, A->ws[6]);
-    launch_perm_out(A, A->stream, r2, y_vm, A->n, 1, nv, n_set0);
+    launch_perm_out(A, A->stream, r2, y_vm, A->n, 1, nv, n_set0, nv);
 }
 
 static void check_sandwich(hymor_op *A, hymor_op *C, int order, int64_t nsub, int64_t n_set0) {
@@ -649,8 +746,8 @@ int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int reps, doubl
                             double *bytes_per_launch) {
     return guarded([&] {
         ARG_CHECK(op && ms_per_launch && bytes_per_launch, "null argument");
-        ARG_CHECK(nv == 1 || nv == 2, "nv must be 1 or 2");
-        ARG_CHECK(!(op->is_complex && nv == 1), "complex operator needs nv = 2");
+        ARG_CHECK(nv >= 1 && nv <= 64 && (nv & (nv - 1)) == 0, "nv must be a power of two <= 64");
+        ARG_CHECK(!(op->is_complex && nv == 1), "complex operator needs nv >= 2");
         ARG_CHECK(reps >= 1, "reps must be >= 1");
         ARG_CHECK(!transpose || op->has_adj, "operator was built without its adjoint");
         ensure_ws(op, nv);
@@ -681,6 +778,200 @@ int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int reps, doubl
         const DevExtra &xe = transpose ? op->xa : op->xf;
         bytes += (double)xe.nnz * ((xe.is_complex ? 16.0 : 8.0) + 4.0);
         *bytes_per_launch = bytes;
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ energy tracking
+
+namespace {
+
+void launch_quad(const hymor_weights *W, hipStream_t st, const double *v, int nv, double *out,
+                 int64_t ostride) {
+    const int64_t items = W->ncell + W->n_extra;
+    const int64_t nblk = (items + 255) / 256;
+    if (nblk != W->nblocks) throw ArgError("weights partial buffer has the wrong size");
+#define HYMOR_QUAD(BV, NVV)                                                                  \
+    hipLaunchKernelGGL((k_quad_blockdiag<BV, NVV, kMaxWeights>), dim3((unsigned)nblk),       \
+                       dim3(256), 0, st, v, W->blocks, W->xdiag, W->nmat, W->ncell,          \
+                       W->n_extra, W->partial)
+    if (W->b == 4 && nv == 1) HYMOR_QUAD(4, 1);
+    else if (W->b == 4 && nv == 2) HYMOR_QUAD(4, 2);
+    else if (W->b == 8 && nv == 1) HYMOR_QUAD(8, 1);
+    else if (W->b == 8 && nv == 2) HYMOR_QUAD(8, 2);
+    else throw ArgError("unsupported weights block size / vector width");
+#undef HYMOR_QUAD
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_partials, dim3(W->nmat), dim3(256), 0, st, W->partial, nblk, out,
+                       ostride);
+    HIP_TRY(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" {
+
+int hymor_bench_copy(int64_t bytes, int reps, double *ms_per_launch) {
+    return guarded([&] {
+        ARG_CHECK(ms_per_launch && reps >= 1 && bytes >= 16 && bytes % 16 == 0,
+                  "bytes must be a positive multiple of 16, reps >= 1");
+        std::vector<void *> allocs;
+        struct Free {
+            std::vector<void *> &a;
+            ~Free() {
+                for (void *p : a) (void)hipFree(p);
+            }
+        } guard{allocs};
+        d2 *in = (d2 *)dalloc_raw(allocs, (size_t)bytes);
+        d2 *out = (d2 *)dalloc_raw(allocs, (size_t)bytes);
+        HIP_TRY(hipMemset(in, 0, (size_t)bytes));
+        const int64_t n2 = bytes / 16;
+        const int64_t grid = std::min<int64_t>((n2 + 255) / 256, 8192);
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        hipLaunchKernelGGL(k_copy, dim3((unsigned)grid), dim3(256), 0, nullptr, in, out, n2);
+        HIP_TRY(hipEventRecord(e0, nullptr));
+        for (int i = 0; i < reps; ++i)
+            hipLaunchKernelGGL(k_copy, dim3((unsigned)grid), dim3(256), 0, nullptr, in, out, n2);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(e1, nullptr));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms_per_launch = (double)ms / reps;
+    });
+}
+
+int hymor_weights_create_csc(int64_t n, int nmat, const int64_t *const *colptr,
+                             const int64_t *const *rowval, const double *const *nzval,
+                             int index_base, int vars_per_cell, int64_t nchi, int64_t neta,
+                             hymor_weights **out) {
+    hymor_weights *W = nullptr;
+    int rc = guarded([&] {
+        ARG_CHECK(out != nullptr, "out is null");
+        *out = nullptr;
+        ARG_CHECK(nmat >= 1 && nmat <= kMaxWeights, "nmat must be in [1, 8]");
+        ARG_CHECK(colptr && rowval && nzval, "null CSC array list");
+        ARG_CHECK(index_base == 0 || index_base == 1, "index_base must be 0 or 1");
+        ARG_CHECK(vars_per_cell == 4 || vars_per_cell == 8, "vars_per_cell must be 4 or 8");
+        ARG_CHECK(nchi > 0 && neta > 0, "grid must be non-empty");
+        const int b = vars_per_cell;
+        const int64_t ncell = nchi * neta, nflow = (int64_t)b * ncell;
+        ARG_CHECK(n >= nflow, "n smaller than vars_per_cell*nchi*neta");
+        const int64_t n_extra = n - nflow;
+        std::vector<double> blocks((size_t)nmat * ncell * b * b, 0.0);
+        std::vector<double> xdiag((size_t)nmat * std::max<int64_t>(n_extra, 1), 0.0);
+        for (int m = 0; m < nmat; ++m) {
+            ARG_CHECK(colptr[m] && rowval[m] && nzval[m], "null CSC array");
+            ARG_CHECK(colptr[m][0] == index_base, "colptr[0] must equal the index base");
+            for (int64_t j = 0; j < n; ++j) {
+                const int64_t cmc = j < nflow ? (j % ncell) * b + j / ncell : j;
+                for (int64_t e = colptr[m][j] - index_base; e < colptr[m][j + 1] - index_base; ++e) {
+                    const int64_t i = rowval[m][e] - index_base;
+                    ARG_CHECK(i >= 0 && i < n, "rowval out of range");
+                    const int64_t cmr = i < nflow ? (i % ncell) * b + i / ncell : i;
+                    const double v = nzval[m][e];
+                    if (cmr < nflow && cmc < nflow) {
+                        ARG_CHECK(cmr / b == cmc / b,
+                                  "weight has an entry outside the per-cell block diagonal");
+                        blocks[(((size_t)m * ncell + cmr / b) * b + cmr % b) * b + cmc % b] += v;
+                    } else {
+                        ARG_CHECK(cmr == cmc || v == 0.0,
+                                  "weight has an off-diagonal entry on the extra DOFs");
+                        if (cmr == cmc) xdiag[(size_t)m * n_extra + (cmr - nflow)] += v;
+                    }
+                }
+            }
+        }
+        W = new hymor_weights();
+        W->b = b;
+        W->nmat = nmat;
+        W->nchi = nchi;
+        W->neta = neta;
+        W->ncell = ncell;
+        W->n = n;
+        W->n_extra = n_extra;
+        W->nblocks = (ncell + n_extra + 255) / 256;
+        W->blocks = (double *)dalloc_raw(W->allocs, blocks.size() * sizeof(double));
+        HIP_TRY(hipMemcpy(W->blocks, blocks.data(), blocks.size() * sizeof(double),
+                          hipMemcpyHostToDevice));
+        W->xdiag = (double *)dalloc_raw(W->allocs, xdiag.size() * sizeof(double));
+        HIP_TRY(hipMemcpy(W->xdiag, xdiag.data(), xdiag.size() * sizeof(double),
+                          hipMemcpyHostToDevice));
+        W->partial = (double *)dalloc_raw(W->allocs, (size_t)nmat * W->nblocks * sizeof(double));
+        *out = W;
+    });
+    if (rc != HYMOR_OK && W) {
+        for (void *p : W->allocs) (void)hipFree(p);
+        delete W;
+    }
+    return rc;
+}
+
+int hymor_weights_destroy(hymor_weights *W) {
+    return guarded([&] {
+        if (!W) return;
+        HIP_TRY(hipDeviceSynchronize());
+        for (void *p : W->allocs) (void)hipFree(p);
+        delete W;
+    });
+}
+
+int hymor_energy_track(hymor_op *A, const hymor_weights *W, int order, int64_t n_t, double dt,
+                       int form, const void *x, int dtype, double *E, void *x_final,
+                       int64_t n_skip, int64_t n_samples, void *samples) {
+    return guarded([&] {
+        check_taylor_args(A, 0, order, n_t, form);
+        ARG_CHECK(W != nullptr && x && E, "null argument");
+        ARG_CHECK(W->n == A->n && W->b == A->b && W->ncell == A->ncell,
+                  "weights shape differs from the operator");
+        ARG_CHECK(n_samples == 0 || (samples != nullptr && n_skip >= 1),
+                  "sampling needs a buffer and n_skip >= 1");
+        const int nv = nv_of(A, dtype);
+        ensure_ws(A, nv);
+        const int64_t ncol = n_t + 1;
+        // per-call series buffer, released when this call returns
+        double *dE = (double *)dalloc_raw(A->allocs, (size_t)W->nmat * ncol * sizeof(double));
+        struct Free {
+            hymor_op *op;
+            double *p;
+            ~Free() {
+                for (size_t k = 0; k < op->allocs.size(); ++k)
+                    if (op->allocs[k] == p) {
+                        (void)hipFree(p);
+                        op->allocs.erase(op->allocs.begin() + (long)k);
+                        break;
+                    }
+            }
+        } guard{A, dE};
+        const size_t bytes = (size_t)A->n * nv * sizeof(double);
+        hipStream_t st = A->stream;
+        HIP_TRY(hipMemcpyAsync(A->ws[0], x, bytes, hipMemcpyHostToDevice, st));
+        launch_perm_in(A, st, A->ws[0], A->n, A->ws[1], 1, nv, 0, nv);
+        const double *cur = A->ws[1];
+        launch_quad(W, st, cur, nv, dE, ncol);
+        int64_t count = 0;
+        for (int64_t i = 1; i <= n_t; ++i) {
+            cur = run_taylor(A, false, order, 1, dt, form, nv, cur);
+            launch_quad(W, st, cur, nv, dE + i, ncol);
+            if (n_samples > 0 && i % n_skip == 0 && count < n_samples) {
+                launch_perm_out(A, st, cur, A->ws[0], A->n, 1, nv, 0, nv);
+                HIP_TRY(hipMemcpyAsync((char *)samples + (size_t)count * bytes, A->ws[0], bytes,
+                                       hipMemcpyDeviceToHost, st));
+                ++count;
+            }
+        }
+        if (x_final) {
+            launch_perm_out(A, st, cur, A->ws[0], A->n, 1, nv, 0, nv);
+            HIP_TRY(hipMemcpyAsync(x_final, A->ws[0], bytes, hipMemcpyDeviceToHost, st));
+        }
+        HIP_TRY(hipMemcpyAsync(E, dE, (size_t)W->nmat * ncol * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
     });
 }
 

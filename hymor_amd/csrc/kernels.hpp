@@ -168,15 +168,22 @@ __global__ __launch_bounds__(256) void k_extra(
 // elements with leading dimension ld (elements); an element is ncomp
 // doubles.  Internal rows hold nrhs*ncomp doubles.  Var-major indices below
 // nzero read as 0 (CONSTRUCT_C's x[1:N_set_0] .= 0).
+// Internal rows are `width` doubles wide; components past nrhs*ncomp are
+// padding and read as 0.
 __global__ void k_perm_in(const double *__restrict__ x_vm, int64_t ld, double *__restrict__ out,
-                          int64_t n, int64_t ncell, int b, int nrhs, int ncomp, int64_t nzero) {
-    const int64_t per = (int64_t)nrhs * ncomp;
-    const int64_t total = n * per;
+                          int64_t n, int64_t ncell, int b, int nrhs, int ncomp, int64_t nzero,
+                          int width) {
+    const int64_t valid = (int64_t)nrhs * ncomp;
+    const int64_t total = n * width;
     const int64_t nflow = ncell * b;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t row = i / per;
-        const int64_t rem = i - row * per;
+        const int64_t row = i / width;
+        const int64_t rem = i - row * width;
+        if (rem >= valid) {
+            out[i] = 0.0;
+            continue;
+        }
         const int64_t rhs = rem / ncomp, comp = rem - rhs * ncomp;
         const int64_t vm = row < nflow ? (row % b) * ncell + row / b : row;
         out[i] = vm < nzero ? 0.0 : x_vm[(rhs * ld + vm) * ncomp + comp];
@@ -184,9 +191,9 @@ __global__ void k_perm_in(const double *__restrict__ x_vm, int64_t ld, double *_
 }
 
 __global__ void k_perm_out(const double *__restrict__ in, double *__restrict__ y_vm, int64_t ld,
-                           int64_t n, int64_t ncell, int b, int nrhs, int ncomp, int64_t nzero) {
-    const int64_t per = (int64_t)nrhs * ncomp;
-    const int64_t total = n * per;
+                           int64_t n, int64_t ncell, int b, int nrhs, int ncomp, int64_t nzero,
+                           int width) {
+    const int64_t total = n * nrhs * ncomp;
     const int64_t nflow = ncell * b;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -194,7 +201,8 @@ __global__ void k_perm_out(const double *__restrict__ in, double *__restrict__ y
         const int64_t rem = i - rhs * n * ncomp;
         const int64_t vm = rem / ncomp, comp = rem - vm * ncomp;
         const int64_t row = vm < nflow ? (vm % ncell) * b + vm / ncell : vm;
-        y_vm[(rhs * ld + vm) * ncomp + comp] = vm < nzero ? 0.0 : in[(row * nrhs + rhs) * ncomp + comp];
+        y_vm[(rhs * ld + vm) * ncomp + comp] =
+            vm < nzero ? 0.0 : in[row * width + rhs * ncomp + comp];
     }
 }
 
@@ -273,6 +281,79 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double *__restric
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) out[(int64_t)m * ostride] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+}  // namespace hymor
+
+namespace hymor {
+
+// Streaming copy, 16 bytes per lane per iteration: the reference point for
+// achievable HBM bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE
+// counters (a copy of S bytes fetches S and writes S).
+__global__ __launch_bounds__(256) void k_copy(const d2 *__restrict__ in, d2 *__restrict__ out,
+                                              int64_t n2) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+
+}  // namespace hymor
+
+namespace hymor {
+
+// Batched Taylor term: R right-hand sides per row (R doubles, rhs fastest).
+// One wavefront per cell; lane = (row group g, rhs r) with r = lane % R and
+// g = lane / R, each lane owning RPL = B*R/64 consecutive block rows.  Per
+// block, a lane reads its RPL block rows (broadcast across the R rhs lanes)
+// and, for each k, one double of X (the R lanes of a group read R
+// consecutive doubles, one 128-byte line for R = 16); the block's column
+// index is wave-uniform.  Same epilogue as k_term.
+template <int B, typename VT, int R>
+__global__ __launch_bounds__(256) void k_spmm(
+    const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
+    const VT *__restrict__ val, int64_t ncell, const double *__restrict__ x,
+    const double *__restrict__ yx, double *__restrict__ tout, const double *vin, double *vout,
+    double c1, double c2) {
+    constexpr int CPS = 64 / B;
+    constexpr int BB = B * B;
+    constexpr int G = 64 / R;
+    static_assert(64 % R == 0 && B % G == 0, "unsupported B/R");
+    constexpr int RPL = B / G;
+    const int lane = threadIdx.x & 63;
+    const int64_t wpb = blockDim.x >> 6;
+    const int64_t cell = __builtin_amdgcn_readfirstlane(
+        (int)(xcd_tile(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6)));
+    if (cell >= ncell) return;
+    const int r = lane % R, g = lane / R;
+    const int64_t slice = cell / CPS, q = cell - slice * CPS;
+    const int64_t p0 = slice_ptr[slice];
+    const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
+    double acc[RPL];
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) acc[i] = 0.0;
+    for (int t = 0; t < width; ++t) {
+        const int64_t blk = p0 + (int64_t)t * CPS + q;
+        const int64_t c = col[blk];
+        const VT *ap = val + blk * BB + (g * RPL) * B;
+        double a[RPL * B];
+        load_vals<RPL * B>(ap, a);
+        const double *xp = x + c * (B * R) + r;
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            const double xk = xp[k * R];
+#pragma unroll
+            for (int i = 0; i < RPL; ++i) acc[i] = fma(a[i * B + k], xk, acc[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) {
+        const int64_t idx = (cell * B + g * RPL + i) * R + r;
+        double y = acc[i];
+        if (yx) y += yx[idx];
+        const double t = c1 * y;
+        if (tout) tout[idx] = t;
+        if (vout) vout[idx] = fma(c2, t, vin[idx]);
+    }
 }
 
 }  // namespace hymor

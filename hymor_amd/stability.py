@@ -23,7 +23,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from . import _lib
-from .core import DeviceOperator
+from .core import DeviceOperator, DeviceWeights, energy_track
 
 ORDER = 5            # Taylor order hard-coded by the reference
 B = 4                # conserved variables per cell in the reference
@@ -456,3 +456,52 @@ def FREESTREAM_RECEPTIVITY(A_, s, n_modes, T, w_infty, *, tol=1e-4, maxiter=None
         for d in (devA, devC):
             if d is not None:
                 d.close()
+
+
+# ====================================================================== energy tracking
+
+def LINEAR_INTEGRATION_AND_ENERGY_GPU_NO_SAMPLES(x, n_t, dt, t_0, orderTaylor, Adt: DeviceOperator,
+                                                 C_no_exp_all, C_no_exp_p, C_no_exp_k, C_no_exp_S,
+                                                 use_gpu=True, v_buf=None, vout_buf=None,
+                                                 term_buf=None, *, weights=None):
+    """Forward integration with the four Chu energies per step
+    (Freestream_receptivity/LINEAR_INTEGRATION_AND_GAINS.jl:436-477).
+
+    Returns (t, E_all, E_p, E_k, E_S); as in the reference, entry 0 of each
+    energy series is 0 (its caller fills it).  Adt is the device operator of
+    A*dt; the C matrices are the scipy weights (or pass ``weights``, a
+    DeviceWeights built from them, to reuse the upload).
+    """
+    t, E, _, _ = _energy_run(x, n_t, dt, t_0, orderTaylor, Adt,
+                             (C_no_exp_all, C_no_exp_p, C_no_exp_k, C_no_exp_S), weights, 0, 0)
+    return (t, *E)
+
+
+def LINEAR_INTEGRATION_AND_ENERGY_GPU(x, n_t, N_samples, N_skip, dt, t_0, orderTaylor,
+                                      Adt: DeviceOperator, C_no_exp_all, C_no_exp_p, C_no_exp_k,
+                                      C_no_exp_S, use_gpu=True, v_buf=None, vout_buf=None,
+                                      term_buf=None, *, weights=None):
+    """As _NO_SAMPLES plus snapshots Vf[:, count] of the state after every
+    N_skip-th step, N_samples columns (zeros where the run had fewer)
+    (LINEAR_INTEGRATION_AND_GAINS.jl:375-427)."""
+    t, E, Vf, _ = _energy_run(x, n_t, dt, t_0, orderTaylor, Adt,
+                              (C_no_exp_all, C_no_exp_p, C_no_exp_k, C_no_exp_S), weights,
+                              N_skip, N_samples)
+    return (t, *E, Vf)
+
+
+def _energy_run(x, n_t, dt, t_0, order, Adt, mats, weights, n_skip, n_samples):
+    own = weights is None
+    if own:
+        nc = Adt.ncell
+        # the grid only matters through ncell for block-diagonal weights
+        weights = DeviceWeights(list(mats), nchi=nc, neta=1, vars_per_cell=Adt.b)
+    try:
+        E, xf, samp = energy_track(Adt, weights, x, n_t=int(n_t), order=int(order), dt=1.0,
+                                   form="powers", n_skip=int(n_skip), n_samples=int(n_samples))
+    finally:
+        if own:
+            weights.close()
+    E[:, 0] = 0.0                     # the reference leaves entry 1 to its caller
+    t = t_0 + dt * np.arange(int(n_t) + 1)
+    return t, [E[k] for k in range(E.shape[0])], samp, xf

@@ -116,6 +116,21 @@ int hymor_expm_taylor(hymor_op *op, int transpose, int order, int64_t nsub, doub
 int hymor_expm_taylor_dev(hymor_op *op, int transpose, int order, int64_t nsub, double dt,
                           int form, const void *d_x, void *d_y, int dtype);
 
+/* Batched action on nrhs vectors at once: X, Y are column-major n x nrhs
+ * (leading dimensions ldx, ldy >= n), each column one vector of the
+ * unbatched call.  The operator's blocks are read once per Taylor term for
+ * all columns (an SpMM).  nrhs*(1 or 2 for complex) <= 64; internally the
+ * width is padded to a power of two.  Used for many Krylov vectors or
+ * streamwise wavenumbers at once (no reference counterpart: the reference
+ * applies one vector per ARPACK callback, ARNOLDI_POWER_GPU.jl:36-52). */
+int hymor_expm_taylor_batched(hymor_op *op, int transpose, int order, int64_t nsub, double dt,
+                              int form, int64_t nrhs, const void *X, int64_t ldx, void *Y,
+                              int64_t ldy, int dtype);
+
+int hymor_expm_taylor_batched_dev(hymor_op *op, int transpose, int order, int64_t nsub, double dt,
+                                  int form, int64_t nrhs, const void *d_X, int64_t ldx, void *d_Y,
+                                  int64_t ldy, int dtype);
+
 /* Chu-norm sandwich y = T_p(A'dt)^n * C * T_p(A dt)^n * x with the zeroing
  * of CONSTRUCT_C (CONSTRUCT_C.jl:33-58 around EXPONENTIAL! :78-111):
  * x[0:n_set0) is zeroed before and y[0:n_set0) after (var-major indices).
@@ -139,9 +154,48 @@ int hymor_memcpy_d2h(void *h_dst, const void *d_src, int64_t bytes);
  * handle's stream (a middle term: gathers x, writes the new term, updates
  * vout) between two HIP events; returns the mean duration per launch in ms
  * and the kernel's algorithmic bytes per launch.  nv = doubles per row
- * (1 real, 2 complex). */
+ * (1 real, 2 complex, up to 64 for batches; a power of two). */
 int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int reps,
                             double *ms_per_launch, double *bytes_per_launch);
+
+/* Calibration probe, no reference counterpart: a streaming copy of `bytes`
+ * (multiple of 16) between two fresh device buffers, launched `reps` times;
+ * returns ms per launch.  Used to measure achievable HBM bandwidth and to
+ * scale the FETCH_SIZE / WRITE_SIZE profiler counters. */
+int hymor_bench_copy(int64_t bytes, int reps, double *ms_per_launch);
+
+/* ------------------------------------------------------------------------
+ * Energy-tracked integration (Freestream_receptivity/
+ * LINEAR_INTEGRATION_AND_GAINS.jl:375-477, LINEAR_INTEGRATION_AND_ENERGY_GPU
+ * and _NO_SAMPLES).  The Chu weights C_w = R'M_wR are block-diagonal per
+ * cell; the library keeps them as b x b blocks (plus a diagonal on extra
+ * DOFs) and forms the energies on the device, so only scalars (and the
+ * optional snapshots) cross to the host.
+ * ------------------------------------------------------------------------ */
+typedef struct hymor_weights hymor_weights;
+
+/* nmat (1..8) weight matrices given as CSC arrays in the reference layout
+ * (C_no_exp_all, _p, _k, _S at LINEAR_INTEGRATION_AND_GAINS.jl:63-111).
+ * Fails if an entry lies off the per-cell block diagonal, or off the
+ * diagonal on extra DOFs. */
+int hymor_weights_create_csc(int64_t n, int nmat, const int64_t *const *colptr,
+                             const int64_t *const *rowval, const double *const *nzval,
+                             int index_base, int vars_per_cell, int64_t nchi, int64_t neta,
+                             hymor_weights **out);
+
+int hymor_weights_destroy(hymor_weights *W);
+
+/* n_t Taylor steps of T_order(A*dt) from x; after each step i (and for the
+ * start, i = 0) E[m*(n_t+1) + i] = real(v)' * C_m * real(v).  x_final
+ * (optional) receives the last state.  If n_samples > 0, the state after
+ * every step i with i % n_skip == 0 is copied to samples[count*n ...] for
+ * the first n_samples such steps (the reference's Vf snapshots, :402-409).
+ * The reference leaves E[:,1] = 0 and fills it on the host (:168-173);
+ * here entry 0 holds that same initial energy.  Concurrent calls sharing
+ * one hymor_weights handle race on its reduction buffer. */
+int hymor_energy_track(hymor_op *A, const hymor_weights *W, int order, int64_t n_t, double dt,
+                       int form, const void *x, int dtype, double *E, void *x_final,
+                       int64_t n_skip, int64_t n_samples, void *samples);
 
 #ifdef __cplusplus
 }

@@ -104,3 +104,57 @@ extern "C" int emu_spmv_csc(int64_t n, const int64_t *colptr, const int64_t *row
     }
     return g_oob;
 }
+
+// Batched form: k_perm_in (padded width) -> k_spmm's index math for R
+// doubles per row -> k_perm_out.  Real operators without extras.
+extern "C" int emu_spmm_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                            const double *nzval, int b, int64_t ncell, int nrhs, int R,
+                            const double *X_vm, double *Y_vm) {
+    g_oob = 0;
+    HostBSR fwd, adj;
+    HostExtra xf, xa;
+    csc_to_blocks(n, colptr, rowval, nzval, 1, false, 1.0, b, ncell, fwd, adj, xf, xa);
+    if (!xf.rows.empty()) return -1;
+    HostSliced s;
+    bsr_to_sliced(fwd, s);
+    const int64_t nflow = (int64_t)b * ncell;
+    std::vector<double> x((size_t)n * R), y((size_t)n * R, 0.0);
+    // k_perm_in with width R, ncomp 1
+    for (int64_t i = 0; i < n * R; ++i) {
+        const int64_t row = i / R, rem = i % R;
+        if (rem >= nrhs) { x[(size_t)i] = 0.0; continue; }
+        const int64_t vm = row < nflow ? (row % b) * ncell + row / b : row;
+        x[(size_t)i] = X_vm[rem * n + vm];
+    }
+    const int CPS = 64 / b, BB = b * b, G = 64 / R, RPL = b / G;
+    for (int64_t cell = 0; cell < ncell; ++cell) {
+        const int64_t slice = cell / CPS, q = cell - slice * CPS;
+        const int64_t p0 = at(s.slice_ptr, slice);
+        const int64_t width = (at(s.slice_ptr, slice + 1) - p0) / CPS;
+        for (int lane = 0; lane < 64; ++lane) {
+            const int r = lane % R, g = lane / R;
+            double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int64_t t = 0; t < width; ++t) {
+                const int64_t blk = p0 + t * CPS + q;
+                const int64_t c = at(s.col, blk);
+                for (int k = 0; k < b; ++k) {
+                    const double xk = at(x, c * (b * R) + r + k * R);
+                    for (int i = 0; i < RPL; ++i)
+                        acc[i] += at(s.val, blk * BB + (g * RPL) * b + i * b + k) * xk;
+                }
+            }
+            for (int i = 0; i < RPL; ++i) {
+                const int64_t idx = (cell * b + g * RPL + i) * R + r;
+                if (idx < 0 || idx >= (int64_t)y.size()) ++g_oob;
+                else y[(size_t)idx] = acc[i];
+            }
+        }
+    }
+    // k_perm_out
+    for (int64_t i = 0; i < n * nrhs; ++i) {
+        const int64_t rhs = i / n, vm = i % n;
+        const int64_t row = vm < nflow ? (vm % ncell) * b + vm / ncell : vm;
+        Y_vm[rhs * n + vm] = at(y, row * R + rhs);
+    }
+    return g_oob;
+}

@@ -1,0 +1,199 @@
+"""GPU tests of the host-side mirror of the reference's stability entry
+points and of the device-resident energy tracking.
+
+* Energy tracking vs the oracle's LINEAR_INTEGRATION_AND_ENERGY restatement
+  (relative 1e-12 per energy).
+* EIGENVALUES_GPU_ARNOLDI on a block-circulant operator vs the closed form
+  log(T_5(lambda*dt)^n)/(n*dt) over the exact spectrum (relative 1e-8, the
+  bound the eigen path is held to), with ARPACK tightened to 1e-12.
+* TRANSIENT_GROWTH_DOWNSTREAM / FREESTREAM_RECEPTIVITY gains vs a dense
+  generalized Hermitian eigensolve of the same discrete operator (1e-8).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+import scipy.sparse as sp
+
+from hymor_amd import operators as ops
+from hymor_amd import stability as st
+from hymor_amd.baseflow import synthetic_solution
+from hymor_amd.core import DeviceOperator, DeviceWeights, energy_track
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def relerr(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b))
+
+
+def chu_weights(s, w_infty=None):
+    R = st.CONSTRUCT_R(s) if w_infty is None else st.CONSTRUCT_R_(s, w_infty)
+    out = []
+    for norms in ([1, 1, 1, 1], [1, 0, 0, 0], [0, 1, 1, 0], [0, 0, 0, 1]):
+        M = st.CONSTRUCT_M(s, norms) if w_infty is None else st.CONSTRUCT_M_(s, norms, w_infty)
+        out.append((R.T @ M @ R).tocsc())
+    return out
+
+
+@pytest.mark.parametrize("shock", [False, True])
+def test_energy_track_vs_oracle(orc, shock):
+    nchi, neta = 12, 9
+    say(f"[energy] real, shock={shock}")
+    s = synthetic_solution(nchi, neta, seed=1, perturb_shock=shock)
+    op = ops.stencil_operator(nchi, neta, 4, seed=3, n_shock=nchi if shock else 0, row_sum=2.0)
+    A = op.to_scipy_varmajor() * 0.05
+    Ws = chu_weights(s)
+    x = np.random.default_rng(4).standard_normal(op.n)
+    n_t = 9
+    Eref, vref = orc.linear_integration_energy(orc.CSC.from_scipy(A),
+                                               [orc.CSC.from_scipy(W) for W in Ws], x, n_t)
+    with DeviceOperator.from_scipy(A, nchi=nchi, neta=neta) as dev, \
+            DeviceWeights(Ws, nchi=nchi, neta=neta) as dW:
+        E, xf, samp = energy_track(dev, dW, x, n_t=n_t, n_skip=3, n_samples=2)
+        for k, W in enumerate(Ws):
+            assert abs(E[k, 0] - x @ (W @ x)) <= 1e-12 * abs(x @ (W @ x))
+            assert relerr(E[k, 1:], Eref[k, 1:]) <= 1e-12
+        assert relerr(xf, vref) <= 1e-12
+        assert relerr(samp[:, 1], orc.power_method(orc.CSC.from_scipy(A), x, 6)) <= 1e-12
+        # the mirror of the reference function: entry 0 left at 0
+        t, Ea, Ep, Ek, ES = st.LINEAR_INTEGRATION_AND_ENERGY_GPU_NO_SAMPLES(
+            x, n_t, 0.05, 0.0, 5, dev, *Ws, weights=dW)
+        assert Ea[0] == 0.0 and relerr(Ea[1:], Eref[0, 1:]) <= 1e-12
+        assert np.allclose(t, 0.05 * np.arange(n_t + 1))
+        # energies are additive: all = p + k + S (norm partition)
+        assert relerr(E[0], E[1] + E[2] + E[3]) <= 1e-12
+
+
+def test_energy_track_complex_receptivity(orc):
+    nchi, neta = 8, 6
+    w = [0.0, 1.2, 3.4]
+    say("[energy] complex receptivity operator")
+    s = synthetic_solution(nchi, neta, seed=2, perturb_shock=True)
+    base = ops.stencil_operator(nchi, neta, 4, seed=5, n_shock=nchi, row_sum=2.0)
+    L_ = ops.receptivity_extension(base, w)
+    A = L_.to_scipy_varmajor() * 0.05
+    Ws = chu_weights(s, w)
+    r = np.random.default_rng(6)
+    x = r.standard_normal(L_.n) + 1j * r.standard_normal(L_.n)
+    Eref, vref = orc.linear_integration_energy(orc.CSC.from_scipy(A),
+                                               [orc.CSC.from_scipy(W) for W in Ws], x, 6)
+    with DeviceOperator.from_scipy(A, nchi=nchi, neta=neta) as dev, \
+            DeviceWeights(Ws, nchi=nchi, neta=neta) as dW:
+        E, xf, _ = energy_track(dev, dW, x, n_t=6)
+    for k in range(4):
+        assert relerr(E[k, 1:], Eref[k, 1:]) <= 1e-12
+    assert relerr(xf, vref) <= 1e-12
+
+
+def test_weights_reject_non_block_diagonal():
+    from hymor_amd._lib import HymorError
+    nchi, neta = 4, 3
+    n = 4 * nchi * neta
+    M = sp.eye(n, format="csc") + sp.csc_matrix(([1.0], ([0], [n - 1])), shape=(n, n))
+    with pytest.raises(HymorError, match="block diagonal"):
+        DeviceWeights([M], nchi=nchi, neta=neta)
+
+
+# ------------------------------------------------------------------ eigen drivers
+
+def test_eigenvalues_gpu_arnoldi_circulant_closed_form(orc):
+    nchi, neta, b = 9, 6, 4
+    say("[EIGENVALUES] circulant operator, closed-form spectrum")
+    op, blocks = ops.circulant_operator(nchi, neta, b, seed=4, row_sum=3.0)
+    A = op.to_scipy_varmajor()
+    dt = 0.05
+    s = {"mesh": {"Nchi": nchi, "Neta": neta}, "time_integration": {"dt": dt},
+         "stability_analysis": {"perturb_shock": False,
+                                "eigenvalue_solver": "GPU_TIMESTEPPER_ARNOLDI"}}
+    k = 6
+    V, D = st.EIGENVALUES_GPU_ARNOLDI(A, s, k, tol=1e-12, ncv=40, v0=np.ones(A.shape[0]))
+    n = int(round(0.5 / dt))
+    lam = orc.circulant_eigenvalues(blocks, nchi, neta)
+    t5 = sum((lam * dt) ** j / math.factorial(j) for j in range(6))
+    mu = t5 ** n
+    order = np.argsort(-np.abs(mu), kind="stable")
+    exact = np.log(mu[order]) / (n * dt)
+    # compare as sets (conjugate pairs may come in either order)
+    for d in D:
+        j = np.argmin(np.abs(exact[:k + 4] - d))
+        assert abs(exact[j] - d) <= 1e-8 * abs(exact[j]), (d, exact[j])
+    # eigenvectors: residual of the discrete operator
+    dev = DeviceOperator.from_scipy(A, nchi=nchi, neta=neta, scale=dt)
+    for i in range(k):
+        v = V[:, i]
+        r = dev.expm_taylor(v, order=5, nsub=n) - np.exp(D[i] * n * dt) * v
+        assert np.linalg.norm(r) <= 1e-7 * np.linalg.norm(v)
+    dev.close()
+
+
+def test_eigenvalues_failure_returns_nan():
+    nchi, neta = 6, 4
+    op = ops.stencil_operator(nchi, neta, 4, seed=1, row_sum=2.0)
+    s = {"mesh": {"Nchi": nchi, "Neta": neta}, "time_integration": {"dt": 0.05},
+         "stability_analysis": {"perturb_shock": False,
+                                "eigenvalue_solver": "GPU_TIMESTEPPER_ARNOLDI"}}
+    with pytest.warns(UserWarning, match="did not converge"):
+        V, D = st.EIGENVALUES_GPU_ARNOLDI(op.to_scipy_varmajor(), s, 4, maxiter=1, ncv=9)
+    assert np.all(np.isnan(V)) and np.all(np.isnan(D))
+
+
+def _dense_gains(A, C, D, dt, n_t, n0=0, k=4):
+    Ad = A.toarray() * dt
+    P = np.linalg.matrix_power(sum(np.linalg.matrix_power(Ad, j) / math.factorial(j)
+                                   for j in range(6)), n_t)
+    K = P.conj().T @ C.toarray() @ P
+    if n0:
+        K[:n0, :] = 0
+        K[:, :n0] = 0
+    w = sl.eigh(K, D.toarray(), eigvals_only=True)
+    return np.sort(w)[::-1][:k]
+
+
+def test_transient_growth_downstream_vs_dense():
+    nchi, neta = 8, 6
+    say("[TRANSIENT_GROWTH_DOWNSTREAM] vs dense generalized eigensolve")
+    s = synthetic_solution(nchi, neta, seed=3, perturb_shock=True)
+    s = st.CFL_TIMESTEP(s)
+    op = ops.stencil_operator(nchi, neta, 4, seed=7, n_shock=nchi, row_sum=0.5 / s["time_integration"]["dt"])
+    A = op.to_scipy_varmajor()
+    T = 1.0
+    V, G, T_opt = st.TRANSIENT_GROWTH_DOWNSTREAM(A, s, 4, T, tol=1e-12, ncv=30,
+                                                 v0=np.ones(A.shape[0]))
+    n_t = int(round(T / s["time_integration"]["dt"]))
+    dt = T / n_t
+    assert T_opt == pytest.approx(n_t * dt)
+    R = st.CONSTRUCT_R(s)
+    C = R.T @ st.CONSTRUCT_M(s, [1, 1, 1, 1]) @ R
+    D = R.T @ st.CONSTRUCT_M_HAT(s, [1, 1, 1, 1]) @ R
+    ref = _dense_gains(A, C, D, dt, n_t)
+    assert np.all(np.abs(G - ref) <= 1e-8 * np.abs(ref)), (G, ref)
+
+
+def test_freestream_receptivity_vs_dense():
+    nchi, neta = 6, 5
+    w = [0.0, 2.0]
+    say("[FREESTREAM_RECEPTIVITY] vs dense generalized eigensolve")
+    s = synthetic_solution(nchi, neta, seed=4, perturb_shock=True)
+    s = st.CFL_TIMESTEP(s)
+    base = ops.stencil_operator(nchi, neta, 4, seed=8, n_shock=nchi,
+                                row_sum=0.5 / s["time_integration"]["dt"])
+    L_ = ops.receptivity_extension(base, w)
+    A = L_.to_scipy_varmajor()
+    T = 1.0
+    V, G, T_opt = st.FREESTREAM_RECEPTIVITY(A, s, 3, T, w, tol=1e-12, ncv=24,
+                                            v0=np.ones(A.shape[0], dtype=complex))
+    n_t = int(round(T / s["time_integration"]["dt"]))
+    dt = T / n_t
+    R_ = st.CONSTRUCT_R_(s, w)
+    C_ = R_.T @ st.CONSTRUCT_M_(s, [1, 1, 1, 1], w) @ R_
+    D_ = R_.T @ st.CONSTRUCT_M_INFTY_(s, [1, 1, 1, 1], T, w, True) @ R_
+    ref = _dense_gains(A, C_, D_, dt, n_t, n0=4 * nchi * neta, k=3) / 2
+    assert np.all(np.abs(G - ref) <= 1e-8 * np.abs(ref)), (G, ref)

@@ -77,3 +77,19 @@ def test_emulated_complex_receptivity(emu, transpose):
     assert oob == 0
     ref = (A.conj().T if transpose else A) @ x
     assert np.allclose(y, ref, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("b,R,nrhs", [(4, 16, 16), (4, 16, 11), (4, 32, 32), (4, 64, 40),
+                                      (8, 8, 8), (8, 16, 13), (8, 32, 32), (8, 64, 64)])
+def test_emulated_batched_spmm(emu, b, R, nrhs):
+    emu.emu_spmm_csc.restype = C.c_int
+    emu.emu_spmm_csc.argtypes = [C.c_int64] + [C.c_void_p] * 3 + [C.c_int, C.c_int64, C.c_int,
+                                                                   C.c_int, C.c_void_p, C.c_void_p]
+    op = ops.stencil_operator(11, 7, b, seed=9)
+    colptr, rowval, nzval = op.to_csc_varmajor(one_based=True)
+    X = np.asfortranarray(np.random.default_rng(2).standard_normal((op.n, nrhs)))
+    Y = np.zeros_like(X)
+    oob = emu.emu_spmm_csc(op.n, colptr.ctypes.data, rowval.ctypes.data, nzval.ctypes.data, b,
+                           op.ncell, nrhs, R, X.ctypes.data, Y.ctypes.data)
+    assert oob == 0
+    assert np.allclose(Y, op.to_scipy_varmajor() @ X, rtol=1e-13, atol=1e-13)
