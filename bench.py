@@ -105,6 +105,9 @@ def main():
     op = ops.config_operator(args.config)
     log(f"n={op.n} nnzb={op.nnzb}; uploading FP64 blocks")
     dev = DeviceOperator.from_blocks(op, value_prec="f64", with_adjoint=False)
+    tune_ms, tuning = dev.autotune(reps=10)
+    log(f"autotuned Taylor-term launch: split={tuning[0]} waves={tuning[1]} nt={tuning[2]} "
+        f"({tune_ms:.4f} ms)")
     x = np.random.default_rng(cfg["seed"] - 1000).standard_normal(op.n)
     dx = DeviceVector.from_host(x)
     dy = DeviceVector(op.n)
@@ -209,7 +212,9 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": f"k_term<{cfg['b']}, double, 1>",
+            "kernel": (f"k_term_split<{cfg['b']}, double, 1, {tuning[1]}, {bool(tuning[2])}>"
+                       if tuning[0] else f"k_term<{cfg['b']}, double, 1, {bool(tuning[2])}>"),
+            "launch": {"split": tuning[0], "waves": tuning[1], "nontemporal": tuning[2]},
             "kernel_ms": k_ms,
             "algorithmic_bytes_per_launch": k_bytes,
         },

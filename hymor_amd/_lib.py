@@ -33,6 +33,7 @@ SIGNATURES = {
     "hymor_op_destroy": (C.c_int, [_p]),
     "hymor_op_info": (C.c_int, [_p, _pi64, C.c_int]),
     "hymor_spmv": (C.c_int, [_p, C.c_int, _p, _p, C.c_int]),
+    "hymor_spmv_dev": (C.c_int, [_p, C.c_int, _p, _p, C.c_int]),
     "hymor_expm_taylor": (C.c_int, [_p, C.c_int, C.c_int, _i64, C.c_double, C.c_int, _p, _p,
                                     C.c_int]),
     "hymor_expm_taylor_dev": (C.c_int, [_p, C.c_int, C.c_int, _i64, C.c_double, C.c_int, _p, _p,
@@ -50,7 +51,20 @@ SIGNATURES = {
     "hymor_memcpy_h2d": (C.c_int, [_p, _p, _i64]),
     "hymor_memcpy_d2h": (C.c_int, [_p, _p, _i64]),
     "hymor_bench_term_kernel": (C.c_int, [_p, C.c_int, C.c_int, C.c_int, _pd, _pd]),
+    "hymor_op_autotune": (C.c_int, [_p, C.c_int, C.c_int, _pd]),
+    "hymor_op_get_tuning": (C.c_int, [_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
+    "hymor_op_set_tuning": (C.c_int, [_p, C.c_int, C.c_int, C.c_int]),
     "hymor_bench_copy": (C.c_int, [_i64, C.c_int, _pd]),
+    "hymor_krylov_create": (C.c_int, [_i64, C.c_int, C.c_int, _pp]),
+    "hymor_krylov_destroy": (C.c_int, [_p]),
+    "hymor_krylov_column": (C.c_int, [_p, C.c_int, _pp]),
+    "hymor_krylov_set_column": (C.c_int, [_p, C.c_int, _p]),
+    "hymor_krylov_get_columns": (C.c_int, [_p, C.c_int, C.c_int, _p]),
+    "hymor_krylov_orthonormalize": (C.c_int, [_p, C.c_int, _p, _pd]),
+    "hymor_krylov_combine": (C.c_int, [_p, C.c_int, C.c_int, _pd]),
+    "hymor_krylov_ritz": (C.c_int, [_p, C.c_int, C.c_int, _pd, _p]),
+    "hymor_krylov_copy_column": (C.c_int, [_p, C.c_int, C.c_int]),
     "hymor_weights_create_csc": (C.c_int, [_i64, C.c_int, _p, _p, _p, C.c_int, C.c_int, _i64,
                                            _i64, _pp]),
     "hymor_weights_destroy": (C.c_int, [_p]),

@@ -1,16 +1,22 @@
 """Streaming-copy calibration: achievable HBM bandwidth on this GPU, and the
 reference kernel for scaling the FETCH_SIZE / WRITE_SIZE profiler counters.
 
-``python -m hymor_amd.calibrate [MiB] [reps]`` prints one JSON line: a copy
-of S bytes reads S and writes S, so GB/s = 2*S / time.
+``python hymor_amd/calibrate.py [MiB] [reps]`` (or ``python -m
+hymor_amd.calibrate``) prints one JSON line: a copy of S bytes reads S and
+writes S, so GB/s = 2*S / time.
 """
 from __future__ import annotations
 
 import ctypes as C
 import json
 import sys
+from pathlib import Path
 
-from . import _lib
+if __package__ in (None, ""):      # run as a script (e.g. under a profiler from another cwd)
+    sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
+    from hymor_amd import _lib
+else:
+    from . import _lib
 
 
 def copy_bandwidth(mib: int = 1024, reps: int = 20):

@@ -145,6 +145,23 @@ class DeviceOperator:
     def synchronize(self):
         check(_lib.load().hymor_op_synchronize(self._h))
 
+    def autotune(self, reps=10, nv=None):
+        """Pick the fastest Taylor-term launch variant for this operator;
+        returns (best ms per launch, (split, waves, nt))."""
+        nv = nv or (2 if self.is_complex else 1)
+        ms = C.c_double()
+        check(_lib.load().hymor_op_autotune(self._h, nv, reps, C.byref(ms)))
+        return ms.value, self.tuning
+
+    @property
+    def tuning(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        check(_lib.load().hymor_op_get_tuning(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def set_tuning(self, split, waves, nt):
+        check(_lib.load().hymor_op_set_tuning(self._h, int(split), int(waves), int(nt)))
+
     def bench_term_kernel(self, reps=20, transpose=False, nv=None):
         nv = nv or (2 if self.is_complex else 1)
         ms = C.c_double()
@@ -328,3 +345,28 @@ def energy_track(op: DeviceOperator, weights: DeviceWeights, x, *, n_t, order=5,
                                          float(dt), _FORMS[form], ptr(x), dty, ptr(E), ptr(xf),
                                          int(max(n_skip, 1)), int(n_samples), sp_ptr))
     return E, xf, (samp.T if samp is not None else None)
+
+
+def spmv_dev(op: DeviceOperator, d_x, d_y, *, transpose=False, complex_=None):
+    """Asynchronous y = A x (or A' x) between device pointers (var-major)."""
+    cplx = op.is_complex if complex_ is None else complex_
+    check(_lib.load().hymor_spmv_dev(op.handle, int(bool(transpose)), d_x, d_y,
+                                     C128 if cplx else F64))
+
+
+def expm_taylor_ptr(op: DeviceOperator, d_x, d_y, *, order=5, nsub=1, dt=1.0, form="powers",
+                    transpose=False, complex_=None):
+    """expm_taylor_dev on raw device pointers (e.g. Krylov basis columns)."""
+    cplx = op.is_complex if complex_ is None else complex_
+    check(_lib.load().hymor_expm_taylor_dev(op.handle, int(bool(transpose)), int(order), int(nsub),
+                                            float(dt), _FORMS[form], d_x, d_y,
+                                            C128 if cplx else F64))
+
+
+def chu_sandwich_ptr(op: DeviceOperator, weight: DeviceOperator, d_x, d_y, *, order=5, nsub=1,
+                     dt=1.0, n_set0=0, complex_=None):
+    """hymor_chu_sandwich_dev on raw device pointers."""
+    cplx = op.is_complex if complex_ is None else complex_
+    check(_lib.load().hymor_chu_sandwich_dev(op.handle, weight.handle, int(order), int(nsub),
+                                             float(dt), int(n_set0), d_x, d_y,
+                                             C128 if cplx else F64))

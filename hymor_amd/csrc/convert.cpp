@@ -236,4 +236,24 @@ void bsr_to_sliced(const HostBSR &a, HostSliced &s) {
     }
 }
 
+void interleave_lanes(HostSliced &s, int vpl) {
+    // Inside each slot of a slice the CPS blocks (64 block rows = 64 lanes)
+    // are re-ordered so that value k of lane (q, r) sits at
+    // ((k / vpl) * 64 + q*b + r) * vpl + k % vpl: the lanes' 16-byte chunks
+    // for one k-range are consecutive, so each wave-wide 16-byte load reads
+    // one contiguous 1 KB run.
+    const int b = s.b, bb = b * b, cps = s.cps;
+    const int64_t nslots = s.slice_ptr[s.nslice] / cps;
+    std::vector<double> tmp((size_t)cps * bb);
+    for (int64_t g = 0; g < nslots; ++g) {
+        double *base = s.val.data() + (size_t)g * cps * bb;
+        for (int q = 0; q < cps; ++q)
+            for (int r = 0; r < b; ++r)
+                for (int k = 0; k < b; ++k)
+                    tmp[(size_t)(((k / vpl) * 64 + q * b + r) * vpl + k % vpl)] =
+                        base[q * bb + r * b + k];
+        std::copy(tmp.begin(), tmp.end(), base);
+    }
+}
+
 }  // namespace hymor

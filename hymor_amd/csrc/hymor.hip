@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -42,7 +43,16 @@ struct ArgError : std::invalid_argument {
         if (!(cond)) throw ArgError(msg); \
     } while (0)
 
+// Launch variant of the single-vector Taylor-term kernel, chosen per
+// operator by hymor_op_autotune (defaults: k_term, 4 waves per group).
+struct TermTuning {
+    int split = 0;   // 0: k_term (one wave per slice), 1: k_term_split
+    int waves = 4;   // k_term: waves per workgroup; k_term_split: waves per slice
+    int nt = 0;      // non-temporal loads of the block values
+};
+
 struct DevSliced {
+    TermTuning tune;
     int b = 0, cps = 0, prec = HYMOR_PREC_F64;
     int64_t ncell = 0, nslice = 0, nnzb = 0, nnzb_pad = 0;
     int64_t *slice_ptr = nullptr;
@@ -91,8 +101,25 @@ struct hymor_weights {
     std::vector<void *> allocs;
 };
 
+// Device-resident Krylov basis: V is n x (m_max+1), column-major, in the
+// ABI's var-major order, so its columns feed the *_dev actions directly.
+struct hymor_krylov {
+    int64_t n = 0;
+    int m_max = 0, ncomp = 1;
+    double *V = nullptr;        // (m_max+1) columns
+    double *tmp = nullptr;      // (m_max+1) columns, for the restart combination
+    double *h = nullptr;        // 2*(m_max+1) coefficients
+    double *h2 = nullptr;
+    double *q = nullptr;        // (m_max+1)^2 complex, restart matrix
+    double *partial = nullptr;  // 2*(m_max+1) * kDotBlocks
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+};
+
 namespace {
 
+constexpr int kDotBlocks = 512;   // workgroups of the tall-skinny dot kernel
+constexpr int kDotTile = 16;      // basis columns per dot launch
 constexpr int kMaxWeights = 8;   // matches k_quad_blockdiag's NMAX
 
 void *dalloc_raw(std::vector<void *> &allocs, size_t bytes) {
@@ -130,6 +157,7 @@ T *upload(hymor_op *op, const std::vector<T> &h) {
 void upload_sliced(hymor_op *op, const HostBSR &a, int prec, DevSliced &d) {
     HostSliced s;
     bsr_to_sliced(a, s);
+    interleave_lanes(s, prec == HYMOR_PREC_F32 ? 4 : 2);
     d.b = s.b;
     d.cps = s.cps;
     d.prec = prec;
@@ -206,14 +234,36 @@ void ensure_ws(hymor_op *op, int nv) {
 
 // ---------------------------------------------------------------- launches
 
+template <int B, typename VT, int NV, int NW, bool NT>
+void launch_split(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+                  double *tout, const double *vin, double *vout, double c1, double c2) {
+    hipLaunchKernelGGL((k_term_split<B, VT, NV, NW, NT>), dim3((unsigned)s.nslice), dim3(64 * NW),
+                       0, st, s.slice_ptr, s.col, (const VT *)s.val, s.nslice, s.ncell, x, yx,
+                       tout, vin, vout, c1, c2);
+}
+
+template <int B, typename VT, int NV, bool NT>
+void launch_term_nt(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+                    double *tout, const double *vin, double *vout, double c1, double c2) {
+    const TermTuning &tu = s.tune;
+    if (tu.split) {
+        if (tu.waves == 2) launch_split<B, VT, NV, 2, NT>(s, st, x, yx, tout, vin, vout, c1, c2);
+        else launch_split<B, VT, NV, 4, NT>(s, st, x, yx, tout, vin, vout, c1, c2);
+        return;
+    }
+    // small operators: one wave per group so the slices spread over all CUs
+    const int wpb = s.nslice >= 8192 ? tu.waves : 1;
+    const int64_t grid = (s.nslice + wpb - 1) / wpb;
+    hipLaunchKernelGGL((k_term<B, VT, NV, NT>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
+                       s.slice_ptr, s.col, (const VT *)s.val, s.nslice, s.ncell, x, yx, tout, vin,
+                       vout, c1, c2);
+}
+
 template <int B, typename VT, int NV>
 void launch_term_t(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
                    double *tout, const double *vin, double *vout, double c1, double c2) {
-    const int wpb = s.nslice >= 8192 ? 4 : 1;
-    const int64_t grid = (s.nslice + wpb - 1) / wpb;
-    hipLaunchKernelGGL((k_term<B, VT, NV>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
-                       s.slice_ptr, s.col, (const VT *)s.val, s.nslice, s.ncell, x, yx, tout, vin,
-                       vout, c1, c2);
+    if (s.tune.nt) launch_term_nt<B, VT, NV, true>(s, st, x, yx, tout, vin, vout, c1, c2);
+    else launch_term_nt<B, VT, NV, false>(s, st, x, yx, tout, vin, vout, c1, c2);
 }
 
 template <int B, typename VT, int R>
@@ -580,6 +630,19 @@ int hymor_spmv(hymor_op *op, int transpose, const void *x, void *y, int dtype) {
     });
 }
 
+int hymor_spmv_dev(hymor_op *op, int transpose, const void *d_x, void *d_y, int dtype) {
+    return guarded([&] {
+        ARG_CHECK(op && d_x && d_y, "null argument");
+        ARG_CHECK(!transpose || op->has_adj, "operator was built without its adjoint");
+        const int nv = nv_of(op, dtype);
+        ensure_ws(op, nv);
+        launch_perm_in(op, op->stream, (const double *)d_x, op->n, op->ws[1], 1, nv, 0, nv);
+        launch_term(op, transpose != 0, nv, op->stream, op->ws[1], op->ws[2], nullptr, nullptr,
+                    1.0, 0.0);
+        launch_perm_out(op, op->stream, op->ws[2], (double *)d_y, op->n, 1, nv, 0, nv);
+    });
+}
+
 static void check_batch(hymor_op *op, int64_t nrhs, int64_t ldx, int64_t ldy) {
     ARG_CHECK(nrhs >= 1, "nrhs must be >= 1");
     ARG_CHECK(ldx >= op->n && ldy >= op->n, "leading dimension smaller than n");
@@ -742,6 +805,66 @@ int hymor_memcpy_d2h(void *h_dst, const void *d_src, int64_t bytes) {
     });
 }
 
+int hymor_op_autotune(hymor_op *op, int nv, int reps, double *best_ms) {
+    return guarded([&] {
+        ARG_CHECK(op && best_ms, "null argument");
+        ARG_CHECK(nv == 1 || nv == 2 || nv == 4, "autotune covers the k_term widths 1, 2, 4");
+        ARG_CHECK(!(op->is_complex && nv == 1), "complex operator needs nv >= 2");
+        ARG_CHECK(reps >= 1, "reps must be >= 1");
+        ensure_ws(op, nv);
+        const TermTuning cands[] = {{0, 4, 0}, {0, 4, 1}, {0, 2, 0}, {1, 2, 0},
+                                    {1, 2, 1}, {1, 4, 0}, {1, 4, 1}};
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        double best = 1e300;
+        TermTuning pick = op->fwd.tune;
+        for (const TermTuning &c : cands) {
+            op->fwd.tune = c;
+            launch_term(op, false, nv, op->stream, op->ws[4], op->ws[5], op->ws[2], op->ws[2], 0.5,
+                        1.0);
+            HIP_TRY(hipEventRecord(e0, op->stream));
+            for (int i = 0; i < reps; ++i)
+                launch_term(op, false, nv, op->stream, op->ws[4], op->ws[5], op->ws[2], op->ws[2],
+                            0.5, 1.0);
+            HIP_TRY(hipEventRecord(e1, op->stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            if (ms / reps < best) {
+                best = ms / reps;
+                pick = c;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        op->fwd.tune = pick;
+        op->adj.tune = pick;
+        *best_ms = best;
+    });
+}
+
+int hymor_op_get_tuning(const hymor_op *op, int *split, int *waves, int *nt) {
+    return guarded([&] {
+        ARG_CHECK(op && split && waves && nt, "null argument");
+        *split = op->fwd.tune.split;
+        *waves = op->fwd.tune.waves;
+        *nt = op->fwd.tune.nt;
+    });
+}
+
+int hymor_op_set_tuning(hymor_op *op, int split, int waves, int nt) {
+    return guarded([&] {
+        ARG_CHECK(op, "null operator");
+        ARG_CHECK((split == 0 && (waves == 1 || waves == 2 || waves == 4)) ||
+                      (split == 1 && (waves == 2 || waves == 4)),
+                  "k_term takes 1, 2 or 4 waves per group; k_term_split 2 or 4 waves per slice");
+        ARG_CHECK(nt == 0 || nt == 1, "nt must be 0 or 1");
+        op->fwd.tune = TermTuning{split, waves, nt};
+        op->adj.tune = op->fwd.tune;
+    });
+}
+
 int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int reps, double *ms_per_launch,
                             double *bytes_per_launch) {
     return guarded([&] {
@@ -827,7 +950,7 @@ int hymor_bench_copy(int64_t bytes, int reps, double *ms_per_launch) {
         d2 *out = (d2 *)dalloc_raw(allocs, (size_t)bytes);
         HIP_TRY(hipMemset(in, 0, (size_t)bytes));
         const int64_t n2 = bytes / 16;
-        const int64_t grid = std::min<int64_t>((n2 + 255) / 256, 8192);
+        const int64_t grid = std::min<int64_t>((n2 + 255) / 256, 4096);
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
@@ -972,6 +1095,222 @@ int hymor_energy_track(hymor_op *A, const hymor_weights *W, int order, int64_t n
         HIP_TRY(hipMemcpyAsync(E, dE, (size_t)W->nmat * ncol * sizeof(double),
                                hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ Krylov basis
+
+namespace {
+
+// h[0 .. NC*m) <- V[:, 0:m]^H w  (device), deterministic reduction
+void basis_dots(hymor_krylov *K, const double *V, int64_t ld, int m, const double *w,
+                double *h) {
+    const bool cplx = K->ncomp == 2;
+    for (int j0 = 0; j0 < m; j0 += kDotTile) {
+        if (cplx)
+            hipLaunchKernelGGL((k_dots<true, kDotTile>), dim3(kDotBlocks), dim3(256), 0, K->stream,
+                               V, ld, m, j0, w, K->n, K->partial);
+        else
+            hipLaunchKernelGGL((k_dots<false, kDotTile>), dim3(kDotBlocks), dim3(256), 0,
+                               K->stream, V, ld, m, j0, w, K->n, K->partial);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_reduce_partials, dim3(m * K->ncomp), dim3(256), 0, K->stream, K->partial,
+                       (int64_t)kDotBlocks, h, (int64_t)1);
+    HIP_TRY(hipGetLastError());
+}
+
+void basis_sub(hymor_krylov *K, const double *V, int64_t ld, int m, const double *h, double *w) {
+    const int64_t grid = std::min<int64_t>((K->n + 255) / 256, 4096);
+    if (K->ncomp == 2)
+        hipLaunchKernelGGL(k_sub_vh<true>, dim3((unsigned)grid), dim3(256), 0, K->stream, V, ld, m,
+                           h, w, K->n);
+    else
+        hipLaunchKernelGGL(k_sub_vh<false>, dim3((unsigned)grid), dim3(256), 0, K->stream, V, ld,
+                           m, h, w, K->n);
+    HIP_TRY(hipGetLastError());
+}
+
+double *col(hymor_krylov *K, double *base, int j) {
+    return base + (size_t)j * K->n * K->ncomp;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hymor_krylov_create(int64_t n, int m_max, int dtype, hymor_krylov **out) {
+    hymor_krylov *K = nullptr;
+    int rc = guarded([&] {
+        ARG_CHECK(out != nullptr, "out is null");
+        *out = nullptr;
+        ARG_CHECK(n >= 1 && m_max >= 1 && m_max <= 1024, "need n >= 1 and 1 <= m_max <= 1024");
+        ARG_CHECK(dtype == HYMOR_F64 || dtype == HYMOR_C128, "bad dtype");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+            (void)hipGetLastError();
+            throw HipError("no HIP device available");
+        }
+        K = new hymor_krylov();
+        K->n = n;
+        K->m_max = m_max;
+        K->ncomp = dtype == HYMOR_C128 ? 2 : 1;
+        const size_t colb = (size_t)n * K->ncomp * sizeof(double);
+        K->V = (double *)dalloc_raw(K->allocs, colb * (m_max + 1));
+        K->tmp = (double *)dalloc_raw(K->allocs, colb * (m_max + 1));
+        K->h = (double *)dalloc_raw(K->allocs, 2 * (m_max + 1) * sizeof(double));
+        K->h2 = (double *)dalloc_raw(K->allocs, 2 * (m_max + 1) * sizeof(double));
+        K->q = (double *)dalloc_raw(K->allocs, 2 * (size_t)(m_max + 1) * (m_max + 1) * sizeof(double));
+        K->partial = (double *)dalloc_raw(K->allocs, 2 * (size_t)(m_max + 1) * kDotBlocks * sizeof(double));
+        HIP_TRY(hipStreamCreateWithFlags(&K->stream, hipStreamNonBlocking));
+        HIP_TRY(hipMemsetAsync(K->V, 0, colb * (m_max + 1), K->stream));
+        HIP_TRY(hipStreamSynchronize(K->stream));
+        *out = K;
+    });
+    if (rc != HYMOR_OK && K) {
+        for (void *p : K->allocs) (void)hipFree(p);
+        if (K->stream) (void)hipStreamDestroy(K->stream);
+        delete K;
+    }
+    return rc;
+}
+
+int hymor_krylov_destroy(hymor_krylov *K) {
+    return guarded([&] {
+        if (!K) return;
+        HIP_TRY(hipStreamSynchronize(K->stream));
+        for (void *p : K->allocs) (void)hipFree(p);
+        (void)hipStreamDestroy(K->stream);
+        delete K;
+    });
+}
+
+int hymor_krylov_column(hymor_krylov *K, int j, void **d_col) {
+    return guarded([&] {
+        ARG_CHECK(K && d_col, "null argument");
+        ARG_CHECK(j >= 0 && j <= K->m_max, "column out of range");
+        *d_col = col(K, K->V, j);
+    });
+}
+
+int hymor_krylov_set_column(hymor_krylov *K, int j, const void *x) {
+    return guarded([&] {
+        ARG_CHECK(K && x, "null argument");
+        ARG_CHECK(j >= 0 && j <= K->m_max, "column out of range");
+        HIP_TRY(hipMemcpyAsync(col(K, K->V, j), x, (size_t)K->n * K->ncomp * sizeof(double),
+                               hipMemcpyHostToDevice, K->stream));
+        HIP_TRY(hipStreamSynchronize(K->stream));
+    });
+}
+
+int hymor_krylov_get_columns(hymor_krylov *K, int j0, int ncols, void *out) {
+    return guarded([&] {
+        ARG_CHECK(K && out, "null argument");
+        ARG_CHECK(j0 >= 0 && ncols >= 0 && j0 + ncols <= K->m_max + 1, "columns out of range");
+        HIP_TRY(hipMemcpyAsync(out, col(K, K->V, j0),
+                               (size_t)ncols * K->n * K->ncomp * sizeof(double),
+                               hipMemcpyDeviceToHost, K->stream));
+        HIP_TRY(hipStreamSynchronize(K->stream));
+    });
+}
+
+int hymor_krylov_orthonormalize(hymor_krylov *K, int j, void *d_w, double *h_out) {
+    return guarded([&] {
+        ARG_CHECK(K && d_w && h_out, "null argument");
+        ARG_CHECK(j >= 0 && j <= K->m_max, "j out of range");
+        double *w = (double *)d_w;
+        const int nc = K->ncomp;
+        const int64_t len = K->n * nc;
+        // classical Gram-Schmidt, twice (CGS2)
+        if (j > 0) {
+            basis_dots(K, K->V, K->n, j, w, K->h);
+            basis_sub(K, K->V, K->n, j, K->h, w);
+            basis_dots(K, K->V, K->n, j, w, K->h2);
+            basis_sub(K, K->V, K->n, j, K->h2, w);
+        }
+        basis_dots(K, w, K->n, 1, w, K->h2 + 2 * (size_t)K->m_max);   // |w|^2 (re part)
+        std::vector<double> h1((size_t)nc * j + 2, 0.0), h2((size_t)nc * j + 2, 0.0);
+        if (j > 0) {
+            HIP_TRY(hipMemcpyAsync(h1.data(), K->h, (size_t)nc * j * sizeof(double),
+                                   hipMemcpyDeviceToHost, K->stream));
+            HIP_TRY(hipMemcpyAsync(h2.data(), K->h2, (size_t)nc * j * sizeof(double),
+                                   hipMemcpyDeviceToHost, K->stream));
+        }
+        double nrm2 = 0.0;
+        HIP_TRY(hipMemcpyAsync(&nrm2, K->h2 + 2 * (size_t)K->m_max, sizeof(double),
+                               hipMemcpyDeviceToHost, K->stream));
+        HIP_TRY(hipStreamSynchronize(K->stream));
+        for (int i = 0; i < nc * j; ++i) h_out[i] = h1[i] + h2[i];
+        const double nrm = std::sqrt(std::max(nrm2, 0.0));
+        h_out[nc * j] = nrm;
+        if (nc == 2) h_out[nc * j + 1] = 0.0;
+        if (nrm > 0.0) {
+            const int64_t grid = std::min<int64_t>((len + 255) / 256, 4096);
+            hipLaunchKernelGGL(k_scale, dim3((unsigned)grid), dim3(256), 0, K->stream, w, len,
+                               1.0 / nrm);
+            HIP_TRY(hipGetLastError());
+            if (w != col(K, K->V, j))
+                HIP_TRY(hipMemcpyAsync(col(K, K->V, j), w, (size_t)len * sizeof(double),
+                                       hipMemcpyDeviceToDevice, K->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(K->stream));
+    });
+}
+
+int hymor_krylov_combine(hymor_krylov *K, int m, int k, const double *Q) {
+    return guarded([&] {
+        ARG_CHECK(K && Q, "null argument");
+        ARG_CHECK(m >= 1 && m <= K->m_max + 1 && k >= 1 && k <= m, "need 1 <= k <= m <= m_max+1");
+        const int nc = K->ncomp;
+        HIP_TRY(hipMemcpyAsync(K->q, Q, (size_t)nc * m * k * sizeof(double), hipMemcpyHostToDevice,
+                               K->stream));
+        const int64_t grid = std::min<int64_t>((K->n + 255) / 256, 4096);
+        if (nc == 2)
+            hipLaunchKernelGGL(k_basis_combine<true>, dim3((unsigned)grid), dim3(256), 0, K->stream,
+                               K->V, K->n, m, K->q, k, K->tmp, K->n, K->n);
+        else
+            hipLaunchKernelGGL(k_basis_combine<false>, dim3((unsigned)grid), dim3(256), 0,
+                               K->stream, K->V, K->n, m, K->q, k, K->tmp, K->n, K->n);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(K->V, K->tmp, (size_t)k * K->n * nc * sizeof(double),
+                               hipMemcpyDeviceToDevice, K->stream));
+        HIP_TRY(hipStreamSynchronize(K->stream));
+    });
+}
+
+int hymor_krylov_ritz(hymor_krylov *K, int m, int k, const double *Q, void *out) {
+    return guarded([&] {
+        ARG_CHECK(K && Q && out, "null argument");
+        ARG_CHECK(m >= 1 && m <= K->m_max + 1 && k >= 1 && k <= K->m_max + 1,
+                  "need 1 <= m, k <= m_max+1");
+        const int nc = K->ncomp;
+        HIP_TRY(hipMemcpyAsync(K->q, Q, (size_t)nc * m * k * sizeof(double), hipMemcpyHostToDevice,
+                               K->stream));
+        const int64_t grid = std::min<int64_t>((K->n + 255) / 256, 4096);
+        if (nc == 2)
+            hipLaunchKernelGGL(k_basis_combine<true>, dim3((unsigned)grid), dim3(256), 0, K->stream,
+                               K->V, K->n, m, K->q, k, K->tmp, K->n, K->n);
+        else
+            hipLaunchKernelGGL(k_basis_combine<false>, dim3((unsigned)grid), dim3(256), 0,
+                               K->stream, K->V, K->n, m, K->q, k, K->tmp, K->n, K->n);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out, K->tmp, (size_t)k * K->n * nc * sizeof(double),
+                               hipMemcpyDeviceToHost, K->stream));
+        HIP_TRY(hipStreamSynchronize(K->stream));
+    });
+}
+
+int hymor_krylov_copy_column(hymor_krylov *K, int src, int dst) {
+    return guarded([&] {
+        ARG_CHECK(K, "null argument");
+        ARG_CHECK(src >= 0 && src <= K->m_max && dst >= 0 && dst <= K->m_max, "column out of range");
+        if (src == dst) return;
+        HIP_TRY(hipMemcpyAsync(col(K, K->V, dst), col(K, K->V, src),
+                               (size_t)K->n * K->ncomp * sizeof(double), hipMemcpyDeviceToDevice,
+                               K->stream));
+        HIP_TRY(hipStreamSynchronize(K->stream));
     });
 }
 

@@ -44,7 +44,7 @@ struct HostSliced {
     int64_t nslice = 0;
     std::vector<int64_t> slice_ptr;  // nslice+1, in blocks
     std::vector<int32_t> col;        // padded blocks
-    std::vector<double> val;         // padded blocks * b*b
+    std::vector<double> val;         // padded blocks * b*b (see interleave_lanes)
     int64_t nnzb = 0;                // unpadded
 };
 
@@ -60,5 +60,9 @@ void extra_from_coo(int64_t n, int64_t nflow, const std::vector<int64_t> &r,
                     bool is_complex, bool adjoint, HostExtra &out);
 
 void bsr_to_sliced(const HostBSR &a, HostSliced &s);
+
+// Lane-interleaved value order inside every slot (vpl values per 16-byte
+// chunk: 2 for FP64 storage, 4 for FP32).
+void interleave_lanes(HostSliced &s, int vpl);
 
 }  // namespace hymor

@@ -6,9 +6,10 @@
 //     (re,im), 2*nrhs for batches).
 //   - A's flow-flow part is sliced ELL over cells: a slice is CPS = 64/b
 //     cells, one wavefront per slice, lane = (cell q, block row r).  Slot t
-//     of cell q is block slice_ptr[s] + t*CPS + q, so each slot of a slice is
-//     one contiguous CPS*b*b-value run: lanes read their block row with
-//     16-byte loads and the wavefront covers the run without gaps.
+//     of a slice is one contiguous CPS*b*b-value run starting at block
+//     slice_ptr[s] + t*CPS; inside it value k of lane (q, r) sits at
+//     ((k / VPL) * 64 + lane) * VPL + k % VPL, VPL = 16 bytes / value size,
+//     so every wave-wide 16-byte load covers one contiguous 1 KB run.
 //   - everything else (shock bands, freestream coupling, imaginary parts) is
 //     a short CSR over the listed rows, one wavefront per row.
 #pragma once
@@ -42,22 +43,47 @@ __device__ __forceinline__ void load_f64(const double *__restrict__ p, double *a
     }
 }
 
-template <int N>
+// Block values are streamed once per term; NT = true marks the loads
+// non-temporal so they do not displace the gathered x entries in L2.
+template <int N, bool NT = false>
 __device__ __forceinline__ void load_vals(const double *__restrict__ p, double *a) {
-    load_f64<N>(p, a);
+#pragma unroll
+    for (int k = 0; k < N; k += 2) {
+        const d2 v = NT ? __builtin_nontemporal_load(reinterpret_cast<const d2 *>(p + k))
+                        : *reinterpret_cast<const d2 *>(p + k);
+        a[k] = v.x;
+        a[k + 1] = v.y;
+    }
 }
 
-template <int N>
+template <int N, bool NT = false>
 __device__ __forceinline__ void load_vals(const float *__restrict__ p, double *a) {
     static_assert(N % 4 == 0, "quads");
 #pragma unroll
     for (int k = 0; k < N; k += 4) {
-        const f4 v = *reinterpret_cast<const f4 *>(p + k);
+        const f4 v = NT ? __builtin_nontemporal_load(reinterpret_cast<const f4 *>(p + k))
+                        : *reinterpret_cast<const f4 *>(p + k);
         a[k] = (double)v.x;
         a[k + 1] = (double)v.y;
         a[k + 2] = (double)v.z;
         a[k + 3] = (double)v.w;
     }
+}
+
+// Values per 16-byte chunk of the lane-interleaved block layout.
+template <typename VT>
+struct Vpl {
+    static constexpr int value = 16 / (int)sizeof(VT);
+};
+
+// Block row of lane `lane` in one slot (base = first value of the slot):
+// B values, chunk kc at base + (kc*64 + lane)*VPL.
+template <int B, typename VT, bool NT = false>
+__device__ __forceinline__ void load_lane_row(const VT *__restrict__ base, int lane, double *a) {
+    constexpr int V = Vpl<VT>::value;
+#pragma unroll
+    for (int kc = 0; kc < B / V; ++kc)
+        load_vals<V, NT>(base + ((int64_t)kc * 64 + lane) * V, a + kc * V);
 }
 
 // One Taylor term over the block part:  y = A*x (+ yx from the extra CSR),
@@ -66,7 +92,7 @@ __device__ __forceinline__ void load_vals(const float *__restrict__ p, double *a
 // (c1 = dt, c2 = 1/j!).  With tout = y and vout = null this is a plain SpMV.
 // vin may equal vout (in-place accumulation of the own row) or x (first term
 // of a substep); x is never written by this kernel.
-template <int B, typename VT, int NV>
+template <int B, typename VT, int NV, bool NT = false>
 __global__ __launch_bounds__(256) void k_term(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ val, int64_t nslice, int64_t ncell,
@@ -82,7 +108,7 @@ __global__ __launch_bounds__(256) void k_term(
     const int64_t p0 = slice_ptr[slice];
     const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
     const int32_t *cp = col + p0 + q;
-    const VT *vp = val + (p0 + q) * BB + r * B;
+    const VT *vp = val + p0 * BB;
     double acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = 0.0;
@@ -90,7 +116,7 @@ __global__ __launch_bounds__(256) void k_term(
     for (int t = 0; t < width; ++t) {
         const int64_t c = cp[(int64_t)t * CPS];
         double a[B];
-        load_vals<B>(vp + (int64_t)t * CPS * BB, a);
+        load_lane_row<B, VT, NT>(vp + (int64_t)t * CPS * BB, lane, a);
         double xv[B * NV];
         load_f64<B * NV>(x + c * (B * NV), xv);
 #pragma unroll
@@ -290,11 +316,20 @@ namespace hymor {
 // Streaming copy, 16 bytes per lane per iteration: the reference point for
 // achievable HBM bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE
 // counters (a copy of S bytes fetches S and writes S).
+// Each thread moves 4 x 16 bytes per iteration (four loads in flight
+// before the stores) so the copy is limited by HBM, not by latency.
 __global__ __launch_bounds__(256) void k_copy(const d2 *__restrict__ in, d2 *__restrict__ out,
                                               int64_t n2) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2;
-         i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = in[i];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n2; i += 4 * stride) {
+        const d2 a = in[i], b = in[i + stride], c = in[i + 2 * stride], d = in[i + 3 * stride];
+        out[i] = a;
+        out[i + stride] = b;
+        out[i + 2 * stride] = c;
+        out[i + 3 * stride] = d;
+    }
+    for (; i < n2; i += stride) out[i] = in[i];
 }
 
 }  // namespace hymor
@@ -334,9 +369,11 @@ __global__ __launch_bounds__(256) void k_spmm(
     for (int t = 0; t < width; ++t) {
         const int64_t blk = p0 + (int64_t)t * CPS + q;
         const int64_t c = col[blk];
-        const VT *ap = val + blk * BB + (g * RPL) * B;
+        const VT *sb = val + (p0 + (int64_t)t * CPS) * BB;   // slot base
         double a[RPL * B];
-        load_vals<RPL * B>(ap, a);
+#pragma unroll
+        for (int i = 0; i < RPL; ++i)
+            load_lane_row<B, VT>(sb, (int)(q * B + g * RPL + i), a + i * B);
         const double *xp = x + c * (B * R) + r;
 #pragma unroll
         for (int k = 0; k < B; ++k) {
@@ -353,6 +390,192 @@ __global__ __launch_bounds__(256) void k_spmm(
         const double t = c1 * y;
         if (tout) tout[idx] = t;
         if (vout) vout[idx] = fma(c2, t, vin[idx]);
+    }
+}
+
+}  // namespace hymor
+
+namespace hymor {
+
+// ------------------------------------------------------------ Krylov basis
+// V is column-major n x m (leading dimension ld, in elements); an element is
+// one double (real) or an interleaved (re, im) pair (CPLX).
+
+// Partial dot products h_j = sum_rows conj(V[row, j]) * w[row] for columns
+// j0..j0+JT-1 (those < m); each workgroup writes one partial per column
+// (and per component for complex): partial[(j*ncomp + c) * gridDim.x + bid].
+template <bool CPLX, int JT>
+__global__ __launch_bounds__(256) void k_dots(const double *__restrict__ V, int64_t ld, int m,
+                                              int j0, const double *__restrict__ w, int64_t n,
+                                              double *__restrict__ partial) {
+    constexpr int NC = CPLX ? 2 : 1;
+    __shared__ double red[JT * NC][4];
+    double acc[JT * NC];
+#pragma unroll
+    for (int t = 0; t < JT * NC; ++t) acc[t] = 0.0;
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n;
+         row += (int64_t)gridDim.x * blockDim.x) {
+        if constexpr (CPLX) {
+            const double wr = w[2 * row], wi = w[2 * row + 1];
+#pragma unroll
+            for (int t = 0; t < JT; ++t) {
+                if (j0 + t >= m) break;
+                const double *v = V + 2 * ((int64_t)(j0 + t) * ld + row);
+                const double vr = v[0], vi = v[1];
+                acc[2 * t] = fma(vr, wr, fma(vi, wi, acc[2 * t]));          // re(conj(v) w)
+                acc[2 * t + 1] = fma(vr, wi, fma(-vi, wr, acc[2 * t + 1]));  // im(conj(v) w)
+            }
+        } else {
+            const double wv = w[row];
+#pragma unroll
+            for (int t = 0; t < JT; ++t) {
+                if (j0 + t >= m) break;
+                acc[t] = fma(V[(int64_t)(j0 + t) * ld + row], wv, acc[t]);
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int t = 0; t < JT * NC; ++t) {
+        double a = acc[t];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+        if (lane == 0) red[t][wave] = a;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < JT * NC; t += blockDim.x) {
+        const int j = j0 + t / NC;
+        if (j < m)
+            partial[((int64_t)j * NC + t % NC) * gridDim.x + blockIdx.x] =
+                (red[t][0] + red[t][1]) + (red[t][2] + red[t][3]);
+    }
+}
+
+// w[row] -= sum_j V[row, j] * h[j]  (h on the device, m entries)
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_sub_vh(const double *__restrict__ V, int64_t ld, int m,
+                                                const double *__restrict__ h,
+                                                double *__restrict__ w, int64_t n) {
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n;
+         row += (int64_t)gridDim.x * blockDim.x) {
+        if constexpr (CPLX) {
+            double sr = 0.0, si = 0.0;
+            for (int j = 0; j < m; ++j) {
+                const double *v = V + 2 * ((int64_t)j * ld + row);
+                const double hr = h[2 * j], hi = h[2 * j + 1];
+                sr = fma(v[0], hr, fma(-v[1], hi, sr));
+                si = fma(v[0], hi, fma(v[1], hr, si));
+            }
+            w[2 * row] -= sr;
+            w[2 * row + 1] -= si;
+        } else {
+            double s = 0.0;
+            for (int j = 0; j < m; ++j) s = fma(V[(int64_t)j * ld + row], h[j], s);
+            w[row] -= s;
+        }
+    }
+}
+
+// out[:, c] = sum_j V[:, j] * Q[j, c] for c < k (Q column-major m x k on the
+// device, complex interleaved when CPLX).  out must not alias V.
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_basis_combine(const double *__restrict__ V, int64_t ld,
+                                                       int m, const double *__restrict__ Q, int k,
+                                                       double *__restrict__ out, int64_t ldo,
+                                                       int64_t n) {
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n;
+         row += (int64_t)gridDim.x * blockDim.x) {
+        for (int c = 0; c < k; ++c) {
+            if constexpr (CPLX) {
+                double sr = 0.0, si = 0.0;
+                for (int j = 0; j < m; ++j) {
+                    const double *v = V + 2 * ((int64_t)j * ld + row);
+                    const double qr = Q[2 * ((int64_t)c * m + j)], qi = Q[2 * ((int64_t)c * m + j) + 1];
+                    sr = fma(v[0], qr, fma(-v[1], qi, sr));
+                    si = fma(v[0], qi, fma(v[1], qr, si));
+                }
+                out[2 * ((int64_t)c * ldo + row)] = sr;
+                out[2 * ((int64_t)c * ldo + row) + 1] = si;
+            } else {
+                double s = 0.0;
+                for (int j = 0; j < m; ++j) s = fma(V[(int64_t)j * ld + row], Q[(int64_t)c * m + j], s);
+                out[(int64_t)c * ldo + row] = s;
+            }
+        }
+    }
+}
+
+// x[0 : ncomp*n) *= s (real scale)
+__global__ __launch_bounds__(256) void k_scale(double *__restrict__ x, int64_t len, double s) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] *= s;
+}
+
+}  // namespace hymor
+
+namespace hymor {
+
+// Split-slot form of k_term: one workgroup per slice, NW waves dividing the
+// slice's block slots round-robin (wave w takes slots w, w+NW, ...), partial
+// sums combined through LDS in wave order (deterministic), epilogue by wave
+// 0.  Shorter waves than k_term (about 1/NW of a slice each): more loads in
+// flight per wave and a smaller idle tail at the end of each launch.
+template <int B, typename VT, int NV, int NW, bool NT>
+__global__ __launch_bounds__(64 * NW) void k_term_split(
+    const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
+    const VT *__restrict__ val, int64_t nslice, int64_t ncell,
+    const double *__restrict__ x, const double *__restrict__ yx,
+    double *__restrict__ tout, const double *vin, double *vout, double c1, double c2) {
+    constexpr int CPS = 64 / B;
+    constexpr int BB = B * B;
+    __shared__ double part[NW > 1 ? NW - 1 : 1][NV][64];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int64_t slice = xcd_tile(blockIdx.x, gridDim.x);
+    if (slice >= nslice) return;
+    const int q = lane / B, r = lane % B;
+    const int64_t p0 = slice_ptr[slice];
+    const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
+    const int32_t *cp = col + p0 + q;
+    const VT *vp = val + p0 * BB;
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+#pragma unroll 2
+    for (int t = w; t < width; t += NW) {
+        const int64_t c = cp[(int64_t)t * CPS];
+        double a[B];
+        load_lane_row<B, VT, NT>(vp + (int64_t)t * CPS * BB, lane, a);
+        double xv[B * NV];
+        load_f64<B * NV>(x + c * (B * NV), xv);
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] = fma(a[k], xv[k * NV + v], acc[v]);
+    }
+    if constexpr (NW > 1) {
+        if (w > 0) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) part[w - 1][v][lane] = acc[v];
+        }
+        __syncthreads();
+        if (w > 0) return;
+#pragma unroll
+        for (int i = 0; i < NW - 1; ++i)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] += part[i][v][lane];
+    }
+    const int64_t cell = slice * CPS + q;
+    if (cell >= ncell) return;
+    const int64_t row = cell * B + r;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double y = acc[v];
+        if (yx) y += yx[row * NV + v];
+        const double t = c1 * y;
+        if (tout) tout[row * NV + v] = t;
+        if (vout) vout[row * NV + v] = fma(c2, t, vin[row * NV + v]);
     }
 }
 

@@ -1,0 +1,183 @@
+"""Krylov-Schur eigensolver over a device-resident basis.
+
+The reference calls ARPACK (Arpack.jl) on the host; every matvec copies the
+vector to the GPU and back (utils/Stability_analysis/Eigenvalues/
+ARNOLDI_POWER_GPU.jl:42-44) and all orthogonalisation runs in host BLAS.
+Here the basis V lives in HBM (``hymor_krylov``): the operator is applied
+device-to-device, classical Gram-Schmidt with reorthogonalisation (CGS2) runs
+as tall-skinny GPU kernels, restarts rotate the basis on the GPU, and the
+host only handles the (m+1) x m Hessenberg matrix -- real Schur form with
+reordering (Stewart's Krylov-Schur restart), Ritz values and residual
+estimates.
+
+``which="LM"`` (largest magnitude) is the selection all three reference
+drivers use.  Convergence: ||A y - theta y|| <= tol * |theta| for each of the
+k wanted Ritz pairs, the estimate ARPACK uses.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import scipy.linalg as sl
+
+from . import _lib
+from ._lib import C128, F64, check, ptr
+from .core import DeviceVector
+
+
+class DeviceKrylov:
+    """A device basis of m_max+1 vectors of length n (real or complex)."""
+
+    def __init__(self, n: int, m_max: int, complex_: bool = False):
+        self.n, self.m_max = n, m_max
+        self.cplx = complex_
+        self.dtype = np.complex128 if complex_ else np.float64
+        self._h = C.c_void_p()
+        check(_lib.load().hymor_krylov_create(n, m_max, C128 if complex_ else F64,
+                                              C.byref(self._h)))
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            check(_lib.load().hymor_krylov_destroy(self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def column(self, j: int) -> C.c_void_p:
+        p = C.c_void_p()
+        check(_lib.load().hymor_krylov_column(self._h, j, C.byref(p)))
+        return p
+
+    def set_column(self, j: int, x):
+        x = np.ascontiguousarray(x, dtype=self.dtype)
+        check(_lib.load().hymor_krylov_set_column(self._h, j, ptr(x)))
+
+    def get_columns(self, j0: int, ncols: int):
+        out = np.empty((ncols, self.n), dtype=self.dtype)
+        check(_lib.load().hymor_krylov_get_columns(self._h, j0, ncols, ptr(out)))
+        return out.T
+
+    def orthonormalize(self, j: int, d_w) -> np.ndarray:
+        """CGS2 of device vector d_w against columns 0..j-1, stored as column
+        j; returns the j coefficients followed by the norm."""
+        nc = 2 if self.cplx else 1
+        h = np.zeros(nc * (j + 1))
+        check(_lib.load().hymor_krylov_orthonormalize(self._h, j, d_w, h.ctypes.data_as(
+            C.POINTER(C.c_double))))
+        if self.cplx:
+            hc = h.view(np.complex128)
+            return hc
+        return h
+
+    def combine(self, Q: np.ndarray):
+        """V[:, :k] <- V[:, :m] Q for Q of shape (m, k)."""
+        m, k = Q.shape
+        Qf = np.asfortranarray(Q, dtype=self.dtype)
+        check(_lib.load().hymor_krylov_combine(self._h, m, k,
+                                               Qf.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def ritz(self, Q: np.ndarray) -> np.ndarray:
+        """V[:, :m] Q (n x k) to the host, leaving V unchanged."""
+        m, k = Q.shape
+        Qf = np.asfortranarray(Q, dtype=self.dtype)
+        out = np.empty((k, self.n), dtype=self.dtype)
+        check(_lib.load().hymor_krylov_ritz(self._h, m, k,
+                                            Qf.ctypes.data_as(C.POINTER(C.c_double)), ptr(out)))
+        return out.T
+
+    def copy_column(self, src: int, dst: int):
+        check(_lib.load().hymor_krylov_copy_column(self._h, src, dst))
+
+
+def _select_lm(evals: np.ndarray, k: int) -> float:
+    """Magnitude threshold so that the k largest-|lambda| values are kept."""
+    mags = np.sort(np.abs(evals))[::-1]
+    return mags[min(k, len(mags)) - 1]
+
+
+def krylov_schur(apply_dev, n: int, k: int, m: int, *, complex_: bool = False, tol: float = 1e-10,
+                 maxiter: int = 300, v0=None, return_vectors: bool = True):
+    """Leading-magnitude eigenpairs of a linear operator given as
+    ``apply_dev(d_in, d_out)`` (device pointers, synchronous).
+
+    Returns (theta, Y, info): theta the k Ritz values sorted by decreasing
+    magnitude, Y the Ritz vectors (n x k, or None), info a dict with
+    iterations, operator applications and residual estimates.  Raises
+    RuntimeError if not converged within maxiter restarts (the callers turn
+    that into NaN results, as the reference does).
+    """
+    if not 1 <= k < m:
+        raise ValueError("need 1 <= k < m")
+    if m >= n:
+        raise ValueError("Krylov dimension must be smaller than n")
+    dtype = np.complex128 if complex_ else np.float64
+    K = DeviceKrylov(n, m, complex_)
+    scratch = DeviceVector(n, dtype)
+    try:
+        if v0 is None:
+            rng = np.random.default_rng(0)
+            v0 = rng.standard_normal(n) + (1j * rng.standard_normal(n) if complex_ else 0)
+        K.set_column(0, np.asarray(v0, dtype=dtype))
+        K.orthonormalize(0, K.column(0))                    # normalise in place
+        H = np.zeros((m + 1, m), dtype=dtype)
+        p = 0
+        n_apply = 0
+        for it in range(1, maxiter + 1):
+            for j in range(p, m):
+                apply_dev(K.column(j), scratch.ptr)
+                n_apply += 1
+                h = K.orthonormalize(j + 1, scratch.ptr)
+                H[: j + 2, j] = h[: j + 2]
+            beta = H[m, m - 1]
+            Hm = H[:m, :m]
+            evals = np.linalg.eigvals(Hm)
+            thr = _select_lm(evals, k)
+            if complex_:
+                T, Z, sdim = sl.schur(Hm, output="complex", sort=lambda z: abs(z) >= thr * (1 - 1e-12))
+            else:
+                T, Z, sdim = sl.schur(Hm, output="real",
+                                      sort=lambda re, im: abs(complex(re, im)) >= thr * (1 - 1e-12))
+            p_keep = max(sdim, k)
+            # Ritz pairs of the leading block and their residual estimates
+            w, Yt = sl.eig(T[:p_keep, :p_keep])
+            order = np.argsort(-np.abs(w), kind="stable")[:k]
+            w = w[order]
+            Yt = Yt[:, order]
+            zlast = Z[m - 1, :p_keep] @ Yt                     # last row of Z*y
+            res = np.abs(beta) * np.abs(zlast)
+            conv = res <= tol * np.maximum(np.abs(w), np.finfo(float).tiny)
+            if np.all(conv):
+                info = {"iterations": it, "applications": n_apply, "residual_estimates": res}
+                Y = None
+                if return_vectors:
+                    Q = Z[:, :p_keep] @ Yt                     # m x k
+                    if complex_:
+                        Y = K.ritz(Q)
+                    else:                                      # real basis, complex Ritz vectors
+                        Y = K.ritz(np.ascontiguousarray(Q.real)) + 1j * K.ritz(
+                            np.ascontiguousarray(Q.imag))
+                    Y = Y / np.linalg.norm(Y, axis=0)
+                return w, Y, info
+            # Krylov-Schur restart: keep the p_keep leading Schur vectors
+            p = min(p_keep + (k - int(conv.sum())) // 2, m - 2)
+            p = max(p, p_keep)
+            if not complex_ and p < m and p > p_keep:
+                # do not split a 2x2 block of the real Schur form
+                if abs(T[p, p - 1]) > 0:
+                    p += 1
+            p = min(p, m - 1)
+            K.combine(Z[:, :p])
+            K.copy_column(m, p)
+            Hn = np.zeros((m + 1, m), dtype=dtype)
+            Hn[:p, :p] = T[:p, :p]
+            Hn[p, :p] = beta * Z[m - 1, :p]
+            H = Hn
+        raise RuntimeError(f"Krylov-Schur did not converge in {maxiter} restarts")
+    finally:
+        scratch.free()
+        K.close()

@@ -23,7 +23,9 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from . import _lib
-from .core import DeviceOperator, DeviceWeights, energy_track
+from .core import (DeviceOperator, DeviceVector, DeviceWeights, chu_sandwich_ptr, energy_track,
+                   expm_taylor_ptr, spmv_dev)
+from .krylov import krylov_schur
 
 ORDER = 5            # Taylor order hard-coded by the reference
 B = 4                # conserved variables per cell in the reference
@@ -348,13 +350,17 @@ def EIGENVALUES(A, s, n_modes):
 
 
 def EIGENVALUES_GPU_ARNOLDI(A, s, n_modes, *, T_ref=0.5, tol=1e-4, maxiter=None, ncv=None,
-                            v0=None):
+                            v0=None, krylov="arpack"):
     """Arnoldi on the exponential transform (EIGENVALUES.jl:112-205).
 
     dt from CFL_TIMESTEP, timesteps = round(T_ref/dt), A*dt uploaded once,
     ARPACK which=LM, tol=1e-4, maxiter=40k; discrete eigenvalues mapped by
     lambda = log(mu)/(dt*timesteps).  On any failure returns NaN V, D with a
     warning, as the reference does.
+
+    krylov="device" replaces ARPACK by the Krylov-Schur solver over a
+    device-resident basis (hymor_amd.krylov): same selection (largest
+    magnitude) and convergence test, no host round trip per matvec.
     """
     s = CFL_TIMESTEP(s)
     dt = s["time_integration"]["dt"]
@@ -365,10 +371,17 @@ def EIGENVALUES_GPU_ARNOLDI(A, s, n_modes, *, T_ref=0.5, tol=1e-4, maxiter=None,
     dev = None
     try:
         dev = upload(A, s, scale=dt)
-        op = spla.LinearOperator((n, n), dtype=np.float64,
-                                 matvec=lambda x: ARNOLDI_POWER_GPU(x, dev, timesteps))
-        vals, vecs = spla.eigs(op, k=k, which="LM", tol=tol, maxiter=maxiter or 40 * k, ncv=ncv,
-                               v0=v0)
+        if krylov == "device":
+            def apply(din, dout):
+                expm_taylor_ptr(dev, din, dout, order=ORDER, nsub=timesteps, complex_=False)
+                dev.synchronize()
+            vals, vecs, _ = krylov_schur(apply, n, k, ncv or max(20, 2 * k + 1), tol=tol,
+                                         maxiter=maxiter or 40 * k, v0=v0)
+        else:
+            op = spla.LinearOperator((n, n), dtype=np.float64,
+                                     matvec=lambda x: ARNOLDI_POWER_GPU(x, dev, timesteps))
+            vals, vecs = spla.eigs(op, k=k, which="LM", tol=tol, maxiter=maxiter or 40 * k,
+                                   ncv=ncv, v0=v0)
         vals, vecs = _sort_lm(vals, vecs)
         D = np.log(vals.astype(np.complex128)) / dt / timesteps          # :164-168
         return vecs, D
@@ -381,7 +394,8 @@ def EIGENVALUES_GPU_ARNOLDI(A, s, n_modes, *, T_ref=0.5, tol=1e-4, maxiter=None,
         s.setdefault("timings", {})["eigenvalues"] = time.time() - t0
 
 
-def TRANSIENT_GROWTH_DOWNSTREAM(A, s, n_modes, T, *, tol=1e-4, maxiter=None, ncv=None, v0=None):
+def TRANSIENT_GROWTH_DOWNSTREAM(A, s, n_modes, T, *, tol=1e-4, maxiter=None, ncv=None, v0=None,
+                                krylov="arpack"):
     """Optimal transient growth (TRANSIENT_GROWTH_DOWNSTREAM.jl:34-153).
 
     Returns (V, D, T_opt): D are the gains, eigenvalues of
@@ -402,10 +416,16 @@ def TRANSIENT_GROWTH_DOWNSTREAM(A, s, n_modes, T, *, tol=1e-4, maxiter=None, ncv
     try:
         devA = upload(A, s, scale=dt)
         devC = upload(C_no_exp, s)
-        op = spla.LinearOperator((n, n), dtype=np.float64,
-                                 matvec=lambda x: CONSTRUCT_C(x, devA, None, devC, n_t, False, 0))
-        vals, vecs = spla.eigsh(op, k=k, M=D_mat, which="LM", tol=tol, maxiter=maxiter or 10 * k,
-                                ncv=ncv, v0=v0)
+        if krylov == "device":
+            vals, vecs = _device_generalized(devA, devC, D_mat, s, n, k, n_t, 0, False, tol,
+                                             maxiter or 10 * k, ncv, v0)
+            vals = np.real(vals)
+        else:
+            op = spla.LinearOperator((n, n), dtype=np.float64,
+                                     matvec=lambda x: CONSTRUCT_C(x, devA, None, devC, n_t, False,
+                                                                  0))
+            vals, vecs = spla.eigsh(op, k=k, M=D_mat, which="LM", tol=tol,
+                                    maxiter=maxiter or 10 * k, ncv=ncv, v0=v0)
         order = np.argsort(-np.abs(vals), kind="stable")
         return vecs[:, order], vals[order], T_opt
     except Exception as e:
@@ -418,7 +438,7 @@ def TRANSIENT_GROWTH_DOWNSTREAM(A, s, n_modes, T, *, tol=1e-4, maxiter=None, ncv
 
 
 def FREESTREAM_RECEPTIVITY(A_, s, n_modes, T, w_infty, *, tol=1e-4, maxiter=None, ncv=None,
-                           v0=None):
+                           v0=None, krylov="arpack"):
     """Freestream receptivity (FREESTREAM_RECEPTIVITY.jl:37-162).
 
     A_ is the complex extended operator [L B..B; 0 kron(diag(-i w), I)].
@@ -441,11 +461,15 @@ def FREESTREAM_RECEPTIVITY(A_, s, n_modes, T, w_infty, *, tol=1e-4, maxiter=None
     try:
         devA = upload(sp.csc_matrix(A_, dtype=np.complex128), s, scale=dt)
         devC = upload(C_, s)
-        op = spla.LinearOperator((n, n), dtype=np.complex128,
-                                 matvec=lambda x: CONSTRUCT_C(x, devA, None, devC, n_t, True,
-                                                              n_set_0))
-        vals, vecs = spla.eigsh(op, k=k, M=D_.astype(np.complex128), which="LM", tol=tol,
-                                maxiter=maxiter or 10 * k, ncv=ncv, v0=v0)
+        if krylov == "device":
+            vals, vecs = _device_generalized(devA, devC, D_, s, n, k, n_t, n_set_0, True, tol,
+                                             maxiter or 10 * k, ncv, v0)
+        else:
+            op = spla.LinearOperator((n, n), dtype=np.complex128,
+                                     matvec=lambda x: CONSTRUCT_C(x, devA, None, devC, n_t, True,
+                                                                  n_set_0))
+            vals, vecs = spla.eigsh(op, k=k, M=D_.astype(np.complex128), which="LM", tol=tol,
+                                    maxiter=maxiter or 10 * k, ncv=ncv, v0=v0)
         D = np.real(vals) / 2                                            # :115-119
         order = np.argsort(-np.abs(D), kind="stable")
         return vecs[:, order], D[order], T_opt
@@ -505,3 +529,73 @@ def _energy_run(x, n_t, dt, t_0, order, Adt, mats, weights, n_skip, n_samples):
     E[:, 0] = 0.0                     # the reference leaves entry 1 to its caller
     t = t_0 + dt * np.arange(int(n_t) + 1)
     return t, [E[k] for k in range(E.shape[0])], samp, xf
+
+
+# ====================================================================== device generalized solver
+
+def block_cholesky_inverse(D):
+    """For a symmetric positive definite D that is block diagonal up to a
+    permutation (the Chu matrices R'M_hat R and R_'M_infty_R_: 4x4 per cell
+    or freestream station, 1x1 per shock DOF), return (Linv, LinvH) with
+    D = L L^H, as sparse matrices with the same block structure."""
+    from scipy.sparse.csgraph import connected_components
+    D = sp.csr_matrix(D)
+    n = D.shape[0]
+    ncomp, labels = connected_components(D, directed=False)
+    order = np.argsort(labels, kind="stable")
+    counts = np.bincount(labels, minlength=ncomp)
+    starts = np.zeros(ncomp + 1, dtype=np.int64)
+    np.cumsum(counts, out=starts[1:])
+    pos = np.empty(n, dtype=np.int64)
+    pos[order] = np.arange(n) - starts[labels[order]]
+    coo = D.tocoo()
+    rows, cols, vals = [], [], []
+    for size in np.unique(counts):
+        comps = np.nonzero(counts == size)[0]
+        slot = np.full(ncomp, -1, dtype=np.int64)
+        slot[comps] = np.arange(comps.shape[0])
+        blk = np.zeros((comps.shape[0], size, size), dtype=D.dtype)
+        sel = slot[labels[coo.row]] >= 0
+        blk[slot[labels[coo.row[sel]]], pos[coo.row[sel]], pos[coo.col[sel]]] = coo.data[sel]
+        Li = np.linalg.inv(np.linalg.cholesky(blk))
+        members = order[starts[comps][:, None] + np.arange(size)[None, :]]   # (count, size)
+        rows.append(np.repeat(members, size, axis=1).reshape(-1))
+        cols.append(np.tile(members, (1, size)).reshape(-1))
+        vals.append(Li.reshape(-1))
+    Linv = sp.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(n, n))
+    return Linv, Linv.conj().T.tocsc()
+
+
+def _device_generalized(devA, devC, D, s, n, k, n_t, n_set0, cplx, tol, maxiter, ncv, v0):
+    """Largest eigenpairs of T(A')^n C T(A)^n x = lambda D x (with the
+    freestream zeroing) through the exact block-Cholesky reduction
+    L^-1 (.) L^-H and Krylov-Schur on a device basis.  Returns eigenvectors
+    x = L^-H y, D-orthonormal as ARPACK's generalized mode returns them."""
+    Linv, LinvH = block_cholesky_inverse(D)
+    dLi = upload(Linv, s)
+    dLiH = upload(LinvH, s)
+    dtype = np.complex128 if cplx else np.float64
+    t1, t2 = DeviceVector(n, dtype), DeviceVector(n, dtype)
+    try:
+        def apply(din, dout):
+            spmv_dev(dLiH, din, t1.ptr, complex_=cplx)
+            dLiH.synchronize()
+            chu_sandwich_ptr(devA, devC, t1.ptr, t2.ptr, order=ORDER, nsub=n_t, n_set0=n_set0,
+                             complex_=cplx)
+            devA.synchronize()
+            spmv_dev(dLi, t2.ptr, dout, complex_=cplx)
+            dLi.synchronize()
+        if v0 is not None:
+            v0 = np.asarray(v0, dtype=dtype)
+        vals, Y, _ = krylov_schur(apply, n, k, ncv or max(20, 2 * k + 1), complex_=cplx, tol=tol,
+                                  maxiter=maxiter, v0=v0)
+        X = LinvH @ Y
+        if not cplx:
+            X = np.real(X)
+        return vals, X
+    finally:
+        t1.free()
+        t2.free()
+        dLi.close()
+        dLiH.close()

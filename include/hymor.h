@@ -105,6 +105,9 @@ int hymor_op_info(const hymor_op *op, int64_t *out, int nout);
  * mul!(y, A, x, 1.0, 0.0) on CuSparseMatrixCSC (ARNOLDI_POWER_GPU.jl:73). */
 int hymor_spmv(hymor_op *op, int transpose, const void *x, void *y, int dtype);
 
+/* Device-pointer form of hymor_spmv (asynchronous). */
+int hymor_spmv_dev(hymor_op *op, int transpose, const void *d_x, void *d_y, int dtype);
+
 /* y = T_order(A*dt)^nsub * x, where T_p is the degree-p Taylor polynomial
  * of exp.  order=5, form=HYMOR_FORM_POWERS is POWER_METHOD!
  * (ARNOLDI_POWER_GPU.jl:64-82) behind ARNOLDI_POWER_GPU (:36-52);
@@ -158,6 +161,17 @@ int hymor_memcpy_d2h(void *h_dst, const void *d_src, int64_t bytes);
 int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int reps,
                             double *ms_per_launch, double *bytes_per_launch);
 
+/* Autotuner, no reference counterpart: times the launch variants of the
+ * single-vector Taylor-term kernel (one wave per slice with 2 or 4 waves per
+ * group, or split-slot with 2 or 4 waves per slice; plain or non-temporal
+ * block loads) on this operator, `reps` launches each, and keeps the
+ * fastest for both directions.  nv: 1, 2 or 4 doubles per row. */
+int hymor_op_autotune(hymor_op *op, int nv, int reps, double *best_ms);
+
+/* Read / force the chosen variant (split 0|1, waves, nt 0|1). */
+int hymor_op_get_tuning(const hymor_op *op, int *split, int *waves, int *nt);
+int hymor_op_set_tuning(hymor_op *op, int split, int waves, int nt);
+
 /* Calibration probe, no reference counterpart: a streaming copy of `bytes`
  * (multiple of 16) between two fresh device buffers, launched `reps` times;
  * returns ms per launch.  Used to measure achievable HBM bandwidth and to
@@ -196,6 +210,44 @@ int hymor_weights_destroy(hymor_weights *W);
 int hymor_energy_track(hymor_op *A, const hymor_weights *W, int order, int64_t n_t, double dt,
                        int form, const void *x, int dtype, double *E, void *x_final,
                        int64_t n_skip, int64_t n_samples, void *samples);
+
+/* ------------------------------------------------------------------------
+ * Device-resident Krylov basis.  Replaces the host-side basis ARPACK keeps
+ * (Arpack.eigs at EIGENVALUES.jl:159, TRANSIENT_GROWTH_DOWNSTREAM.jl:214,
+ * FREESTREAM_RECEPTIVITY.jl:107), whose every matvec crosses PCIe twice
+ * (ARNOLDI_POWER_GPU.jl:42,44).  The basis V (n x (m_max+1), column-major,
+ * var-major vectors) stays in HBM; the host keeps only the small Hessenberg
+ * or tridiagonal matrix.  Columns are device pointers usable with the *_dev
+ * actions.  All calls are synchronous.
+ * ------------------------------------------------------------------------ */
+typedef struct hymor_krylov hymor_krylov;
+
+int hymor_krylov_create(int64_t n, int m_max, int dtype, hymor_krylov **out);
+int hymor_krylov_destroy(hymor_krylov *K);
+
+/* Device address of column j (0 <= j <= m_max). */
+int hymor_krylov_column(hymor_krylov *K, int j, void **d_col);
+
+/* Upload / download columns (host vectors, var-major). */
+int hymor_krylov_set_column(hymor_krylov *K, int j, const void *x);
+int hymor_krylov_get_columns(hymor_krylov *K, int j0, int ncols, void *out);
+
+/* Classical Gram-Schmidt with one reorthogonalisation (CGS2): w (device,
+ * length n) is orthogonalised against columns 0..j-1, normalised and stored
+ * as column j.  h_out receives the j projection coefficients (complex:
+ * interleaved) followed by the norm, i.e. column j-1 of the Hessenberg
+ * matrix when w = Op(V[:, j-1]). */
+int hymor_krylov_orthonormalize(hymor_krylov *K, int j, void *d_w, double *h_out);
+
+/* Restart: V[:, 0:k] <- V[:, 0:m] * Q with Q column-major m x k (host,
+ * complex interleaved for complex bases). */
+int hymor_krylov_combine(hymor_krylov *K, int m, int k, const double *Q);
+
+/* Ritz vectors: out (host, n x k column-major) = V[:, 0:m] * Q; V is left
+ * unchanged. */
+int hymor_krylov_ritz(hymor_krylov *K, int m, int k, const double *Q, void *out);
+
+int hymor_krylov_copy_column(hymor_krylov *K, int src, int dst);
 
 #ifdef __cplusplus
 }

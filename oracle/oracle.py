@@ -288,3 +288,34 @@ def circulant_eigenvalues(blocks, nchi, neta):
     S = circulant_symbol(blocks, nchi, neta)
     b = S.shape[-1]
     return np.linalg.eigvals(S.reshape(-1, b, b)).reshape(-1)
+
+
+# --------------------------------------------------------------------------
+# Second statement of the Chu energy (pins the package's R'MR constructors)
+# --------------------------------------------------------------------------
+
+def chu_components(rho0, rho_u0, rho_v0, p0, a0, gam0, prho, prho_u, prho_v, prho_E):
+    """GET_CHU_COMPONENTS (utils/Energy_budgets/GET_CHU_COMPONENTS.jl:1-80):
+    conservative perturbations -> (p', u', v', s'/R), elementwise."""
+    u0 = rho_u0 / rho0
+    v0 = rho_v0 / rho0
+    pu = (prho_u - u0 * prho) / rho0
+    pv = (prho_v - v0 * prho) / rho0
+    t1 = -u0 * prho_u
+    t2 = -v0 * prho_v
+    t3 = (u0 ** 2 + v0 ** 2) * prho / 2
+    pp = (gam0 - 1) * (prho_E + t1 + t2 + t3)
+    ps = pp / p0 / (gam0 - 1) - prho / rho0 * gam0 / (gam0 - 1)
+    return pp, pu, pv, ps
+
+
+def energy_norm(rho0, rho_u0, rho_v0, p0, a0, gam0, prho, prho_u, prho_v, prho_E):
+    """GET_ENERGY_NORM (utils/Energy_budgets/GET_ENERGY_NORM.jl:1-55):
+    acoustic, kinetic and entropic Chu energy densities (real parts)."""
+    pp, pu, pv, ps = chu_components(rho0, rho_u0, rho_v0, p0, a0, gam0, prho, prho_u, prho_v,
+                                    prho_E)
+    return {
+        "acoustic": rho0 * a0 ** 2 / (2 * (gam0 * p0) ** 2) * np.real(pp) ** 2,
+        "kinetic": rho0 / 2 * np.real(pu) ** 2 + rho0 / 2 * np.real(pv) ** 2,
+        "entropic": (gam0 - 1) * p0 / (2 * gam0) * np.real(ps) ** 2,
+    }

@@ -39,6 +39,7 @@ extern "C" int emu_spmv_csc(int64_t n, const int64_t *colptr, const int64_t *row
     const HostExtra &X = transpose ? xa : xf;
     HostSliced s;
     bsr_to_sliced(A, s);
+    interleave_lanes(s, 2);   // FP64 storage: 2 values per 16-byte chunk
     const int NV = is_complex ? 2 : 1;
     const int64_t nflow = (int64_t)b * ncell;
     // perm_in
@@ -85,8 +86,9 @@ extern "C" int emu_spmv_csc(int64_t n, const int64_t *colptr, const int64_t *row
                 const int64_t blk = p0 + q + t * CPS;
                 const int64_t c = at(s.col, blk);
                 if (c < 0 || c >= ncell) ++g_oob;
+                const int64_t sb = (p0 + t * CPS) * BB;   // slot base
                 for (int k = 0; k < b; ++k) {
-                    const double a = at(s.val, blk * BB + r * b + k);
+                    const double a = at(s.val, sb + ((k / 2) * 64 + lane) * 2 + k % 2);
                     for (int v = 0; v < NV; ++v) acc[v] += a * at(x, (c * b + k) * NV + v);
                 }
             }
@@ -117,6 +119,7 @@ extern "C" int emu_spmm_csc(int64_t n, const int64_t *colptr, const int64_t *row
     if (!xf.rows.empty()) return -1;
     HostSliced s;
     bsr_to_sliced(fwd, s);
+    interleave_lanes(s, 2);
     const int64_t nflow = (int64_t)b * ncell;
     std::vector<double> x((size_t)n * R), y((size_t)n * R, 0.0);
     // k_perm_in with width R, ncomp 1
@@ -137,10 +140,13 @@ extern "C" int emu_spmm_csc(int64_t n, const int64_t *colptr, const int64_t *row
             for (int64_t t = 0; t < width; ++t) {
                 const int64_t blk = p0 + t * CPS + q;
                 const int64_t c = at(s.col, blk);
+                const int64_t sb = (p0 + t * CPS) * BB;
                 for (int k = 0; k < b; ++k) {
                     const double xk = at(x, c * (b * R) + r + k * R);
-                    for (int i = 0; i < RPL; ++i)
-                        acc[i] += at(s.val, blk * BB + (g * RPL) * b + i * b + k) * xk;
+                    for (int i = 0; i < RPL; ++i) {
+                        const int64_t ln = q * b + g * RPL + i;
+                        acc[i] += at(s.val, sb + ((k / 2) * 64 + ln) * 2 + k % 2) * xk;
+                    }
                 }
             }
             for (int i = 0; i < RPL; ++i) {

@@ -285,3 +285,40 @@ def test_forward_after_adjoint_with_extras(orc):
             assert relerr(got, orc.power_method(A.scaled(0.1), x, 3)) <= 1e-13
             got = dev.expm_taylor(x, order=5, nsub=3, dt=0.1, transpose=True)
             assert relerr(got, orc.power_method(At.scaled(0.1), x, 3)) <= 1e-13
+
+
+@pytest.mark.parametrize("split,waves,nt", [(0, 1, 0), (0, 2, 1), (0, 4, 0), (0, 4, 1),
+                                            (1, 2, 0), (1, 2, 1), (1, 4, 0), (1, 4, 1)])
+def test_launch_variants_agree(orc, split, waves, nt):
+    """Every Taylor-term launch variant the autotuner may pick matches the
+    oracle (real with shock extras, and complex)."""
+    say(f"[variants] split={split} waves={waves} nt={nt}")
+    op = make_case("shock12x10_b4")
+    A = orc.CSC.from_arrays(*op.to_csc_varmajor())
+    x = np.random.default_rng(14).standard_normal(op.n)
+    with DeviceOperator.from_blocks(op) as dev:
+        dev.set_tuning(split, waves, nt)
+        assert dev.tuning == (split, waves, nt)
+        got = dev.expm_taylor(x, order=5, nsub=4, dt=0.1)
+        assert relerr(got, orc.power_method(A.scaled(0.1), x, 4)) <= 1e-12
+        got = dev.expm_taylor(x, order=5, nsub=4, dt=0.1, transpose=True)
+        assert relerr(got, orc.power_method(orc.csc_adjoint(A).scaled(0.1), x, 4)) <= 1e-12
+    op8 = make_case("grid10x8_b8")
+    A8 = orc.CSC.from_arrays(*op8.to_csc_varmajor())
+    xc = np.random.default_rng(15).standard_normal(op8.n) * (1 + 1j)
+    with DeviceOperator.from_blocks(op8) as dev:
+        dev.set_tuning(split, waves, nt)
+        got = dev.expm_taylor(xc, order=5, nsub=3, dt=0.1)
+        assert relerr(got, orc.power_method(A8.scaled(0.1), xc, 3)) <= 1e-12
+
+
+def test_autotune_picks_a_valid_variant(orc):
+    op = ops.config_operator("C1")
+    say("[autotune] C1")
+    with DeviceOperator.from_blocks(op) as dev:
+        ms, tun = dev.autotune(reps=3)
+        assert ms > 0 and tun[0] in (0, 1)
+        x = np.random.default_rng(16).standard_normal(op.n)
+        A = orc.CSC.from_arrays(*op.to_csc_varmajor())
+        got = dev.expm_taylor(x, order=12, nsub=4, dt=1 / 64)
+        assert relerr(got, orc.power_method(A.scaled(1 / 64), x, 4, order=12)) <= 1e-12

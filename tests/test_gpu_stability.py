@@ -197,3 +197,73 @@ def test_freestream_receptivity_vs_dense():
     D_ = R_.T @ st.CONSTRUCT_M_INFTY_(s, [1, 1, 1, 1], T, w, True) @ R_
     ref = _dense_gains(A, C_, D_, dt, n_t, n0=4 * nchi * neta, k=3) / 2
     assert np.all(np.abs(G - ref) <= 1e-8 * np.abs(ref)), (G, ref)
+
+
+# ------------------------------------------------------------------ device Krylov
+
+def test_eigenvalues_device_krylov_matches_closed_form(orc):
+    nchi, neta, b = 12, 9, 4
+    say("[EIGENVALUES krylov=device] circulant operator")
+    op, blocks = ops.circulant_operator(nchi, neta, b, seed=5, row_sum=3.0)
+    A = op.to_scipy_varmajor()
+    dt = 0.05
+    s = {"mesh": {"Nchi": nchi, "Neta": neta}, "time_integration": {"dt": dt},
+         "stability_analysis": {"perturb_shock": False,
+                                "eigenvalue_solver": "GPU_TIMESTEPPER_ARNOLDI"}}
+    k = 8
+    V, D = st.EIGENVALUES_GPU_ARNOLDI(A, s, k, tol=1e-12, ncv=40, krylov="device",
+                                      v0=np.ones(A.shape[0]))
+    assert not np.any(np.isnan(D))
+    n = int(round(0.5 / dt))
+    lam = orc.circulant_eigenvalues(blocks, nchi, neta)
+    mu = sum((lam * dt) ** j / math.factorial(j) for j in range(6)) ** n
+    order = np.argsort(-np.abs(mu), kind="stable")
+    exact = np.log(mu[order]) / (n * dt)
+    for d in D:
+        j = np.argmin(np.abs(exact[:k + 6] - d))
+        assert abs(exact[j] - d) <= 1e-8 * abs(exact[j]), (d, exact[j])
+    # same values as the ARPACK path
+    _, D2 = st.EIGENVALUES_GPU_ARNOLDI(A, s, k, tol=1e-12, ncv=40, v0=np.ones(A.shape[0]))
+    for d in D:
+        assert np.min(np.abs(D2 - d)) <= 1e-8 * abs(d)
+    dev = DeviceOperator.from_scipy(A, nchi=nchi, neta=neta, scale=dt)
+    for i in range(k):
+        v = V[:, i]
+        r = dev.expm_taylor(v, order=5, nsub=n) - np.exp(D[i] * n * dt) * v
+        assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(v)
+    dev.close()
+
+
+def test_transient_growth_device_krylov_vs_dense():
+    nchi, neta = 8, 6
+    say("[TRANSIENT_GROWTH_DOWNSTREAM krylov=device] vs dense")
+    s = st.CFL_TIMESTEP(synthetic_solution(nchi, neta, seed=3, perturb_shock=True))
+    op = ops.stencil_operator(nchi, neta, 4, seed=7, n_shock=nchi,
+                              row_sum=0.5 / s["time_integration"]["dt"])
+    A = op.to_scipy_varmajor()
+    V, G, T_opt = st.TRANSIENT_GROWTH_DOWNSTREAM(A, s, 4, 1.0, tol=1e-12, ncv=30, krylov="device")
+    n_t = int(round(1.0 / s["time_integration"]["dt"]))
+    R = st.CONSTRUCT_R(s)
+    C = R.T @ st.CONSTRUCT_M(s, [1, 1, 1, 1]) @ R
+    D = R.T @ st.CONSTRUCT_M_HAT(s, [1, 1, 1, 1]) @ R
+    ref = _dense_gains(A, C, D, 1.0 / n_t, n_t)
+    assert np.all(np.abs(G - ref) <= 1e-8 * np.abs(ref)), (G, ref)
+    # D-orthonormal eigenvectors satisfying the generalized problem
+    assert np.allclose(V.T @ (D @ V), np.eye(4), atol=1e-8)
+
+
+def test_freestream_receptivity_device_krylov_vs_dense():
+    nchi, neta = 6, 5
+    w = [0.0, 2.0]
+    say("[FREESTREAM_RECEPTIVITY krylov=device] vs dense")
+    s = st.CFL_TIMESTEP(synthetic_solution(nchi, neta, seed=4, perturb_shock=True))
+    base = ops.stencil_operator(nchi, neta, 4, seed=8, n_shock=nchi,
+                                row_sum=0.5 / s["time_integration"]["dt"])
+    A = ops.receptivity_extension(base, w).to_scipy_varmajor()
+    V, G, T_opt = st.FREESTREAM_RECEPTIVITY(A, s, 3, 1.0, w, tol=1e-12, ncv=24, krylov="device")
+    n_t = int(round(1.0 / s["time_integration"]["dt"]))
+    R_ = st.CONSTRUCT_R_(s, w)
+    C_ = R_.T @ st.CONSTRUCT_M_(s, [1, 1, 1, 1], w) @ R_
+    D_ = R_.T @ st.CONSTRUCT_M_INFTY_(s, [1, 1, 1, 1], 1.0, w, True) @ R_
+    ref = _dense_gains(A, C_, D_, 1.0 / n_t, n_t, n0=4 * nchi * neta, k=3) / 2
+    assert np.all(np.abs(G - ref) <= 1e-8 * np.abs(ref)), (G, ref)
