@@ -159,6 +159,10 @@ class DeviceOperator:
         check(_lib.load().hymor_op_get_tuning(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
 
+    def set_batch_kernel(self, mfma: bool):
+        """Matrix-core (True, default) or vector-FMA batch kernel."""
+        check(_lib.load().hymor_op_set_batch_kernel(self._h, int(bool(mfma))))
+
     def set_tuning(self, split, waves, nt):
         check(_lib.load().hymor_op_set_tuning(self._h, int(split), int(waves), int(nt)))
 

@@ -49,6 +49,7 @@ struct TermTuning {
     int split = 0;   // 0: k_term (one wave per slice), 1: k_term_split
     int waves = 4;   // k_term: waves per workgroup; k_term_split: waves per slice
     int nt = 0;      // non-temporal loads of the block values
+    int spmm_mfma = 1;  // batches of >= 16 doubles per row: matrix-core kernel
 };
 
 struct DevSliced {
@@ -271,6 +272,14 @@ void launch_spmm_t(const DevSliced &s, hipStream_t st, const double *x, const do
                    double *tout, const double *vin, double *vout, double c1, double c2) {
     const int wpb = 4;
     const int64_t grid = (s.ncell + wpb - 1) / wpb;
+    if constexpr (R % 16 == 0) {
+        if (s.tune.spmm_mfma) {
+            hipLaunchKernelGGL((k_spmm_mfma<B, VT, R>), dim3((unsigned)grid), dim3(64 * wpb), 0,
+                               st, s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, yx, tout,
+                               vin, vout, c1, c2);
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_spmm<B, VT, R>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
                        s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, yx, tout, vin, vout, c1,
                        c2);
@@ -812,8 +821,8 @@ int hymor_op_autotune(hymor_op *op, int nv, int reps, double *best_ms) {
         ARG_CHECK(!(op->is_complex && nv == 1), "complex operator needs nv >= 2");
         ARG_CHECK(reps >= 1, "reps must be >= 1");
         ensure_ws(op, nv);
-        const TermTuning cands[] = {{0, 4, 0}, {0, 4, 1}, {0, 2, 0}, {1, 2, 0},
-                                    {1, 2, 1}, {1, 4, 0}, {1, 4, 1}};
+        const TermTuning cands[] = {{0, 4, 0, 1}, {0, 4, 1, 1}, {0, 2, 0, 1}, {1, 2, 0, 1},
+                                    {1, 2, 1, 1}, {1, 4, 0, 1}, {1, 4, 1, 1}};
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
@@ -844,6 +853,14 @@ int hymor_op_autotune(hymor_op *op, int nv, int reps, double *best_ms) {
     });
 }
 
+int hymor_op_set_batch_kernel(hymor_op *op, int use_mfma) {
+    return guarded([&] {
+        ARG_CHECK(op, "null operator");
+        op->fwd.tune.spmm_mfma = use_mfma ? 1 : 0;
+        op->adj.tune.spmm_mfma = op->fwd.tune.spmm_mfma;
+    });
+}
+
 int hymor_op_get_tuning(const hymor_op *op, int *split, int *waves, int *nt) {
     return guarded([&] {
         ARG_CHECK(op && split && waves && nt, "null argument");
@@ -860,7 +877,9 @@ int hymor_op_set_tuning(hymor_op *op, int split, int waves, int nt) {
                       (split == 1 && (waves == 2 || waves == 4)),
                   "k_term takes 1, 2 or 4 waves per group; k_term_split 2 or 4 waves per slice");
         ARG_CHECK(nt == 0 || nt == 1, "nt must be 0 or 1");
-        op->fwd.tune = TermTuning{split, waves, nt};
+        op->fwd.tune.split = split;
+        op->fwd.tune.waves = waves;
+        op->fwd.tune.nt = nt;
         op->adj.tune = op->fwd.tune;
     });
 }

@@ -580,3 +580,69 @@ __global__ __launch_bounds__(64 * NW) void k_term_split(
 }
 
 }  // namespace hymor
+
+namespace hymor {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// Batched Taylor term on the FP64 matrix cores.  One wavefront per cell;
+// per block slot and k-chunk of 4 the block product Y(b x 16 rhs) +=
+// A_blk(b x 4) * X(4 x 16 rhs) is one v_mfma_f64_16x16x4f64 per 16-rhs tile
+// (rows >= b are zero padding).  Operand layout of that instruction on
+// gfx950 (measured): A[m][k] from lane m + 16k, B[k][n] from lane n + 16k,
+// D[m][n] in lane n + 16*(m % 4), register m / 4.  So lane (m = lane % 16,
+// kk = lane / 16) supplies block value A[m][4kc+kk] and x value
+// X[4kc+kk][16nt + m]: every lane loads one value per operand instead of a
+// whole block row, and the x loads are 4 x 128-byte lines per instruction.
+template <int B, typename VT, int R>
+__global__ __launch_bounds__(256) void k_spmm_mfma(
+    const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
+    const VT *__restrict__ val, int64_t ncell, const double *__restrict__ x,
+    const double *__restrict__ yx, double *__restrict__ tout, const double *vin, double *vout,
+    double c1, double c2) {
+    constexpr int CPS = 64 / B;
+    constexpr int BB = B * B;
+    constexpr int NT = R / 16;
+    constexpr int KC = B / 4;
+    constexpr int V = Vpl<VT>::value;
+    static_assert(R % 16 == 0 && B % 4 == 0, "unsupported B/R");
+    const int lane = threadIdx.x & 63;
+    const int64_t wpb = blockDim.x >> 6;
+    const int64_t cell = __builtin_amdgcn_readfirstlane(
+        (int)(xcd_tile(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6)));
+    if (cell >= ncell) return;
+    const int m = lane & 15, kk = lane >> 4;
+    const int64_t slice = cell / CPS, q = cell - slice * CPS;
+    const int64_t p0 = slice_ptr[slice];
+    const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
+    v4d acc[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[nt] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < width; ++t) {
+        const int64_t c = col[p0 + (int64_t)t * CPS + q];
+        const VT *sb = val + (p0 + (int64_t)t * CPS) * BB;
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) {
+            const int k = 4 * kc + kk;
+            const double a = m < B ? (double)sb[((int64_t)(k / V) * 64 + q * B + m) * V + k % V] : 0.0;
+            const double *xp = x + (c * B + k) * R + m;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xp[nt * 16], acc[nt], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+#pragma unroll
+        for (int r = 0; r < KC; ++r) {   // output rows kk + 4r < B
+            const int64_t idx = (cell * B + kk + 4 * r) * R + nt * 16 + m;
+            double y = acc[nt][r];
+            if (yx) y += yx[idx];
+            const double t = c1 * y;
+            if (tout) tout[idx] = t;
+            if (vout) vout[idx] = fma(c2, t, vin[idx]);
+        }
+    }
+}
+
+}  // namespace hymor

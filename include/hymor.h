@@ -168,6 +168,11 @@ int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int reps,
  * fastest for both directions.  nv: 1, 2 or 4 doubles per row. */
 int hymor_op_autotune(hymor_op *op, int nv, int reps, double *best_ms);
 
+/* Batches of 16+ doubles per row run on the FP64 matrix cores
+ * (k_spmm_mfma) by default; use_mfma = 0 selects the vector-FMA kernel
+ * (k_spmm).  Both give FP64-accumulated results. */
+int hymor_op_set_batch_kernel(hymor_op *op, int use_mfma);
+
 /* Read / force the chosen variant (split 0|1, waves, nt 0|1). */
 int hymor_op_get_tuning(const hymor_op *op, int *split, int *waves, int *nt);
 int hymor_op_set_tuning(hymor_op *op, int split, int waves, int nt);

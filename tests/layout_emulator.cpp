@@ -164,3 +164,78 @@ extern "C" int emu_spmm_csc(int64_t n, const int64_t *colptr, const int64_t *row
     }
     return g_oob;
 }
+
+// k_spmm_mfma: per cell, lanes supply MFMA operands exactly as the kernel
+// indexes them; the instruction is replayed with the gfx950 operand layout
+// of v_mfma_f64_16x16x4f64 (A[m][k] from lane m+16k, B[k][n] from lane
+// n+16k, D[m][n] in lane n+16*(m%4), register m/4).  vpl: values per
+// 16-byte chunk of the block storage (2 FP64, 4 FP32; FP32 values are
+// emulated in double).
+extern "C" int emu_spmm_mfma_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                                 const double *nzval, int b, int64_t ncell, int nrhs, int R,
+                                 int vpl, const double *X_vm, double *Y_vm) {
+    g_oob = 0;
+    HostBSR fwd, adj;
+    HostExtra xf, xa;
+    csc_to_blocks(n, colptr, rowval, nzval, 1, false, 1.0, b, ncell, fwd, adj, xf, xa);
+    if (!xf.rows.empty()) return -1;
+    HostSliced s;
+    bsr_to_sliced(fwd, s);
+    interleave_lanes(s, vpl);
+    const int64_t nflow = (int64_t)b * ncell;
+    std::vector<double> x((size_t)n * R), y((size_t)n * R, 0.0);
+    for (int64_t i = 0; i < n * R; ++i) {
+        const int64_t row = i / R, rem = i % R;
+        if (rem >= nrhs) { x[(size_t)i] = 0.0; continue; }
+        const int64_t vm = row < nflow ? (row % b) * ncell + row / b : row;
+        x[(size_t)i] = X_vm[rem * n + vm];
+    }
+    const int CPS = 64 / b, BB = b * b, NT = R / 16, KC = b / 4;
+    for (int64_t cell = 0; cell < ncell; ++cell) {
+        const int64_t slice = cell / CPS, q = cell - slice * CPS;
+        const int64_t p0 = at(s.slice_ptr, slice);
+        const int64_t width = (at(s.slice_ptr, slice + 1) - p0) / CPS;
+        std::vector<double> D((size_t)NT * 256, 0.0);  // D[nt][m][n]
+        for (int64_t t = 0; t < width; ++t) {
+            const int64_t c = at(s.col, p0 + t * CPS + q);
+            const int64_t sb = (p0 + t * CPS) * BB;
+            for (int kc = 0; kc < KC; ++kc) {
+                double A[16][4], Bm[NT > 0 ? 4 : 1][16 * 4];
+                double opa[64], opb[4][64];
+                for (int lane = 0; lane < 64; ++lane) {
+                    const int m = lane & 15, kk = lane >> 4, k = 4 * kc + kk;
+                    opa[lane] = m < b ? at(s.val, sb + ((int64_t)(k / vpl) * 64 + q * b + m) * vpl + k % vpl) : 0.0;
+                    for (int nt = 0; nt < NT; ++nt)
+                        opb[nt][lane] = at(x, (c * b + k) * R + m + nt * 16);
+                }
+                for (int mm = 0; mm < 16; ++mm)
+                    for (int k = 0; k < 4; ++k) A[mm][k] = opa[mm + 16 * k];
+                (void)Bm;
+                for (int nt = 0; nt < NT; ++nt)
+                    for (int mm = 0; mm < 16; ++mm)
+                        for (int nn = 0; nn < 16; ++nn) {
+                            double sacc = 0.0;
+                            for (int k = 0; k < 4; ++k) sacc += A[mm][k] * opb[nt][nn + 16 * k];
+                            D[(size_t)nt * 256 + mm * 16 + nn] += sacc;
+                        }
+            }
+        }
+        for (int lane = 0; lane < 64; ++lane) {
+            const int m = lane & 15, kk = lane >> 4;
+            for (int nt = 0; nt < NT; ++nt)
+                for (int r = 0; r < KC; ++r) {
+                    // register r of this lane holds D[kk + 4r][m]
+                    const double v = D[(size_t)nt * 256 + (kk + 4 * r) * 16 + m];
+                    const int64_t idx = (cell * b + kk + 4 * r) * R + nt * 16 + m;
+                    if (idx < 0 || idx >= (int64_t)y.size()) ++g_oob;
+                    else y[(size_t)idx] = v;
+                }
+        }
+    }
+    for (int64_t i = 0; i < n * nrhs; ++i) {
+        const int64_t rhs = i / n, vm = i % n;
+        const int64_t row = vm < nflow ? (vm % ncell) * b + vm / ncell : vm;
+        Y_vm[rhs * n + vm] = at(y, row * R + rhs);
+    }
+    return g_oob;
+}

@@ -67,3 +67,19 @@ def test_batched_f32_and_device_path(orc):
         assert np.array_equal(Y, dev.expm_taylor_batched(X, order=12, nsub=8, dt=1 / 64))
     ref = orc.power_method(A.scaled(1 / 64), X[:, 5], 8, order=12)
     assert 1e-12 < relerr(Y[:, 5], ref) <= 2e-6
+
+
+@pytest.mark.parametrize("mfma", [True, False])
+@pytest.mark.parametrize("b,nrhs,prec", [(8, 16, "f64"), (8, 32, "f32"), (4, 16, "f32"),
+                                         (4, 64, "f64"), (8, 64, "f32")])
+def test_batch_kernels_agree_with_oracle(orc, mfma, b, nrhs, prec):
+    say(f"[batched] kernel mfma={mfma} b={b} nrhs={nrhs} {prec}")
+    op = ops.stencil_operator(14, 10, b, seed=31, n_shock=14, row_sum=2.0)
+    A = orc.CSC.from_arrays(*op.to_csc_varmajor())
+    X = np.random.default_rng(5).standard_normal((op.n, nrhs))
+    with DeviceOperator.from_blocks(op, value_prec=prec) as dev:
+        dev.set_batch_kernel(mfma)
+        Y = dev.expm_taylor_batched(X, order=5, nsub=3, dt=0.1)
+    tol = 1e-12 if prec == "f64" else 2e-6
+    for j in (0, nrhs - 1):
+        assert relerr(Y[:, j], orc.power_method(A.scaled(0.1), X[:, j], 3)) <= tol

@@ -93,3 +93,20 @@ def test_emulated_batched_spmm(emu, b, R, nrhs):
                            op.ncell, nrhs, R, X.ctypes.data, Y.ctypes.data)
     assert oob == 0
     assert np.allclose(Y, op.to_scipy_varmajor() @ X, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("b,R,nrhs,vpl", [(4, 16, 16, 2), (4, 32, 20, 4), (8, 16, 16, 2),
+                                          (8, 16, 9, 4), (8, 32, 32, 2), (8, 64, 50, 4),
+                                          (4, 64, 64, 2)])
+def test_emulated_batched_mfma(emu, b, R, nrhs, vpl):
+    emu.emu_spmm_mfma_csc.restype = C.c_int
+    emu.emu_spmm_mfma_csc.argtypes = [C.c_int64] + [C.c_void_p] * 3 + [
+        C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    op = ops.stencil_operator(9, 7, b, seed=10)
+    colptr, rowval, nzval = op.to_csc_varmajor(one_based=True)
+    X = np.asfortranarray(np.random.default_rng(3).standard_normal((op.n, nrhs)))
+    Y = np.zeros_like(X)
+    oob = emu.emu_spmm_mfma_csc(op.n, colptr.ctypes.data, rowval.ctypes.data, nzval.ctypes.data,
+                                b, op.ncell, nrhs, R, vpl, X.ctypes.data, Y.ctypes.data)
+    assert oob == 0
+    assert np.allclose(Y, op.to_scipy_varmajor() @ X, rtol=1e-13, atol=1e-13)
