@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--nrhs", type=int, default=16, help="vectors per batched sweep")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--kernel-only", choices=["plain", "batched"],
+                    help="time only the Taylor-term kernel of one form (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the engine has no "
@@ -103,6 +105,17 @@ def main():
     cfg = ops.CONFIGS[args.config]
     log(f"generating {args.config}: {cfg}")
     op = ops.config_operator(args.config)
+    if args.kernel_only:
+        prec, nv = ("f64", 1) if args.kernel_only == "plain" else ("f32", args.nrhs)
+        devk = DeviceOperator.from_blocks(op, value_prec=prec, with_adjoint=False)
+        if nv == 1:
+            devk.autotune(reps=5)
+        elif nv in (16, 32, 64):
+            devk.autotune_batch(nv=nv, reps=3)
+        ms, by = devk.bench_term_kernel(reps=args.kernel_reps, nv=nv)
+        print(json.dumps({"kernel_only": args.kernel_only, "nv": nv, "ms": ms, "bytes": by,
+                          "GBps": by / (ms * 1e-3) / 1e9}), flush=True)
+        return
     log(f"n={op.n} nnzb={op.nnzb}; uploading FP64 blocks")
     dev = DeviceOperator.from_blocks(op, value_prec="f64", with_adjoint=False)
     tune_ms, tuning = dev.autotune(reps=10)
@@ -140,6 +153,9 @@ def main():
         nb = args.nrhs
         log(f"batched leg: FP32-stored values, {nb} vectors per sweep")
         devb = DeviceOperator.from_blocks(op, value_prec="f32", with_adjoint=False)
+        bkern = None
+        if nb in (16, 32, 64):
+            bkern, _ = devb.autotune_batch(nv=nb, reps=5)
         X = np.random.default_rng(7).standard_normal((op.n, nb))
         X[:, 0] = x
         dX = DeviceMatrix.from_host(X)
@@ -173,7 +189,10 @@ def main():
             "value_storage": "f32",
             "accumulation": "f64",
             "rel_err_vs_plain_fp64_action": err,
-            "kernel": f"k_spmm<{cfg['b']}, float, {nb}>",
+            "kernel": {0: f"k_spmm<{cfg['b']}, float, {nb}>",
+                       1: f"k_spmm_mfma<{cfg['b']}, float, {nb}>",
+                       2: f"k_spmm_mfma_mc<{cfg['b']}, float, {nb}, U, {2 if nb == 16 else 1}>",
+                       3: f"k_spmm_mfma_mc<{cfg['b']}, float, {nb}, U, 1>"}.get(bkern, "k_spmm"),
             "kernel_ms": bk_ms,
             "algorithmic_bytes_per_launch": bk_bytes,
             "achieved_GBps": b_gbs,

@@ -52,7 +52,9 @@ SIGNATURES = {
     "hymor_memcpy_d2h": (C.c_int, [_p, _p, _i64]),
     "hymor_bench_term_kernel": (C.c_int, [_p, C.c_int, C.c_int, C.c_int, _pd, _pd]),
     "hymor_op_autotune": (C.c_int, [_p, C.c_int, C.c_int, _pd]),
+    "hymor_op_set_graphs": (C.c_int, [_p, C.c_int]),
     "hymor_op_set_batch_kernel": (C.c_int, [_p, C.c_int]),
+    "hymor_op_autotune_batch": (C.c_int, [_p, C.c_int, C.c_int, C.POINTER(C.c_int), _pd]),
     "hymor_op_get_tuning": (C.c_int, [_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
     "hymor_op_set_tuning": (C.c_int, [_p, C.c_int, C.c_int, C.c_int]),

@@ -159,9 +159,25 @@ class DeviceOperator:
         check(_lib.load().hymor_op_get_tuning(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
 
-    def set_batch_kernel(self, mfma: bool):
-        """Matrix-core (True, default) or vector-FMA batch kernel."""
-        check(_lib.load().hymor_op_set_batch_kernel(self._h, int(bool(mfma))))
+    def set_graphs(self, enable: bool):
+        """Replay the Taylor sweep from a captured HIP graph (see hymor.h)."""
+        check(_lib.load().hymor_op_set_graphs(self._h, int(bool(enable))))
+
+    def set_batch_kernel(self, kernel):
+        """Batch kernel: 0 vector FMA, 1 matrix cores, 2/3 matrix cores with
+        two/one cells per wave (default 2).  True/False map to 2/0."""
+        if kernel is True:
+            kernel = 2
+        elif kernel is False:
+            kernel = 0
+        check(_lib.load().hymor_op_set_batch_kernel(self._h, int(kernel)))
+
+    def autotune_batch(self, nv=16, reps=5):
+        """Pick the fastest batch kernel at width nv; returns (id, ms)."""
+        k = C.c_int()
+        ms = C.c_double()
+        check(_lib.load().hymor_op_autotune_batch(self._h, nv, reps, C.byref(k), C.byref(ms)))
+        return k.value, ms.value
 
     def set_tuning(self, split, waves, nt):
         check(_lib.load().hymor_op_set_tuning(self._h, int(split), int(waves), int(nt)))
@@ -374,3 +390,14 @@ def chu_sandwich_ptr(op: DeviceOperator, weight: DeviceOperator, d_x, d_y, *, or
     check(_lib.load().hymor_chu_sandwich_dev(op.handle, weight.handle, int(order), int(nsub),
                                              float(dt), int(n_set0), d_x, d_y,
                                              C128 if cplx else F64))
+
+
+def expm_taylor_batched_ptr(op: DeviceOperator, d_X, ldx: int, nrhs: int, d_Y, ldy: int, *,
+                            order=5, nsub=1, dt=1.0, form="powers", transpose=False,
+                            complex_=None):
+    """hymor_expm_taylor_batched_dev on raw device pointers (e.g. a block of
+    Krylov basis columns)."""
+    cplx = op.is_complex if complex_ is None else complex_
+    check(_lib.load().hymor_expm_taylor_batched_dev(
+        op.handle, int(bool(transpose)), int(order), int(nsub), float(dt), _FORMS[form],
+        int(nrhs), d_X, int(ldx), d_Y, int(ldy), C128 if cplx else F64))

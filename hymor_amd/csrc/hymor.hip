@@ -49,7 +49,8 @@ struct TermTuning {
     int split = 0;   // 0: k_term (one wave per slice), 1: k_term_split
     int waves = 4;   // k_term: waves per workgroup; k_term_split: waves per slice
     int nt = 0;      // non-temporal loads of the block values
-    int spmm_mfma = 1;  // batches of >= 16 doubles per row: matrix-core kernel
+    int spmm_mfma = 2;  // batches of >= 16 doubles per row: 0 vector-FMA k_spmm,
+                        // 1 k_spmm_mfma, 2/3 k_spmm_mfma_mc (2 or 1 cells per wave)
 };
 
 struct DevSliced {
@@ -88,6 +89,18 @@ struct hymor_op {
     double *yx = nullptr, *yxa = nullptr;
     int64_t dev_bytes = 0;
     std::vector<void *> allocs;
+    // captured Taylor sweep (internal buffers only), replayed while the
+    // parameters repeat; cleared when buffers or kernel variants change
+    bool use_graphs = false;
+    struct {
+        bool valid = false;
+        bool adj = false;
+        int order = 0, form = 0, nv = 0;
+        int64_t nsub = 0;
+        double dt = 0.0;
+        hipGraphExec_t exec = nullptr;
+        const double *result = nullptr;
+    } graph;
 };
 
 // Block-diagonal weights (the Chu matrices R'MR): one b x b block per cell
@@ -187,8 +200,15 @@ void upload_extra(hymor_op *op, const HostExtra &h, DevExtra &d) {
     d.val = upload(op, h.val);
 }
 
+void drop_graph(hymor_op *op) {
+    if (op->graph.exec) (void)hipGraphExecDestroy(op->graph.exec);
+    op->graph.exec = nullptr;
+    op->graph.valid = false;
+}
+
 void release(hymor_op *op) {
     if (!op) return;
+    drop_graph(op);
     for (void *p : op->allocs) (void)hipFree(p);
     op->allocs.clear();
     if (op->stream) (void)hipStreamDestroy(op->stream);
@@ -197,6 +217,7 @@ void release(hymor_op *op) {
 
 void ensure_ws(hymor_op *op, int nv) {
     if (op->ws_nv >= nv) return;
+    drop_graph(op);
     // free the old set (tracked in allocs): rebuild the list without them
     for (int i = 0; i < 7; ++i) {
         if (!op->ws[i]) continue;
@@ -273,10 +294,27 @@ void launch_spmm_t(const DevSliced &s, hipStream_t st, const double *x, const do
     const int wpb = 4;
     const int64_t grid = (s.ncell + wpb - 1) / wpb;
     if constexpr (R % 16 == 0) {
-        if (s.tune.spmm_mfma) {
+        if (s.tune.spmm_mfma == 1) {
             hipLaunchKernelGGL((k_spmm_mfma<B, VT, R>), dim3((unsigned)grid), dim3(64 * wpb), 0,
                                st, s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, yx, tout,
                                vin, vout, c1, c2);
+            return;
+        }
+        if (s.tune.spmm_mfma >= 2) {
+            // 2: two cells per wave at width 16 (one above, to stay within the
+            // register file), 3: one cell per wave; slot groups of U
+            constexpr int U = R == 16 ? 3 : (R == 32 ? 2 : 1);
+            if (s.tune.spmm_mfma == 2 && R == 16) {
+                const int64_t waves = (s.ncell + 1) / 2;
+                hipLaunchKernelGGL((k_spmm_mfma_mc<B, VT, R, U, 2>),
+                                   dim3((unsigned)((waves + wpb - 1) / wpb)), dim3(64 * wpb), 0,
+                                   st, s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, yx,
+                                   tout, vin, vout, c1, c2);
+            } else {
+                hipLaunchKernelGGL((k_spmm_mfma_mc<B, VT, R, U, 1>), dim3((unsigned)grid),
+                                   dim3(64 * wpb), 0, st, s.slice_ptr, s.col, (const VT *)s.val,
+                                   s.ncell, x, yx, tout, vin, vout, c1, c2);
+            }
             return;
         }
     }
@@ -423,6 +461,44 @@ const double *run_taylor(hymor_op *op, bool adj, int order, int64_t nsub, double
         cur = nxt;
     }
     return cur;
+}
+
+// run_taylor from op->ws[1], through a captured graph when enabled: the
+// sweep touches only internal buffers, so one capture serves every call with
+// the same parameters (the caller-facing permutations stay outside).
+const double *run_taylor_ws1(hymor_op *op, bool adj, int order, int64_t nsub, double dt, int form,
+                             int nv) {
+    if (!op->use_graphs || nsub == 0)
+        return run_taylor(op, adj, order, nsub, dt, form, nv, op->ws[1]);
+    auto &g = op->graph;
+    if (!(g.valid && g.adj == adj && g.order == order && g.nsub == nsub && g.dt == dt &&
+          g.form == form && g.nv == nv)) {
+        drop_graph(op);
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
+        const double *res = nullptr;
+        try {
+            res = run_taylor(op, adj, order, nsub, dt, form, nv, op->ws[1]);
+        } catch (...) {
+            (void)hipStreamEndCapture(op->stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        HIP_TRY(hipStreamEndCapture(op->stream, &graph));
+        hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_TRY(e);
+        g.valid = true;
+        g.adj = adj;
+        g.order = order;
+        g.nsub = nsub;
+        g.dt = dt;
+        g.form = form;
+        g.nv = nv;
+        g.result = res;
+    }
+    HIP_TRY(hipGraphLaunch(g.exec, op->stream));
+    return g.result;
 }
 
 int nv_of(hymor_op *op, int dtype) {
@@ -668,7 +744,7 @@ int hymor_expm_taylor_batched_dev(hymor_op *op, int transpose, int order, int64_
         const int w = pad_width(nrhs * ncomp);
         ensure_ws(op, w);
         launch_perm_in(op, op->stream, (const double *)d_X, ldx, op->ws[1], (int)nrhs, ncomp, 0, w);
-        const double *res = run_taylor(op, transpose != 0, order, nsub, dt, form, w, op->ws[1]);
+        const double *res = run_taylor_ws1(op, transpose != 0, order, nsub, dt, form, w);
         launch_perm_out(op, op->stream, res, (double *)d_Y, ldy, (int)nrhs, ncomp, 0, w);
     });
 }
@@ -690,7 +766,7 @@ int hymor_expm_taylor_batched(hymor_op *op, int transpose, int order, int64_t ns
                                    (const char *)X + (size_t)j * ldx * esz, (size_t)op->n * esz,
                                    hipMemcpyHostToDevice, op->stream));
         launch_perm_in(op, op->stream, op->ws[0], op->n, op->ws[1], (int)nrhs, ncomp, 0, w);
-        const double *res = run_taylor(op, transpose != 0, order, nsub, dt, form, w, op->ws[1]);
+        const double *res = run_taylor_ws1(op, transpose != 0, order, nsub, dt, form, w);
         launch_perm_out(op, op->stream, res, op->ws[0], op->n, (int)nrhs, ncomp, 0, w);
         for (int64_t j = 0; j < nrhs; ++j)
             HIP_TRY(hipMemcpyAsync((char *)Y + (size_t)j * ldy * esz,
@@ -708,7 +784,7 @@ int hymor_expm_taylor_dev(hymor_op *op, int transpose, int order, int64_t nsub, 
         const int nv = nv_of(op, dtype);
         ensure_ws(op, nv);
         launch_perm_in(op, op->stream, (const double *)d_x, op->n, op->ws[1], 1, nv, 0, nv);
-        const double *res = run_taylor(op, transpose != 0, order, nsub, dt, form, nv, op->ws[1]);
+        const double *res = run_taylor_ws1(op, transpose != 0, order, nsub, dt, form, nv);
         launch_perm_out(op, op->stream, res, (double *)d_y, op->n, 1, nv, 0, nv);
     });
 }
@@ -723,7 +799,7 @@ int hymor_expm_taylor(hymor_op *op, int transpose, int order, int64_t nsub, doub
         const size_t bytes = (size_t)op->n * nv * sizeof(double);
         HIP_TRY(hipMemcpyAsync(op->ws[0], x, bytes, hipMemcpyHostToDevice, op->stream));
         launch_perm_in(op, op->stream, op->ws[0], op->n, op->ws[1], 1, nv, 0, nv);
-        const double *res = run_taylor(op, transpose != 0, order, nsub, dt, form, nv, op->ws[1]);
+        const double *res = run_taylor_ws1(op, transpose != 0, order, nsub, dt, form, nv);
         launch_perm_out(op, op->stream, res, op->ws[0], op->n, 1, nv, 0, nv);
         HIP_TRY(hipMemcpyAsync(y, op->ws[0], bytes, hipMemcpyDeviceToHost, op->stream));
         HIP_TRY(hipStreamSynchronize(op->stream));
@@ -849,15 +925,65 @@ int hymor_op_autotune(hymor_op *op, int nv, int reps, double *best_ms) {
         (void)hipEventDestroy(e1);
         op->fwd.tune = pick;
         op->adj.tune = pick;
+        drop_graph(op);
         *best_ms = best;
+    });
+}
+
+int hymor_op_set_graphs(hymor_op *op, int enable) {
+    return guarded([&] {
+        ARG_CHECK(op, "null operator");
+        HIP_TRY(hipStreamSynchronize(op->stream));
+        op->use_graphs = enable != 0;
+        if (!op->use_graphs) drop_graph(op);
     });
 }
 
 int hymor_op_set_batch_kernel(hymor_op *op, int use_mfma) {
     return guarded([&] {
         ARG_CHECK(op, "null operator");
-        op->fwd.tune.spmm_mfma = use_mfma ? 1 : 0;
+        ARG_CHECK(use_mfma >= 0 && use_mfma <= 3, "batch kernel must be 0..3");
+        op->fwd.tune.spmm_mfma = use_mfma;
         op->adj.tune.spmm_mfma = op->fwd.tune.spmm_mfma;
+        drop_graph(op);
+    });
+}
+
+int hymor_op_autotune_batch(hymor_op *op, int nv, int reps, int *best_kernel, double *best_ms) {
+    return guarded([&] {
+        ARG_CHECK(op && best_kernel && best_ms, "null argument");
+        ARG_CHECK(nv == 16 || nv == 32 || nv == 64, "batch autotune covers widths 16, 32, 64");
+        ARG_CHECK(reps >= 1, "reps must be >= 1");
+        ensure_ws(op, nv);
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        double best = 1e300;
+        int pick = op->fwd.tune.spmm_mfma;
+        for (int kern = 0; kern <= 3; ++kern) {
+            op->fwd.tune.spmm_mfma = kern;
+            launch_term(op, false, nv, op->stream, op->ws[4], op->ws[5], op->ws[2], op->ws[2], 0.5,
+                        1.0);
+            HIP_TRY(hipEventRecord(e0, op->stream));
+            for (int i = 0; i < reps; ++i)
+                launch_term(op, false, nv, op->stream, op->ws[4], op->ws[5], op->ws[2], op->ws[2],
+                            0.5, 1.0);
+            HIP_TRY(hipEventRecord(e1, op->stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            if (ms / reps < best) {
+                best = ms / reps;
+                pick = kern;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        op->fwd.tune.spmm_mfma = pick;
+        op->adj.tune.spmm_mfma = pick;
+        drop_graph(op);
+        *best_kernel = pick;
+        *best_ms = best;
     });
 }
 
@@ -881,6 +1007,7 @@ int hymor_op_set_tuning(hymor_op *op, int split, int waves, int nt) {
         op->fwd.tune.waves = waves;
         op->fwd.tune.nt = nt;
         op->adj.tune = op->fwd.tune;
+        drop_graph(op);
     });
 }
 

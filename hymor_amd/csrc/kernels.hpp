@@ -605,6 +605,9 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(
     constexpr int NT = R / 16;
     constexpr int KC = B / 4;
     constexpr int V = Vpl<VT>::value;
+    // slots whose operands are loaded together before their MFMAs: enough
+    // loads in flight to cover memory latency within one wave
+    constexpr int U = NT == 1 ? 9 : (NT == 2 ? 4 : 2);
     static_assert(R % 16 == 0 && B % 4 == 0, "unsupported B/R");
     const int lane = threadIdx.x & 63;
     const int64_t wpb = blockDim.x >> 6;
@@ -618,17 +621,35 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(
     v4d acc[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) acc[nt] = v4d{0.0, 0.0, 0.0, 0.0};
-    for (int t = 0; t < width; ++t) {
-        const int64_t c = col[p0 + (int64_t)t * CPS + q];
-        const VT *sb = val + (p0 + (int64_t)t * CPS) * BB;
+    for (int t0 = 0; t0 < width; t0 += U) {
+        double a[U][KC], bx[U][KC][NT];
 #pragma unroll
-        for (int kc = 0; kc < KC; ++kc) {
-            const int k = 4 * kc + kk;
-            const double a = m < B ? (double)sb[((int64_t)(k / V) * 64 + q * B + m) * V + k % V] : 0.0;
-            const double *xp = x + (c * B + k) * R + m;
+        for (int u = 0; u < U; ++u) {
+            const int t = t0 + u;
+            if (t < width) {
+                const int64_t c = col[p0 + (int64_t)t * CPS + q];
+                const VT *sb = val + (p0 + (int64_t)t * CPS) * BB;
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
-                acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xp[nt * 16], acc[nt], 0, 0, 0);
+                for (int kc = 0; kc < KC; ++kc) {
+                    const int k = 4 * kc + kk;
+                    a[u][kc] = m < B ? (double)sb[((int64_t)(k / V) * 64 + q * B + m) * V + k % V]
+                                     : 0.0;
+                    const double *xp = x + (c * B + k) * R + m;
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt) bx[u][kc][nt] = xp[nt * 16];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (t0 + u < width) {
+#pragma unroll
+                for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt)
+                        acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][kc], bx[u][kc][nt],
+                                                                       acc[nt], 0, 0, 0);
+            }
         }
     }
 #pragma unroll
@@ -642,6 +663,108 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(
             if (tout) tout[idx] = t;
             if (vout) vout[idx] = fma(c2, t, vin[idx]);
         }
+    }
+}
+
+}  // namespace hymor
+
+namespace hymor {
+
+// k_spmm_mfma with more memory work in flight per wave: CPW cells per wave
+// (consecutive cells of one slice), their U-slot operand loads issued
+// together, and the epilogue's vin / yx reads issued at wave start.  Same
+// arithmetic, operand layout and summation order as k_spmm_mfma.
+template <int B, typename VT, int R, int U, int CPW>
+__global__ __launch_bounds__(256) void k_spmm_mfma_mc(
+    const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
+    const VT *__restrict__ val, int64_t ncell, const double *__restrict__ x,
+    const double *__restrict__ yx, double *__restrict__ tout, const double *vin, double *vout,
+    double c1, double c2) {
+    constexpr int CPS = 64 / B;
+    constexpr int BB = B * B;
+    constexpr int NT = R / 16;
+    constexpr int KC = B / 4;
+    constexpr int V = Vpl<VT>::value;
+    static_assert(R % 16 == 0 && B % 4 == 0 && CPS % CPW == 0, "unsupported B/R/CPW");
+    const int lane = threadIdx.x & 63;
+    const int64_t wpb = blockDim.x >> 6;
+    const int64_t cell0 = (int64_t)__builtin_amdgcn_readfirstlane(
+                              (int)(xcd_tile(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6))) *
+                          CPW;
+    if (cell0 >= ncell) return;
+    const int m = lane & 15, kk = lane >> 4;
+    const int64_t slice = cell0 / CPS, q0 = cell0 - slice * CPS;
+    const int64_t p0 = slice_ptr[slice];
+    const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
+    int ncw = (int)(ncell - cell0 < CPW ? ncell - cell0 : CPW);
+    // epilogue operands first: independent of everything below
+    double pre_v[CPW][NT][KC], pre_y[CPW][NT][KC];
+#pragma unroll
+    for (int ci = 0; ci < CPW; ++ci)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int r = 0; r < KC; ++r) {
+                const int64_t idx = ((cell0 + ci) * B + kk + 4 * r) * R + nt * 16 + m;
+                const bool ok = ci < ncw;
+                pre_v[ci][nt][r] = (ok && vout) ? vin[idx] : 0.0;
+                pre_y[ci][nt][r] = (ok && yx) ? yx[idx] : 0.0;
+            }
+    v4d acc[CPW][NT];
+#pragma unroll
+    for (int ci = 0; ci < CPW; ++ci)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[ci][nt] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int t0 = 0; t0 < width; t0 += U) {
+        double a[U][CPW][KC], bx[U][CPW][KC][NT];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = t0 + u;
+            if (t < width) {
+                const VT *sb = val + (p0 + (int64_t)t * CPS) * BB;
+#pragma unroll
+                for (int ci = 0; ci < CPW; ++ci) {
+                    const int64_t q = q0 + ci;
+                    const int64_t c = col[p0 + (int64_t)t * CPS + q];
+#pragma unroll
+                    for (int kc = 0; kc < KC; ++kc) {
+                        const int k = 4 * kc + kk;
+                        a[u][ci][kc] =
+                            m < B ? (double)sb[((int64_t)(k / V) * 64 + q * B + m) * V + k % V] : 0.0;
+                        const double *xp = x + (c * B + k) * R + m;
+#pragma unroll
+                        for (int nt = 0; nt < NT; ++nt) bx[u][ci][kc][nt] = xp[nt * 16];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (t0 + u < width) {
+#pragma unroll
+                for (int ci = 0; ci < CPW; ++ci)
+#pragma unroll
+                    for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+                        for (int nt = 0; nt < NT; ++nt)
+                            acc[ci][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                a[u][ci][kc], bx[u][ci][kc][nt], acc[ci][nt], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int ci = 0; ci < CPW; ++ci) {
+        if (ci >= ncw) break;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int r = 0; r < KC; ++r) {
+                const int64_t idx = ((cell0 + ci) * B + kk + 4 * r) * R + nt * 16 + m;
+                const double y = acc[ci][nt][r] + pre_y[ci][nt][r];
+                const double t = c1 * y;
+                if (tout) tout[idx] = t;
+                if (vout) vout[idx] = fma(c2, t, pre_v[ci][nt][r]);
+            }
     }
 }
 

@@ -23,7 +23,7 @@ import scipy.linalg as sl
 
 from . import _lib
 from ._lib import C128, F64, check, ptr
-from .core import DeviceVector
+from .core import DeviceMatrix, DeviceVector
 
 
 class DeviceKrylov:
@@ -180,4 +180,92 @@ def krylov_schur(apply_dev, n: int, k: int, m: int, *, complex_: bool = False, t
         raise RuntimeError(f"Krylov-Schur did not converge in {maxiter} restarts")
     finally:
         scratch.free()
+        K.close()
+
+
+def block_krylov_schur(apply_block, n: int, k: int, m: int, s: int, *, complex_: bool = False,
+                       tol: float = 1e-10, maxiter: int = 200, V0=None):
+    """Block Krylov-Schur: ``apply_block(d_in, ld, nvec, d_out)`` applies the
+    operator to nvec consecutive device columns (leading dimension ld) at
+    once -- the batched SpMM action -- and every expansion step feeds it the
+    s newest basis vectors.  The s results are appended by sequential CGS2
+    (the same device orthonormaliser as krylov_schur), so the projected
+    matrix is band Hessenberg with s sub-diagonals.
+
+    V0 (n x s, optional) seeds the first block, e.g. Ritz vectors of an
+    earlier, cheaper solve.  Returns (theta, Y, info) as krylov_schur.
+    """
+    if not (1 <= k and k + s <= m):
+        raise ValueError("need k + s <= m")
+    dtype = np.complex128 if complex_ else np.float64
+    K = DeviceKrylov(n, m + s, complex_)
+    W = DeviceMatrix(n, s, dtype)
+    try:
+        rng = np.random.default_rng(0)
+        if V0 is None:
+            V0 = rng.standard_normal((n, s)) + (1j * rng.standard_normal((n, s)) if complex_ else 0)
+        V0 = np.asarray(V0, dtype=dtype)
+        if V0.shape[1] < s:    # pad a short seed block with random directions
+            extra = rng.standard_normal((n, s - V0.shape[1]))
+            V0 = np.concatenate([V0, extra.astype(dtype)], axis=1)
+        H = np.zeros((m + s, m + s), dtype=dtype)
+        # orthonormal first block (dependent seed columns are replaced)
+        for i in range(s):
+            K.set_column(i, V0[:, i])
+            h = K.orthonormalize(i, K.column(i))
+            if abs(h[i]) == 0:
+                K.set_column(i, rng.standard_normal(n).astype(dtype))
+                K.orthonormalize(i, K.column(i))
+        j = 0              # first column whose image is not yet in the basis
+        n_apply = 0
+        colbytes = n * np.dtype(dtype).itemsize
+        for it in range(1, maxiter + 1):
+            while j + s <= m:
+                apply_block(K.column(j), n, s, W.ptr)
+                n_apply += s
+                for i in range(s):
+                    wcol = C.c_void_p(W.ptr.value + i * colbytes)
+                    h = K.orthonormalize(j + s + i, wcol)
+                    H[: j + s + i + 1, j + i] = h[: j + s + i + 1]
+                j += s
+            mm = j
+            Hm = H[:mm, :mm]
+            Bres = H[mm:mm + s, :mm]          # residual rows
+            evals = np.linalg.eigvals(Hm)
+            thr = _select_lm(evals, k)
+            if complex_:
+                T, Z, sdim = sl.schur(Hm, output="complex", sort=lambda z: abs(z) >= thr * (1 - 1e-12))
+            else:
+                T, Z, sdim = sl.schur(Hm, output="real",
+                                      sort=lambda re, im: abs(complex(re, im)) >= thr * (1 - 1e-12))
+            p_keep = max(sdim, k)
+            w, Yt = sl.eig(T[:p_keep, :p_keep])
+            order = np.argsort(-np.abs(w), kind="stable")[:k]
+            w, Yt = w[order], Yt[:, order]
+            res = np.linalg.norm(Bres @ (Z[:, :p_keep] @ Yt), axis=0)
+            conv = res <= tol * np.maximum(np.abs(w), np.finfo(float).tiny)
+            if np.all(conv):
+                Q = Z[:, :p_keep] @ Yt
+                if complex_:
+                    Y = K.ritz(Q)
+                else:
+                    Y = K.ritz(np.ascontiguousarray(Q.real)) + 1j * K.ritz(np.ascontiguousarray(Q.imag))
+                Y = Y / np.linalg.norm(Y, axis=0)
+                return w, Y, {"iterations": it, "applications": n_apply, "residual_estimates": res}
+            # restart: keep p Schur vectors, then the residual block
+            p = max(p_keep, min(p_keep + (k - int(conv.sum())), mm - s))
+            if not complex_ and p < mm and p > p_keep and abs(T[p, p - 1]) > 0:
+                p += 1
+            p = min(p, mm - 1)
+            Bz = Bres @ Z[:, :p]
+            K.combine(Z[:, :p])
+            for i in range(s):
+                K.copy_column(mm + i, p + i)
+            H = np.zeros((m + s, m + s), dtype=dtype)
+            H[:p, :p] = T[:p, :p]
+            H[p:p + s, :p] = Bz
+            j = p
+        raise RuntimeError(f"block Krylov-Schur did not converge in {maxiter} restarts")
+    finally:
+        W.free()
         K.close()

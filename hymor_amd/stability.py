@@ -24,8 +24,8 @@ import scipy.sparse.linalg as spla
 
 from . import _lib
 from .core import (DeviceOperator, DeviceVector, DeviceWeights, chu_sandwich_ptr, energy_track,
-                   expm_taylor_ptr, spmv_dev)
-from .krylov import krylov_schur
+                   expm_taylor_batched_ptr, expm_taylor_ptr, spmv_dev)
+from .krylov import block_krylov_schur, krylov_schur
 
 ORDER = 5            # Taylor order hard-coded by the reference
 B = 4                # conserved variables per cell in the reference
@@ -599,3 +599,64 @@ def _device_generalized(devA, devC, D, s, n, k, n_t, n_set0, cplx, tol, maxiter,
         t2.free()
         dLi.close()
         dLiH.close()
+
+
+# ====================================================================== batched eigen solve
+
+def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=16, ncv=None, tol=1e-10,
+                     tol_f32=1e-6, maxiter=200, order=ORDER, value_prec="f32", refine=True,
+                     timings=None):
+    """Leading eigenvalues of the exponential transform with batched actions.
+
+    Phase 1: block Krylov-Schur whose every expansion applies T_order(A dt)^n
+    to `block` basis vectors in one SpMM sweep, block values stored in
+    `value_prec` (FP32 by default) and accumulated in FP64, converged to
+    tol_f32 -- the accuracy the stored values allow.  Phase 2 (refine): the
+    same solver on FP64-stored values, seeded with the phase-1 Ritz block, to
+    tol.  Eigenvalues are mapped as EIGENVALUES_GPU_ARNOLDI does:
+    lambda = log(mu) / (n dt).  Returns (V, D, info).
+    """
+    s = CFL_TIMESTEP(s)
+    dt = s["time_integration"]["dt"]
+    timesteps = int(round(T_ref / dt))
+    n = A.shape[0]
+    k = n_modes
+    m = ncv or max(3 * block, 2 * k + 2 * block)
+    info = {}
+
+    def make_apply(dev):
+        def apply(d_in, ld, nvec, d_out):
+            expm_taylor_batched_ptr(dev, d_in, ld, nvec, d_out, n, order=order, nsub=timesteps,
+                                    complex_=False)
+            dev.synchronize()
+        return apply
+
+    t0 = time.perf_counter()
+    dev = upload(A, s, scale=dt, value_prec=value_prec)
+    try:
+        tol1 = tol_f32 if (refine and value_prec == "f32") else tol
+        vals, Y, i1 = block_krylov_schur(make_apply(dev), n, k, m, block, tol=tol1,
+                                         maxiter=maxiter)
+    finally:
+        dev.close()
+    info["phase1"] = dict(i1, seconds=time.perf_counter() - t0, value_prec=value_prec)
+    if refine and value_prec == "f32":
+        t1 = time.perf_counter()
+        seed = np.concatenate([Y.real, Y.imag], axis=1)
+        q, rr = np.linalg.qr(seed)
+        keep = np.abs(np.diag(rr)) > 1e-10 * np.abs(rr[0, 0])
+        seed = q[:, keep][:, :block]
+        dev = upload(A, s, scale=dt, value_prec="f64")
+        try:
+            vals, Y, i2 = block_krylov_schur(make_apply(dev), n, k, m, block, tol=tol,
+                                             maxiter=maxiter, V0=seed)
+        finally:
+            dev.close()
+        info["phase2"] = dict(i2, seconds=time.perf_counter() - t1, value_prec="f64")
+    info["seconds"] = time.perf_counter() - t0
+    order_ = np.argsort(-np.abs(vals), kind="stable")
+    vals, Y = vals[order_], Y[:, order_]
+    D = np.log(vals.astype(np.complex128)) / dt / timesteps
+    if timings is not None:
+        timings.update(info)
+    return Y, D, info

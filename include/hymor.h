@@ -168,10 +168,21 @@ int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int reps,
  * fastest for both directions.  nv: 1, 2 or 4 doubles per row. */
 int hymor_op_autotune(hymor_op *op, int nv, int reps, double *best_ms);
 
-/* Batches of 16+ doubles per row run on the FP64 matrix cores
- * (k_spmm_mfma) by default; use_mfma = 0 selects the vector-FMA kernel
- * (k_spmm).  Both give FP64-accumulated results. */
+/* enable != 0: the Taylor sweep of hymor_expm_taylor(_dev) and the batched
+ * forms is captured into a HIP graph on first use and replayed while
+ * (transpose, order, nsub, dt, form, width) repeat -- one graph launch
+ * instead of order*nsub kernel launches.  Off by default. */
+int hymor_op_set_graphs(hymor_op *op, int enable);
+
+/* Kernel for batches of 16+ doubles per row: 0 vector FMA (k_spmm),
+ * 1 FP64 matrix cores (k_spmm_mfma), 2 / 3 matrix cores with two / one
+ * cells per wave and prefetched epilogue operands (k_spmm_mfma_mc, the
+ * default is 2).  All accumulate in FP64. */
 int hymor_op_set_batch_kernel(hymor_op *op, int use_mfma);
+
+/* Times the four batch kernels at width nv (16, 32 or 64) and keeps the
+ * fastest; returns its id and ms per launch. */
+int hymor_op_autotune_batch(hymor_op *op, int nv, int reps, int *best_kernel, double *best_ms);
 
 /* Read / force the chosen variant (split 0|1, waves, nt 0|1). */
 int hymor_op_get_tuning(const hymor_op *op, int *split, int *waves, int *nt);

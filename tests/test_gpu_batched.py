@@ -69,17 +69,23 @@ def test_batched_f32_and_device_path(orc):
     assert 1e-12 < relerr(Y[:, 5], ref) <= 2e-6
 
 
-@pytest.mark.parametrize("mfma", [True, False])
+@pytest.mark.parametrize("mfma", [0, 1, 2, 3])
 @pytest.mark.parametrize("b,nrhs,prec", [(8, 16, "f64"), (8, 32, "f32"), (4, 16, "f32"),
-                                         (4, 64, "f64"), (8, 64, "f32")])
+                                         (4, 64, "f64"), (8, 64, "f32"), (8, 16, "f32")])
 def test_batch_kernels_agree_with_oracle(orc, mfma, b, nrhs, prec):
-    say(f"[batched] kernel mfma={mfma} b={b} nrhs={nrhs} {prec}")
-    op = ops.stencil_operator(14, 10, b, seed=31, n_shock=14, row_sum=2.0)
+    say(f"[batched] kernel {mfma} b={b} nrhs={nrhs} {prec}")
+    # 13 x 9 cells: an odd cell count exercises the partial last wave
+    op = ops.stencil_operator(13, 9, b, seed=31, n_shock=13, row_sum=2.0)
     A = orc.CSC.from_arrays(*op.to_csc_varmajor())
     X = np.random.default_rng(5).standard_normal((op.n, nrhs))
     with DeviceOperator.from_blocks(op, value_prec=prec) as dev:
         dev.set_batch_kernel(mfma)
         Y = dev.expm_taylor_batched(X, order=5, nsub=3, dt=0.1)
+        if nrhs >= 16:
+            kern, ms = dev.autotune_batch(nv=nrhs, reps=2)
+            assert kern in (0, 1, 2, 3) and ms > 0
+            Y2 = dev.expm_taylor_batched(X, order=5, nsub=3, dt=0.1)
+            assert relerr(Y2, Y) <= 1e-13
     tol = 1e-12 if prec == "f64" else 2e-6
     for j in (0, nrhs - 1):
         assert relerr(Y[:, j], orc.power_method(A.scaled(0.1), X[:, j], 3)) <= tol

@@ -322,3 +322,32 @@ def test_autotune_picks_a_valid_variant(orc):
         A = orc.CSC.from_arrays(*op.to_csc_varmajor())
         got = dev.expm_taylor(x, order=12, nsub=4, dt=1 / 64)
         assert relerr(got, orc.power_method(A.scaled(1 / 64), x, 4, order=12)) <= 1e-12
+
+
+def test_graph_replays_match_eager(orc):
+    """The captured Taylor sweep, replayed many times with eager work and
+    synchronisations in between, gives bit-identical results to the eager
+    launches; a parameter change recaptures."""
+    op = make_case("shock12x10_b4")
+    say("[graphs] replays vs eager")
+    A = orc.CSC.from_arrays(*op.to_csc_varmajor())
+    r = np.random.default_rng(17)
+    xs = [r.standard_normal(op.n) for _ in range(4)]
+    with DeviceOperator.from_blocks(op) as eager, DeviceOperator.from_blocks(op) as g:
+        g.set_graphs(True)
+        for rep in range(3):
+            for x in xs:
+                want = eager.expm_taylor(x, order=5, nsub=6, dt=0.1)
+                got = g.expm_taylor(x, order=5, nsub=6, dt=0.1)
+                assert np.array_equal(got, want), rep
+                eager.synchronize()
+            # change of parameters -> recapture, then back
+            got = g.expm_taylor(xs[0], order=4, nsub=3, dt=0.1, transpose=True)
+            assert np.array_equal(got, eager.expm_taylor(xs[0], order=4, nsub=3, dt=0.1,
+                                                         transpose=True))
+        X = np.stack(xs, axis=1)
+        for rep in range(3):
+            assert np.array_equal(g.expm_taylor_batched(X, order=5, nsub=2, dt=0.1),
+                                  eager.expm_taylor_batched(X, order=5, nsub=2, dt=0.1))
+        assert relerr(g.expm_taylor(xs[1], order=5, nsub=6, dt=0.1),
+                      orc.power_method(A.scaled(0.1), xs[1], 6)) <= 1e-12

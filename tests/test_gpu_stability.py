@@ -267,3 +267,32 @@ def test_freestream_receptivity_device_krylov_vs_dense():
     D_ = R_.T @ st.CONSTRUCT_M_INFTY_(s, [1, 1, 1, 1], 1.0, w, True) @ R_
     ref = _dense_gains(A, C_, D_, 1.0 / n_t, n_t, n0=4 * nchi * neta, k=3) / 2
     assert np.all(np.abs(G - ref) <= 1e-8 * np.abs(ref)), (G, ref)
+
+
+def test_batched_fp32_eigs_with_fp64_refinement(orc):
+    """Batched (SpMM) block Krylov-Schur on FP32-stored values, refined on
+    FP64-stored values: eigenvalues within 1e-8 of the closed-form spectrum,
+    while the FP32 phase alone is limited to about the FP32 rounding."""
+    nchi, neta, b = 15, 12, 4
+    say("[batched eigs] FP32 storage + FP64 refinement, circulant closed form")
+    op, blocks = ops.circulant_operator(nchi, neta, b, seed=6, row_sum=3.0)
+    A = op.to_scipy_varmajor()
+    dt = 0.05
+    s = {"mesh": {"Nchi": nchi, "Neta": neta}, "time_integration": {"dt": dt},
+         "stability_analysis": {"perturb_shock": False,
+                                "eigenvalue_solver": "GPU_TIMESTEPPER_ARNOLDI"}}
+    k = 6
+    n = int(round(0.5 / dt))
+    lam = orc.circulant_eigenvalues(blocks, nchi, neta)
+    mu = sum((lam * dt) ** j / math.factorial(j) for j in range(6)) ** n
+    exact = (np.log(mu) / (n * dt))[np.argsort(-np.abs(mu), kind="stable")][: k + 8]
+
+    def err(D):
+        return max(np.min(np.abs(exact - d)) / abs(d) for d in D)
+
+    V, D, info = st.batched_exp_eigs(A, s, k, block=16, tol=1e-12)
+    assert err(D) <= 1e-8, (err(D), info)
+    V1, D1, info1 = st.batched_exp_eigs(A, s, k, block=16, tol=1e-12, refine=False)
+    assert err(D1) > 1e-10            # FP32-stored values alone do not reach FP64 accuracy
+    V2, D2, _ = st.batched_exp_eigs(A, s, k, block=16, tol=1e-12, value_prec="f64")
+    assert err(D2) <= 1e-8
