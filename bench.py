@@ -17,10 +17,14 @@ supported: this engine runs on one GPU.
 from __future__ import annotations
 
 import argparse
+import csv
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -68,6 +72,120 @@ def cpu_baseline(op, order, nsub, sample_substeps=3):
     }
 
 
+def eigen_leg(k=10, block=16, tol=1e-10):
+    """Leading eigenvalues of the exponential transform (T_ref = 0.5,
+    dt = 1/64, order 5 as EIGENVALUES_GPU_ARNOLDI) of a block-circulant
+    operator on 40 x 30 cells (b = 4, N = 4 800) -- small enough that its
+    leading eigenvalues are resolved (a smooth 3x3-stencil symbol clusters
+    them as O(1/N^2) on larger grids) -- whose spectrum is known in closed
+    form: plain FP64 device Krylov-Schur versus
+    the batched form (block Krylov-Schur, SpMM actions on FP32-stored values,
+    then FP64-stored refinement).  Both errors are measured against the
+    closed form."""
+    import math
+
+    from hymor_amd import operators as ops
+    from hymor_amd import stability as st
+
+    nchi, neta = 40, 30
+    op, blocks = ops.circulant_operator(nchi, neta, 4, seed=1, row_sum=8.0)
+    A = op.to_scipy_varmajor()
+    dt = 1.0 / 64
+    s = {"mesh": {"Nchi": nchi, "Neta": neta}, "time_integration": {"dt": dt},
+         "stability_analysis": {"perturb_shock": False,
+                                "eigenvalue_solver": "GPU_TIMESTEPPER_ARNOLDI"}}
+    lam = ops.circulant_spectrum(blocks, nchi, neta)
+    n_t = int(round(0.5 / dt))
+    mu = sum((lam * dt) ** j / math.factorial(j) for j in range(6)) ** n_t
+    exact = (np.log(mu) / (n_t * dt))[np.argsort(-np.abs(mu), kind="stable")][: 4 * k]
+
+    def err(D):
+        return float(max(np.min(np.abs(exact - d)) / abs(d) for d in D))
+
+    t0 = time.perf_counter()
+    _, D = st.EIGENVALUES_GPU_ARNOLDI(A, s, k, krylov="device", tol=tol, ncv=40, maxiter=400)
+    tp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, Db, info = st.batched_exp_eigs(A, s, k, block=block, tol=tol, maxiter=400)
+    tb = time.perf_counter() - t0
+    return {
+        "workload": ("10 leading eigenvalues of T_5(A/64)^32 (T_ref 0.5), block-circulant "
+                     "40x30 cells, b=4, N=4800; exact spectrum in closed form"),
+        "plain_fp64_seconds": tp,
+        "plain_fp64_max_rel_eig_err": err(D),
+        "batched_seconds": tb,
+        "batched_max_rel_eig_err": err(Db),
+        "batched_block": block,
+        "batched_phase1_f32_applications": info["phase1"]["applications"],
+        "batched_phase2_f64_applications": info.get("phase2", {}).get("applications"),
+        "speedup_vs_plain_fp64": tp / tb,
+    }
+
+
+COPY_MIB = 512
+
+
+def pmc_passes(config, timeout_s=150):
+    """HBM traffic per kernel launch from two rocprofv3 --pmc passes
+    (FETCH_SIZE, then WRITE_SIZE; they do not fit one pass) over this script
+    in --kernel-only plain mode, which also launches the streaming-copy
+    kernel k_copy on COPY_MIB MiB.  Runs as child processes before this
+    process touches the GPU.  Returns {counter: {kernel name: mean KiB per
+    dispatch}} or None with the reason."""
+    out = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix=f"hymor_pmc_{counter}_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", str(timeout_s), "rocprofv3", "--pmc", counter,
+               "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable, "-u",
+               str(Path(__file__).resolve()), "--config", config, "--kernel-only", "plain",
+               "--kernel-reps", "5"]
+        log(f"traffic: rocprofv3 --pmc {counter} pass")
+        logf = Path(d) / "pass.log"
+        try:
+            with open(logf, "w") as lf:
+                r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=lf, stderr=subprocess.STDOUT,
+                                   timeout=timeout_s + 30)
+        except (OSError, subprocess.TimeoutExpired) as e:
+            return None, f"{counter} pass did not run: {e}"
+        files = list(Path(d).rglob("*counter_collection.csv"))
+        # rocprofv3 may end with SIGSEGV after a complete run; the CSV decides
+        if not files or (r.returncode not in (0, 139, -11)):
+            tail = [ln for ln in logf.read_text(errors="replace").splitlines()
+                    if ln.strip() and not ln.startswith(("W20", "I20"))][-4:]
+            log(f"traffic: {counter} pass failed (exit {r.returncode}): " + " | ".join(tail))
+            return None, f"{counter} pass failed (exit {r.returncode}): " + " | ".join(tail)[-400:]
+        acc = {}
+        with open(files[0]) as f:
+            for row in csv.DictReader(f):
+                if row["Counter_Name"] != counter:
+                    continue
+                name = row["Kernel_Name"]
+                acc.setdefault(name, []).append(float(row["Counter_Value"]))
+        out[counter] = {k: sum(v) / len(v) for k, v in acc.items()}
+    return out, None
+
+
+def traffic_from_passes(passes, kernel):
+    """Bytes per launch of `kernel` (a name like "k_term_split<8, double, 1,
+    4, true>"), each counter scaled by bytes / counter-reading of k_copy."""
+    copy_bytes = COPY_MIB * 1024 * 1024
+
+    def mean_kib(counter, name):
+        vals = [v for k, v in passes[counter].items() if f"hymor::{name}(" in k]
+        return vals[0] if vals else None
+
+    total = 0.0
+    factors = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        c_kib, k_kib = mean_kib(counter, "k_copy"), mean_kib(counter, kernel)
+        if not c_kib or k_kib is None:
+            return None, None
+        factors[counter] = copy_bytes / (c_kib * 1024.0)
+        total += k_kib * 1024.0 * factors[counter]
+    return total, factors
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -89,13 +207,20 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--nrhs", type=int, default=16, help="vectors per batched sweep")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-eigen", action="store_true", help="skip the eigenvalue leg")
     ap.add_argument("--kernel-only", choices=["plain", "batched"],
                     help="time only the Taylor-term kernel of one form (for profiler runs)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc passes that measure HBM traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the engine has no "
               "multi-GPU path", file=sys.stderr)
         sys.exit(2)
+
+    passes, traffic_note = None, "skipped (--no-traffic)"
+    if not args.kernel_only and not args.no_traffic:
+        passes, traffic_note = pmc_passes(args.config)
 
     from hymor_amd import operators as ops
     from hymor_amd.core import (DeviceMatrix, DeviceOperator, DeviceVector,
@@ -113,8 +238,11 @@ def main():
         elif nv in (16, 32, 64):
             devk.autotune_batch(nv=nv, reps=3)
         ms, by = devk.bench_term_kernel(reps=args.kernel_reps, nv=nv)
+        from hymor_amd.calibrate import copy_bandwidth
+        cp = copy_bandwidth(COPY_MIB, 3)
         print(json.dumps({"kernel_only": args.kernel_only, "nv": nv, "ms": ms, "bytes": by,
-                          "GBps": by / (ms * 1e-3) / 1e9}), flush=True)
+                          "GBps": by / (ms * 1e-3) / 1e9, "tuning": devk.tuning if nv == 1
+                          else None, "copy": cp}), flush=True)
         return
     log(f"n={op.n} nnzb={op.nnzb}; uploading FP64 blocks")
     dev = DeviceOperator.from_blocks(op, value_prec="f64", with_adjoint=False)
@@ -147,6 +275,14 @@ def main():
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     y_plain = dy.download()
     dev.close()
+    kname = (f"k_term_split<{cfg['b']}, double, 1, {tuning[1]}, {str(bool(tuning[2])).lower()}>"
+             if tuning[0] else f"k_term<{cfg['b']}, double, 1, {str(bool(tuning[2])).lower()}>")
+    traffic, factors = None, None
+    if passes is not None:
+        traffic, factors = traffic_from_passes(passes, kname)
+        traffic_note = ("FETCH_SIZE + WRITE_SIZE per launch, each scaled by bytes / counter "
+                        "reading of k_copy" if traffic is not None else
+                        f"{kname} not found in the pmc passes")
 
     batched = None
     if not args.no_batched:
@@ -230,15 +366,20 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": (f"k_term_split<{cfg['b']}, double, 1, {tuning[1]}, {bool(tuning[2])}>"
-                       if tuning[0] else f"k_term<{cfg['b']}, double, 1, {bool(tuning[2])}>"),
+            "traffic": traffic,
+            "traffic_note": traffic_note,
+            "traffic_counter_scale": factors,
+            "kernel": kname,
             "launch": {"split": tuning[0], "waves": tuning[1], "nontemporal": tuning[2]},
             "kernel_ms": k_ms,
             "algorithmic_bytes_per_launch": k_bytes,
         },
         "batched": batched,
     }
+    if not args.no_eigen:
+        log("eigen leg: plain FP64 Krylov-Schur vs batched FP32 + FP64 refinement")
+        result["batched_eigen"] = eigen_leg()
+        log(f"eigen leg: {result['batched_eigen']}")
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(op, order, nsub)
     else:

@@ -42,6 +42,9 @@ SIGNATURES = {
                                             _p, _i64, _p, _i64, C.c_int]),
     "hymor_expm_taylor_batched_dev": (C.c_int, [_p, C.c_int, C.c_int, _i64, C.c_double, C.c_int,
                                                 _i64, _p, _i64, _p, _i64, C.c_int]),
+    "hymor_expm_taylor_batched_coldiag_dev": (C.c_int, [_p, C.c_int, C.c_int, _i64, C.c_double,
+                                                        C.c_int, _i64, _p, _i64, _i64, _p, _i64,
+                                                        _p, _i64]),
     "hymor_chu_sandwich": (C.c_int, [_p, _p, C.c_int, _i64, C.c_double, _i64, _p, _p, C.c_int]),
     "hymor_chu_sandwich_dev": (C.c_int, [_p, _p, C.c_int, _i64, C.c_double, _i64, _p, _p,
                                          C.c_int]),

@@ -89,6 +89,11 @@ struct hymor_op {
     double *yx = nullptr, *yxa = nullptr;
     int64_t dev_bytes = 0;
     std::vector<void *> allocs;
+    // per-column complex diagonal on extra rows [cd0, cd1) for the current
+    // batched call (factored receptivity); null otherwise
+    double *cdiag = nullptr, *cdiag_adj = nullptr;
+    int64_t cd0 = 0, cd1 = 0;
+    bool cdiag_on = false;
     // captured Taylor sweep (internal buffers only), replayed while the
     // parameters repeat; cleared when buffers or kernel variants change
     bool use_graphs = false;
@@ -358,33 +363,38 @@ void launch_block(const DevSliced &s, int nv, hipStream_t st, const double *x, c
     }
 }
 
+struct ColDiag {
+    const double *d = nullptr;
+    int64_t r0 = 0, r1 = 0;
+};
+
 template <int NV>
 void launch_extra_nv(const DevExtra &xe, int64_t nflow, hipStream_t st, const double *x,
                      double *yx, double *tout, const double *vin, double *vout, double c1,
-                     double c2) {
+                     double c2, const ColDiag &cd) {
     const int wpb = 4;
     const int64_t grid = (xe.nrows + wpb - 1) / wpb;
     if (xe.is_complex)
         hipLaunchKernelGGL((k_extra<NV, true>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
                            xe.rows, xe.ptr, xe.col, xe.val, xe.nrows, nflow, x, yx, tout, vin,
-                           vout, c1, c2);
+                           vout, c1, c2, cd.d, cd.r0, cd.r1);
     else
         hipLaunchKernelGGL((k_extra<NV, false>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
                            xe.rows, xe.ptr, xe.col, xe.val, xe.nrows, nflow, x, yx, tout, vin,
-                           vout, c1, c2);
+                           vout, c1, c2, cd.d, cd.r0, cd.r1);
 }
 
 void launch_extra(const DevExtra &xe, int nv, int64_t nflow, hipStream_t st, const double *x,
                   double *yx, double *tout, const double *vin, double *vout, double c1,
-                  double c2) {
+                  double c2, const ColDiag &cd) {
     switch (nv) {
-    case 1: launch_extra_nv<1>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2); break;
-    case 2: launch_extra_nv<2>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2); break;
-    case 4: launch_extra_nv<4>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2); break;
-    case 8: launch_extra_nv<8>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2); break;
-    case 16: launch_extra_nv<16>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2); break;
-    case 32: launch_extra_nv<32>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2); break;
-    case 64: launch_extra_nv<64>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2); break;
+    case 1: launch_extra_nv<1>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd); break;
+    case 2: launch_extra_nv<2>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd); break;
+    case 4: launch_extra_nv<4>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd); break;
+    case 8: launch_extra_nv<8>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd); break;
+    case 16: launch_extra_nv<16>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd); break;
+    case 32: launch_extra_nv<32>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd); break;
+    case 64: launch_extra_nv<64>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd); break;
     default: throw ArgError("unsupported vector width");
     }
 }
@@ -398,7 +408,9 @@ void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, const double *x
     double *yx = adj ? op->yxa : op->yx;
     const bool has_x = xe.nrows > 0;
     if (has_x) {
-        launch_extra(xe, nv, op->nflow, st, x, yx, tout, vin, vout, c1, c2);
+        ColDiag cd;
+        if (op->cdiag_on) cd = ColDiag{adj ? op->cdiag_adj : op->cdiag, op->cd0, op->cd1};
+        launch_extra(xe, nv, op->nflow, st, x, yx, tout, vin, vout, c1, c2, cd);
         HIP_TRY(hipGetLastError());
     }
     if (op->ncell > 0) {
@@ -746,6 +758,50 @@ int hymor_expm_taylor_batched_dev(hymor_op *op, int transpose, int order, int64_
         launch_perm_in(op, op->stream, (const double *)d_X, ldx, op->ws[1], (int)nrhs, ncomp, 0, w);
         const double *res = run_taylor_ws1(op, transpose != 0, order, nsub, dt, form, w);
         launch_perm_out(op, op->stream, res, (double *)d_Y, ldy, (int)nrhs, ncomp, 0, w);
+    });
+}
+
+int hymor_expm_taylor_batched_coldiag_dev(hymor_op *op, int transpose, int order, int64_t nsub,
+                                          double dt, int form, int64_t nrhs, const double *diag,
+                                          int64_t row0, int64_t row1, const void *d_X,
+                                          int64_t ldx, void *d_Y, int64_t ldy) {
+    return guarded([&] {
+        check_taylor_args(op, transpose, order, nsub, form);
+        ARG_CHECK(d_X && d_Y && diag, "null argument");
+        check_batch(op, nrhs, ldx, ldy);
+        ARG_CHECK(row0 >= op->nflow && row0 <= row1 && row1 <= op->n,
+                  "the diagonal block must lie on extra DOFs: nflow <= row0 <= row1 <= n");
+        const int ncomp = 2;
+        const int w = pad_width(nrhs * ncomp);
+        ensure_ws(op, w);
+        // per-column diagonal (and its conjugate for the adjoint), padded
+        std::vector<double> d((size_t)w, 0.0), da((size_t)w, 0.0);
+        for (int64_t c = 0; c < nrhs; ++c) {
+            d[2 * c] = diag[2 * c];
+            d[2 * c + 1] = diag[2 * c + 1];
+            da[2 * c] = diag[2 * c];
+            da[2 * c + 1] = -diag[2 * c + 1];
+        }
+        if (!op->cdiag) {
+            op->cdiag = (double *)dalloc(op, 64 * sizeof(double));
+            op->cdiag_adj = (double *)dalloc(op, 64 * sizeof(double));
+        }
+        HIP_TRY(hipMemcpyAsync(op->cdiag, d.data(), (size_t)w * sizeof(double),
+                               hipMemcpyHostToDevice, op->stream));
+        HIP_TRY(hipMemcpyAsync(op->cdiag_adj, da.data(), (size_t)w * sizeof(double),
+                               hipMemcpyHostToDevice, op->stream));
+        op->cd0 = row0;
+        op->cd1 = row1;
+        op->cdiag_on = true;
+        struct Off {
+            hymor_op *o;
+            ~Off() { o->cdiag_on = false; }
+        } off{op};
+        launch_perm_in(op, op->stream, (const double *)d_X, ldx, op->ws[1], (int)nrhs, ncomp, 0, w);
+        // not graph-captured: the diagonal buffer is rewritten per call
+        const double *res = run_taylor(op, transpose != 0, order, nsub, dt, form, w, op->ws[1]);
+        launch_perm_out(op, op->stream, res, (double *)d_Y, ldy, (int)nrhs, ncomp, 0, w);
+        HIP_TRY(hipStreamSynchronize(op->stream));
     });
 }
 

@@ -142,12 +142,16 @@ __global__ __launch_bounds__(256) void k_term(
 // here.  CPLX: values are complex (re,im) and NV counts doubles of complex
 // vectors ((re,im) pairs); otherwise values are real and every one of the
 // NV components is an independent right-hand side.
+// cdiag (optional): a complex diagonal that differs per column (per pair
+// (re, im) of the NV doubles), added on rows [cd0, cd1) -- the -i*omega_k
+// block of the factored receptivity systems, one wavenumber per column.
 template <int NV, bool CPLX>
 __global__ __launch_bounds__(256) void k_extra(
     const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
     const int32_t *__restrict__ col, const double *__restrict__ val, int64_t nrows,
     int64_t nflow, const double *__restrict__ x, double *__restrict__ yx,
-    double *__restrict__ tout, const double *vin, double *vout, double c1, double c2) {
+    double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
+    const double *__restrict__ cdiag, int64_t cd0, int64_t cd1) {
     const int lane = threadIdx.x & 63;
     const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (w >= nrows) return;
@@ -177,6 +181,17 @@ __global__ __launch_bounds__(256) void k_extra(
         for (int off = 32; off > 0; off >>= 1) acc[v] += __shfl_xor(acc[v], off, 64);
     if (lane == 0) {
         const int64_t row = rows[w];
+        if constexpr (NV % 2 == 0) {
+            if (cdiag && row >= cd0 && row < cd1) {
+#pragma unroll
+                for (int v = 0; v < NV; v += 2) {
+                    const double dr = cdiag[v], di = cdiag[v + 1];
+                    const double xr = x[row * NV + v], xi = x[row * NV + v + 1];
+                    acc[v] = fma(dr, xr, fma(-di, xi, acc[v]));
+                    acc[v + 1] = fma(dr, xi, fma(di, xr, acc[v + 1]));
+                }
+            }
+        }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             if (row < nflow) {

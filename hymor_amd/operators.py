@@ -389,6 +389,20 @@ def circulant_operator(nchi: int, neta: int, b: int, seed: int = 0, row_sum: flo
     return op, blocks
 
 
+def circulant_spectrum(blocks: dict, nchi: int, neta: int) -> np.ndarray:
+    """All eigenvalues of the block-circulant operator of circulant_operator:
+    the union over the 2-D wavenumbers (p, q) of the eigenvalues of its
+    symbol sum_{(di,dj)} B_{di,dj} exp(2 pi i (di p/nchi + dj q/neta))."""
+    p = np.arange(nchi)
+    q = np.arange(neta)
+    b = next(iter(blocks.values())).shape[0]
+    S = np.zeros((neta, nchi, b, b), dtype=np.complex128)
+    for (di, dj), B in blocks.items():
+        ph = np.exp(2j * np.pi * (di * p[None, :] / nchi + dj * q[:, None] / neta))
+        S += ph[:, :, None, None] * B[None, None, :, :]
+    return np.linalg.eigvals(S.reshape(-1, b, b)).reshape(-1)
+
+
 # Config shapes (grid, block size, extra DOFs) used by tests and the bench.
 CONFIGS = {
     "C1": dict(nchi=100, neta=80, b=4, n_shock=0, seed=1001),

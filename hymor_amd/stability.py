@@ -372,6 +372,7 @@ def EIGENVALUES_GPU_ARNOLDI(A, s, n_modes, *, T_ref=0.5, tol=1e-4, maxiter=None,
     try:
         dev = upload(A, s, scale=dt)
         if krylov == "device":
+            dev.set_graphs(True)      # every matvec repeats the same sweep
             def apply(din, dout):
                 expm_taylor_ptr(dev, din, dout, order=ORDER, nsub=timesteps, complex_=False)
                 dev.synchronize()
@@ -611,9 +612,9 @@ def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=16, ncv=None, tol=1e-10,
     Phase 1: block Krylov-Schur whose every expansion applies T_order(A dt)^n
     to `block` basis vectors in one SpMM sweep, block values stored in
     `value_prec` (FP32 by default) and accumulated in FP64, converged to
-    tol_f32 -- the accuracy the stored values allow.  Phase 2 (refine): the
-    same solver on FP64-stored values, seeded with the phase-1 Ritz block, to
-    tol.  Eigenvalues are mapped as EIGENVALUES_GPU_ARNOLDI does:
+    tol_f32 -- the accuracy the stored values allow.  Phase 2 (refine):
+    single-vector Krylov-Schur on FP64-stored values started from a
+    combination of the phase-1 Ritz vectors, to tol.  Eigenvalues are mapped as EIGENVALUES_GPU_ARNOLDI does:
     lambda = log(mu) / (n dt).  Returns (V, D, info).
     """
     s = CFL_TIMESTEP(s)
@@ -621,7 +622,7 @@ def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=16, ncv=None, tol=1e-10,
     timesteps = int(round(T_ref / dt))
     n = A.shape[0]
     k = n_modes
-    m = ncv or max(3 * block, 2 * k + 2 * block)
+    m = ncv or (k + 9 * block)          # several block steps between restarts
     info = {}
 
     def make_apply(dev):
@@ -633,6 +634,7 @@ def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=16, ncv=None, tol=1e-10,
 
     t0 = time.perf_counter()
     dev = upload(A, s, scale=dt, value_prec=value_prec)
+    dev.set_graphs(True)
     try:
         tol1 = tol_f32 if (refine and value_prec == "f32") else tol
         vals, Y, i1 = block_krylov_schur(make_apply(dev), n, k, m, block, tol=tol1,
@@ -641,15 +643,20 @@ def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=16, ncv=None, tol=1e-10,
         dev.close()
     info["phase1"] = dict(i1, seconds=time.perf_counter() - t0, value_prec=value_prec)
     if refine and value_prec == "f32":
+        # FP64 phase: single-vector Krylov-Schur on FP64-stored values,
+        # started from a random combination of the phase-1 Ritz vectors, so
+        # its Krylov space holds the wanted invariant subspace from the start
         t1 = time.perf_counter()
-        seed = np.concatenate([Y.real, Y.imag], axis=1)
-        q, rr = np.linalg.qr(seed)
-        keep = np.abs(np.diag(rr)) > 1e-10 * np.abs(rr[0, 0])
-        seed = q[:, keep][:, :block]
+        rng = np.random.default_rng(1)
+        v0 = (Y @ rng.standard_normal(Y.shape[1])).real + (Y @ rng.standard_normal(Y.shape[1])).imag
         dev = upload(A, s, scale=dt, value_prec="f64")
+        dev.set_graphs(True)
         try:
-            vals, Y, i2 = block_krylov_schur(make_apply(dev), n, k, m, block, tol=tol,
-                                             maxiter=maxiter, V0=seed)
+            def apply64(din, dout):
+                expm_taylor_ptr(dev, din, dout, order=order, nsub=timesteps, complex_=False)
+                dev.synchronize()
+            vals, Y, i2 = krylov_schur(apply64, n, k, max(20, 2 * k + 1) + k, tol=tol,
+                                       maxiter=maxiter, v0=v0)
         finally:
             dev.close()
         info["phase2"] = dict(i2, seconds=time.perf_counter() - t1, value_prec="f64")

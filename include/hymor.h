@@ -134,6 +134,19 @@ int hymor_expm_taylor_batched_dev(hymor_op *op, int transpose, int order, int64_
                                   int form, int64_t nrhs, const void *d_X, int64_t ldx, void *d_Y,
                                   int64_t ldy, int dtype);
 
+/* Factored receptivity action (no reference counterpart in this form: the
+ * reference applies the extended operator [L repeat(B,1,N); 0
+ * kron(diag(-i*w), I_p)] of LINEARIZE_L_.jl:58-67 to one long vector).  op
+ * is one per-wavenumber system [L B; 0 0] (rows [row0, row1) are the p
+ * freestream DOFs); column c of X (complex, nrhs <= 32) is evolved under
+ * [L B; 0 diag[c] I]: diag holds one complex value per column (-i*w_c), and
+ * its conjugate is used for transpose.  By linearity the N columns together
+ * carry exactly the extended-operator action.  Synchronous. */
+int hymor_expm_taylor_batched_coldiag_dev(hymor_op *op, int transpose, int order, int64_t nsub,
+                                          double dt, int form, int64_t nrhs, const double *diag,
+                                          int64_t row0, int64_t row1, const void *d_X,
+                                          int64_t ldx, void *d_Y, int64_t ldy);
+
 /* Chu-norm sandwich y = T_p(A'dt)^n * C * T_p(A dt)^n * x with the zeroing
  * of CONSTRUCT_C (CONSTRUCT_C.jl:33-58 around EXPONENTIAL! :78-111):
  * x[0:n_set0) is zeroed before and y[0:n_set0) after (var-major indices).
