@@ -275,8 +275,11 @@ def main():
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     y_plain = dy.download()
     dev.close()
-    kname = (f"k_term_split<{cfg['b']}, double, 1, {tuning[1]}, {str(bool(tuning[2])).lower()}>"
-             if tuning[0] else f"k_term<{cfg['b']}, double, 1, {str(bool(tuning[2])).lower()}>")
+    nt_s = str(bool(tuning[2])).lower()
+    kname = {0: f"k_term<{cfg['b']}, double, 1, {nt_s}>",
+             1: f"k_term_split<{cfg['b']}, double, 1, {tuning[1]}, {nt_s}>",
+             2: f"k_term_split_p<{cfg['b']}, double, 1, {tuning[1]}, "
+                f"{9 // tuning[1]}, {nt_s}>"}[tuning[0]]
     traffic, factors = None, None
     if passes is not None:
         traffic, factors = traffic_from_passes(passes, kname)
@@ -328,7 +331,13 @@ def main():
             "kernel": {0: f"k_spmm<{cfg['b']}, float, {nb}>",
                        1: f"k_spmm_mfma<{cfg['b']}, float, {nb}>",
                        2: f"k_spmm_mfma_mc<{cfg['b']}, float, {nb}, U, {2 if nb == 16 else 1}>",
-                       3: f"k_spmm_mfma_mc<{cfg['b']}, float, {nb}, U, 1>"}.get(bkern, "k_spmm"),
+                       3: f"k_spmm_mfma_mc<{cfg['b']}, float, {nb}, U, 1>",
+                       4: f"k_spmm_mfma_w<{cfg['b']}, float, {nb}, "
+                          f"{ {16: 9, 32: 5, 64: 3}.get(nb)}>",
+                       5: f"k_spmm_mfma_w<{cfg['b']}, float, {nb}, "
+                          f"{ {16: 5, 32: 3, 64: 2}.get(nb)}>",
+                       6: f"k_spmm_tile<{cfg['b']}, float, {nb}, "
+                          f"{ {8: 9, 16: 9, 32: 5, 64: 3}.get(nb)}>"}.get(bkern, "k_spmm"),
             "kernel_ms": bk_ms,
             "algorithmic_bytes_per_launch": bk_bytes,
             "achieved_GBps": b_gbs,

@@ -165,9 +165,11 @@ class DeviceOperator:
 
     def set_batch_kernel(self, kernel):
         """Batch kernel: 0 vector FMA, 1 matrix cores, 2/3 matrix cores with
-        two/one cells per wave (default 2).  True/False map to 2/0."""
+        two/one cells per wave, 4/5 matrix cores with grouped operand loads,
+        6 matrix cores on the tile layout (default; also used at 8 doubles
+        per row).  True/False map to 6/0."""
         if kernel is True:
-            kernel = 2
+            kernel = 6
         elif kernel is False:
             kernel = 0
         check(_lib.load().hymor_op_set_batch_kernel(self._h, int(kernel)))

@@ -46,11 +46,14 @@ struct ArgError : std::invalid_argument {
 // Launch variant of the single-vector Taylor-term kernel, chosen per
 // operator by hymor_op_autotune (defaults: k_term, 4 waves per group).
 struct TermTuning {
-    int split = 0;   // 0: k_term (one wave per slice), 1: k_term_split
+    int split = 0;   // 0: k_term (one wave per slice), 1: k_term_split,
+                     // 2: k_term_split_p (grouped loads)
     int waves = 4;   // k_term: waves per workgroup; k_term_split: waves per slice
     int nt = 0;      // non-temporal loads of the block values
-    int spmm_mfma = 2;  // batches of >= 16 doubles per row: 0 vector-FMA k_spmm,
-                        // 1 k_spmm_mfma, 2/3 k_spmm_mfma_mc (2 or 1 cells per wave)
+    int spmm_mfma = 6;  // batches of >= 16 doubles per row: 0 vector-FMA k_spmm,
+                        // 1 k_spmm_mfma, 2/3 k_spmm_mfma_mc (2 or 1 cells per wave),
+                        // 4/5 k_spmm_mfma_w (large / small load groups),
+                        // 6 k_spmm_tile (tile layout, also at 8 doubles per row)
 };
 
 struct DevSliced {
@@ -60,6 +63,7 @@ struct DevSliced {
     int64_t *slice_ptr = nullptr;
     int32_t *col = nullptr;
     void *val = nullptr;
+    void *val_t = nullptr;  // the same values in k_spmm_tile's order (made on first use)
 };
 
 struct DevExtra {
@@ -220,7 +224,19 @@ void release(hymor_op *op) {
     op->stream = nullptr;
 }
 
+void ensure_tile(hymor_op *op);
+int tile_b_of(const hymor_op *op, int nv);
+
+void ensure_ws_only(hymor_op *op, int nv);
+
+// workspace for nv doubles per row, plus the tile-ordered values when the
+// width runs k_spmm_tile
 void ensure_ws(hymor_op *op, int nv) {
+    ensure_ws_only(op, nv);
+    if (tile_b_of(op, nv)) ensure_tile(op);
+}
+
+void ensure_ws_only(hymor_op *op, int nv) {
     if (op->ws_nv >= nv) return;
     drop_graph(op);
     // free the old set (tracked in allocs): rebuild the list without them
@@ -259,6 +275,40 @@ void ensure_ws(hymor_op *op, int nv) {
     op->ws_nv = nv;
 }
 
+// Batched widths of 8+ doubles per row run k_spmm_tile when it is the
+// chosen batch kernel; their internal buffers then use the tile layout for
+// flow rows.  Returns the block size to pass as tile_b, or 0 (row-major).
+int tile_b_of(const hymor_op *op, int nv) {
+    return (nv >= 8 && op->fwd.tune.spmm_mfma == 6) ? op->b : 0;
+}
+
+void tile_vals_of(hymor_op *op, DevSliced &s) {
+    if (s.val_t || s.nnzb_pad == 0) return;
+    const int64_t total = s.nnzb_pad * s.b * s.b;
+    const size_t esz = s.prec == HYMOR_PREC_F32 ? sizeof(float) : sizeof(double);
+    s.val_t = dalloc(op, (size_t)total * esz);
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 65536));
+    if (s.prec == HYMOR_PREC_F32)
+        hipLaunchKernelGGL(k_tile_vals<float>, dim3((unsigned)grid), dim3(256), 0, op->stream,
+                           (const float *)s.val, (float *)s.val_t, total, s.b);
+    else
+        hipLaunchKernelGGL(k_tile_vals<double>, dim3((unsigned)grid), dim3(256), 0, op->stream,
+                           (const double *)s.val, (double *)s.val_t, total, s.b);
+    HIP_TRY(hipGetLastError());
+}
+
+// Tile-ordered values for both directions; called before any launch (or
+// graph capture) of k_spmm_tile.
+void ensure_tile(hymor_op *op) {
+    if (op->ncell == 0) return;
+    const bool fresh = !op->fwd.val_t || (op->has_adj && !op->adj.val_t);
+    if (!fresh) return;
+    drop_graph(op);
+    tile_vals_of(op, op->fwd);
+    if (op->has_adj) tile_vals_of(op, op->adj);
+    HIP_TRY(hipStreamSynchronize(op->stream));
+}
+
 // ---------------------------------------------------------------- launches
 
 template <int B, typename VT, int NV, int NW, bool NT>
@@ -269,10 +319,26 @@ void launch_split(const DevSliced &s, hipStream_t st, const double *x, const dou
                        tout, vin, vout, c1, c2);
 }
 
+// split == 2: k_term_split_p with NW = 3 or 9 waves per slice and slot
+// groups of S = 9 / NW, so one group per wave covers a 3x3-stencil row
+template <int B, typename VT, int NV, int NW, bool NT>
+void launch_split_p(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+                    double *tout, const double *vin, double *vout, double c1, double c2) {
+    constexpr int S = (9 + NW - 1) / NW;
+    hipLaunchKernelGGL((k_term_split_p<B, VT, NV, NW, S, NT>), dim3((unsigned)s.nslice),
+                       dim3(64 * NW), 0, st, s.slice_ptr, s.col, (const VT *)s.val, s.nslice,
+                       s.ncell, x, yx, tout, vin, vout, c1, c2);
+}
+
 template <int B, typename VT, int NV, bool NT>
 void launch_term_nt(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
                     double *tout, const double *vin, double *vout, double c1, double c2) {
     const TermTuning &tu = s.tune;
+    if (tu.split == 2) {
+        if (tu.waves == 9) launch_split_p<B, VT, NV, 9, NT>(s, st, x, yx, tout, vin, vout, c1, c2);
+        else launch_split_p<B, VT, NV, 3, NT>(s, st, x, yx, tout, vin, vout, c1, c2);
+        return;
+    }
     if (tu.split) {
         if (tu.waves == 2) launch_split<B, VT, NV, 2, NT>(s, st, x, yx, tout, vin, vout, c1, c2);
         else launch_split<B, VT, NV, 4, NT>(s, st, x, yx, tout, vin, vout, c1, c2);
@@ -305,6 +371,22 @@ void launch_spmm_t(const DevSliced &s, hipStream_t st, const double *x, const do
                                vin, vout, c1, c2);
             return;
         }
+        if (s.tune.spmm_mfma >= 4) {
+            // 4: all loads of the row's slots in one group (W = 9 covers the
+            // 3x3 stencil) at width 16, fewer at 32/64 to bound registers;
+            // 5: smaller groups
+            constexpr int W4 = R == 16 ? 9 : (R == 32 ? 5 : 3);
+            constexpr int W5 = R == 16 ? 5 : (R == 32 ? 3 : 2);
+            if (s.tune.spmm_mfma == 4)
+                hipLaunchKernelGGL((k_spmm_mfma_w<B, VT, R, W4>), dim3((unsigned)grid),
+                                   dim3(64 * wpb), 0, st, s.slice_ptr, s.col, (const VT *)s.val,
+                                   s.ncell, x, yx, tout, vin, vout, c1, c2);
+            else
+                hipLaunchKernelGGL((k_spmm_mfma_w<B, VT, R, W5>), dim3((unsigned)grid),
+                                   dim3(64 * wpb), 0, st, s.slice_ptr, s.col, (const VT *)s.val,
+                                   s.ncell, x, yx, tout, vin, vout, c1, c2);
+            return;
+        }
         if (s.tune.spmm_mfma >= 2) {
             // 2: two cells per wave at width 16 (one above, to stay within the
             // register file), 3: one cell per wave; slot groups of U
@@ -330,10 +412,33 @@ void launch_spmm_t(const DevSliced &s, hipStream_t st, const double *x, const do
 
 // Kernel choice per (block size, row width): lane-per-row k_term up to 4
 // doubles per row (8 for b = 4), the rhs-parallel k_spmm above.
+template <int B, typename VT, int R>
+void launch_tile_t(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+                   double *tout, const double *vin, double *vout, double c1, double c2) {
+    if (!s.val_t) throw ArgError("tile-ordered values missing (internal)");
+    // slot groups: the 3x3 stencil's 9 slots in one group up to width 16,
+    // fewer at 32 / 64 to bound the registers holding operands in flight
+    constexpr int W = R <= 16 ? 9 : (R == 32 ? 5 : 3);
+    const int wpb = 4;
+    const int64_t grid = (s.ncell + wpb - 1) / wpb;
+    hipLaunchKernelGGL((k_spmm_tile<B, VT, R, W>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
+                       s.slice_ptr, s.col, (const VT *)s.val_t, s.ncell, x, yx, tout, vin, vout,
+                       c1, c2);
+}
+
 template <int B, typename VT>
 void launch_block_part(const DevSliced &s, int nv, hipStream_t st, const double *x,
                        const double *yx, double *tout, const double *vin, double *vout, double c1,
                        double c2) {
+    if (nv >= 8 && s.tune.spmm_mfma == 6) {
+        switch (nv) {
+        case 8: launch_tile_t<B, VT, 8>(s, st, x, yx, tout, vin, vout, c1, c2); return;
+        case 16: launch_tile_t<B, VT, 16>(s, st, x, yx, tout, vin, vout, c1, c2); return;
+        case 32: launch_tile_t<B, VT, 32>(s, st, x, yx, tout, vin, vout, c1, c2); return;
+        case 64: launch_tile_t<B, VT, 64>(s, st, x, yx, tout, vin, vout, c1, c2); return;
+        default: throw ArgError("unsupported vector width");
+        }
+    }
     switch (nv) {
     case 1: launch_term_t<B, VT, 1>(s, st, x, yx, tout, vin, vout, c1, c2); break;
     case 2: launch_term_t<B, VT, 2>(s, st, x, yx, tout, vin, vout, c1, c2); break;
@@ -371,30 +476,30 @@ struct ColDiag {
 template <int NV>
 void launch_extra_nv(const DevExtra &xe, int64_t nflow, hipStream_t st, const double *x,
                      double *yx, double *tout, const double *vin, double *vout, double c1,
-                     double c2, const ColDiag &cd) {
+                     double c2, const ColDiag &cd, int tile_b) {
     const int wpb = 4;
     const int64_t grid = (xe.nrows + wpb - 1) / wpb;
     if (xe.is_complex)
         hipLaunchKernelGGL((k_extra<NV, true>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
                            xe.rows, xe.ptr, xe.col, xe.val, xe.nrows, nflow, x, yx, tout, vin,
-                           vout, c1, c2, cd.d, cd.r0, cd.r1);
+                           vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
     else
         hipLaunchKernelGGL((k_extra<NV, false>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
                            xe.rows, xe.ptr, xe.col, xe.val, xe.nrows, nflow, x, yx, tout, vin,
-                           vout, c1, c2, cd.d, cd.r0, cd.r1);
+                           vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
 }
 
 void launch_extra(const DevExtra &xe, int nv, int64_t nflow, hipStream_t st, const double *x,
                   double *yx, double *tout, const double *vin, double *vout, double c1,
-                  double c2, const ColDiag &cd) {
+                  double c2, const ColDiag &cd, int tile_b) {
     switch (nv) {
-    case 1: launch_extra_nv<1>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd); break;
-    case 2: launch_extra_nv<2>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd); break;
-    case 4: launch_extra_nv<4>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd); break;
-    case 8: launch_extra_nv<8>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd); break;
-    case 16: launch_extra_nv<16>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd); break;
-    case 32: launch_extra_nv<32>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd); break;
-    case 64: launch_extra_nv<64>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd); break;
+    case 1: launch_extra_nv<1>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd, tile_b); break;
+    case 2: launch_extra_nv<2>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd, tile_b); break;
+    case 4: launch_extra_nv<4>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd, tile_b); break;
+    case 8: launch_extra_nv<8>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd, tile_b); break;
+    case 16: launch_extra_nv<16>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd, tile_b); break;
+    case 32: launch_extra_nv<32>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd, tile_b); break;
+    case 64: launch_extra_nv<64>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd, tile_b); break;
     default: throw ArgError("unsupported vector width");
     }
 }
@@ -410,7 +515,8 @@ void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, const double *x
     if (has_x) {
         ColDiag cd;
         if (op->cdiag_on) cd = ColDiag{adj ? op->cdiag_adj : op->cdiag, op->cd0, op->cd1};
-        launch_extra(xe, nv, op->nflow, st, x, yx, tout, vin, vout, c1, c2, cd);
+        launch_extra(xe, nv, op->nflow, st, x, yx, tout, vin, vout, c1, c2, cd,
+                     tile_b_of(op, nv));
         HIP_TRY(hipGetLastError());
     }
     if (op->ncell > 0) {
@@ -422,19 +528,21 @@ void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, const double *x
 // width: doubles per internal row (>= nrhs*ncomp; the rest is zero padding)
 void launch_perm_in(hymor_op *op, hipStream_t st, const double *x_vm, int64_t ld, double *out,
                     int nrhs, int ncomp, int64_t nzero, int width) {
+    const int tile_b = tile_b_of(op, width);
     const int64_t total = op->n * width;
     const int64_t grid = std::min<int64_t>((total + 255) / 256, 65536);
     hipLaunchKernelGGL(k_perm_in, dim3((unsigned)std::max<int64_t>(grid, 1)), dim3(256), 0, st,
-                       x_vm, ld, out, op->n, op->ncell, op->b, nrhs, ncomp, nzero, width);
+                       x_vm, ld, out, op->n, op->ncell, op->b, nrhs, ncomp, nzero, width, tile_b);
     HIP_TRY(hipGetLastError());
 }
 
 void launch_perm_out(hymor_op *op, hipStream_t st, const double *in, double *y_vm, int64_t ld,
                      int nrhs, int ncomp, int64_t nzero, int width) {
+    const int tile_b = tile_b_of(op, width);
     const int64_t total = op->n * nrhs * ncomp;
     const int64_t grid = std::min<int64_t>((total + 255) / 256, 65536);
     hipLaunchKernelGGL(k_perm_out, dim3((unsigned)std::max<int64_t>(grid, 1)), dim3(256), 0, st,
-                       in, y_vm, ld, op->n, op->ncell, op->b, nrhs, ncomp, nzero, width);
+                       in, y_vm, ld, op->n, op->ncell, op->b, nrhs, ncomp, nzero, width, tile_b);
     HIP_TRY(hipGetLastError());
 }
 
@@ -954,13 +1062,16 @@ int hymor_op_autotune(hymor_op *op, int nv, int reps, double *best_ms) {
         ARG_CHECK(reps >= 1, "reps must be >= 1");
         ensure_ws(op, nv);
         const TermTuning cands[] = {{0, 4, 0, 1}, {0, 4, 1, 1}, {0, 2, 0, 1}, {1, 2, 0, 1},
-                                    {1, 2, 1, 1}, {1, 4, 0, 1}, {1, 4, 1, 1}};
+                                    {1, 2, 1, 1}, {1, 4, 0, 1}, {1, 4, 1, 1}, {2, 3, 0, 1},
+                                    {2, 3, 1, 1}, {2, 9, 0, 1}, {2, 9, 1, 1}};
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         double best = 1e300;
         TermTuning pick = op->fwd.tune;
-        for (const TermTuning &c : cands) {
+        const int batch_kernel = op->fwd.tune.spmm_mfma;   // not this autotuner's choice
+        for (TermTuning c : cands) {
+            c.spmm_mfma = batch_kernel;
             op->fwd.tune = c;
             launch_term(op, false, nv, op->stream, op->ws[4], op->ws[5], op->ws[2], op->ws[2], 0.5,
                         1.0);
@@ -998,7 +1109,7 @@ int hymor_op_set_graphs(hymor_op *op, int enable) {
 int hymor_op_set_batch_kernel(hymor_op *op, int use_mfma) {
     return guarded([&] {
         ARG_CHECK(op, "null operator");
-        ARG_CHECK(use_mfma >= 0 && use_mfma <= 3, "batch kernel must be 0..3");
+        ARG_CHECK(use_mfma >= 0 && use_mfma <= 6, "batch kernel must be 0..6");
         op->fwd.tune.spmm_mfma = use_mfma;
         op->adj.tune.spmm_mfma = op->fwd.tune.spmm_mfma;
         drop_graph(op);
@@ -1016,8 +1127,9 @@ int hymor_op_autotune_batch(hymor_op *op, int nv, int reps, int *best_kernel, do
         HIP_TRY(hipEventCreate(&e1));
         double best = 1e300;
         int pick = op->fwd.tune.spmm_mfma;
-        for (int kern = 0; kern <= 3; ++kern) {
+        for (int kern = 0; kern <= 6; ++kern) {
             op->fwd.tune.spmm_mfma = kern;
+            if (kern == 6) ensure_tile(op);
             launch_term(op, false, nv, op->stream, op->ws[4], op->ws[5], op->ws[2], op->ws[2], 0.5,
                         1.0);
             HIP_TRY(hipEventRecord(e0, op->stream));
@@ -1056,8 +1168,10 @@ int hymor_op_set_tuning(hymor_op *op, int split, int waves, int nt) {
     return guarded([&] {
         ARG_CHECK(op, "null operator");
         ARG_CHECK((split == 0 && (waves == 1 || waves == 2 || waves == 4)) ||
-                      (split == 1 && (waves == 2 || waves == 4)),
-                  "k_term takes 1, 2 or 4 waves per group; k_term_split 2 or 4 waves per slice");
+                      (split == 1 && (waves == 2 || waves == 4)) ||
+                      (split == 2 && (waves == 3 || waves == 9)),
+                  "k_term takes 1, 2 or 4 waves per group, k_term_split 2 or 4 waves per "
+                  "slice, k_term_split_p 3 or 9");
         ARG_CHECK(nt == 0 || nt == 1, "nt must be 0 or 1");
         op->fwd.tune.split = split;
         op->fwd.tune.waves = waves;

@@ -32,6 +32,31 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t bid, int64_t nwg) {
     return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
+// ---------------------------------------------------------------- tile layout
+// Batched rows of R doubles (R = 8, 16, 32, 64) can be kept in the matrix
+// cores' operand order instead of row-major: inside a cell's B*R doubles,
+// variable var = 4*kc + kk and column j = W*nt + n (W = min(R, 16)) sit at
+//     ((kk*NT + nt)*W + n)*KC + kc          (KC = B/4, NT = R/W),
+// so the KC values a lane (n, kk) feeds to (or receives from) one
+// v_mfma_f64_16x16x4f64 per k-chunk are adjacent: one 16-byte access per
+// lane instead of KC 8-byte ones.  Extra-DOF rows (row >= nflow) stay
+// row-major.
+__device__ __forceinline__ int64_t tile_pos(int64_t row, int j, int b, int r) {
+    const int w = r < 16 ? r : 16;
+    const int nt_ = r / w, kc_ = b >> 2;
+    const int64_t cell = row / b;
+    const int var = (int)(row - cell * b);
+    const int kk = var & 3, kc = var >> 2;
+    const int nt = j / w, n = j - nt * w;
+    return cell * b * r + ((int64_t)(kk * nt_ + nt) * w + n) * kc_ + kc;
+}
+
+// position of (row, j) in an internal batched buffer of width r; tile_b = 0
+// selects row-major, else the tile layout above with b = tile_b
+__device__ __forceinline__ int64_t buf_pos(int64_t row, int j, int r, int tile_b, int64_t nflow) {
+    return (tile_b && row < nflow) ? tile_pos(row, j, tile_b, r) : row * r + j;
+}
+
 template <int N>
 __device__ __forceinline__ void load_f64(const double *__restrict__ p, double *a) {
     static_assert(N % 2 == 0, "pairs");
@@ -151,7 +176,7 @@ __global__ __launch_bounds__(256) void k_extra(
     const int32_t *__restrict__ col, const double *__restrict__ val, int64_t nrows,
     int64_t nflow, const double *__restrict__ x, double *__restrict__ yx,
     double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
-    const double *__restrict__ cdiag, int64_t cd0, int64_t cd1) {
+    const double *__restrict__ cdiag, int64_t cd0, int64_t cd1, int tile_b) {
     const int lane = threadIdx.x & 63;
     const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (w >= nrows) return;
@@ -165,14 +190,15 @@ __global__ __launch_bounds__(256) void k_extra(
             const double ar = val[2 * e], ai = val[2 * e + 1];
 #pragma unroll
             for (int v = 0; v < NV; v += 2) {
-                const double xr = x[c * NV + v], xi = x[c * NV + v + 1];
+                const double xr = x[buf_pos(c, v, NV, tile_b, nflow)];
+                const double xi = x[buf_pos(c, v + 1, NV, tile_b, nflow)];
                 acc[v] = fma(ar, xr, fma(-ai, xi, acc[v]));
                 acc[v + 1] = fma(ar, xi, fma(ai, xr, acc[v + 1]));
             }
         } else {
             const double a = val[e];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) acc[v] = fma(a, x[c * NV + v], acc[v]);
+            for (int v = 0; v < NV; ++v) acc[v] = fma(a, x[buf_pos(c, v, NV, tile_b, nflow)], acc[v]);
         }
     }
 #pragma unroll
@@ -195,7 +221,7 @@ __global__ __launch_bounds__(256) void k_extra(
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             if (row < nflow) {
-                yx[row * NV + v] = acc[v];
+                yx[buf_pos(row, v, NV, tile_b, nflow)] = acc[v];
             } else {
                 const double t = c1 * acc[v];
                 if (tout) tout[row * NV + v] = t;
@@ -210,10 +236,10 @@ __global__ __launch_bounds__(256) void k_extra(
 // doubles.  Internal rows hold nrhs*ncomp doubles.  Var-major indices below
 // nzero read as 0 (CONSTRUCT_C's x[1:N_set_0] .= 0).
 // Internal rows are `width` doubles wide; components past nrhs*ncomp are
-// padding and read as 0.
+// padding and read as 0.  tile_b != 0: flow rows in the tile layout.
 __global__ void k_perm_in(const double *__restrict__ x_vm, int64_t ld, double *__restrict__ out,
                           int64_t n, int64_t ncell, int b, int nrhs, int ncomp, int64_t nzero,
-                          int width) {
+                          int width, int tile_b) {
     const int64_t valid = (int64_t)nrhs * ncomp;
     const int64_t total = n * width;
     const int64_t nflow = ncell * b;
@@ -221,19 +247,20 @@ __global__ void k_perm_in(const double *__restrict__ x_vm, int64_t ld, double *_
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t row = i / width;
         const int64_t rem = i - row * width;
+        const int64_t o = buf_pos(row, (int)rem, width, tile_b, nflow);
         if (rem >= valid) {
-            out[i] = 0.0;
+            out[o] = 0.0;
             continue;
         }
         const int64_t rhs = rem / ncomp, comp = rem - rhs * ncomp;
         const int64_t vm = row < nflow ? (row % b) * ncell + row / b : row;
-        out[i] = vm < nzero ? 0.0 : x_vm[(rhs * ld + vm) * ncomp + comp];
+        out[o] = vm < nzero ? 0.0 : x_vm[(rhs * ld + vm) * ncomp + comp];
     }
 }
 
 __global__ void k_perm_out(const double *__restrict__ in, double *__restrict__ y_vm, int64_t ld,
                            int64_t n, int64_t ncell, int b, int nrhs, int ncomp, int64_t nzero,
-                           int width) {
+                           int width, int tile_b) {
     const int64_t total = n * nrhs * ncomp;
     const int64_t nflow = ncell * b;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
@@ -243,7 +270,7 @@ __global__ void k_perm_out(const double *__restrict__ in, double *__restrict__ y
         const int64_t vm = rem / ncomp, comp = rem - vm * ncomp;
         const int64_t row = vm < nflow ? (vm % ncell) * b + vm / ncell : vm;
         y_vm[(rhs * ld + vm) * ncomp + comp] =
-            vm < nzero ? 0.0 : in[row * width + rhs * ncomp + comp];
+            vm < nzero ? 0.0 : in[buf_pos(row, (int)(rhs * ncomp + comp), width, tile_b, nflow)];
     }
 }
 
@@ -594,6 +621,125 @@ __global__ __launch_bounds__(64 * NW) void k_term_split(
     }
 }
 
+// k_term_split with the loads of each wave grouped: the block columns of
+// the wave's next S slots and their values are issued together, then the x
+// entries they select, and wave 0 issues its epilogue operands (vin, yx)
+// before the slot loop -- two waits on memory per group instead of two per
+// slot pair plus one at the end.  With NW * S = 9 a 3x3-stencil row has no
+// idle slot; an idle slot (shorter or longer rows) contributes exact zeros.
+// Same per-wave summation order and LDS combination as k_term_split.
+template <typename VT>
+struct Chunk;
+template <>
+struct Chunk<double> {
+    typedef d2 type;
+};
+template <>
+struct Chunk<float> {
+    typedef f4 type;
+};
+
+template <int B, typename VT, int NV, int NW, int S, bool NT>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_term_split_p(
+    const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
+    const VT *__restrict__ val, int64_t nslice, int64_t ncell,
+    const double *__restrict__ x, const double *__restrict__ yx,
+    double *__restrict__ tout, const double *vin, double *vout, double c1, double c2) {
+    constexpr int CPS = 64 / B;
+    constexpr int BB = B * B;
+    constexpr int V = Vpl<VT>::value;
+    constexpr int NC = B / V;  // 16-byte chunks per lane and slot
+    typedef typename Chunk<VT>::type CT;
+    __shared__ double part[NW > 1 ? NW - 1 : 1][NV][64];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int64_t slice = xcd_tile(blockIdx.x, gridDim.x);
+    if (slice >= nslice) return;
+    const int q = lane / B, r = lane % B;
+    const int64_t p0 = slice_ptr[slice];
+    const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
+    const int32_t *cp = col + p0 + q;
+    const VT *vp = val + p0 * BB;
+    const int64_t cell = slice * CPS + q;
+    const int64_t row = (cell < ncell ? cell : ncell - 1) * B + r;
+    double pre_v[NV], pre_y[NV];
+    {
+        const double *pv = vout ? vin : x;
+        const double *py = yx ? yx : x;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            pre_v[v] = w == 0 ? pv[row * NV + v] : 0.0;
+            pre_y[v] = w == 0 ? py[row * NV + v] : 0.0;
+        }
+    }
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+    // slots t = w, w + NW, ... of this wave, S at a time
+    const int nmine = width > w ? (width - w + NW - 1) / NW : 0;
+    for (int g = 0; g < nmine; g += S) {
+        int32_t c[S];
+        CT raw[S][NC];
+        // columns first: the x loads wait for them, not for the values
+        // (the memory counter retires loads in issue order), and no load
+        // sits behind a branch, so no wait is placed before the x loads
+        // beyond the columns'.  A slot past the wave's last one re-reads
+        // that slot (cache hits) and is zeroed before the product.
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const int i = g + u < nmine ? g + u : nmine - 1;
+            c[u] = cp[(int64_t)(w + i * NW) * CPS];
+        }
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const int i = g + u < nmine ? g + u : nmine - 1;
+            const VT *base = vp + (int64_t)(w + i * NW) * CPS * BB;
+#pragma unroll
+            for (int kc = 0; kc < NC; ++kc) {
+                const CT *pc = reinterpret_cast<const CT *>(base + ((int64_t)kc * 64 + lane) * V);
+                raw[u][kc] = NT ? __builtin_nontemporal_load(pc) : *pc;
+            }
+        }
+        double xv[S][B * NV];
+#pragma unroll
+        for (int u = 0; u < S; ++u) load_f64<B * NV>(x + (int64_t)c[u] * (B * NV), xv[u]);
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const bool live = g + u < nmine;
+#pragma unroll
+            for (int kc = 0; kc < NC; ++kc)
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    const int k = kc * V + e;
+                    const double ak = live ? (double)raw[u][kc][e] : 0.0;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) acc[v] = fma(ak, xv[u][k * NV + v], acc[v]);
+                }
+        }
+    }
+    if constexpr (NW > 1) {
+        if (w > 0) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) part[w - 1][v][lane] = acc[v];
+        }
+        __syncthreads();
+        if (w > 0) return;
+#pragma unroll
+        for (int i = 0; i < NW - 1; ++i)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] += part[i][v][lane];
+    }
+    if (cell >= ncell) return;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double y = acc[v];
+        if (yx) y += pre_y[v];
+        const double t = c1 * y;
+        if (tout) tout[row * NV + v] = t;
+        if (vout) vout[row * NV + v] = fma(c2, t, pre_v[v]);
+    }
+}
+
 }  // namespace hymor
 
 namespace hymor {
@@ -780,6 +926,261 @@ __global__ __launch_bounds__(256) void k_spmm_mfma_mc(
                 if (tout) tout[idx] = t;
                 if (vout) vout[idx] = fma(c2, t, pre_v[ci][nt][r]);
             }
+    }
+}
+
+// k_spmm_mfma with every operand load of a group of W slots issued before
+// the first product and no branch around any load, so a wave waits for
+// memory once per group instead of once per operand.  Rows m >= B of the
+// 16-row tile read a duplicate row (m mod B) and slots past the row's width
+// read the row's last slot; both are zeroed by a select before the MFMA.
+// Block values are streamed with non-temporal loads (read once per launch);
+// x, re-read by the neighbouring cells, stays cacheable.  Same arithmetic,
+// operand layout and summation order as k_spmm_mfma (a zeroed slot adds
+// exact zeros).
+template <int B, typename VT, int R, int W>
+__global__ __launch_bounds__(256) void k_spmm_mfma_w(
+    const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
+    const VT *__restrict__ val, int64_t ncell, const double *__restrict__ x,
+    const double *__restrict__ yx, double *__restrict__ tout, const double *vin, double *vout,
+    double c1, double c2) {
+    constexpr int CPS = 64 / B;
+    constexpr int BB = B * B;
+    constexpr int NT = R / 16;
+    constexpr int KC = B / 4;
+    constexpr int V = Vpl<VT>::value;
+    static_assert(R % 16 == 0 && (B == 4 || B == 8), "unsupported B/R");
+    const int lane = threadIdx.x & 63;
+    const int64_t wpb = blockDim.x >> 6;
+    const int64_t cell = (int64_t)__builtin_amdgcn_readfirstlane(
+        (int)(xcd_tile(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6)));
+    if (cell >= ncell) return;
+    const int m = lane & 15, kk = lane >> 4;
+    const int mr = m & (B - 1);
+    const bool live_row = m < B;
+    const int64_t slice = cell / CPS, q = cell - slice * CPS;
+    const int64_t p0 = slice_ptr[slice];
+    const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
+    // epilogue operands, read from a valid array when the caller passes none
+    const double *pv = vout ? vin : x;
+    const double *py = yx ? yx : x;
+    double pre_v[NT][KC], pre_y[NT][KC];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int r = 0; r < KC; ++r) {
+            const int64_t idx = (cell * B + kk + 4 * r) * R + nt * 16 + m;
+            pre_v[nt][r] = pv[idx];
+            pre_y[nt][r] = py[idx];
+        }
+    v4d acc[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[nt] = v4d{0.0, 0.0, 0.0, 0.0};
+    static_assert(W <= 64, "one column per lane");
+    for (int t0 = 0; t0 < width; t0 += W) {
+        // the group's block columns with one vector load (lane u < W reads
+        // slot t0 + u), broadcast by readlane: one wait instead of one per
+        // scalar load, whose out-of-order returns force a full wait each
+        const int tl = t0 + (lane < W ? lane : 0);
+        const int32_t cl = col[p0 + (int64_t)(tl < width ? tl : width - 1) * CPS + q];
+        int32_t c[W];
+#pragma unroll
+        for (int u = 0; u < W; ++u) c[u] = __builtin_amdgcn_readlane(cl, u);
+        VT araw[W][KC];
+        double bx[W][KC][NT];
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            const int t = t0 + u < width ? t0 + u : width - 1;
+            const VT *sb = val + (p0 + (int64_t)t * CPS) * BB;
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) {
+                const int k = 4 * kc + kk;
+                araw[u][kc] = __builtin_nontemporal_load(
+                    sb + ((int64_t)(k / V) * 64 + q * B + mr) * V + k % V);
+                const double *xp = x + ((int64_t)c[u] * B + k) * R + m;
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) bx[u][kc][nt] = xp[nt * 16];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            const bool live = live_row && (t0 + u < width);
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) {
+                const double a = live ? (double)araw[u][kc] : 0.0;
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bx[u][kc][nt], acc[nt], 0,
+                                                                   0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int r = 0; r < KC; ++r) {
+            const int64_t idx = (cell * B + kk + 4 * r) * R + nt * 16 + m;
+            const double y = acc[nt][r] + (yx ? pre_y[nt][r] : 0.0);
+            const double t = c1 * y;
+            if (tout) tout[idx] = t;
+            if (vout) vout[idx] = fma(c2, t, pre_v[nt][r]);
+        }
+}
+
+// Block values in the tile kernel's order: inside each block, value (m, k)
+// with k = 4*kc + kk at (m*4 + kk)*KC + kc (a lane's KC values adjacent).
+// Rewritten on the device from the lane-interleaved layout of the same
+// slots; one thread per value.
+template <typename VT>
+__global__ void k_tile_vals(const VT *__restrict__ src, VT *__restrict__ dst, int64_t total,
+                            int b) {
+    constexpr int V = Vpl<VT>::value;
+    const int cps = 64 / b, bb = b * b, kc_ = b >> 2;
+    const int64_t slot_len = (int64_t)cps * bb;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t slot = i / slot_len;
+        const int within = (int)(i - slot * slot_len);
+        const int q = within / bb, rem = within - q * bb;
+        const int kc = rem % kc_, tmp = rem / kc_;
+        const int kk = tmp & 3, m = tmp >> 2;
+        const int k = kk + 4 * kc;
+        dst[i] = src[slot * slot_len + ((int64_t)(k / V) * 64 + q * b + m) * V + k % V];
+    }
+}
+
+template <typename VT, int KC>
+struct TileVals;
+template <>
+struct TileVals<float, 2> {
+    typedef float __attribute__((ext_vector_type(2))) type;  // f2
+};
+template <>
+struct TileVals<double, 2> {
+    typedef d2 type;
+};
+template <typename VT>
+struct TileVals<VT, 1> {
+    typedef VT type;
+};
+
+template <int KC>
+struct TileX;
+template <>
+struct TileX<2> {
+    typedef d2 type;
+};
+template <>
+struct TileX<1> {
+    typedef double type;
+};
+
+typedef float f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double lane_elem(double v, int) { return v; }
+__device__ __forceinline__ double lane_elem(float v, int) { return (double)v; }
+__device__ __forceinline__ double lane_elem(const d2 &v, int i) { return v[i]; }
+__device__ __forceinline__ double lane_elem(const f2 &v, int i) { return (double)v[i]; }
+
+// Batched Taylor term on the FP64 matrix cores with x, yx, vin, tout, vout
+// in the tile layout and block values in TileVals order: per slot one
+// value load and NT x loads per lane (16 bytes each at B = 8), the
+// operands of KC x NT v_mfma_f64_16x16x4f64; the epilogue reads and writes
+// 16 bytes per lane.  One cell per wave; the group's W block columns come
+// from one vector load.  For R = 8 lanes n >= 8 duplicate columns n - 8 and
+// their results are dropped; rows m >= B read row m mod B and are zeroed.
+// Summation order: slots ascending, k-chunks ascending (as k_spmm_mfma).
+template <int B, typename VT, int R, int W>
+__global__ __launch_bounds__(256) void k_spmm_tile(
+    const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
+    const VT *__restrict__ valt, int64_t ncell, const double *__restrict__ x,
+    const double *__restrict__ yx, double *__restrict__ tout, const double *vin, double *vout,
+    double c1, double c2) {
+    constexpr int CPS = 64 / B;
+    constexpr int BB = B * B;
+    constexpr int WN = R < 16 ? R : 16;
+    constexpr int NT = R / WN;
+    constexpr int KC = B / 4;
+    typedef typename TileVals<VT, KC>::type AT;
+    typedef typename TileX<KC>::type XT;
+    static_assert(B == 4 || B == 8, "block size 4 or 8");
+    static_assert(R == 8 || R == 16 || R == 32 || R == 64, "width 8..64");
+    static_assert(W <= 64, "one column per lane");
+    const int lane = threadIdx.x & 63;
+    const int64_t wpb = blockDim.x >> 6;
+    const int64_t cell = (int64_t)__builtin_amdgcn_readfirstlane(
+        (int)(xcd_tile(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6)));
+    if (cell >= ncell) return;
+    const int n = lane & 15, kk = lane >> 4;
+    const int nr = n & (WN - 1);                  // column this lane reads
+    const int mr = n & (B - 1);                   // block row this lane reads
+    const bool live_row = n < B;
+    const int64_t slice = cell / CPS, q = cell - slice * CPS;
+    const int64_t p0 = slice_ptr[slice];
+    const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
+    const int64_t cbase = cell * B * R;
+    // epilogue operands (a valid array stands in for a missing one)
+    const double *pv = vout ? vin : x;
+    const double *py = yx ? yx : x;
+    XT pre_v[NT], pre_y[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        const int64_t o = cbase + ((int64_t)(kk * NT + nt) * WN + nr) * KC;
+        pre_v[nt] = *reinterpret_cast<const XT *>(pv + o);
+        pre_y[nt] = *reinterpret_cast<const XT *>(py + o);
+    }
+    v4d acc[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[nt] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int t0 = 0; t0 < width; t0 += W) {
+        const int tl = t0 + (lane < W ? lane : 0);
+        const int32_t cl = col[p0 + (int64_t)(tl < width ? tl : width - 1) * CPS + q];
+        int32_t c[W];
+#pragma unroll
+        for (int u = 0; u < W; ++u) c[u] = __builtin_amdgcn_readlane(cl, u);
+        AT araw[W];
+        XT bx[W][NT];
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            const int t = t0 + u < width ? t0 + u : width - 1;
+            const VT *sb = valt + (p0 + (int64_t)t * CPS + q) * BB + (mr * 4 + kk) * KC;
+            araw[u] = __builtin_nontemporal_load(reinterpret_cast<const AT *>(sb));
+            const double *xp = x + (int64_t)c[u] * B * R + ((int64_t)kk * NT * WN + nr) * KC;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                bx[u][nt] = *reinterpret_cast<const XT *>(xp + (int64_t)nt * WN * KC);
+        }
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            const bool live = live_row && (t0 + u < width);
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) {
+                const double a = live ? lane_elem(araw[u], kc) : 0.0;
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, lane_elem(bx[u][nt], kc),
+                                                                   acc[nt], 0, 0, 0);
+            }
+        }
+    }
+    if (n >= WN) return;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        const int64_t o = cbase + ((int64_t)(kk * NT + nt) * WN + n) * KC;
+        XT tv, vv;
+#pragma unroll
+        for (int r = 0; r < KC; ++r) {
+            const double y = acc[nt][r] + (yx ? lane_elem(pre_y[nt], r) : 0.0);
+            const double t = c1 * y;
+            if constexpr (KC == 1) {
+                tv = t;
+                vv = fma(c2, t, lane_elem(pre_v[nt], 0));
+            } else {
+                tv[r] = t;
+                vv[r] = fma(c2, t, lane_elem(pre_v[nt], r));
+            }
+        }
+        if (tout) *reinterpret_cast<XT *>(tout + o) = tv;
+        if (vout) *reinterpret_cast<XT *>(vout + o) = vv;
     }
 }
 

@@ -348,6 +348,58 @@ def receptivity_extension(op: BlockOperator, w_infty, seed: int = 0, b_scale: fl
                          np.concatenate(xv))
 
 
+def marching_operator(nchi: int, neta: int, b: int, seed: int = 0, n_lead: int = 12,
+                      lead_top: float = 0.55, lead_gap: float = 0.1, bulk=(-3.0, -1.0),
+                      couple: float = 0.15):
+    """3x3-stencil block operator with an exactly known spectrum.
+
+    Every cell couples only to the neighbours with a larger cell index (east,
+    and the three cells of the next eta row): a space-marching, one-way
+    discretisation.  The blocks towards the other four neighbours are kept in
+    the pattern as explicit zeros, so storage, bytes and kernel work are those
+    of the full 9-block stencil.  The operator is block upper triangular in
+    cell order, so its eigenvalues are exactly those of the diagonal blocks.
+    Diagonal block i is S_i diag(l_i) S_i^-1 with S_i = I + E_i, E_i = 0.2 N(0,1)
+    scaled to ||E_i||_F <= 0.5 (so cond(S_i) <= 3), and real eigenvalues l_i
+    uniform in `bulk`, except `n_lead` cells spread over the
+    grid whose leading eigenvalue is lead_top - lead_gap * j (j = 0..n_lead-1).
+    Returns (op, eigenvalues of A) with the eigenvalues computed from the
+    factors (l_i), not from the stored blocks.
+    """
+    rng = np.random.Generator(np.random.PCG64(20_000 + seed))
+    ncell = nchi * neta
+    ci = np.arange(ncell, dtype=np.int64) % nchi
+    cj = np.arange(ncell, dtype=np.int64) // nchi
+    rows, cols, live = [], [], []
+    for di, dj in _OFFSETS:
+        ni, nj = ci + di, cj + dj
+        ok = (ni >= 0) & (ni < nchi) & (nj >= 0) & (nj < neta)
+        r = np.nonzero(ok)[0]
+        c = (ni + nj * nchi)[ok]
+        rows.append(r)
+        cols.append(c)
+        live.append(c > r)
+    rows, cols, live = (np.concatenate(v) for v in (rows, cols, live))
+    order = np.lexsort((cols, rows))
+    rows, cols, live = rows[order], cols[order], live[order]
+    bval = np.zeros((rows.shape[0], b, b))
+    bval[live] = rng.standard_normal((int(live.sum()), b, b)) * (couple / np.sqrt(b))
+    lam = rng.uniform(bulk[0], bulk[1], size=(ncell, b))
+    lead_cells = np.linspace(0, ncell - 1, n_lead + 2).astype(np.int64)[1:-1]
+    lam[lead_cells, 0] = lead_top - lead_gap * np.arange(n_lead)
+    E = 0.2 * rng.standard_normal((ncell, b, b))
+    fro = np.sqrt((E * E).sum(axis=(1, 2)))
+    E *= np.minimum(1.0, 0.5 / fro)[:, None, None]     # ||E||_2 <= 0.5: cond(S) <= 3
+    S = np.eye(b)[None] + E
+    D = np.einsum("cij,cj,cjk->cik", S, lam, np.linalg.inv(S))
+    diag_pos = np.nonzero(rows == cols)[0]
+    bval[diag_pos] = D[rows[diag_pos]]
+    brow_ptr = np.zeros(ncell + 1, dtype=np.int64)
+    np.cumsum(np.bincount(rows, minlength=ncell), out=brow_ptr[1:])
+    op = BlockOperator(nchi, neta, b, 0, brow_ptr, cols.astype(np.int32), bval)
+    return op, lam.reshape(-1)
+
+
 def circulant_operator(nchi: int, neta: int, b: int, seed: int = 0, row_sum: float | None = None):
     """Doubly periodic, constant-coefficient 9-cell stencil.
 

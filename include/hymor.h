@@ -189,15 +189,20 @@ int hymor_op_set_graphs(hymor_op *op, int enable);
 
 /* Kernel for batches of 16+ doubles per row: 0 vector FMA (k_spmm),
  * 1 FP64 matrix cores (k_spmm_mfma), 2 / 3 matrix cores with two / one
- * cells per wave and prefetched epilogue operands (k_spmm_mfma_mc, the
- * default is 2).  All accumulate in FP64. */
+ * cells per wave and prefetched epilogue operands (k_spmm_mfma_mc),
+ * 4 / 5 matrix cores with all operand loads of a large / small group of
+ * slots issued together (k_spmm_mfma_w), 6 matrix cores with the batch
+ * held in the operand ("tile") order, 16-byte accesses per lane
+ * (k_spmm_tile, the default; it also serves 8 doubles per row).  All
+ * accumulate in FP64. */
 int hymor_op_set_batch_kernel(hymor_op *op, int use_mfma);
 
-/* Times the four batch kernels at width nv (16, 32 or 64) and keeps the
+/* Times the seven batch kernels at width nv (16, 32 or 64) and keeps the
  * fastest; returns its id and ms per launch. */
 int hymor_op_autotune_batch(hymor_op *op, int nv, int reps, int *best_kernel, double *best_ms);
 
-/* Read / force the chosen variant (split 0|1, waves, nt 0|1). */
+/* Read / force the chosen variant (split 0 k_term | 1 k_term_split |
+ * 2 k_term_split_p, waves, nt 0|1). */
 int hymor_op_get_tuning(const hymor_op *op, int *split, int *waves, int *nt);
 int hymor_op_set_tuning(hymor_op *op, int split, int waves, int nt);
 

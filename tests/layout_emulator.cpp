@@ -239,3 +239,111 @@ extern "C" int emu_spmm_mfma_csc(int64_t n, const int64_t *colptr, const int64_t
     }
     return g_oob;
 }
+
+// Tile path: k_perm_in / k_perm_out with tile_b = b, k_tile_vals, and
+// k_spmm_tile's per-lane operand fetches (KC adjacent values per lane),
+// MFMA operand placement and epilogue stores, for R = 8, 16, 32, 64.
+namespace {
+int64_t tile_pos_h(int64_t row, int j, int b, int r) {
+    const int w = r < 16 ? r : 16;
+    const int nt_ = r / w, kc_ = b >> 2;
+    const int64_t cell = row / b;
+    const int var = (int)(row - cell * b);
+    const int kk = var & 3, kc = var >> 2;
+    const int nt = j / w, n = j - nt * w;
+    return cell * b * r + ((int64_t)(kk * nt_ + nt) * w + n) * kc_ + kc;
+}
+}  // namespace
+
+extern "C" int emu_spmm_tile_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                                 const double *nzval, int b, int64_t ncell, int nrhs, int R,
+                                 int vpl, const double *X_vm, double *Y_vm) {
+    g_oob = 0;
+    HostBSR fwd, adj;
+    HostExtra xf, xa;
+    csc_to_blocks(n, colptr, rowval, nzval, 1, false, 1.0, b, ncell, fwd, adj, xf, xa);
+    if (!xf.rows.empty()) return -1;
+    HostSliced s;
+    bsr_to_sliced(fwd, s);
+    interleave_lanes(s, vpl);
+    const int64_t nflow = (int64_t)b * ncell;
+    // k_perm_in (tile layout for flow rows)
+    std::vector<double> x((size_t)n * R, -1e300), y((size_t)n * R, 0.0);
+    for (int64_t i = 0; i < n * R; ++i) {
+        const int64_t row = i / R, rem = i % R;
+        const int64_t o = row < nflow ? tile_pos_h(row, (int)rem, b, R) : row * R + rem;
+        if (o < 0 || o >= (int64_t)x.size()) { ++g_oob; continue; }
+        if (rem >= nrhs) { x[(size_t)o] = 0.0; continue; }
+        const int64_t vm = row < nflow ? (row % b) * ncell + row / b : row;
+        x[(size_t)o] = X_vm[rem * n + vm];
+    }
+    // k_tile_vals
+    const int CPS = 64 / b, BB = b * b, KC = b / 4;
+    const int64_t slot_len = (int64_t)CPS * BB;
+    const int64_t total = (int64_t)s.val.size();
+    std::vector<double> valt((size_t)total, 0.0);
+    for (int64_t i = 0; i < total; ++i) {
+        const int64_t slot = i / slot_len;
+        const int within = (int)(i - slot * slot_len);
+        const int q = within / BB, rem = within - q * BB;
+        const int kc = rem % KC, tmp = rem / KC;
+        const int kk = tmp & 3, m = tmp >> 2;
+        const int k = kk + 4 * kc;
+        valt[(size_t)i] = at(s.val, slot * slot_len + ((int64_t)(k / vpl) * 64 + q * b + m) * vpl + k % vpl);
+    }
+    const int WN = R < 16 ? R : 16, NT = R / WN;
+    for (int64_t cell = 0; cell < ncell; ++cell) {
+        const int64_t slice = cell / CPS, q = cell - slice * CPS;
+        const int64_t p0 = at(s.slice_ptr, slice);
+        const int64_t width = (at(s.slice_ptr, slice + 1) - p0) / CPS;
+        const int64_t cbase = cell * b * R;
+        std::vector<double> D((size_t)NT * 256, 0.0);  // D[nt][m][n]
+        for (int64_t t = 0; t < width; ++t) {
+            const int64_t c = at(s.col, p0 + t * CPS + q);
+            // per lane: KC adjacent values and NT x KC adjacent x entries
+            double opa[64][2], opb[4][64][2];
+            for (int lane = 0; lane < 64; ++lane) {
+                const int nn = lane & 15, kk = lane >> 4;
+                const int nr = nn & (WN - 1), mr = nn & (b - 1);
+                const int64_t sb = (p0 + t * CPS + q) * BB + (mr * 4 + kk) * KC;
+                const int64_t xb = c * b * R + ((int64_t)kk * NT * WN + nr) * KC;
+                for (int kc = 0; kc < KC; ++kc) {
+                    opa[lane][kc] = nn < b ? at(valt, sb + kc) : 0.0;
+                    for (int nt = 0; nt < NT; ++nt)
+                        opb[nt][lane][kc] = at(x, xb + (int64_t)nt * WN * KC + kc);
+                }
+            }
+            for (int kc = 0; kc < KC; ++kc)
+                for (int nt = 0; nt < NT; ++nt)
+                    for (int mm = 0; mm < 16; ++mm)
+                        for (int nn = 0; nn < 16; ++nn) {
+                            double sacc = 0.0;
+                            // A[m][k] from lane m + 16k, B[k][n] from lane n + 16k
+                            for (int k = 0; k < 4; ++k)
+                                sacc += opa[mm + 16 * k][kc] * opb[nt][nn + 16 * k][kc];
+                            D[(size_t)nt * 256 + mm * 16 + nn] += sacc;
+                        }
+        }
+        for (int lane = 0; lane < 64; ++lane) {
+            const int nn = lane & 15, kk = lane >> 4;
+            if (nn >= WN) continue;
+            for (int nt = 0; nt < NT; ++nt) {
+                const int64_t o = cbase + ((int64_t)(kk * NT + nt) * WN + nn) * KC;
+                for (int r = 0; r < KC; ++r) {
+                    // register r of lane n + 16 kk holds D[kk + 4r][n]
+                    const double v = D[(size_t)nt * 256 + (kk + 4 * r) * 16 + nn];
+                    if (o + r < 0 || o + r >= (int64_t)y.size()) ++g_oob;
+                    else y[(size_t)(o + r)] = v;
+                }
+            }
+        }
+    }
+    // k_perm_out
+    for (int64_t i = 0; i < n * nrhs; ++i) {
+        const int64_t rhs = i / n, vm = i % n;
+        const int64_t row = vm < nflow ? (vm % ncell) * b + vm / ncell : vm;
+        const int64_t o = row < nflow ? tile_pos_h(row, (int)rhs, b, R) : row * R + rhs;
+        Y_vm[rhs * n + vm] = at(y, o);
+    }
+    return g_oob;
+}

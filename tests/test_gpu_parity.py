@@ -287,8 +287,11 @@ def test_forward_after_adjoint_with_extras(orc):
             assert relerr(got, orc.power_method(At.scaled(0.1), x, 3)) <= 1e-13
 
 
-@pytest.mark.parametrize("split,waves,nt", [(0, 1, 0), (0, 2, 1), (0, 4, 0), (0, 4, 1),
-                                            (1, 2, 0), (1, 2, 1), (1, 4, 0), (1, 4, 1)])
+VARIANTS = [(0, 1, 0), (0, 2, 1), (0, 4, 0), (0, 4, 1), (1, 2, 0), (1, 2, 1), (1, 4, 0),
+            (1, 4, 1), (2, 3, 0), (2, 3, 1), (2, 9, 0), (2, 9, 1)]
+
+
+@pytest.mark.parametrize("split,waves,nt", VARIANTS)
 def test_launch_variants_agree(orc, split, waves, nt):
     """Every Taylor-term launch variant the autotuner may pick matches the
     oracle (real with shock extras, and complex)."""
@@ -317,7 +320,7 @@ def test_autotune_picks_a_valid_variant(orc):
     say("[autotune] C1")
     with DeviceOperator.from_blocks(op) as dev:
         ms, tun = dev.autotune(reps=3)
-        assert ms > 0 and tun[0] in (0, 1)
+        assert ms > 0 and tun[0] in (0, 1, 2)
         x = np.random.default_rng(16).standard_normal(op.n)
         A = orc.CSC.from_arrays(*op.to_csc_varmajor())
         got = dev.expm_taylor(x, order=12, nsub=4, dt=1 / 64)
@@ -351,3 +354,35 @@ def test_graph_replays_match_eager(orc):
                                   eager.expm_taylor_batched(X, order=5, nsub=2, dt=0.1))
         assert relerr(g.expm_taylor(xs[1], order=5, nsub=6, dt=0.1),
                       orc.power_method(A.scaled(0.1), xs[1], 6)) <= 1e-12
+
+
+def random_pattern_operator(nchi, neta, b, per_row, seed):
+    """Block operator with `per_row` random block columns per cell (plus the
+    diagonal), wider than any 3x3 stencil row: exercises slot loops that
+    need more than one group of loads per wave."""
+    r = np.random.default_rng(seed)
+    ncell = nchi * neta
+    cols = [np.unique(np.concatenate([[i], r.choice(ncell, per_row, replace=False)]))
+            for i in range(ncell)]
+    brow_ptr = np.zeros(ncell + 1, np.int64)
+    brow_ptr[1:] = np.cumsum([len(c) for c in cols])
+    bcol = np.concatenate(cols).astype(np.int32)
+    bval = r.standard_normal((bcol.shape[0], b, b)) / (b * per_row)
+    return ops.BlockOperator(nchi, neta, b, 0, brow_ptr, bcol, bval)
+
+
+@pytest.mark.parametrize("split,waves,nt", VARIANTS)
+@pytest.mark.parametrize("b", [4, 8])
+def test_launch_variants_wide_rows(orc, split, waves, nt, b):
+    """Rows of up to 20 blocks (several load groups per wave, ragged widths
+    inside a slice) for every launch variant, forward and adjoint."""
+    say(f"[variants wide] b={b} split={split} waves={waves} nt={nt}")
+    op = random_pattern_operator(9, 7, b, 19, seed=b)
+    A = orc.CSC.from_arrays(*op.to_csc_varmajor())
+    x = np.random.default_rng(21).standard_normal(op.n)
+    with DeviceOperator.from_blocks(op) as dev:
+        dev.set_tuning(split, waves, nt)
+        got = dev.expm_taylor(x, order=5, nsub=3, dt=0.1)
+        assert relerr(got, orc.power_method(A.scaled(0.1), x, 3)) <= 1e-12
+        got = dev.expm_taylor(x, order=5, nsub=3, dt=0.1, transpose=True)
+        assert relerr(got, orc.power_method(orc.csc_adjoint(A).scaled(0.1), x, 3)) <= 1e-12

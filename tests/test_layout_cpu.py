@@ -110,3 +110,29 @@ def test_emulated_batched_mfma(emu, b, R, nrhs, vpl):
                                 b, op.ncell, nrhs, R, vpl, X.ctypes.data, Y.ctypes.data)
     assert oob == 0
     assert np.allclose(Y, op.to_scipy_varmajor() @ X, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("b,R,nrhs,vpl", [(4, 8, 8, 2), (4, 8, 5, 4), (8, 8, 8, 2), (8, 8, 3, 4),
+                                          (4, 16, 16, 2), (8, 16, 16, 4), (8, 16, 11, 2),
+                                          (8, 32, 32, 2), (4, 32, 20, 4), (8, 64, 50, 4),
+                                          (4, 64, 64, 2)])
+@pytest.mark.parametrize("wide", [False, True])
+def test_emulated_batched_tile(emu, b, R, nrhs, vpl, wide):
+    """Tile layout path (permutations, value relayout, lane operand fetches,
+    MFMA placement, epilogue) replayed on the CPU with every access checked,
+    against scipy; also on rows of 20 blocks with ragged slice widths."""
+    emu.emu_spmm_tile_csc.restype = C.c_int
+    emu.emu_spmm_tile_csc.argtypes = [C.c_int64] + [C.c_void_p] * 3 + [
+        C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    if wide:
+        from test_gpu_parity import random_pattern_operator
+        op = random_pattern_operator(9, 7, b, 19, seed=b + R)
+    else:
+        op = ops.stencil_operator(9, 7, b, seed=10)
+    colptr, rowval, nzval = op.to_csc_varmajor(one_based=True)
+    X = np.asfortranarray(np.random.default_rng(3).standard_normal((op.n, nrhs)))
+    Y = np.zeros_like(X)
+    oob = emu.emu_spmm_tile_csc(op.n, colptr.ctypes.data, rowval.ctypes.data, nzval.ctypes.data,
+                                b, op.ncell, nrhs, R, vpl, X.ctypes.data, Y.ctypes.data)
+    assert oob == 0
+    assert np.allclose(Y, op.to_scipy_varmajor() @ X, rtol=1e-13, atol=1e-13)
