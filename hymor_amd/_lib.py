@@ -68,6 +68,7 @@ SIGNATURES = {
     "hymor_krylov_set_column": (C.c_int, [_p, C.c_int, _p]),
     "hymor_krylov_get_columns": (C.c_int, [_p, C.c_int, C.c_int, _p]),
     "hymor_krylov_orthonormalize": (C.c_int, [_p, C.c_int, _p, _pd]),
+    "hymor_krylov_orthonormalize_block": (C.c_int, [_p, C.c_int, C.c_int, _p, _i64, _pd]),
     "hymor_krylov_combine": (C.c_int, [_p, C.c_int, C.c_int, _pd]),
     "hymor_krylov_ritz": (C.c_int, [_p, C.c_int, C.c_int, _pd, _p]),
     "hymor_krylov_copy_column": (C.c_int, [_p, C.c_int, C.c_int]),

@@ -44,12 +44,13 @@ struct ArgError : std::invalid_argument {
     } while (0)
 
 // Launch variant of the single-vector Taylor-term kernel, chosen per
-// operator by hymor_op_autotune (defaults: k_term, 4 waves per group).
+// operator by hymor_op_autotune (defaults: k_term_split, 4 waves per slice,
+// non-temporal value loads -- the usual pick on large operators).
 struct TermTuning {
-    int split = 0;   // 0: k_term (one wave per slice), 1: k_term_split,
+    int split = 1;   // 0: k_term (one wave per slice), 1: k_term_split,
                      // 2: k_term_split_p (grouped loads)
     int waves = 4;   // k_term: waves per workgroup; k_term_split: waves per slice
-    int nt = 0;      // non-temporal loads of the block values
+    int nt = 1;      // non-temporal loads of the block values
     int spmm_mfma = 6;  // batches of >= 16 doubles per row: 0 vector-FMA k_spmm,
                         // 1 k_spmm_mfma, 2/3 k_spmm_mfma_mc (2 or 1 cells per wave),
                         // 4/5 k_spmm_mfma_w (large / small load groups),
@@ -135,6 +136,9 @@ struct hymor_krylov {
     double *h2 = nullptr;
     double *q = nullptr;        // (m_max+1)^2 complex, restart matrix
     double *partial = nullptr;  // 2*(m_max+1) * kDotBlocks
+    // block Gram-Schmidt (made on first use): partial products and the
+    // two projection passes' coefficients
+    double *bpartial = nullptr, *hb = nullptr, *hb2 = nullptr;
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
 };
@@ -1453,6 +1457,40 @@ double *col(hymor_krylov *K, double *base, int j) {
     return base + (size_t)j * K->n * K->ncomp;
 }
 
+constexpr int kVtwGroups = 512;   // row chunks of the block projection kernel
+
+// h (ncol x s, column-major) = V[:, :ncol]^T W, deterministic
+void block_dots(hymor_krylov *K, int ncol, const double *W, int64_t ldw, int s, double *h) {
+    const int ctiles = (ncol + 15) / 16;
+    const int64_t rows_per = ((K->n + kVtwGroups - 1) / kVtwGroups + 15) / 16 * 16;
+    const int64_t groups = (K->n + rows_per - 1) / rows_per;
+    const bool aligned = K->n % 2 == 0 && ldw % 2 == 0 &&
+                         ((uintptr_t)K->V % 16) == 0 && ((uintptr_t)W % 16) == 0;
+    if (aligned)
+        hipLaunchKernelGGL(k_vtw_mfma<true>, dim3((unsigned)groups), dim3(256), 0, K->stream, K->V,
+                           K->n, ncol, W, ldw, s, K->n, rows_per, ctiles, K->bpartial);
+    else
+        hipLaunchKernelGGL(k_vtw_mfma<false>, dim3((unsigned)groups), dim3(256), 0, K->stream,
+                           K->V, K->n, ncol, W, ldw, s, K->n, rows_per, ctiles, K->bpartial);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_vtw, dim3((unsigned)((ncol * s + 3) / 4)), dim3(256), 0,
+                       K->stream, K->bpartial, groups, ctiles, ncol, s, h);
+    HIP_TRY(hipGetLastError());
+}
+
+// W[:, :s] -= V[:, :ncol] h
+void block_sub(hymor_krylov *K, int ncol, const double *h, double *W, int64_t ldw, int s) {
+    const int64_t grid = std::min<int64_t>((K->n + 255) / 256, 4096);
+    const size_t lds = (size_t)ncol * s * sizeof(double);
+    if (s <= 8)
+        hipLaunchKernelGGL(k_sub_vh_blk<8>, dim3((unsigned)grid), dim3(256), lds, K->stream, K->V,
+                           K->n, ncol, h, W, ldw, s, K->n);
+    else
+        hipLaunchKernelGGL(k_sub_vh_blk<16>, dim3((unsigned)grid), dim3(256), lds, K->stream,
+                           K->V, K->n, ncol, h, W, ldw, s, K->n);
+    HIP_TRY(hipGetLastError());
+}
+
 }  // namespace
 
 extern "C" {
@@ -1570,6 +1608,86 @@ int hymor_krylov_orthonormalize(hymor_krylov *K, int j, void *d_w, double *h_out
             if (w != col(K, K->V, j))
                 HIP_TRY(hipMemcpyAsync(col(K, K->V, j), w, (size_t)len * sizeof(double),
                                        hipMemcpyDeviceToDevice, K->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(K->stream));
+    });
+}
+
+int hymor_krylov_orthonormalize_block(hymor_krylov *K, int j, int s, void *d_W, int64_t ldw,
+                                      double *h_out) {
+    return guarded([&] {
+        ARG_CHECK(K && d_W && h_out, "null argument");
+        ARG_CHECK(K->ncomp == 1, "block orthonormalisation covers real bases");
+        ARG_CHECK(s >= 1 && s <= 16, "block of 1..16 vectors");
+        ARG_CHECK(j >= 0 && j + s <= K->m_max + 1, "columns j..j+s-1 must fit the basis");
+        ARG_CHECK(ldw >= K->n, "leading dimension smaller than n");
+        ARG_CHECK((size_t)j * s * sizeof(double) <= 65536,
+                  "j * s too large for the projection update (at most 8192 coefficients)");
+        double *W = (double *)d_W;
+        const int64_t n = K->n;
+        const int ldh = j + s;
+        std::fill(h_out, h_out + (size_t)ldh * s, 0.0);
+        if (j > 0) {
+            if (!K->bpartial) {
+                const int ct = (K->m_max + 1 + 15) / 16;
+                K->bpartial = (double *)dalloc_raw(K->allocs, (size_t)kVtwGroups * ct * 256 *
+                                                                  sizeof(double));
+                K->hb = (double *)dalloc_raw(K->allocs, (size_t)(K->m_max + 1) * 16 * sizeof(double));
+                K->hb2 = (double *)dalloc_raw(K->allocs, (size_t)(K->m_max + 1) * 16 * sizeof(double));
+            }
+            // block CGS2 against the j accepted columns: one pass over V per
+            // projection for all s vectors
+            block_dots(K, j, W, ldw, s, K->hb);
+            block_sub(K, j, K->hb, W, ldw, s);
+            block_dots(K, j, W, ldw, s, K->hb2);
+            block_sub(K, j, K->hb2, W, ldw, s);
+            std::vector<double> a((size_t)j * s), b((size_t)j * s);
+            HIP_TRY(hipMemcpyAsync(a.data(), K->hb, a.size() * sizeof(double),
+                                   hipMemcpyDeviceToHost, K->stream));
+            HIP_TRY(hipMemcpyAsync(b.data(), K->hb2, b.size() * sizeof(double),
+                                   hipMemcpyDeviceToHost, K->stream));
+            HIP_TRY(hipStreamSynchronize(K->stream));
+            for (int i = 0; i < s; ++i)
+                for (int c = 0; c < j; ++c)
+                    h_out[(size_t)i * ldh + c] = a[(size_t)i * j + c] + b[(size_t)i * j + c];
+        }
+        // inside the block: CGS2 of vector i against the block's accepted
+        // columns j..j+i-1, then normalisation into column j+i
+        for (int i = 0; i < s; ++i) {
+            double *w = W + (size_t)i * ldw;
+            const double *Vb = col(K, K->V, j);
+            std::vector<double> h1((size_t)i + 1, 0.0), h2((size_t)i + 1, 0.0);
+            if (i > 0) {
+                basis_dots(K, Vb, n, i, w, K->h);
+                basis_sub(K, Vb, n, i, K->h, w);
+                basis_dots(K, Vb, n, i, w, K->h2);
+                basis_sub(K, Vb, n, i, K->h2, w);
+                HIP_TRY(hipMemcpyAsync(h1.data(), K->h, (size_t)i * sizeof(double),
+                                       hipMemcpyDeviceToHost, K->stream));
+                HIP_TRY(hipMemcpyAsync(h2.data(), K->h2, (size_t)i * sizeof(double),
+                                       hipMemcpyDeviceToHost, K->stream));
+            }
+            basis_dots(K, w, n, 1, w, K->h2 + 2 * (size_t)K->m_max);
+            double nrm2 = 0.0;
+            HIP_TRY(hipMemcpyAsync(&nrm2, K->h2 + 2 * (size_t)K->m_max, sizeof(double),
+                                   hipMemcpyDeviceToHost, K->stream));
+            HIP_TRY(hipStreamSynchronize(K->stream));
+            for (int c = 0; c < i; ++c) h_out[(size_t)i * ldh + j + c] = h1[c] + h2[c];
+            const double nrm = std::sqrt(std::max(nrm2, 0.0));
+            h_out[(size_t)i * ldh + j + i] = nrm;
+            double *dst = col(K, K->V, j + i);
+            if (nrm > 0.0) {
+                const int64_t grid = std::min<int64_t>((n + 255) / 256, 4096);
+                hipLaunchKernelGGL(k_scale, dim3((unsigned)grid), dim3(256), 0, K->stream, w, n,
+                                   1.0 / nrm);
+                HIP_TRY(hipGetLastError());
+                if (w != dst)
+                    HIP_TRY(hipMemcpyAsync(dst, w, (size_t)n * sizeof(double),
+                                           hipMemcpyDeviceToDevice, K->stream));
+            } else {
+                // dependent vector: a zero column, reported by a zero norm
+                HIP_TRY(hipMemsetAsync(dst, 0, (size_t)n * sizeof(double), K->stream));
+            }
         }
         HIP_TRY(hipStreamSynchronize(K->stream));
     });

@@ -1184,4 +1184,114 @@ __global__ __launch_bounds__(256) void k_spmm_tile(
     }
 }
 
+// ------------------------------------------------------- block Gram-Schmidt
+// Partial products of V^T W for a block of s <= 16 new Krylov vectors W
+// against ncol basis columns V, on the FP64 matrix cores: one
+// v_mfma_f64_16x16x4f64 per 16 basis columns x 16 block columns x 4 rows,
+// the 4-row index being a dummy summation index.  Lane (m = lane % 16,
+// kk = lane / 16) supplies V[row, c0 + m] and W[row, m] for rows
+// r + 4 kk + i (i = 0..3, adjacent: two 16-byte loads each when ALIGNED).
+// Workgroup g covers rows [g*rows_per, (g+1)*rows_per) (rows_per a
+// multiple of 16), wave w the 16-column tiles w, w+4, ...; partial
+// [(g*ctiles*16 + c)*16 + n] holds its sum for basis column c, block column n.
+template <bool ALIGNED>
+__global__ __launch_bounds__(256) void k_vtw_mfma(const double *__restrict__ V, int64_t ldv,
+                                                  int ncol, const double *__restrict__ W,
+                                                  int64_t ldw, int s, int64_t n,
+                                                  int64_t rows_per, int ctiles,
+                                                  double *__restrict__ partial) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, kk = lane >> 4;
+    const int64_t g = blockIdx.x;
+    const int64_t r0 = g * rows_per;
+    const int64_t r1 = r0 + rows_per < n ? r0 + rows_per : n;
+    const int64_t rfull = r0 + ((r1 > r0 ? r1 - r0 : 0) / 16) * 16;
+    const bool wv = m < s;
+    const double *wp = W + (int64_t)(wv ? m : 0) * ldw;
+    for (int ct = wave; ct < ctiles; ct += 4) {
+        const int c = ct * 16 + m;
+        const bool cv = c < ncol;
+        const double *vp = V + (int64_t)(cv ? c : 0) * ldv;
+        v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
+        for (int64_t r = r0; r < rfull; r += 16) {
+            const int64_t rr = r + 4 * kk;
+            double a[4], b[4];
+            if constexpr (ALIGNED) {
+                const d2 a0 = *reinterpret_cast<const d2 *>(vp + rr);
+                const d2 a1 = *reinterpret_cast<const d2 *>(vp + rr + 2);
+                const d2 b0 = *reinterpret_cast<const d2 *>(wp + rr);
+                const d2 b1 = *reinterpret_cast<const d2 *>(wp + rr + 2);
+                a[0] = a0.x; a[1] = a0.y; a[2] = a1.x; a[3] = a1.y;
+                b[0] = b0.x; b[1] = b0.y; b[2] = b1.x; b[3] = b1.y;
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    a[i] = vp[rr + i];
+                    b[i] = wp[rr + i];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(cv ? a[i] : 0.0, wv ? b[i] : 0.0, acc,
+                                                           0, 0, 0);
+        }
+        // tail rows (< 16, only in the last workgroup): clamped reads, zeroed
+        for (int64_t r = rfull; r < r1; r += 4) {
+            const int64_t row = r + kk;
+            const bool ok = row < r1;
+            const int64_t rc = ok ? row : r1 - 1;
+            const double a = (cv && ok) ? vp[rc] : 0.0;
+            const double b = (wv && ok) ? wp[rc] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+        // D[c][n] sits in lane n + 16 (c % 4), register c / 4
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            partial[((g * ctiles + ct) * 16 + 4 * r + kk) * 16 + m] = acc[r];
+    }
+}
+
+// h[nn*ncol + c] = sum over workgroups of the partials above: one wave per
+// entry, lane l summing groups l, l+64, ... in order, then a fixed
+// butterfly across lanes (deterministic)
+__global__ __launch_bounds__(256) void k_reduce_vtw(const double *__restrict__ partial,
+                                                    int64_t ngroups, int ctiles, int ncol, int s,
+                                                    double *__restrict__ h) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (i >= ncol * s) return;
+    const int c = i % ncol, nn = i / ncol;
+    double acc = 0.0;
+    for (int64_t g = lane; g < ngroups; g += 64) acc += partial[((g * ctiles) * 16 + c) * 16 + nn];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) h[i] = acc;
+}
+
+// W[row, nn] -= sum_c V[row, c] h[nn*ncol + c]   (nn < S), one thread per row
+template <int S>
+__global__ __launch_bounds__(256) void k_sub_vh_blk(const double *__restrict__ V, int64_t ldv,
+                                                    int ncol, const double *__restrict__ h,
+                                                    double *__restrict__ W, int64_t ldw, int s,
+                                                    int64_t n) {
+    extern __shared__ double hs[];
+    for (int i = threadIdx.x; i < ncol * s; i += blockDim.x) hs[i] = h[i];
+    __syncthreads();
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n;
+         row += (int64_t)gridDim.x * blockDim.x) {
+        double acc[S];
+#pragma unroll
+        for (int nn = 0; nn < S; ++nn) acc[nn] = 0.0;
+        for (int c = 0; c < ncol; ++c) {
+            const double v = V[(int64_t)c * ldv + row];
+#pragma unroll
+            for (int nn = 0; nn < S; ++nn)
+                if (nn < s) acc[nn] = fma(v, hs[nn * ncol + c], acc[nn]);
+        }
+#pragma unroll
+        for (int nn = 0; nn < S; ++nn)
+            if (nn < s) W[(int64_t)nn * ldw + row] -= acc[nn];
+    }
+}
+
 }  // namespace hymor

@@ -74,6 +74,18 @@ class DeviceKrylov:
             return hc
         return h
 
+    def orthonormalize_block(self, j: int, d_W, ldw: int, s: int) -> np.ndarray:
+        """Block CGS2 of the s device columns d_W (leading dimension ldw)
+        against columns 0..j-1, then among themselves, stored as columns
+        j..j+s-1; returns the (j+s) x s coefficients (W = V[:, :j+s] H).
+        Real bases only."""
+        if self.cplx:
+            raise ValueError("block orthonormalisation covers real bases")
+        h = np.zeros((s, j + s))
+        check(_lib.load().hymor_krylov_orthonormalize_block(
+            self._h, j, s, d_W, ldw, h.ctypes.data_as(C.POINTER(C.c_double))))
+        return h.T
+
     def combine(self, Q: np.ndarray):
         """V[:, :k] <- V[:, :m] Q for Q of shape (m, k)."""
         m, k = Q.shape
@@ -223,10 +235,18 @@ def block_krylov_schur(apply_block, n: int, k: int, m: int, s: int, *, complex_:
             while j + s <= m:
                 apply_block(K.column(j), n, s, W.ptr)
                 n_apply += s
-                for i in range(s):
-                    wcol = C.c_void_p(W.ptr.value + i * colbytes)
-                    h = K.orthonormalize(j + s + i, wcol)
-                    H[: j + s + i + 1, j + i] = h[: j + s + i + 1]
+                if not complex_:
+                    # one block CGS2: a pass over the basis per projection
+                    Hb = K.orthonormalize_block(j + s, W.ptr, n, s)
+                    for i in range(s):
+                        if Hb[j + s + i, i] == 0.0:
+                            raise RuntimeError("block Krylov-Schur: the block lost rank")
+                    H[: j + 2 * s, j:j + s] = Hb
+                else:
+                    for i in range(s):
+                        wcol = C.c_void_p(W.ptr.value + i * colbytes)
+                        h = K.orthonormalize(j + s + i, wcol)
+                        H[: j + s + i + 1, j + i] = h[: j + s + i + 1]
                 j += s
             mm = j
             Hm = H[:mm, :mm]

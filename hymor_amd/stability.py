@@ -349,6 +349,15 @@ def EIGENVALUES(A, s, n_modes):
     raise ValueError(f"Unknown eigenvalue solver: {solver}")
 
 
+def _device_op(A, s, scale, value_prec):
+    """Device operator of A*scale from a scipy matrix in the reference's
+    var-major order, or from a hymor_amd.operators.BlockOperator."""
+    if hasattr(A, "brow_ptr"):
+        return DeviceOperator.from_blocks(A, scale=scale, value_prec=value_prec,
+                                          with_adjoint=False)
+    return upload(A, s, scale=scale, value_prec=value_prec)
+
+
 def EIGENVALUES_GPU_ARNOLDI(A, s, n_modes, *, T_ref=0.5, tol=1e-4, maxiter=None, ncv=None,
                             v0=None, krylov="arpack"):
     """Arnoldi on the exponential transform (EIGENVALUES.jl:112-205).
@@ -365,19 +374,20 @@ def EIGENVALUES_GPU_ARNOLDI(A, s, n_modes, *, T_ref=0.5, tol=1e-4, maxiter=None,
     s = CFL_TIMESTEP(s)
     dt = s["time_integration"]["dt"]
     timesteps = int(round(T_ref / dt))            # Julia round(Int, .) ties to even, as Python
-    n = A.shape[0]
+    n = A.n if hasattr(A, "brow_ptr") else A.shape[0]
     k = n_modes
     t0 = time.time()
     dev = None
     try:
-        dev = upload(A, s, scale=dt)
+        dev = _device_op(A, s, dt, "f64")
         if krylov == "device":
             dev.set_graphs(True)      # every matvec repeats the same sweep
             def apply(din, dout):
                 expm_taylor_ptr(dev, din, dout, order=ORDER, nsub=timesteps, complex_=False)
                 dev.synchronize()
-            vals, vecs, _ = krylov_schur(apply, n, k, ncv or max(20, 2 * k + 1), tol=tol,
-                                         maxiter=maxiter or 40 * k, v0=v0)
+            vals, vecs, kinfo = krylov_schur(apply, n, k, ncv or max(20, 2 * k + 1), tol=tol,
+                                             maxiter=maxiter or 40 * k, v0=v0)
+            s.setdefault("timings", {})["eigenvalues_krylov"] = kinfo
         else:
             op = spla.LinearOperator((n, n), dtype=np.float64,
                                      matvec=lambda x: ARNOLDI_POWER_GPU(x, dev, timesteps))
@@ -604,25 +614,29 @@ def _device_generalized(devA, devC, D, s, n, k, n_t, n_set0, cplx, tol, maxiter,
 
 # ====================================================================== batched eigen solve
 
-def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=16, ncv=None, tol=1e-10,
-                     tol_f32=1e-6, maxiter=200, order=ORDER, value_prec="f32", refine=True,
-                     timings=None):
+def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=8, ncv=None, tol=1e-10,
+                     tol_f32=1e-9, maxiter=200, order=ORDER, value_prec="f32", refine=True,
+                     refine_block=16, refine_ncv=64, timings=None):
     """Leading eigenvalues of the exponential transform with batched actions.
 
     Phase 1: block Krylov-Schur whose every expansion applies T_order(A dt)^n
-    to `block` basis vectors in one SpMM sweep, block values stored in
-    `value_prec` (FP32 by default) and accumulated in FP64, converged to
-    tol_f32 -- the accuracy the stored values allow.  Phase 2 (refine):
-    single-vector Krylov-Schur on FP64-stored values started from a
-    combination of the phase-1 Ritz vectors, to tol.  Eigenvalues are mapped as EIGENVALUES_GPU_ARNOLDI does:
-    lambda = log(mu) / (n dt).  Returns (V, D, info).
+    to `block` basis vectors in one SpMM sweep (block Gram-Schmidt on the
+    device), block values stored in `value_prec` (FP32 by default) and
+    accumulated in FP64, converged to tol_f32 on the stored operator.
+    Phase 2 (refine, FP32 storage only): block Krylov-Schur on FP64-stored
+    values, `refine_block` wide, seeded with an orthonormal basis of the
+    phase-1 Ritz vectors (real and imaginary parts), to tol.  Eigenvalues are
+    mapped as EIGENVALUES_GPU_ARNOLDI does: lambda = log(mu) / (n dt).
+    A is a scipy matrix in the reference's var-major order or a
+    BlockOperator.  Returns (V, D, info); info holds per-phase iterations,
+    operator applications and seconds.
     """
     s = CFL_TIMESTEP(s)
     dt = s["time_integration"]["dt"]
     timesteps = int(round(T_ref / dt))
-    n = A.shape[0]
+    n = A.n if hasattr(A, "brow_ptr") else A.shape[0]
     k = n_modes
-    m = ncv or (k + 9 * block)          # several block steps between restarts
+    m = ncv or max(k + 4 * block, 8 * block)
     info = {}
 
     def make_apply(dev):
@@ -633,7 +647,7 @@ def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=16, ncv=None, tol=1e-10,
         return apply
 
     t0 = time.perf_counter()
-    dev = upload(A, s, scale=dt, value_prec=value_prec)
+    dev = _device_op(A, s, dt, value_prec)
     dev.set_graphs(True)
     try:
         tol1 = tol_f32 if (refine and value_prec == "f32") else tol
@@ -641,25 +655,23 @@ def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=16, ncv=None, tol=1e-10,
                                          maxiter=maxiter)
     finally:
         dev.close()
-    info["phase1"] = dict(i1, seconds=time.perf_counter() - t0, value_prec=value_prec)
+    info["phase1"] = dict(i1, seconds=time.perf_counter() - t0, value_prec=value_prec,
+                          block=block)
     if refine and value_prec == "f32":
-        # FP64 phase: single-vector Krylov-Schur on FP64-stored values,
-        # started from a random combination of the phase-1 Ritz vectors, so
-        # its Krylov space holds the wanted invariant subspace from the start
         t1 = time.perf_counter()
-        rng = np.random.default_rng(1)
-        v0 = (Y @ rng.standard_normal(Y.shape[1])).real + (Y @ rng.standard_normal(Y.shape[1])).imag
-        dev = upload(A, s, scale=dt, value_prec="f64")
+        Z = np.concatenate([Y.real, Y.imag], axis=1)
+        nz = np.linalg.norm(Z, axis=0)
+        Q, _ = np.linalg.qr(Z[:, nz > 1e-8 * nz.max()])       # drop the zero imaginary parts
+        dev = _device_op(A, s, dt, "f64")
         dev.set_graphs(True)
         try:
-            def apply64(din, dout):
-                expm_taylor_ptr(dev, din, dout, order=order, nsub=timesteps, complex_=False)
-                dev.synchronize()
-            vals, Y, i2 = krylov_schur(apply64, n, k, max(20, 2 * k + 1) + k, tol=tol,
-                                       maxiter=maxiter, v0=v0)
+            vals, Y, i2 = block_krylov_schur(make_apply(dev), n, k, max(refine_ncv, k + refine_block),
+                                             refine_block, tol=tol, maxiter=maxiter,
+                                             V0=Q[:, :refine_block])
         finally:
             dev.close()
-        info["phase2"] = dict(i2, seconds=time.perf_counter() - t1, value_prec="f64")
+        info["phase2"] = dict(i2, seconds=time.perf_counter() - t1, value_prec="f64",
+                              block=refine_block)
     info["seconds"] = time.perf_counter() - t0
     order_ = np.argsort(-np.abs(vals), kind="stable")
     vals, Y = vals[order_], Y[:, order_]

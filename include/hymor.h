@@ -273,6 +273,19 @@ int hymor_krylov_get_columns(hymor_krylov *K, int j0, int ncols, void *out);
  * matrix when w = Op(V[:, j-1]). */
 int hymor_krylov_orthonormalize(hymor_krylov *K, int j, void *d_w, double *h_out);
 
+/* Block form of hymor_krylov_orthonormalize for block Krylov methods (no
+ * single counterpart in the reference, whose ARPACK expands one vector at a
+ * time): the s <= 16 device columns W (leading dimension ldw) are
+ * orthogonalised against basis columns 0..j-1 by block classical
+ * Gram-Schmidt done twice, each projection one pass over the basis for all
+ * s vectors (V^T W on the FP64 matrix cores), then orthonormalised among
+ * themselves in order into columns j..j+s-1.  h_out receives the
+ * (j+s) x s coefficient matrix, column-major: W = V[:, :j+s] h_out; a zero
+ * diagonal entry marks a dependent vector (its column is zeroed).  W is
+ * overwritten.  Real bases only. */
+int hymor_krylov_orthonormalize_block(hymor_krylov *K, int j, int s, void *d_W, int64_t ldw,
+                                      double *h_out);
+
 /* Restart: V[:, 0:k] <- V[:, 0:m] * Q with Q column-major m x k (host,
  * complex interleaved for complex bases). */
 int hymor_krylov_combine(hymor_krylov *K, int m, int k, const double *Q);

@@ -1,0 +1,80 @@
+"""Device Krylov basis: block Gram-Schmidt (V^T W on the matrix cores)
+against a float64 numpy reference, and the batched eigensolver on an operator
+with an exactly known spectrum."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from hymor_amd import operators as ops
+from hymor_amd import stability as st
+from hymor_amd.core import DeviceMatrix
+from hymor_amd.krylov import DeviceKrylov
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("n,j,s", [(1440, 0, 8), (1440, 5, 8), (1003, 37, 16), (100003, 20, 3),
+                                   (4096, 63, 1), (998, 16, 16)])
+def test_block_orthonormalize(n, j, s):
+    """Columns j..j+s-1 come out orthonormal to the basis and each other,
+    W = V[:, :j+s] H holds, and the top j rows of H are V^T W (CGS2 sums).
+    Odd n exercises the unaligned loads, n % 16 != 0 the tail rows."""
+    say(f"[block gs] n={n} j={j} s={s}")
+    r = np.random.default_rng(n + j + s)
+    K = DeviceKrylov(n, j + s + 1)
+    try:
+        Q, _ = np.linalg.qr(r.standard_normal((n, max(j, 1))))
+        for c in range(j):
+            K.set_column(c, Q[:, c])
+        W0 = r.standard_normal((n, s))
+        W0[:, -1] += Q[:, : min(j, 3)].sum(axis=1) * 5.0 if j else 0.0   # a component inside span(V)
+        dW = DeviceMatrix.from_host(W0)
+        H = K.orthonormalize_block(j, dW.ptr, n, s)
+        dW.free()
+        V = K.get_columns(0, j + s)
+        assert H.shape == (j + s, s)
+        G = V.T @ V
+        assert np.abs(G - np.eye(j + s)).max() <= 1e-13
+        assert np.abs(V @ H - W0).max() <= 1e-12 * np.abs(W0).max()
+        if j:
+            assert np.allclose(H[:j], Q[:, :j].T @ W0, rtol=0, atol=1e-12 * np.abs(W0).max())
+        assert np.all(np.diag(H[j:]) > 0) and np.allclose(np.tril(H[j:], -1), 0.0)
+    finally:
+        K.close()
+
+
+def test_batched_eigs_exact_spectrum():
+    """Batched block Krylov-Schur (8 vectors per sweep) on FP64 values, and
+    on FP32 values with FP64 refinement, against the exact spectrum of a
+    one-way (block triangular) operator; plain FP64 Krylov-Schur alongside."""
+    nchi, neta, k = 40, 30, 10
+    op, lam = ops.marching_operator(nchi, neta, 8, seed=3)
+    dt = 1.0 / 64
+    n_t = 32
+    mu = sum((lam * dt) ** j / math.factorial(j) for j in range(6)) ** n_t
+    exact = np.log(mu[np.argsort(-np.abs(mu), kind="stable")][: 3 * k]) / (n_t * dt)
+
+    def err(D):
+        return max(np.min(np.abs(exact - d)) / abs(d) for d in D)
+
+    def sd():
+        return {"mesh": {"Nchi": nchi, "Neta": neta}, "time_integration": {"dt": dt},
+                "stability_analysis": {"perturb_shock": False,
+                                       "eigenvalue_solver": "GPU_TIMESTEPPER_ARNOLDI"}}
+    say("[batched eigs] marching operator 40x30 b=8")
+    _, D, _ = st.batched_exp_eigs(op, sd(), k, block=8, value_prec="f64", tol=1e-11)
+    assert err(D) <= 1e-10
+    _, D, info = st.batched_exp_eigs(op, sd(), k, block=8, value_prec="f32", tol=1e-11)
+    assert err(D) <= 1e-10 and "phase2" in info
+    _, D, _ = st.batched_exp_eigs(op, sd(), k, block=8, value_prec="f32", tol=1e-11,
+                                  refine=False)
+    assert 1e-12 < err(D) <= 1e-6          # FP32 storage alone: about its rounding
+    _, D = st.EIGENVALUES_GPU_ARNOLDI(op, sd(), k, krylov="device", tol=1e-11, ncv=25)
+    assert err(D) <= 1e-10
