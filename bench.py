@@ -225,6 +225,9 @@ def main():
     ap.add_argument("--no-eigen", action="store_true", help="skip the eigenvalue leg")
     ap.add_argument("--kernel-only", choices=["plain", "batched"],
                     help="time only the Taylor-term kernel of one form (for profiler runs)")
+    ap.add_argument("--batch-kernel", type=int, default=None,
+                    help="--kernel-only batched: use this batch kernel (0..6) instead of "
+                         "autotuning (profiler passes on one kernel)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure HBM traffic")
     args = ap.parse_args()
@@ -250,6 +253,8 @@ def main():
         devk = DeviceOperator.from_blocks(op, value_prec=prec, with_adjoint=False)
         if nv == 1:
             devk.autotune(reps=5)
+        elif args.batch_kernel is not None:
+            devk.set_batch_kernel(args.batch_kernel)
         elif nv in (16, 32, 64):
             devk.autotune_batch(nv=nv, reps=3)
         ms, by = devk.bench_term_kernel(reps=args.kernel_reps, nv=nv)
