@@ -226,7 +226,7 @@ def main():
     ap.add_argument("--kernel-only", choices=["plain", "batched"],
                     help="time only the Taylor-term kernel of one form (for profiler runs)")
     ap.add_argument("--batch-kernel", type=int, default=None,
-                    help="--kernel-only batched: use this batch kernel (0..6) instead of "
+                    help="--kernel-only batched: use this batch kernel (0..7) instead of "
                          "autotuning (profiler passes on one kernel)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure HBM traffic")
@@ -357,7 +357,9 @@ def main():
                        5: f"k_spmm_mfma_w<{cfg['b']}, float, {nb}, "
                           f"{ {16: 5, 32: 3, 64: 2}.get(nb)}>",
                        6: f"k_spmm_tile<{cfg['b']}, float, {nb}, "
-                          f"{ {8: 9, 16: 9, 32: 5, 64: 3}.get(nb)}>"}.get(bkern, "k_spmm"),
+                          f"{ {8: 9, 16: 9, 32: 5, 64: 3}.get(nb)}>",
+                       7: f"k_spmm_quad<float, {nb}, { {8: 9, 16: 9, 32: 5, 64: 3}.get(nb)}>"
+                       }.get(bkern, "k_spmm"),
             "kernel_ms": bk_ms,
             "algorithmic_bytes_per_launch": bk_bytes,
             "achieved_GBps": b_gbs,

@@ -54,7 +54,8 @@ struct TermTuning {
     int spmm_mfma = 6;  // batches of >= 16 doubles per row: 0 vector-FMA k_spmm,
                         // 1 k_spmm_mfma, 2/3 k_spmm_mfma_mc (2 or 1 cells per wave),
                         // 4/5 k_spmm_mfma_w (large / small load groups),
-                        // 6 k_spmm_tile (tile layout, also at 8 doubles per row)
+                        // 6 k_spmm_tile (tile layout, also at 8 doubles per row),
+                        // 7 k_spmm_quad (4x4x4 matrix cores, b = 8; tile at b = 4)
 };
 
 struct DevSliced {
@@ -65,6 +66,7 @@ struct DevSliced {
     int32_t *col = nullptr;
     void *val = nullptr;
     void *val_t = nullptr;  // the same values in k_spmm_tile's order (made on first use)
+    void *val_q = nullptr;  // ... in k_spmm_quad's order (b = 8, made on first use)
 };
 
 struct DevExtra {
@@ -283,7 +285,22 @@ void ensure_ws_only(hymor_op *op, int nv) {
 // chosen batch kernel; their internal buffers then use the tile layout for
 // flow rows.  Returns the block size to pass as tile_b, or 0 (row-major).
 int tile_b_of(const hymor_op *op, int nv) {
-    return (nv >= 8 && op->fwd.tune.spmm_mfma == 6) ? op->b : 0;
+    return (nv >= 8 && op->fwd.tune.spmm_mfma >= 6) ? op->b : 0;
+}
+
+void quad_vals_of(hymor_op *op, DevSliced &s) {
+    if (s.val_q || s.nnzb_pad == 0 || s.b != 8) return;
+    const int64_t total = s.nnzb_pad * 64;
+    const size_t esz = s.prec == HYMOR_PREC_F32 ? sizeof(float) : sizeof(double);
+    s.val_q = dalloc(op, (size_t)total * esz);
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 65536));
+    if (s.prec == HYMOR_PREC_F32)
+        hipLaunchKernelGGL(k_quad_vals<float>, dim3((unsigned)grid), dim3(256), 0, op->stream,
+                           (const float *)s.val, (float *)s.val_q, total);
+    else
+        hipLaunchKernelGGL(k_quad_vals<double>, dim3((unsigned)grid), dim3(256), 0, op->stream,
+                           (const double *)s.val, (double *)s.val_q, total);
+    HIP_TRY(hipGetLastError());
 }
 
 void tile_vals_of(hymor_op *op, DevSliced &s) {
@@ -301,15 +318,20 @@ void tile_vals_of(hymor_op *op, DevSliced &s) {
     HIP_TRY(hipGetLastError());
 }
 
-// Tile-ordered values for both directions; called before any launch (or
-// graph capture) of k_spmm_tile.
+// The value copies the chosen tile-layout kernel reads (k_spmm_tile: tile
+// order; k_spmm_quad at b = 8: quad order), for both directions; called
+// before any launch (or graph capture) of those kernels.
 void ensure_tile(hymor_op *op) {
     if (op->ncell == 0) return;
-    const bool fresh = !op->fwd.val_t || (op->has_adj && !op->adj.val_t);
-    if (!fresh) return;
+    const bool quad = op->fwd.tune.spmm_mfma == 7 && op->b == 8;
+    auto missing = [&](const DevSliced &s) { return quad ? !s.val_q : !s.val_t; };
+    if (!missing(op->fwd) && !(op->has_adj && missing(op->adj))) return;
     drop_graph(op);
-    tile_vals_of(op, op->fwd);
-    if (op->has_adj) tile_vals_of(op, op->adj);
+    for (DevSliced *s : {&op->fwd, &op->adj}) {
+        if (s == &op->adj && !op->has_adj) continue;
+        if (quad) quad_vals_of(op, *s);
+        else tile_vals_of(op, *s);
+    }
     HIP_TRY(hipStreamSynchronize(op->stream));
 }
 
@@ -430,11 +452,34 @@ void launch_tile_t(const DevSliced &s, hipStream_t st, const double *x, const do
                        c1, c2);
 }
 
+template <typename VT, int R>
+void launch_quad_t(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+                   double *tout, const double *vin, double *vout, double c1, double c2) {
+    if (!s.val_q) throw ArgError("quad-ordered values missing (internal)");
+    constexpr int W = R <= 16 ? 9 : (R == 32 ? 5 : 3);
+    const int wpb = 4;
+    const int64_t grid = (s.ncell + wpb - 1) / wpb;
+    hipLaunchKernelGGL((k_spmm_quad<VT, R, W>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
+                       s.slice_ptr, s.col, (const VT *)s.val_q, s.ncell, x, yx, tout, vin, vout,
+                       c1, c2);
+}
+
 template <int B, typename VT>
 void launch_block_part(const DevSliced &s, int nv, hipStream_t st, const double *x,
                        const double *yx, double *tout, const double *vin, double *vout, double c1,
                        double c2) {
-    if (nv >= 8 && s.tune.spmm_mfma == 6) {
+    if constexpr (B == 8) {
+        if (nv >= 8 && s.tune.spmm_mfma == 7) {
+            switch (nv) {
+            case 8: launch_quad_t<VT, 8>(s, st, x, yx, tout, vin, vout, c1, c2); return;
+            case 16: launch_quad_t<VT, 16>(s, st, x, yx, tout, vin, vout, c1, c2); return;
+            case 32: launch_quad_t<VT, 32>(s, st, x, yx, tout, vin, vout, c1, c2); return;
+            case 64: launch_quad_t<VT, 64>(s, st, x, yx, tout, vin, vout, c1, c2); return;
+            default: throw ArgError("unsupported vector width");
+            }
+        }
+    }
+    if (nv >= 8 && s.tune.spmm_mfma >= 6) {
         switch (nv) {
         case 8: launch_tile_t<B, VT, 8>(s, st, x, yx, tout, vin, vout, c1, c2); return;
         case 16: launch_tile_t<B, VT, 16>(s, st, x, yx, tout, vin, vout, c1, c2); return;
@@ -1113,7 +1158,7 @@ int hymor_op_set_graphs(hymor_op *op, int enable) {
 int hymor_op_set_batch_kernel(hymor_op *op, int use_mfma) {
     return guarded([&] {
         ARG_CHECK(op, "null operator");
-        ARG_CHECK(use_mfma >= 0 && use_mfma <= 6, "batch kernel must be 0..6");
+        ARG_CHECK(use_mfma >= 0 && use_mfma <= 7, "batch kernel must be 0..7");
         op->fwd.tune.spmm_mfma = use_mfma;
         op->adj.tune.spmm_mfma = op->fwd.tune.spmm_mfma;
         drop_graph(op);
@@ -1131,9 +1176,9 @@ int hymor_op_autotune_batch(hymor_op *op, int nv, int reps, int *best_kernel, do
         HIP_TRY(hipEventCreate(&e1));
         double best = 1e300;
         int pick = op->fwd.tune.spmm_mfma;
-        for (int kern = 0; kern <= 6; ++kern) {
+        for (int kern = 0; kern <= 7; ++kern) {
             op->fwd.tune.spmm_mfma = kern;
-            if (kern == 6) ensure_tile(op);
+            if (kern >= 6) ensure_tile(op);
             launch_term(op, false, nv, op->stream, op->ws[4], op->ws[5], op->ws[2], op->ws[2], 0.5,
                         1.0);
             HIP_TRY(hipEventRecord(e0, op->stream));

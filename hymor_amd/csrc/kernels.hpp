@@ -16,6 +16,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 namespace hymor {
 
@@ -1291,6 +1292,184 @@ __global__ __launch_bounds__(256) void k_sub_vh_blk(const double *__restrict__ V
 #pragma unroll
         for (int nn = 0; nn < S; ++nn)
             if (nn < s) W[(int64_t)nn * ldw + row] -= acc[nn];
+    }
+}
+
+// ------------------------------------------------------ 4x4x4 matrix cores
+// v_mfma_f64_4x4x4f64 on gfx950 (measured): four independent 4x4x4
+// products per instruction; block blk takes A[blk][m][k] from lane
+// m + 4 blk + 16 k and B[blk][k][n] from lane n + 4 blk + 16 k, and leaves
+// D[blk][m][n] in lane n + 4 blk + 16 m.  For b = 8 an 8 x R output tile
+// splits into 4 x 4 tiles with no idle rows, where the 16 x 16 x 4 form
+// leaves rows 8..15 idle.
+//
+// Value order ("quad") inside each 8 x 8 block: lane group g = m + 4 k
+// (m, k < 4) holds A[4 rq + m][4 kc + k] for (kc, rq) at g*4 + kc*2 + rq, so
+// a lane's four A operands are one 16-byte (FP32) or 32-byte (FP64) load,
+// and the two of one k-chunk one 8- or 16-byte load.
+template <typename VT>
+__global__ void k_quad_vals(const VT *__restrict__ src, VT *__restrict__ dst, int64_t total) {
+    constexpr int V = Vpl<VT>::value;
+    constexpr int CPS = 8;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t blkidx = i >> 6;
+        const int within = (int)(i & 63);
+        const int64_t slot = blkidx / CPS;
+        const int q = (int)(blkidx - slot * CPS);
+        const int g = within >> 2, kc = (within >> 1) & 1, rq = within & 1;
+        const int m = g & 3, k = g >> 2;
+        const int row = 4 * rq + m, colk = 4 * kc + k;
+        dst[i] = src[slot * CPS * 64 + ((int64_t)(colk / V) * 64 + q * 8 + row) * V + colk % V];
+    }
+}
+
+template <typename VT>
+struct Quad4;
+template <>
+struct Quad4<float> {
+    typedef f4 raw;
+    typedef f2 half;
+    __device__ __forceinline__ static raw load(const float *p) {
+        return __builtin_nontemporal_load(reinterpret_cast<const f4 *>(p));
+    }
+    __device__ __forceinline__ static half load_half(const float *p) {
+        return __builtin_nontemporal_load(reinterpret_cast<const f2 *>(p));
+    }
+    __device__ __forceinline__ static double get(const raw &r, int i) { return (double)r[i]; }
+    __device__ __forceinline__ static double get_half(const half &r, int i) { return (double)r[i]; }
+};
+template <>
+struct Quad4<double> {
+    struct raw {
+        d2 lo, hi;
+    };
+    __device__ __forceinline__ static raw load(const double *p) {
+        return raw{__builtin_nontemporal_load(reinterpret_cast<const d2 *>(p)),
+                   __builtin_nontemporal_load(reinterpret_cast<const d2 *>(p + 2))};
+    }
+    typedef d2 half;
+    __device__ __forceinline__ static half load_half(const double *p) {
+        return __builtin_nontemporal_load(reinterpret_cast<const d2 *>(p));
+    }
+    __device__ __forceinline__ static double get(const raw &r, int i) {
+        return i < 2 ? r.lo[i] : r.hi[i - 2];
+    }
+    __device__ __forceinline__ static double get_half(const half &r, int i) { return r[i]; }
+};
+
+// Batched Taylor term for b = 8 on v_mfma_f64_4x4x4f64, vectors in the tile
+// order (R doubles per row, R = 8..64) and values in the quad order.
+//   R >= 16: block blk = column quarter of a 16-column tile; per slot and
+//     tile 4 instructions (rq, kc), accumulators acc[nt][rq]; the lane's
+//     outputs (rows kk and kk + 4) are one 16-byte store.
+//   R = 8: blk = (column half nq, k-chunk kc); per slot 2 instructions (rq),
+//     the two k-chunks' partial sums added across lanes l and l ^ 8 at the
+//     end; lanes with kc = 0 store.
+// One cell per wave; the group's W block columns from one vector load.
+// Summation order: slots ascending, k-chunks in order (R >= 16) or the two
+// k-chunk sums added last (R = 8).
+template <typename VT, int R, int W>
+__global__ __launch_bounds__(256) void k_spmm_quad(
+    const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
+    const VT *__restrict__ valq, int64_t ncell, const double *__restrict__ x,
+    const double *__restrict__ yx, double *__restrict__ tout, const double *vin, double *vout,
+    double c1, double c2) {
+    constexpr int B = 8, CPS = 8, BB = 64;
+    constexpr int NT = R >= 16 ? R / 16 : 1;
+    constexpr int WN = R >= 16 ? 16 : 8;
+    static_assert(R == 8 || R == 16 || R == 32 || R == 64, "width 8..64");
+    static_assert(W <= 64, "one column per lane");
+    const int lane = threadIdx.x & 63;
+    const int64_t wpb = blockDim.x >> 6;
+    const int64_t cell = (int64_t)__builtin_amdgcn_readfirstlane(
+        (int)(xcd_tile(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6)));
+    if (cell >= ncell) return;
+    const int m4 = lane & 3, k4 = lane >> 4;
+    const int g = m4 + 4 * k4;
+    const int kcl = (lane >> 3) & 1;            // R = 8: this lane's k-chunk
+    const int jl = R >= 16 ? (lane & 15) : (lane & 7);
+    const int64_t slice = cell / CPS, q = cell - slice * CPS;
+    const int64_t p0 = slice_ptr[slice];
+    const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
+    const int64_t cbase = cell * B * R;
+    const double *pv = vout ? vin : x;
+    const double *py = yx ? yx : x;
+    d2 pre_v[NT], pre_y[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        const int64_t o = cbase + ((int64_t)(k4 * NT + nt) * WN + jl) * 2;
+        pre_v[nt] = *reinterpret_cast<const d2 *>(pv + o);
+        pre_y[nt] = *reinterpret_cast<const d2 *>(py + o);
+    }
+    double acc[NT][2];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[nt][0] = acc[nt][1] = 0.0;
+    for (int t0 = 0; t0 < width; t0 += W) {
+        const int tl = t0 + (lane < W ? lane : 0);
+        const int32_t cl = col[p0 + (int64_t)(tl < width ? tl : width - 1) * CPS + q];
+        int32_t c[W];
+#pragma unroll
+        for (int u = 0; u < W; ++u) c[u] = __builtin_amdgcn_readlane(cl, u);
+        // R >= 16: all four operands (kc, rq); R = 8: the two of this lane's kc
+        typedef typename std::conditional<(R >= 16), typename Quad4<VT>::raw,
+                                          typename Quad4<VT>::half>::type AR;
+        AR a[W];
+        d2 bx[W][NT];
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            const int t = t0 + u < width ? t0 + u : width - 1;
+            const VT *ap = valq + (p0 + (int64_t)t * CPS + q) * BB + g * 4;
+            if constexpr (R >= 16) a[u] = Quad4<VT>::load(ap);
+            else a[u] = Quad4<VT>::load_half(ap + kcl * 2);
+            const double *xp = x + (int64_t)c[u] * B * R;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                bx[u][nt] = *reinterpret_cast<const d2 *>(
+                    xp + ((int64_t)(k4 * NT + nt) * WN + jl) * 2);
+        }
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            const bool live = t0 + u < width;
+            if constexpr (R >= 16) {
+#pragma unroll
+                for (int rq = 0; rq < 2; ++rq)
+#pragma unroll
+                    for (int kc = 0; kc < 2; ++kc) {
+                        const double av = live ? Quad4<VT>::get(a[u], kc * 2 + rq) : 0.0;
+#pragma unroll
+                        for (int nt = 0; nt < NT; ++nt)
+                            acc[nt][rq] = __builtin_amdgcn_mfma_f64_4x4x4f64(
+                                av, bx[u][nt][kc], acc[nt][rq], 0, 0, 0);
+                    }
+            } else {
+                const double xv = kcl ? bx[u][0].y : bx[u][0].x;
+#pragma unroll
+                for (int rq = 0; rq < 2; ++rq) {
+                    const double av = live ? Quad4<VT>::get_half(a[u], rq) : 0.0;
+                    acc[0][rq] = __builtin_amdgcn_mfma_f64_4x4x4f64(av, xv, acc[0][rq], 0, 0, 0);
+                }
+            }
+        }
+    }
+    if constexpr (R < 16) {
+#pragma unroll
+        for (int rq = 0; rq < 2; ++rq) acc[0][rq] += __shfl_xor(acc[0][rq], 8, 64);
+        if (kcl) return;
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        const int64_t o = cbase + ((int64_t)(k4 * NT + nt) * WN + jl) * 2;
+        d2 tv, vv;
+#pragma unroll
+        for (int rq = 0; rq < 2; ++rq) {
+            const double y = acc[nt][rq] + (yx ? pre_y[nt][rq] : 0.0);
+            const double t = c1 * y;
+            tv[rq] = t;
+            vv[rq] = fma(c2, t, pre_v[nt][rq]);
+        }
+        if (tout) *reinterpret_cast<d2 *>(tout + o) = tv;
+        if (vout) *reinterpret_cast<d2 *>(vout + o) = vv;
     }
 }
 

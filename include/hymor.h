@@ -193,11 +193,12 @@ int hymor_op_set_graphs(hymor_op *op, int enable);
  * 4 / 5 matrix cores with all operand loads of a large / small group of
  * slots issued together (k_spmm_mfma_w), 6 matrix cores with the batch
  * held in the operand ("tile") order, 16-byte accesses per lane
- * (k_spmm_tile, the default; it also serves 8 doubles per row).  All
- * accumulate in FP64. */
+ * (k_spmm_tile, the default; it also serves 8 doubles per row), 7 the same
+ * order on the 4x4x4 FP64 matrix cores with no idle rows at b = 8
+ * (k_spmm_quad; k_spmm_tile at b = 4).  All accumulate in FP64. */
 int hymor_op_set_batch_kernel(hymor_op *op, int use_mfma);
 
-/* Times the seven batch kernels at width nv (16, 32 or 64) and keeps the
+/* Times the eight batch kernels at width nv (16, 32 or 64) and keeps the
  * fastest; returns its id and ms per launch. */
 int hymor_op_autotune_batch(hymor_op *op, int nv, int reps, int *best_kernel, double *best_ms);
 

@@ -347,3 +347,126 @@ extern "C" int emu_spmm_tile_csc(int64_t n, const int64_t *colptr, const int64_t
     }
     return g_oob;
 }
+
+// Quad path (b = 8): k_quad_vals, and k_spmm_quad's lane operand fetches
+// and v_mfma_f64_4x4x4f64 placement (block blk: A[blk][m][k] from lane
+// m + 4 blk + 16 k, B[blk][k][n] from lane n + 4 blk + 16 k, D[blk][m][n] in
+// lane n + 4 blk + 16 m), in the tile vector layout.
+extern "C" int emu_spmm_quad_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                                 const double *nzval, int64_t ncell, int nrhs, int R, int vpl,
+                                 const double *X_vm, double *Y_vm) {
+    g_oob = 0;
+    const int b = 8;
+    HostBSR fwd, adj;
+    HostExtra xf, xa;
+    csc_to_blocks(n, colptr, rowval, nzval, 1, false, 1.0, b, ncell, fwd, adj, xf, xa);
+    if (!xf.rows.empty()) return -1;
+    HostSliced s;
+    bsr_to_sliced(fwd, s);
+    interleave_lanes(s, vpl);
+    const int64_t nflow = (int64_t)b * ncell;
+    std::vector<double> x((size_t)n * R, -1e300), y((size_t)n * R, 0.0);
+    for (int64_t i = 0; i < n * R; ++i) {
+        const int64_t row = i / R, rem = i % R;
+        const int64_t o = row < nflow ? tile_pos_h(row, (int)rem, b, R) : row * R + rem;
+        if (o < 0 || o >= (int64_t)x.size()) { ++g_oob; continue; }
+        if (rem >= nrhs) { x[(size_t)o] = 0.0; continue; }
+        const int64_t vm = row < nflow ? (row % b) * ncell + row / b : row;
+        x[(size_t)o] = X_vm[rem * n + vm];
+    }
+    // k_quad_vals
+    const int CPS = 8;
+    const int64_t total = (int64_t)s.val.size();
+    std::vector<double> valq((size_t)total, 0.0);
+    for (int64_t i = 0; i < total; ++i) {
+        const int64_t blkidx = i >> 6;
+        const int within = (int)(i & 63);
+        const int64_t slot = blkidx / CPS;
+        const int q = (int)(blkidx - slot * CPS);
+        const int g = within >> 2, kc = (within >> 1) & 1, rq = within & 1;
+        const int m = g & 3, k = g >> 2;
+        const int row = 4 * rq + m, colk = 4 * kc + k;
+        valq[(size_t)i] = at(s.val, slot * CPS * 64 + ((int64_t)(colk / vpl) * 64 + q * 8 + row) * vpl + colk % vpl);
+    }
+    const int NT = R >= 16 ? R / 16 : 1, WN = R >= 16 ? 16 : 8;
+    for (int64_t cell = 0; cell < ncell; ++cell) {
+        const int64_t slice = cell / CPS, q = cell - slice * CPS;
+        const int64_t p0 = at(s.slice_ptr, slice);
+        const int64_t width = (at(s.slice_ptr, slice + 1) - p0) / CPS;
+        const int64_t cbase = cell * b * R;
+        double acc[4][2][64];          // [nt][rq][lane]
+        for (int nt = 0; nt < 4; ++nt)
+            for (int rq = 0; rq < 2; ++rq)
+                for (int l = 0; l < 64; ++l) acc[nt][rq][l] = 0.0;
+        for (int64_t t = 0; t < width; ++t) {
+            const int64_t c = at(s.col, p0 + t * CPS + q);
+            double a[64][4], bx[64][4][2];
+            for (int l = 0; l < 64; ++l) {
+                const int m4 = l & 3, k4 = l >> 4, g = m4 + 4 * k4;
+                const int jl = R >= 16 ? (l & 15) : (l & 7);
+                for (int e = 0; e < 4; ++e) a[l][e] = at(valq, (p0 + t * CPS + q) * 64 + g * 4 + e);
+                for (int nt = 0; nt < NT; ++nt)
+                    for (int e = 0; e < 2; ++e)
+                        bx[l][nt][e] = at(x, c * b * R + ((int64_t)(k4 * NT + nt) * WN + jl) * 2 + e);
+            }
+            // one 4x4x4 instruction: operands per lane -> D per lane
+            auto mfma4 = [](const double *opa, const double *opb, double *d) {
+                for (int blk = 0; blk < 4; ++blk)
+                    for (int m = 0; m < 4; ++m)
+                        for (int nn = 0; nn < 4; ++nn) {
+                            double sacc = 0.0;
+                            for (int k = 0; k < 4; ++k)
+                                sacc += opa[m + 4 * blk + 16 * k] * opb[nn + 4 * blk + 16 * k];
+                            d[nn + 4 * blk + 16 * m] += sacc;
+                        }
+            };
+            if (R >= 16) {
+                for (int rq = 0; rq < 2; ++rq)
+                    for (int kc = 0; kc < 2; ++kc)
+                        for (int nt = 0; nt < NT; ++nt) {
+                            double opa[64], opb[64];
+                            for (int l = 0; l < 64; ++l) {
+                                opa[l] = a[l][kc * 2 + rq];
+                                opb[l] = bx[l][nt][kc];
+                            }
+                            mfma4(opa, opb, acc[nt][rq]);
+                        }
+            } else {
+                for (int rq = 0; rq < 2; ++rq) {
+                    double opa[64], opb[64];
+                    for (int l = 0; l < 64; ++l) {
+                        const int kcl = (l >> 3) & 1;
+                        opa[l] = a[l][kcl * 2 + rq];
+                        opb[l] = bx[l][0][kcl];
+                    }
+                    mfma4(opa, opb, acc[0][rq]);
+                }
+            }
+        }
+        if (R < 16)
+            for (int rq = 0; rq < 2; ++rq) {
+                double tmp[64];
+                for (int l = 0; l < 64; ++l) tmp[l] = acc[0][rq][l] + acc[0][rq][l ^ 8];
+                for (int l = 0; l < 64; ++l) acc[0][rq][l] = tmp[l];
+            }
+        for (int l = 0; l < 64; ++l) {
+            if (R < 16 && ((l >> 3) & 1)) continue;
+            const int k4 = l >> 4;
+            const int jl = R >= 16 ? (l & 15) : (l & 7);
+            for (int nt = 0; nt < NT; ++nt) {
+                const int64_t o = cbase + ((int64_t)(k4 * NT + nt) * WN + jl) * 2;
+                for (int rq = 0; rq < 2; ++rq) {
+                    if (o + rq < 0 || o + rq >= (int64_t)y.size()) ++g_oob;
+                    else y[(size_t)(o + rq)] = acc[nt][rq][l];
+                }
+            }
+        }
+    }
+    for (int64_t i = 0; i < n * nrhs; ++i) {
+        const int64_t rhs = i / n, vm = i % n;
+        const int64_t row = vm < nflow ? (vm % ncell) * b + vm / ncell : vm;
+        const int64_t o = row < nflow ? tile_pos_h(row, (int)rhs, b, R) : row * R + rhs;
+        Y_vm[rhs * n + vm] = at(y, o);
+    }
+    return g_oob;
+}
