@@ -199,6 +199,29 @@ def test_freestream_receptivity_vs_dense():
     assert np.all(np.abs(G - ref) <= 1e-8 * np.abs(ref)), (G, ref)
 
 
+def test_freestream_receptivity_factored_matches_assembled():
+    """FREESTREAM_RECEPTIVITY through the factored extended operator (one
+    batched sweep over the N wavenumber systems per direction) gives the
+    gains of the assembled path and of the dense reference."""
+    nchi, neta = 6, 5
+    w = [0.0, 1.0, 2.5, 4.0]
+    say("[FREESTREAM_RECEPTIVITY factored] vs assembled and dense")
+    s = st.CFL_TIMESTEP(synthetic_solution(nchi, neta, seed=5, perturb_shock=True))
+    base = ops.stencil_operator(nchi, neta, 4, seed=9, n_shock=nchi,
+                                row_sum=0.5 / s["time_integration"]["dt"])
+    A = ops.receptivity_extension(base, w).to_scipy_varmajor()
+    v0 = np.ones(A.shape[0], dtype=complex)
+    _, Gf, _ = st.FREESTREAM_RECEPTIVITY(A, s, 3, 1.0, w, tol=1e-12, ncv=24, v0=v0, factored=True)
+    _, Ga, _ = st.FREESTREAM_RECEPTIVITY(A, s, 3, 1.0, w, tol=1e-12, ncv=24, v0=v0)
+    assert np.all(np.abs(Gf - Ga) <= 1e-9 * np.abs(Ga)), (Gf, Ga)
+    n_t = int(round(1.0 / s["time_integration"]["dt"]))
+    R_ = st.CONSTRUCT_R_(s, w)
+    C_ = R_.T @ st.CONSTRUCT_M_(s, [1, 1, 1, 1], w) @ R_
+    D_ = R_.T @ st.CONSTRUCT_M_INFTY_(s, [1, 1, 1, 1], 1.0, w, True) @ R_
+    ref = _dense_gains(A, C_, D_, 1.0 / n_t, n_t, n0=4 * nchi * neta, k=3) / 2
+    assert np.all(np.abs(Gf - ref) <= 1e-8 * np.abs(ref)), (Gf, ref)
+
+
 # ------------------------------------------------------------------ device Krylov
 
 def test_eigenvalues_device_krylov_matches_closed_form(orc):
