@@ -1168,7 +1168,8 @@ int hymor_op_set_batch_kernel(hymor_op *op, int use_mfma) {
 int hymor_op_autotune_batch(hymor_op *op, int nv, int reps, int *best_kernel, double *best_ms) {
     return guarded([&] {
         ARG_CHECK(op && best_kernel && best_ms, "null argument");
-        ARG_CHECK(nv == 16 || nv == 32 || nv == 64, "batch autotune covers widths 16, 32, 64");
+        ARG_CHECK(nv == 8 || nv == 16 || nv == 32 || nv == 64,
+                  "batch autotune covers widths 8, 16, 32, 64");
         ARG_CHECK(reps >= 1, "reps must be >= 1");
         ensure_ws(op, nv);
         hipEvent_t e0, e1;

@@ -694,9 +694,14 @@ def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=8, ncv=None, tol=1e-10,
             dev.synchronize()
         return apply
 
+    def prepare(dev, width):
+        if width in (8, 16, 32, 64):
+            dev.autotune_batch(nv=width, reps=3)
+        dev.set_graphs(True)
+
     t0 = time.perf_counter()
     dev = _device_op(A, s, dt, value_prec)
-    dev.set_graphs(True)
+    prepare(dev, block)
     try:
         tol1 = tol_f32 if (refine and value_prec == "f32") else tol
         vals, Y, i1 = block_krylov_schur(make_apply(dev), n, k, m, block, tol=tol1,
@@ -711,7 +716,7 @@ def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=8, ncv=None, tol=1e-10,
         nz = np.linalg.norm(Z, axis=0)
         Q, _ = np.linalg.qr(Z[:, nz > 1e-8 * nz.max()])       # drop the zero imaginary parts
         dev = _device_op(A, s, dt, "f64")
-        dev.set_graphs(True)
+        prepare(dev, refine_block)
         try:
             vals, Y, i2 = block_krylov_schur(make_apply(dev), n, k, max(refine_ncv, k + refine_block),
                                              refine_block, tol=tol, maxiter=maxiter,

@@ -198,7 +198,7 @@ int hymor_op_set_graphs(hymor_op *op, int enable);
  * (k_spmm_quad; k_spmm_tile at b = 4).  All accumulate in FP64. */
 int hymor_op_set_batch_kernel(hymor_op *op, int use_mfma);
 
-/* Times the eight batch kernels at width nv (16, 32 or 64) and keeps the
+/* Times the eight batch kernels at width nv (8, 16, 32 or 64) and keeps the
  * fastest; returns its id and ms per launch. */
 int hymor_op_autotune_batch(hymor_op *op, int nv, int reps, int *best_kernel, double *best_ms);
 
