@@ -32,6 +32,7 @@ SIGNATURES = {
                                       C.c_int, C.c_double, C.c_int, C.c_int, _pp]),
     "hymor_op_destroy": (C.c_int, [_p]),
     "hymor_op_info": (C.c_int, [_p, _pi64, C.c_int]),
+    "hymor_op_set_value_prec": (C.c_int, [_p, C.c_int]),
     "hymor_spmv": (C.c_int, [_p, C.c_int, _p, _p, C.c_int]),
     "hymor_spmv_dev": (C.c_int, [_p, C.c_int, _p, _p, C.c_int]),
     "hymor_expm_taylor": (C.c_int, [_p, C.c_int, C.c_int, _i64, C.c_double, C.c_int, _p, _p,

@@ -83,6 +83,20 @@ class DeviceOperator:
                                       1 if with_adjoint else 0, C.byref(h)))
         return cls(h)
 
+    def set_value_prec(self, value_prec: str):
+        """Switch the block values the kernels read between "f64" and "f32"
+        in place.  An FP64 upload keeps its values and casts an FP32 copy on
+        the device the first time; an FP32 upload cannot go back to FP64."""
+        check(_lib.load().hymor_op_set_value_prec(self._h, _PRECS[value_prec]))
+        self._refresh_info()
+        return self
+
+    def _refresh_info(self):
+        info = (C.c_int64 * 10)()
+        check(_lib.load().hymor_op_info(self._h, info, 10))
+        self.device_bytes = int(info[9])
+        self.value_prec = "f32" if int(info[8]) == PREC_F32 else "f64"
+
     # ------------------------------------------------------------ lifetime
     def close(self):
         if self._h is not None and self._h.value:

@@ -64,7 +64,9 @@ struct DevSliced {
     int64_t ncell = 0, nslice = 0, nnzb = 0, nnzb_pad = 0;
     int64_t *slice_ptr = nullptr;
     int32_t *col = nullptr;
-    void *val = nullptr;
+    void *val = nullptr;    // the values the kernels read: val64 or val32
+    void *val64 = nullptr;  // FP64 lane-interleaved values (an FP64 upload)
+    void *val32 = nullptr;  // FP32 values (an FP32 upload, or cast from val64)
     void *val_t = nullptr;  // the same values in k_spmm_tile's order (made on first use)
     void *val_q = nullptr;  // ... in k_spmm_quad's order (b = 8, made on first use)
 };
@@ -199,10 +201,22 @@ void upload_sliced(hymor_op *op, const HostBSR &a, int prec, DevSliced &d) {
     if (prec == HYMOR_PREC_F32) {
         std::vector<float> f(s.val.size());
         for (size_t i = 0; i < f.size(); ++i) f[i] = (float)s.val[i];
-        d.val = upload(op, f);
+        d.val = d.val32 = upload(op, f);
     } else {
-        d.val = upload(op, s.val);
+        d.val = d.val64 = upload(op, s.val);
     }
+}
+
+// frees one tracked allocation
+void dfree(hymor_op *op, void *p, size_t bytes) {
+    if (!p) return;
+    for (size_t k = 0; k < op->allocs.size(); ++k)
+        if (op->allocs[k] == p) {
+            (void)hipFree(p);
+            op->allocs.erase(op->allocs.begin() + (long)k);
+            op->dev_bytes -= (int64_t)bytes;
+            return;
+        }
 }
 
 void upload_extra(hymor_op *op, const HostExtra &h, DevExtra &d) {
@@ -854,6 +868,43 @@ int hymor_op_destroy(hymor_op *op) {
         if (op->stream) HIP_TRY(hipStreamSynchronize(op->stream));
         release(op);
         delete op;
+    });
+}
+
+int hymor_op_set_value_prec(hymor_op *op, int value_prec) {
+    return guarded([&] {
+        ARG_CHECK(op, "null argument");
+        ARG_CHECK(value_prec == HYMOR_PREC_F64 || value_prec == HYMOR_PREC_F32, "bad value_prec");
+        if (value_prec == op->prec) return;
+        for (DevSliced *s : {&op->fwd, &op->adj}) {
+            if (s == &op->adj && !op->has_adj) continue;
+            if (s->nnzb_pad == 0) { s->prec = value_prec; continue; }
+            ARG_CHECK(value_prec == HYMOR_PREC_F32 || s->val64,
+                      "the values were uploaded in FP32; FP64 needs a new upload");
+        }
+        drop_graph(op);
+        for (DevSliced *s : {&op->fwd, &op->adj}) {
+            if (s == &op->adj && !op->has_adj) continue;
+            if (s->nnzb_pad == 0) continue;
+            const int64_t total = s->nnzb_pad * s->b * s->b;
+            const size_t esz = s->prec == HYMOR_PREC_F32 ? sizeof(float) : sizeof(double);
+            // the tile/quad copies hold the old precision: rebuilt on next use
+            dfree(op, s->val_t, (size_t)total * esz);
+            dfree(op, s->val_q, (size_t)total * esz);
+            s->val_t = s->val_q = nullptr;
+            if (value_prec == HYMOR_PREC_F32 && !s->val32) {
+                s->val32 = dalloc(op, (size_t)total * sizeof(float));
+                const int64_t grid =
+                    std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 65536));
+                hipLaunchKernelGGL(k_vals_to_f32, dim3((unsigned)grid), dim3(256), 0, op->stream,
+                                   (const double *)s->val64, (float *)s->val32, total);
+                HIP_TRY(hipGetLastError());
+            }
+            s->val = value_prec == HYMOR_PREC_F32 ? s->val32 : s->val64;
+            s->prec = value_prec;
+        }
+        op->prec = value_prec;
+        HIP_TRY(hipStreamSynchronize(op->stream));
     });
 }
 

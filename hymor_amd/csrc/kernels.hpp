@@ -1030,6 +1030,21 @@ __global__ __launch_bounds__(256) void k_spmm_mfma_w(
 
 // Block values in the tile kernel's order: inside each block, value (m, k)
 // with k = 4*kc + kk at (m*4 + kk)*KC + kc (a lane's KC values adjacent).
+// FP64 values in the two-per-lane interleave (value k of lane L of a slot
+// at ((k/2)*64 + L)*2 + k%2) rewritten as FP32 in the four-per-lane
+// interleave ((k/4)*64 + L)*4 + k%4: the layout an FP32 upload builds.  A
+// slot holds 64*b values with b a multiple of 4, so slots are whole 256-value
+// runs and the index map needs no block size.
+__global__ void k_vals_to_f32(const double *__restrict__ src, float *__restrict__ dst,
+                              int64_t total) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = 4 * (i >> 8) + (i & 3);
+        const int64_t lane = (i & 255) >> 2;
+        dst[i] = (float)src[((k >> 1) * 64 + lane) * 2 + (k & 1)];
+    }
+}
+
 // Rewritten on the device from the lane-interleaved layout of the same
 // slots; one thread per value.
 template <typename VT>

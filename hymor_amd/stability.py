@@ -673,7 +673,9 @@ def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=8, ncv=None, tol=1e-10,
     accumulated in FP64, converged to tol_f32 on the stored operator.
     Phase 2 (refine, FP32 storage only): block Krylov-Schur on FP64-stored
     values, `refine_block` wide, seeded with an orthonormal basis of the
-    phase-1 Ritz vectors (real and imaginary parts), to tol.  Eigenvalues are
+    phase-1 Ritz vectors (real and imaginary parts), to tol.  Both phases
+    share one FP64 upload; phase 1 reads an FP32 copy cast on the device
+    (DeviceOperator.set_value_prec).  Eigenvalues are
     mapped as EIGENVALUES_GPU_ARNOLDI does: lambda = log(mu) / (n dt).
     A is a scipy matrix in the reference's var-major order or a
     BlockOperator.  Returns (V, D, info); info holds per-phase iterations,
@@ -700,31 +702,32 @@ def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=8, ncv=None, tol=1e-10,
         dev.set_graphs(True)
 
     t0 = time.perf_counter()
-    dev = _device_op(A, s, dt, value_prec)
-    prepare(dev, block)
+    two_phase = refine and value_prec == "f32"
+    # one FP64 upload serves both phases: FP32 values are cast on the device
+    dev = _device_op(A, s, dt, "f64" if two_phase else value_prec)
     try:
-        tol1 = tol_f32 if (refine and value_prec == "f32") else tol
-        vals, Y, i1 = block_krylov_schur(make_apply(dev), n, k, m, block, tol=tol1,
-                                         maxiter=maxiter)
+        info["upload_seconds"] = time.perf_counter() - t0
+        if two_phase:
+            dev.set_value_prec("f32")
+        prepare(dev, block)
+        vals, Y, i1 = block_krylov_schur(make_apply(dev), n, k, m, block,
+                                         tol=tol_f32 if two_phase else tol, maxiter=maxiter)
+        info["phase1"] = dict(i1, seconds=time.perf_counter() - t0, value_prec=value_prec,
+                              block=block)
+        if two_phase:
+            t1 = time.perf_counter()
+            Z = np.concatenate([Y.real, Y.imag], axis=1)
+            nz = np.linalg.norm(Z, axis=0)
+            Q, _ = np.linalg.qr(Z[:, nz > 1e-8 * nz.max()])   # drop the zero imaginary parts
+            dev.set_value_prec("f64")
+            prepare(dev, refine_block)
+            vals, Y, i2 = block_krylov_schur(make_apply(dev), n, k,
+                                             max(refine_ncv, k + refine_block), refine_block,
+                                             tol=tol, maxiter=maxiter, V0=Q[:, :refine_block])
+            info["phase2"] = dict(i2, seconds=time.perf_counter() - t1, value_prec="f64",
+                                  block=refine_block)
     finally:
         dev.close()
-    info["phase1"] = dict(i1, seconds=time.perf_counter() - t0, value_prec=value_prec,
-                          block=block)
-    if refine and value_prec == "f32":
-        t1 = time.perf_counter()
-        Z = np.concatenate([Y.real, Y.imag], axis=1)
-        nz = np.linalg.norm(Z, axis=0)
-        Q, _ = np.linalg.qr(Z[:, nz > 1e-8 * nz.max()])       # drop the zero imaginary parts
-        dev = _device_op(A, s, dt, "f64")
-        prepare(dev, refine_block)
-        try:
-            vals, Y, i2 = block_krylov_schur(make_apply(dev), n, k, max(refine_ncv, k + refine_block),
-                                             refine_block, tol=tol, maxiter=maxiter,
-                                             V0=Q[:, :refine_block])
-        finally:
-            dev.close()
-        info["phase2"] = dict(i2, seconds=time.perf_counter() - t1, value_prec="f64",
-                              block=refine_block)
     info["seconds"] = time.perf_counter() - t0
     order_ = np.argsort(-np.abs(vals), kind="stable")
     vals, Y = vals[order_], Y[:, order_]

@@ -101,6 +101,15 @@ int hymor_op_destroy(hymor_op *op);
  * [9]=device bytes held.  nout entries are written (at most 10). */
 int hymor_op_info(const hymor_op *op, int64_t *out, int nout);
 
+/* Switches the block values the kernels read to value_prec in place.  An
+ * operator uploaded with HYMOR_PREC_F64 keeps its FP64 values and casts an
+ * FP32 copy on the device on its first switch to HYMOR_PREC_F32 (both copies
+ * then stay resident); one uploaded with HYMOR_PREC_F32 cannot switch to
+ * FP64 (error).  Drops the captured Taylor graph.  No counterpart in the
+ * reference, which builds one CuSparseMatrixCSC per precision
+ * (EIGENVALUES.jl:139); it lets a mixed-precision eigensolve pay one upload. */
+int hymor_op_set_value_prec(hymor_op *op, int value_prec);
+
 /* y = A*x (transpose != 0: y = A'*x, the adjoint).  The reference's
  * mul!(y, A, x, 1.0, 0.0) on CuSparseMatrixCSC (ARNOLDI_POWER_GPU.jl:73). */
 int hymor_spmv(hymor_op *op, int transpose, const void *x, void *y, int dtype);

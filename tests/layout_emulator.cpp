@@ -470,3 +470,28 @@ extern "C" int emu_spmm_quad_csc(int64_t n, const int64_t *colptr, const int64_t
     }
     return g_oob;
 }
+
+// k_vals_to_f32's index map replayed over an FP64 (two-per-lane) interleave
+// and compared with the host's own four-per-lane interleave of the same
+// slots: returns the number of mismatching or out-of-range values.
+extern "C" int emu_vals_to_f32_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                                   const double *nzval, int b, int64_t ncell) {
+    g_oob = 0;
+    HostBSR fwd, adj;
+    HostExtra xf, xa;
+    csc_to_blocks(n, colptr, rowval, nzval, 1, false, 1.0, b, ncell, fwd, adj, xf, xa);
+    HostSliced s2, s4;
+    bsr_to_sliced(fwd, s2);
+    bsr_to_sliced(fwd, s4);
+    interleave_lanes(s2, 2);
+    interleave_lanes(s4, 4);
+    const int64_t total = s2.slice_ptr[s2.nslice] * b * b;
+    int bad = (int64_t)s2.val.size() == total ? 0 : 1;
+    for (int64_t i = 0; i < total; ++i) {
+        const int64_t k = 4 * (i >> 8) + (i & 3);
+        const int64_t lane = (i & 255) >> 2;
+        const float v = (float)at(s2.val, ((k >> 1) * 64 + lane) * 2 + (k & 1));
+        if (v != (float)at(s4.val, i)) ++bad;
+    }
+    return bad + g_oob;
+}

@@ -149,6 +149,41 @@ def test_f32_storage_error_bound(case, orc):
         assert 1e-12 < err <= 2e-6       # really FP32 values, bounded error
 
 
+def test_value_prec_switch_matches_fresh_upload(case):
+    """An FP64 upload switched to FP32 reads the same rounded values, in the
+    same interleave, as an FP32 upload: identical bits for the plain and the
+    batched (tile-order) actions; switching back restores the FP64 bits."""
+    name, op, A, dev = case
+    say(f"[{name}] value precision switch")
+    r = np.random.default_rng(11)
+    x = r.standard_normal(op.n)
+    X = np.asfortranarray(r.standard_normal((op.n, 8)))
+    kw = dict(order=5, nsub=3, dt=0.1)
+    with DeviceOperator.from_blocks(op, value_prec="f64") as d64, \
+            DeviceOperator.from_blocks(op, value_prec="f32") as d32:
+        ref64 = d64.expm_taylor(x, **kw)
+        ref64t = d64.expm_taylor(x, transpose=True, **kw)
+        ref64b = d64.expm_taylor_batched(X, **kw)
+        ref32 = d32.expm_taylor(x, **kw)
+        ref32t = d32.expm_taylor(x, transpose=True, **kw)
+        ref32b = d32.expm_taylor_batched(X, **kw)
+        bytes0 = d64.device_bytes
+        d64.set_value_prec("f32")
+        assert d64.value_prec == "f32" and d64.device_bytes > bytes0
+        np.testing.assert_array_equal(d64.expm_taylor(x, **kw), ref32)
+        np.testing.assert_array_equal(d64.expm_taylor(x, transpose=True, **kw), ref32t)
+        np.testing.assert_array_equal(d64.expm_taylor_batched(X, **kw), ref32b)
+        d64.set_value_prec("f64")
+        assert d64.value_prec == "f64"
+        np.testing.assert_array_equal(d64.expm_taylor(x, **kw), ref64)
+        np.testing.assert_array_equal(d64.expm_taylor(x, transpose=True, **kw), ref64t)
+        np.testing.assert_array_equal(d64.expm_taylor_batched(X, **kw), ref64b)
+        d64.set_value_prec("f32")      # the cast copy is reused
+        np.testing.assert_array_equal(d64.expm_taylor(x, **kw), ref32)
+        with pytest.raises(RuntimeError, match="FP32"):
+            d32.set_value_prec("f64")
+
+
 # ------------------------------------------------------------------ complex (receptivity)
 
 @pytest.fixture(scope="module")

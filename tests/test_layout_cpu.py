@@ -160,3 +160,15 @@ def test_emulated_batched_quad(emu, R, nrhs, vpl, wide):
                                 op.ncell, nrhs, R, vpl, X.ctypes.data, Y.ctypes.data)
     assert oob == 0
     assert np.allclose(Y, op.to_scipy_varmajor() @ X, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("b,seed", [(4, 1), (8, 2)])
+def test_emulated_value_cast_map(emu, b, seed):
+    """The on-device FP64 -> FP32 value rewrite (hymor_op_set_value_prec)
+    lands every value where an FP32 upload puts it."""
+    emu.emu_vals_to_f32_csc.restype = C.c_int
+    emu.emu_vals_to_f32_csc.argtypes = [C.c_int64] + [C.c_void_p] * 3 + [C.c_int, C.c_int64]
+    op = ops.stencil_operator(11, 9, b, seed=seed, n_shock=11)
+    colptr, rowval, nzval = op.to_csc_varmajor(one_based=True)
+    assert emu.emu_vals_to_f32_csc(op.n, colptr.ctypes.data, rowval.ctypes.data,
+                                   nzval.ctypes.data, b, op.ncell) == 0
