@@ -296,10 +296,11 @@ def main():
     y_plain = dy.download()
     dev.close()
     nt_s = str(bool(tuning[2])).lower()
+    # split 3 (x staged through LDS) runs k_term_split_p at one double per row
     kname = {0: f"k_term<{cfg['b']}, double, 1, {nt_s}>",
              1: f"k_term_split<{cfg['b']}, double, 1, {tuning[1]}, {nt_s}>",
              2: f"k_term_split_p<{cfg['b']}, double, 1, {tuning[1]}, "
-                f"{9 // tuning[1]}, {nt_s}>"}[tuning[0]]
+                f"{9 // tuning[1]}, {nt_s}>"}[min(tuning[0], 2)]
     traffic, factors = None, None
     if passes is not None:
         traffic, factors = traffic_from_passes(passes, kname)

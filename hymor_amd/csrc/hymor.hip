@@ -47,9 +47,10 @@ struct ArgError : std::invalid_argument {
 // operator by hymor_op_autotune (defaults: k_term_split, 4 waves per slice,
 // non-temporal value loads -- the usual pick on large operators).
 struct TermTuning {
-    int split = 1;   // 0: k_term (one wave per slice), 1: k_term_split,
-                     // 2: k_term_split_p (grouped loads)
-    int waves = 4;   // k_term: waves per workgroup; k_term_split: waves per slice
+    int split = 3;   // 0: k_term (one wave per slice), 1: k_term_split,
+                     // 2: k_term_split_p (grouped loads), 3: k_term_lds
+                     // (grouped loads, x staged through LDS; width 1 runs 2)
+    int waves = 3;   // k_term: waves per workgroup; the others: waves per slice
     int nt = 1;      // non-temporal loads of the block values
     int spmm_mfma = 6;  // batches of >= 16 doubles per row: 0 vector-FMA k_spmm,
                         // 1 k_spmm_mfma, 2/3 k_spmm_mfma_mc (2 or 1 cells per wave),
@@ -96,6 +97,9 @@ struct hymor_op {
     // direction: the forward and adjoint operators list different rows, and a
     // row the current direction never writes must read as zero
     double *yx = nullptr, *yxa = nullptr;
+    // the width and tile layout yx/yxa were last cleared for: k_extra writes
+    // only its listed rows, so a call in another layout starts from zeros
+    int yx_nv = -1, yx_tile = -1;
     int64_t dev_bytes = 0;
     std::vector<void *> allocs;
     // per-column complex diagonal on extra rows [cd0, cd1) for the current
@@ -253,7 +257,15 @@ void ensure_ws_only(hymor_op *op, int nv);
 // width runs k_spmm_tile
 void ensure_ws(hymor_op *op, int nv) {
     ensure_ws_only(op, nv);
-    if (tile_b_of(op, nv)) ensure_tile(op);
+    const int tb = tile_b_of(op, nv);
+    if (tb) ensure_tile(op);
+    if (op->yx_nv != nv || op->yx_tile != tb) {
+        const size_t yb = (size_t)op->nflow * op->ws_nv * sizeof(double);
+        HIP_TRY(hipMemsetAsync(op->yx, 0, yb, op->stream));
+        HIP_TRY(hipMemsetAsync(op->yxa, 0, yb, op->stream));
+        op->yx_nv = nv;
+        op->yx_tile = tb;
+    }
 }
 
 void ensure_ws_only(hymor_op *op, int nv) {
@@ -370,11 +382,28 @@ void launch_split_p(const DevSliced &s, hipStream_t st, const double *x, const d
                        s.ncell, x, yx, tout, vin, vout, c1, c2);
 }
 
+// split == 3: k_term_lds, as split 2 with the x blocks staged through LDS
+template <int B, typename VT, int NV, int NW, bool NT>
+void launch_lds(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+                double *tout, const double *vin, double *vout, double c1, double c2) {
+    constexpr int S = (9 + NW - 1) / NW;
+    hipLaunchKernelGGL((k_term_lds<B, VT, NV, NW, S, NT>), dim3((unsigned)s.nslice),
+                       dim3(64 * NW), 0, st, s.slice_ptr, s.col, (const VT *)s.val, s.nslice,
+                       s.ncell, x, yx, tout, vin, vout, c1, c2);
+}
+
 template <int B, typename VT, int NV, bool NT>
 void launch_term_nt(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
                     double *tout, const double *vin, double *vout, double c1, double c2) {
     const TermTuning &tu = s.tune;
-    if (tu.split == 2) {
+    if constexpr (NV >= 2) {
+        if (tu.split == 3) {
+            if (tu.waves == 9) launch_lds<B, VT, NV, 9, NT>(s, st, x, yx, tout, vin, vout, c1, c2);
+            else launch_lds<B, VT, NV, 3, NT>(s, st, x, yx, tout, vin, vout, c1, c2);
+            return;
+        }
+    }
+    if (tu.split >= 2) {
         if (tu.waves == 9) launch_split_p<B, VT, NV, 9, NT>(s, st, x, yx, tout, vin, vout, c1, c2);
         else launch_split_p<B, VT, NV, 3, NT>(s, st, x, yx, tout, vin, vout, c1, c2);
         return;
@@ -1163,14 +1192,17 @@ int hymor_op_autotune(hymor_op *op, int nv, int reps, double *best_ms) {
         ensure_ws(op, nv);
         const TermTuning cands[] = {{0, 4, 0, 1}, {0, 4, 1, 1}, {0, 2, 0, 1}, {1, 2, 0, 1},
                                     {1, 2, 1, 1}, {1, 4, 0, 1}, {1, 4, 1, 1}, {2, 3, 0, 1},
-                                    {2, 3, 1, 1}, {2, 9, 0, 1}, {2, 9, 1, 1}};
+                                    {2, 3, 1, 1}, {2, 9, 0, 1}, {2, 9, 1, 1}, {3, 3, 0, 1},
+                                    {3, 3, 1, 1}, {3, 9, 0, 1}, {3, 9, 1, 1}};
+        const int ncand = nv >= 2 ? 15 : 11;   // k_term_lds needs two doubles per row
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         double best = 1e300;
         TermTuning pick = op->fwd.tune;
         const int batch_kernel = op->fwd.tune.spmm_mfma;   // not this autotuner's choice
-        for (TermTuning c : cands) {
+        for (int ci = 0; ci < ncand; ++ci) {
+            TermTuning c = cands[ci];
             c.spmm_mfma = batch_kernel;
             op->fwd.tune = c;
             launch_term(op, false, nv, op->stream, op->ws[4], op->ws[5], op->ws[2], op->ws[2], 0.5,
@@ -1252,6 +1284,7 @@ int hymor_op_autotune_batch(hymor_op *op, int nv, int reps, int *best_kernel, do
         op->adj.tune.spmm_mfma = pick;
         drop_graph(op);
         *best_kernel = pick;
+        op->yx_nv = -1;   // launches above wrote yx in several layouts
         *best_ms = best;
     });
 }
@@ -1270,9 +1303,9 @@ int hymor_op_set_tuning(hymor_op *op, int split, int waves, int nt) {
         ARG_CHECK(op, "null operator");
         ARG_CHECK((split == 0 && (waves == 1 || waves == 2 || waves == 4)) ||
                       (split == 1 && (waves == 2 || waves == 4)) ||
-                      (split == 2 && (waves == 3 || waves == 9)),
+                      ((split == 2 || split == 3) && (waves == 3 || waves == 9)),
                   "k_term takes 1, 2 or 4 waves per group, k_term_split 2 or 4 waves per "
-                  "slice, k_term_split_p 3 or 9");
+                  "slice, k_term_split_p and k_term_lds 3 or 9");
         ARG_CHECK(nt == 0 || nt == 1, "nt must be 0 or 1");
         op->fwd.tune.split = split;
         op->fwd.tune.waves = waves;

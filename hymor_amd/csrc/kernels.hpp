@@ -741,6 +741,147 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
     }
 }
 
+// Orders one wave's LDS accesses around a hand-off between its lanes: the
+// LDS unit runs a wave's instructions in issue order, so keeping the
+// compiler from moving accesses across this point is all that is needed.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// k_term_split_p for NV >= 2 doubles per row with the gathered x blocks
+// staged through LDS.  In k_term_split_p every lane of a cell loads the
+// cell's whole B*NV-double x block (the B lanes of a cell read the same
+// addresses); here the B lanes load it once as contiguous 16-byte chunks
+// (NV/2 per lane), write it to the wave's LDS area and read it back with
+// broadcast reads: B times fewer vector-memory instructions for x.  Cell
+// areas are padded by one chunk so the CPS cells of a read fall in
+// different banks.  Same slot order, FMA order and LDS combination as
+// k_term_split_p, hence the same bits.
+template <int B, typename VT, int NV, int NW, int S, bool NT>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_term_lds(
+    const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
+    const VT *__restrict__ val, int64_t nslice, int64_t ncell,
+    const double *__restrict__ x, const double *__restrict__ yx,
+    double *__restrict__ tout, const double *vin, double *vout, double c1, double c2) {
+    constexpr int CPS = 64 / B;
+    constexpr int BB = B * B;
+    constexpr int V = Vpl<VT>::value;
+    constexpr int NC = B / V;       // 16-byte value chunks per lane and slot
+    constexpr int XC = B * NV / 2;  // 16-byte chunks per x block
+    constexpr int CPL = NV / 2;     // x chunks each lane loads
+    constexpr int XS = XC + 1;      // padded chunk stride of a cell's area
+    static_assert(NV >= 2 && NV % 2 == 0, "x blocks move as pairs of doubles");
+    typedef typename Chunk<VT>::type CT;
+    __shared__ double part[NW > 1 ? NW - 1 : 1][NV][64];
+    __shared__ d2 xs[NW][S][CPS * XS];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int64_t slice = xcd_tile(blockIdx.x, gridDim.x);
+    if (slice >= nslice) return;
+    const int q = lane / B, r = lane % B;
+    const int64_t p0 = slice_ptr[slice];
+    const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
+    const int32_t *cp = col + p0 + q;
+    const VT *vp = val + p0 * BB;
+    const int64_t cell = slice * CPS + q;
+    const int64_t row = (cell < ncell ? cell : ncell - 1) * B + r;
+    double pre_v[NV], pre_y[NV];
+    {
+        const double *pv = vout ? vin : x;
+        const double *py = yx ? yx : x;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            pre_v[v] = w == 0 ? pv[row * NV + v] : 0.0;
+            pre_y[v] = w == 0 ? py[row * NV + v] : 0.0;
+        }
+    }
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+    d2 *mine = &xs[w][0][0];
+    const int nmine = width > w ? (width - w + NW - 1) / NW : 0;
+    for (int g = 0; g < nmine; g += S) {
+        int32_t c[S];
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const int i = g + u < nmine ? g + u : nmine - 1;
+            c[u] = cp[(int64_t)(w + i * NW) * CPS];
+        }
+        // the values (the HBM stream) go out with the columns; the x chunks
+        // follow once the columns land.  Nothing may move above the x loads
+        // (converting a value early would wait for every value first).
+        CT raw[S][NC];
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const int i = g + u < nmine ? g + u : nmine - 1;
+            const VT *base = vp + (int64_t)(w + i * NW) * CPS * BB;
+#pragma unroll
+            for (int kc = 0; kc < NC; ++kc) {
+                const CT *pc = reinterpret_cast<const CT *>(base + ((int64_t)kc * 64 + lane) * V);
+                raw[u][kc] = NT ? __builtin_nontemporal_load(pc) : *pc;
+            }
+        }
+        d2 xr[S][CPL];
+#pragma unroll
+        for (int u = 0; u < S; ++u)
+#pragma unroll
+            for (int i = 0; i < CPL; ++i)
+                xr[u][i] = *reinterpret_cast<const d2 *>(x + (int64_t)c[u] * (B * NV) +
+                                                         (r * CPL + i) * 2);
+        __builtin_amdgcn_sched_barrier(0);
+        wave_lds_sync();  // the previous group's reads are issued
+#pragma unroll
+        for (int u = 0; u < S; ++u)
+#pragma unroll
+            for (int i = 0; i < CPL; ++i) mine[u * CPS * XS + q * XS + r * CPL + i] = xr[u][i];
+        wave_lds_sync();
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const bool live = g + u < nmine;
+            const d2 *xb = mine + u * CPS * XS + q * XS;
+            // the whole block first: all reads in flight before the FMAs
+            d2 xx[XC];
+#pragma unroll
+            for (int i = 0; i < XC; ++i) xx[i] = xb[i];
+#pragma unroll
+            for (int kc = 0; kc < NC; ++kc)
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    const int k = kc * V + e;
+                    const double ak = live ? (double)raw[u][kc][e] : 0.0;
+#pragma unroll
+                    for (int vp2 = 0; vp2 < NV / 2; ++vp2) {
+                        acc[2 * vp2] = fma(ak, xx[k * (NV / 2) + vp2].x, acc[2 * vp2]);
+                        acc[2 * vp2 + 1] = fma(ak, xx[k * (NV / 2) + vp2].y, acc[2 * vp2 + 1]);
+                    }
+                }
+        }
+    }
+    if constexpr (NW > 1) {
+        if (w > 0) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) part[w - 1][v][lane] = acc[v];
+        }
+        __syncthreads();
+        if (w > 0) return;
+#pragma unroll
+        for (int i = 0; i < NW - 1; ++i)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] += part[i][v][lane];
+    }
+    if (cell >= ncell) return;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double y = acc[v];
+        if (yx) y += pre_y[v];
+        const double t = c1 * y;
+        if (tout) tout[row * NV + v] = t;
+        if (vout) vout[row * NV + v] = fma(c2, t, pre_v[v]);
+    }
+}
+
 }  // namespace hymor
 
 namespace hymor {

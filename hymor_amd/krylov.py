@@ -196,7 +196,7 @@ def krylov_schur(apply_dev, n: int, k: int, m: int, *, complex_: bool = False, t
 
 
 def block_krylov_schur(apply_block, n: int, k: int, m: int, s: int, *, complex_: bool = False,
-                       tol: float = 1e-10, maxiter: int = 200, V0=None):
+                       tol: float = 1e-10, maxiter: int = 200, V0=None, basis: int = 0):
     """Block Krylov-Schur: ``apply_block(d_in, ld, nvec, d_out)`` applies the
     operator to nvec consecutive device columns (leading dimension ld) at
     once -- the batched SpMM action -- and every expansion step feeds it the
@@ -205,7 +205,10 @@ def block_krylov_schur(apply_block, n: int, k: int, m: int, s: int, *, complex_:
     matrix is band Hessenberg with s sub-diagonals.
 
     V0 (n x s, optional) seeds the first block, e.g. Ritz vectors of an
-    earlier, cheaper solve.  Returns (theta, Y, info) as krylov_schur.
+    earlier, cheaper solve.  Returns (theta, Y, info) as krylov_schur; with
+    basis > 0, info["basis"] also holds an orthonormal basis (n x about
+    `basis` columns) of the Schur subspace of the `basis` largest-magnitude
+    Ritz values -- the seed for a refinement on a more accurate operator.
     """
     if not (1 <= k and k + s <= m):
         raise ValueError("need k + s <= m")
@@ -271,7 +274,17 @@ def block_krylov_schur(apply_block, n: int, k: int, m: int, s: int, *, complex_:
                 else:
                     Y = K.ritz(np.ascontiguousarray(Q.real)) + 1j * K.ritz(np.ascontiguousarray(Q.imag))
                 Y = Y / np.linalg.norm(Y, axis=0)
-                return w, Y, {"iterations": it, "applications": n_apply, "residual_estimates": res}
+                info = {"iterations": it, "applications": n_apply, "residual_estimates": res}
+                if basis:
+                    nb = min(basis, mm)
+                    thr_b = _select_lm(evals, nb) * (1 - 1e-12)
+                    if complex_:
+                        _, Zb, sd = sl.schur(Hm, output="complex", sort=lambda z: abs(z) >= thr_b)
+                    else:
+                        _, Zb, sd = sl.schur(Hm, output="real",
+                                             sort=lambda re, im: abs(complex(re, im)) >= thr_b)
+                    info["basis"] = K.ritz(np.ascontiguousarray(Zb[:, :min(max(sd, nb), mm)]))
+                return w, Y, info
             # restart: keep p Schur vectors, then the residual block
             p = max(p_keep, min(p_keep + (k - int(conv.sum())), mm - s))
             if not complex_ and p < mm and p > p_keep and abs(T[p, p - 1]) > 0:

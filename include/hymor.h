@@ -185,9 +185,11 @@ int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int reps,
 
 /* Autotuner, no reference counterpart: times the launch variants of the
  * single-vector Taylor-term kernel (one wave per slice with 2 or 4 waves per
- * group, or split-slot with 2 or 4 waves per slice; plain or non-temporal
- * block loads) on this operator, `reps` launches each, and keeps the
- * fastest for both directions.  nv: 1, 2 or 4 doubles per row. */
+ * group; split-slot with 2 or 4 waves per slice; grouped loads with 3 or 9
+ * waves per slice; at nv >= 2 also grouped loads with the x blocks staged
+ * through LDS; plain or non-temporal block loads) on this operator, `reps`
+ * launches each, and keeps the fastest for both directions.  nv: 1, 2 or 4
+ * doubles per row. */
 int hymor_op_autotune(hymor_op *op, int nv, int reps, double *best_ms);
 
 /* enable != 0: the Taylor sweep of hymor_expm_taylor(_dev) and the batched
@@ -212,7 +214,8 @@ int hymor_op_set_batch_kernel(hymor_op *op, int use_mfma);
 int hymor_op_autotune_batch(hymor_op *op, int nv, int reps, int *best_kernel, double *best_ms);
 
 /* Read / force the chosen variant (split 0 k_term | 1 k_term_split |
- * 2 k_term_split_p, waves, nt 0|1). */
+ * 2 k_term_split_p | 3 k_term_lds (k_term_split_p at one double per row),
+ * waves, nt 0|1). */
 int hymor_op_get_tuning(const hymor_op *op, int *split, int *waves, int *nt);
 int hymor_op_set_tuning(hymor_op *op, int split, int waves, int nt);
 

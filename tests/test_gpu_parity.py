@@ -149,6 +149,30 @@ def test_f32_storage_error_bound(case, orc):
         assert 1e-12 < err <= 2e-6       # really FP32 values, bounded error
 
 
+def test_mixed_widths_on_one_handle(case):
+    """Plain and batched calls interleaved on one handle give the bits a
+    fresh handle gives: the extra rows' partial sums of one width or layout
+    never leak into another (shock rows exercise them)."""
+    name, op, A, dev = case
+    say(f"[{name}] mixed widths")
+    r = np.random.default_rng(12)
+    x = r.standard_normal(op.n)
+    X = np.asfortranarray(r.standard_normal((op.n, 8)))
+    X2 = np.asfortranarray(r.standard_normal((op.n, 2)))
+    kw = dict(order=5, nsub=2, dt=0.1)
+    with DeviceOperator.from_blocks(op) as fresh:
+        refb = fresh.expm_taylor_batched(X, **kw)
+    with DeviceOperator.from_blocks(op) as fresh:
+        ref2 = fresh.expm_taylor_batched(X2, **kw)
+    with DeviceOperator.from_blocks(op) as fresh:
+        ref1 = fresh.expm_taylor(x, **kw)
+    with DeviceOperator.from_blocks(op) as d:
+        for _ in range(2):
+            np.testing.assert_array_equal(d.expm_taylor_batched(X, **kw), refb)
+            np.testing.assert_array_equal(d.expm_taylor(x, **kw), ref1)
+            np.testing.assert_array_equal(d.expm_taylor_batched(X2, **kw), ref2)
+
+
 def test_value_prec_switch_matches_fresh_upload(case):
     """An FP64 upload switched to FP32 reads the same rounded values, in the
     same interleave, as an FP32 upload: identical bits for the plain and the
@@ -323,7 +347,34 @@ def test_forward_after_adjoint_with_extras(orc):
 
 
 VARIANTS = [(0, 1, 0), (0, 2, 1), (0, 4, 0), (0, 4, 1), (1, 2, 0), (1, 2, 1), (1, 4, 0),
-            (1, 4, 1), (2, 3, 0), (2, 3, 1), (2, 9, 0), (2, 9, 1)]
+            (1, 4, 1), (2, 3, 0), (2, 3, 1), (2, 9, 0), (2, 9, 1), (3, 3, 0), (3, 3, 1),
+            (3, 9, 0), (3, 9, 1)]
+
+
+@pytest.mark.parametrize("waves,nt", [(3, 0), (3, 1), (9, 1)])
+@pytest.mark.parametrize("b,nrhs", [(4, 2), (4, 4), (8, 2), (8, 4)])
+@pytest.mark.parametrize("kind", ["shock", "wide"])
+def test_lds_staged_small_batches(orc, kind, b, nrhs, waves, nt):
+    """k_term_lds (x blocks staged through LDS) at 2 and 4 doubles per row:
+    the oracle's action on every column, forward and adjoint, and the same
+    bits as k_term_split_p with the same waves (same summation order)."""
+    say(f"[lds] {kind} b={b} nrhs={nrhs} waves={waves} nt={nt}")
+    if kind == "shock":
+        op = ops.stencil_operator(12, 10, b, seed=30 + b, n_shock=12, row_sum=2.0)
+    else:
+        op = random_pattern_operator(9, 7, b, 19, seed=40 + b)
+    A = orc.CSC.from_arrays(*op.to_csc_varmajor())
+    At = orc.csc_adjoint(A)
+    X = np.asfortranarray(np.random.default_rng(22).standard_normal((op.n, nrhs)))
+    kw = dict(order=5, nsub=3, dt=0.1)
+    with DeviceOperator.from_blocks(op) as dev, DeviceOperator.from_blocks(op) as ref:
+        dev.set_tuning(3, waves, nt)
+        ref.set_tuning(2, waves, nt)
+        for tr, M in ((False, A), (True, At)):
+            got = dev.expm_taylor_batched(X, transpose=tr, **kw)
+            np.testing.assert_array_equal(got, ref.expm_taylor_batched(X, transpose=tr, **kw))
+            for j in range(nrhs):
+                assert relerr(got[:, j], orc.power_method(M.scaled(0.1), X[:, j], 3)) <= 1e-12
 
 
 @pytest.mark.parametrize("split,waves,nt", VARIANTS)
@@ -355,7 +406,7 @@ def test_autotune_picks_a_valid_variant(orc):
     say("[autotune] C1")
     with DeviceOperator.from_blocks(op) as dev:
         ms, tun = dev.autotune(reps=3)
-        assert ms > 0 and tun[0] in (0, 1, 2)
+        assert ms > 0 and tun[0] in (0, 1, 2, 3)
         x = np.random.default_rng(16).standard_normal(op.n)
         A = orc.CSC.from_arrays(*op.to_csc_varmajor())
         got = dev.expm_taylor(x, order=12, nsub=4, dt=1 / 64)
