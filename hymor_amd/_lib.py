@@ -54,6 +54,8 @@ SIGNATURES = {
     "hymor_dev_free": (C.c_int, [_p]),
     "hymor_memcpy_h2d": (C.c_int, [_p, _p, _i64]),
     "hymor_memcpy_d2h": (C.c_int, [_p, _p, _i64]),
+    "hymor_memcpy_d2d": (C.c_int, [_p, _p, _i64]),
+    "hymor_dev_fill_uniform": (C.c_int, [_p, _i64, C.c_uint64]),
     "hymor_bench_term_kernel": (C.c_int, [_p, C.c_int, C.c_int, C.c_int, _pd, _pd]),
     "hymor_op_autotune": (C.c_int, [_p, C.c_int, C.c_int, _pd]),
     "hymor_op_set_graphs": (C.c_int, [_p, C.c_int]),

@@ -157,6 +157,69 @@ void bsr_transpose(const HostBSR &a, HostBSR &t) {
     }
 }
 
+void bsr_transpose_map(int64_t ncell, const std::vector<int64_t> &ptr,
+                       const std::vector<int32_t> &col, std::vector<int64_t> &tptr,
+                       std::vector<int32_t> &tcol, std::vector<int64_t> &tsrc) {
+    // the block order of bsr_transpose: inside a transposed row, ascending
+    // forward row
+    const int64_t nnzb = ptr[ncell];
+    tptr.assign(ncell + 1, 0);
+    for (int64_t k = 0; k < nnzb; ++k) tptr[(int64_t)col[k] + 1]++;
+    for (int64_t c = 0; c < ncell; ++c) tptr[c + 1] += tptr[c];
+    tcol.resize(nnzb);
+    tsrc.resize(nnzb);
+    std::vector<int64_t> pos(tptr.begin(), tptr.end() - 1);
+    for (int64_t r = 0; r < ncell; ++r)
+        for (int64_t k = ptr[r]; k < ptr[r + 1]; ++k) {
+            const int64_t d = pos[col[k]]++;
+            tcol[d] = (int32_t)r;
+            tsrc[d] = k;
+        }
+}
+
+void bsr_to_sliced_map(int b, int64_t ncell, const std::vector<int64_t> &ptr,
+                       const std::vector<int32_t> &col, const std::vector<int64_t> *src,
+                       HostSliced &s, std::vector<int64_t> &pad_src) {
+    // the slices, slots and padding of bsr_to_sliced, without the values
+    const int cps = 64 / b;
+    s.b = b;
+    s.cps = cps;
+    s.ncell = ncell;
+    s.nslice = (ncell + cps - 1) / cps;
+    s.nnzb = ptr[ncell];
+    s.slice_ptr.assign(s.nslice + 1, 0);
+    for (int64_t sl = 0; sl < s.nslice; ++sl) {
+        int64_t w = 0;
+        for (int q = 0; q < cps; ++q) {
+            const int64_t cell = sl * cps + q;
+            if (cell < ncell) w = std::max(w, ptr[cell + 1] - ptr[cell]);
+        }
+        s.slice_ptr[sl + 1] = s.slice_ptr[sl] + w * cps;
+    }
+    const int64_t tot = s.slice_ptr[s.nslice];
+    s.col.assign(tot, 0);
+    s.val.clear();
+    pad_src.assign(tot, -1);
+    for (int64_t sl = 0; sl < s.nslice; ++sl) {
+        const int64_t w = (s.slice_ptr[sl + 1] - s.slice_ptr[sl]) / cps;
+        for (int q = 0; q < cps; ++q) {
+            const int64_t cell = sl * cps + q;
+            const int64_t len = cell < ncell ? ptr[cell + 1] - ptr[cell] : 0;
+            const int32_t self = (int32_t)(cell < ncell ? cell : 0);
+            for (int64_t t = 0; t < w; ++t) {
+                const int64_t d = s.slice_ptr[sl] + t * cps + q;
+                if (t < len) {
+                    const int64_t k = ptr[cell] + t;
+                    s.col[d] = col[k];
+                    pad_src[d] = src ? (*src)[k] : k;
+                } else {
+                    s.col[d] = self;
+                }
+            }
+        }
+    }
+}
+
 void extra_from_coo(int64_t n, int64_t nflow, const std::vector<int64_t> &r,
                     const std::vector<int64_t> &c, const std::vector<double> &v,
                     bool is_complex, bool adjoint, HostExtra &out) {

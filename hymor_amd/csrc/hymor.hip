@@ -189,10 +189,34 @@ T *upload(hymor_op *op, const std::vector<T> &h) {
     return d;
 }
 
-void upload_sliced(hymor_op *op, const HostBSR &a, int prec, DevSliced &d) {
+// Device memory for one call (upload staging), freed on scope exit.
+struct DevTemp {
+    void *p = nullptr;
+    explicit DevTemp(size_t bytes) {
+        if (hipMalloc(&p, bytes ? bytes : 256) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            throw std::bad_alloc();
+        }
+    }
+    ~DevTemp() {
+        if (p) (void)hipFree(p);
+    }
+    DevTemp(const DevTemp &) = delete;
+    DevTemp &operator=(const DevTemp &) = delete;
+};
+
+// d from a BSR structure (ptr, col) whose block k takes its values from
+// block src[k] (k when src is null) of raw_dev, b x b row-major doubles on
+// the device, read transposed when `transpose`, times scale.  The host
+// builds the slices and the per-slot source indices; k_slice_vals places,
+// scales and casts the values on the device.
+void build_sliced(hymor_op *op, int b, int64_t ncell, const std::vector<int64_t> &ptr,
+                  const std::vector<int32_t> &col, const std::vector<int64_t> *src,
+                  bool transpose, const double *raw_dev, double scale, int prec, DevSliced &d) {
     HostSliced s;
-    bsr_to_sliced(a, s);
-    interleave_lanes(s, prec == HYMOR_PREC_F32 ? 4 : 2);
+    std::vector<int64_t> pad_src;
+    bsr_to_sliced_map(b, ncell, ptr, col, src, s, pad_src);
     d.b = s.b;
     d.cps = s.cps;
     d.prec = prec;
@@ -202,13 +226,35 @@ void upload_sliced(hymor_op *op, const HostBSR &a, int prec, DevSliced &d) {
     d.nnzb_pad = s.slice_ptr[s.nslice];
     d.slice_ptr = upload(op, s.slice_ptr);
     d.col = upload(op, s.col);
+    const int64_t total = d.nnzb_pad * b * b;
+    DevTemp dsrc(pad_src.size() * sizeof(int64_t));
+    if (!pad_src.empty())
+        HIP_TRY(hipMemcpy(dsrc.p, pad_src.data(), pad_src.size() * sizeof(int64_t),
+                          hipMemcpyHostToDevice));
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 65536));
     if (prec == HYMOR_PREC_F32) {
-        std::vector<float> f(s.val.size());
-        for (size_t i = 0; i < f.size(); ++i) f[i] = (float)s.val[i];
-        d.val = d.val32 = upload(op, f);
+        d.val = d.val32 = dalloc(op, (size_t)total * sizeof(float));
+        hipLaunchKernelGGL(k_slice_vals<float>, dim3((unsigned)grid), dim3(256), 0, op->stream,
+                           raw_dev, (const int64_t *)dsrc.p, (float *)d.val32, total, b,
+                           transpose ? 1 : 0, scale);
     } else {
-        d.val = d.val64 = upload(op, s.val);
+        d.val = d.val64 = dalloc(op, (size_t)total * sizeof(double));
+        hipLaunchKernelGGL(k_slice_vals<double>, dim3((unsigned)grid), dim3(256), 0, op->stream,
+                           raw_dev, (const int64_t *)dsrc.p, (double *)d.val64, total, b,
+                           transpose ? 1 : 0, scale);
     }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(op->stream));   // before dsrc (and raw) are freed
+}
+
+// d from a host BSR whose values are final (scaled)
+void upload_sliced(hymor_op *op, const HostBSR &a, int prec, DevSliced &d) {
+    DevTemp raw(a.val.size() * sizeof(double));
+    if (!a.val.empty())
+        HIP_TRY(hipMemcpy(raw.p, a.val.data(), a.val.size() * sizeof(double),
+                          hipMemcpyHostToDevice));
+    build_sliced(op, a.b, a.ncell, a.ptr, a.col, nullptr, false, (const double *)raw.p, 1.0, prec,
+                 d);
 }
 
 // frees one tracked allocation
@@ -854,8 +900,6 @@ int hymor_op_create_bsr(int64_t nchi, int64_t neta, int vars_per_cell, int64_t n
         fwd.col.assign(bcol, bcol + nnzb);
         for (int64_t k = 0; k < nnzb; ++k)
             ARG_CHECK(fwd.col[k] >= 0 && fwd.col[k] < ncell, "bcol out of range");
-        fwd.val.resize((size_t)nnzb * bb);
-        for (size_t i = 0; i < fwd.val.size(); ++i) fwd.val[i] = bval[i] * scale;
         const bool cplx = dtype == HYMOR_C128;
         const int vs = cplx ? 2 : 1;
         std::vector<int64_t> r(xrow, xrow + nx), c(xcol, xcol + nx);
@@ -871,13 +915,21 @@ int hymor_op_create_bsr(int64_t nchi, int64_t neta, int vars_per_cell, int64_t n
         op->n = n;
         op->prec = value_prec;
         op->is_complex = cplx;
-        upload_sliced(op, fwd, value_prec, op->fwd);
+        // the caller's values go to the device once; both directions are
+        // laid out (scaled, cast, transposed for the adjoint) from there
+        DevTemp raw((size_t)nnzb * bb * sizeof(double));
+        if (nnzb)
+            HIP_TRY(hipMemcpy(raw.p, bval, (size_t)nnzb * bb * sizeof(double),
+                              hipMemcpyHostToDevice));
+        build_sliced(op, b, ncell, fwd.ptr, fwd.col, nullptr, false, (const double *)raw.p, scale,
+                     value_prec, op->fwd);
         upload_extra(op, xf, op->xf);
         if (with_adjoint) {
-            HostBSR adj;
-            bsr_transpose(fwd, adj);
-            fwd = HostBSR();
-            upload_sliced(op, adj, value_prec, op->adj);
+            std::vector<int64_t> tptr, tsrc;
+            std::vector<int32_t> tcol;
+            bsr_transpose_map(ncell, fwd.ptr, fwd.col, tptr, tcol, tsrc);
+            build_sliced(op, b, ncell, tptr, tcol, &tsrc, true, (const double *)raw.p, scale,
+                         value_prec, op->adj);
             extra_from_coo(n, nflow, r, c, v, cplx, true, xa);
             upload_extra(op, xa, op->xa);
             op->has_adj = true;
@@ -1169,10 +1221,29 @@ int hymor_dev_free(void *d_ptr) {
     });
 }
 
+int hymor_dev_fill_uniform(void *d_x, int64_t len, uint64_t seed) {
+    return guarded([&] {
+        ARG_CHECK(d_x && len >= 0, "bad argument");
+        if (len == 0) return;
+        const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((len + 255) / 256, 16384));
+        hipLaunchKernelGGL(k_fill_uniform, dim3((unsigned)grid), dim3(256), 0, 0, (double *)d_x,
+                           len, seed);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipDeviceSynchronize());
+    });
+}
+
 int hymor_memcpy_h2d(void *d_dst, const void *h_src, int64_t bytes) {
     return guarded([&] {
         ARG_CHECK(d_dst && h_src && bytes >= 0, "bad argument");
         HIP_TRY(hipMemcpy(d_dst, h_src, (size_t)bytes, hipMemcpyHostToDevice));
+    });
+}
+
+int hymor_memcpy_d2d(void *d_dst, const void *d_src, int64_t bytes) {
+    return guarded([&] {
+        ARG_CHECK(d_dst && d_src && bytes >= 0, "bad argument");
+        HIP_TRY(hipMemcpy(d_dst, d_src, (size_t)bytes, hipMemcpyDeviceToDevice));
     });
 }
 

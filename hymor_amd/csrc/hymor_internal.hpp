@@ -61,6 +61,20 @@ void extra_from_coo(int64_t n, int64_t nflow, const std::vector<int64_t> &r,
 
 void bsr_to_sliced(const HostBSR &a, HostSliced &s);
 
+// Structure of the transpose of a BSR given by (ptr, col): its ptr, col and,
+// per block, the forward block it mirrors (its values read transposed).
+void bsr_transpose_map(int64_t ncell, const std::vector<int64_t> &ptr,
+                       const std::vector<int32_t> &col, std::vector<int64_t> &tptr,
+                       std::vector<int32_t> &tcol, std::vector<int64_t> &tsrc);
+
+// bsr_to_sliced's structure (s.val stays empty) and, per padded block, the
+// source block of its values: src[k] for the BSR's block k (k itself when
+// src is null), -1 for a zero padding block.  The values are placed on the
+// device from these indices.
+void bsr_to_sliced_map(int b, int64_t ncell, const std::vector<int64_t> &ptr,
+                       const std::vector<int32_t> &col, const std::vector<int64_t> *src,
+                       HostSliced &s, std::vector<int64_t> &pad_src);
+
 // Lane-interleaved value order inside every slot (vpl values per 16-byte
 // chunk: 2 for FP64 storage, 4 for FP32).
 void interleave_lanes(HostSliced &s, int vpl);

@@ -548,6 +548,20 @@ __global__ __launch_bounds__(256) void k_basis_combine(const double *__restrict_
     }
 }
 
+// x[i] uniform in [-1, 1): splitmix64 (Steele, Lea and Flood, OOPSLA 2014)
+// of the counter seed + i * golden-ratio increment, top 53 bits
+__global__ __launch_bounds__(256) void k_fill_uniform(double *__restrict__ x, int64_t len,
+                                                      uint64_t seed) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t z = seed + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        x[i] = (double)(z >> 11) * 0x1.0p-52 - 1.0;
+    }
+}
+
 // x[0 : ncomp*n) *= s (real scale)
 __global__ __launch_bounds__(256) void k_scale(double *__restrict__ x, int64_t len, double s) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len;
@@ -1171,6 +1185,32 @@ __global__ __launch_bounds__(256) void k_spmm_mfma_w(
 
 // Block values in the tile kernel's order: inside each block, value (m, k)
 // with k = 4*kc + kk at (m*4 + kk)*KC + kc (a lane's KC values adjacent).
+// Block values placed in the sliced, lane-interleaved layout on the device:
+// padded block d takes block src[d] of raw (b x b, row-major; read
+// transposed for the adjoint) times scale, as VT, value k of lane
+// L = q*b + r of a slot at ((k/V)*64 + L)*V + k%V with V = 16/sizeof(VT);
+// src[d] < 0 is a zero padding block.  A slot holds 64*b values (its CPS
+// blocks), so slot g, lane group q is padded block g*CPS + q.
+template <typename VT>
+__global__ void k_slice_vals(const double *__restrict__ raw, const int64_t *__restrict__ src,
+                             VT *__restrict__ dst, int64_t total, int b, int transpose,
+                             double scale) {
+    constexpr int V = Vpl<VT>::value;
+    const int bb = b * b, cps = 64 / b, slot_len = 64 * b;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t slot = i / slot_len;
+        const int j = (int)(i - slot * slot_len);
+        const int kq = j / (64 * V), rem = j - kq * 64 * V;
+        const int lane = rem / V, k = kq * V + rem % V;
+        const int q = lane / b, r = lane - q * b;
+        const int64_t sb = src[slot * cps + q];
+        double v = 0.0;
+        if (sb >= 0) v = raw[sb * bb + (transpose ? k * b + r : r * b + k)] * scale;
+        dst[i] = (VT)v;
+    }
+}
+
 // FP64 values in the two-per-lane interleave (value k of lane L of a slot
 // at ((k/2)*64 + L)*2 + k%2) rewritten as FP32 in the four-per-lane
 // interleave ((k/4)*64 + L)*4 + k%4: the layout an FP32 upload builds.  A

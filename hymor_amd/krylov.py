@@ -105,6 +105,31 @@ class DeviceKrylov:
     def copy_column(self, src: int, dst: int):
         check(_lib.load().hymor_krylov_copy_column(self._h, src, dst))
 
+    def set_column_dev(self, j: int, d_src: C.c_void_p):
+        """Column j <- n values at device address d_src."""
+        check(_lib.load().hymor_memcpy_d2d(self.column(j), d_src,
+                                           self.n * np.dtype(self.dtype).itemsize))
+
+
+    def set_column_random(self, j: int, seed: int):
+        """Column j <- uniform [-1, 1) entries (real and imaginary parts),
+        generated on the device from (seed, index)."""
+        ncomp = 2 if self.cplx else 1
+        check(_lib.load().hymor_dev_fill_uniform(self.column(j), self.n * ncomp, int(seed)))
+
+
+def _ritz_vectors(K: "DeviceKrylov", Q: np.ndarray, complex_: bool) -> np.ndarray:
+    """Unit Ritz vectors V Q (n x k, complex128).  V is orthonormal, so the
+    columns are normalised in the small space; a real basis takes the real
+    and imaginary parts of Q separately (the imaginary part only if any)."""
+    Q = Q / np.linalg.norm(Q, axis=0)
+    if complex_:
+        return K.ritz(Q)
+    Yt = K.ritz(np.ascontiguousarray(Q.real)).T.astype(np.complex128)
+    if np.any(Q.imag):
+        Yt.imag = K.ritz(np.ascontiguousarray(Q.imag)).T
+    return Yt.T
+
 
 def _select_lm(evals: np.ndarray, k: int) -> float:
     """Magnitude threshold so that the k largest-|lambda| values are kept."""
@@ -132,9 +157,9 @@ def krylov_schur(apply_dev, n: int, k: int, m: int, *, complex_: bool = False, t
     scratch = DeviceVector(n, dtype)
     try:
         if v0 is None:
-            rng = np.random.default_rng(0)
-            v0 = rng.standard_normal(n) + (1j * rng.standard_normal(n) if complex_ else 0)
-        K.set_column(0, np.asarray(v0, dtype=dtype))
+            K.set_column_random(0, 0)
+        else:
+            K.set_column(0, np.asarray(v0, dtype=dtype))
         K.orthonormalize(0, K.column(0))                    # normalise in place
         H = np.zeros((m + 1, m), dtype=dtype)
         p = 0
@@ -167,13 +192,7 @@ def krylov_schur(apply_dev, n: int, k: int, m: int, *, complex_: bool = False, t
                 info = {"iterations": it, "applications": n_apply, "residual_estimates": res}
                 Y = None
                 if return_vectors:
-                    Q = Z[:, :p_keep] @ Yt                     # m x k
-                    if complex_:
-                        Y = K.ritz(Q)
-                    else:                                      # real basis, complex Ritz vectors
-                        Y = K.ritz(np.ascontiguousarray(Q.real)) + 1j * K.ritz(
-                            np.ascontiguousarray(Q.imag))
-                    Y = Y / np.linalg.norm(Y, axis=0)
+                    Y = _ritz_vectors(K, Z[:, :p_keep] @ Yt, complex_)
                 return w, Y, info
             # Krylov-Schur restart: keep the p_keep leading Schur vectors
             p = min(p_keep + (k - int(conv.sum())) // 2, m - 2)
@@ -196,7 +215,8 @@ def krylov_schur(apply_dev, n: int, k: int, m: int, *, complex_: bool = False, t
 
 
 def block_krylov_schur(apply_block, n: int, k: int, m: int, s: int, *, complex_: bool = False,
-                       tol: float = 1e-10, maxiter: int = 200, V0=None, basis: int = 0):
+                       tol: float = 1e-10, maxiter: int = 200, V0=None, basis: int = 0,
+                       return_vectors: bool = True):
     """Block Krylov-Schur: ``apply_block(d_in, ld, nvec, d_out)`` applies the
     operator to nvec consecutive device columns (leading dimension ld) at
     once -- the batched SpMM action -- and every expansion step feeds it the
@@ -204,11 +224,14 @@ def block_krylov_schur(apply_block, n: int, k: int, m: int, s: int, *, complex_:
     (the same device orthonormaliser as krylov_schur), so the projected
     matrix is band Hessenberg with s sub-diagonals.
 
-    V0 (n x s, optional) seeds the first block, e.g. Ritz vectors of an
-    earlier, cheaper solve.  Returns (theta, Y, info) as krylov_schur; with
-    basis > 0, info["basis"] also holds an orthonormal basis (n x about
-    `basis` columns) of the Schur subspace of the `basis` largest-magnitude
-    Ritz values -- the seed for a refinement on a more accurate operator.
+    V0 (n x s host array or a DeviceMatrix, optional) seeds the first
+    block, e.g. a basis from an earlier, cheaper solve; a seed of fewer than
+    s columns is padded with random directions.  Returns (theta, Y, info) as
+    krylov_schur; with basis > 0, info["basis"] also holds a DeviceMatrix
+    (n x about `basis` columns, owned by the caller) with an orthonormal
+    basis of the Schur subspace of the `basis` largest-magnitude Ritz values
+    -- the seed for a refinement on a more accurate operator;
+    return_vectors=False skips the Ritz vectors (Y is None).
     """
     if not (1 <= k and k + s <= m):
         raise ValueError("need k + s <= m")
@@ -216,24 +239,26 @@ def block_krylov_schur(apply_block, n: int, k: int, m: int, s: int, *, complex_:
     K = DeviceKrylov(n, m + s, complex_)
     W = DeviceMatrix(n, s, dtype)
     try:
-        rng = np.random.default_rng(0)
-        if V0 is None:
-            V0 = rng.standard_normal((n, s)) + (1j * rng.standard_normal((n, s)) if complex_ else 0)
-        V0 = np.asarray(V0, dtype=dtype)
-        if V0.shape[1] < s:    # pad a short seed block with random directions
-            extra = rng.standard_normal((n, s - V0.shape[1]))
-            V0 = np.concatenate([V0, extra.astype(dtype)], axis=1)
+        colbytes = n * np.dtype(dtype).itemsize
+        on_device = V0 is not None and hasattr(V0, "nrhs")   # a DeviceMatrix
+        n_seed = 0 if V0 is None else (V0.nrhs if on_device else np.shape(V0)[1])
+        if not on_device and V0 is not None:
+            V0 = np.asarray(V0, dtype=dtype)
         H = np.zeros((m + s, m + s), dtype=dtype)
         # orthonormal first block (dependent seed columns are replaced)
         for i in range(s):
-            K.set_column(i, V0[:, i])
+            if i >= n_seed:                   # random directions (pad a short seed)
+                K.set_column_random(i, i)
+            elif on_device:
+                K.set_column_dev(i, C.c_void_p(V0.ptr.value + i * colbytes))
+            else:
+                K.set_column(i, V0[:, i])
             h = K.orthonormalize(i, K.column(i))
             if abs(h[i]) == 0:
-                K.set_column(i, rng.standard_normal(n).astype(dtype))
+                K.set_column_random(i, s + i)
                 K.orthonormalize(i, K.column(i))
         j = 0              # first column whose image is not yet in the basis
         n_apply = 0
-        colbytes = n * np.dtype(dtype).itemsize
         for it in range(1, maxiter + 1):
             while j + s <= m:
                 apply_block(K.column(j), n, s, W.ptr)
@@ -268,12 +293,7 @@ def block_krylov_schur(apply_block, n: int, k: int, m: int, s: int, *, complex_:
             res = np.linalg.norm(Bres @ (Z[:, :p_keep] @ Yt), axis=0)
             conv = res <= tol * np.maximum(np.abs(w), np.finfo(float).tiny)
             if np.all(conv):
-                Q = Z[:, :p_keep] @ Yt
-                if complex_:
-                    Y = K.ritz(Q)
-                else:
-                    Y = K.ritz(np.ascontiguousarray(Q.real)) + 1j * K.ritz(np.ascontiguousarray(Q.imag))
-                Y = Y / np.linalg.norm(Y, axis=0)
+                Y = _ritz_vectors(K, Z[:, :p_keep] @ Yt, complex_) if return_vectors else None
                 info = {"iterations": it, "applications": n_apply, "residual_estimates": res}
                 if basis:
                     nb = min(basis, mm)
@@ -283,7 +303,11 @@ def block_krylov_schur(apply_block, n: int, k: int, m: int, s: int, *, complex_:
                     else:
                         _, Zb, sd = sl.schur(Hm, output="real",
                                              sort=lambda re, im: abs(complex(re, im)) >= thr_b)
-                    info["basis"] = K.ritz(np.ascontiguousarray(Zb[:, :min(max(sd, nb), mm)]))
+                    nb = min(max(sd, nb), mm)
+                    K.combine(Zb[:, :nb])          # V[:, :nb] <- the Schur basis
+                    Bd = DeviceMatrix(n, nb, dtype)
+                    check(_lib.load().hymor_memcpy_d2d(Bd.ptr, K.column(0), nb * colbytes))
+                    info["basis"] = Bd
                 return w, Y, info
             # restart: keep p Schur vectors, then the residual block
             p = max(p_keep, min(p_keep + (k - int(conv.sum())), mm - s))

@@ -174,6 +174,11 @@ int hymor_dev_alloc(int64_t bytes, void **d_ptr);
 int hymor_dev_free(void *d_ptr);
 int hymor_memcpy_h2d(void *d_dst, const void *h_src, int64_t bytes);
 int hymor_memcpy_d2h(void *h_dst, const void *d_src, int64_t bytes);
+int hymor_memcpy_d2d(void *d_dst, const void *d_src, int64_t bytes);
+/* len doubles at d_x set to pseudo-random values, uniform in [-1, 1), a
+ * function of (seed, index) only (splitmix64): Krylov start blocks made on
+ * the device.  Synchronous. */
+int hymor_dev_fill_uniform(void *d_x, int64_t len, uint64_t seed);
 
 /* Timing probe: launches the fused Taylor-term kernel `reps` times on the
  * handle's stream (a middle term: gathers x, writes the new term, updates

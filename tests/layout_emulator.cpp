@@ -495,3 +495,49 @@ extern "C" int emu_vals_to_f32_csc(int64_t n, const int64_t *colptr, const int64
     }
     return bad + g_oob;
 }
+
+// The upload path's layout: bsr_to_sliced_map (+ bsr_transpose_map for the
+// adjoint) and k_slice_vals' index map replayed on the CPU, against the host
+// layout bsr_to_sliced + interleave_lanes (of bsr_transpose for the
+// adjoint).  Returns the number of differing structure entries or values.
+extern "C" int emu_slice_vals_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                                  const double *nzval, int b, int64_t ncell, int vpl,
+                                  int transpose) {
+    g_oob = 0;
+    HostBSR fwd, adj;
+    HostExtra xf, xa;
+    csc_to_blocks(n, colptr, rowval, nzval, 1, false, 1.0, b, ncell, fwd, adj, xf, xa);
+    HostBSR ref_bsr;
+    if (transpose) bsr_transpose(fwd, ref_bsr);
+    else ref_bsr = fwd;
+    HostSliced ref;
+    bsr_to_sliced(ref_bsr, ref);
+    interleave_lanes(ref, vpl);
+    HostSliced s;
+    std::vector<int64_t> pad_src;
+    if (transpose) {
+        std::vector<int64_t> tptr, tsrc;
+        std::vector<int32_t> tcol;
+        bsr_transpose_map(ncell, fwd.ptr, fwd.col, tptr, tcol, tsrc);
+        bsr_to_sliced_map(b, ncell, tptr, tcol, &tsrc, s, pad_src);
+    } else {
+        bsr_to_sliced_map(b, ncell, fwd.ptr, fwd.col, nullptr, s, pad_src);
+    }
+    int bad = 0;
+    if (s.slice_ptr != ref.slice_ptr || s.col != ref.col || s.nslice != ref.nslice) ++bad;
+    const int bb = b * b, cps = 64 / b, slot_len = 64 * b;
+    const int64_t total = s.slice_ptr[s.nslice] * bb;
+    if ((int64_t)ref.val.size() != total) ++bad;
+    for (int64_t i = 0; i < total; ++i) {
+        const int64_t slot = i / slot_len;
+        const int j = (int)(i - slot * slot_len);
+        const int kq = j / (64 * vpl), rem = j - kq * 64 * vpl;
+        const int lane = rem / vpl, k = kq * vpl + rem % vpl;
+        const int q = lane / b, r = lane - q * b;
+        const int64_t sb = at(pad_src, slot * cps + q);
+        double v = 0.0;
+        if (sb >= 0) v = at(fwd.val, sb * bb + (transpose ? k * b + r : r * b + k));
+        if (v != at(ref.val, i)) ++bad;
+    }
+    return bad + g_oob;
+}

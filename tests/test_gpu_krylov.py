@@ -78,3 +78,25 @@ def test_batched_eigs_exact_spectrum():
     assert 1e-12 < err(D) <= 1e-6          # FP32 storage alone: about its rounding
     _, D = st.EIGENVALUES_GPU_ARNOLDI(op, sd(), k, krylov="device", tol=1e-11, ncv=25)
     assert err(D) <= 1e-10
+
+
+def test_device_uniform_fill():
+    """Start blocks made on the device: values in [-1, 1), a function of
+    (seed, index) only, mean and variance of the uniform distribution."""
+    import ctypes as C
+    from hymor_amd import _lib
+    from hymor_amd.core import DeviceMatrix
+    n = 1 << 20
+    d = DeviceMatrix(n, 3)
+    try:
+        lib = _lib.load()
+        for j, seed in enumerate((7, 7, 8)):
+            _lib.check(lib.hymor_dev_fill_uniform(C.c_void_p(d.ptr.value + j * n * 8), n, seed))
+        X = d.download()
+    finally:
+        d.free()
+    assert np.all(X >= -1.0) and np.all(X < 1.0)
+    np.testing.assert_array_equal(X[:, 0], X[:, 1])
+    assert np.mean(X[:, 0] == X[:, 2]) < 1e-5
+    assert abs(X[:, 0].mean()) < 5e-3 and abs(X[:, 0].var() - 1 / 3) < 5e-3
+    assert abs(np.corrcoef(X[:, 0], X[:, 2])[0, 1]) < 5e-3

@@ -172,3 +172,21 @@ def test_emulated_value_cast_map(emu, b, seed):
     colptr, rowval, nzval = op.to_csc_varmajor(one_based=True)
     assert emu.emu_vals_to_f32_csc(op.n, colptr.ctypes.data, rowval.ctypes.data,
                                    nzval.ctypes.data, b, op.ncell) == 0
+
+
+@pytest.mark.parametrize("b", [4, 8])
+@pytest.mark.parametrize("vpl", [2, 4])
+@pytest.mark.parametrize("transpose", [0, 1])
+def test_emulated_device_value_layout(emu, b, vpl, transpose):
+    """The operator upload places the block values on the device
+    (k_slice_vals over the host's slice map, transposed for the adjoint):
+    same structure and values as the host layout it replaces."""
+    emu.emu_slice_vals_csc.restype = C.c_int
+    emu.emu_slice_vals_csc.argtypes = [C.c_int64] + [C.c_void_p] * 3 + [
+        C.c_int, C.c_int64, C.c_int, C.c_int]
+    from test_gpu_parity import random_pattern_operator
+    for op in (ops.stencil_operator(11, 9, b, seed=b, n_shock=11),
+               random_pattern_operator(9, 7, b, 19, seed=vpl)):
+        colptr, rowval, nzval = op.to_csc_varmajor(one_based=True)
+        assert emu.emu_slice_vals_csc(op.n, colptr.ctypes.data, rowval.ctypes.data,
+                                      nzval.ctypes.data, b, op.ncell, vpl, transpose) == 0
