@@ -49,8 +49,10 @@ struct ArgError : std::invalid_argument {
 struct TermTuning {
     int split = 3;   // 0: k_term (one wave per slice), 1: k_term_split,
                      // 2: k_term_split_p (grouped loads), 3: k_term_lds
-                     // (grouped loads, x staged through LDS; width 1 runs 2)
-    int waves = 3;   // k_term: waves per workgroup; the others: waves per slice
+                     // (grouped loads, x staged through LDS; widths 2 and 4 only,
+                     // the others run 2)
+    int waves = 1;   // k_term: waves per workgroup; the others: waves per slice
+                     // (k_term_split_p: 3 unless 9)
     int nt = 1;      // non-temporal loads of the block values
     int spmm_mfma = 6;  // batches of >= 16 doubles per row: 0 vector-FMA k_spmm,
                         // 1 k_spmm_mfma, 2/3 k_spmm_mfma_mc (2 or 1 cells per wave),
@@ -442,10 +444,14 @@ template <int B, typename VT, int NV, bool NT>
 void launch_term_nt(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
                     double *tout, const double *vin, double *vout, double c1, double c2) {
     const TermTuning &tu = s.tune;
-    if constexpr (NV >= 2) {
+    if constexpr (NV >= 2 && NV <= 4) {
         if (tu.split == 3) {
-            if (tu.waves == 9) launch_lds<B, VT, NV, 9, NT>(s, st, x, yx, tout, vin, vout, c1, c2);
-            else launch_lds<B, VT, NV, 3, NT>(s, st, x, yx, tout, vin, vout, c1, c2);
+            switch (tu.waves) {
+            case 1: launch_lds<B, VT, NV, 1, NT>(s, st, x, yx, tout, vin, vout, c1, c2); break;
+            case 2: launch_lds<B, VT, NV, 2, NT>(s, st, x, yx, tout, vin, vout, c1, c2); break;
+            case 9: launch_lds<B, VT, NV, 9, NT>(s, st, x, yx, tout, vin, vout, c1, c2); break;
+            default: launch_lds<B, VT, NV, 3, NT>(s, st, x, yx, tout, vin, vout, c1, c2); break;
+            }
             return;
         }
     }
@@ -1264,8 +1270,9 @@ int hymor_op_autotune(hymor_op *op, int nv, int reps, double *best_ms) {
         const TermTuning cands[] = {{0, 4, 0, 1}, {0, 4, 1, 1}, {0, 2, 0, 1}, {1, 2, 0, 1},
                                     {1, 2, 1, 1}, {1, 4, 0, 1}, {1, 4, 1, 1}, {2, 3, 0, 1},
                                     {2, 3, 1, 1}, {2, 9, 0, 1}, {2, 9, 1, 1}, {3, 3, 0, 1},
-                                    {3, 3, 1, 1}, {3, 9, 0, 1}, {3, 9, 1, 1}};
-        const int ncand = nv >= 2 ? 15 : 11;   // k_term_lds needs two doubles per row
+                                    {3, 3, 1, 1}, {3, 9, 1, 1}, {3, 1, 0, 1}, {3, 1, 1, 1},
+                                    {3, 2, 1, 1}};
+        const int ncand = nv >= 2 ? 17 : 11;   // k_term_lds needs two doubles per row
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
@@ -1374,9 +1381,10 @@ int hymor_op_set_tuning(hymor_op *op, int split, int waves, int nt) {
         ARG_CHECK(op, "null operator");
         ARG_CHECK((split == 0 && (waves == 1 || waves == 2 || waves == 4)) ||
                       (split == 1 && (waves == 2 || waves == 4)) ||
-                      ((split == 2 || split == 3) && (waves == 3 || waves == 9)),
+                      (split == 2 && (waves == 3 || waves == 9)) ||
+                      (split == 3 && (waves == 1 || waves == 2 || waves == 3 || waves == 9)),
                   "k_term takes 1, 2 or 4 waves per group, k_term_split 2 or 4 waves per "
-                  "slice, k_term_split_p and k_term_lds 3 or 9");
+                  "slice, k_term_split_p 3 or 9, k_term_lds 1, 2, 3 or 9");
         ARG_CHECK(nt == 0 || nt == 1, "nt must be 0 or 1");
         op->fwd.tune.split = split;
         op->fwd.tune.waves = waves;

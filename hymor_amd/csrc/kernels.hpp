@@ -855,22 +855,23 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
         for (int u = 0; u < S; ++u) {
             const bool live = g + u < nmine;
             const d2 *xb = mine + u * CPS * XS + q * XS;
-            // the whole block first: all reads in flight before the FMAs
-            d2 xx[XC];
 #pragma unroll
-            for (int i = 0; i < XC; ++i) xx[i] = xb[i];
+            for (int kc = 0; kc < NC; ++kc) {
+                // the rows of this value chunk first: their reads in flight
+                // together before the FMAs
+                d2 xx[V * (NV / 2)];
 #pragma unroll
-            for (int kc = 0; kc < NC; ++kc)
+                for (int i = 0; i < V * (NV / 2); ++i) xx[i] = xb[kc * V * (NV / 2) + i];
 #pragma unroll
                 for (int e = 0; e < V; ++e) {
-                    const int k = kc * V + e;
                     const double ak = live ? (double)raw[u][kc][e] : 0.0;
 #pragma unroll
                     for (int vp2 = 0; vp2 < NV / 2; ++vp2) {
-                        acc[2 * vp2] = fma(ak, xx[k * (NV / 2) + vp2].x, acc[2 * vp2]);
-                        acc[2 * vp2 + 1] = fma(ak, xx[k * (NV / 2) + vp2].y, acc[2 * vp2 + 1]);
+                        acc[2 * vp2] = fma(ak, xx[e * (NV / 2) + vp2].x, acc[2 * vp2]);
+                        acc[2 * vp2 + 1] = fma(ak, xx[e * (NV / 2) + vp2].y, acc[2 * vp2 + 1]);
                     }
                 }
+            }
         }
     }
     if constexpr (NW > 1) {

@@ -138,7 +138,7 @@ def _select_lm(evals: np.ndarray, k: int) -> float:
 
 
 def krylov_schur(apply_dev, n: int, k: int, m: int, *, complex_: bool = False, tol: float = 1e-10,
-                 maxiter: int = 300, v0=None, return_vectors: bool = True):
+                 maxiter: int = 300, v0=None, return_vectors: bool = True, seed: int = 0):
     """Leading-magnitude eigenpairs of a linear operator given as
     ``apply_dev(d_in, d_out)`` (device pointers, synchronous).
 
@@ -156,8 +156,8 @@ def krylov_schur(apply_dev, n: int, k: int, m: int, *, complex_: bool = False, t
     K = DeviceKrylov(n, m, complex_)
     scratch = DeviceVector(n, dtype)
     try:
-        if v0 is None:
-            K.set_column_random(0, 0)
+        if v0 is None:                                     # random start on the device
+            K.set_column_random(0, seed)
         else:
             K.set_column(0, np.asarray(v0, dtype=dtype))
         K.orthonormalize(0, K.column(0))                    # normalise in place
@@ -216,7 +216,7 @@ def krylov_schur(apply_dev, n: int, k: int, m: int, *, complex_: bool = False, t
 
 def block_krylov_schur(apply_block, n: int, k: int, m: int, s: int, *, complex_: bool = False,
                        tol: float = 1e-10, maxiter: int = 200, V0=None, basis: int = 0,
-                       return_vectors: bool = True):
+                       return_vectors: bool = True, seed: int = 0):
     """Block Krylov-Schur: ``apply_block(d_in, ld, nvec, d_out)`` applies the
     operator to nvec consecutive device columns (leading dimension ld) at
     once -- the batched SpMM action -- and every expansion step feeds it the
@@ -231,7 +231,8 @@ def block_krylov_schur(apply_block, n: int, k: int, m: int, s: int, *, complex_:
     (n x about `basis` columns, owned by the caller) with an orthonormal
     basis of the Schur subspace of the `basis` largest-magnitude Ritz values
     -- the seed for a refinement on a more accurate operator;
-    return_vectors=False skips the Ritz vectors (Y is None).
+    return_vectors=False skips the Ritz vectors (Y is None); `seed` selects
+    the random start directions.
     """
     if not (1 <= k and k + s <= m):
         raise ValueError("need k + s <= m")
@@ -248,14 +249,14 @@ def block_krylov_schur(apply_block, n: int, k: int, m: int, s: int, *, complex_:
         # orthonormal first block (dependent seed columns are replaced)
         for i in range(s):
             if i >= n_seed:                   # random directions (pad a short seed)
-                K.set_column_random(i, i)
+                K.set_column_random(i, 2 * s * seed + i)
             elif on_device:
                 K.set_column_dev(i, C.c_void_p(V0.ptr.value + i * colbytes))
             else:
                 K.set_column(i, V0[:, i])
             h = K.orthonormalize(i, K.column(i))
             if abs(h[i]) == 0:
-                K.set_column_random(i, s + i)
+                K.set_column_random(i, 2 * s * seed + s + i)
                 K.orthonormalize(i, K.column(i))
         j = 0              # first column whose image is not yet in the basis
         n_apply = 0

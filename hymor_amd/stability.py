@@ -359,7 +359,7 @@ def _device_op(A, s, scale, value_prec):
 
 
 def EIGENVALUES_GPU_ARNOLDI(A, s, n_modes, *, T_ref=0.5, tol=1e-4, maxiter=None, ncv=None,
-                            v0=None, krylov="arpack"):
+                            v0=None, krylov="arpack", seed=0):
     """Arnoldi on the exponential transform (EIGENVALUES.jl:112-205).
 
     dt from CFL_TIMESTEP, timesteps = round(T_ref/dt), A*dt uploaded once,
@@ -369,7 +369,8 @@ def EIGENVALUES_GPU_ARNOLDI(A, s, n_modes, *, T_ref=0.5, tol=1e-4, maxiter=None,
 
     krylov="device" replaces ARPACK by the Krylov-Schur solver over a
     device-resident basis (hymor_amd.krylov): same selection (largest
-    magnitude) and convergence test, no host round trip per matvec.
+    magnitude) and convergence test, no host round trip per matvec; with no
+    v0 its start vector is random on the device (`seed`).
     """
     s = CFL_TIMESTEP(s)
     dt = s["time_integration"]["dt"]
@@ -386,7 +387,7 @@ def EIGENVALUES_GPU_ARNOLDI(A, s, n_modes, *, T_ref=0.5, tol=1e-4, maxiter=None,
                 expm_taylor_ptr(dev, din, dout, order=ORDER, nsub=timesteps, complex_=False)
                 dev.synchronize()
             vals, vecs, kinfo = krylov_schur(apply, n, k, ncv or max(20, 2 * k + 1), tol=tol,
-                                             maxiter=maxiter or 40 * k, v0=v0)
+                                             maxiter=maxiter or 40 * k, v0=v0, seed=seed)
             s.setdefault("timings", {})["eigenvalues_krylov"] = kinfo
         else:
             op = spla.LinearOperator((n, n), dtype=np.float64,
@@ -664,7 +665,7 @@ def _device_generalized(devA, devC, D, s, n, k, n_t, n_set0, cplx, tol, maxiter,
 
 def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=4, ncv=None, tol=1e-10,
                      tol_f32=1e-7, maxiter=200, order=ORDER, value_prec="f32", refine=True,
-                     refine_block=16, refine_ncv=32, autotune=False, timings=None):
+                     refine_block=16, refine_ncv=32, autotune=False, seed=0, timings=None):
     """Leading eigenvalues of the exponential transform with batched actions.
 
     Phase 1: block Krylov-Schur whose every expansion applies T_order(A dt)^n
@@ -681,7 +682,8 @@ def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=4, ncv=None, tol=1e-10,
     as EIGENVALUES_GPU_ARNOLDI does: lambda = log(mu) / (n dt).
     A is a scipy matrix in the reference's var-major order or a
     BlockOperator.  autotune: time the batch kernels at each block width
-    first (else the default tile kernel).  Returns (V, D, info); info holds
+    first (else the default tile kernel); `seed` selects the random start
+    block.  Returns (V, D, info); info holds
     per-phase iterations, operator applications and seconds.
     """
     s = CFL_TIMESTEP(s)
@@ -716,7 +718,7 @@ def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=4, ncv=None, tol=1e-10,
         vals, Y, i1 = block_krylov_schur(make_apply(dev), n, k, m, block,
                                          tol=tol_f32 if two_phase else tol, maxiter=maxiter,
                                          basis=refine_block if two_phase else 0,
-                                         return_vectors=not two_phase)
+                                         return_vectors=not two_phase, seed=seed)
         seed = i1.pop("basis", None)
         info["phase1"] = dict(i1, seconds=time.perf_counter() - t0, value_prec=value_prec,
                               block=block)
