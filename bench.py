@@ -155,6 +155,10 @@ def eigen_leg(k=10, tol=1e-9, block=4, seeds=(0, 1, 2)):
 
 
 COPY_MIB = 512
+# the library's default launches: k_term_lds (k_term_split_p with 3 waves per
+# slice at one vector) and the tile-order matrix-core batch kernel
+PLAIN_TUNING = (3, 1, 1)
+BATCH_KERNEL = 6
 
 
 def pmc_passes(config, timeout_s=150):
@@ -214,6 +218,7 @@ def traffic_from_passes(passes, kernel):
         if not c_kib or k_kib is None:
             return None, None
         factors[counter] = copy_bytes / (c_kib * 1024.0)
+        factors[counter + "_KiB_per_dispatch"] = k_kib
         total += k_kib * 1024.0 * factors[counter]
     return total, factors
 
@@ -240,11 +245,13 @@ def main():
     ap.add_argument("--nrhs", type=int, default=16, help="vectors per batched sweep")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-eigen", action="store_true", help="skip the eigenvalue leg")
-    ap.add_argument("--kernel-only", choices=["plain", "batched"],
-                    help="time only the Taylor-term kernel of one form (for profiler runs)")
+    ap.add_argument("--kernel-only", choices=["plain", "batched", "block4"],
+                    help="time only the Taylor-term kernel of one form (for profiler runs): "
+                         "plain FP64 one vector, batched FP32 --nrhs vectors, block4 FP32 "
+                         "four vectors (the eigen leg's FP32 phase)")
     ap.add_argument("--batch-kernel", type=int, default=None,
                     help="--kernel-only batched: use this batch kernel (0..7) instead of "
-                         "autotuning (profiler passes on one kernel)")
+                         "the default tile kernel")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure HBM traffic")
     args = ap.parse_args()
@@ -266,14 +273,14 @@ def main():
     log(f"generating {args.config}: {cfg}")
     op = ops.config_operator(args.config)
     if args.kernel_only:
-        prec, nv = ("f64", 1) if args.kernel_only == "plain" else ("f32", args.nrhs)
+        prec, nv = {"plain": ("f64", 1), "batched": ("f32", args.nrhs),
+                    "block4": ("f32", 4)}[args.kernel_only]
         devk = DeviceOperator.from_blocks(op, value_prec=prec, with_adjoint=False)
-        if nv == 1:
-            devk.autotune(reps=5)
-        elif args.batch_kernel is not None:
-            devk.set_batch_kernel(args.batch_kernel)
-        elif nv in (16, 32, 64):
-            devk.autotune_batch(nv=nv, reps=3)
+        if nv < 8:
+            devk.set_tuning(*PLAIN_TUNING)
+        else:
+            devk.set_batch_kernel(args.batch_kernel if args.batch_kernel is not None
+                                  else BATCH_KERNEL)
         ms, by = devk.bench_term_kernel(reps=args.kernel_reps, nv=nv)
         from hymor_amd.calibrate import copy_bandwidth
         cp = copy_bandwidth(COPY_MIB, 3)
@@ -283,9 +290,14 @@ def main():
         return
     log(f"n={op.n} nnzb={op.nnzb}; uploading FP64 blocks")
     dev = DeviceOperator.from_blocks(op, value_prec="f64", with_adjoint=False)
-    tune_ms, tuning = dev.autotune(reps=10)
-    log(f"autotuned Taylor-term launch: split={tuning[0]} waves={tuning[1]} nt={tuning[2]} "
+    # the autotuner's pick is reported; the timed legs (and the profiler and
+    # counter runs) use the library's default launch, so every process runs
+    # the same kernel (the variants are within about 2 % of each other)
+    tune_ms, tune_pick = dev.autotune(reps=10)
+    log(f"autotuner's pick: split={tune_pick[0]} waves={tune_pick[1]} nt={tune_pick[2]} "
         f"({tune_ms:.4f} ms)")
+    dev.set_tuning(*PLAIN_TUNING)
+    tuning = dev.tuning
     x = np.random.default_rng(cfg["seed"] - 1000).standard_normal(op.n)
     dx = DeviceVector.from_host(x)
     dy = DeviceVector(op.n)
@@ -332,9 +344,11 @@ def main():
         nb = args.nrhs
         log(f"batched leg: FP32-stored values, {nb} vectors per sweep")
         devb = DeviceOperator.from_blocks(op, value_prec="f32", with_adjoint=False)
-        bkern = None
+        bkern_pick = None
         if nb in (16, 32, 64):
-            bkern, _ = devb.autotune_batch(nv=nb, reps=5)
+            bkern_pick, _ = devb.autotune_batch(nv=nb, reps=5)
+        bkern = args.batch_kernel if args.batch_kernel is not None else BATCH_KERNEL
+        devb.set_batch_kernel(bkern)
         X = np.random.default_rng(7).standard_normal((op.n, nb))
         X[:, 0] = x
         dX = DeviceMatrix.from_host(X)
@@ -380,6 +394,8 @@ def main():
                           f"{ {8: 9, 16: 9, 32: 5, 64: 3}.get(nb)}>",
                        7: f"k_spmm_quad<float, {nb}, { {8: 9, 16: 9, 32: 5, 64: 3}.get(nb)}>"
                        }.get(bkern, "k_spmm"),
+            "batch_kernel_id": bkern,
+            "autotuner_pick": bkern_pick,
             "kernel_ms": bk_ms,
             "algorithmic_bytes_per_launch": bk_bytes,
             "achieved_GBps": b_gbs,
@@ -422,6 +438,8 @@ def main():
             "traffic_counter_scale": factors,
             "kernel": kname,
             "launch": {"split": tuning[0], "waves": tuning[1], "nontemporal": tuning[2]},
+            "autotuner_pick": {"split": tune_pick[0], "waves": tune_pick[1],
+                               "nontemporal": tune_pick[2], "ms": tune_ms},
             "kernel_ms": k_ms,
             "algorithmic_bytes_per_launch": k_bytes,
         },
