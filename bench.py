@@ -322,6 +322,9 @@ def main():
     log(f"timing the Taylor-term kernel alone ({args.kernel_reps} launches, HIP events)")
     k_ms, k_bytes = dev.bench_term_kernel(reps=args.kernel_reps)
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
+    from hymor_amd.calibrate import read_bandwidth
+    read_ceiling = read_bandwidth(max(1, int(k_bytes) >> 20), 20)
+    log(f"streaming read of the same byte count: {read_ceiling['GBps']:.0f} GB/s")
     y_plain = dy.download()
     dev.close()
     nt_s = str(bool(tuning[2])).lower()
@@ -442,6 +445,8 @@ def main():
                                "nontemporal": tune_pick[2], "ms": tune_ms},
             "kernel_ms": k_ms,
             "algorithmic_bytes_per_launch": k_bytes,
+            "read_stream_GBps": read_ceiling["GBps"],
+            "frac_of_read_stream": achieved / read_ceiling["GBps"],
         },
         "batched": batched,
     }

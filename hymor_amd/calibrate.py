@@ -1,9 +1,10 @@
-"""Streaming-copy calibration: achievable HBM bandwidth on this GPU, and the
-reference kernel for scaling the FETCH_SIZE / WRITE_SIZE profiler counters.
+"""Streaming calibration: achievable HBM bandwidth on this GPU (a copy, and a
+read-only stream), and the reference kernel for scaling the FETCH_SIZE /
+WRITE_SIZE profiler counters.
 
 ``python hymor_amd/calibrate.py [MiB] [reps]`` (or ``python -m
 hymor_amd.calibrate``) prints one JSON line: a copy of S bytes reads S and
-writes S, so GB/s = 2*S / time.
+writes S, so its GB/s = 2*S / time; the read stream's GB/s = S / time.
 """
 from __future__ import annotations
 
@@ -27,7 +28,17 @@ def copy_bandwidth(mib: int = 1024, reps: int = 20):
             "GBps": 2.0 * nbytes / (ms.value * 1e-3) / 1e9, "reps": reps}
 
 
+def read_bandwidth(mib: int = 1024, reps: int = 20):
+    """Streaming read of `mib` MiB: GB/s = S / time."""
+    nbytes = mib * 1024 * 1024
+    ms = C.c_double()
+    _lib.check(_lib.load().hymor_bench_read(nbytes, reps, C.byref(ms)))
+    return {"kernel": "k_read", "bytes": nbytes, "ms_per_launch": ms.value,
+            "GBps": nbytes / (ms.value * 1e-3) / 1e9, "reps": reps}
+
+
 if __name__ == "__main__":
     mib = int(sys.argv[1]) if len(sys.argv) > 1 else 1024
     reps = int(sys.argv[2]) if len(sys.argv) > 2 else 20
-    print(json.dumps(copy_bandwidth(mib, reps)), flush=True)
+    print(json.dumps({"copy": copy_bandwidth(mib, reps), "read": read_bandwidth(mib, reps)}),
+          flush=True)

@@ -1498,6 +1498,43 @@ int hymor_bench_copy(int64_t bytes, int reps, double *ms_per_launch) {
     });
 }
 
+int hymor_bench_read(int64_t bytes, int reps, double *ms_per_launch) {
+    return guarded([&] {
+        ARG_CHECK(ms_per_launch && reps >= 1 && bytes >= 16 && bytes % 16 == 0,
+                  "bytes must be a positive multiple of 16, reps >= 1");
+        std::vector<void *> allocs;
+        struct Free {
+            std::vector<void *> &a;
+            ~Free() {
+                for (void *p : a) (void)hipFree(p);
+            }
+        } guard{allocs};
+        const int64_t n2 = bytes / 16;
+        const int64_t grid = std::min<int64_t>((n2 + 255) / 256, 4096);
+        d2 *in = (d2 *)dalloc_raw(allocs, (size_t)bytes);
+        double *out = (double *)dalloc_raw(allocs, (size_t)grid * 256 * sizeof(double));
+        HIP_TRY(hipMemset(in, 0, (size_t)bytes));
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        const int64_t chunk = (n2 + grid - 1) / grid;
+        hipLaunchKernelGGL(k_read, dim3((unsigned)grid), dim3(256), 0, nullptr, in, out, n2,
+                           chunk);
+        HIP_TRY(hipEventRecord(e0, nullptr));
+        for (int i = 0; i < reps; ++i)
+            hipLaunchKernelGGL(k_read, dim3((unsigned)grid), dim3(256), 0, nullptr, in, out, n2,
+                               chunk);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(e1, nullptr));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms_per_launch = (double)ms / reps;
+    });
+}
+
 int hymor_weights_create_csc(int64_t n, int nmat, const int64_t *const *colptr,
                              const int64_t *const *rowval, const double *const *nzval,
                              int index_base, int vars_per_cell, int64_t nchi, int64_t neta,

@@ -375,6 +375,36 @@ __global__ __launch_bounds__(256) void k_copy(const d2 *__restrict__ in, d2 *__r
     for (; i < n2; i += stride) out[i] = in[i];
 }
 
+// Streaming read of n2 16-byte chunks, one partial sum stored per thread:
+// the read-only ceiling the value stream of the Taylor-term kernels is
+// measured against.  Workgroup g reads the contiguous run [g*chunk,
+// (g+1)*chunk), eight non-temporal loads in flight per thread (on MI355X
+// this beats a grid-strided sweep by about 10 %).
+__global__ __launch_bounds__(256) void k_read(const d2 *__restrict__ in, double *__restrict__ out,
+                                              int64_t n2, int64_t chunk) {
+    constexpr int U = 8;
+    const int64_t b0 = (int64_t)blockIdx.x * chunk;
+    const int64_t e = b0 + chunk < n2 ? b0 + chunk : n2;
+    double acc0 = 0.0, acc1 = 0.0;
+    int64_t i = b0 + threadIdx.x;
+    for (; i + (U - 1) * 256 < e; i += U * 256) {
+        d2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(in + i + u * 256);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            acc0 += v[u].x;
+            acc1 += v[u].y;
+        }
+    }
+    for (; i < e; i += 256) {
+        const d2 a = __builtin_nontemporal_load(in + i);
+        acc0 += a.x;
+        acc1 += a.y;
+    }
+    out[(int64_t)blockIdx.x * 256 + threadIdx.x] = acc0 + acc1;
+}
+
 }  // namespace hymor
 
 namespace hymor {

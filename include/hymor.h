@@ -230,6 +230,11 @@ int hymor_op_set_tuning(hymor_op *op, int split, int waves, int nt);
  * scale the FETCH_SIZE / WRITE_SIZE profiler counters. */
 int hymor_bench_copy(int64_t bytes, int reps, double *ms_per_launch);
 
+/* Timing probe, no reference counterpart: a streaming read of `bytes`
+ * (non-temporal 16-byte loads, one partial sum per thread), the read-only
+ * bandwidth ceiling for the value stream of the term kernels. */
+int hymor_bench_read(int64_t bytes, int reps, double *ms_per_launch);
+
 /* ------------------------------------------------------------------------
  * Energy-tracked integration (Freestream_receptivity/
  * LINEAR_INTEGRATION_AND_GAINS.jl:375-477, LINEAR_INTEGRATION_AND_ENERGY_GPU

@@ -109,3 +109,12 @@ def test_batch_kernels_wide_rows(orc, mfma, b, nrhs):
     for j in (0, nrhs - 1):
         assert relerr(Y[:, j], orc.power_method(A.scaled(0.1), X[:, j], 3)) <= 1e-12
         assert relerr(Yt[:, j], orc.power_method(At.scaled(0.1), X[:, j], 3)) <= 1e-12
+
+
+def test_calibration_probes():
+    """The streaming copy and read probes run and report plausible rates
+    (between 0.5 TB/s and the 8 TB/s HBM peak) at 256 MiB."""
+    from hymor_amd.calibrate import copy_bandwidth, read_bandwidth
+    c = copy_bandwidth(256, 3)
+    r = read_bandwidth(256, 3)
+    assert 500 < c["GBps"] < 8000 and 500 < r["GBps"] < 8000, (c, r)
