@@ -36,38 +36,47 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(op, order, nsub, sample_substeps=3):
-    """Time the oracle's CPU forms on a bounded sample (whole substeps of the
-    same workload) and scale to actions/s."""
+def cpu_baseline(op, order, nsub, serial_samples=3, omp_samples=5):
+    """Time the oracle's CPU forms on a bounded sample of the same workload
+    (single Taylor substeps, each timed on its own; the median is scaled to
+    actions/s, so a noisy host moves the figure less)."""
     from oracle import oracle
 
     x = np.random.default_rng(5).standard_normal(op.n)
     log("cpu baseline: building the reference-layout CSC (var-major, 1-based)")
     A = oracle.CSC.from_arrays(*op.to_csc_varmajor_blocks())
     A = A.scaled(1.0 / nsub)
-    log(f"cpu baseline: serial POWER_METHOD! restatement, {sample_substeps} substep(s)")
-    t0 = time.perf_counter()
-    oracle.power_method(A, x, sample_substeps, order=order)
-    t_serial = (time.perf_counter() - t0) / sample_substeps
+    log(f"cpu baseline: serial POWER_METHOD! restatement, {serial_samples} substeps")
+    t_serial = []
+    for _ in range(serial_samples):
+        t0 = time.perf_counter()
+        oracle.power_method(A, x, 1, order=order)
+        t_serial.append(time.perf_counter() - t0)
     del A
-    log("cpu baseline: row-parallel OpenMP CSR form")
-    csr = op.to_csr_cellmajor() * (1.0 / nsub)
+    log(f"cpu baseline: row-parallel OpenMP CSR form, {omp_samples} substeps")
+    csr = oracle.CsrArrays(op.to_csr_cellmajor() * (1.0 / nsub))
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
-    oracle.taylor_csr_omp(csr, x, 1, 1, nthreads)      # spin up the thread pool
-    t0 = time.perf_counter()
-    _, used = oracle.taylor_csr_omp(csr, x, sample_substeps, order, nthreads)
-    t_omp = (time.perf_counter() - t0) / sample_substeps
+    oracle.taylor_csr_omp(csr, x, 1, order, nthreads)      # spin up the thread pool
+    t_omp = []
+    for _ in range(omp_samples):
+        t0 = time.perf_counter()
+        _, used = oracle.taylor_csr_omp(csr, x, 1, order, nthreads)
+        t_omp.append(time.perf_counter() - t0)
+    med_omp, med_serial = float(np.median(t_omp)), float(np.median(t_serial))
     return {
-        "value": 1.0 / (t_omp * nsub),
+        "value": 1.0 / (med_omp * nsub),
         "unit": "exp(AT)v actions/s",
         "cores": int(used),
         "kind": "port",
-        "sample": (f"{sample_substeps} substep(s) of the same action ({order} SpMV + updates), "
-                   f"scaled x{nsub}; OpenMP row-parallel CSR over {used} threads"),
-        "serial_faithful_value": 1.0 / (t_serial * nsub),
+        "sample": (f"median of {omp_samples} single substeps of the same action ({order} SpMV "
+                   f"+ updates each, after one warm-up substep), scaled x{nsub}; OpenMP "
+                   f"row-parallel CSR over {used} threads"),
+        "substep_seconds_each": t_omp,
+        "serial_faithful_value": 1.0 / (med_serial * nsub),
         "serial_faithful_cores": 1,
-        "serial_faithful_sample": ("same sample, 1-based var-major CSC column-scatter SpMV, "
-                                   "restating ARNOLDI_POWER_GPU.jl:64-82 with order 12"),
+        "serial_faithful_sample": (f"median of {serial_samples} single substeps, 1-based "
+                                   "var-major CSC column-scatter SpMV, restating "
+                                   "ARNOLDI_POWER_GPU.jl:64-82 with order 12"),
         "cpu_model": _cpu_model(),
     }
 

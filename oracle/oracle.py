@@ -208,16 +208,27 @@ def linear_integration_energy(A: CSC, weights, x, n_t: int, order: int = 5):
     return E, v
 
 
+class CsrArrays:
+    """A scipy CSR matrix's arrays in the C form the OpenMP baseline reads
+    (int64 indices), converted once so that timed calls do not convert."""
+
+    def __init__(self, A_csr):
+        self.shape = A_csr.shape
+        self.indptr = np.ascontiguousarray(A_csr.indptr, dtype=np.int64)
+        self.indices = np.ascontiguousarray(A_csr.indices, dtype=np.int64)
+        self.data = np.ascontiguousarray(A_csr.data, dtype=np.float64)
+
+
 def taylor_csr_omp(A_csr, v, timesteps: int, order: int, nthreads: int = 0):
     """Row-parallel CPU form of the same action (timing baseline only).
 
-    Normalised recursion; A_csr is a scipy CSR matrix (already scaled by dt).
-    Returns (result, threads_used).
+    Normalised recursion; A_csr is a scipy CSR matrix (already scaled by dt)
+    or its CsrArrays.  Returns (result, threads_used).
     """
     lib = load()
-    rp = np.ascontiguousarray(A_csr.indptr, dtype=np.int64)
-    ci = np.ascontiguousarray(A_csr.indices, dtype=np.int64)
-    va = np.ascontiguousarray(A_csr.data, dtype=np.float64)
+    if not isinstance(A_csr, CsrArrays):
+        A_csr = CsrArrays(A_csr)
+    rp, ci, va = A_csr.indptr, A_csr.indices, A_csr.data
     v = np.array(v, dtype=np.float64)
     w0 = np.empty_like(v)
     w1 = np.empty_like(v)
