@@ -131,7 +131,7 @@ def eigen_leg(k=10, tol=1e-9, block=4, seeds=(0, 1, 2)):
         s = s_dict()
         t0 = time.perf_counter()
         _, D = st.EIGENVALUES_GPU_ARNOLDI(op, s, k, krylov="device", tol=tol, ncv=25,
-                                          maxiter=200, seed=sd_)
+                                          maxiter=200, seed=sd_, form=FORM)
         plain.append({"seconds": time.perf_counter() - t0, "err": err(D),
                       "apps": s["timings"]["eigenvalues_krylov"]["applications"]})
     out["plain_fp64"] = dict(summarize(plain), krylov_dim=25)
@@ -141,7 +141,7 @@ def eigen_leg(k=10, tol=1e-9, block=4, seeds=(0, 1, 2)):
         runs = []
         for sd_ in seeds:
             _, Db, info = st.batched_exp_eigs(op, s_dict(), k, block=block, value_prec=prec,
-                                              tol=tol, refine=refine, seed=sd_)
+                                              tol=tol, refine=refine, seed=sd_, form=FORM)
             r = {"seconds": info["seconds"], "err": err(Db),
                  "apps": info["phase1"]["applications"]}
             if "phase2" in info:
@@ -168,6 +168,10 @@ COPY_MIB = 512
 # slice at one vector) and the tile-order matrix-core batch kernel
 PLAIN_TUNING = (3, 1, 1)
 BATCH_KERNEL = 6
+# every leg evaluates POWER_METHOD!'s polynomial by Horner's rule
+# (w = v + (A dt/j) w, j = order..1): three vector passes per term instead of
+# the powers recursion's four; equal to rounding (tests/test_gpu_parity.py)
+FORM = "horner"
 
 
 def pmc_passes(config, timeout_s=150):
@@ -290,7 +294,8 @@ def main():
         else:
             devk.set_batch_kernel(args.batch_kernel if args.batch_kernel is not None
                                   else BATCH_KERNEL)
-        ms, by = devk.bench_term_kernel(reps=args.kernel_reps, nv=nv)
+        ms, by = devk.bench_term_kernel(reps=args.kernel_reps, nv=nv,
+                                        form=FORM)
         from hymor_amd.calibrate import copy_bandwidth
         cp = copy_bandwidth(COPY_MIB, 3)
         print(json.dumps({"kernel_only": args.kernel_only, "nv": nv, "ms": ms, "bytes": by,
@@ -312,7 +317,7 @@ def main():
     dy = DeviceVector(op.n)
 
     def step():
-        expm_taylor_dev(dev, dx, dy, order=order, nsub=nsub, dt=T / nsub, form="powers")
+        expm_taylor_dev(dev, dx, dy, order=order, nsub=nsub, dt=T / nsub, form=FORM)
 
     log(f"warmup {args.warmup}")
     for _ in range(args.warmup):
@@ -329,7 +334,7 @@ def main():
     log(f"{ms_per_step:.3f} ms/action")
 
     log(f"timing the Taylor-term kernel alone ({args.kernel_reps} launches, HIP events)")
-    k_ms, k_bytes = dev.bench_term_kernel(reps=args.kernel_reps)
+    k_ms, k_bytes = dev.bench_term_kernel(reps=args.kernel_reps, form=FORM)
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     from hymor_amd.calibrate import read_bandwidth
     read_ceiling = read_bandwidth(max(1, int(k_bytes) >> 20), 20)
@@ -368,7 +373,7 @@ def main():
 
         def bstep():
             expm_taylor_batched_dev(devb, dX, dY, order=order, nsub=nsub, dt=T / nsub,
-                                    form="powers")
+                                    form=FORM)
 
         bstep()
         devb.synchronize()
@@ -380,7 +385,8 @@ def main():
         tb = time.perf_counter() - t0
         y0 = dY.download()[:, 0]
         err = float(np.linalg.norm(y0 - y_plain) / np.linalg.norm(y_plain))
-        bk_ms, bk_bytes = devb.bench_term_kernel(reps=max(5, args.kernel_reps // 5), nv=nb)
+        bk_ms, bk_bytes = devb.bench_term_kernel(reps=max(5, args.kernel_reps // 5), nv=nb,
+                                                 form=FORM)
         devb.close()
         b_aps = nb * ksteps / tb
         b_gbs = bk_bytes / (bk_ms * 1e-3) / 1e9
@@ -393,6 +399,7 @@ def main():
             "speedup_vs_plain_fp64": b_aps / actions_per_s,
             "value_storage": "f32",
             "accumulation": "f64",
+            "taylor_form": FORM,
             "rel_err_vs_plain_fp64_action": err,
             "kernel": {0: f"k_spmm<{cfg['b']}, float, {nb}>",
                        1: f"k_spmm_mfma<{cfg['b']}, float, {nb}>",
@@ -437,6 +444,7 @@ def main():
             "nnzb": op.nnzb,
             "taylor_order": order,
             "substeps": nsub,
+            "taylor_form": FORM + " (POWER_METHOD!'s polynomial)",
             "kernels_per_action": order * nsub,
         },
         "roofline": {

@@ -14,7 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libhymor_hip.so"
 
 F64, C128 = 0, 1
 PREC_F64, PREC_F32 = 0, 1
-FORM_NORMALISED, FORM_POWERS = 0, 1
+FORM_NORMALISED, FORM_POWERS, FORM_HORNER = 0, 1, 2
 
 _i64 = C.c_int64
 _p = C.c_void_p
@@ -56,7 +56,7 @@ SIGNATURES = {
     "hymor_memcpy_d2h": (C.c_int, [_p, _p, _i64]),
     "hymor_memcpy_d2d": (C.c_int, [_p, _p, _i64]),
     "hymor_dev_fill_uniform": (C.c_int, [_p, _i64, C.c_uint64]),
-    "hymor_bench_term_kernel": (C.c_int, [_p, C.c_int, C.c_int, C.c_int, _pd, _pd]),
+    "hymor_bench_term_kernel": (C.c_int, [_p, C.c_int, C.c_int, C.c_int, C.c_int, _pd, _pd]),
     "hymor_op_autotune": (C.c_int, [_p, C.c_int, C.c_int, _pd]),
     "hymor_op_set_graphs": (C.c_int, [_p, C.c_int]),
     "hymor_op_set_batch_kernel": (C.c_int, [_p, C.c_int]),

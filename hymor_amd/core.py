@@ -14,9 +14,10 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import C128, F64, FORM_NORMALISED, FORM_POWERS, PREC_F32, PREC_F64, check, ptr
+from ._lib import (C128, F64, FORM_HORNER, FORM_NORMALISED, FORM_POWERS, PREC_F32, PREC_F64,
+                   check, ptr)
 
-_FORMS = {"normalised": FORM_NORMALISED, "powers": FORM_POWERS}
+_FORMS = {"normalised": FORM_NORMALISED, "powers": FORM_POWERS, "horner": FORM_HORNER}
 _PRECS = {"f64": PREC_F64, "f32": PREC_F32}
 
 
@@ -199,12 +200,14 @@ class DeviceOperator:
     def set_tuning(self, split, waves, nt):
         check(_lib.load().hymor_op_set_tuning(self._h, int(split), int(waves), int(nt)))
 
-    def bench_term_kernel(self, reps=20, transpose=False, nv=None):
+    def bench_term_kernel(self, reps=20, transpose=False, nv=None, form="powers"):
+        """(ms per launch, algorithmic bytes per launch) of a middle Taylor
+        term of `form` at nv doubles per row."""
         nv = nv or (2 if self.is_complex else 1)
         ms = C.c_double()
         by = C.c_double()
-        check(_lib.load().hymor_bench_term_kernel(self._h, int(bool(transpose)), nv, reps,
-                                                  C.byref(ms), C.byref(by)))
+        check(_lib.load().hymor_bench_term_kernel(self._h, int(bool(transpose)), nv,
+                                                  _FORMS[form], reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
 

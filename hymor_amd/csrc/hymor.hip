@@ -697,12 +697,33 @@ int pad_width(int64_t valid) {
     throw ArgError("nrhs too large: at most 64 doubles per row (64 real or 32 complex rhs)");
 }
 
-// T_order(A*dt)^nsub applied to `in` (cell-major, internal buffer).  Returns
-// the buffer that holds the result (ws[2] or ws[3], or `in` if nsub == 0).
+// T_order(A*dt)^nsub applied to `in` (cell-major, internal buffer, never
+// written).  Returns the buffer that holds the result: one of ws[2..5], or
+// `in` if nsub == 0.
 const double *run_taylor(hymor_op *op, bool adj, int order, int64_t nsub, double dt, int form,
                          int nv, const double *in) {
     double *va = op->ws[2], *vb = op->ws[3], *t0 = op->ws[4], *t1 = op->ws[5];
     const double *cur = in;
+    if (form == HYMOR_FORM_HORNER) {
+        // w = v; w = v + (dt/j) A w for j = order, ..., 1.  Each term reads
+        // the gathered w and v and writes the new w; v (the substep's input)
+        // is never written, w ping-pongs between two other buffers
+        double *const pool[4] = {va, vb, t0, t1};
+        for (int64_t s = 0; s < nsub; ++s) {
+            double *pp[2] = {nullptr, nullptr};
+            int np = 0;
+            for (double *p : pool)
+                if (p != cur && np < 2) pp[np++] = p;
+            const double *w = cur;
+            for (int j = order; j >= 1; --j) {
+                double *wn = pp[(order - j) & 1];
+                launch_term(op, adj, nv, op->stream, w, nullptr, cur, wn, dt / (double)j, 1.0);
+                w = wn;
+            }
+            cur = w;
+        }
+        return cur;
+    }
     for (int64_t s = 0; s < nsub; ++s) {
         double *nxt = (cur == va) ? vb : va;
         const double *x = cur;
@@ -776,7 +797,9 @@ void check_taylor_args(hymor_op *op, int transpose, int order, int64_t nsub, int
     ARG_CHECK(op != nullptr, "null operator");
     ARG_CHECK(order >= 1 && order <= 20, "order must be in [1, 20]");
     ARG_CHECK(nsub >= 0, "nsub must be >= 0");
-    ARG_CHECK(form == HYMOR_FORM_NORMALISED || form == HYMOR_FORM_POWERS, "unknown Taylor form");
+    ARG_CHECK(form == HYMOR_FORM_NORMALISED || form == HYMOR_FORM_POWERS ||
+                  form == HYMOR_FORM_HORNER,
+              "unknown Taylor form");
     ARG_CHECK(!transpose || op->has_adj, "operator was built without its adjoint");
 }
 
@@ -1394,10 +1417,14 @@ int hymor_op_set_tuning(hymor_op *op, int split, int waves, int nt) {
     });
 }
 
-int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int reps, double *ms_per_launch,
-                            double *bytes_per_launch) {
+int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int form, int reps,
+                            double *ms_per_launch, double *bytes_per_launch) {
     return guarded([&] {
         ARG_CHECK(op && ms_per_launch && bytes_per_launch, "null argument");
+        ARG_CHECK(form == HYMOR_FORM_NORMALISED || form == HYMOR_FORM_POWERS ||
+                      form == HYMOR_FORM_HORNER,
+                  "unknown Taylor form");
+        const bool horner = form == HYMOR_FORM_HORNER;
         ARG_CHECK(nv >= 1 && nv <= 64 && (nv & (nv - 1)) == 0, "nv must be a power of two <= 64");
         ARG_CHECK(!(op->is_complex && nv == 1), "complex operator needs nv >= 2");
         ARG_CHECK(reps >= 1, "reps must be >= 1");
@@ -1407,13 +1434,20 @@ int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int reps, doubl
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
-        // one warm launch, then the timed ones
-        launch_term(op, transpose != 0, nv, op->stream, op->ws[4], op->ws[5], op->ws[2], op->ws[2],
-                    0.5, 1.0);
+        // a middle term of the form: powers / normalised gather x, write the
+        // term and update vout in place; Horner gathers w, reads v, writes
+        // the new w.  One warm launch, then the timed ones
+        auto term = [&] {
+            if (horner)
+                launch_term(op, transpose != 0, nv, op->stream, op->ws[4], nullptr, op->ws[5],
+                            op->ws[2], 0.5, 1.0);
+            else
+                launch_term(op, transpose != 0, nv, op->stream, op->ws[4], op->ws[5], op->ws[2],
+                            op->ws[2], 0.5, 1.0);
+        };
+        term();
         HIP_TRY(hipEventRecord(e0, op->stream));
-        for (int i = 0; i < reps; ++i)
-            launch_term(op, transpose != 0, nv, op->stream, op->ws[4], op->ws[5], op->ws[2],
-                        op->ws[2], 0.5, 1.0);
+        for (int i = 0; i < reps; ++i) term();
         HIP_TRY(hipEventRecord(e1, op->stream));
         HIP_TRY(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -1424,9 +1458,10 @@ int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int reps, doubl
         const double sval = op->prec == HYMOR_PREC_F32 ? 4.0 : 8.0;
         const double bb = (double)s.b * s.b;
         // blocks (values + int32 column) + block-row pointers + x read,
-        // term written, vout read and written
+        // term written, vout read and written (Horner: w read, v read, new w
+        // written)
         double bytes = (double)s.nnzb * (bb * sval + 4.0) + (double)(s.ncell + 1) * 4.0 +
-                       4.0 * (double)op->n * nv * 8.0;
+                       (horner ? 3.0 : 4.0) * (double)op->n * nv * 8.0;
         const DevExtra &xe = transpose ? op->xa : op->xf;
         bytes += (double)xe.nnz * ((xe.is_complex ? 16.0 : 8.0) + 4.0);
         *bytes_per_launch = bytes;

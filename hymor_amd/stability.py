@@ -359,7 +359,7 @@ def _device_op(A, s, scale, value_prec):
 
 
 def EIGENVALUES_GPU_ARNOLDI(A, s, n_modes, *, T_ref=0.5, tol=1e-4, maxiter=None, ncv=None,
-                            v0=None, krylov="arpack", seed=0):
+                            v0=None, krylov="arpack", seed=0, form="powers"):
     """Arnoldi on the exponential transform (EIGENVALUES.jl:112-205).
 
     dt from CFL_TIMESTEP, timesteps = round(T_ref/dt), A*dt uploaded once,
@@ -370,7 +370,9 @@ def EIGENVALUES_GPU_ARNOLDI(A, s, n_modes, *, T_ref=0.5, tol=1e-4, maxiter=None,
     krylov="device" replaces ARPACK by the Krylov-Schur solver over a
     device-resident basis (hymor_amd.krylov): same selection (largest
     magnitude) and convergence test, no host round trip per matvec; with no
-    v0 its start vector is random on the device (`seed`).
+    v0 its start vector is random on the device (`seed`).  `form` "horner"
+    evaluates POWER_METHOD!'s polynomial by Horner's rule (three vector
+    passes per term instead of four; equal to rounding).
     """
     s = CFL_TIMESTEP(s)
     dt = s["time_integration"]["dt"]
@@ -384,7 +386,8 @@ def EIGENVALUES_GPU_ARNOLDI(A, s, n_modes, *, T_ref=0.5, tol=1e-4, maxiter=None,
         if krylov == "device":
             dev.set_graphs(True)      # every matvec repeats the same sweep
             def apply(din, dout):
-                expm_taylor_ptr(dev, din, dout, order=ORDER, nsub=timesteps, complex_=False)
+                expm_taylor_ptr(dev, din, dout, order=ORDER, nsub=timesteps, form=form,
+                                complex_=False)
                 dev.synchronize()
             vals, vecs, kinfo = krylov_schur(apply, n, k, ncv or max(20, 2 * k + 1), tol=tol,
                                              maxiter=maxiter or 40 * k, v0=v0, seed=seed)
@@ -665,7 +668,8 @@ def _device_generalized(devA, devC, D, s, n, k, n_t, n_set0, cplx, tol, maxiter,
 
 def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=4, ncv=None, tol=1e-10,
                      tol_f32=1e-7, maxiter=200, order=ORDER, value_prec="f32", refine=True,
-                     refine_block=16, refine_ncv=32, autotune=False, seed=0, timings=None):
+                     refine_block=16, refine_ncv=32, autotune=False, seed=0, form="horner",
+                     timings=None):
     """Leading eigenvalues of the exponential transform with batched actions.
 
     Phase 1: block Krylov-Schur whose every expansion applies T_order(A dt)^n
@@ -683,7 +687,9 @@ def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=4, ncv=None, tol=1e-10,
     A is a scipy matrix in the reference's var-major order or a
     BlockOperator.  autotune: time the batch kernels at each block width
     first (else the default tile kernel); `seed` selects the random start
-    block.  Returns (V, D, info); info holds
+    block; `form` is the Taylor recursion ("horner": the same polynomial as
+    POWER_METHOD!'s powers form, with three vector passes per term instead
+    of four).  Returns (V, D, info); info holds
     per-phase iterations, operator applications and seconds.
     """
     s = CFL_TIMESTEP(s)
@@ -697,7 +703,7 @@ def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=4, ncv=None, tol=1e-10,
     def make_apply(dev):
         def apply(d_in, ld, nvec, d_out):
             expm_taylor_batched_ptr(dev, d_in, ld, nvec, d_out, n, order=order, nsub=timesteps,
-                                    complex_=False)
+                                    form=form, complex_=False)
             dev.synchronize()
         return apply
 

@@ -52,6 +52,10 @@ extern "C" {
                                    (CONSTRUCT_C.jl:84-91, EXPONENTIAL!) */
 #define HYMOR_FORM_POWERS 1     /* v = A*v; vout += v/j!
                                    (ARNOLDI_POWER_GPU.jl:70-79, POWER_METHOD!) */
+#define HYMOR_FORM_HORNER 2     /* the same polynomial by Horner's rule,
+                                   w = v + (A*dt/j) w for j = order..1: three
+                                   vector passes per term instead of four (no
+                                   reference counterpart) */
 
 typedef struct hymor_op hymor_op;
 
@@ -181,11 +185,12 @@ int hymor_memcpy_d2d(void *d_dst, const void *d_src, int64_t bytes);
 int hymor_dev_fill_uniform(void *d_x, int64_t len, uint64_t seed);
 
 /* Timing probe: launches the fused Taylor-term kernel `reps` times on the
- * handle's stream (a middle term: gathers x, writes the new term, updates
- * vout) between two HIP events; returns the mean duration per launch in ms
- * and the kernel's algorithmic bytes per launch.  nv = doubles per row
- * (1 real, 2 complex, up to 64 for batches; a power of two). */
-int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int reps,
+ * handle's stream (a middle term of `form`: powers / normalised gather x,
+ * write the new term and update vout; Horner gathers w, reads v and writes
+ * the new w) between two HIP events; returns the mean duration per launch
+ * in ms and the kernel's algorithmic bytes per launch.  nv = doubles per
+ * row (1 real, 2 complex, up to 64 for batches; a power of two). */
+int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int form, int reps,
                             double *ms_per_launch, double *bytes_per_launch);
 
 /* Autotuner, no reference counterpart: times the launch variants of the

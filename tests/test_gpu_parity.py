@@ -208,6 +208,30 @@ def test_value_prec_switch_matches_fresh_upload(case):
             d32.set_value_prec("f64")
 
 
+@pytest.mark.parametrize("nrhs", [1, 2, 4, 8, 16])
+def test_horner_form(case, orc, nrhs):
+    """The Horner form evaluates POWER_METHOD!'s polynomial: the oracle's
+    powers-form action on every column at 1e-12, forward and adjoint; graph
+    replays give the eager bits."""
+    name, op, A, dev = case
+    say(f"[{name}] Horner form, {nrhs} vector(s)")
+    X = np.asfortranarray(np.random.default_rng(31).standard_normal((op.n, nrhs)))
+    kw = dict(order=5, nsub=3, dt=0.1, form="horner")
+    for tr, M in ((False, A), (True, orc.csc_adjoint(A))):
+        if nrhs == 1:
+            got = dev.expm_taylor(X[:, 0], transpose=tr, **kw)[:, None]
+        else:
+            got = dev.expm_taylor_batched(X, transpose=tr, **kw)
+        for j in range(nrhs):
+            assert relerr(got[:, j], orc.power_method(M.scaled(0.1), X[:, j], 3)) <= 1e-12
+    with DeviceOperator.from_blocks(op) as g:
+        g.set_graphs(True)
+        want = g.expm_taylor_batched(X, **kw) if nrhs > 1 else g.expm_taylor(X[:, 0], **kw)
+        for _ in range(3):
+            again = g.expm_taylor_batched(X, **kw) if nrhs > 1 else g.expm_taylor(X[:, 0], **kw)
+            np.testing.assert_array_equal(again, want)
+
+
 # ------------------------------------------------------------------ complex (receptivity)
 
 @pytest.fixture(scope="module")
@@ -232,6 +256,8 @@ def test_complex_spmv_and_taylor(recept, orc):
     x = _cvec(L_.n, 9)
     say("[receptivity] complex spmv / Taylor")
     assert relerr(dev.spmv(x), orc.csc_mul(A, x)) <= 1e-14
+    assert relerr(dev.expm_taylor(x, order=5, nsub=3, dt=0.1, form="horner"),
+                  orc.power_method(A.scaled(0.1), x, 3)) <= 1e-12
     assert relerr(dev.spmv(x, transpose=True), orc.csc_mul(orc.csc_adjoint(A), x)) <= 1e-14
     ref = orc.power_method(A.scaled(0.05), x, 6)
     assert relerr(dev.expm_taylor(x, order=5, nsub=6, dt=0.05), ref) <= 1e-12
@@ -267,7 +293,8 @@ def test_true_exponential_block_circulant(orc):
         ref = orc.circulant_expm(blocks, op.nchi, op.neta, x_cm, 1.0)[perm]
         with DeviceOperator.from_blocks(op) as dev:
             got = dev.expm_taylor(x, order=12, nsub=64, dt=1.0 / 64, form="normalised")
-        assert relerr(got, ref) <= 1e-10
+            got_h = dev.expm_taylor(x, order=12, nsub=64, dt=1.0 / 64, form="horner")
+        assert relerr(got, ref) <= 1e-10 and relerr(got_h, ref) <= 1e-10
 
 
 # ------------------------------------------------------------------ full config sizes
@@ -289,6 +316,8 @@ def test_c1_action_vs_oracle(c1, orc):
     x = np.random.default_rng(1).standard_normal(op.n)
     ref = orc.power_method(A.scaled(1.0 / 64), x, 64, order=12)
     got = dev.expm_taylor(x, order=12, nsub=64, dt=1.0 / 64, form="powers")
+    assert relerr(got, ref) <= 1e-11
+    got = dev.expm_taylor(x, order=12, nsub=64, dt=1.0 / 64, form="horner")
     assert relerr(got, ref) <= 1e-11
 
 
