@@ -163,7 +163,9 @@ int hymor_expm_taylor_batched_coldiag_dev(hymor_op *op, int transpose, int order
 /* Chu-norm sandwich y = T_p(A'dt)^n * C * T_p(A dt)^n * x with the zeroing
  * of CONSTRUCT_C (CONSTRUCT_C.jl:33-58 around EXPONENTIAL! :78-111):
  * x[0:n_set0) is zeroed before and y[0:n_set0) after (var-major indices).
- * C is a handle built from R'*M*R (TRANSIENT_GROWTH_DOWNSTREAM.jl:162). */
+ * C is a handle built from R'*M*R (TRANSIENT_GROWTH_DOWNSTREAM.jl:162).
+ * EXPONENTIAL!'s polynomial is evaluated by Horner's rule (equal to
+ * rounding; three vector passes per term instead of four). */
 int hymor_chu_sandwich(hymor_op *A, hymor_op *C, int order, int64_t nsub, double dt,
                        int64_t n_set0, const void *x, void *y, int dtype);
 
