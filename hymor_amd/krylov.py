@@ -260,6 +260,11 @@ def block_krylov_schur(apply_block, n: int, k: int, m: int, s: int, *, complex_:
                 K.orthonormalize(i, K.column(i))
         j = 0              # first column whose image is not yet in the basis
         n_apply = 0
+        # the wanted set is widened by one block where the basis leaves room:
+        # with clustered magnitudes the k-th and (k+1)-th Ritz values
+        # otherwise trade places between restarts, and each restart drops the
+        # directions that were converging
+        k_int = max(k, min(k + s, m - 2 * s))
         for it in range(1, maxiter + 1):
             while j + s <= m:
                 apply_block(K.column(j), n, s, W.ptr)
@@ -281,19 +286,20 @@ def block_krylov_schur(apply_block, n: int, k: int, m: int, s: int, *, complex_:
             Hm = H[:mm, :mm]
             Bres = H[mm:mm + s, :mm]          # residual rows
             evals = np.linalg.eigvals(Hm)
-            thr = _select_lm(evals, k)
+            thr = _select_lm(evals, k_int)
             if complex_:
                 T, Z, sdim = sl.schur(Hm, output="complex", sort=lambda z: abs(z) >= thr * (1 - 1e-12))
             else:
                 T, Z, sdim = sl.schur(Hm, output="real",
                                       sort=lambda re, im: abs(complex(re, im)) >= thr * (1 - 1e-12))
-            p_keep = max(sdim, k)
+            p_keep = max(sdim, k_int)
             w, Yt = sl.eig(T[:p_keep, :p_keep])
-            order = np.argsort(-np.abs(w), kind="stable")[:k]
+            order = np.argsort(-np.abs(w), kind="stable")[:k_int]
             w, Yt = w[order], Yt[:, order]
             res = np.linalg.norm(Bres @ (Z[:, :p_keep] @ Yt), axis=0)
             conv = res <= tol * np.maximum(np.abs(w), np.finfo(float).tiny)
-            if np.all(conv):
+            if np.all(conv[:k]):
+                w, Yt, res = w[:k], Yt[:, :k], res[:k]
                 Y = _ritz_vectors(K, Z[:, :p_keep] @ Yt, complex_) if return_vectors else None
                 info = {"iterations": it, "applications": n_apply, "residual_estimates": res}
                 if basis:
@@ -311,7 +317,7 @@ def block_krylov_schur(apply_block, n: int, k: int, m: int, s: int, *, complex_:
                     info["basis"] = Bd
                 return w, Y, info
             # restart: keep p Schur vectors, then the residual block
-            p = max(p_keep, min(p_keep + (k - int(conv.sum())), mm - s))
+            p = max(p_keep, min(p_keep + (k_int - int(conv.sum())), mm - s))
             if not complex_ and p < mm and p > p_keep and abs(T[p, p - 1]) > 0:
                 p += 1
             p = min(p, mm - 1)

@@ -70,3 +70,4 @@ def test_small_channel_eigenvalues_through_device_arnoldi():
     assert np.all(np.isfinite(D)), D
     for d in D:
         assert np.min(np.abs(exact - d)) <= 1e-8 * max(abs(d), 1.0), (d, exact[:8])
+
