@@ -50,3 +50,17 @@ def test_argument_errors_are_reported():
     assert rc == -1
     assert b"vars_per_cell" in lib.hymor_last_error()
     assert not h.value
+
+
+def test_new_entry_points_check_their_arguments():
+    """The precision switch, the device fill, the copies and the probes
+    reject null or malformed arguments before touching a device."""
+    lib = _lib.load()
+    assert lib.hymor_op_set_value_prec(None, 0) == -1
+    assert b"null" in lib.hymor_last_error()
+    assert lib.hymor_dev_fill_uniform(None, 8, 0) == -1
+    assert lib.hymor_memcpy_d2d(None, None, 8) == -1
+    ms = ctypes.c_double()
+    assert lib.hymor_bench_read(15, 1, ctypes.byref(ms)) == -1
+    assert b"multiple of 16" in lib.hymor_last_error()
+    assert lib.hymor_bench_term_kernel(None, 0, 1, 2, 1, ctypes.byref(ms), ctypes.byref(ms)) == -1
