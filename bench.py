@@ -163,6 +163,87 @@ def eigen_leg(k=10, tol=1e-9, block=4, seeds=(0, 1, 2)):
     return out
 
 
+def receptivity_leg(steps=3, order=12, nsub=64, T=1.0, n_w=32):
+    """C4: the action T_12(L_ T/64)^64 on the extended receptivity operator
+    L_ = [L repeat(B,1,32); 0 kron(diag(-i w), I)] (LINEARIZE_L_.jl:58-67) of
+    a 500 x 250-cell, b = 4 operator with 32 wavenumbers w_k = 0.1 *
+    100^((k-1)/31): the assembled operator with FP64 values applied to one
+    complex vector (the reference's form) against the factored form, the 32
+    per-wavenumber systems [L B; 0 -i w_k I] as 32 complex columns of one
+    batched sweep (B stored once), with FP64 and with FP32 values.  Vectors
+    resident on the device; error of each factored action against the
+    assembled FP64 one."""
+    from hymor_amd import _lib
+    from hymor_amd import operators as ops
+    from hymor_amd.core import _FORMS, DeviceMatrix, DeviceOperator, DeviceVector, expm_taylor_dev
+    from hymor_amd.receptivity import FactoredReceptivity, split_extended
+
+    base = ops.config_operator("C4")
+    w = 0.1 * 100.0 ** (np.arange(n_w) / (n_w - 1.0))
+    L_ = ops.receptivity_extension(base, w)
+    A_ = L_.to_scipy_varmajor()
+    n_L, p = base.n, 4 * base.nchi
+    L, B = split_extended(A_, n_L, p, w)
+    n_ext = A_.shape[0]
+    rng = np.random.default_rng(44)
+    x = rng.standard_normal(n_ext) + 1j * rng.standard_normal(n_ext)
+    dt = T / nsub
+    out = {"workload": (f"C4: 500x250 cells, b=4, {n_w} wavenumbers, N_ext={n_ext} complex; "
+                        f"action T_{order}(L_ T/{nsub})^{nsub} x, T={T}, {FORM} form"),
+           "steps": steps}
+    log(f"receptivity leg: assembled L_ (n_ext={n_ext}), FP64 values")
+    with DeviceOperator.from_blocks(L_, with_adjoint=False) as dev:
+        dev.set_graphs(True)
+        dx = DeviceVector.from_host(x)
+        dy = DeviceVector(n_ext, np.complex128)
+        expm_taylor_dev(dev, dx, dy, order=order, nsub=nsub, dt=dt, form=FORM)
+        dev.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            expm_taylor_dev(dev, dx, dy, order=order, nsub=nsub, dt=dt, form=FORM)
+        dev.synchronize()
+        t_asm = (time.perf_counter() - t0) / steps
+        y_asm = dy.download()
+        dx.free()
+        dy.free()
+    out["assembled_fp64"] = {"seconds_per_action": t_asm}
+    for prec in ("f64", "f32"):
+        log(f"receptivity leg: factored, {prec} values")
+        fr = FactoredReceptivity(L, B, w, nchi=base.nchi, neta=base.neta, vars_per_cell=4,
+                                 scale=1.0, value_prec=prec)
+        try:
+            X = np.zeros((fr.n_sys, n_w), dtype=np.complex128, order="F")
+            X[n_L:, :] = x[n_L:].reshape(n_w, p).T
+            X[:n_L, 0] = x[:n_L]
+            dX = DeviceMatrix.from_host(X)
+            dY = DeviceMatrix(fr.n_sys, n_w, np.complex128)
+            lib = _lib.load()
+
+            def run():
+                _lib.check(lib.hymor_expm_taylor_batched_coldiag_dev(
+                    fr.dev.handle, 0, order, nsub, dt, _FORMS[FORM], n_w, fr.diag.ctypes.data, fr.n_L, fr.n_sys, dX.ptr, fr.n_sys, dY.ptr, fr.n_sys))
+
+            run()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                run()
+            t_f = (time.perf_counter() - t0) / steps
+            Y = dY.download()
+            y = np.empty(n_ext, dtype=np.complex128)
+            y[:n_L] = Y[:n_L, :].sum(axis=1)
+            y[n_L:] = Y[n_L:, :].T.reshape(-1)
+            dX.free()
+            dY.free()
+        finally:
+            fr.close()
+        err = float(np.linalg.norm(y - y_asm) / np.linalg.norm(y_asm))
+        out[f"factored_{prec}_values"] = {"seconds_per_action": t_f,
+                                          "speedup_vs_assembled_fp64": t_asm / t_f,
+                                          "rel_err_vs_assembled_fp64": err}
+        log(f"receptivity leg: factored {prec}: {t_f:.3f} s ({t_asm / t_f:.2f}x), err {err:.1e}")
+    return out
+
+
 COPY_MIB = 512
 # the library's default launches: k_term_lds (k_term_split_p with 3 waves per
 # slice at one vector) and the tile-order matrix-core batch kernel
@@ -258,6 +339,8 @@ def main():
     ap.add_argument("--nrhs", type=int, default=16, help="vectors per batched sweep")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-eigen", action="store_true", help="skip the eigenvalue leg")
+    ap.add_argument("--no-receptivity", action="store_true",
+                    help="skip the C4 receptivity leg (assembled vs factored)")
     ap.add_argument("--kernel-only", choices=["plain", "batched", "block4"],
                     help="time only the Taylor-term kernel of one form (for profiler runs): "
                          "plain FP64 one vector, batched FP32 --nrhs vectors, block4 FP32 "
@@ -467,6 +550,8 @@ def main():
         },
         "batched": batched,
     }
+    if not args.no_receptivity:
+        result["batched_receptivity"] = receptivity_leg()
     if not args.no_eigen:
         log("eigen leg: plain FP64 Krylov-Schur vs batched FP32 + FP64 refinement")
         result["batched_eigen"] = eigen_leg()

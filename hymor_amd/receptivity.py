@@ -53,9 +53,13 @@ def split_extended(A_, n_L: int, p: int, w_infty):
         Bk = A_[:n_L, n_L + k * p:n_L + (k + 1) * p]
         if abs(Bk - B).max() != 0:
             raise ValueError("the freestream columns are not repeated copies of B")
-    D = A_[n_L:, n_L:].toarray()
-    want = np.kron(np.diag(-1j * np.asarray(w_infty, dtype=float)), np.eye(p))
-    if not np.allclose(D, want, rtol=0, atol=1e-14 * max(1.0, np.abs(want).max())):
+    # compared sparse: the block is N*p square (64 000 at 32 wavenumbers)
+    D = sp.csr_matrix(A_[n_L:, n_L:])
+    w = np.asarray(w_infty, dtype=float)
+    want = sp.kron(sp.diags(-1j * w), sp.identity(p), format="csr")
+    diff = (D - want).tocoo()
+    tol = 1e-14 * max(1.0, float(np.abs(w).max()))
+    if diff.nnz and np.abs(diff.data).max() > tol:
         raise ValueError("the freestream block is not kron(diag(-i w), I)")
     if A_[n_L:, :n_L].nnz and abs(A_[n_L:, :n_L]).max() != 0:
         raise ValueError("the lower-left block must be zero")
