@@ -163,19 +163,26 @@ def eigen_leg(k=10, tol=1e-9, block=4, seeds=(0, 1, 2)):
     return out
 
 
-def receptivity_leg(steps=3, order=12, nsub=64, T=1.0, n_w=32):
+def receptivity_leg(steps=3, order=12, nsub=64, T=1.0, n_w=32, nrhs=16):
     """C4: the action T_12(L_ T/64)^64 on the extended receptivity operator
     L_ = [L repeat(B,1,32); 0 kron(diag(-i w), I)] (LINEARIZE_L_.jl:58-67) of
     a 500 x 250-cell, b = 4 operator with 32 wavenumbers w_k = 0.1 *
-    100^((k-1)/31): the assembled operator with FP64 values applied to one
-    complex vector (the reference's form) against the factored form, the 32
-    per-wavenumber systems [L B; 0 -i w_k I] as 32 complex columns of one
-    batched sweep (B stored once), with FP64 and with FP32 values.  Vectors
-    resident on the device; error of each factored action against the
-    assembled FP64 one."""
+    100^((k-1)/31), in extended actions per second:
+      * assembled L_, FP64 values, one complex vector per sweep (the
+        reference's form);
+      * assembled L_, `nrhs` complex vectors per sweep (2*nrhs doubles per
+        row), FP32 and FP64 values -- the batch a block eigensolver over
+        the extended space applies;
+      * factored: the 32 per-wavenumber systems [L B; 0 -i w_k I] of ONE
+        extended vector as 32 complex columns of one sweep (B stored once).
+        Every flow SpMV is paid 32 times, against once in the assembled
+        form, so it trades time for the N-fold storage of B.
+    Vectors resident on the device; errors against the assembled FP64
+    single-vector action."""
     from hymor_amd import _lib
     from hymor_amd import operators as ops
-    from hymor_amd.core import _FORMS, DeviceMatrix, DeviceOperator, DeviceVector, expm_taylor_dev
+    from hymor_amd.core import (_FORMS, DeviceMatrix, DeviceOperator, DeviceVector,
+                                expm_taylor_batched_dev, expm_taylor_dev)
     from hymor_amd.receptivity import FactoredReceptivity, split_extended
 
     base = ops.config_operator("C4")
@@ -186,61 +193,81 @@ def receptivity_leg(steps=3, order=12, nsub=64, T=1.0, n_w=32):
     L, B = split_extended(A_, n_L, p, w)
     n_ext = A_.shape[0]
     rng = np.random.default_rng(44)
-    x = rng.standard_normal(n_ext) + 1j * rng.standard_normal(n_ext)
+    X = (rng.standard_normal((n_ext, nrhs)) + 1j * rng.standard_normal((n_ext, nrhs)))
+    x = X[:, 0].copy()
     dt = T / nsub
     out = {"workload": (f"C4: 500x250 cells, b=4, {n_w} wavenumbers, N_ext={n_ext} complex; "
                         f"action T_{order}(L_ T/{nsub})^{nsub} x, T={T}, {FORM} form"),
-           "steps": steps}
-    log(f"receptivity leg: assembled L_ (n_ext={n_ext}), FP64 values")
-    with DeviceOperator.from_blocks(L_, with_adjoint=False) as dev:
+           "unit": "extended actions/s", "steps": steps}
+
+    def timed(run, sync):
+        run()
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run()
+        sync()
+        return (time.perf_counter() - t0) / steps
+
+    log(f"receptivity leg: assembled L_ (n_ext={n_ext}), FP64 values, one vector")
+    with DeviceOperator.from_blocks(L_, value_prec="f64", with_adjoint=False) as dev:
         dev.set_graphs(True)
         dx = DeviceVector.from_host(x)
         dy = DeviceVector(n_ext, np.complex128)
-        expm_taylor_dev(dev, dx, dy, order=order, nsub=nsub, dt=dt, form=FORM)
-        dev.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            expm_taylor_dev(dev, dx, dy, order=order, nsub=nsub, dt=dt, form=FORM)
-        dev.synchronize()
-        t_asm = (time.perf_counter() - t0) / steps
+        t1 = timed(lambda: expm_taylor_dev(dev, dx, dy, order=order, nsub=nsub, dt=dt,
+                                           form=FORM), dev.synchronize)
         y_asm = dy.download()
         dx.free()
         dy.free()
-    out["assembled_fp64"] = {"seconds_per_action": t_asm}
-    for prec in ("f64", "f32"):
-        log(f"receptivity leg: factored, {prec} values")
-        fr = FactoredReceptivity(L, B, w, nchi=base.nchi, neta=base.neta, vars_per_cell=4,
-                                 scale=1.0, value_prec=prec)
-        try:
-            X = np.zeros((fr.n_sys, n_w), dtype=np.complex128, order="F")
-            X[n_L:, :] = x[n_L:].reshape(n_w, p).T
-            X[:n_L, 0] = x[:n_L]
-            dX = DeviceMatrix.from_host(X)
-            dY = DeviceMatrix(fr.n_sys, n_w, np.complex128)
-            lib = _lib.load()
-
-            def run():
-                _lib.check(lib.hymor_expm_taylor_batched_coldiag_dev(
-                    fr.dev.handle, 0, order, nsub, dt, _FORMS[FORM], n_w, fr.diag.ctypes.data, fr.n_L, fr.n_sys, dX.ptr, fr.n_sys, dY.ptr, fr.n_sys))
-
-            run()
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                run()
-            t_f = (time.perf_counter() - t0) / steps
-            Y = dY.download()
-            y = np.empty(n_ext, dtype=np.complex128)
-            y[:n_L] = Y[:n_L, :].sum(axis=1)
-            y[n_L:] = Y[n_L:, :].T.reshape(-1)
+        out["assembled_fp64_single"] = {"value": 1.0 / t1, "seconds_per_sweep": t1}
+        for prec in ("f64", "f32"):
+            dev.set_value_prec(prec)
+            log(f"receptivity leg: assembled L_, {prec} values, {nrhs} vectors per sweep")
+            dX = DeviceMatrix.from_host(np.asfortranarray(X))
+            dY = DeviceMatrix(n_ext, nrhs, np.complex128)
+            tb = timed(lambda: expm_taylor_batched_dev(dev, dX, dY, order=order, nsub=nsub,
+                                                       dt=dt, form=FORM), dev.synchronize)
+            y0 = dY.download()[:, 0]
             dX.free()
             dY.free()
-        finally:
-            fr.close()
-        err = float(np.linalg.norm(y - y_asm) / np.linalg.norm(y_asm))
-        out[f"factored_{prec}_values"] = {"seconds_per_action": t_f,
-                                          "speedup_vs_assembled_fp64": t_asm / t_f,
-                                          "rel_err_vs_assembled_fp64": err}
-        log(f"receptivity leg: factored {prec}: {t_f:.3f} s ({t_asm / t_f:.2f}x), err {err:.1e}")
+            out[f"assembled_{prec}_values_x{nrhs}"] = {
+                "value": nrhs / tb, "seconds_per_sweep": tb,
+                "speedup_vs_assembled_fp64_single": t1 * nrhs / tb,
+                "rel_err_vs_assembled_fp64_single": float(np.linalg.norm(y0 - y_asm) /
+                                                          np.linalg.norm(y_asm))}
+            log(f"receptivity leg: {prec} x{nrhs}: {nrhs / tb:.1f} actions/s "
+                f"({t1 * nrhs / tb:.2f}x)")
+    log("receptivity leg: factored (32 per-wavenumber systems of one vector), FP32 values")
+    fr = FactoredReceptivity(L, B, w, nchi=base.nchi, neta=base.neta, vars_per_cell=4,
+                             scale=1.0, value_prec="f32")
+    try:
+        Xf = np.zeros((fr.n_sys, n_w), dtype=np.complex128, order="F")
+        Xf[n_L:, :] = x[n_L:].reshape(n_w, p).T
+        Xf[:n_L, 0] = x[:n_L]
+        dX = DeviceMatrix.from_host(Xf)
+        dY = DeviceMatrix(fr.n_sys, n_w, np.complex128)
+        lib = _lib.load()
+
+        def run():
+            _lib.check(lib.hymor_expm_taylor_batched_coldiag_dev(
+                fr.dev.handle, 0, order, nsub, dt, _FORMS[FORM], n_w, fr.diag.ctypes.data,
+                fr.n_L, fr.n_sys, dX.ptr, fr.n_sys, dY.ptr, fr.n_sys))
+
+        tf = timed(run, fr.dev.synchronize)
+        Y = dY.download()
+        y = np.empty(n_ext, dtype=np.complex128)
+        y[:n_L] = Y[:n_L, :].sum(axis=1)
+        y[n_L:] = Y[n_L:, :].T.reshape(-1)
+        dX.free()
+        dY.free()
+    finally:
+        fr.close()
+    out["factored_f32_values"] = {
+        "value": 1.0 / tf, "seconds_per_sweep": tf,
+        "speedup_vs_assembled_fp64_single": t1 / tf,
+        "rel_err_vs_assembled_fp64_single": float(np.linalg.norm(y - y_asm) /
+                                                  np.linalg.norm(y_asm))}
+    log(f"receptivity leg: factored: {1.0 / tf:.2f} actions/s ({t1 / tf:.2f}x)")
     return out
 
 

@@ -77,6 +77,7 @@ struct DevSliced {
 struct DevExtra {
     bool is_complex = false;
     int64_t nrows = 0, nnz = 0;
+    int64_t nlong = 0;   // listed rows [0, nlong) have more than kShortRow entries
     int64_t *rows = nullptr, *ptr = nullptr;
     int32_t *col = nullptr;
     double *val = nullptr;
@@ -271,14 +272,41 @@ void dfree(hymor_op *op, void *p, size_t bytes) {
         }
 }
 
+// Rows of the extra CSR with at most this many entries go to k_extra_short
+// (lane per row and column); longer ones to k_extra (lane per entry).
+constexpr int64_t kShortRow = 32;
+
+// The listed rows reordered long rows first (each group keeps its order),
+// so the two kernels take contiguous ranges.
 void upload_extra(hymor_op *op, const HostExtra &h, DevExtra &d) {
+    const int64_t nr = (int64_t)h.rows.size();
+    const int vs = h.is_complex ? 2 : 1;
+    std::vector<int64_t> order;
+    order.reserve((size_t)nr);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int64_t r = 0; r < nr; ++r)
+            if ((h.ptr[r + 1] - h.ptr[r] > kShortRow) == (pass == 0)) order.push_back(r);
+    std::vector<int64_t> rows((size_t)nr), ptr((size_t)nr + 1, 0);
+    std::vector<int32_t> col(h.col.size());
+    std::vector<double> val(h.val.size());
+    int64_t nlong = 0;
+    for (int64_t i = 0; i < nr; ++i) {
+        const int64_t r = order[(size_t)i], b0 = h.ptr[r], len = h.ptr[r + 1] - b0;
+        if (len > kShortRow) ++nlong;
+        rows[(size_t)i] = h.rows[(size_t)r];
+        ptr[(size_t)i + 1] = ptr[(size_t)i] + len;
+        std::copy(h.col.begin() + b0, h.col.begin() + b0 + len, col.begin() + ptr[(size_t)i]);
+        std::copy(h.val.begin() + b0 * vs, h.val.begin() + (b0 + len) * vs,
+                  val.begin() + ptr[(size_t)i] * vs);
+    }
     d.is_complex = h.is_complex;
-    d.nrows = (int64_t)h.rows.size();
+    d.nrows = nr;
+    d.nlong = nlong;
     d.nnz = (int64_t)h.col.size();
-    d.rows = upload(op, h.rows);
-    d.ptr = upload(op, h.ptr);
-    d.col = upload(op, h.col);
-    d.val = upload(op, h.val);
+    d.rows = upload(op, rows);
+    d.ptr = upload(op, ptr);
+    d.col = upload(op, col);
+    d.val = upload(op, val);
 }
 
 void drop_graph(hymor_op *op) {
@@ -622,15 +650,35 @@ void launch_extra_nv(const DevExtra &xe, int64_t nflow, hipStream_t st, const do
                      double *yx, double *tout, const double *vin, double *vout, double c1,
                      double c2, const ColDiag &cd, int tile_b) {
     const int wpb = 4;
-    const int64_t grid = (xe.nrows + wpb - 1) / wpb;
-    if (xe.is_complex)
-        hipLaunchKernelGGL((k_extra<NV, true>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
-                           xe.rows, xe.ptr, xe.col, xe.val, xe.nrows, nflow, x, yx, tout, vin,
-                           vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
-    else
-        hipLaunchKernelGGL((k_extra<NV, false>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
-                           xe.rows, xe.ptr, xe.col, xe.val, xe.nrows, nflow, x, yx, tout, vin,
-                           vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
+    if (xe.nlong > 0) {
+        const int64_t grid = (xe.nlong + wpb - 1) / wpb;
+        if (xe.is_complex)
+            hipLaunchKernelGGL((k_extra<NV, true>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
+                               xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, nflow, x, yx, tout, vin,
+                               vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
+        else
+            hipLaunchKernelGGL((k_extra<NV, false>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
+                               xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, nflow, x, yx, tout, vin,
+                               vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
+    }
+    const int64_t nshort = xe.nrows - xe.nlong;
+    if (nshort > 0) {
+        if constexpr (NV < 2) {
+            if (xe.is_complex) throw ArgError("complex extras need two doubles per row");
+        }
+        const int lpr = xe.is_complex ? (NV >= 2 ? NV / 2 : 1) : NV;
+        const int64_t rpw = 64 / lpr;
+        const int64_t waves = (nshort + rpw - 1) / rpw;
+        const int64_t grid = (waves + wpb - 1) / wpb;
+        if (xe.is_complex)
+            hipLaunchKernelGGL((k_extra_short<NV, true>), dim3((unsigned)grid), dim3(64 * wpb),
+                               0, st, xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, xe.nrows, nflow,
+                               x, yx, tout, vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
+        else
+            hipLaunchKernelGGL((k_extra_short<NV, false>), dim3((unsigned)grid), dim3(64 * wpb),
+                               0, st, xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, xe.nrows, nflow,
+                               x, yx, tout, vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
+    }
 }
 
 void launch_extra(const DevExtra &xe, int nv, int64_t nflow, hipStream_t st, const double *x,

@@ -232,6 +232,65 @@ __global__ __launch_bounds__(256) void k_extra(
     }
 }
 
+// Extra CSR rows with few entries (the -i*omega diagonal of the freestream
+// DOFs, rows touching one shock DOF): no cross-lane reduction, each lane
+// owns one (row, column) pair -- a complex column pair when CPLX -- and
+// loops over the row's entries; RPW = 64 / lanes-per-row rows per wave.
+// Rows [r0, nrows) of the listed rows; same outputs as k_extra.
+template <int NV, bool CPLX>
+__global__ __launch_bounds__(256) void k_extra_short(
+    const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
+    const int32_t *__restrict__ col, const double *__restrict__ val, int64_t r0, int64_t nrows,
+    int64_t nflow, const double *__restrict__ x, double *__restrict__ yx,
+    double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
+    const double *__restrict__ cdiag, int64_t cd0, int64_t cd1, int tile_b) {
+    constexpr int LPR = CPLX ? (NV >= 2 ? NV / 2 : 1) : NV;   // lanes per row
+    constexpr int RPW = 64 / LPR;                             // rows per wave
+    constexpr int NC = CPLX ? 2 : 1;                          // doubles per lane
+    static_assert(64 % LPR == 0, "lanes per row divide the wave");
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t ri = r0 + w * RPW + lane / LPR;
+    if (ri >= nrows) return;
+    const int v0 = (lane % LPR) * NC;    // first of this lane's doubles in the row
+    const int64_t row = rows[ri];
+    double acc[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) acc[q] = 0.0;
+    const int64_t e1 = ptr[ri + 1];
+    for (int64_t e = ptr[ri]; e < e1; ++e) {
+        const int64_t c = col[e];
+        if constexpr (CPLX) {
+            const double ar = val[2 * e], ai = val[2 * e + 1];
+            const double xr = x[buf_pos(c, v0, NV, tile_b, nflow)];
+            const double xi = x[buf_pos(c, v0 + 1, NV, tile_b, nflow)];
+            acc[0] = fma(ar, xr, fma(-ai, xi, acc[0]));
+            acc[1] = fma(ar, xi, fma(ai, xr, acc[1]));
+        } else {
+            acc[0] = fma(val[e], x[buf_pos(c, v0, NV, tile_b, nflow)], acc[0]);
+        }
+    }
+    if constexpr (CPLX) {
+        if (cdiag && row >= cd0 && row < cd1) {
+            const double dr = cdiag[v0], di = cdiag[v0 + 1];
+            const double xr = x[row * NV + v0], xi = x[row * NV + v0 + 1];
+            acc[0] = fma(dr, xr, fma(-di, xi, acc[0]));
+            acc[1] = fma(dr, xi, fma(di, xr, acc[1]));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        const int v = v0 + q;
+        if (row < nflow) {
+            yx[buf_pos(row, v, NV, tile_b, nflow)] = acc[q];
+        } else {
+            const double t = c1 * acc[q];
+            if (tout) tout[row * NV + v] = t;
+            if (vout) vout[row * NV + v] = fma(c2, t, vin[row * NV + v]);
+        }
+    }
+}
+
 // var-major (ABI) -> cell-major (internal).  x_vm holds nrhs columns of n
 // elements with leading dimension ld (elements); an element is ncomp
 // doubles.  Internal rows hold nrhs*ncomp doubles.  Var-major indices below
