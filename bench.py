@@ -163,16 +163,16 @@ def eigen_leg(k=10, tol=1e-9, block=4, seeds=(0, 1, 2)):
     return out
 
 
-def receptivity_leg(steps=3, order=12, nsub=64, T=1.0, n_w=32, nrhs=16):
+def receptivity_leg(steps=3, order=12, nsub=64, T=1.0, n_w=32, widths=(4, 16)):
     """C4: the action T_12(L_ T/64)^64 on the extended receptivity operator
     L_ = [L repeat(B,1,32); 0 kron(diag(-i w), I)] (LINEARIZE_L_.jl:58-67) of
     a 500 x 250-cell, b = 4 operator with 32 wavenumbers w_k = 0.1 *
     100^((k-1)/31), in extended actions per second:
       * assembled L_, FP64 values, one complex vector per sweep (the
         reference's form);
-      * assembled L_, `nrhs` complex vectors per sweep (2*nrhs doubles per
-        row), FP32 and FP64 values -- the batch a block eigensolver over
-        the extended space applies;
+      * assembled L_, 4 and 16 complex vectors per sweep (8 and 32 doubles
+        per row), FP32 and FP64 values -- the batches a block eigensolver
+        over the extended space applies;
       * factored: the 32 per-wavenumber systems [L B; 0 -i w_k I] of ONE
         extended vector as 32 complex columns of one sweep (B stored once).
         Every flow SpMV is paid 32 times, against once in the assembled
@@ -193,7 +193,8 @@ def receptivity_leg(steps=3, order=12, nsub=64, T=1.0, n_w=32, nrhs=16):
     L, B = split_extended(A_, n_L, p, w)
     n_ext = A_.shape[0]
     rng = np.random.default_rng(44)
-    X = (rng.standard_normal((n_ext, nrhs)) + 1j * rng.standard_normal((n_ext, nrhs)))
+    nmax = max(widths)
+    X = (rng.standard_normal((n_ext, nmax)) + 1j * rng.standard_normal((n_ext, nmax)))
     x = X[:, 0].copy()
     dt = T / nsub
     out = {"workload": (f"C4: 500x250 cells, b=4, {n_w} wavenumbers, N_ext={n_ext} complex; "
@@ -220,10 +221,10 @@ def receptivity_leg(steps=3, order=12, nsub=64, T=1.0, n_w=32, nrhs=16):
         dx.free()
         dy.free()
         out["assembled_fp64_single"] = {"value": 1.0 / t1, "seconds_per_sweep": t1}
-        for prec in ("f64", "f32"):
+        for prec, nrhs in [(p_, w_) for w_ in widths for p_ in ("f64", "f32")]:
             dev.set_value_prec(prec)
             log(f"receptivity leg: assembled L_, {prec} values, {nrhs} vectors per sweep")
-            dX = DeviceMatrix.from_host(np.asfortranarray(X))
+            dX = DeviceMatrix.from_host(np.asfortranarray(X[:, :nrhs]))
             dY = DeviceMatrix(n_ext, nrhs, np.complex128)
             tb = timed(lambda: expm_taylor_batched_dev(dev, dX, dY, order=order, nsub=nsub,
                                                        dt=dt, form=FORM), dev.synchronize)

@@ -425,6 +425,8 @@ void tile_vals_of(hymor_op *op, DevSliced &s) {
 // before any launch (or graph capture) of those kernels.
 void ensure_tile(hymor_op *op) {
     if (op->ncell == 0) return;
+    // k_spmm_quad4 (kernel 7 at b = 4) reads the lane-interleaved values
+    if (op->fwd.tune.spmm_mfma == 7 && op->b == 4) return;
     const bool quad = op->fwd.tune.spmm_mfma == 7 && op->b == 8;
     auto missing = [&](const DevSliced &s) { return quad ? !s.val_q : !s.val_t; };
     if (!missing(op->fwd) && !(op->has_adj && missing(op->adj))) return;
@@ -472,7 +474,7 @@ template <int B, typename VT, int NV, bool NT>
 void launch_term_nt(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
                     double *tout, const double *vin, double *vout, double c1, double c2) {
     const TermTuning &tu = s.tune;
-    if constexpr (NV >= 2 && NV <= 4) {
+    if constexpr (NV >= 2 && (NV <= 4 || (B == 4 && NV == 8))) {
         if (tu.split == 3) {
             switch (tu.waves) {
             case 1: launch_lds<B, VT, NV, 1, NT>(s, st, x, yx, tout, vin, vout, c1, c2); break;
@@ -587,10 +589,39 @@ void launch_quad_t(const DevSliced &s, hipStream_t st, const double *x, const do
                        c1, c2);
 }
 
+template <typename VT, int R>
+void launch_quad4_t(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+                    double *tout, const double *vin, double *vout, double c1, double c2) {
+    constexpr int W = R <= 32 ? 9 : 5;
+    const int wpb = 4;
+    const int64_t grid = (s.ncell + wpb - 1) / wpb;
+    hipLaunchKernelGGL((k_spmm_quad4<VT, R, W>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
+                       s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, yx, tout, vin, vout,
+                       c1, c2);
+}
+
 template <int B, typename VT>
 void launch_block_part(const DevSliced &s, int nv, hipStream_t st, const double *x,
                        const double *yx, double *tout, const double *vin, double *vout, double c1,
                        double c2) {
+    if constexpr (B == 4) {
+        // 8 doubles per row at b = 4: the LDS-staged vector-FMA term kernel
+        // beats the matrix-core ones (whose one-cell waves do too little
+        // work at b = 4); the batch order is row-major at b = 4 either way
+        if (nv == 8 && s.tune.spmm_mfma == 6) {
+            launch_term_t<B, VT, 8>(s, st, x, yx, tout, vin, vout, c1, c2);
+            return;
+        }
+        if (nv >= 8 && s.tune.spmm_mfma == 7) {
+            switch (nv) {
+            case 8: launch_quad4_t<VT, 8>(s, st, x, yx, tout, vin, vout, c1, c2); return;
+            case 16: launch_quad4_t<VT, 16>(s, st, x, yx, tout, vin, vout, c1, c2); return;
+            case 32: launch_quad4_t<VT, 32>(s, st, x, yx, tout, vin, vout, c1, c2); return;
+            case 64: launch_quad4_t<VT, 64>(s, st, x, yx, tout, vin, vout, c1, c2); return;
+            default: throw ArgError("unsupported vector width");
+            }
+        }
+    }
     if constexpr (B == 8) {
         if (nv >= 8 && s.tune.spmm_mfma == 7) {
             switch (nv) {

@@ -1759,4 +1759,84 @@ __global__ __launch_bounds__(256) void k_spmm_quad(
     }
 }
 
+// Batched Taylor term for b = 4 on v_mfma_f64_4x4x4f64, one wave per cell,
+// R = 8..64 vectors per row in row-major order (at b = 4 the tile order is
+// row-major).  Per slot the cell's 4x4 block times its 4 x R slice of x is
+// R/4 block products, four per instruction: lane (n, blk, k) = n + 4 blk +
+// 16 k supplies A[m = n][k] (the block value, read in the lane-interleaved
+// order: no second copy of the values) and x[k][16 i + 4 blk + n]; the
+// product of column group blk lands in lane n + 4 blk + 16 m, so lane l
+// owns row m = l / 16 and columns 16 i + 4 blk + n.  No padding rows (the
+// 16x16x4 form idles 12 of 16 rows at b = 4).  R = 8: groups 2, 3 repeat
+// groups 0, 1 and are dropped.  W slots' loads are issued together; the
+// group's block columns come from one vector load.  Summation order: slots
+// ascending.
+template <typename VT, int R, int W>
+__global__ __launch_bounds__(256) void k_spmm_quad4(
+    const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
+    const VT *__restrict__ val, int64_t ncell, const double *__restrict__ x,
+    const double *__restrict__ yx, double *__restrict__ tout, const double *vin, double *vout,
+    double c1, double c2) {
+    constexpr int B = 4, CPS = 16, BB = 16;
+    constexpr int V = Vpl<VT>::value;
+    constexpr int NI = R >= 16 ? R / 16 : 1;
+    static_assert(R == 8 || R == 16 || R == 32 || R == 64, "width 8..64");
+    static_assert(W <= 64, "one column per lane");
+    const int lane = threadIdx.x & 63;
+    const int64_t wpb = blockDim.x >> 6;
+    const int64_t cell = (int64_t)__builtin_amdgcn_readfirstlane(
+        (int)(xcd_tile(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6)));
+    if (cell >= ncell) return;
+    const int n4 = lane & 3, blk = (lane >> 2) & 3, k4 = lane >> 4;
+    const int g = R >= 16 ? 4 * blk + n4 : 4 * (blk & 1) + n4;
+    const int64_t slice = cell / CPS, q = cell - slice * CPS;
+    const int64_t p0 = slice_ptr[slice];
+    const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
+    const int64_t cbase = cell * B * R;
+    const int aoff = ((k4 / V) * 64 + (int)q * 4 + n4) * V + k4 % V;
+    const double *pv = vout ? vin : x;
+    const double *py = yx ? yx : x;
+    double pre_v[NI], pre_y[NI], acc[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int64_t o = cbase + (int64_t)k4 * R + 16 * i + g;
+        pre_v[i] = pv[o];
+        pre_y[i] = py[o];
+        acc[i] = 0.0;
+    }
+    for (int t0 = 0; t0 < width; t0 += W) {
+        const int tl = t0 + (lane < W ? lane : 0);
+        const int32_t cl = col[p0 + (int64_t)(tl < width ? tl : width - 1) * CPS + q];
+        int32_t c[W];
+#pragma unroll
+        for (int u = 0; u < W; ++u) c[u] = __builtin_amdgcn_readlane(cl, u);
+        VT a[W];
+        double bx[W][NI];
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            const int t = t0 + u < width ? t0 + u : width - 1;
+            a[u] = __builtin_nontemporal_load(val + (p0 + (int64_t)t * CPS) * BB + aoff);
+            const double *xp = x + (int64_t)c[u] * B * R + (int64_t)k4 * R + g;
+#pragma unroll
+            for (int i = 0; i < NI; ++i) bx[u][i] = xp[16 * i];
+        }
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            const double av = t0 + u < width ? (double)a[u] : 0.0;
+#pragma unroll
+            for (int i = 0; i < NI; ++i)
+                acc[i] = __builtin_amdgcn_mfma_f64_4x4x4f64(av, bx[u][i], acc[i], 0, 0, 0);
+        }
+    }
+    if (R < 16 && blk >= 2) return;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int64_t o = cbase + (int64_t)k4 * R + 16 * i + g;
+        const double y = acc[i] + (yx ? pre_y[i] : 0.0);
+        const double t = c1 * y;
+        if (tout) tout[o] = t;
+        if (vout) vout[o] = fma(c2, t, pre_v[i]);
+    }
+}
+
 }  // namespace hymor

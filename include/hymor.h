@@ -216,9 +216,13 @@ int hymor_op_set_graphs(hymor_op *op, int enable);
  * 4 / 5 matrix cores with all operand loads of a large / small group of
  * slots issued together (k_spmm_mfma_w), 6 matrix cores with the batch
  * held in the operand ("tile") order, 16-byte accesses per lane
- * (k_spmm_tile, the default; it also serves 8 doubles per row), 7 the same
- * order on the 4x4x4 FP64 matrix cores with no idle rows at b = 8
- * (k_spmm_quad; k_spmm_tile at b = 4).  All accumulate in FP64. */
+ * (k_spmm_tile, the default; it also serves 8 doubles per row), 7 the
+ * 4x4x4 FP64 matrix cores with no idle rows: at b = 8 the tile order
+ * (k_spmm_quad), at b = 4 the row-major batch and the single-vector value
+ * order (k_spmm_quad4).  Under the default (6), 8 doubles per row at b = 4
+ * run the LDS-staged vector-FMA term kernel instead (k_term_lds: the
+ * matrix-core kernels' one-cell waves do too little work at b = 4).  All
+ * accumulate in FP64. */
 int hymor_op_set_batch_kernel(hymor_op *op, int use_mfma);
 
 /* Times the eight batch kernels at width nv (8, 16, 32 or 64) and keeps the

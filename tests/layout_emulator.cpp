@@ -541,3 +541,74 @@ extern "C" int emu_slice_vals_csc(int64_t n, const int64_t *colptr, const int64_
     }
     return bad + g_oob;
 }
+
+// k_spmm_quad4 (b = 4, row-major batch = tile order at b = 4) replayed on
+// the CPU with the measured v_mfma_f64_4x4x4f64 operand placement: lane
+// (n, blk, k) = n + 4 blk + 16 k supplies A[m = n][k] and X[k][16 i + g],
+// g = 4 blk + n (R = 8: blk & 1); D[blk][m][n] lands in lane n + 4 blk + 16 m.
+extern "C" int emu_spmm_quad4_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                                  const double *nzval, int64_t ncell, int nrhs, int R, int vpl,
+                                  const double *X_vm, double *Y_vm) {
+    g_oob = 0;
+    const int b = 4, CPS = 16, BB = 16;
+    HostBSR fwd, adj;
+    HostExtra xf, xa;
+    csc_to_blocks(n, colptr, rowval, nzval, 1, false, 1.0, b, ncell, fwd, adj, xf, xa);
+    if (!xf.rows.empty()) return -1;
+    HostSliced s;
+    bsr_to_sliced(fwd, s);
+    interleave_lanes(s, vpl);
+    std::vector<double> x((size_t)n * R, 0.0), y((size_t)n * R, 0.0);
+    for (int64_t row = 0; row < n; ++row)
+        for (int j = 0; j < nrhs; ++j) {
+            const int64_t vm = (row % b) * ncell + row / b;
+            x[(size_t)(row * R + j)] = X_vm[j * n + vm];
+        }
+    const int NI = R >= 16 ? R / 16 : 1;
+    for (int64_t cell = 0; cell < ncell; ++cell) {
+        const int64_t slice = cell / CPS, q = cell - slice * CPS;
+        const int64_t p0 = at(s.slice_ptr, slice);
+        const int64_t width = (at(s.slice_ptr, slice + 1) - p0) / CPS;
+        const int64_t cbase = cell * b * R;
+        double acc[4][64];
+        for (int i = 0; i < 4; ++i)
+            for (int l = 0; l < 64; ++l) acc[i][l] = 0.0;
+        for (int64_t t = 0; t < width; ++t) {
+            const int64_t c = at(s.col, p0 + t * CPS + q);
+            for (int i = 0; i < NI; ++i) {
+                double opa[64], opb[64];
+                for (int l = 0; l < 64; ++l) {
+                    const int n4 = l & 3, blk = (l >> 2) & 3, k4 = l >> 4;
+                    const int g = R >= 16 ? 4 * blk + n4 : 4 * (blk & 1) + n4;
+                    const int aoff = ((k4 / vpl) * 64 + (int)q * 4 + n4) * vpl + k4 % vpl;
+                    opa[l] = at(s.val, (p0 + t * CPS) * BB + aoff);
+                    opb[l] = at(x, c * b * R + (int64_t)k4 * R + 16 * i + g);
+                }
+                for (int blk = 0; blk < 4; ++blk)
+                    for (int m = 0; m < 4; ++m)
+                        for (int nn = 0; nn < 4; ++nn) {
+                            double sacc = 0.0;
+                            for (int k = 0; k < 4; ++k)
+                                sacc += opa[m + 4 * blk + 16 * k] * opb[nn + 4 * blk + 16 * k];
+                            acc[i][nn + 4 * blk + 16 * m] += sacc;
+                        }
+            }
+        }
+        for (int l = 0; l < 64; ++l) {
+            const int n4 = l & 3, blk = (l >> 2) & 3, k4 = l >> 4;
+            if (R < 16 && blk >= 2) continue;
+            const int g = R >= 16 ? 4 * blk + n4 : 4 * (blk & 1) + n4;
+            for (int i = 0; i < NI; ++i) {
+                const int64_t o = cbase + (int64_t)k4 * R + 16 * i + g;
+                if (o < 0 || o >= (int64_t)y.size()) { ++g_oob; continue; }
+                y[(size_t)o] = acc[i][l];
+            }
+        }
+    }
+    for (int64_t row = 0; row < n; ++row)
+        for (int j = 0; j < nrhs; ++j) {
+            const int64_t vm = (row % b) * ncell + row / b;
+            Y_vm[j * n + vm] = y[(size_t)(row * R + j)];
+        }
+    return g_oob;
+}

@@ -190,3 +190,27 @@ def test_emulated_device_value_layout(emu, b, vpl, transpose):
         colptr, rowval, nzval = op.to_csc_varmajor(one_based=True)
         assert emu.emu_slice_vals_csc(op.n, colptr.ctypes.data, rowval.ctypes.data,
                                       nzval.ctypes.data, b, op.ncell, vpl, transpose) == 0
+
+
+@pytest.mark.parametrize("R,nrhs,vpl", [(8, 8, 2), (8, 5, 4), (16, 16, 4), (16, 13, 2),
+                                        (32, 32, 2), (64, 40, 4)])
+@pytest.mark.parametrize("wide", [False, True])
+def test_emulated_batched_quad4(emu, R, nrhs, vpl, wide):
+    """4x4x4 matrix-core path at b = 4 (values read in the lane-interleaved
+    order, row-major batch, lane operand placement) replayed on the CPU with
+    every access checked, against scipy."""
+    emu.emu_spmm_quad4_csc.restype = C.c_int
+    emu.emu_spmm_quad4_csc.argtypes = [C.c_int64] + [C.c_void_p] * 3 + [
+        C.c_int64, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    if wide:
+        from test_gpu_parity import random_pattern_operator
+        op = random_pattern_operator(9, 7, 4, 19, seed=R)
+    else:
+        op = ops.stencil_operator(9, 7, 4, seed=12)
+    colptr, rowval, nzval = op.to_csc_varmajor(one_based=True)
+    X = np.asfortranarray(np.random.default_rng(4).standard_normal((op.n, nrhs)))
+    Y = np.zeros_like(X)
+    oob = emu.emu_spmm_quad4_csc(op.n, colptr.ctypes.data, rowval.ctypes.data, nzval.ctypes.data,
+                                 op.ncell, nrhs, R, vpl, X.ctypes.data, Y.ctypes.data)
+    assert oob == 0
+    assert np.allclose(Y, op.to_scipy_varmajor() @ X, rtol=1e-13, atol=1e-13)
