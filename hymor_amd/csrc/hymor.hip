@@ -57,8 +57,9 @@ struct TermTuning {
     int spmm_mfma = 6;  // batches of >= 16 doubles per row: 0 vector-FMA k_spmm,
                         // 1 k_spmm_mfma, 2/3 k_spmm_mfma_mc (2 or 1 cells per wave),
                         // 4/5 k_spmm_mfma_w (large / small load groups),
-                        // 6 k_spmm_tile (tile layout, also at 8 doubles per row),
-                        // 7 k_spmm_quad (4x4x4 matrix cores, b = 8; tile at b = 4)
+                        // 6 k_spmm_tile (tile layout, also at 8 doubles per row;
+                        // b = 4: k_term_lds at 8, k_spmm_quad4c at 16+),
+                        // 7 k_spmm_quad (4x4x4 matrix cores; b = 4: k_spmm_quad4)
 };
 
 struct DevSliced {
@@ -96,13 +97,6 @@ struct hymor_op {
     // workspace: [0] stage (var-major), [1] in0, [2] va, [3] vb, [4] t0, [5] t1, [6] spare
     int ws_nv = 0;
     double *ws[7] = {};
-    // partial sums of the extra rows feeding the block kernel, one buffer per
-    // direction: the forward and adjoint operators list different rows, and a
-    // row the current direction never writes must read as zero
-    double *yx = nullptr, *yxa = nullptr;
-    // the width and tile layout yx/yxa were last cleared for: k_extra writes
-    // only its listed rows, so a call in another layout starts from zeros
-    int yx_nv = -1, yx_tile = -1;
     int64_t dev_bytes = 0;
     std::vector<void *> allocs;
     // per-column complex diagonal on extra rows [cd0, cd1) for the current
@@ -273,8 +267,11 @@ void dfree(hymor_op *op, void *p, size_t bytes) {
 }
 
 // Rows of the extra CSR with at most this many entries go to k_extra_short
-// (lane per row and column); longer ones to k_extra (lane per entry).
+// (lanes per row, no reduction); longer ones to k_extra (a workgroup per
+// row, lanes per entry).
 constexpr int64_t kShortRow = 32;
+// waves per long extra row (k_extra: one workgroup per row)
+constexpr int kLongWaves = 4;
 
 // The listed rows reordered long rows first (each group keeps its order),
 // so the two kernels take contiguous ranges.
@@ -333,15 +330,7 @@ void ensure_ws_only(hymor_op *op, int nv);
 // width runs k_spmm_tile
 void ensure_ws(hymor_op *op, int nv) {
     ensure_ws_only(op, nv);
-    const int tb = tile_b_of(op, nv);
-    if (tb) ensure_tile(op);
-    if (op->yx_nv != nv || op->yx_tile != tb) {
-        const size_t yb = (size_t)op->nflow * op->ws_nv * sizeof(double);
-        HIP_TRY(hipMemsetAsync(op->yx, 0, yb, op->stream));
-        HIP_TRY(hipMemsetAsync(op->yxa, 0, yb, op->stream));
-        op->yx_nv = nv;
-        op->yx_tile = tb;
-    }
+    if (tile_b_of(op, nv)) ensure_tile(op);
 }
 
 void ensure_ws_only(hymor_op *op, int nv) {
@@ -358,34 +347,19 @@ void ensure_ws_only(hymor_op *op, int nv) {
             }
         op->ws[i] = nullptr;
     }
-    for (double **py : {&op->yx, &op->yxa}) {
-        if (!*py) continue;
-        for (size_t k = 0; k < op->allocs.size(); ++k)
-            if (op->allocs[k] == *py) {
-                (void)hipFree(*py);
-                op->allocs.erase(op->allocs.begin() + (long)k);
-                break;
-            }
-        *py = nullptr;
-    }
     const size_t vb = (size_t)op->n * nv * sizeof(double);
     for (int i = 0; i < 7; ++i) {
         op->ws[i] = (double *)dalloc(op, vb);
         HIP_TRY(hipMemsetAsync(op->ws[i], 0, vb, op->stream));
     }
-    const size_t yb = (size_t)op->nflow * nv * sizeof(double);
-    // rows without extra entries are never written: they must read as zero
-    op->yx = (double *)dalloc(op, yb);
-    HIP_TRY(hipMemsetAsync(op->yx, 0, yb, op->stream));
-    op->yxa = (double *)dalloc(op, yb);
-    HIP_TRY(hipMemsetAsync(op->yxa, 0, yb, op->stream));
     HIP_TRY(hipStreamSynchronize(op->stream));
     op->ws_nv = nv;
 }
 
-// Batched widths of 8+ doubles per row run k_spmm_tile when it is the
-// chosen batch kernel; their internal buffers then use the tile layout for
-// flow rows.  Returns the block size to pass as tile_b, or 0 (row-major).
+// Batched widths of 8+ doubles per row run the tile-layout kernels under
+// batch kernels 6 and 7; their internal buffers then use the tile layout for
+// flow rows (row-major itself at b = 4).  Returns the block size to pass as
+// tile_b, or 0 (row-major).
 int tile_b_of(const hymor_op *op, int nv) {
     return (nv >= 8 && op->fwd.tune.spmm_mfma >= 6) ? op->b : 0;
 }
@@ -425,8 +399,9 @@ void tile_vals_of(hymor_op *op, DevSliced &s) {
 // before any launch (or graph capture) of those kernels.
 void ensure_tile(hymor_op *op) {
     if (op->ncell == 0) return;
-    // k_spmm_quad4 (kernel 7 at b = 4) reads the lane-interleaved values
-    if (op->fwd.tune.spmm_mfma == 7 && op->b == 4) return;
+    // at b = 4 every batch kernel of 6 and 7 (k_term_lds, k_spmm_quad4c,
+    // k_spmm_quad4) reads the lane-interleaved values
+    if (op->b == 4) return;
     const bool quad = op->fwd.tune.spmm_mfma == 7 && op->b == 8;
     auto missing = [&](const DevSliced &s) { return quad ? !s.val_q : !s.val_t; };
     if (!missing(op->fwd) && !(op->has_adj && missing(op->adj))) return;
@@ -442,83 +417,83 @@ void ensure_tile(hymor_op *op) {
 // ---------------------------------------------------------------- launches
 
 template <int B, typename VT, int NV, int NW, bool NT>
-void launch_split(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+void launch_split(const DevSliced &s, hipStream_t st, const double *x,
                   double *tout, const double *vin, double *vout, double c1, double c2) {
     hipLaunchKernelGGL((k_term_split<B, VT, NV, NW, NT>), dim3((unsigned)s.nslice), dim3(64 * NW),
-                       0, st, s.slice_ptr, s.col, (const VT *)s.val, s.nslice, s.ncell, x, yx,
+                       0, st, s.slice_ptr, s.col, (const VT *)s.val, s.nslice, s.ncell, x,
                        tout, vin, vout, c1, c2);
 }
 
 // split == 2: k_term_split_p with NW = 3 or 9 waves per slice and slot
 // groups of S = 9 / NW, so one group per wave covers a 3x3-stencil row
 template <int B, typename VT, int NV, int NW, bool NT>
-void launch_split_p(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+void launch_split_p(const DevSliced &s, hipStream_t st, const double *x,
                     double *tout, const double *vin, double *vout, double c1, double c2) {
     constexpr int S = (9 + NW - 1) / NW;
     hipLaunchKernelGGL((k_term_split_p<B, VT, NV, NW, S, NT>), dim3((unsigned)s.nslice),
                        dim3(64 * NW), 0, st, s.slice_ptr, s.col, (const VT *)s.val, s.nslice,
-                       s.ncell, x, yx, tout, vin, vout, c1, c2);
+                       s.ncell, x, tout, vin, vout, c1, c2);
 }
 
 // split == 3: k_term_lds, as split 2 with the x blocks staged through LDS
 template <int B, typename VT, int NV, int NW, bool NT>
-void launch_lds(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+void launch_lds(const DevSliced &s, hipStream_t st, const double *x,
                 double *tout, const double *vin, double *vout, double c1, double c2) {
     constexpr int S = (9 + NW - 1) / NW;
     hipLaunchKernelGGL((k_term_lds<B, VT, NV, NW, S, NT>), dim3((unsigned)s.nslice),
                        dim3(64 * NW), 0, st, s.slice_ptr, s.col, (const VT *)s.val, s.nslice,
-                       s.ncell, x, yx, tout, vin, vout, c1, c2);
+                       s.ncell, x, tout, vin, vout, c1, c2);
 }
 
 template <int B, typename VT, int NV, bool NT>
-void launch_term_nt(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+void launch_term_nt(const DevSliced &s, hipStream_t st, const double *x,
                     double *tout, const double *vin, double *vout, double c1, double c2) {
     const TermTuning &tu = s.tune;
     if constexpr (NV >= 2 && (NV <= 4 || (B == 4 && NV == 8))) {
         if (tu.split == 3) {
             switch (tu.waves) {
-            case 1: launch_lds<B, VT, NV, 1, NT>(s, st, x, yx, tout, vin, vout, c1, c2); break;
-            case 2: launch_lds<B, VT, NV, 2, NT>(s, st, x, yx, tout, vin, vout, c1, c2); break;
-            case 9: launch_lds<B, VT, NV, 9, NT>(s, st, x, yx, tout, vin, vout, c1, c2); break;
-            default: launch_lds<B, VT, NV, 3, NT>(s, st, x, yx, tout, vin, vout, c1, c2); break;
+            case 1: launch_lds<B, VT, NV, 1, NT>(s, st, x, tout, vin, vout, c1, c2); break;
+            case 2: launch_lds<B, VT, NV, 2, NT>(s, st, x, tout, vin, vout, c1, c2); break;
+            case 9: launch_lds<B, VT, NV, 9, NT>(s, st, x, tout, vin, vout, c1, c2); break;
+            default: launch_lds<B, VT, NV, 3, NT>(s, st, x, tout, vin, vout, c1, c2); break;
             }
             return;
         }
     }
     if (tu.split >= 2) {
-        if (tu.waves == 9) launch_split_p<B, VT, NV, 9, NT>(s, st, x, yx, tout, vin, vout, c1, c2);
-        else launch_split_p<B, VT, NV, 3, NT>(s, st, x, yx, tout, vin, vout, c1, c2);
+        if (tu.waves == 9) launch_split_p<B, VT, NV, 9, NT>(s, st, x, tout, vin, vout, c1, c2);
+        else launch_split_p<B, VT, NV, 3, NT>(s, st, x, tout, vin, vout, c1, c2);
         return;
     }
     if (tu.split) {
-        if (tu.waves == 2) launch_split<B, VT, NV, 2, NT>(s, st, x, yx, tout, vin, vout, c1, c2);
-        else launch_split<B, VT, NV, 4, NT>(s, st, x, yx, tout, vin, vout, c1, c2);
+        if (tu.waves == 2) launch_split<B, VT, NV, 2, NT>(s, st, x, tout, vin, vout, c1, c2);
+        else launch_split<B, VT, NV, 4, NT>(s, st, x, tout, vin, vout, c1, c2);
         return;
     }
     // small operators: one wave per group so the slices spread over all CUs
     const int wpb = s.nslice >= 8192 ? tu.waves : 1;
     const int64_t grid = (s.nslice + wpb - 1) / wpb;
     hipLaunchKernelGGL((k_term<B, VT, NV, NT>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
-                       s.slice_ptr, s.col, (const VT *)s.val, s.nslice, s.ncell, x, yx, tout, vin,
+                       s.slice_ptr, s.col, (const VT *)s.val, s.nslice, s.ncell, x, tout, vin,
                        vout, c1, c2);
 }
 
 template <int B, typename VT, int NV>
-void launch_term_t(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+void launch_term_t(const DevSliced &s, hipStream_t st, const double *x,
                    double *tout, const double *vin, double *vout, double c1, double c2) {
-    if (s.tune.nt) launch_term_nt<B, VT, NV, true>(s, st, x, yx, tout, vin, vout, c1, c2);
-    else launch_term_nt<B, VT, NV, false>(s, st, x, yx, tout, vin, vout, c1, c2);
+    if (s.tune.nt) launch_term_nt<B, VT, NV, true>(s, st, x, tout, vin, vout, c1, c2);
+    else launch_term_nt<B, VT, NV, false>(s, st, x, tout, vin, vout, c1, c2);
 }
 
 template <int B, typename VT, int R>
-void launch_spmm_t(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+void launch_spmm_t(const DevSliced &s, hipStream_t st, const double *x,
                    double *tout, const double *vin, double *vout, double c1, double c2) {
     const int wpb = 4;
     const int64_t grid = (s.ncell + wpb - 1) / wpb;
     if constexpr (R % 16 == 0) {
         if (s.tune.spmm_mfma == 1) {
             hipLaunchKernelGGL((k_spmm_mfma<B, VT, R>), dim3((unsigned)grid), dim3(64 * wpb), 0,
-                               st, s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, yx, tout,
+                               st, s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, tout,
                                vin, vout, c1, c2);
             return;
         }
@@ -531,11 +506,11 @@ void launch_spmm_t(const DevSliced &s, hipStream_t st, const double *x, const do
             if (s.tune.spmm_mfma == 4)
                 hipLaunchKernelGGL((k_spmm_mfma_w<B, VT, R, W4>), dim3((unsigned)grid),
                                    dim3(64 * wpb), 0, st, s.slice_ptr, s.col, (const VT *)s.val,
-                                   s.ncell, x, yx, tout, vin, vout, c1, c2);
+                                   s.ncell, x, tout, vin, vout, c1, c2);
             else
                 hipLaunchKernelGGL((k_spmm_mfma_w<B, VT, R, W5>), dim3((unsigned)grid),
                                    dim3(64 * wpb), 0, st, s.slice_ptr, s.col, (const VT *)s.val,
-                                   s.ncell, x, yx, tout, vin, vout, c1, c2);
+                                   s.ncell, x, tout, vin, vout, c1, c2);
             return;
         }
         if (s.tune.spmm_mfma >= 2) {
@@ -546,25 +521,25 @@ void launch_spmm_t(const DevSliced &s, hipStream_t st, const double *x, const do
                 const int64_t waves = (s.ncell + 1) / 2;
                 hipLaunchKernelGGL((k_spmm_mfma_mc<B, VT, R, U, 2>),
                                    dim3((unsigned)((waves + wpb - 1) / wpb)), dim3(64 * wpb), 0,
-                                   st, s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, yx,
+                                   st, s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x,
                                    tout, vin, vout, c1, c2);
             } else {
                 hipLaunchKernelGGL((k_spmm_mfma_mc<B, VT, R, U, 1>), dim3((unsigned)grid),
                                    dim3(64 * wpb), 0, st, s.slice_ptr, s.col, (const VT *)s.val,
-                                   s.ncell, x, yx, tout, vin, vout, c1, c2);
+                                   s.ncell, x, tout, vin, vout, c1, c2);
             }
             return;
         }
     }
     hipLaunchKernelGGL((k_spmm<B, VT, R>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
-                       s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, yx, tout, vin, vout, c1,
+                       s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, tout, vin, vout, c1,
                        c2);
 }
 
 // Kernel choice per (block size, row width): lane-per-row k_term up to 4
 // doubles per row (8 for b = 4), the rhs-parallel k_spmm above.
 template <int B, typename VT, int R>
-void launch_tile_t(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+void launch_tile_t(const DevSliced &s, hipStream_t st, const double *x,
                    double *tout, const double *vin, double *vout, double c1, double c2) {
     if (!s.val_t) throw ArgError("tile-ordered values missing (internal)");
     // slot groups: the 3x3 stencil's 9 slots in one group up to width 16,
@@ -573,51 +548,72 @@ void launch_tile_t(const DevSliced &s, hipStream_t st, const double *x, const do
     const int wpb = 4;
     const int64_t grid = (s.ncell + wpb - 1) / wpb;
     hipLaunchKernelGGL((k_spmm_tile<B, VT, R, W>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
-                       s.slice_ptr, s.col, (const VT *)s.val_t, s.ncell, x, yx, tout, vin, vout,
+                       s.slice_ptr, s.col, (const VT *)s.val_t, s.ncell, x, tout, vin, vout,
                        c1, c2);
 }
 
 template <typename VT, int R>
-void launch_quad_t(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+void launch_quad_t(const DevSliced &s, hipStream_t st, const double *x,
                    double *tout, const double *vin, double *vout, double c1, double c2) {
     if (!s.val_q) throw ArgError("quad-ordered values missing (internal)");
     constexpr int W = R <= 16 ? 9 : (R == 32 ? 5 : 3);
     const int wpb = 4;
     const int64_t grid = (s.ncell + wpb - 1) / wpb;
     hipLaunchKernelGGL((k_spmm_quad<VT, R, W>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
-                       s.slice_ptr, s.col, (const VT *)s.val_q, s.ncell, x, yx, tout, vin, vout,
+                       s.slice_ptr, s.col, (const VT *)s.val_q, s.ncell, x, tout, vin, vout,
                        c1, c2);
 }
 
 template <typename VT, int R>
-void launch_quad4_t(const DevSliced &s, hipStream_t st, const double *x, const double *yx,
+void launch_quad4_t(const DevSliced &s, hipStream_t st, const double *x,
                     double *tout, const double *vin, double *vout, double c1, double c2) {
     constexpr int W = R <= 32 ? 9 : 5;
     const int wpb = 4;
     const int64_t grid = (s.ncell + wpb - 1) / wpb;
     hipLaunchKernelGGL((k_spmm_quad4<VT, R, W>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
-                       s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, yx, tout, vin, vout,
+                       s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, tout, vin, vout,
+                       c1, c2);
+}
+
+template <typename VT, int R, int W>
+void launch_quad4c_t(const DevSliced &s, hipStream_t st, const double *x,
+                     double *tout, const double *vin, double *vout, double c1, double c2) {
+    const int wpb = 4;
+    const int64_t ngrp = (s.ncell + 3) / 4;
+    const int64_t grid = (ngrp + wpb - 1) / wpb;
+    hipLaunchKernelGGL((k_spmm_quad4c<VT, R, W>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
+                       s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, tout, vin, vout,
                        c1, c2);
 }
 
 template <int B, typename VT>
 void launch_block_part(const DevSliced &s, int nv, hipStream_t st, const double *x,
-                       const double *yx, double *tout, const double *vin, double *vout, double c1,
+                       double *tout, const double *vin, double *vout, double c1,
                        double c2) {
     if constexpr (B == 4) {
         // 8 doubles per row at b = 4: the LDS-staged vector-FMA term kernel
         // beats the matrix-core ones (whose one-cell waves do too little
         // work at b = 4); the batch order is row-major at b = 4 either way
         if (nv == 8 && s.tune.spmm_mfma == 6) {
-            launch_term_t<B, VT, 8>(s, st, x, yx, tout, vin, vout, c1, c2);
+            launch_term_t<B, VT, 8>(s, st, x, tout, vin, vout, c1, c2);
             return;
+        }
+        // 16+ doubles per row: four cells per wave on the 4x4x4 matrix
+        // cores, slot groups sized to the registers the x chunks take
+        if (nv >= 16 && s.tune.spmm_mfma == 6) {
+            switch (nv) {
+            case 16: launch_quad4c_t<VT, 16, 9>(s, st, x, tout, vin, vout, c1, c2); return;
+            case 32: launch_quad4c_t<VT, 32, 5>(s, st, x, tout, vin, vout, c1, c2); return;
+            case 64: launch_quad4c_t<VT, 64, 2>(s, st, x, tout, vin, vout, c1, c2); return;
+            default: throw ArgError("unsupported vector width");
+            }
         }
         if (nv >= 8 && s.tune.spmm_mfma == 7) {
             switch (nv) {
-            case 8: launch_quad4_t<VT, 8>(s, st, x, yx, tout, vin, vout, c1, c2); return;
-            case 16: launch_quad4_t<VT, 16>(s, st, x, yx, tout, vin, vout, c1, c2); return;
-            case 32: launch_quad4_t<VT, 32>(s, st, x, yx, tout, vin, vout, c1, c2); return;
-            case 64: launch_quad4_t<VT, 64>(s, st, x, yx, tout, vin, vout, c1, c2); return;
+            case 8: launch_quad4_t<VT, 8>(s, st, x, tout, vin, vout, c1, c2); return;
+            case 16: launch_quad4_t<VT, 16>(s, st, x, tout, vin, vout, c1, c2); return;
+            case 32: launch_quad4_t<VT, 32>(s, st, x, tout, vin, vout, c1, c2); return;
+            case 64: launch_quad4_t<VT, 64>(s, st, x, tout, vin, vout, c1, c2); return;
             default: throw ArgError("unsupported vector width");
             }
         }
@@ -625,47 +621,50 @@ void launch_block_part(const DevSliced &s, int nv, hipStream_t st, const double 
     if constexpr (B == 8) {
         if (nv >= 8 && s.tune.spmm_mfma == 7) {
             switch (nv) {
-            case 8: launch_quad_t<VT, 8>(s, st, x, yx, tout, vin, vout, c1, c2); return;
-            case 16: launch_quad_t<VT, 16>(s, st, x, yx, tout, vin, vout, c1, c2); return;
-            case 32: launch_quad_t<VT, 32>(s, st, x, yx, tout, vin, vout, c1, c2); return;
-            case 64: launch_quad_t<VT, 64>(s, st, x, yx, tout, vin, vout, c1, c2); return;
+            case 8: launch_quad_t<VT, 8>(s, st, x, tout, vin, vout, c1, c2); return;
+            case 16: launch_quad_t<VT, 16>(s, st, x, tout, vin, vout, c1, c2); return;
+            case 32: launch_quad_t<VT, 32>(s, st, x, tout, vin, vout, c1, c2); return;
+            case 64: launch_quad_t<VT, 64>(s, st, x, tout, vin, vout, c1, c2); return;
             default: throw ArgError("unsupported vector width");
             }
         }
     }
-    if (nv >= 8 && s.tune.spmm_mfma >= 6) {
-        switch (nv) {
-        case 8: launch_tile_t<B, VT, 8>(s, st, x, yx, tout, vin, vout, c1, c2); return;
-        case 16: launch_tile_t<B, VT, 16>(s, st, x, yx, tout, vin, vout, c1, c2); return;
-        case 32: launch_tile_t<B, VT, 32>(s, st, x, yx, tout, vin, vout, c1, c2); return;
-        case 64: launch_tile_t<B, VT, 64>(s, st, x, yx, tout, vin, vout, c1, c2); return;
-        default: throw ArgError("unsupported vector width");
+    if constexpr (B == 8) {
+        // (b = 4 returned above for every batch of 8+ under kernels 6 and 7)
+        if (nv >= 8 && s.tune.spmm_mfma >= 6) {
+            switch (nv) {
+            case 8: launch_tile_t<B, VT, 8>(s, st, x, tout, vin, vout, c1, c2); return;
+            case 16: launch_tile_t<B, VT, 16>(s, st, x, tout, vin, vout, c1, c2); return;
+            case 32: launch_tile_t<B, VT, 32>(s, st, x, tout, vin, vout, c1, c2); return;
+            case 64: launch_tile_t<B, VT, 64>(s, st, x, tout, vin, vout, c1, c2); return;
+            default: throw ArgError("unsupported vector width");
+            }
         }
     }
     switch (nv) {
-    case 1: launch_term_t<B, VT, 1>(s, st, x, yx, tout, vin, vout, c1, c2); break;
-    case 2: launch_term_t<B, VT, 2>(s, st, x, yx, tout, vin, vout, c1, c2); break;
-    case 4: launch_term_t<B, VT, 4>(s, st, x, yx, tout, vin, vout, c1, c2); break;
+    case 1: launch_term_t<B, VT, 1>(s, st, x, tout, vin, vout, c1, c2); break;
+    case 2: launch_term_t<B, VT, 2>(s, st, x, tout, vin, vout, c1, c2); break;
+    case 4: launch_term_t<B, VT, 4>(s, st, x, tout, vin, vout, c1, c2); break;
     case 8:
-        if constexpr (B == 8) launch_spmm_t<B, VT, 8>(s, st, x, yx, tout, vin, vout, c1, c2);
-        else launch_term_t<B, VT, 8>(s, st, x, yx, tout, vin, vout, c1, c2);
+        if constexpr (B == 8) launch_spmm_t<B, VT, 8>(s, st, x, tout, vin, vout, c1, c2);
+        else launch_term_t<B, VT, 8>(s, st, x, tout, vin, vout, c1, c2);
         break;
-    case 16: launch_spmm_t<B, VT, 16>(s, st, x, yx, tout, vin, vout, c1, c2); break;
-    case 32: launch_spmm_t<B, VT, 32>(s, st, x, yx, tout, vin, vout, c1, c2); break;
-    case 64: launch_spmm_t<B, VT, 64>(s, st, x, yx, tout, vin, vout, c1, c2); break;
+    case 16: launch_spmm_t<B, VT, 16>(s, st, x, tout, vin, vout, c1, c2); break;
+    case 32: launch_spmm_t<B, VT, 32>(s, st, x, tout, vin, vout, c1, c2); break;
+    case 64: launch_spmm_t<B, VT, 64>(s, st, x, tout, vin, vout, c1, c2); break;
     default: throw ArgError("unsupported vector width");
     }
 }
 
-void launch_block(const DevSliced &s, int nv, hipStream_t st, const double *x, const double *yx,
+void launch_block(const DevSliced &s, int nv, hipStream_t st, const double *x,
                   double *tout, const double *vin, double *vout, double c1, double c2) {
     const bool f32 = s.prec == HYMOR_PREC_F32;
     if (s.b == 4) {
-        if (f32) launch_block_part<4, float>(s, nv, st, x, yx, tout, vin, vout, c1, c2);
-        else launch_block_part<4, double>(s, nv, st, x, yx, tout, vin, vout, c1, c2);
+        if (f32) launch_block_part<4, float>(s, nv, st, x, tout, vin, vout, c1, c2);
+        else launch_block_part<4, double>(s, nv, st, x, tout, vin, vout, c1, c2);
     } else if (s.b == 8) {
-        if (f32) launch_block_part<8, float>(s, nv, st, x, yx, tout, vin, vout, c1, c2);
-        else launch_block_part<8, double>(s, nv, st, x, yx, tout, vin, vout, c1, c2);
+        if (f32) launch_block_part<8, float>(s, nv, st, x, tout, vin, vout, c1, c2);
+        else launch_block_part<8, double>(s, nv, st, x, tout, vin, vout, c1, c2);
     } else {
         throw ArgError("vars_per_cell must be 4 or 8");
     }
@@ -678,19 +677,25 @@ struct ColDiag {
 
 template <int NV>
 void launch_extra_nv(const DevExtra &xe, int64_t nflow, hipStream_t st, const double *x,
-                     double *yx, double *tout, const double *vin, double *vout, double c1,
+                     double *tout, const double *vin, double *vout, double c1,
                      double c2, const ColDiag &cd, int tile_b) {
     const int wpb = 4;
     if (xe.nlong > 0) {
-        const int64_t grid = (xe.nlong + wpb - 1) / wpb;
-        if (xe.is_complex)
-            hipLaunchKernelGGL((k_extra<NV, true>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
-                               xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, nflow, x, yx, tout, vin,
-                               vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
-        else
-            hipLaunchKernelGGL((k_extra<NV, false>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
-                               xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, nflow, x, yx, tout, vin,
-                               vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
+        // one workgroup of kLongWaves waves per long row
+        constexpr int NW = kLongWaves;
+        if (xe.is_complex) {
+            if constexpr (NV >= 2)
+                hipLaunchKernelGGL((k_extra<NV, true, NW>), dim3((unsigned)xe.nlong),
+                                   dim3(64 * NW), 0, st, xe.rows, xe.ptr, xe.col, xe.val,
+                                   xe.nlong, nflow, x, tout, vin, vout, c1, c2, cd.d, cd.r0,
+                                   cd.r1, tile_b);
+            else
+                throw ArgError("complex extras need two doubles per row");
+        } else {
+            hipLaunchKernelGGL((k_extra<NV, false, NW>), dim3((unsigned)xe.nlong), dim3(64 * NW),
+                               0, st, xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, nflow, x,
+                               tout, vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
+        }
     }
     const int64_t nshort = xe.nrows - xe.nlong;
     if (nshort > 0) {
@@ -704,25 +709,25 @@ void launch_extra_nv(const DevExtra &xe, int64_t nflow, hipStream_t st, const do
         if (xe.is_complex)
             hipLaunchKernelGGL((k_extra_short<NV, true>), dim3((unsigned)grid), dim3(64 * wpb),
                                0, st, xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, xe.nrows, nflow,
-                               x, yx, tout, vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
+                               x, tout, vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
         else
             hipLaunchKernelGGL((k_extra_short<NV, false>), dim3((unsigned)grid), dim3(64 * wpb),
                                0, st, xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, xe.nrows, nflow,
-                               x, yx, tout, vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
+                               x, tout, vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
     }
 }
 
 void launch_extra(const DevExtra &xe, int nv, int64_t nflow, hipStream_t st, const double *x,
-                  double *yx, double *tout, const double *vin, double *vout, double c1,
+                  double *tout, const double *vin, double *vout, double c1,
                   double c2, const ColDiag &cd, int tile_b) {
     switch (nv) {
-    case 1: launch_extra_nv<1>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd, tile_b); break;
-    case 2: launch_extra_nv<2>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd, tile_b); break;
-    case 4: launch_extra_nv<4>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd, tile_b); break;
-    case 8: launch_extra_nv<8>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd, tile_b); break;
-    case 16: launch_extra_nv<16>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd, tile_b); break;
-    case 32: launch_extra_nv<32>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd, tile_b); break;
-    case 64: launch_extra_nv<64>(xe, nflow, st, x, yx, tout, vin, vout, c1, c2, cd, tile_b); break;
+    case 1: launch_extra_nv<1>(xe, nflow, st, x, tout, vin, vout, c1, c2, cd, tile_b); break;
+    case 2: launch_extra_nv<2>(xe, nflow, st, x, tout, vin, vout, c1, c2, cd, tile_b); break;
+    case 4: launch_extra_nv<4>(xe, nflow, st, x, tout, vin, vout, c1, c2, cd, tile_b); break;
+    case 8: launch_extra_nv<8>(xe, nflow, st, x, tout, vin, vout, c1, c2, cd, tile_b); break;
+    case 16: launch_extra_nv<16>(xe, nflow, st, x, tout, vin, vout, c1, c2, cd, tile_b); break;
+    case 32: launch_extra_nv<32>(xe, nflow, st, x, tout, vin, vout, c1, c2, cd, tile_b); break;
+    case 64: launch_extra_nv<64>(xe, nflow, st, x, tout, vin, vout, c1, c2, cd, tile_b); break;
     default: throw ArgError("unsupported vector width");
     }
 }
@@ -733,17 +738,17 @@ void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, const double *x
                  const double *vin, double *vout, double c1, double c2) {
     const DevSliced &s = adj ? op->adj : op->fwd;
     const DevExtra &xe = adj ? op->xa : op->xf;
-    double *yx = adj ? op->yxa : op->yx;
-    const bool has_x = xe.nrows > 0;
-    if (has_x) {
-        ColDiag cd;
-        if (op->cdiag_on) cd = ColDiag{adj ? op->cdiag_adj : op->cdiag, op->cd0, op->cd1};
-        launch_extra(xe, nv, op->nflow, st, x, yx, tout, vin, vout, c1, c2, cd,
-                     tile_b_of(op, nv));
+    if (op->ncell > 0) {
+        launch_block(s, nv, st, x, tout, vin, vout, c1, c2);
         HIP_TRY(hipGetLastError());
     }
-    if (op->ncell > 0) {
-        launch_block(s, nv, st, x, has_x ? yx : nullptr, tout, vin, vout, c1, c2);
+    // then the extra rows: extra-DOF rows in full, flow rows adding their
+    // extra entries' part to what the block kernel wrote
+    if (xe.nrows > 0) {
+        ColDiag cd;
+        if (op->cdiag_on) cd = ColDiag{adj ? op->cdiag_adj : op->cdiag, op->cd0, op->cd1};
+        launch_extra(xe, nv, op->nflow, st, x, tout, vin, vout, c1, c2, cd,
+                     tile_b_of(op, nv));
         HIP_TRY(hipGetLastError());
     }
 }
@@ -1466,7 +1471,6 @@ int hymor_op_autotune_batch(hymor_op *op, int nv, int reps, int *best_kernel, do
         op->adj.tune.spmm_mfma = pick;
         drop_graph(op);
         *best_kernel = pick;
-        op->yx_nv = -1;   // launches above wrote yx in several layouts
         *best_ms = best;
     });
 }

@@ -112,8 +112,9 @@ __device__ __forceinline__ void load_lane_row(const VT *__restrict__ base, int l
         load_vals<V, NT>(base + ((int64_t)kc * 64 + lane) * V, a + kc * V);
 }
 
-// One Taylor term over the block part:  y = A*x (+ yx from the extra CSR),
-// t = c1*y, tout = t (if tout), vout = vin + c2*t (if vout).
+// One Taylor term over the block part:  y = A*x (the extra CSR rows add
+// theirs afterwards, k_extra / k_extra_short), t = c1*y, tout = t (if
+// tout), vout = vin + c2*t (if vout).
 // c1/c2 select the recursion: normalised (c1 = dt/j, c2 = 1) or powers
 // (c1 = dt, c2 = 1/j!).  With tout = y and vout = null this is a plain SpMV.
 // vin may equal vout (in-place accumulation of the own row) or x (first term
@@ -122,7 +123,7 @@ template <int B, typename VT, int NV, bool NT = false>
 __global__ __launch_bounds__(256) void k_term(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ val, int64_t nslice, int64_t ncell,
-    const double *__restrict__ x, const double *__restrict__ yx,
+    const double *__restrict__ x,
     double *__restrict__ tout, const double *vin, double *vout, double c1, double c2) {
     constexpr int CPS = 64 / B;
     constexpr int BB = B * B;
@@ -156,78 +157,105 @@ __global__ __launch_bounds__(256) void k_term(
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         double y = acc[v];
-        if (yx) y += yx[row * NV + v];
         const double t = c1 * y;
         if (tout) tout[row * NV + v] = t;
         if (vout) vout[row * NV + v] = fma(c2, t, vin[row * NV + v]);
     }
 }
 
-// Extra CSR rows, one wavefront per listed row.  Flow rows (row < nflow)
-// leave their partial sum in yx for k_term; extra-DOF rows finish the term
-// here.  CPLX: values are complex (re,im) and NV counts doubles of complex
-// vectors ((re,im) pairs); otherwise values are real and every one of the
-// NV components is an independent right-hand side.
+// Extra CSR rows, NW waves (one workgroup) per listed row.  Flow rows
+// (row < nflow) add their extra entries' part to the block kernel's output
+// (which runs first); extra-DOF rows finish the term here.  CPLX: values
+// are complex (re,im) and NV counts doubles of complex vectors ((re,im)
+// pairs); otherwise values are real and every one of the NV components is
+// an independent right-hand side.
+// A lane owns NC doubles of the row (one complex pair when CPLX): LPR lanes
+// cover the row's NV doubles, so the LPR lanes of an entry read its x row as
+// adjacent doubles, and a wave takes EPW = 64 / LPR entries per step, the
+// workgroup NW * EPW.  Partial sums meet by lane exchange inside each wave,
+// then through LDS in wave order.
 // cdiag (optional): a complex diagonal that differs per column (per pair
 // (re, im) of the NV doubles), added on rows [cd0, cd1) -- the -i*omega_k
 // block of the factored receptivity systems, one wavenumber per column.
-template <int NV, bool CPLX>
-__global__ __launch_bounds__(256) void k_extra(
+template <int NV, bool CPLX, int NW>
+__global__ __launch_bounds__(64 * NW) void k_extra(
     const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
     const int32_t *__restrict__ col, const double *__restrict__ val, int64_t nrows,
-    int64_t nflow, const double *__restrict__ x, double *__restrict__ yx,
+    int64_t nflow, const double *__restrict__ x,
     double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
     const double *__restrict__ cdiag, int64_t cd0, int64_t cd1, int tile_b) {
-    const int lane = threadIdx.x & 63;
-    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (w >= nrows) return;
-    double acc[NV];
+    constexpr int LPR = CPLX ? NV / 2 : NV;    // lanes per entry
+    constexpr int EPW = 64 / LPR;              // entries per wave and step
+    constexpr int NC = CPLX ? 2 : 1;           // doubles per lane
+    static_assert(!CPLX || NV >= 2, "complex rows need two doubles");
+    static_assert(64 % LPR == 0, "lanes per entry divide the wave");
+    __shared__ double part[NW > 1 ? NW - 1 : 1][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t ri = blockIdx.x;
+    if (ri >= nrows) return;
+    const int j = lane / LPR;                  // entry of the step
+    const int v0 = (lane % LPR) * NC;          // first of this lane's doubles
+    double acc[NC];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-    const int64_t e1 = ptr[w + 1];
-    for (int64_t e = ptr[w] + lane; e < e1; e += 64) {
+    for (int q = 0; q < NC; ++q) acc[q] = 0.0;
+    const int64_t e1 = ptr[ri + 1];
+#pragma unroll 4
+    for (int64_t e = ptr[ri] + wv * EPW + j; e < e1; e += NW * EPW) {
         const int64_t c = col[e];
         if constexpr (CPLX) {
             const double ar = val[2 * e], ai = val[2 * e + 1];
-#pragma unroll
-            for (int v = 0; v < NV; v += 2) {
-                const double xr = x[buf_pos(c, v, NV, tile_b, nflow)];
-                const double xi = x[buf_pos(c, v + 1, NV, tile_b, nflow)];
-                acc[v] = fma(ar, xr, fma(-ai, xi, acc[v]));
-                acc[v + 1] = fma(ar, xi, fma(ai, xr, acc[v + 1]));
-            }
+            const double xr = x[buf_pos(c, v0, NV, tile_b, nflow)];
+            const double xi = x[buf_pos(c, v0 + 1, NV, tile_b, nflow)];
+            acc[0] = fma(ar, xr, fma(-ai, xi, acc[0]));
+            acc[1] = fma(ar, xi, fma(ai, xr, acc[1]));
         } else {
-            const double a = val[e];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) acc[v] = fma(a, x[buf_pos(c, v, NV, tile_b, nflow)], acc[v]);
+            acc[0] = fma(val[e], x[buf_pos(c, v0, NV, tile_b, nflow)], acc[0]);
         }
     }
 #pragma unroll
-    for (int v = 0; v < NV; ++v)
+    for (int q = 0; q < NC; ++q)
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) acc[v] += __shfl_xor(acc[v], off, 64);
-    if (lane == 0) {
-        const int64_t row = rows[w];
-        if constexpr (NV % 2 == 0) {
-            if (cdiag && row >= cd0 && row < cd1) {
+        for (int off = 32; off >= LPR; off >>= 1) acc[q] += __shfl_xor(acc[q], off, 64);
+    if constexpr (NW > 1) {
+        if (wv > 0 && j == 0) {
 #pragma unroll
-                for (int v = 0; v < NV; v += 2) {
-                    const double dr = cdiag[v], di = cdiag[v + 1];
-                    const double xr = x[row * NV + v], xi = x[row * NV + v + 1];
-                    acc[v] = fma(dr, xr, fma(-di, xi, acc[v]));
-                    acc[v + 1] = fma(dr, xi, fma(di, xr, acc[v + 1]));
-                }
+            for (int q = 0; q < NC; ++q) part[wv - 1][v0 + q] = acc[q];
+        }
+        __syncthreads();
+        if (wv > 0) return;
+#pragma unroll
+        for (int i = 0; i < NW - 1; ++i)
+#pragma unroll
+            for (int q = 0; q < NC; ++q) acc[q] += part[i][v0 + q];
+    }
+    if (j != 0) return;
+    const int64_t row = rows[ri];
+    if constexpr (NV % 2 == 0) {
+        if (cdiag && row >= cd0 && row < cd1) {
+#pragma unroll
+            for (int q = 0; q < NC; ++q) {
+                // the pair (re, im) this double belongs to
+                const int v = v0 + q, vr = v & ~1;
+                const double dr = cdiag[vr], di = cdiag[vr + 1];
+                const double xr = x[row * NV + vr], xi = x[row * NV + vr + 1];
+                acc[q] = (v & 1) ? fma(dr, xi, fma(di, xr, acc[q]))
+                                 : fma(dr, xr, fma(-di, xi, acc[q]));
             }
         }
+    }
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            if (row < nflow) {
-                yx[buf_pos(row, v, NV, tile_b, nflow)] = acc[v];
-            } else {
-                const double t = c1 * acc[v];
-                if (tout) tout[row * NV + v] = t;
-                if (vout) vout[row * NV + v] = fma(c2, t, vin[row * NV + v]);
-            }
+    for (int q = 0; q < NC; ++q) {
+        const int v = v0 + q;
+        if (row < nflow) {
+            // flow row: the block kernel has written its part already
+            const int64_t o = buf_pos(row, v, NV, tile_b, nflow);
+            const double t = c1 * acc[q];
+            if (tout) tout[o] += t;
+            if (vout) vout[o] = fma(c2, t, vout[o]);
+        } else {
+            const double t = c1 * acc[q];
+            if (tout) tout[row * NV + v] = t;
+            if (vout) vout[row * NV + v] = fma(c2, t, vin[row * NV + v]);
         }
     }
 }
@@ -241,7 +269,7 @@ template <int NV, bool CPLX>
 __global__ __launch_bounds__(256) void k_extra_short(
     const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
     const int32_t *__restrict__ col, const double *__restrict__ val, int64_t r0, int64_t nrows,
-    int64_t nflow, const double *__restrict__ x, double *__restrict__ yx,
+    int64_t nflow, const double *__restrict__ x,
     double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
     const double *__restrict__ cdiag, int64_t cd0, int64_t cd1, int tile_b) {
     constexpr int LPR = CPLX ? (NV >= 2 ? NV / 2 : 1) : NV;   // lanes per row
@@ -282,7 +310,11 @@ __global__ __launch_bounds__(256) void k_extra_short(
     for (int q = 0; q < NC; ++q) {
         const int v = v0 + q;
         if (row < nflow) {
-            yx[buf_pos(row, v, NV, tile_b, nflow)] = acc[q];
+            // flow row: the block kernel has written its part already
+            const int64_t o = buf_pos(row, v, NV, tile_b, nflow);
+            const double t = c1 * acc[q];
+            if (tout) tout[o] += t;
+            if (vout) vout[o] = fma(c2, t, vout[o]);
         } else {
             const double t = c1 * acc[q];
             if (tout) tout[row * NV + v] = t;
@@ -479,7 +511,7 @@ template <int B, typename VT, int R>
 __global__ __launch_bounds__(256) void k_spmm(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ val, int64_t ncell, const double *__restrict__ x,
-    const double *__restrict__ yx, double *__restrict__ tout, const double *vin, double *vout,
+    double *__restrict__ tout, const double *vin, double *vout,
     double c1, double c2) {
     constexpr int CPS = 64 / B;
     constexpr int BB = B * B;
@@ -518,7 +550,6 @@ __global__ __launch_bounds__(256) void k_spmm(
     for (int i = 0; i < RPL; ++i) {
         const int64_t idx = (cell * B + g * RPL + i) * R + r;
         double y = acc[i];
-        if (yx) y += yx[idx];
         const double t = c1 * y;
         if (tout) tout[idx] = t;
         if (vout) vout[idx] = fma(c2, t, vin[idx]);
@@ -671,7 +702,7 @@ template <int B, typename VT, int NV, int NW, bool NT>
 __global__ __launch_bounds__(64 * NW) void k_term_split(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ val, int64_t nslice, int64_t ncell,
-    const double *__restrict__ x, const double *__restrict__ yx,
+    const double *__restrict__ x,
     double *__restrict__ tout, const double *vin, double *vout, double c1, double c2) {
     constexpr int CPS = 64 / B;
     constexpr int BB = B * B;
@@ -718,7 +749,6 @@ __global__ __launch_bounds__(64 * NW) void k_term_split(
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         double y = acc[v];
-        if (yx) y += yx[row * NV + v];
         const double t = c1 * y;
         if (tout) tout[row * NV + v] = t;
         if (vout) vout[row * NV + v] = fma(c2, t, vin[row * NV + v]);
@@ -727,7 +757,7 @@ __global__ __launch_bounds__(64 * NW) void k_term_split(
 
 // k_term_split with the loads of each wave grouped: the block columns of
 // the wave's next S slots and their values are issued together, then the x
-// entries they select, and wave 0 issues its epilogue operands (vin, yx)
+// entries they select, and wave 0 issues its epilogue operand (vin)
 // before the slot loop -- two waits on memory per group instead of two per
 // slot pair plus one at the end.  With NW * S = 9 a 3x3-stencil row has no
 // idle slot; an idle slot (shorter or longer rows) contributes exact zeros.
@@ -747,7 +777,7 @@ template <int B, typename VT, int NV, int NW, int S, bool NT>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_term_split_p(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ val, int64_t nslice, int64_t ncell,
-    const double *__restrict__ x, const double *__restrict__ yx,
+    const double *__restrict__ x,
     double *__restrict__ tout, const double *vin, double *vout, double c1, double c2) {
     constexpr int CPS = 64 / B;
     constexpr int BB = B * B;
@@ -766,15 +796,11 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
     const VT *vp = val + p0 * BB;
     const int64_t cell = slice * CPS + q;
     const int64_t row = (cell < ncell ? cell : ncell - 1) * B + r;
-    double pre_v[NV], pre_y[NV];
+    double pre_v[NV];
     {
         const double *pv = vout ? vin : x;
-        const double *py = yx ? yx : x;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            pre_v[v] = w == 0 ? pv[row * NV + v] : 0.0;
-            pre_y[v] = w == 0 ? py[row * NV + v] : 0.0;
-        }
+        for (int v = 0; v < NV; ++v) pre_v[v] = w == 0 ? pv[row * NV + v] : 0.0;
     }
     double acc[NV];
 #pragma unroll
@@ -834,13 +860,15 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
             for (int v = 0; v < NV; ++v) acc[v] += part[i][v][lane];
     }
     if (cell >= ncell) return;
+    // the vout stores first, behind one wait for the epilogue operands (a
+    // store between two such waits would be waited for as well)
+    if (vout) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        double y = acc[v];
-        if (yx) y += pre_y[v];
-        const double t = c1 * y;
-        if (tout) tout[row * NV + v] = t;
-        if (vout) vout[row * NV + v] = fma(c2, t, pre_v[v]);
+        for (int v = 0; v < NV; ++v) vout[row * NV + v] = fma(c2, c1 * acc[v], pre_v[v]);
+    }
+    if (tout) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) tout[row * NV + v] = c1 * acc[v];
     }
 }
 
@@ -866,7 +894,7 @@ template <int B, typename VT, int NV, int NW, int S, bool NT>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_term_lds(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ val, int64_t nslice, int64_t ncell,
-    const double *__restrict__ x, const double *__restrict__ yx,
+    const double *__restrict__ x,
     double *__restrict__ tout, const double *vin, double *vout, double c1, double c2) {
     constexpr int CPS = 64 / B;
     constexpr int BB = B * B;
@@ -890,15 +918,11 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
     const VT *vp = val + p0 * BB;
     const int64_t cell = slice * CPS + q;
     const int64_t row = (cell < ncell ? cell : ncell - 1) * B + r;
-    double pre_v[NV], pre_y[NV];
+    double pre_v[NV];
     {
         const double *pv = vout ? vin : x;
-        const double *py = yx ? yx : x;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            pre_v[v] = w == 0 ? pv[row * NV + v] : 0.0;
-            pre_y[v] = w == 0 ? py[row * NV + v] : 0.0;
-        }
+        for (int v = 0; v < NV; ++v) pre_v[v] = w == 0 ? pv[row * NV + v] : 0.0;
     }
     double acc[NV];
 #pragma unroll
@@ -976,13 +1000,15 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
             for (int v = 0; v < NV; ++v) acc[v] += part[i][v][lane];
     }
     if (cell >= ncell) return;
+    // the vout stores first, behind one wait for the epilogue operands (a
+    // store between two such waits would be waited for as well)
+    if (vout) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        double y = acc[v];
-        if (yx) y += pre_y[v];
-        const double t = c1 * y;
-        if (tout) tout[row * NV + v] = t;
-        if (vout) vout[row * NV + v] = fma(c2, t, pre_v[v]);
+        for (int v = 0; v < NV; ++v) vout[row * NV + v] = fma(c2, c1 * acc[v], pre_v[v]);
+    }
+    if (tout) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) tout[row * NV + v] = c1 * acc[v];
     }
 }
 
@@ -1005,7 +1031,7 @@ template <int B, typename VT, int R>
 __global__ __launch_bounds__(256) void k_spmm_mfma(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ val, int64_t ncell, const double *__restrict__ x,
-    const double *__restrict__ yx, double *__restrict__ tout, const double *vin, double *vout,
+    double *__restrict__ tout, const double *vin, double *vout,
     double c1, double c2) {
     constexpr int CPS = 64 / B;
     constexpr int BB = B * B;
@@ -1065,7 +1091,6 @@ __global__ __launch_bounds__(256) void k_spmm_mfma(
         for (int r = 0; r < KC; ++r) {   // output rows kk + 4r < B
             const int64_t idx = (cell * B + kk + 4 * r) * R + nt * 16 + m;
             double y = acc[nt][r];
-            if (yx) y += yx[idx];
             const double t = c1 * y;
             if (tout) tout[idx] = t;
             if (vout) vout[idx] = fma(c2, t, vin[idx]);
@@ -1079,13 +1104,13 @@ namespace hymor {
 
 // k_spmm_mfma with more memory work in flight per wave: CPW cells per wave
 // (consecutive cells of one slice), their U-slot operand loads issued
-// together, and the epilogue's vin / yx reads issued at wave start.  Same
+// together, and the epilogue's vin reads issued at wave start.  Same
 // arithmetic, operand layout and summation order as k_spmm_mfma.
 template <int B, typename VT, int R, int U, int CPW>
 __global__ __launch_bounds__(256) void k_spmm_mfma_mc(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ val, int64_t ncell, const double *__restrict__ x,
-    const double *__restrict__ yx, double *__restrict__ tout, const double *vin, double *vout,
+    double *__restrict__ tout, const double *vin, double *vout,
     double c1, double c2) {
     constexpr int CPS = 64 / B;
     constexpr int BB = B * B;
@@ -1105,7 +1130,7 @@ __global__ __launch_bounds__(256) void k_spmm_mfma_mc(
     const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
     int ncw = (int)(ncell - cell0 < CPW ? ncell - cell0 : CPW);
     // epilogue operands first: independent of everything below
-    double pre_v[CPW][NT][KC], pre_y[CPW][NT][KC];
+    double pre_v[CPW][NT][KC];
 #pragma unroll
     for (int ci = 0; ci < CPW; ++ci)
 #pragma unroll
@@ -1115,7 +1140,6 @@ __global__ __launch_bounds__(256) void k_spmm_mfma_mc(
                 const int64_t idx = ((cell0 + ci) * B + kk + 4 * r) * R + nt * 16 + m;
                 const bool ok = ci < ncw;
                 pre_v[ci][nt][r] = (ok && vout) ? vin[idx] : 0.0;
-                pre_y[ci][nt][r] = (ok && yx) ? yx[idx] : 0.0;
             }
     v4d acc[CPW][NT];
 #pragma unroll
@@ -1167,7 +1191,7 @@ __global__ __launch_bounds__(256) void k_spmm_mfma_mc(
 #pragma unroll
             for (int r = 0; r < KC; ++r) {
                 const int64_t idx = ((cell0 + ci) * B + kk + 4 * r) * R + nt * 16 + m;
-                const double y = acc[ci][nt][r] + pre_y[ci][nt][r];
+                const double y = acc[ci][nt][r];
                 const double t = c1 * y;
                 if (tout) tout[idx] = t;
                 if (vout) vout[idx] = fma(c2, t, pre_v[ci][nt][r]);
@@ -1188,7 +1212,7 @@ template <int B, typename VT, int R, int W>
 __global__ __launch_bounds__(256) void k_spmm_mfma_w(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ val, int64_t ncell, const double *__restrict__ x,
-    const double *__restrict__ yx, double *__restrict__ tout, const double *vin, double *vout,
+    double *__restrict__ tout, const double *vin, double *vout,
     double c1, double c2) {
     constexpr int CPS = 64 / B;
     constexpr int BB = B * B;
@@ -1209,15 +1233,13 @@ __global__ __launch_bounds__(256) void k_spmm_mfma_w(
     const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
     // epilogue operands, read from a valid array when the caller passes none
     const double *pv = vout ? vin : x;
-    const double *py = yx ? yx : x;
-    double pre_v[NT][KC], pre_y[NT][KC];
+    double pre_v[NT][KC];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
         for (int r = 0; r < KC; ++r) {
             const int64_t idx = (cell * B + kk + 4 * r) * R + nt * 16 + m;
             pre_v[nt][r] = pv[idx];
-            pre_y[nt][r] = py[idx];
         }
     v4d acc[NT];
 #pragma unroll
@@ -1266,7 +1288,7 @@ __global__ __launch_bounds__(256) void k_spmm_mfma_w(
 #pragma unroll
         for (int r = 0; r < KC; ++r) {
             const int64_t idx = (cell * B + kk + 4 * r) * R + nt * 16 + m;
-            const double y = acc[nt][r] + (yx ? pre_y[nt][r] : 0.0);
+            const double y = acc[nt][r];
             const double t = c1 * y;
             if (tout) tout[idx] = t;
             if (vout) vout[idx] = fma(c2, t, pre_v[nt][r]);
@@ -1368,7 +1390,7 @@ __device__ __forceinline__ double lane_elem(float v, int) { return (double)v; }
 __device__ __forceinline__ double lane_elem(const d2 &v, int i) { return v[i]; }
 __device__ __forceinline__ double lane_elem(const f2 &v, int i) { return (double)v[i]; }
 
-// Batched Taylor term on the FP64 matrix cores with x, yx, vin, tout, vout
+// Batched Taylor term on the FP64 matrix cores with x, vin, tout, vout
 // in the tile layout and block values in TileVals order: per slot one
 // value load and NT x loads per lane (16 bytes each at B = 8), the
 // operands of KC x NT v_mfma_f64_16x16x4f64; the epilogue reads and writes
@@ -1380,7 +1402,7 @@ template <int B, typename VT, int R, int W>
 __global__ __launch_bounds__(256) void k_spmm_tile(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ valt, int64_t ncell, const double *__restrict__ x,
-    const double *__restrict__ yx, double *__restrict__ tout, const double *vin, double *vout,
+    double *__restrict__ tout, const double *vin, double *vout,
     double c1, double c2) {
     constexpr int CPS = 64 / B;
     constexpr int BB = B * B;
@@ -1407,13 +1429,11 @@ __global__ __launch_bounds__(256) void k_spmm_tile(
     const int64_t cbase = cell * B * R;
     // epilogue operands (a valid array stands in for a missing one)
     const double *pv = vout ? vin : x;
-    const double *py = yx ? yx : x;
-    XT pre_v[NT], pre_y[NT];
+    XT pre_v[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
         const int64_t o = cbase + ((int64_t)(kk * NT + nt) * WN + nr) * KC;
         pre_v[nt] = *reinterpret_cast<const XT *>(pv + o);
-        pre_y[nt] = *reinterpret_cast<const XT *>(py + o);
     }
     v4d acc[NT];
 #pragma unroll
@@ -1450,24 +1470,32 @@ __global__ __launch_bounds__(256) void k_spmm_tile(
         }
     }
     if (n >= WN) return;
+    // the vout stores first, behind one wait for the epilogue operands (a
+    // store between two such waits would be waited for as well)
+    if (vout) {
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-        const int64_t o = cbase + ((int64_t)(kk * NT + nt) * WN + n) * KC;
-        XT tv, vv;
+        for (int nt = 0; nt < NT; ++nt) {
+            XT vv;
 #pragma unroll
-        for (int r = 0; r < KC; ++r) {
-            const double y = acc[nt][r] + (yx ? lane_elem(pre_y[nt], r) : 0.0);
-            const double t = c1 * y;
-            if constexpr (KC == 1) {
-                tv = t;
-                vv = fma(c2, t, lane_elem(pre_v[nt], 0));
-            } else {
-                tv[r] = t;
-                vv[r] = fma(c2, t, lane_elem(pre_v[nt], r));
+            for (int r = 0; r < KC; ++r) {
+                const double w = fma(c2, c1 * acc[nt][r], lane_elem(pre_v[nt], r));
+                if constexpr (KC == 1) vv = w;
+                else vv[r] = w;
             }
+            *reinterpret_cast<XT *>(vout + cbase + ((int64_t)(kk * NT + nt) * WN + n) * KC) = vv;
         }
-        if (tout) *reinterpret_cast<XT *>(tout + o) = tv;
-        if (vout) *reinterpret_cast<XT *>(vout + o) = vv;
+    }
+    if (tout) {
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            XT tv;
+#pragma unroll
+            for (int r = 0; r < KC; ++r) {
+                if constexpr (KC == 1) tv = c1 * acc[nt][r];
+                else tv[r] = c1 * acc[nt][r];
+            }
+            *reinterpret_cast<XT *>(tout + cbase + ((int64_t)(kk * NT + nt) * WN + n) * KC) = tv;
+        }
     }
 }
 
@@ -1659,7 +1687,7 @@ template <typename VT, int R, int W>
 __global__ __launch_bounds__(256) void k_spmm_quad(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ valq, int64_t ncell, const double *__restrict__ x,
-    const double *__restrict__ yx, double *__restrict__ tout, const double *vin, double *vout,
+    double *__restrict__ tout, const double *vin, double *vout,
     double c1, double c2) {
     constexpr int B = 8, CPS = 8, BB = 64;
     constexpr int NT = R >= 16 ? R / 16 : 1;
@@ -1680,13 +1708,11 @@ __global__ __launch_bounds__(256) void k_spmm_quad(
     const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
     const int64_t cbase = cell * B * R;
     const double *pv = vout ? vin : x;
-    const double *py = yx ? yx : x;
-    d2 pre_v[NT], pre_y[NT];
+    d2 pre_v[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
         const int64_t o = cbase + ((int64_t)(k4 * NT + nt) * WN + jl) * 2;
         pre_v[nt] = *reinterpret_cast<const d2 *>(pv + o);
-        pre_y[nt] = *reinterpret_cast<const d2 *>(py + o);
     }
     double acc[NT][2];
 #pragma unroll
@@ -1749,7 +1775,7 @@ __global__ __launch_bounds__(256) void k_spmm_quad(
         d2 tv, vv;
 #pragma unroll
         for (int rq = 0; rq < 2; ++rq) {
-            const double y = acc[nt][rq] + (yx ? pre_y[nt][rq] : 0.0);
+            const double y = acc[nt][rq];
             const double t = c1 * y;
             tv[rq] = t;
             vv[rq] = fma(c2, t, pre_v[nt][rq]);
@@ -1775,7 +1801,7 @@ template <typename VT, int R, int W>
 __global__ __launch_bounds__(256) void k_spmm_quad4(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ val, int64_t ncell, const double *__restrict__ x,
-    const double *__restrict__ yx, double *__restrict__ tout, const double *vin, double *vout,
+    double *__restrict__ tout, const double *vin, double *vout,
     double c1, double c2) {
     constexpr int B = 4, CPS = 16, BB = 16;
     constexpr int V = Vpl<VT>::value;
@@ -1795,13 +1821,11 @@ __global__ __launch_bounds__(256) void k_spmm_quad4(
     const int64_t cbase = cell * B * R;
     const int aoff = ((k4 / V) * 64 + (int)q * 4 + n4) * V + k4 % V;
     const double *pv = vout ? vin : x;
-    const double *py = yx ? yx : x;
-    double pre_v[NI], pre_y[NI], acc[NI];
+    double pre_v[NI], acc[NI];
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
         const int64_t o = cbase + (int64_t)k4 * R + 16 * i + g;
         pre_v[i] = pv[o];
-        pre_y[i] = py[o];
         acc[i] = 0.0;
     }
     for (int t0 = 0; t0 < width; t0 += W) {
@@ -1832,10 +1856,120 @@ __global__ __launch_bounds__(256) void k_spmm_quad4(
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
         const int64_t o = cbase + (int64_t)k4 * R + 16 * i + g;
-        const double y = acc[i] + (yx ? pre_y[i] : 0.0);
+        const double y = acc[i];
         const double t = c1 * y;
         if (tout) tout[o] = t;
         if (vout) vout[o] = fma(c2, t, pre_v[i]);
+    }
+}
+
+// b = 4 batches on the 4x4x4 FP64 matrix cores with four cells per wave:
+// block blk of v_mfma_f64_4x4x4f64 is cell q0 + blk of a group of four
+// (a slice holds four groups), so one instruction covers 4 cells x 4 rows x
+// 4 vectors and each slot costs one value load (the four cells' blocks are
+// 64 adjacent values of the slice) and 32 bytes of x per lane and 16
+// vectors.  Lane (lo, blk, hi) = lane % 4, lane / 4 % 4, lane / 16 feeds
+// A = value (row lo, input hi) of cell blk, B = x (input hi of cell blk's
+// block column, vectors 16 i + 4 lo + g) to product g of vector group i,
+// and receives D = row hi, vectors 16 i + 4 lo + g: a lane's four x doubles
+// feed its four products and its four results are adjacent in the row.
+// Slots in groups of W, ascending; the group's 4 W block columns come from
+// one vector load and reach their lanes by ds_bpermute.
+template <typename VT, int R, int W>
+__global__ __launch_bounds__(256) void k_spmm_quad4c(
+    const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
+    const VT *__restrict__ val, int64_t ncell, const double *__restrict__ x,
+    double *__restrict__ tout, const double *vin, double *vout,
+    double c1, double c2) {
+    constexpr int B = 4, CPS = 16, BB = 16;
+    constexpr int V = Vpl<VT>::value;
+    constexpr int NG = R / 16;
+    static_assert(R == 16 || R == 32 || R == 64, "width 16..64");
+    static_assert(4 * W <= 64, "four columns per slot");
+    const int lane = threadIdx.x & 63;
+    const int64_t wpb = blockDim.x >> 6;
+    const int64_t cell0 = 4 * (int64_t)__builtin_amdgcn_readfirstlane(
+        (int)(xcd_tile(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6)));
+    if (cell0 >= ncell) return;
+    const int lo = lane & 3, blk = (lane >> 2) & 3, hi = lane >> 4;
+    const int64_t slice = cell0 / CPS;
+    const int q0 = (int)(cell0 - slice * CPS);
+    const int64_t p0 = slice_ptr[slice];
+    const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
+    // lane's value in a slot: row lo, input hi of cell q0 + blk
+    const int aoff = ((hi / V) * 64 + (q0 + blk) * 4 + lo) * V + hi % V;
+    // lane's output (and epilogue operand) offset: cell q0 + blk (clamped for
+    // the partial last group, whose extra cells are not written), row hi
+    const int64_t mycell = cell0 + blk < ncell ? cell0 + blk : ncell - 1;
+    const int64_t obase = (mycell * B + hi) * R + 4 * lo;
+    const double *pv = vout ? vin : x;
+    d2 pre_v[NG][2];
+#pragma unroll
+    for (int i = 0; i < NG; ++i)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            pre_v[i][h] = *reinterpret_cast<const d2 *>(pv + obase + 16 * i + 2 * h);
+        }
+    double acc[NG][4];
+#pragma unroll
+    for (int i = 0; i < NG; ++i)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) acc[i][g] = 0.0;
+    for (int t0 = 0; t0 < width; t0 += W) {
+        // lane 4u + b < 4W holds the column of slot t0 + u of cell q0 + b
+        const int lu = lane < 4 * W ? lane >> 2 : 0;
+        const int tl = t0 + lu < width ? t0 + lu : width - 1;
+        const int32_t cl = col[p0 + (int64_t)tl * CPS + q0 + (lane & 3)];
+        VT a[W];
+        d2 bx[W][NG][2];
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            const int t = t0 + u < width ? t0 + u : width - 1;
+            a[u] = __builtin_nontemporal_load(val + (p0 + (int64_t)t * CPS) * BB + aoff);
+            const int32_t c = __builtin_amdgcn_ds_bpermute(4 * (4 * u + blk), cl);
+            const double *xp = x + ((int64_t)c * B + hi) * R + 4 * lo;
+#pragma unroll
+            for (int i = 0; i < NG; ++i)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    bx[u][i][h] = *reinterpret_cast<const d2 *>(xp + 16 * i + 2 * h);
+        }
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            const double av = t0 + u < width ? (double)a[u] : 0.0;
+#pragma unroll
+            for (int i = 0; i < NG; ++i)
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+                    acc[i][g] = __builtin_amdgcn_mfma_f64_4x4x4f64(av, bx[u][i][g >> 1][g & 1],
+                                                                   acc[i][g], 0, 0, 0);
+        }
+    }
+    if (cell0 + blk >= ncell) return;
+    // the vout stores first, behind one wait for the epilogue operands (a
+    // store between two such waits would be waited for as well)
+    if (vout) {
+#pragma unroll
+        for (int i = 0; i < NG; ++i)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                d2 vv;
+#pragma unroll
+                for (int e = 0; e < 2; ++e)
+                    vv[e] = fma(c2, c1 * acc[i][2 * h + e], pre_v[i][h][e]);
+                *reinterpret_cast<d2 *>(vout + obase + 16 * i + 2 * h) = vv;
+            }
+    }
+    if (tout) {
+#pragma unroll
+        for (int i = 0; i < NG; ++i)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                d2 tv;
+#pragma unroll
+                for (int e = 0; e < 2; ++e) tv[e] = c1 * acc[i][2 * h + e];
+                *reinterpret_cast<d2 *>(tout + obase + 16 * i + 2 * h) = tv;
+            }
     }
 }
 

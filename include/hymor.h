@@ -219,10 +219,11 @@ int hymor_op_set_graphs(hymor_op *op, int enable);
  * (k_spmm_tile, the default; it also serves 8 doubles per row), 7 the
  * 4x4x4 FP64 matrix cores with no idle rows: at b = 8 the tile order
  * (k_spmm_quad), at b = 4 the row-major batch and the single-vector value
- * order (k_spmm_quad4).  Under the default (6), 8 doubles per row at b = 4
- * run the LDS-staged vector-FMA term kernel instead (k_term_lds: the
- * matrix-core kernels' one-cell waves do too little work at b = 4).  All
- * accumulate in FP64. */
+ * order (k_spmm_quad4, one cell per wave).  Under the default (6), b = 4
+ * batches run other kernels (one-cell waves do too little work at b = 4):
+ * 8 doubles per row the LDS-staged vector-FMA term kernel (k_term_lds),
+ * 16+ the 4x4x4 matrix cores with four cells per wave, one cell per
+ * product block (k_spmm_quad4c).  All accumulate in FP64. */
 int hymor_op_set_batch_kernel(hymor_op *op, int use_mfma);
 
 /* Times the eight batch kernels at width nv (8, 16, 32 or 64) and keeps the

@@ -612,3 +612,160 @@ extern "C" int emu_spmm_quad4_csc(int64_t n, const int64_t *colptr, const int64_
         }
     return g_oob;
 }
+
+// k_spmm_quad4c (b = 4, four cells per wave) replayed on the CPU: block blk
+// of v_mfma_f64_4x4x4f64 is cell 4 grp + blk; lane (lo, blk, hi) = lo + 4 blk
+// + 16 hi supplies A[m = lo][k = hi] of its cell and X[k = hi][16 i + 4 lo + g]
+// of its cell's block column to product g; D[blk][m][n] lands in lane
+// n + 4 blk + 16 m.  Slots in groups of W; lane 4u + b of the group's column
+// load holds slot t0 + u of cell b and is read back by lane index
+// 4 (4u + blk) (ds_bpermute's byte address).  The cells past ncell in the
+// last group are computed (padded slice) but not written.
+extern "C" int emu_spmm_quad4c_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                                   const double *nzval, int64_t ncell, int nrhs, int R, int W,
+                                   int vpl, const double *X_vm, double *Y_vm) {
+    g_oob = 0;
+    const int b = 4, CPS = 16, BB = 16;
+    if (R % 16 != 0 || 4 * W > 64 || W < 1) return -2;
+    HostBSR fwd, adj;
+    HostExtra xf, xa;
+    csc_to_blocks(n, colptr, rowval, nzval, 1, false, 1.0, b, ncell, fwd, adj, xf, xa);
+    if (!xf.rows.empty()) return -1;
+    HostSliced s;
+    bsr_to_sliced(fwd, s);
+    interleave_lanes(s, vpl);
+    std::vector<double> x((size_t)n * R, 0.0), y((size_t)n * R, 0.0);
+    for (int64_t row = 0; row < n; ++row)
+        for (int j = 0; j < nrhs; ++j) {
+            const int64_t vm = (row % b) * ncell + row / b;
+            x[(size_t)(row * R + j)] = X_vm[j * n + vm];
+        }
+    const int NG = R / 16;
+    const int64_t ngrp = (ncell + 3) / 4;
+    for (int64_t grp = 0; grp < ngrp; ++grp) {
+        const int64_t cell0 = 4 * grp;
+        const int64_t slice = cell0 / CPS;
+        const int q0 = (int)(cell0 - slice * CPS);
+        const int64_t p0 = at(s.slice_ptr, slice);
+        const int width = (int)((at(s.slice_ptr, slice + 1) - p0) / CPS);
+        std::vector<double> acc((size_t)NG * 4 * 64, 0.0);   // [i][g][lane]
+        for (int t0 = 0; t0 < width; t0 += W) {
+            int32_t cl[64];
+            for (int l = 0; l < 64; ++l) {
+                const int lu = l < 4 * W ? l >> 2 : 0;
+                const int tl = t0 + lu < width ? t0 + lu : width - 1;
+                cl[l] = at(s.col, p0 + (int64_t)tl * CPS + q0 + (l & 3));
+            }
+            for (int u = 0; u < W; ++u) {
+                const int t = t0 + u < width ? t0 + u : width - 1;
+                double opa[64];
+                double opb[64][16];   // [lane][4 i + g]
+                for (int l = 0; l < 64; ++l) {
+                    const int lo = l & 3, blk = (l >> 2) & 3, hi = l >> 4;
+                    const int aoff = ((hi / vpl) * 64 + (q0 + blk) * 4 + lo) * vpl + hi % vpl;
+                    const double a = at(s.val, (p0 + (int64_t)t * CPS) * BB + aoff);
+                    opa[l] = t0 + u < width ? a : 0.0;
+                    const int src = 4 * u + blk;      // bpermute source lane
+                    if (src < 0 || src >= 64) { ++g_oob; continue; }
+                    const int64_t c = cl[src];
+                    for (int i = 0; i < NG; ++i)
+                        for (int g = 0; g < 4; ++g)
+                            opb[l][4 * i + g] = at(x, (c * b + hi) * R + 4 * lo + 16 * i + g);
+                }
+                for (int i = 0; i < NG; ++i)
+                    for (int g = 0; g < 4; ++g)
+                        for (int blk = 0; blk < 4; ++blk)
+                            for (int m = 0; m < 4; ++m)
+                                for (int nn = 0; nn < 4; ++nn) {
+                                    double sacc = 0.0;
+                                    for (int k = 0; k < 4; ++k)
+                                        sacc += opa[m + 4 * blk + 16 * k] *
+                                                opb[nn + 4 * blk + 16 * k][4 * i + g];
+                                    acc[((size_t)i * 4 + g) * 64 + nn + 4 * blk + 16 * m] += sacc;
+                                }
+            }
+        }
+        for (int l = 0; l < 64; ++l) {
+            const int lo = l & 3, blk = (l >> 2) & 3, hi = l >> 4;
+            if (cell0 + blk >= ncell) continue;
+            const int64_t obase = ((cell0 + blk) * b + hi) * R + 4 * lo;
+            for (int i = 0; i < NG; ++i)
+                for (int g = 0; g < 4; ++g) {
+                    const int64_t o = obase + 16 * i + g;
+                    if (o < 0 || o >= (int64_t)y.size()) { ++g_oob; continue; }
+                    y[(size_t)o] = acc[((size_t)i * 4 + g) * 64 + l];
+                }
+        }
+    }
+    for (int64_t row = 0; row < n; ++row)
+        for (int j = 0; j < nrhs; ++j) {
+            const int64_t vm = (row % b) * ncell + row / b;
+            Y_vm[j * n + vm] = y[(size_t)(row * R + j)];
+        }
+    return g_oob;
+}
+
+// k_extra (long extra rows) replayed on the CPU: one workgroup of NW waves
+// per listed row; lane (j, i) = j LPR + i of wave wv takes entries
+// ptr[row] + wv EPW + j + k NW EPW and the NC doubles from v0 = i NC of the
+// row's NV (row-major rows: the extra-DOF layout); partial sums are combined
+// by the xor butterfly over offsets 32 .. LPR inside each wave, then over
+// the waves in order.  x / y hold n rows of NV doubles; the listed rows are
+// written to y (flow rows included: the emulation has no yx split).
+extern "C" int emu_extra_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                             const double *nzval, int is_complex, int b, int64_t ncell, int NV,
+                             int NW, const double *x, double *y) {
+    g_oob = 0;
+    HostBSR fwd, adj;
+    HostExtra xf, xa;
+    csc_to_blocks(n, colptr, rowval, nzval, 1, is_complex != 0, 1.0, b, ncell, fwd, adj, xf, xa);
+    const bool cplx = is_complex != 0;
+    if (cplx && NV < 2) return -2;
+    const int LPR = cplx ? NV / 2 : NV, EPW = 64 / LPR, NC = cplx ? 2 : 1;
+    if (64 % LPR != 0) return -3;
+    std::vector<double> xv(x, x + n * NV);
+    for (int64_t ri = 0; ri < (int64_t)xf.rows.size(); ++ri) {
+        std::vector<double> wsum((size_t)NW * 64 * 2, 0.0);   // [wave][lane][q]
+        for (int wv = 0; wv < NW; ++wv)
+            for (int lane = 0; lane < 64; ++lane) {
+                const int j = lane / LPR, v0 = (lane % LPR) * NC;
+                double acc[2] = {0, 0};
+                for (int64_t e = at(xf.ptr, ri) + wv * EPW + j; e < at(xf.ptr, ri + 1);
+                     e += (int64_t)NW * EPW) {
+                    const int64_t c = at(xf.col, e);
+                    if (cplx) {
+                        const double ar = at(xf.val, 2 * e), ai = at(xf.val, 2 * e + 1);
+                        const double xr = at(xv, c * NV + v0), xi = at(xv, c * NV + v0 + 1);
+                        acc[0] = ar * xr + (-ai * xi + acc[0]);
+                        acc[1] = ar * xi + (ai * xr + acc[1]);
+                    } else {
+                        acc[0] = at(xf.val, e) * at(xv, c * NV + v0) + acc[0];
+                    }
+                }
+                for (int q = 0; q < NC; ++q) wsum[((size_t)wv * 64 + lane) * 2 + q] = acc[q];
+            }
+        // butterfly inside each wave: every lane ends with its group's sum
+        for (int wv = 0; wv < NW; ++wv)
+            for (int off = 32; off >= LPR; off >>= 1) {
+                std::vector<double> nxt(64 * 2);
+                for (int lane = 0; lane < 64; ++lane)
+                    for (int q = 0; q < 2; ++q)
+                        nxt[lane * 2 + q] = wsum[((size_t)wv * 64 + lane) * 2 + q] +
+                                            wsum[((size_t)wv * 64 + (lane ^ off)) * 2 + q];
+                for (int lane = 0; lane < 64; ++lane)
+                    for (int q = 0; q < 2; ++q) wsum[((size_t)wv * 64 + lane) * 2 + q] = nxt[lane * 2 + q];
+            }
+        const int64_t row = at(xf.rows, ri);
+        for (int lane = 0; lane < LPR; ++lane) {   // j == 0 lanes of wave 0
+            const int v0 = lane * NC;
+            for (int q = 0; q < NC; ++q) {
+                double sacc = wsum[(size_t)lane * 2 + q];
+                for (int wv = 1; wv < NW; ++wv) sacc += wsum[((size_t)wv * 64 + lane) * 2 + q];
+                const int64_t o = row * NV + v0 + q;
+                if (o < 0 || o >= n * NV) { ++g_oob; continue; }
+                y[o] = sacc;
+            }
+        }
+    }
+    return g_oob;
+}

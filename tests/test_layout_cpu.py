@@ -214,3 +214,75 @@ def test_emulated_batched_quad4(emu, R, nrhs, vpl, wide):
                                  op.ncell, nrhs, R, vpl, X.ctypes.data, Y.ctypes.data)
     assert oob == 0
     assert np.allclose(Y, op.to_scipy_varmajor() @ X, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("R,W,nrhs,vpl", [(16, 9, 16, 4), (16, 5, 13, 2), (32, 5, 32, 4),
+                                          (32, 4, 27, 2), (64, 2, 64, 4), (64, 1, 40, 2)])
+@pytest.mark.parametrize("wide", [False, True])
+def test_emulated_batched_quad4c(emu, R, W, nrhs, vpl, wide):
+    """Four cells per wave on the 4x4x4 matrix cores at b = 4 (block = cell,
+    column distribution by lane permute, slot groups of W, a partial last
+    group of cells) replayed on the CPU with every access checked, against
+    scipy."""
+    emu.emu_spmm_quad4c_csc.restype = C.c_int
+    emu.emu_spmm_quad4c_csc.argtypes = [C.c_int64] + [C.c_void_p] * 3 + [
+        C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    if wide:
+        from test_gpu_parity import random_pattern_operator
+        op = random_pattern_operator(9, 7, 4, 19, seed=R + W)
+    else:
+        op = ops.stencil_operator(9, 7, 4, seed=13)   # 63 cells: last group of 3
+    assert op.ncell % 4 != 0
+    colptr, rowval, nzval = op.to_csc_varmajor(one_based=True)
+    X = np.asfortranarray(np.random.default_rng(5).standard_normal((op.n, nrhs)))
+    Y = np.zeros_like(X)
+    oob = emu.emu_spmm_quad4c_csc(op.n, colptr.ctypes.data, rowval.ctypes.data,
+                                  nzval.ctypes.data, op.ncell, nrhs, R, W, vpl, X.ctypes.data,
+                                  Y.ctypes.data)
+    assert oob == 0
+    assert np.allclose(Y, op.to_scipy_varmajor() @ X, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("NV,NW", [(1, 4), (2, 4), (8, 4), (32, 4), (64, 4), (4, 1), (16, 2)])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_emulated_long_extra_rows(emu, NV, NW, cplx):
+    """k_extra's lane mapping (NW waves per long row, lanes per entry, xor
+    butterfly, waves combined in order) replayed on the CPU with every
+    access checked: the long rows of the shock bands (real) and of the
+    receptivity coupling (complex) against scipy on the same rows."""
+    if cplx and NV < 2:
+        pytest.skip("complex rows need two doubles")
+    emu.emu_extra_csc.restype = C.c_int
+    emu.emu_extra_csc.argtypes = [C.c_int64] + [C.c_void_p] * 3 + [
+        C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    if cplx:
+        base = ops.stencil_operator(6, 4, 4, seed=3)
+        op = ops.receptivity_extension(base, np.array([0.5, 1.0, 2.0]))
+    else:
+        op = ops.stencil_operator(12, 10, 4, seed=2, n_shock=12)
+    colptr, rowval, nzval = op.to_csc_varmajor(one_based=True)
+    nrhs = NV // 2 if cplx else NV
+    rng = np.random.default_rng(NV)
+    Xvm = rng.standard_normal((op.n, nrhs))
+    if cplx:
+        Xvm = Xvm + 1j * rng.standard_normal((op.n, nrhs))
+    # rows in the internal order: flow rows cell-major, extra DOFs after
+    b, ncell = op.b, op.ncell
+    nflow = b * ncell
+    perm = np.arange(op.n)
+    r = np.arange(nflow)
+    perm[:nflow] = (r % b) * ncell + r // b        # internal row -> var-major index
+    Xi = Xvm[perm]
+    x = np.ascontiguousarray(Xi.view(np.float64) if cplx else Xi)
+    y = np.zeros_like(x)
+    oob = emu.emu_extra_csc(op.n, colptr.ctypes.data, rowval.ctypes.data,
+                            np.ascontiguousarray(nzval).ctypes.data, int(cplx), b, ncell, NV, NW,
+                            x.ctypes.data, y.ctypes.data)
+    assert oob == 0
+    # the extra entries: every entry outside the flow-flow block part
+    A = op.to_scipy_varmajor().tolil()
+    A[:nflow, :nflow] = 0
+    want = (A.tocsr() @ Xvm)[perm]
+    got = y.view(np.complex128) if cplx else y
+    assert np.abs(want).max() > 0
+    assert np.allclose(got, want, rtol=1e-13, atol=1e-13)
