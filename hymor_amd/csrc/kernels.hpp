@@ -191,7 +191,10 @@ __global__ __launch_bounds__(64 * NW) void k_extra(
     static_assert(64 % LPR == 0, "lanes per entry divide the wave");
     __shared__ double part[NW > 1 ? NW - 1 : 1][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int64_t ri = blockIdx.x;
+    // contiguous row ranges per XCD: neighbouring rows gather the same x
+    // rows (the receptivity B rows of adjacent cells), which then stay in
+    // one L2
+    const int64_t ri = xcd_tile(blockIdx.x, gridDim.x);
     if (ri >= nrows) return;
     const int j = lane / LPR;                  // entry of the step
     const int v0 = (lane % LPR) * NC;          // first of this lane's doubles
