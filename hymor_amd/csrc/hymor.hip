@@ -266,11 +266,11 @@ void dfree(hymor_op *op, void *p, size_t bytes) {
         }
 }
 
-// Rows of the extra CSR with at most this many entries go to k_extra_short
-// (lanes per row, no reduction); longer ones to k_extra (a workgroup per
-// row, lanes per entry).
+// Rows of the extra CSR with more than this many entries get a workgroup
+// each (lanes per entry, cross-lane sums); shorter ones lanes per row, no
+// reduction (both in k_extra).
 constexpr int64_t kShortRow = 32;
-// waves per long extra row (k_extra: one workgroup per row)
+// waves per workgroup of k_extra (one workgroup per long row)
 constexpr int kLongWaves = 4;
 
 // The listed rows reordered long rows first (each group keeps its order),
@@ -679,41 +679,26 @@ template <int NV>
 void launch_extra_nv(const DevExtra &xe, int64_t nflow, hipStream_t st, const double *x,
                      double *tout, const double *vin, double *vout, double c1,
                      double c2, const ColDiag &cd, int tile_b) {
-    const int wpb = 4;
-    if (xe.nlong > 0) {
-        // one workgroup of kLongWaves waves per long row
-        constexpr int NW = kLongWaves;
-        if (xe.is_complex) {
-            if constexpr (NV >= 2)
-                hipLaunchKernelGGL((k_extra<NV, true, NW>), dim3((unsigned)xe.nlong),
-                                   dim3(64 * NW), 0, st, xe.rows, xe.ptr, xe.col, xe.val,
-                                   xe.nlong, nflow, x, tout, vin, vout, c1, c2, cd.d, cd.r0,
-                                   cd.r1, tile_b);
-            else
-                throw ArgError("complex extras need two doubles per row");
-        } else {
-            hipLaunchKernelGGL((k_extra<NV, false, NW>), dim3((unsigned)xe.nlong), dim3(64 * NW),
-                               0, st, xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, nflow, x,
-                               tout, vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
-        }
-    }
+    // one launch: a workgroup of kLongWaves waves per long row, then
+    // workgroups of kLongWaves waves over the short rows
+    constexpr int NW = kLongWaves;
+    const int lpr = xe.is_complex ? (NV >= 2 ? NV / 2 : 1) : NV;
+    const int64_t rpw = 64 / lpr;
     const int64_t nshort = xe.nrows - xe.nlong;
-    if (nshort > 0) {
-        if constexpr (NV < 2) {
-            if (xe.is_complex) throw ArgError("complex extras need two doubles per row");
-        }
-        const int lpr = xe.is_complex ? (NV >= 2 ? NV / 2 : 1) : NV;
-        const int64_t rpw = 64 / lpr;
-        const int64_t waves = (nshort + rpw - 1) / rpw;
-        const int64_t grid = (waves + wpb - 1) / wpb;
-        if (xe.is_complex)
-            hipLaunchKernelGGL((k_extra_short<NV, true>), dim3((unsigned)grid), dim3(64 * wpb),
-                               0, st, xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, xe.nrows, nflow,
-                               x, tout, vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
+    const int64_t short_waves = (nshort + rpw - 1) / rpw;
+    const int64_t grid = xe.nlong + (short_waves + NW - 1) / NW;
+    if (grid == 0) return;
+    if (xe.is_complex) {
+        if constexpr (NV >= 2)
+            hipLaunchKernelGGL((k_extra<NV, true, NW>), dim3((unsigned)grid), dim3(64 * NW), 0, st,
+                               xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, xe.nrows, nflow, x,
+                               tout, vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
         else
-            hipLaunchKernelGGL((k_extra_short<NV, false>), dim3((unsigned)grid), dim3(64 * wpb),
-                               0, st, xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, xe.nrows, nflow,
-                               x, tout, vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
+            throw ArgError("complex extras need two doubles per row");
+    } else {
+        hipLaunchKernelGGL((k_extra<NV, false, NW>), dim3((unsigned)grid), dim3(64 * NW), 0, st,
+                           xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, xe.nrows, nflow, x, tout,
+                           vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
     }
 }
 

@@ -113,7 +113,7 @@ __device__ __forceinline__ void load_lane_row(const VT *__restrict__ base, int l
 }
 
 // One Taylor term over the block part:  y = A*x (the extra CSR rows add
-// theirs afterwards, k_extra / k_extra_short), t = c1*y, tout = t (if
+// theirs afterwards, k_extra), t = c1*y, tout = t (if
 // tout), vout = vin + c2*t (if vout).
 // c1/c2 select the recursion: normalised (c1 = dt/j, c2 = 1) or powers
 // (c1 = dt, c2 = 1/j!).  With tout = y and vout = null this is a plain SpMV.
@@ -163,9 +163,9 @@ __global__ __launch_bounds__(256) void k_term(
     }
 }
 
-// Extra CSR rows, NW waves (one workgroup) per listed row.  Flow rows
-// (row < nflow) add their extra entries' part to the block kernel's output
-// (which runs first); extra-DOF rows finish the term here.  CPLX: values
+// One extra CSR row with many entries on the NW waves of a workgroup.  Flow
+// rows (row < nflow) add their extra entries' part to the block kernel's
+// output (which runs first); extra-DOF rows finish the term here.  CPLX: values
 // are complex (re,im) and NV counts doubles of complex vectors ((re,im)
 // pairs); otherwise values are real and every one of the NV components is
 // an independent right-hand side.
@@ -178,9 +178,9 @@ __global__ __launch_bounds__(256) void k_term(
 // (re, im) of the NV doubles), added on rows [cd0, cd1) -- the -i*omega_k
 // block of the factored receptivity systems, one wavenumber per column.
 template <int NV, bool CPLX, int NW>
-__global__ __launch_bounds__(64 * NW) void k_extra(
-    const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
-    const int32_t *__restrict__ col, const double *__restrict__ val, int64_t nrows,
+__device__ __forceinline__ void extra_long_row(
+    int64_t ri, const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
+    const int32_t *__restrict__ col, const double *__restrict__ val,
     int64_t nflow, const double *__restrict__ x,
     double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
     const double *__restrict__ cdiag, int64_t cd0, int64_t cd1, int tile_b) {
@@ -191,11 +191,6 @@ __global__ __launch_bounds__(64 * NW) void k_extra(
     static_assert(64 % LPR == 0, "lanes per entry divide the wave");
     __shared__ double part[NW > 1 ? NW - 1 : 1][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // contiguous row ranges per XCD: neighbouring rows gather the same x
-    // rows (the receptivity B rows of adjacent cells), which then stay in
-    // one L2
-    const int64_t ri = xcd_tile(blockIdx.x, gridDim.x);
-    if (ri >= nrows) return;
     const int j = lane / LPR;                  // entry of the step
     const int v0 = (lane % LPR) * NC;          // first of this lane's doubles
     double acc[NC];
@@ -267,10 +262,11 @@ __global__ __launch_bounds__(64 * NW) void k_extra(
 // DOFs, rows touching one shock DOF): no cross-lane reduction, each lane
 // owns one (row, column) pair -- a complex column pair when CPLX -- and
 // loops over the row's entries; RPW = 64 / lanes-per-row rows per wave.
-// Rows [r0, nrows) of the listed rows; same outputs as k_extra.
+// Wave w takes rows [r0 + w RPW, r0 + (w + 1) RPW) of [r0, nrows); same
+// outputs as extra_long_row.
 template <int NV, bool CPLX>
-__global__ __launch_bounds__(256) void k_extra_short(
-    const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
+__device__ __forceinline__ void extra_short_rows(
+    int64_t w, const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
     const int32_t *__restrict__ col, const double *__restrict__ val, int64_t r0, int64_t nrows,
     int64_t nflow, const double *__restrict__ x,
     double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
@@ -280,7 +276,6 @@ __global__ __launch_bounds__(256) void k_extra_short(
     constexpr int NC = CPLX ? 2 : 1;                          // doubles per lane
     static_assert(64 % LPR == 0, "lanes per row divide the wave");
     const int lane = threadIdx.x & 63;
-    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t ri = r0 + w * RPW + lane / LPR;
     if (ri >= nrows) return;
     const int v0 = (lane % LPR) * NC;    // first of this lane's doubles in the row
@@ -324,6 +319,28 @@ __global__ __launch_bounds__(256) void k_extra_short(
             if (vout) vout[row * NV + v] = fma(c2, t, vin[row * NV + v]);
         }
     }
+}
+
+// The extra CSR rows in one launch of NW-wave workgroups: workgroups
+// [0, nlong) take the long rows [0, nlong) (more than 32 entries), one each,
+// in contiguous ranges per XCD (neighbouring rows gather the same x rows --
+// the receptivity B rows of adjacent cells -- which then stay in one L2);
+// the rest take the short rows [nlong, nrows), RPW per wave.
+template <int NV, bool CPLX, int NW>
+__global__ __launch_bounds__(64 * NW) void k_extra(
+    const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
+    const int32_t *__restrict__ col, const double *__restrict__ val, int64_t nlong,
+    int64_t nrows, int64_t nflow, const double *__restrict__ x,
+    double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
+    const double *__restrict__ cdiag, int64_t cd0, int64_t cd1, int tile_b) {
+    const int64_t bid = blockIdx.x;
+    if (bid < nlong)
+        extra_long_row<NV, CPLX, NW>(xcd_tile(bid, nlong), rows, ptr, col, val, nflow, x, tout,
+                                     vin, vout, c1, c2, cdiag, cd0, cd1, tile_b);
+    else
+        extra_short_rows<NV, CPLX>((bid - nlong) * NW + (threadIdx.x >> 6), rows, ptr, col, val,
+                                   nlong, nrows, nflow, x, tout, vin, vout, c1, c2, cdiag, cd0,
+                                   cd1, tile_b);
 }
 
 // var-major (ABI) -> cell-major (internal).  x_vm holds nrhs columns of n
