@@ -113,8 +113,8 @@ __device__ __forceinline__ void load_lane_row(const VT *__restrict__ base, int l
 }
 
 // One Taylor term over the block part:  y = A*x (the extra CSR rows add
-// theirs afterwards, k_extra), t = c1*y, tout = t (if
-// tout), vout = vin + c2*t (if vout).
+// theirs afterwards, k_extra), t = c1*y, tout = t (if tout),
+// vout = vin + c2*t (if vout).
 // c1/c2 select the recursion: normalised (c1 = dt/j, c2 = 1) or powers
 // (c1 = dt, c2 = 1/j!).  With tout = y and vout = null this is a plain SpMV.
 // vin may equal vout (in-place accumulation of the own row) or x (first term
