@@ -40,6 +40,25 @@ def test_batched_matches_oracle_columns(orc, b, nrhs, shock):
                                                     transpose=True)) <= 1e-13
 
 
+@pytest.mark.parametrize("nchi,neta", [(1, 1), (3, 1), (2, 3), (5, 1)])
+@pytest.mark.parametrize("b,nrhs", [(4, 16), (4, 32), (4, 64), (8, 16), (4, 8)])
+def test_batched_tiny_grids(orc, nchi, neta, b, nrhs):
+    """Grids of 1-6 cells: at b = 4 the four-cells-per-wave kernel then has
+    one partial group of cells (the only wave, or the last one), and the
+    single slice of every kernel is partial."""
+    say(f"[batched tiny] {nchi}x{neta} b={b} nrhs={nrhs}")
+    op = ops.stencil_operator(nchi, neta, b, seed=nchi + 7 * neta, n_shock=nchi, row_sum=2.0)
+    A = orc.CSC.from_arrays(*op.to_csc_varmajor())
+    X = np.random.default_rng(nrhs).standard_normal((op.n, nrhs))
+    with DeviceOperator.from_blocks(op) as dev:
+        Y = dev.expm_taylor_batched(X, order=5, nsub=3, dt=0.1)
+        Yt = dev.expm_taylor_batched(X, order=5, nsub=3, dt=0.1, transpose=True)
+    At = orc.csc_adjoint(A)
+    for j in (0, nrhs - 1):
+        assert relerr(Y[:, j], orc.power_method(A.scaled(0.1), X[:, j], 3)) <= 1e-12
+        assert relerr(Yt[:, j], orc.power_method(At.scaled(0.1), X[:, j], 3)) <= 1e-12
+
+
 def test_batched_complex_receptivity(orc):
     say("[batched] complex receptivity operator, 12 rhs")
     base = ops.stencil_operator(8, 6, 4, seed=12, n_shock=8, row_sum=2.0)
