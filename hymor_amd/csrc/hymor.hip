@@ -1615,7 +1615,7 @@ int hymor_bench_read(int64_t bytes, int reps, double *ms_per_launch) {
             }
         } guard{allocs};
         const int64_t n2 = bytes / 16;
-        const int64_t grid = std::min<int64_t>((n2 + 255) / 256, 4096);
+        const int64_t grid = std::min<int64_t>((n2 + 255) / 256, 2048);
         d2 *in = (d2 *)dalloc_raw(allocs, (size_t)bytes);
         double *out = (double *)dalloc_raw(allocs, (size_t)grid * 256 * sizeof(double));
         HIP_TRY(hipMemset(in, 0, (size_t)bytes));

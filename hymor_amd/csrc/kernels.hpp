@@ -489,11 +489,14 @@ __global__ __launch_bounds__(256) void k_copy(const d2 *__restrict__ in, d2 *__r
 // Streaming read of n2 16-byte chunks, one partial sum stored per thread:
 // the read-only ceiling the value stream of the Taylor-term kernels is
 // measured against.  Workgroup g reads the contiguous run [g*chunk,
-// (g+1)*chunk), eight non-temporal loads in flight per thread (on MI355X
-// this beats a grid-strided sweep by about 10 %).
+// (g+1)*chunk), sixteen non-temporal loads in flight per thread, 2048
+// workgroups: the fastest of a sweep over contiguous-chunk and grid-strided
+// reads, 2048-32768 workgroups, 4-16 loads per thread, with and without the
+// non-temporal hint, on an 828 MiB buffer (6.5 TB/s in that sweep, 5.2-6.3
+// for the others; 6.4-6.5 TB/s between runs).
 __global__ __launch_bounds__(256) void k_read(const d2 *__restrict__ in, double *__restrict__ out,
                                               int64_t n2, int64_t chunk) {
-    constexpr int U = 8;
+    constexpr int U = 16;
     const int64_t b0 = (int64_t)blockIdx.x * chunk;
     const int64_t e = b0 + chunk < n2 ? b0 + chunk : n2;
     double acc0 = 0.0, acc1 = 0.0;
