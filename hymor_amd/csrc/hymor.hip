@@ -599,12 +599,13 @@ void launch_block_part(const DevSliced &s, int nv, hipStream_t st, const double 
             return;
         }
         // 16+ doubles per row: four cells per wave on the 4x4x4 matrix
-        // cores, slot groups sized to the registers the x chunks take
+        // cores; slot groups of 9 at 16 doubles, 3 at 32 and 64 (C4: 2-10 %
+        // faster than groups of 5 / 2 there, 12-15 % slower at 16)
         if (nv >= 16 && s.tune.spmm_mfma == 6) {
             switch (nv) {
             case 16: launch_quad4c_t<VT, 16, 9>(s, st, x, tout, vin, vout, c1, c2); return;
-            case 32: launch_quad4c_t<VT, 32, 5>(s, st, x, tout, vin, vout, c1, c2); return;
-            case 64: launch_quad4c_t<VT, 64, 2>(s, st, x, tout, vin, vout, c1, c2); return;
+            case 32: launch_quad4c_t<VT, 32, 3>(s, st, x, tout, vin, vout, c1, c2); return;
+            case 64: launch_quad4c_t<VT, 64, 3>(s, st, x, tout, vin, vout, c1, c2); return;
             default: throw ArgError("unsupported vector width");
             }
         }
