@@ -521,7 +521,7 @@ def main():
                        5: f"k_spmm_mfma_w<{cfg['b']}, float, {nb}, "
                           f"{ {16: 5, 32: 3, 64: 2}.get(nb)}>",
                        6: f"k_spmm_tile<{cfg['b']}, float, {nb}, "
-                          f"{ {8: 9, 16: 9, 32: 5, 64: 3}.get(nb)}>",
+                          f"{ {8: 9, 16: 9, 32: 3, 64: 3}.get(nb)}>",
                        7: f"k_spmm_quad<float, {nb}, { {8: 9, 16: 9, 32: 5, 64: 3}.get(nb)}>"
                        }.get(bkern, "k_spmm") if cfg["b"] == 8 else {
                        # b = 4 under batch kernels 6 and 7

@@ -543,8 +543,9 @@ void launch_tile_t(const DevSliced &s, hipStream_t st, const double *x,
                    double *tout, const double *vin, double *vout, double c1, double c2) {
     if (!s.val_t) throw ArgError("tile-ordered values missing (internal)");
     // slot groups: the 3x3 stencil's 9 slots in one group up to width 16,
-    // fewer at 32 / 64 to bound the registers holding operands in flight
-    constexpr int W = R <= 16 ? 9 : (R == 32 ? 5 : 3);
+    // 3 at 32 / 64 to bound the registers holding operands in flight (C5:
+    // groups of 5 or 9 at 32, 2 or 5 at 64 measured 0-11 % slower)
+    constexpr int W = R <= 16 ? 9 : 3;
     const int wpb = 4;
     const int64_t grid = (s.ncell + wpb - 1) / wpb;
     hipLaunchKernelGGL((k_spmm_tile<B, VT, R, W>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
