@@ -1,7 +1,7 @@
 """The physical channel-flow operator (hymor_amd.channel) on the GPU.
 
-* BASELINE config 1's operator (99 x 80 uniform grid -- Nchi divisible by
-  3 for the periodic colouring -- M 0.5, Re 2000, Pr 0.72): the device
+* The 99 x 80-cell channel operator (uniform grid -- Nchi divisible by 3
+  for the periodic colouring -- M 0.5, Re 2000, Pr 0.72): the device
   Taylor action T_5(A dt)^n at the CFL step vs the oracle's POWER_METHOD!
   (1e-11: the operator is far from normal, ||A dt||_inf ~ 13).
 * A small physical channel through the device eigenvalue driver, against

@@ -2,12 +2,12 @@
 
 Tolerances (relative 2-norm unless stated):
   * FP64 storage vs the FP64 oracle: 1e-12 for a few substeps, 1e-11 for the
-    768-SpMV config-1 action (summation order and FMA differ from the
+    768-SpMV action on the 100 x 80-cell, b = 4 operator (summation order and FMA differ from the
     reference's column scatter, so agreement is to rounding, not bits);
   * FP32 storage (FP64 accumulation): 2e-6 -- the values themselves carry a
     2^-24 relative rounding;
   * true exponential of a block-circulant operator vs its FFT closed form:
-    1e-10 (the north-star bound for exp(A*T)v).
+    1e-10 (the accuracy target for exp(A*T)v).
 """
 import sys
 

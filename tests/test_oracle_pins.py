@@ -188,7 +188,7 @@ def test_power_method_block_circulant_closed_form(orc, b):
 
 
 def test_taylor12_approaches_true_exponential(orc):
-    # config-1 discretisation T_12(A T/64)^64 against exact exp(A T)
+    # the benchmark's discretisation T_12(A T/64)^64 against exact exp(A T)
     nchi, neta, b = 6, 6, 4
     op, blocks = ops.circulant_operator(nchi, neta, b, seed=2, row_sum=8.0)
     A = orc.CSC.from_arrays(*op.to_csc_varmajor())
