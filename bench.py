@@ -374,7 +374,7 @@ def main():
                          "plain FP64 one vector, batched FP32 --nrhs vectors, block4 FP32 "
                          "four vectors (the eigen leg's FP32 phase)")
     ap.add_argument("--batch-kernel", type=int, default=None,
-                    help="--kernel-only batched: use this batch kernel (0..7) instead of "
+                    help="--kernel-only batched: use this batch kernel (0..8) instead of "
                          "the default tile kernel")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure HBM traffic")
@@ -520,7 +520,9 @@ def main():
                           f"{ {16: 9, 32: 5, 64: 3}.get(nb)}>",
                        5: f"k_spmm_mfma_w<{cfg['b']}, float, {nb}, "
                           f"{ {16: 5, 32: 3, 64: 2}.get(nb)}>",
-                       6: f"k_spmm_tile<{cfg['b']}, float, {nb}, "
+                       6: (f"k_spmm_pair<float, {nb}, { {16: 12, 32: 6, 64: 4}.get(nb)}>"
+                           if nb >= 16 else f"k_spmm_tile<{cfg['b']}, float, {nb}, 9>"),
+                       8: f"k_spmm_tile<{cfg['b']}, float, {nb}, "
                           f"{ {8: 9, 16: 9, 32: 3, 64: 3}.get(nb)}>",
                        7: f"k_spmm_quad<float, {nb}, { {8: 9, 16: 9, 32: 5, 64: 3}.get(nb)}>"
                        }.get(bkern, "k_spmm") if cfg["b"] == 8 else {

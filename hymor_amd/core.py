@@ -181,11 +181,12 @@ class DeviceOperator:
     def set_batch_kernel(self, kernel):
         """Batch kernel: 0 vector FMA, 1 matrix cores, 2/3 matrix cores with
         two/one cells per wave, 4/5 matrix cores with grouped operand loads,
-        6 matrix cores on the tile layout (default; also used at 8 doubles
-        per row; at b = 4 the LDS-staged term kernel at 8 doubles per row and
-        the four-cells-per-wave 4x4x4 matrix-core kernel at 16+), 7 the
-        4x4x4 matrix-core form (one cell per wave at b = 4).
-        True/False map to 6/0."""
+        6 matrix cores on the tile layout (default: at b = 8 two cells per
+        product at 16+ doubles per row, one cell per wave at 8; at b = 4 the
+        four-cells-per-wave 4x4x4 matrix-core kernel at 16+ and the
+        LDS-staged term kernel at 8), 7 the 4x4x4 matrix-core form (one cell
+        per wave at b = 4), 8 one cell per wave on the tile layout (6 at
+        b = 4).  True/False map to 6/0."""
         if kernel is True:
             kernel = 6
         elif kernel is False:

@@ -177,6 +177,45 @@ void bsr_transpose_map(int64_t ncell, const std::vector<int64_t> &ptr,
         }
 }
 
+void bsr_pair_map(int64_t ncell, const std::vector<int64_t> &ptr,
+                  const std::vector<int32_t> &col, std::vector<int64_t> &pptr,
+                  std::vector<int32_t> &pcol, std::vector<int32_t> &pslot) {
+    const int64_t npair = (ncell + 1) / 2;
+    pptr.assign((size_t)npair + 1, 0);
+    pcol.clear();
+    pslot.clear();
+    struct Ent {
+        int32_t c, ta, tb;
+    };
+    std::vector<Ent> u;
+    for (int64_t p = 0; p < npair; ++p) {
+        u.clear();
+        for (int side = 0; side < 2; ++side) {
+            const int64_t cell = 2 * p + side;
+            if (cell >= ncell) break;
+            const int64_t k0 = ptr[(size_t)cell], k1 = ptr[(size_t)cell + 1];
+            if (k1 - k0 >= kPairNone) throw std::invalid_argument("block row too long for the pair map");
+            for (int64_t k = k0; k < k1; ++k) {
+                const int32_t c = col[(size_t)k], t = (int32_t)(k - k0);
+                bool found = false;
+                for (Ent &e : u)
+                    if (e.c == c && (side == 0 ? e.ta : e.tb) == kPairNone) {
+                        (side == 0 ? e.ta : e.tb) = t;
+                        found = true;
+                        break;
+                    }
+                if (!found) u.push_back(side == 0 ? Ent{c, t, kPairNone} : Ent{c, kPairNone, t});
+            }
+        }
+        std::stable_sort(u.begin(), u.end(), [](const Ent &a, const Ent &b2) { return a.c < b2.c; });
+        for (const Ent &e : u) {
+            pcol.push_back(e.c);
+            pslot.push_back((int32_t)((uint32_t)e.ta | ((uint32_t)e.tb << 16)));
+        }
+        pptr[(size_t)p + 1] = (int64_t)pcol.size();
+    }
+}
+
 void bsr_to_sliced_map(int b, int64_t ncell, const std::vector<int64_t> &ptr,
                        const std::vector<int32_t> &col, const std::vector<int64_t> *src,
                        HostSliced &s, std::vector<int64_t> &pad_src) {

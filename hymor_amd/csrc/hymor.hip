@@ -57,9 +57,10 @@ struct TermTuning {
     int spmm_mfma = 6;  // batches of >= 16 doubles per row: 0 vector-FMA k_spmm,
                         // 1 k_spmm_mfma, 2/3 k_spmm_mfma_mc (2 or 1 cells per wave),
                         // 4/5 k_spmm_mfma_w (large / small load groups),
-                        // 6 k_spmm_tile (tile layout, also at 8 doubles per row;
-                        // b = 4: k_term_lds at 8, k_spmm_quad4c at 16+),
-                        // 7 k_spmm_quad (4x4x4 matrix cores; b = 4: k_spmm_quad4)
+                        // 6 tile layout: b = 8 k_spmm_pair at 16+, k_spmm_tile at 8;
+                        // b = 4 k_spmm_quad4c at 16+, k_term_lds at 8,
+                        // 7 k_spmm_quad (4x4x4 matrix cores; b = 4: k_spmm_quad4),
+                        // 8 k_spmm_tile (one cell per wave, tile layout)
 };
 
 struct DevSliced {
@@ -73,6 +74,10 @@ struct DevSliced {
     void *val32 = nullptr;  // FP32 values (an FP32 upload, or cast from val64)
     void *val_t = nullptr;  // the same values in k_spmm_tile's order (made on first use)
     void *val_q = nullptr;  // ... in k_spmm_quad's order (b = 8, made on first use)
+    // b = 8: block columns of cell pairs for k_spmm_pair (bsr_pair_map)
+    int64_t npair = 0;
+    int64_t *pptr = nullptr;
+    int32_t *pcol = nullptr, *pslot = nullptr;
 };
 
 struct DevExtra {
@@ -223,6 +228,15 @@ void build_sliced(hymor_op *op, int b, int64_t ncell, const std::vector<int64_t>
     d.nnzb_pad = s.slice_ptr[s.nslice];
     d.slice_ptr = upload(op, s.slice_ptr);
     d.col = upload(op, s.col);
+    if (b == 8) {
+        std::vector<int64_t> pptr;
+        std::vector<int32_t> pcol, pslot;
+        bsr_pair_map(ncell, ptr, col, pptr, pcol, pslot);
+        d.npair = (int64_t)pptr.size() - 1;
+        d.pptr = upload(op, pptr);
+        d.pcol = upload(op, pcol);
+        d.pslot = upload(op, pslot);
+    }
     const int64_t total = d.nnzb_pad * b * b;
     DevTemp dsrc(pad_src.size() * sizeof(int64_t));
     if (!pad_src.empty())
@@ -554,6 +568,21 @@ void launch_tile_t(const DevSliced &s, hipStream_t st, const double *x,
 }
 
 template <typename VT, int R>
+void launch_pair_t(const DevSliced &s, hipStream_t st, const double *x,
+                   double *tout, const double *vin, double *vout, double c1, double c2) {
+    if (!s.val_t || !s.pptr) throw ArgError("pair layout missing (internal)");
+    // column groups: a 3x3-stencil pair's 12 columns in one group at 16
+    // vectors, 6 and 4 at 32 and 64 (C5: 2-11 % faster than the
+    // neighbouring choices measured)
+    constexpr int W = R <= 16 ? 12 : (R == 32 ? 6 : 4);
+    const int wpb = 4;
+    const int64_t grid = (s.npair + wpb - 1) / wpb;
+    hipLaunchKernelGGL((k_spmm_pair<VT, R, W>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
+                       s.slice_ptr, s.pptr, s.pcol, s.pslot, (const VT *)s.val_t, s.ncell, x,
+                       tout, vin, vout, c1, c2);
+}
+
+template <typename VT, int R>
 void launch_quad_t(const DevSliced &s, hipStream_t st, const double *x,
                    double *tout, const double *vin, double *vout, double c1, double c2) {
     if (!s.val_q) throw ArgError("quad-ordered values missing (internal)");
@@ -592,17 +621,19 @@ void launch_block_part(const DevSliced &s, int nv, hipStream_t st, const double 
                        double *tout, const double *vin, double *vout, double c1,
                        double c2) {
     if constexpr (B == 4) {
+        // (batch kernel 8, one cell per wave, is 6 at b = 4)
+        const bool dflt = s.tune.spmm_mfma == 6 || s.tune.spmm_mfma == 8;
         // 8 doubles per row at b = 4: the LDS-staged vector-FMA term kernel
         // beats the matrix-core ones (whose one-cell waves do too little
         // work at b = 4); the batch order is row-major at b = 4 either way
-        if (nv == 8 && s.tune.spmm_mfma == 6) {
+        if (nv == 8 && dflt) {
             launch_term_t<B, VT, 8>(s, st, x, tout, vin, vout, c1, c2);
             return;
         }
         // 16+ doubles per row: four cells per wave on the 4x4x4 matrix
         // cores; slot groups of 9 at 16 doubles, 3 at 32 and 64 (C4: 2-10 %
         // faster than groups of 5 / 2 there, 12-15 % slower at 16)
-        if (nv >= 16 && s.tune.spmm_mfma == 6) {
+        if (nv >= 16 && dflt) {
             switch (nv) {
             case 16: launch_quad4c_t<VT, 16, 9>(s, st, x, tout, vin, vout, c1, c2); return;
             case 32: launch_quad4c_t<VT, 32, 3>(s, st, x, tout, vin, vout, c1, c2); return;
@@ -621,6 +652,15 @@ void launch_block_part(const DevSliced &s, int nv, hipStream_t st, const double 
         }
     }
     if constexpr (B == 8) {
+        // 16+ vectors under the default: two cells per matrix-core product
+        if (nv >= 16 && s.tune.spmm_mfma == 6) {
+            switch (nv) {
+            case 16: launch_pair_t<VT, 16>(s, st, x, tout, vin, vout, c1, c2); return;
+            case 32: launch_pair_t<VT, 32>(s, st, x, tout, vin, vout, c1, c2); return;
+            case 64: launch_pair_t<VT, 64>(s, st, x, tout, vin, vout, c1, c2); return;
+            default: throw ArgError("unsupported vector width");
+            }
+        }
         if (nv >= 8 && s.tune.spmm_mfma == 7) {
             switch (nv) {
             case 8: launch_quad_t<VT, 8>(s, st, x, tout, vin, vout, c1, c2); return;
@@ -1415,7 +1455,7 @@ int hymor_op_set_graphs(hymor_op *op, int enable) {
 int hymor_op_set_batch_kernel(hymor_op *op, int use_mfma) {
     return guarded([&] {
         ARG_CHECK(op, "null operator");
-        ARG_CHECK(use_mfma >= 0 && use_mfma <= 7, "batch kernel must be 0..7");
+        ARG_CHECK(use_mfma >= 0 && use_mfma <= 8, "batch kernel must be 0..8");
         op->fwd.tune.spmm_mfma = use_mfma;
         op->adj.tune.spmm_mfma = op->fwd.tune.spmm_mfma;
         drop_graph(op);
@@ -1434,7 +1474,7 @@ int hymor_op_autotune_batch(hymor_op *op, int nv, int reps, int *best_kernel, do
         HIP_TRY(hipEventCreate(&e1));
         double best = 1e300;
         int pick = op->fwd.tune.spmm_mfma;
-        for (int kern = 0; kern <= 7; ++kern) {
+        for (int kern = 0; kern <= 8; ++kern) {
             op->fwd.tune.spmm_mfma = kern;
             if (kern >= 6) ensure_tile(op);
             launch_term(op, false, nv, op->stream, op->ws[4], op->ws[5], op->ws[2], op->ws[2], 0.5,

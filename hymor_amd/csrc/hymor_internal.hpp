@@ -79,4 +79,16 @@ void bsr_to_sliced_map(int b, int64_t ncell, const std::vector<int64_t> &ptr,
 // chunk: 2 for FP64 storage, 4 for FP32).
 void interleave_lanes(HostSliced &s, int vpl);
 
+// Block columns of cell pairs (2p, 2p + 1), for the kernel that puts both
+// cells of a pair into one matrix-core product: pair p lists the union of
+// its two cells' block columns, ascending, in pcol[pptr[p] .. pptr[p+1]),
+// and pslot holds each column's slot in the two cells' block rows (the
+// position k - ptr[cell] of the block in the cell's row, which is its slot
+// in the sliced layout): bits 0-15 cell 2p, bits 16-31 cell 2p + 1,
+// kPairNone where the cell has no block in that column.
+constexpr int32_t kPairNone = 0xFFFF;
+void bsr_pair_map(int64_t ncell, const std::vector<int64_t> &ptr,
+                  const std::vector<int32_t> &col, std::vector<int64_t> &pptr,
+                  std::vector<int32_t> &pcol, std::vector<int32_t> &pslot);
+
 }  // namespace hymor

@@ -1522,6 +1522,131 @@ __global__ __launch_bounds__(256) void k_spmm_tile(
     }
 }
 
+// k_spmm_tile for b = 8 with the two cells of a pair (2p, 2p + 1) in one
+// wave: rows 0-7 of each v_mfma_f64_16x16x4f64 are cell 2p's block rows,
+// rows 8-15 cell 2p + 1's (k_spmm_tile leaves them as padding).  One
+// product (per k-chunk and 16-vector tile) serves a block column of either
+// cell: the pair's columns are the union of its cells' (bsr_pair_map;
+// 12 for two neighbours of a 3x3 stencil, against 18 products and 18 x
+// loads for the two cells apart), and a cell without a block in a column
+// contributes zero rows without loading values.  Lane (n, kk): A operand
+// row n of the pair (cell n / 8, block row n % 8), B operand x (variable
+// 4 kc + kk, vector 16 nt + n) of the column, D rows kk, 4 + kk (cell 2p)
+// and 8 + kk, 12 + kk (cell 2p + 1).  Summation order: the pair's columns
+// ascending, k-chunks ascending.
+template <typename VT, int R, int W>
+__global__ __launch_bounds__(256) void k_spmm_pair(
+    const int64_t *__restrict__ slice_ptr, const int64_t *__restrict__ pptr,
+    const int32_t *__restrict__ pcol, const int32_t *__restrict__ pslot,
+    const VT *__restrict__ valt, int64_t ncell, const double *__restrict__ x,
+    double *__restrict__ tout, const double *vin, double *vout, double c1, double c2) {
+    constexpr int B = 8, CPS = 8, BB = 64, KC = 2, WN = 16;
+    constexpr int NT = R / WN;
+    typedef typename TileVals<VT, KC>::type AT;
+    typedef d2 XT;
+    static_assert(R == 16 || R == 32 || R == 64, "width 16..64");
+    static_assert(W <= 64, "one column per lane");
+    const int lane = threadIdx.x & 63;
+    const int64_t wpb = blockDim.x >> 6;
+    const int64_t pair = (int64_t)__builtin_amdgcn_readfirstlane(
+        (int)(xcd_tile(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6)));
+    const int64_t cell0 = 2 * pair;
+    if (cell0 >= ncell) return;
+    const bool has1 = cell0 + 1 < ncell;
+    const int n = lane & 15, kk = lane >> 4;
+    const int half = n >> 3;                     // the cell whose row this lane feeds
+    const int mr = n & 7;                        // its block row
+    const int64_t slice = cell0 / CPS;           // a pair never straddles slices
+    const int q = (int)(cell0 - slice * CPS) + half;
+    const int64_t p0 = slice_ptr[slice];
+    const int64_t e0 = pptr[pair], e1 = pptr[pair + 1];
+    const int ncol = (int)(e1 - e0);
+    const int la = (mr * 4 + kk) * KC;           // lane's value offset in a block
+    const int lx = (kk * NT * WN + n) * KC;      // lane's x offset in a cell's batch
+    const double *pv = vout ? vin : x;
+    // epilogue operands of both cells (the second cell's only when present)
+    const int64_t cb0 = cell0 * B * R, cb1 = (has1 ? cell0 + 1 : cell0) * B * R;
+    XT pre0[NT], pre1[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        const int64_t o = (int64_t)(kk * NT + nt) * WN * KC + n * KC;
+        pre0[nt] = *reinterpret_cast<const XT *>(pv + cb0 + o);
+        pre1[nt] = *reinterpret_cast<const XT *>(pv + cb1 + o);
+    }
+    v4d acc[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[nt] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int t0 = 0; t0 < ncol; t0 += W) {
+        const int tl = t0 + (lane < W ? lane : 0);
+        const int64_t ei = e0 + (tl < ncol ? tl : ncol - 1);
+        const int32_t cl = pcol[ei], sl = pslot[ei];
+        int32_t c[W], sp[W];
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            c[u] = __builtin_amdgcn_readlane(cl, u);
+            sp[u] = __builtin_amdgcn_readlane(sl, u);
+        }
+        AT araw[W];
+        XT bx[W][NT];
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            // this lane's cell's slot for the column (none: zero rows)
+            const int t = half ? (int)(((uint32_t)sp[u] >> 16) & 0xFFFF) : (sp[u] & 0xFFFF);
+            const bool have = t != 0xFFFF && t0 + u < ncol;
+            AT a{};
+            if (have)
+                a = __builtin_nontemporal_load(reinterpret_cast<const AT *>(
+                    valt + (p0 + (int64_t)t * CPS + q) * BB + la));
+            araw[u] = a;
+            const double *xp = x + (int64_t)c[u] * B * R + lx;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                bx[u][nt] = *reinterpret_cast<const XT *>(xp + (int64_t)nt * WN * KC);
+        }
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) {
+                const double a = lane_elem(araw[u], kc);
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, lane_elem(bx[u][nt], kc),
+                                                                   acc[nt], 0, 0, 0);
+            }
+        }
+    }
+    // vout first (one wait for the epilogue operands), then tout
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (h == 1 && !has1) break;
+        const int64_t cb = h ? cb1 : cb0;
+        if (vout) {
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const XT pv2 = h ? pre1[nt] : pre0[nt];
+                XT vv;
+                vv[0] = fma(c2, c1 * acc[nt][2 * h], pv2[0]);
+                vv[1] = fma(c2, c1 * acc[nt][2 * h + 1], pv2[1]);
+                *reinterpret_cast<XT *>(vout + cb + (int64_t)(kk * NT + nt) * WN * KC + n * KC) = vv;
+            }
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (h == 1 && !has1) break;
+        const int64_t cb = h ? cb1 : cb0;
+        if (tout) {
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                XT tv;
+                tv[0] = c1 * acc[nt][2 * h];
+                tv[1] = c1 * acc[nt][2 * h + 1];
+                *reinterpret_cast<XT *>(tout + cb + (int64_t)(kk * NT + nt) * WN * KC + n * KC) = tv;
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------- block Gram-Schmidt
 // Partial products of V^T W for a block of s <= 16 new Krylov vectors W
 // against ncol basis columns V, on the FP64 matrix cores: one

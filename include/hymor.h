@@ -214,19 +214,20 @@ int hymor_op_set_graphs(hymor_op *op, int enable);
  * 1 FP64 matrix cores (k_spmm_mfma), 2 / 3 matrix cores with two / one
  * cells per wave and prefetched epilogue operands (k_spmm_mfma_mc),
  * 4 / 5 matrix cores with all operand loads of a large / small group of
- * slots issued together (k_spmm_mfma_w), 6 matrix cores with the batch
- * held in the operand ("tile") order, 16-byte accesses per lane
- * (k_spmm_tile, the default; it also serves 8 doubles per row), 7 the
- * 4x4x4 FP64 matrix cores with no idle rows: at b = 8 the tile order
- * (k_spmm_quad), at b = 4 the row-major batch and the single-vector value
- * order (k_spmm_quad4, one cell per wave).  Under the default (6), b = 4
- * batches run other kernels (one-cell waves do too little work at b = 4):
- * 8 doubles per row the LDS-staged vector-FMA term kernel (k_term_lds),
- * 16+ the 4x4x4 matrix cores with four cells per wave, one cell per
- * product block (k_spmm_quad4c).  All accumulate in FP64. */
+ * slots issued together (k_spmm_mfma_w), 6 (the default) matrix cores with
+ * the batch held in the operand ("tile") order, 16-byte accesses per lane:
+ * at b = 8 two cells per 16x16x4 product, rows 8-15 carrying the second
+ * cell of a pair (k_spmm_pair; 8 doubles per row: k_spmm_tile), at b = 4
+ * the 4x4x4 FP64 matrix cores with four cells per wave, one cell per
+ * product block (k_spmm_quad4c; 8 doubles per row: the LDS-staged
+ * vector-FMA term kernel k_term_lds), 7 the 4x4x4 FP64 matrix cores with
+ * no idle rows: at b = 8 the tile order (k_spmm_quad), at b = 4 the
+ * row-major batch and the single-vector value order (k_spmm_quad4, one
+ * cell per wave), 8 one cell per wave on the 16x16x4 products (k_spmm_tile;
+ * the same as 6 at b = 4).  All accumulate in FP64. */
 int hymor_op_set_batch_kernel(hymor_op *op, int use_mfma);
 
-/* Times the eight batch kernels at width nv (8, 16, 32 or 64) and keeps the
+/* Times the nine batch kernels at width nv (8, 16, 32 or 64) and keeps the
  * fastest; returns its id and ms per launch. */
 int hymor_op_autotune_batch(hymor_op *op, int nv, int reps, int *best_kernel, double *best_ms);
 

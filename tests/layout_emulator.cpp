@@ -769,3 +769,115 @@ extern "C" int emu_extra_csc(int64_t n, const int64_t *colptr, const int64_t *ro
     }
     return g_oob;
 }
+
+// k_spmm_pair (b = 8, two cells per product) replayed on the CPU: the pair
+// map of the library's host code (bsr_pair_map), tile-ordered values and
+// vectors, lane operand fetches (A row n of the pair: cell n / 8, block row
+// n % 8, zero where that cell has no block in the column; B as
+// k_spmm_tile), D rows 0-7 to the pair's first cell and 8-15 to its second,
+// slot groups of W columns.
+extern "C" int emu_spmm_pair_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                                 const double *nzval, int64_t ncell, int nrhs, int R, int W,
+                                 int vpl, const double *X_vm, double *Y_vm) {
+    g_oob = 0;
+    const int b = 8;
+    HostBSR fwd, adj;
+    HostExtra xf, xa;
+    csc_to_blocks(n, colptr, rowval, nzval, 1, false, 1.0, b, ncell, fwd, adj, xf, xa);
+    if (!xf.rows.empty()) return -1;
+    if (R < 16 || W < 1 || W > 64) return -2;
+    HostSliced s;
+    bsr_to_sliced(fwd, s);
+    interleave_lanes(s, vpl);
+    std::vector<int64_t> pptr;
+    std::vector<int32_t> pcol, pslot;
+    bsr_pair_map(ncell, fwd.ptr, fwd.col, pptr, pcol, pslot);
+    const int64_t nflow = (int64_t)b * ncell;
+    std::vector<double> x((size_t)n * R, -1e300), y((size_t)n * R, 0.0);
+    for (int64_t i = 0; i < n * R; ++i) {
+        const int64_t row = i / R, rem = i % R;
+        const int64_t o = tile_pos_h(row, (int)rem, b, R);
+        if (o < 0 || o >= (int64_t)x.size()) { ++g_oob; continue; }
+        if (rem >= nrhs) { x[(size_t)o] = 0.0; continue; }
+        x[(size_t)o] = X_vm[rem * n + (row % b) * ncell + row / b];
+    }
+    const int CPS = 8, BB = 64, KC = 2, WN = 16, NT = R / WN;
+    const int64_t slot_len = (int64_t)CPS * BB;
+    const int64_t total = (int64_t)s.val.size();
+    std::vector<double> valt((size_t)total, 0.0);
+    for (int64_t i = 0; i < total; ++i) {
+        const int64_t slot = i / slot_len;
+        const int within = (int)(i - slot * slot_len);
+        const int q = within / BB, rem = within - q * BB;
+        const int kc = rem % KC, tmp = rem / KC;
+        const int kk = tmp & 3, m = tmp >> 2;
+        const int k = kk + 4 * kc;
+        valt[(size_t)i] = at(s.val, slot * slot_len + ((int64_t)(k / vpl) * 64 + q * b + m) * vpl + k % vpl);
+    }
+    const int64_t npair = (ncell + 1) / 2;
+    if ((int64_t)pptr.size() != npair + 1) return -3;
+    for (int64_t pair = 0; pair < npair; ++pair) {
+        const int64_t cell0 = 2 * pair;
+        const bool has1 = cell0 + 1 < ncell;
+        const int64_t slice = cell0 / CPS;
+        const int64_t p0 = at(s.slice_ptr, slice);
+        const int64_t e0 = at(pptr, pair), e1 = at(pptr, pair + 1);
+        const int ncol = (int)(e1 - e0);
+        std::vector<double> D((size_t)NT * 256, 0.0);
+        for (int t0 = 0; t0 < ncol; t0 += W) {
+            int32_t cl[64], sl[64];
+            for (int lane = 0; lane < 64; ++lane) {
+                const int tl = t0 + (lane < W ? lane : 0);
+                const int64_t ei = e0 + (tl < ncol ? tl : ncol - 1);
+                cl[lane] = at(pcol, ei);
+                sl[lane] = at(pslot, ei);
+            }
+            for (int u = 0; u < W; ++u) {
+                const int64_t c = cl[u];
+                if (c < 0 || c >= ncell) { ++g_oob; continue; }
+                double opa[64][2], opb[4][64][2];
+                for (int lane = 0; lane < 64; ++lane) {
+                    const int nn = lane & 15, kk = lane >> 4, half = nn >> 3, mr = nn & 7;
+                    const int q = (int)(cell0 - slice * CPS) + half;
+                    const int t = half ? (int)(((uint32_t)sl[u] >> 16) & 0xFFFF) : (sl[u] & 0xFFFF);
+                    const bool have = t != 0xFFFF && t0 + u < ncol;
+                    for (int kc = 0; kc < KC; ++kc) {
+                        opa[lane][kc] = have ? at(valt, (p0 + (int64_t)t * CPS + q) * BB + (mr * 4 + kk) * KC + kc) : 0.0;
+                        for (int nt = 0; nt < NT; ++nt)
+                            opb[nt][lane][kc] = at(x, c * b * R + ((int64_t)(kk * NT + nt) * WN + nn) * KC + kc);
+                    }
+                }
+                for (int kc = 0; kc < KC; ++kc)
+                    for (int nt = 0; nt < NT; ++nt)
+                        for (int mm = 0; mm < 16; ++mm)
+                            for (int nn = 0; nn < 16; ++nn) {
+                                double sacc = 0.0;
+                                for (int k = 0; k < 4; ++k)
+                                    sacc += opa[mm + 16 * k][kc] * opb[nt][nn + 16 * k][kc];
+                                D[(size_t)nt * 256 + mm * 16 + nn] += sacc;
+                            }
+            }
+        }
+        for (int lane = 0; lane < 64; ++lane) {
+            const int nn = lane & 15, kk = lane >> 4;
+            for (int h = 0; h < 2; ++h) {
+                if (h == 1 && !has1) continue;
+                const int64_t cb = (cell0 + h) * b * R;
+                for (int nt = 0; nt < NT; ++nt)
+                    for (int r = 0; r < KC; ++r) {
+                        // register 2h + r of lane n + 16 kk holds D[kk + 4 (2h + r)][n]
+                        const double v = D[(size_t)nt * 256 + (kk + 4 * (2 * h + r)) * 16 + nn];
+                        const int64_t o = cb + ((int64_t)(kk * NT + nt) * WN + nn) * KC + r;
+                        if (o < 0 || o >= (int64_t)y.size()) ++g_oob;
+                        else y[(size_t)o] = v;
+                    }
+            }
+        }
+    }
+    for (int64_t i = 0; i < n * nrhs; ++i) {
+        const int64_t rhs = i / n, vm = i % n;
+        const int64_t row = (vm % ncell) * b + vm / ncell;
+        Y_vm[rhs * n + vm] = at(y, tile_pos_h(row, (int)rhs, b, R));
+    }
+    return g_oob;
+}

@@ -88,7 +88,7 @@ def test_batched_f32_and_device_path(orc):
     assert 1e-12 < relerr(Y[:, 5], ref) <= 2e-6
 
 
-@pytest.mark.parametrize("mfma", [0, 1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("mfma", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("b,nrhs,prec", [(8, 16, "f64"), (8, 32, "f32"), (4, 16, "f32"),
                                          (4, 64, "f64"), (8, 64, "f32"), (8, 16, "f32")])
 def test_batch_kernels_agree_with_oracle(orc, mfma, b, nrhs, prec):
@@ -102,7 +102,7 @@ def test_batch_kernels_agree_with_oracle(orc, mfma, b, nrhs, prec):
         Y = dev.expm_taylor_batched(X, order=5, nsub=3, dt=0.1)
         if nrhs >= 16:
             kern, ms = dev.autotune_batch(nv=nrhs, reps=2)
-            assert kern in range(8) and ms > 0
+            assert kern in range(9) and ms > 0
             Y2 = dev.expm_taylor_batched(X, order=5, nsub=3, dt=0.1)
             assert relerr(Y2, Y) <= 1e-13
     tol = 1e-12 if prec == "f64" else 2e-6
@@ -110,7 +110,7 @@ def test_batch_kernels_agree_with_oracle(orc, mfma, b, nrhs, prec):
         assert relerr(Y[:, j], orc.power_method(A.scaled(0.1), X[:, j], 3)) <= tol
 
 
-@pytest.mark.parametrize("mfma", [1, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("mfma", [1, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("b,nrhs", [(4, 16), (8, 32), (8, 64), (8, 8), (4, 8)])
 def test_batch_kernels_wide_rows(orc, mfma, b, nrhs):
     """Batch kernels on rows of up to 20 blocks with ragged widths inside a

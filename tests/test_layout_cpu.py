@@ -286,3 +286,30 @@ def test_emulated_long_extra_rows(emu, NV, NW, cplx):
     got = y.view(np.complex128) if cplx else y
     assert np.abs(want).max() > 0
     assert np.allclose(got, want, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("R,W,nrhs,vpl", [(16, 12, 16, 4), (16, 5, 11, 2), (32, 6, 32, 4),
+                                          (64, 6, 50, 2)])
+@pytest.mark.parametrize("shape", ["stencil_odd", "wide", "single"])
+def test_emulated_batched_pair(emu, R, W, nrhs, vpl, shape):
+    """Two cells per matrix-core product at b = 8 (pair map of the host code,
+    zero rows where a cell lacks a column, the pair's second cell on rows
+    8-15, slot groups of W) replayed on the CPU with every access checked,
+    against scipy; an odd cell count leaves the last pair with one cell."""
+    emu.emu_spmm_pair_csc.restype = C.c_int
+    emu.emu_spmm_pair_csc.argtypes = [C.c_int64] + [C.c_void_p] * 3 + [
+        C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    if shape == "wide":
+        from test_gpu_parity import random_pattern_operator
+        op = random_pattern_operator(9, 7, 8, 19, seed=R + W)
+    elif shape == "single":
+        op = ops.stencil_operator(1, 1, 8, seed=2)
+    else:
+        op = ops.stencil_operator(9, 7, 8, seed=14)        # 63 cells
+    colptr, rowval, nzval = op.to_csc_varmajor(one_based=True)
+    X = np.asfortranarray(np.random.default_rng(6).standard_normal((op.n, nrhs)))
+    Y = np.zeros_like(X)
+    oob = emu.emu_spmm_pair_csc(op.n, colptr.ctypes.data, rowval.ctypes.data, nzval.ctypes.data,
+                                op.ncell, nrhs, R, W, vpl, X.ctypes.data, Y.ctypes.data)
+    assert oob == 0
+    assert np.allclose(Y, op.to_scipy_varmajor() @ X, rtol=1e-13, atol=1e-13)
