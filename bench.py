@@ -527,8 +527,7 @@ def main():
                        7: f"k_spmm_quad<float, {nb}, { {8: 9, 16: 9, 32: 5, 64: 3}.get(nb)}>"
                        }.get(bkern, "k_spmm") if cfg["b"] == 8 else {
                        # b = 4 under batch kernels 6 and 7
-                       6: ("k_term_lds<4, float, 8, 1, 9, true>" if nb == 8 else
-                           f"k_spmm_quad4c<float, {nb}, { {16: 9, 32: 3, 64: 3}.get(nb)}>"),
+                       6: f"k_spmm_quad4c<float, {nb}, { {8: 9, 16: 9, 32: 3, 64: 3}.get(nb)}>",
                        7: f"k_spmm_quad4<float, {nb}, { {8: 9, 16: 9, 32: 9, 64: 5}.get(nb)}>"
                        }.get(bkern, f"batch kernel {bkern}"),
             "batch_kernel_id": bkern,

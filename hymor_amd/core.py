@@ -183,10 +183,9 @@ class DeviceOperator:
         two/one cells per wave, 4/5 matrix cores with grouped operand loads,
         6 matrix cores on the tile layout (default: at b = 8 two cells per
         product at 16+ doubles per row, one cell per wave at 8; at b = 4 the
-        four-cells-per-wave 4x4x4 matrix-core kernel at 16+ and the
-        LDS-staged term kernel at 8), 7 the 4x4x4 matrix-core form (one cell
-        per wave at b = 4), 8 one cell per wave on the tile layout (6 at
-        b = 4).  True/False map to 6/0."""
+        four-cells-per-wave 4x4x4 matrix-core kernel), 7 the 4x4x4
+        matrix-core form (one cell per wave at b = 4), 8 one cell per wave on
+        the tile layout (6 at b = 4).  True/False map to 6/0."""
         if kernel is True:
             kernel = 6
         elif kernel is False:

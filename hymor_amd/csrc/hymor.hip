@@ -623,18 +623,14 @@ void launch_block_part(const DevSliced &s, int nv, hipStream_t st, const double 
     if constexpr (B == 4) {
         // (batch kernel 8, one cell per wave, is 6 at b = 4)
         const bool dflt = s.tune.spmm_mfma == 6 || s.tune.spmm_mfma == 8;
-        // 8 doubles per row at b = 4: the LDS-staged vector-FMA term kernel
-        // beats the matrix-core ones (whose one-cell waves do too little
-        // work at b = 4); the batch order is row-major at b = 4 either way
-        if (nv == 8 && dflt) {
-            launch_term_t<B, VT, 8>(s, st, x, tout, vin, vout, c1, c2);
-            return;
-        }
-        // 16+ doubles per row: four cells per wave on the 4x4x4 matrix
-        // cores; slot groups of 9 at 16 doubles, 3 at 32 and 64 (C4: 2-10 %
-        // faster than groups of 5 / 2 there, 12-15 % slower at 16)
-        if (nv >= 16 && dflt) {
+        // four cells per wave on the 4x4x4 matrix cores (one-cell waves do
+        // too little work at b = 4); slot groups of 9 at 8 and 16 doubles,
+        // 3 at 32 and 64 (C4: 2-10 % faster than groups of 5 / 2 there,
+        // 12-15 % slower at 16).  At 8 doubles per row this beats the
+        // LDS-staged vector-FMA term kernel by 25-27 %.
+        if (nv >= 8 && dflt) {
             switch (nv) {
+            case 8: launch_quad4c_t<VT, 8, 9>(s, st, x, tout, vin, vout, c1, c2); return;
             case 16: launch_quad4c_t<VT, 16, 9>(s, st, x, tout, vin, vout, c1, c2); return;
             case 32: launch_quad4c_t<VT, 32, 3>(s, st, x, tout, vin, vout, c1, c2); return;
             case 64: launch_quad4c_t<VT, 64, 3>(s, st, x, tout, vin, vout, c1, c2); return;

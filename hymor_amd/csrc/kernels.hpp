@@ -2018,9 +2018,10 @@ __global__ __launch_bounds__(256) void k_spmm_quad4(
 // 64 adjacent values of the slice) and 32 bytes of x per lane and 16
 // vectors.  Lane (lo, blk, hi) = lane % 4, lane / 4 % 4, lane / 16 feeds
 // A = value (row lo, input hi) of cell blk, B = x (input hi of cell blk's
-// block column, vectors 16 i + 4 lo + g) to product g of vector group i,
-// and receives D = row hi, vectors 16 i + 4 lo + g: a lane's four x doubles
-// feed its four products and its four results are adjacent in the row.
+// block column, vectors GW i + VPL lo + g) to product g of vector group i,
+// and receives D = row hi, vectors GW i + VPL lo + g (VPL = 4 vectors per
+// lane and GW = 16 per group at R >= 16, 2 and 8 at R = 8): a lane's VPL x
+// doubles feed its VPL products and its VPL results are adjacent in the row.
 // Slots in groups of W, ascending; the group's 4 W block columns come from
 // one vector load and reach their lanes by ds_bpermute.
 template <typename VT, int R, int W>
@@ -2031,8 +2032,11 @@ __global__ __launch_bounds__(256) void k_spmm_quad4c(
     double c1, double c2) {
     constexpr int B = 4, CPS = 16, BB = 16;
     constexpr int V = Vpl<VT>::value;
-    constexpr int NG = R / 16;
-    static_assert(R == 16 || R == 32 || R == 64, "width 16..64");
+    constexpr int VPL = R >= 16 ? 4 : 2;          // vectors of a lane's x chunk
+    constexpr int GW = 4 * VPL;                   // vectors per group of products
+    constexpr int NG = R / GW;
+    constexpr int H = VPL / 2;                    // 16-byte pieces of a chunk
+    static_assert(R == 8 || R == 16 || R == 32 || R == 64, "width 8..64");
     static_assert(4 * W <= 64, "four columns per slot");
     const int lane = threadIdx.x & 63;
     const int64_t wpb = blockDim.x >> 6;
@@ -2049,38 +2053,38 @@ __global__ __launch_bounds__(256) void k_spmm_quad4c(
     // lane's output (and epilogue operand) offset: cell q0 + blk (clamped for
     // the partial last group, whose extra cells are not written), row hi
     const int64_t mycell = cell0 + blk < ncell ? cell0 + blk : ncell - 1;
-    const int64_t obase = (mycell * B + hi) * R + 4 * lo;
+    const int64_t obase = (mycell * B + hi) * R + VPL * lo;
     const double *pv = vout ? vin : x;
-    d2 pre_v[NG][2];
+    d2 pre_v[NG][H];
 #pragma unroll
     for (int i = 0; i < NG; ++i)
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            pre_v[i][h] = *reinterpret_cast<const d2 *>(pv + obase + 16 * i + 2 * h);
+        for (int h = 0; h < H; ++h) {
+            pre_v[i][h] = *reinterpret_cast<const d2 *>(pv + obase + GW * i + 2 * h);
         }
-    double acc[NG][4];
+    double acc[NG][VPL];
 #pragma unroll
     for (int i = 0; i < NG; ++i)
 #pragma unroll
-        for (int g = 0; g < 4; ++g) acc[i][g] = 0.0;
+        for (int g = 0; g < VPL; ++g) acc[i][g] = 0.0;
     for (int t0 = 0; t0 < width; t0 += W) {
         // lane 4u + b < 4W holds the column of slot t0 + u of cell q0 + b
         const int lu = lane < 4 * W ? lane >> 2 : 0;
         const int tl = t0 + lu < width ? t0 + lu : width - 1;
         const int32_t cl = col[p0 + (int64_t)tl * CPS + q0 + (lane & 3)];
         VT a[W];
-        d2 bx[W][NG][2];
+        d2 bx[W][NG][H];
 #pragma unroll
         for (int u = 0; u < W; ++u) {
             const int t = t0 + u < width ? t0 + u : width - 1;
             a[u] = __builtin_nontemporal_load(val + (p0 + (int64_t)t * CPS) * BB + aoff);
             const int32_t c = __builtin_amdgcn_ds_bpermute(4 * (4 * u + blk), cl);
-            const double *xp = x + ((int64_t)c * B + hi) * R + 4 * lo;
+            const double *xp = x + ((int64_t)c * B + hi) * R + VPL * lo;
 #pragma unroll
             for (int i = 0; i < NG; ++i)
 #pragma unroll
-                for (int h = 0; h < 2; ++h)
-                    bx[u][i][h] = *reinterpret_cast<const d2 *>(xp + 16 * i + 2 * h);
+                for (int h = 0; h < H; ++h)
+                    bx[u][i][h] = *reinterpret_cast<const d2 *>(xp + GW * i + 2 * h);
         }
 #pragma unroll
         for (int u = 0; u < W; ++u) {
@@ -2088,7 +2092,7 @@ __global__ __launch_bounds__(256) void k_spmm_quad4c(
 #pragma unroll
             for (int i = 0; i < NG; ++i)
 #pragma unroll
-                for (int g = 0; g < 4; ++g)
+                for (int g = 0; g < VPL; ++g)
                     acc[i][g] = __builtin_amdgcn_mfma_f64_4x4x4f64(av, bx[u][i][g >> 1][g & 1],
                                                                    acc[i][g], 0, 0, 0);
         }
@@ -2100,23 +2104,23 @@ __global__ __launch_bounds__(256) void k_spmm_quad4c(
 #pragma unroll
         for (int i = 0; i < NG; ++i)
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < H; ++h) {
                 d2 vv;
 #pragma unroll
                 for (int e = 0; e < 2; ++e)
                     vv[e] = fma(c2, c1 * acc[i][2 * h + e], pre_v[i][h][e]);
-                *reinterpret_cast<d2 *>(vout + obase + 16 * i + 2 * h) = vv;
+                *reinterpret_cast<d2 *>(vout + obase + GW * i + 2 * h) = vv;
             }
     }
     if (tout) {
 #pragma unroll
         for (int i = 0; i < NG; ++i)
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < H; ++h) {
                 d2 tv;
 #pragma unroll
                 for (int e = 0; e < 2; ++e) tv[e] = c1 * acc[i][2 * h + e];
-                *reinterpret_cast<d2 *>(tout + obase + 16 * i + 2 * h) = tv;
+                *reinterpret_cast<d2 *>(tout + obase + GW * i + 2 * h) = tv;
             }
     }
 }

@@ -219,8 +219,7 @@ int hymor_op_set_graphs(hymor_op *op, int enable);
  * at b = 8 two cells per 16x16x4 product, rows 8-15 carrying the second
  * cell of a pair (k_spmm_pair; 8 doubles per row: k_spmm_tile), at b = 4
  * the 4x4x4 FP64 matrix cores with four cells per wave, one cell per
- * product block (k_spmm_quad4c; 8 doubles per row: the LDS-staged
- * vector-FMA term kernel k_term_lds), 7 the 4x4x4 FP64 matrix cores with
+ * product block (k_spmm_quad4c), 7 the 4x4x4 FP64 matrix cores with
  * no idle rows: at b = 8 the tile order (k_spmm_quad), at b = 4 the
  * row-major batch and the single-vector value order (k_spmm_quad4, one
  * cell per wave), 8 one cell per wave on the 16x16x4 products (k_spmm_tile;

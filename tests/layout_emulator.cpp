@@ -626,7 +626,7 @@ extern "C" int emu_spmm_quad4c_csc(int64_t n, const int64_t *colptr, const int64
                                    int vpl, const double *X_vm, double *Y_vm) {
     g_oob = 0;
     const int b = 4, CPS = 16, BB = 16;
-    if (R % 16 != 0 || 4 * W > 64 || W < 1) return -2;
+    if ((R != 8 && R % 16 != 0) || 4 * W > 64 || W < 1) return -2;
     HostBSR fwd, adj;
     HostExtra xf, xa;
     csc_to_blocks(n, colptr, rowval, nzval, 1, false, 1.0, b, ncell, fwd, adj, xf, xa);
@@ -640,7 +640,7 @@ extern "C" int emu_spmm_quad4c_csc(int64_t n, const int64_t *colptr, const int64
             const int64_t vm = (row % b) * ncell + row / b;
             x[(size_t)(row * R + j)] = X_vm[j * n + vm];
         }
-    const int NG = R / 16;
+    const int VPL = R >= 16 ? 4 : 2, GW = 4 * VPL, NG = R / GW;
     const int64_t ngrp = (ncell + 3) / 4;
     for (int64_t grp = 0; grp < ngrp; ++grp) {
         const int64_t cell0 = 4 * grp;
@@ -669,11 +669,11 @@ extern "C" int emu_spmm_quad4c_csc(int64_t n, const int64_t *colptr, const int64
                     if (src < 0 || src >= 64) { ++g_oob; continue; }
                     const int64_t c = cl[src];
                     for (int i = 0; i < NG; ++i)
-                        for (int g = 0; g < 4; ++g)
-                            opb[l][4 * i + g] = at(x, (c * b + hi) * R + 4 * lo + 16 * i + g);
+                        for (int g = 0; g < VPL; ++g)
+                            opb[l][4 * i + g] = at(x, (c * b + hi) * R + VPL * lo + GW * i + g);
                 }
                 for (int i = 0; i < NG; ++i)
-                    for (int g = 0; g < 4; ++g)
+                    for (int g = 0; g < VPL; ++g)
                         for (int blk = 0; blk < 4; ++blk)
                             for (int m = 0; m < 4; ++m)
                                 for (int nn = 0; nn < 4; ++nn) {
@@ -688,10 +688,10 @@ extern "C" int emu_spmm_quad4c_csc(int64_t n, const int64_t *colptr, const int64
         for (int l = 0; l < 64; ++l) {
             const int lo = l & 3, blk = (l >> 2) & 3, hi = l >> 4;
             if (cell0 + blk >= ncell) continue;
-            const int64_t obase = ((cell0 + blk) * b + hi) * R + 4 * lo;
+            const int64_t obase = ((cell0 + blk) * b + hi) * R + VPL * lo;
             for (int i = 0; i < NG; ++i)
-                for (int g = 0; g < 4; ++g) {
-                    const int64_t o = obase + 16 * i + g;
+                for (int g = 0; g < VPL; ++g) {
+                    const int64_t o = obase + GW * i + g;
                     if (o < 0 || o >= (int64_t)y.size()) { ++g_oob; continue; }
                     y[(size_t)o] = acc[((size_t)i * 4 + g) * 64 + l];
                 }

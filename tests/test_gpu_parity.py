@@ -400,7 +400,8 @@ def test_lds_staged_small_batches(orc, kind, b, nrhs, waves, nt):
     with DeviceOperator.from_blocks(op) as dev, DeviceOperator.from_blocks(op) as ref:
         dev.set_tuning(3, waves, nt)
         ref.set_tuning(2, waves if waves in (3, 9) else 3, nt)
-        if nrhs >= 8:      # b = 4: 8 doubles per row run k_term_lds under the default
+        if nrhs >= 8:      # b = 4: batch kernel 0 sends 8 doubles per row to the term kernels
+            dev.set_batch_kernel(0)
             ref.set_batch_kernel(0)
         for tr, M in ((False, A), (True, At)):
             got = dev.expm_taylor_batched(X, transpose=tr, **kw)

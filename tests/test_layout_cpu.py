@@ -217,7 +217,8 @@ def test_emulated_batched_quad4(emu, R, nrhs, vpl, wide):
 
 
 @pytest.mark.parametrize("R,W,nrhs,vpl", [(16, 9, 16, 4), (16, 5, 13, 2), (32, 5, 32, 4),
-                                          (32, 4, 27, 2), (64, 2, 64, 4), (64, 1, 40, 2)])
+                                          (32, 4, 27, 2), (64, 2, 64, 4), (64, 1, 40, 2),
+                                          (8, 9, 8, 4), (8, 5, 7, 2)])
 @pytest.mark.parametrize("wide", [False, True])
 def test_emulated_batched_quad4c(emu, R, W, nrhs, vpl, wide):
     """Four cells per wave on the 4x4x4 matrix cores at b = 4 (block = cell,
