@@ -576,7 +576,10 @@ def main():
             "kernel": kname,
             "launch": {"split": tuning[0], "waves": tuning[1], "nontemporal": tuning[2]},
             "autotuner_pick": {"split": tune_pick[0], "waves": tune_pick[1],
-                               "nontemporal": tune_pick[2], "ms": tune_ms},
+                               "nontemporal": tune_pick[2], "ms": tune_ms,
+                               "ms_note": "timed on a powers-form term (t and v written: "
+                                          "four vector passes), kernel_ms on the Horner "
+                                          "step (three)"},
             "kernel_ms": k_ms,
             "algorithmic_bytes_per_launch": k_bytes,
             "read_stream_GBps": read_ceiling["GBps"],
