@@ -369,10 +369,12 @@ def main():
     ap.add_argument("--no-eigen", action="store_true", help="skip the eigenvalue leg")
     ap.add_argument("--no-receptivity", action="store_true",
                     help="skip the C4 receptivity leg (assembled vs factored)")
-    ap.add_argument("--kernel-only", choices=["plain", "batched", "block4"],
+    ap.add_argument("--kernel-only", choices=["plain", "batched", "block4", "receptivity"],
                     help="time only the Taylor-term kernel of one form (for profiler runs): "
                          "plain FP64 one vector, batched FP32 --nrhs vectors, block4 FP32 "
-                         "four vectors (the eigen leg's FP32 phase)")
+                         "four vectors (the eigen leg's FP32 phase), receptivity: the C4 "
+                         "operator extended by 32 wavenumbers, FP64 values, one complex "
+                         "vector (block kernel + extra rows)")
     ap.add_argument("--batch-kernel", type=int, default=None,
                     help="--kernel-only batched: use this batch kernel (0..8) instead of "
                          "the default tile kernel")
@@ -398,9 +400,15 @@ def main():
     op = ops.config_operator(args.config)
     if args.kernel_only:
         prec, nv = {"plain": ("f64", 1), "batched": ("f32", args.nrhs),
-                    "block4": ("f32", 4)}[args.kernel_only]
+                    "block4": ("f32", 4), "receptivity": ("f64", 2)}[args.kernel_only]
+        if args.kernel_only == "receptivity":
+            # the receptivity leg's extended operator, default launch tuning
+            op = ops.receptivity_extension(ops.config_operator("C4"),
+                                           0.1 * 100.0 ** (np.arange(32) / 31.0))
         devk = DeviceOperator.from_blocks(op, value_prec=prec, with_adjoint=False)
-        if nv < 8:
+        if args.kernel_only == "receptivity":
+            pass  # default launch, as in the receptivity leg
+        elif nv < 8:
             devk.set_tuning(*PLAIN_TUNING)
         else:
             devk.set_batch_kernel(args.batch_kernel if args.batch_kernel is not None
