@@ -377,7 +377,7 @@ def main():
                          "vector (block kernel + extra rows)")
     ap.add_argument("--batch-kernel", type=int, default=None,
                     help="--kernel-only batched: use this batch kernel (0..8) instead of "
-                         "the default tile kernel")
+                         "the default (6: the cell-pair kernel at b=8 and 16+ vectors)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc passes that measure HBM traffic")
     args = ap.parse_args()
