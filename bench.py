@@ -288,9 +288,16 @@ def pmc_passes(config, timeout_s=150):
     (FETCH_SIZE, then WRITE_SIZE; they do not fit one pass) over this script
     in --kernel-only plain mode, which also launches the streaming-copy
     kernel k_copy on COPY_MIB MiB.  Runs as child processes before this
-    process touches the GPU.  Returns {counter: {kernel name: mean KiB per
-    dispatch}} or None with the reason."""
+    process touches the GPU.  Returns ({counter: {kernel name: mean KiB per
+    dispatch}}, {counter: exit status of the pass}) or (None, the reason).
+
+    A pass counts only if the profiled program printed its result line (its
+    last action) and the counter CSV exists.  The exit status is reported
+    either way: rocprofv3 itself can end with SIGSEGV after the program has
+    finished and the counters are written, and a nonzero status is then
+    shown, not hidden."""
     out = {}
+    codes = {}
     env = dict(os.environ, TMPDIR="/tmp")
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix=f"hymor_pmc_{counter}_", dir="/tmp")
@@ -307,9 +314,14 @@ def pmc_passes(config, timeout_s=150):
         except (OSError, subprocess.TimeoutExpired) as e:
             return None, f"{counter} pass did not run: {e}"
         files = list(Path(d).rglob("*counter_collection.csv"))
-        # rocprofv3 may end with SIGSEGV after a complete run; the CSV decides
-        if not files or (r.returncode not in (0, 139, -11)):
-            tail = [ln for ln in logf.read_text(errors="replace").splitlines()
+        text = logf.read_text(errors="replace")
+        finished = '"kernel_only"' in text      # the program's result line
+        codes[counter] = r.returncode
+        if r.returncode != 0:
+            log(f"traffic: {counter} pass exited with status {r.returncode} "
+                f"(program {'finished' if finished else 'did not finish'})")
+        if not files or not finished:
+            tail = [ln for ln in text.splitlines()
                     if ln.strip() and not ln.startswith(("W20", "I20"))][-4:]
             log(f"traffic: {counter} pass failed (exit {r.returncode}): " + " | ".join(tail))
             return None, f"{counter} pass failed (exit {r.returncode}): " + " | ".join(tail)[-400:]
@@ -321,7 +333,7 @@ def pmc_passes(config, timeout_s=150):
                 name = row["Kernel_Name"]
                 acc.setdefault(name, []).append(float(row["Counter_Value"]))
         out[counter] = {k: sum(v) / len(v) for k, v in acc.items()}
-    return out, None
+    return out, codes
 
 
 def traffic_from_passes(passes, kernel):
@@ -386,9 +398,13 @@ def main():
               "multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
-    passes, traffic_note = None, "skipped (--no-traffic)"
+    passes, traffic_note, pass_codes = None, "skipped (--no-traffic)", None
     if not args.kernel_only and not args.no_traffic:
-        passes, traffic_note = pmc_passes(args.config)
+        passes, info = pmc_passes(args.config)
+        if passes is None:
+            traffic_note = info
+        else:
+            pass_codes = info
 
     from hymor_amd import operators as ops
     from hymor_amd.core import (DeviceMatrix, DeviceOperator, DeviceVector,
@@ -580,6 +596,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_note": traffic_note,
+            "traffic_pass_exit_status": pass_codes,
             "traffic_counter_scale": factors,
             "kernel": kname,
             "launch": {"split": tuning[0], "waves": tuning[1], "nontemporal": tuning[2]},

@@ -2,7 +2,7 @@
 
 The reference maximises the gain G(v) = (v' S v) / (v' D v) with
 S = T_5(A'dt)^n C T_5(A dt)^n by ARPACK
-(Transient_growth_downstream/TRANSIENT_GROWTH_DOWNSTREAM.jl:214,
+(Transient_growth_downstream/TRANSIENT_GROWTH_DOWNSTREAM.jl:103,
 Freestream_receptivity/FREESTREAM_RECEPTIVITY.jl:107); S is applied by
 EXPONENTIAL! (CONSTRUCT_C.jl:78-111) and, for receptivity, CONSTRUCT_C zeroes
 the first N_set_0 entries before and after (P S P).  Because the discrete

@@ -2,7 +2,7 @@
 
 ``DeviceOperator`` replaces the reference's ``CuSparseMatrixCSC(A*dt)``
 uploads (utils/Stability_analysis/Eigenvalues/EIGENVALUES.jl:139,
-Transient_growth_downstream/TRANSIENT_GROWTH_DOWNSTREAM.jl:189-191) and the
+Transient_growth_downstream/TRANSIENT_GROWTH_DOWNSTREAM.jl:78-80) and the
 Taylor loops that run on them.  Vectors are in the reference's variable-major
 order; complex vectors are numpy complex128 (interleaved re/im, as Julia's
 ComplexF64).

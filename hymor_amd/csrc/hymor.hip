@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -855,12 +856,27 @@ const double *run_taylor(hymor_op *op, bool adj, int order, int64_t nsub, double
     return cur;
 }
 
+// True when a rocprofiler tool library is configured for this process
+// (rocprofv3 sets ROCP_TOOL_LIBRARIES for the program it runs).  Under
+// `rocprofv3 --kernel-trace` the host process took a SIGSEGV inside the
+// tool's frames on the first hipGraphLaunch of a captured Taylor sweep,
+// while the same captured sweeps replay correctly without the tool; with a
+// tool present the sweep is therefore launched kernel by kernel (the same
+// kernels, in the same order, on the same stream).
+bool profiler_tool_present() {
+    static const bool present = [] {
+        const char *v = std::getenv("ROCP_TOOL_LIBRARIES");
+        return v != nullptr && v[0] != '\0';
+    }();
+    return present;
+}
+
 // run_taylor from op->ws[1], through a captured graph when enabled: the
 // sweep touches only internal buffers, so one capture serves every call with
 // the same parameters (the caller-facing permutations stay outside).
 const double *run_taylor_ws1(hymor_op *op, bool adj, int order, int64_t nsub, double dt, int form,
                              int nv) {
-    if (!op->use_graphs || nsub == 0)
+    if (!op->use_graphs || nsub == 0 || profiler_tool_present())
         return run_taylor(op, adj, order, nsub, dt, form, nv, op->ws[1]);
     auto &g = op->graph;
     if (!(g.valid && g.adj == adj && g.order == order && g.nsub == nsub && g.dt == dt &&
@@ -1846,6 +1862,23 @@ double *col(hymor_krylov *K, double *base, int j) {
     return base + (size_t)j * K->n * K->ncomp;
 }
 
+// A vector left with norm nrm after projecting out coefficients of squared
+// sum hsq had norm sqrt(hsq + nrm^2) before (Pythagoras over an orthonormal
+// basis).  It counts as dependent on the basis when projection removed all
+// but a rounding-level part of it: normalising that remainder would add a
+// direction made of rounding errors.
+constexpr double kDepTol = 64.0 * 2.220446049250313e-16;
+
+bool is_dependent(double nrm, double hsq) {
+    return !(nrm > kDepTol * std::sqrt(hsq + nrm * nrm));
+}
+
+void launch_zero(hymor_krylov *K, double *p, int64_t len) {
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((len + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)grid), dim3(256), 0, K->stream, p, len);
+    HIP_TRY(hipGetLastError());
+}
+
 constexpr int kVtwGroups = 512;   // row chunks of the block projection kernel
 
 // h (ncol x s, column-major) = V[:, :ncol]^T W, deterministic
@@ -1985,18 +2018,27 @@ int hymor_krylov_orthonormalize(hymor_krylov *K, int j, void *d_w, double *h_out
         HIP_TRY(hipMemcpyAsync(&nrm2, K->h2 + 2 * (size_t)K->m_max, sizeof(double),
                                hipMemcpyDeviceToHost, K->stream));
         HIP_TRY(hipStreamSynchronize(K->stream));
-        for (int i = 0; i < nc * j; ++i) h_out[i] = h1[i] + h2[i];
+        double hsq = 0.0;
+        for (int i = 0; i < nc * j; ++i) {
+            h_out[i] = h1[i] + h2[i];
+            hsq += h_out[i] * h_out[i];
+        }
         const double nrm = std::sqrt(std::max(nrm2, 0.0));
-        h_out[nc * j] = nrm;
+        const bool dependent = is_dependent(nrm, hsq);
+        h_out[nc * j] = dependent ? 0.0 : nrm;
         if (nc == 2) h_out[nc * j + 1] = 0.0;
-        if (nrm > 0.0) {
+        double *dst = col(K, K->V, j);
+        if (!dependent) {
             const int64_t grid = std::min<int64_t>((len + 255) / 256, 4096);
             hipLaunchKernelGGL(k_scale, dim3((unsigned)grid), dim3(256), 0, K->stream, w, len,
                                1.0 / nrm);
             HIP_TRY(hipGetLastError());
-            if (w != col(K, K->V, j))
-                HIP_TRY(hipMemcpyAsync(col(K, K->V, j), w, (size_t)len * sizeof(double),
+            if (w != dst)
+                HIP_TRY(hipMemcpyAsync(dst, w, (size_t)len * sizeof(double),
                                        hipMemcpyDeviceToDevice, K->stream));
+        } else {
+            // dependent vector: a zero column, reported by a zero norm
+            launch_zero(K, dst, len);
         }
         HIP_TRY(hipStreamSynchronize(K->stream));
     });
@@ -2062,10 +2104,14 @@ int hymor_krylov_orthonormalize_block(hymor_krylov *K, int j, int s, void *d_W, 
                                    hipMemcpyDeviceToHost, K->stream));
             HIP_TRY(hipStreamSynchronize(K->stream));
             for (int c = 0; c < i; ++c) h_out[(size_t)i * ldh + j + c] = h1[c] + h2[c];
+            double hsq = 0.0;
+            for (int c = 0; c < j + i; ++c)
+                hsq += h_out[(size_t)i * ldh + c] * h_out[(size_t)i * ldh + c];
             const double nrm = std::sqrt(std::max(nrm2, 0.0));
-            h_out[(size_t)i * ldh + j + i] = nrm;
+            const bool dependent = is_dependent(nrm, hsq);
+            h_out[(size_t)i * ldh + j + i] = dependent ? 0.0 : nrm;
             double *dst = col(K, K->V, j + i);
-            if (nrm > 0.0) {
+            if (!dependent) {
                 const int64_t grid = std::min<int64_t>((n + 255) / 256, 4096);
                 hipLaunchKernelGGL(k_scale, dim3((unsigned)grid), dim3(256), 0, K->stream, w, n,
                                    1.0 / nrm);
@@ -2075,7 +2121,7 @@ int hymor_krylov_orthonormalize_block(hymor_krylov *K, int j, int s, void *d_W, 
                                            hipMemcpyDeviceToDevice, K->stream));
             } else {
                 // dependent vector: a zero column, reported by a zero norm
-                HIP_TRY(hipMemsetAsync(dst, 0, (size_t)n * sizeof(double), K->stream));
+                launch_zero(K, dst, n);
             }
         }
         HIP_TRY(hipStreamSynchronize(K->stream));

@@ -296,12 +296,19 @@ __device__ __forceinline__ void extra_short_rows(
             acc[0] = fma(val[e], x[buf_pos(c, v0, NV, tile_b, nflow)], acc[0]);
         }
     }
-    if constexpr (CPLX) {
+    // the per-column complex diagonal acts on (re, im) pairs of the row
+    // whether the operator's values are complex (a lane owns a pair) or real
+    // (a lane owns one double of a pair)
+    if constexpr (NV % 2 == 0) {
         if (cdiag && row >= cd0 && row < cd1) {
-            const double dr = cdiag[v0], di = cdiag[v0 + 1];
-            const double xr = x[row * NV + v0], xi = x[row * NV + v0 + 1];
-            acc[0] = fma(dr, xr, fma(-di, xi, acc[0]));
-            acc[1] = fma(dr, xi, fma(di, xr, acc[1]));
+#pragma unroll
+            for (int q = 0; q < NC; ++q) {
+                const int v = v0 + q, vr = v & ~1;
+                const double dr = cdiag[vr], di = cdiag[vr + 1];
+                const double xr = x[row * NV + vr], xi = x[row * NV + vr + 1];
+                acc[q] = (v & 1) ? fma(dr, xi, fma(di, xr, acc[q]))
+                                 : fma(dr, xr, fma(-di, xi, acc[q]));
+            }
         }
     }
 #pragma unroll
@@ -710,6 +717,13 @@ __global__ __launch_bounds__(256) void k_scale(double *__restrict__ x, int64_t l
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len;
          i += (int64_t)gridDim.x * blockDim.x)
         x[i] *= s;
+}
+
+// x[0 : len) = 0 (a kernel, so that it may sit inside a captured graph)
+__global__ __launch_bounds__(256) void k_zero(double *__restrict__ x, int64_t len) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = 0.0;
 }
 
 }  // namespace hymor

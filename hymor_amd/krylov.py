@@ -131,6 +131,15 @@ def _ritz_vectors(K: "DeviceKrylov", Q: np.ndarray, complex_: bool) -> np.ndarra
     return Yt.T
 
 
+def _fresh_direction(K: "DeviceKrylov", j: int, seed: int, tries: int = 3):
+    """Column j <- a random unit vector orthogonal to columns 0..j-1."""
+    for t in range(tries):
+        K.set_column_random(j, seed + 104729 * t)
+        if K.orthonormalize(j, K.column(j))[j] != 0:
+            return
+    raise RuntimeError("no direction orthogonal to the Krylov basis was found")
+
+
 def _select_lm(evals: np.ndarray, k: int) -> float:
     """Magnitude threshold so that the k largest-|lambda| values are kept."""
     mags = np.sort(np.abs(evals))[::-1]
@@ -160,16 +169,24 @@ def krylov_schur(apply_dev, n: int, k: int, m: int, *, complex_: bool = False, t
             K.set_column_random(0, seed)
         else:
             K.set_column(0, np.asarray(v0, dtype=dtype))
-        K.orthonormalize(0, K.column(0))                    # normalise in place
+        if K.orthonormalize(0, K.column(0))[0] == 0:        # normalise in place
+            raise ValueError("the start vector is zero")
         H = np.zeros((m + 1, m), dtype=dtype)
         p = 0
         n_apply = 0
+        n_fresh = 0
         for it in range(1, maxiter + 1):
             for j in range(p, m):
                 apply_dev(K.column(j), scratch.ptr)
                 n_apply += 1
                 h = K.orthonormalize(j + 1, scratch.ptr)
                 H[: j + 2, j] = h[: j + 2]
+                if h[j + 1] == 0:
+                    # V[:, :j+1] spans an invariant subspace (H[j+1, j] = 0
+                    # keeps the Krylov relation exact): continue from a
+                    # random direction orthogonal to it, as ARPACK does
+                    n_fresh += 1
+                    _fresh_direction(K, j + 1, 7919 * (seed + 1) + n_fresh)
             beta = H[m, m - 1]
             Hm = H[:m, :m]
             evals = np.linalg.eigvals(Hm)

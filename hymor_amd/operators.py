@@ -350,7 +350,7 @@ def receptivity_extension(op: BlockOperator, w_infty, seed: int = 0, b_scale: fl
 
 def marching_operator(nchi: int, neta: int, b: int, seed: int = 0, n_lead: int = 12,
                       lead_top: float = 0.55, lead_gap: float = 0.1, bulk=(-3.0, -1.0),
-                      couple: float = 0.15):
+                      couple: float = 0.15, n_shock: int = 0):
     """3x3-stencil block operator with an exactly known spectrum.
 
     Every cell couples only to the neighbours with a larger cell index (east,
@@ -365,7 +365,16 @@ def marching_operator(nchi: int, neta: int, b: int, seed: int = 0, n_lead: int =
     grid whose leading eigenvalue is lead_top - lead_gap * j (j = 0..n_lead-1).
     Returns (op, eigenvalues of A) with the eigenvalues computed from the
     factors (l_i), not from the stored blocks.
+
+    n_shock = nchi appends shock-position DOFs with the band shapes of
+    stencil_operator (A12, A21, A22 dense), kept one-way as well: A12 holds
+    explicit zeros, A21 random values, A22 is upper triangular with its
+    diagonal drawn from `bulk` (explicit zeros below).  The operator is then
+    block lower triangular over [flow; shock], and the spectrum is the flow
+    part's plus A22's diagonal.
     """
+    if n_shock not in (0, nchi):
+        raise ValueError("shock DOFs are one per chi station (n_shock == nchi)")
     rng = np.random.Generator(np.random.PCG64(20_000 + seed))
     ncell = nchi * neta
     ci = np.arange(ncell, dtype=np.int64) % nchi
@@ -396,8 +405,28 @@ def marching_operator(nchi: int, neta: int, b: int, seed: int = 0, n_lead: int =
     bval[diag_pos] = D[rows[diag_pos]]
     brow_ptr = np.zeros(ncell + 1, dtype=np.int64)
     np.cumsum(np.bincount(rows, minlength=ncell), out=brow_ptr[1:])
-    op = BlockOperator(nchi, neta, b, 0, brow_ptr, cols.astype(np.int32), bval)
-    return op, lam.reshape(-1)
+    op = BlockOperator(nchi, neta, b, n_shock, brow_ptr, cols.astype(np.int32), bval)
+    lam = lam.reshape(-1)
+    if n_shock:
+        nf = b * ncell
+        top = (neta - 1) * nchi + np.arange(nchi)
+        top2 = (neta - 2) * nchi + np.arange(nchi) if neta > 1 else top
+        shock = nf + np.arange(nchi)
+        r12 = (top[:, None] * b + np.arange(b)[None, :]).reshape(-1)
+        c21 = np.unique(np.concatenate([(top[:, None] * b + np.arange(b)).reshape(-1),
+                                        (top2[:, None] * b + np.arange(b)).reshape(-1)]))
+        xr = [np.repeat(r12, nchi), np.repeat(shock, c21.shape[0]), np.repeat(shock, nchi)]
+        xc = [np.tile(shock, r12.shape[0]), np.tile(c21, nchi), np.tile(shock, nchi)]
+        v12 = np.zeros(r12.shape[0] * nchi)                       # one-way: explicit zeros
+        v21 = rng.standard_normal(nchi * c21.shape[0]) * (couple / np.sqrt(c21.shape[0]))
+        d22 = rng.uniform(bulk[0], bulk[1], size=nchi)
+        A22 = np.triu(rng.standard_normal((nchi, nchi)) * (couple / np.sqrt(nchi)), 1)
+        A22[np.arange(nchi), np.arange(nchi)] = d22
+        op.xrow = np.concatenate(xr)
+        op.xcol = np.concatenate(xc)
+        op.xval = np.concatenate([v12, v21, A22.reshape(-1)])
+        lam = np.concatenate([lam, d22])
+    return op, lam
 
 
 def circulant_operator(nchi: int, neta: int, b: int, seed: int = 0, row_sum: float | None = None):

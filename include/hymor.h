@@ -68,7 +68,7 @@ const char *hymor_last_error(void);
 
 /* Build a device operator from a CSC matrix in the reference's layout.
  * Replaces `CuSparseMatrixCSC(A * dt)` (EIGENVALUES.jl:139,
- * TRANSIENT_GROWTH_DOWNSTREAM.jl:189-191, FREESTREAM_RECEPTIVITY.jl:83-85).
+ * TRANSIENT_GROWTH_DOWNSTREAM.jl:78-80, FREESTREAM_RECEPTIVITY.jl:83-85).
  *   n, colptr[n+1], rowval[nnz], nzval[nnz] : SparseMatrixCSC fields;
  *       index_base 1 for Julia arrays as-is, 0 for 0-based arrays.
  *   dtype        : HYMOR_F64 (nzval double) or HYMOR_C128 (interleaved).
@@ -78,7 +78,7 @@ const char *hymor_last_error(void);
  *                  b*nchi*neta (shock, freestream DOFs) are "extra".
  *   value_prec   : HYMOR_PREC_F64 or HYMOR_PREC_F32 storage of block values.
  * The transpose (adjoint) operator is built at the same time, replacing
- * `sparse(A_dt')` (TRANSIENT_GROWTH_DOWNSTREAM.jl:186). */
+ * `sparse(A_dt')` (TRANSIENT_GROWTH_DOWNSTREAM.jl:75). */
 int hymor_op_create_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
                         const void *nzval, int index_base, int dtype, double scale,
                         int vars_per_cell, int64_t nchi, int64_t neta,
@@ -154,7 +154,9 @@ int hymor_expm_taylor_batched_dev(hymor_op *op, int transpose, int order, int64_
  * freestream DOFs); column c of X (complex, nrhs <= 32) is evolved under
  * [L B; 0 diag[c] I]: diag holds one complex value per column (-i*w_c), and
  * its conjugate is used for transpose.  By linearity the N columns together
- * carry exactly the extended-operator action.  Synchronous. */
+ * carry exactly the extended-operator action.  op may hold real values
+ * (HYMOR_F64 upload) or complex ones; the vectors are complex either way.
+ * Synchronous. */
 int hymor_expm_taylor_batched_coldiag_dev(hymor_op *op, int transpose, int order, int64_t nsub,
                                           double dt, int form, int64_t nrhs, const double *diag,
                                           int64_t row0, int64_t row1, const void *d_X,
@@ -163,7 +165,7 @@ int hymor_expm_taylor_batched_coldiag_dev(hymor_op *op, int transpose, int order
 /* Chu-norm sandwich y = T_p(A'dt)^n * C * T_p(A dt)^n * x with the zeroing
  * of CONSTRUCT_C (CONSTRUCT_C.jl:33-58 around EXPONENTIAL! :78-111):
  * x[0:n_set0) is zeroed before and y[0:n_set0) after (var-major indices).
- * C is a handle built from R'*M*R (TRANSIENT_GROWTH_DOWNSTREAM.jl:162).
+ * C is a handle built from R'*M*R (TRANSIENT_GROWTH_DOWNSTREAM.jl:51).
  * EXPONENTIAL!'s polynomial is evaluated by Horner's rule (equal to
  * rounding; three vector passes per term instead of four). */
 int hymor_chu_sandwich(hymor_op *A, hymor_op *C, int order, int64_t nsub, double dt,
@@ -207,7 +209,10 @@ int hymor_op_autotune(hymor_op *op, int nv, int reps, double *best_ms);
 /* enable != 0: the Taylor sweep of hymor_expm_taylor(_dev) and the batched
  * forms is captured into a HIP graph on first use and replayed while
  * (transpose, order, nsub, dt, form, width) repeat -- one graph launch
- * instead of order*nsub kernel launches.  Off by default. */
+ * instead of order*nsub kernel launches.  Off by default.  While a
+ * rocprofiler tool is configured for the process (ROCP_TOOL_LIBRARIES set,
+ * as rocprofv3 does) the sweep is launched kernel by kernel instead: the
+ * tool faulted the host inside hipGraphLaunch of such sweeps. */
 int hymor_op_set_graphs(hymor_op *op, int enable);
 
 /* Kernel for batches of 16+ doubles per row: 0 vector FMA (k_spmm),
@@ -282,7 +287,7 @@ int hymor_energy_track(hymor_op *A, const hymor_weights *W, int order, int64_t n
 
 /* ------------------------------------------------------------------------
  * Device-resident Krylov basis.  Replaces the host-side basis ARPACK keeps
- * (Arpack.eigs at EIGENVALUES.jl:159, TRANSIENT_GROWTH_DOWNSTREAM.jl:214,
+ * (Arpack.eigs at EIGENVALUES.jl:159, TRANSIENT_GROWTH_DOWNSTREAM.jl:103,
  * FREESTREAM_RECEPTIVITY.jl:107), whose every matvec crosses PCIe twice
  * (ARNOLDI_POWER_GPU.jl:42,44).  The basis V (n x (m_max+1), column-major,
  * var-major vectors) stays in HBM; the host keeps only the small Hessenberg
@@ -305,7 +310,10 @@ int hymor_krylov_get_columns(hymor_krylov *K, int j0, int ncols, void *out);
  * length n) is orthogonalised against columns 0..j-1, normalised and stored
  * as column j.  h_out receives the j projection coefficients (complex:
  * interleaved) followed by the norm, i.e. column j-1 of the Hessenberg
- * matrix when w = Op(V[:, j-1]). */
+ * matrix when w = Op(V[:, j-1]).  A vector whose norm after projection is
+ * at most 64 * DBL_EPSILON times its norm before (numerically in the span
+ * of the basis) is dependent: column j is set to zero and the norm is
+ * reported as 0. */
 int hymor_krylov_orthonormalize(hymor_krylov *K, int j, void *d_w, double *h_out);
 
 /* Block form of hymor_krylov_orthonormalize for block Krylov methods (no
@@ -316,8 +324,9 @@ int hymor_krylov_orthonormalize(hymor_krylov *K, int j, void *d_w, double *h_out
  * s vectors (V^T W on the FP64 matrix cores), then orthonormalised among
  * themselves in order into columns j..j+s-1.  h_out receives the
  * (j+s) x s coefficient matrix, column-major: W = V[:, :j+s] h_out; a zero
- * diagonal entry marks a dependent vector (its column is zeroed).  W is
- * overwritten.  Real bases only. */
+ * diagonal entry marks a dependent vector (the same relative test as
+ * hymor_krylov_orthonormalize; its column is zeroed).  W is overwritten.
+ * Real bases only. */
 int hymor_krylov_orthonormalize_block(hymor_krylov *K, int j, int s, void *d_W, int64_t ldw,
                                       double *h_out);
 

@@ -63,7 +63,7 @@ void oc_csc_mul_c128(int64_t n, const int64_t *colptr, const int64_t *rowval,
 }
 
 /* Explicit (conjugate) transpose, the reference's sparse(A_dt')
- * (TRANSIENT_GROWTH_DOWNSTREAM.jl:186, FREESTREAM_RECEPTIVITY.jl:80).
+ * (TRANSIENT_GROWTH_DOWNSTREAM.jl:75, FREESTREAM_RECEPTIVITY.jl:80).
  * Output CSC arrays must hold n+1 / nnz entries.  is_complex selects
  * ComplexF64 values; the imaginary part is negated ("'" is the adjoint). */
 void oc_csc_adjoint(int64_t n, const int64_t *colptr, const int64_t *rowval,

@@ -8,7 +8,7 @@ Two layers:
     reference's CPU code path on its own operator format (1-based,
     variable-major Julia CSC):
       - csc_mul        SparseArrays mul!(y, A, x, 1.0, 0.0)  (column scatter)
-      - csc_adjoint    sparse(A_dt')   TRANSIENT_GROWTH_DOWNSTREAM.jl:186
+      - csc_adjoint    sparse(A_dt')   TRANSIENT_GROWTH_DOWNSTREAM.jl:75
       - power_method   POWER_METHOD!   Eigenvalues/ARNOLDI_POWER_GPU.jl:64-82
       - exponential    EXPONENTIAL!    Transient_growth_downstream/CONSTRUCT_C.jl:78-111
       - construct_c    CONSTRUCT_C     CONSTRUCT_C.jl:33-58 (zeroing of x[1:N_set_0])
@@ -306,7 +306,7 @@ def circulant_eigenvalues(blocks, nchi, neta):
 # --------------------------------------------------------------------------
 
 def chu_components(rho0, rho_u0, rho_v0, p0, a0, gam0, prho, prho_u, prho_v, prho_E):
-    """GET_CHU_COMPONENTS (utils/Energy_budgets/GET_CHU_COMPONENTS.jl:1-80):
+    """GET_CHU_COMPONENTS (utils/Energy_budgets/GET_CHU_COMPONENTS.jl:1-62):
     conservative perturbations -> (p', u', v', s'/R), elementwise."""
     u0 = rho_u0 / rho0
     v0 = rho_v0 / rho0
@@ -321,7 +321,7 @@ def chu_components(rho0, rho_u0, rho_v0, p0, a0, gam0, prho, prho_u, prho_v, prh
 
 
 def energy_norm(rho0, rho_u0, rho_v0, p0, a0, gam0, prho, prho_u, prho_v, prho_E):
-    """GET_ENERGY_NORM (utils/Energy_budgets/GET_ENERGY_NORM.jl:1-55):
+    """GET_ENERGY_NORM (utils/Energy_budgets/GET_ENERGY_NORM.jl:1-54):
     acoustic, kinetic and entropic Chu energy densities (real parts)."""
     pp, pu, pv, ps = chu_components(rho0, rho_u0, rho_v0, p0, a0, gam0, prho, prho_u, prho_v,
                                     prho_E)

@@ -3,8 +3,9 @@ action on in seconds) through size-independent properties and single
 oracle SpMVs.
 
 * C3: 400 x 300 cells, b = 4, 400 shock DOFs (N = 480 400): one SpMV
-  against the oracle, the adjoint dot-product identity of the Taylor action
-  and of the Chu sandwich's symmetry, linearity.
+  against the oracle, the adjoint dot-product identity of the Taylor action,
+  linearity.  (The Chu sandwich, transient growth and the gain gradient at
+  this size are in test_gpu_fullsize.py.)
 * C4: 500 x 250 cells, b = 4, 32 freestream wavenumbers
   w_k = 0.1 * 100^((k-1)/31) (N_ext = 564 000 complex): the factored
   receptivity action equals the assembled extended operator's action on the

@@ -50,6 +50,58 @@ def test_block_orthonormalize(n, j, s):
         K.close()
 
 
+@pytest.mark.parametrize("n", [1003, 100003])
+def test_dependent_vectors_are_flagged(n):
+    """A vector inside span(V) (exactly, or up to a rounding-level part) is
+    reported dependent -- norm 0 and a zero column -- by both
+    orthonormalisations; a small but genuine new direction (1e-8 relative)
+    is normalised as usual."""
+    say(f"[dependence] n={n}")
+    r = np.random.default_rng(n)
+    j = 3
+    Q, _ = np.linalg.qr(r.standard_normal((n, j)))
+    coef = np.array([1.0, 2.0, -1.5])
+    cases = [("exact", Q @ coef, True),
+             ("rounding", Q @ coef + 1e-17 * r.standard_normal(n), True),
+             ("small", Q @ coef + 1e-8 * r.standard_normal(n) / np.sqrt(n), False)]
+    for name, w, dep in cases:
+        K = DeviceKrylov(n, j + 2)
+        try:
+            for c in range(j):
+                K.set_column(c, Q[:, c])
+            K.set_column(j + 1, w)
+            K.set_column(j, np.full(n, 7.0))            # stale contents of column j
+            h = K.orthonormalize(j, K.column(j + 1))
+            col = K.get_columns(j, 1)[:, 0]
+            assert np.allclose(h[:j], Q.T @ w, rtol=0, atol=1e-12), name
+            if dep:
+                assert h[j] == 0.0 and not np.any(col), name
+            else:
+                assert h[j] > 0.0 and abs(np.linalg.norm(col) - 1.0) <= 1e-12, name
+                assert np.abs(Q.T @ col).max() <= 1e-12, name
+        finally:
+            K.close()
+    # block form: one column inside span(V), one repeating an earlier column
+    s = 4
+    W0 = r.standard_normal((n, s))
+    W0[:, 1] = Q @ coef
+    W0[:, 3] = 2.0 * W0[:, 0]
+    K = DeviceKrylov(n, j + s + 1)
+    try:
+        for c in range(j):
+            K.set_column(c, Q[:, c])
+        dW = DeviceMatrix.from_host(W0)
+        H = K.orthonormalize_block(j, dW.ptr, n, s)
+        dW.free()
+        d = np.diag(H[j:])
+        assert d[1] == 0.0 and d[3] == 0.0 and d[0] > 0 and d[2] > 0
+        V = K.get_columns(j, s)
+        assert not np.any(V[:, 1]) and not np.any(V[:, 3])
+        assert abs(np.linalg.norm(V[:, 0]) - 1) <= 1e-12 and abs(np.linalg.norm(V[:, 2]) - 1) <= 1e-12
+    finally:
+        K.close()
+
+
 def test_batched_eigs_exact_spectrum():
     """Batched block Krylov-Schur (8 vectors per sweep) on FP64 values, and
     on FP32 values with FP64 refinement, against the exact spectrum of a

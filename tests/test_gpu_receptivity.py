@@ -112,3 +112,49 @@ def test_factorisation_on_host(orc, shock):
     assert relerr(got, orc.power_method(Ao.scaled(0.05), x, 3)) <= 1e-13
     got_t = fr.expm(x, order=5, nsub=3, transpose=True)
     assert relerr(got_t, orc.power_method(orc.csc_adjoint(Ao).scaled(0.05), x, 3)) <= 1e-13
+
+
+class _RealValues(FactoredReceptivity):
+    """[L B; 0 0] uploaded with real values (HYMOR_F64): the per-column -i w
+    diagonal must still act on the complex columns."""
+
+    @staticmethod
+    def _make_device(S, nchi, neta, vars_per_cell, scale, value_prec):
+        from hymor_amd.core import DeviceOperator
+        assert S.imag.nnz == 0 or abs(S.imag).max() == 0
+        return DeviceOperator.from_scipy(sp.csc_matrix(S.real), nchi=nchi, neta=neta,
+                                         vars_per_cell=vars_per_cell, scale=scale,
+                                         value_prec=value_prec)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,shock", [(3, True), (32, False)])
+def test_factored_real_upload_equals_complex_upload(orc, N, shock):
+    """The same per-wavenumber system uploaded as real and as complex values
+    gives the same factored action (forward and adjoint), and both match
+    POWER_METHOD! on the extended operator."""
+    nchi, neta = 8, 6
+    say(f"[factored receptivity, real upload] N={N} shock={shock}")
+    base = ops.stencil_operator(nchi, neta, 4, seed=43, n_shock=nchi if shock else 0,
+                                row_sum=2.0)
+    w = np.linspace(0.2, 5.0, N)
+    A_ = ops.receptivity_extension(base, w).to_scipy_varmajor()
+    n_L, p = base.n, 4 * nchi
+    L, B = split_extended(A_, n_L, p, w)
+    r = np.random.default_rng(50 + N)
+    x = r.standard_normal(A_.shape[0]) + 1j * r.standard_normal(A_.shape[0])
+    Ao = orc.CSC.from_scipy(A_)
+    dt, nsub = 0.05, 4
+    fc = FactoredReceptivity(L, B, w, nchi=nchi, neta=neta, vars_per_cell=4, scale=dt)
+    fr = _RealValues(L, B, w, nchi=nchi, neta=neta, vars_per_cell=4, scale=dt)
+    try:
+        for tr in (False, True):
+            got_r = fr.expm(x, order=5, nsub=nsub, transpose=tr)
+            got_c = fc.expm(x, order=5, nsub=nsub, transpose=tr)
+            M = orc.csc_adjoint(Ao) if tr else Ao
+            ref = orc.power_method(M.scaled(dt), x, nsub)
+            assert relerr(got_r, got_c) <= 1e-14
+            assert relerr(got_r, ref) <= 1e-12
+    finally:
+        fr.close()
+        fc.close()
