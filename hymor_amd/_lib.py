@@ -15,6 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libhymor_hip.so"
 F64, C128 = 0, 1
 PREC_F64, PREC_F32 = 0, 1
 FORM_NORMALISED, FORM_POWERS, FORM_HORNER = 0, 1, 2
+OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_NODEV = 0, -1, -2, -3, -4
 
 _i64 = C.c_int64
 _p = C.c_void_p
@@ -39,6 +40,8 @@ SIGNATURES = {
                                     C.c_int]),
     "hymor_expm_taylor_dev": (C.c_int, [_p, C.c_int, C.c_int, _i64, C.c_double, C.c_int, _p, _p,
                                         C.c_int]),
+    "hymor_expm_taylor_dev_noalloc": (C.c_int, [_p, C.c_int, C.c_int, _i64, C.c_double, C.c_int,
+                                                _p, _p, C.c_int]),
     "hymor_expm_taylor_batched": (C.c_int, [_p, C.c_int, C.c_int, _i64, C.c_double, C.c_int, _i64,
                                             _p, _i64, _p, _i64, C.c_int]),
     "hymor_expm_taylor_batched_dev": (C.c_int, [_p, C.c_int, C.c_int, _i64, C.c_double, C.c_int,

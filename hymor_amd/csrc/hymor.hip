@@ -8,6 +8,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -805,12 +806,17 @@ int pad_width(int64_t valid) {
     throw ArgError("nrhs too large: at most 64 doubles per row (64 real or 32 complex rhs)");
 }
 
+// The four n*nv-double buffers a Taylor sweep works in besides its input.
+struct SweepBufs {
+    double *va, *vb, *t0, *t1;
+};
+
 // T_order(A*dt)^nsub applied to `in` (cell-major, internal buffer, never
-// written).  Returns the buffer that holds the result: one of ws[2..5], or
-// `in` if nsub == 0.
-const double *run_taylor(hymor_op *op, bool adj, int order, int64_t nsub, double dt, int form,
-                         int nv, const double *in) {
-    double *va = op->ws[2], *vb = op->ws[3], *t0 = op->ws[4], *t1 = op->ws[5];
+// written), working in `b`.  Returns the buffer that holds the result: one
+// of b's four, or `in` if nsub == 0.
+const double *run_taylor_in(hymor_op *op, const SweepBufs &b, bool adj, int order, int64_t nsub,
+                            double dt, int form, int nv, const double *in) {
+    double *va = b.va, *vb = b.vb, *t0 = b.t0, *t1 = b.t1;
     const double *cur = in;
     if (form == HYMOR_FORM_HORNER) {
         // w = v; w = v + (dt/j) A w for j = order, ..., 1.  Each term reads
@@ -854,6 +860,13 @@ const double *run_taylor(hymor_op *op, bool adj, int order, int64_t nsub, double
         cur = nxt;
     }
     return cur;
+}
+
+// run_taylor_in on the operator's own workspace (ws[2..5])
+const double *run_taylor(hymor_op *op, bool adj, int order, int64_t nsub, double dt, int form,
+                         int nv, const double *in) {
+    const SweepBufs b{op->ws[2], op->ws[3], op->ws[4], op->ws[5]};
+    return run_taylor_in(op, b, adj, order, nsub, dt, form, nv, in);
 }
 
 // True when a rocprofiler tool library is configured for this process
@@ -965,6 +978,46 @@ hymor_op *new_op(int64_t nchi, int64_t neta, int b) {
     HIP_TRY(hipGetDevice(&op->device));
     HIP_TRY(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
     return op;
+}
+
+}  // namespace
+
+// ------------------------------------------------------ no-alloc scratch
+// Workspace of hymor_expm_taylor_dev_noalloc: one array per GPU, held from
+// the library's code-object load to the end of the process.  The Taylor
+// sweep needs five n*nv-double buffers (input, two ping-pong, two term
+// buffers), each starting on a 256-byte boundary as hipMalloc'd buffers do,
+// so the form serves n*nv <= kNoAllocRowDoubles.  Host code uses the device
+// address from hipGetSymbolAddress (scratch_base), never the array's name.
+// Every call on a GPU shares its array.
+constexpr int64_t kNoAllocDoubles = int64_t(8) << 20;   // 64 MiB
+constexpr int kNoAllocBufs = 5;
+constexpr int64_t kNoAllocRowDoubles = (kNoAllocDoubles / kNoAllocBufs) & ~int64_t(31);
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_base[kMaxDevices] = {};
+
+// Device address of `device`'s scratch array, looked up once per device
+// with that device current.
+double *scratch_base(int device) {
+    ARG_CHECK(device >= 0 && device < kMaxDevices, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_base[device]) {
+        int cur = 0;
+        HIP_TRY(hipGetDevice(&cur));
+        if (cur != device) HIP_TRY(hipSetDevice(device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        HIP_TRY(e);
+        ARG_CHECK(p != nullptr, "no device address for the scratch array");
+        g_scratch_base[device] = (double *)p;
+    }
+    return g_scratch_base[device];
 }
 
 }  // namespace
@@ -1282,6 +1335,30 @@ int hymor_expm_taylor_dev(hymor_op *op, int transpose, int order, int64_t nsub, 
         ensure_ws(op, nv);
         launch_perm_in(op, op->stream, (const double *)d_x, op->n, op->ws[1], 1, nv, 0, nv);
         const double *res = run_taylor_ws1(op, transpose != 0, order, nsub, dt, form, nv);
+        launch_perm_out(op, op->stream, res, (double *)d_y, op->n, 1, nv, 0, nv);
+    });
+}
+
+int hymor_expm_taylor_dev_noalloc(hymor_op *op, int transpose, int order, int64_t nsub,
+                                  double dt, int form, const void *d_x, void *d_y, int dtype) {
+    return guarded([&] {
+        check_taylor_args(op, transpose, order, nsub, form);
+        ARG_CHECK(d_x && d_y, "null vector");
+        const int nv = nv_of(op, dtype);
+        const int64_t len = op->n * nv;
+        const int64_t stride = (len + 31) & ~int64_t(31);   // 256-byte aligned buffers
+        ARG_CHECK(stride <= kNoAllocRowDoubles,
+                  "hymor_expm_taylor_dev_noalloc: n * (1 real, 2 complex) = " +
+                      std::to_string(len) + " exceeds its scratch limit of " +
+                      std::to_string(kNoAllocRowDoubles) + " doubles per vector");
+        double *s = scratch_base(op->device);
+        // every buffer is written in full before it is read (the input by
+        // the permutation, the others by the sweep), so the scratch needs no
+        // clearing between calls
+        double *in = s;
+        const SweepBufs b{s + stride, s + 2 * stride, s + 3 * stride, s + 4 * stride};
+        launch_perm_in(op, op->stream, (const double *)d_x, op->n, in, 1, nv, 0, nv);
+        const double *res = run_taylor_in(op, b, transpose != 0, order, nsub, dt, form, nv, in);
         launch_perm_out(op, op->stream, res, (double *)d_y, op->n, 1, nv, 0, nv);
     });
 }

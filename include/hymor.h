@@ -132,6 +132,20 @@ int hymor_expm_taylor(hymor_op *op, int transpose, int order, int64_t nsub, doub
 int hymor_expm_taylor_dev(hymor_op *op, int transpose, int order, int64_t nsub, double dt,
                           int form, const void *d_x, void *d_y, int dtype);
 
+/* hymor_expm_taylor_dev that allocates nothing: no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  The sweep works in a
+ * library-owned static device array of 64 MiB per GPU (held until the
+ * process ends) instead of the handle's workspace, and never captures or
+ * replays a graph of its own, so the call may itself be captured.  Limit:
+ * n * (1 for HYMOR_F64, 2 for HYMOR_C128) <= 1 677 696; larger inputs fail
+ * with HYMOR_ERR_ARG.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (different handles on different streams, or threads)
+ * race and must be serialised by the caller.  Asynchronous, on the handle's
+ * stream. */
+int hymor_expm_taylor_dev_noalloc(hymor_op *op, int transpose, int order, int64_t nsub,
+                                  double dt, int form, const void *d_x, void *d_y, int dtype);
+
 /* Batched action on nrhs vectors at once: X, Y are column-major n x nrhs
  * (leading dimensions ldx, ldy >= n), each column one vector of the
  * unbatched call.  The operator's blocks are read once per Taylor term for
