@@ -163,6 +163,53 @@ def eigen_leg(k=10, tol=1e-9, block=4, seeds=(0, 1, 2)):
     return out
 
 
+FP64_MFMA_PEAK_TFS = 78.6
+
+
+def krylov_projection_leg(n=1440000, j=32, widths=(4, 16), reps=20):
+    """The dense Krylov projections of the batched eigen leg at its size:
+    one block Gram-Schmidt pass of s new vectors against j basis vectors of
+    length n (hymor_bench_krylov_projection): V^T W on the FP64 matrix
+    cores, the fixed-order reduction of its partials, and W -= V h.  Each
+    kernel's time comes from HIP events over `reps` launches; GB/s and
+    TF/s are algorithmic bytes and flops over that time, matrix-core
+    utilisation the flops the matrix cores execute (products always 16 x 16
+    wide) over time and the FP64 matrix-core peak."""
+    import ctypes
+
+    from hymor_amd import _lib
+
+    lib = _lib.load()
+    cases = []
+    for s in widths:
+        out = (ctypes.c_double * 8)()
+        _lib.check(lib.hymor_bench_krylov_projection(n, j, s, reps, out))
+        ms_vtw, ms_red, ms_sub, by_vtw, fl_vtw, fl_mfma, by_sub, fl_sub = list(out)
+        cases.append({
+            "s": s,
+            "vtw": {"kernel": "k_vtw_mfma<true>", "ms": ms_vtw,
+                    "GBps": by_vtw / (ms_vtw * 1e-3) / 1e9,
+                    "frac_of_hbm_peak": by_vtw / (ms_vtw * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "useful_TFps": fl_vtw / (ms_vtw * 1e-3) / 1e12,
+                    "mfma_TFps_issued": fl_mfma / (ms_vtw * 1e-3) / 1e12,
+                    "mfma_utilisation": fl_mfma / (ms_vtw * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFS,
+                    "algorithmic_bytes": by_vtw, "flops": fl_vtw, "mfma_flops_issued": fl_mfma},
+            "reduce": {"kernel": "k_reduce_vtw", "ms": ms_red},
+            "sub": {"kernel": f"k_sub_vh_blk<{8 if s <= 8 else 16}>", "ms": ms_sub,
+                    "GBps": by_sub / (ms_sub * 1e-3) / 1e9,
+                    "frac_of_hbm_peak": by_sub / (ms_sub * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "TFps": fl_sub / (ms_sub * 1e-3) / 1e12, "algorithmic_bytes": by_sub},
+        })
+        log(f"krylov projection s={s}: V^T W {ms_vtw * 1e3:.1f} us "
+            f"({cases[-1]['vtw']['GBps']:.0f} GB/s, MFMA {cases[-1]['vtw']['mfma_utilisation']:.1%}), "
+            f"reduce {ms_red * 1e3:.1f} us, W -= V h {ms_sub * 1e3:.1f} us "
+            f"({cases[-1]['sub']['GBps']:.0f} GB/s)")
+    return {"n": n, "basis_columns": j, "fp64_mfma_peak_TFps": FP64_MFMA_PEAK_TFS,
+            "bound": "hbm (V^T W at s = 16: 2 s j / ((j + s) 8) = 2.7 flop/B, below the "
+                     "FP64 ridge of 9.8)",
+            "cases": cases}
+
+
 def receptivity_leg(steps=3, order=12, nsub=64, T=1.0, n_w=32, widths=(4, 16)):
     """C4: the action T_12(L_ T/64)^64 on the extended receptivity operator
     L_ = [L repeat(B,1,32); 0 kron(diag(-i w), I)] (LINEARIZE_L_.jl:58-67) of
@@ -381,12 +428,14 @@ def main():
     ap.add_argument("--no-eigen", action="store_true", help="skip the eigenvalue leg")
     ap.add_argument("--no-receptivity", action="store_true",
                     help="skip the C4 receptivity leg (assembled vs factored)")
-    ap.add_argument("--kernel-only", choices=["plain", "batched", "block4", "receptivity"],
+    ap.add_argument("--kernel-only",
+                    choices=["plain", "batched", "block4", "receptivity", "krylov"],
                     help="time only the Taylor-term kernel of one form (for profiler runs): "
                          "plain FP64 one vector, batched FP32 --nrhs vectors, block4 FP32 "
                          "four vectors (the eigen leg's FP32 phase), receptivity: the C4 "
                          "operator extended by 32 wavenumbers, FP64 values, one complex "
-                         "vector (block kernel + extra rows)")
+                         "vector (block kernel + extra rows); krylov: the block Gram-Schmidt "
+                         "projection kernels at N=1.44M, 32 basis vectors, blocks of 4 and 16")
     ap.add_argument("--batch-kernel", type=int, default=None,
                     help="--kernel-only batched: use this batch kernel (0..8) instead of "
                          "the default (6: the cell-pair kernel at b=8 and 16+ vectors)")
@@ -405,6 +454,14 @@ def main():
             traffic_note = info
         else:
             pass_codes = info
+
+    if args.kernel_only == "krylov":
+        from hymor_amd.calibrate import copy_bandwidth
+        kp = krylov_projection_leg(reps=args.kernel_reps)
+        cp = copy_bandwidth(COPY_MIB, 3)
+        print(json.dumps({"kernel_only": "krylov", "krylov_projection": kp, "copy": cp}),
+              flush=True)
+        return
 
     from hymor_amd import operators as ops
     from hymor_amd.core import (DeviceMatrix, DeviceOperator, DeviceVector,
@@ -618,6 +675,7 @@ def main():
         log("eigen leg: plain FP64 Krylov-Schur vs batched FP32 + FP64 refinement")
         result["batched_eigen"] = eigen_leg()
         log(f"eigen leg: {result['batched_eigen']}")
+        result["batched_eigen"]["krylov_projection"] = krylov_projection_leg()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(op, order, nsub)
     else:

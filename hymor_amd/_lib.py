@@ -79,6 +79,7 @@ SIGNATURES = {
     "hymor_krylov_combine": (C.c_int, [_p, C.c_int, C.c_int, _pd]),
     "hymor_krylov_ritz": (C.c_int, [_p, C.c_int, C.c_int, _pd, _p]),
     "hymor_krylov_copy_column": (C.c_int, [_p, C.c_int, C.c_int]),
+    "hymor_bench_krylov_projection": (C.c_int, [_i64, C.c_int, C.c_int, C.c_int, _pd]),
     "hymor_weights_create_csc": (C.c_int, [_i64, C.c_int, _p, _p, _p, C.c_int, C.c_int, _i64,
                                            _i64, _pp]),
     "hymor_weights_destroy": (C.c_int, [_p]),

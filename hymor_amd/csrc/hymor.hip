@@ -1958,23 +1958,56 @@ void launch_zero(hymor_krylov *K, double *p, int64_t len) {
 
 constexpr int kVtwGroups = 512;   // row chunks of the block projection kernel
 
-// h (ncol x s, column-major) = V[:, :ncol]^T W, deterministic
-void block_dots(hymor_krylov *K, int ncol, const double *W, int64_t ldw, int s, double *h) {
-    const int ctiles = (ncol + 15) / 16;
-    const int64_t rows_per = ((K->n + kVtwGroups - 1) / kVtwGroups + 15) / 16 * 16;
-    const int64_t groups = (K->n + rows_per - 1) / rows_per;
+// Row split of the V^T W partial products: kVtwGroups workgroups of a
+// multiple of 16 rows each
+struct VtwShape {
+    int ctiles;
+    int64_t rows_per, groups;
+};
+
+VtwShape vtw_shape(const hymor_krylov *K, int ncol) {
+    VtwShape v;
+    v.ctiles = (ncol + 15) / 16;
+    v.rows_per = ((K->n + kVtwGroups - 1) / kVtwGroups + 15) / 16 * 16;
+    v.groups = (K->n + v.rows_per - 1) / v.rows_per;
+    return v;
+}
+
+// K->bpartial <- the per-workgroup partial products of V[:, :ncol]^T W
+void launch_vtw(hymor_krylov *K, const VtwShape &v, int ncol, const double *W, int64_t ldw,
+                int s) {
     const bool aligned = K->n % 2 == 0 && ldw % 2 == 0 &&
                          ((uintptr_t)K->V % 16) == 0 && ((uintptr_t)W % 16) == 0;
     if (aligned)
-        hipLaunchKernelGGL(k_vtw_mfma<true>, dim3((unsigned)groups), dim3(256), 0, K->stream, K->V,
-                           K->n, ncol, W, ldw, s, K->n, rows_per, ctiles, K->bpartial);
+        hipLaunchKernelGGL(k_vtw_mfma<true>, dim3((unsigned)v.groups), dim3(256), 0, K->stream,
+                           K->V, K->n, ncol, W, ldw, s, K->n, v.rows_per, v.ctiles, K->bpartial);
     else
-        hipLaunchKernelGGL(k_vtw_mfma<false>, dim3((unsigned)groups), dim3(256), 0, K->stream,
-                           K->V, K->n, ncol, W, ldw, s, K->n, rows_per, ctiles, K->bpartial);
+        hipLaunchKernelGGL(k_vtw_mfma<false>, dim3((unsigned)v.groups), dim3(256), 0, K->stream,
+                           K->V, K->n, ncol, W, ldw, s, K->n, v.rows_per, v.ctiles, K->bpartial);
     HIP_TRY(hipGetLastError());
+}
+
+// h (ncol x s, column-major) <- the sum of those partials, in a fixed order
+void launch_reduce_vtw(hymor_krylov *K, const VtwShape &v, int ncol, int s, double *h) {
     hipLaunchKernelGGL(k_reduce_vtw, dim3((unsigned)((ncol * s + 3) / 4)), dim3(256), 0,
-                       K->stream, K->bpartial, groups, ctiles, ncol, s, h);
+                       K->stream, K->bpartial, v.groups, v.ctiles, ncol, s, h);
     HIP_TRY(hipGetLastError());
+}
+
+// h (ncol x s, column-major) = V[:, :ncol]^T W, deterministic
+void block_dots(hymor_krylov *K, int ncol, const double *W, int64_t ldw, int s, double *h) {
+    const VtwShape v = vtw_shape(K, ncol);
+    launch_vtw(K, v, ncol, W, ldw, s);
+    launch_reduce_vtw(K, v, ncol, s, h);
+}
+
+// the block Gram-Schmidt buffers, made on first use
+void ensure_block_bufs(hymor_krylov *K) {
+    if (K->bpartial) return;
+    const int ct = (K->m_max + 1 + 15) / 16;
+    K->bpartial = (double *)dalloc_raw(K->allocs, (size_t)kVtwGroups * ct * 256 * sizeof(double));
+    K->hb = (double *)dalloc_raw(K->allocs, (size_t)(K->m_max + 1) * 16 * sizeof(double));
+    K->hb2 = (double *)dalloc_raw(K->allocs, (size_t)(K->m_max + 1) * 16 * sizeof(double));
 }
 
 // W[:, :s] -= V[:, :ncol] h
@@ -2136,13 +2169,7 @@ int hymor_krylov_orthonormalize_block(hymor_krylov *K, int j, int s, void *d_W, 
         const int ldh = j + s;
         std::fill(h_out, h_out + (size_t)ldh * s, 0.0);
         if (j > 0) {
-            if (!K->bpartial) {
-                const int ct = (K->m_max + 1 + 15) / 16;
-                K->bpartial = (double *)dalloc_raw(K->allocs, (size_t)kVtwGroups * ct * 256 *
-                                                                  sizeof(double));
-                K->hb = (double *)dalloc_raw(K->allocs, (size_t)(K->m_max + 1) * 16 * sizeof(double));
-                K->hb2 = (double *)dalloc_raw(K->allocs, (size_t)(K->m_max + 1) * 16 * sizeof(double));
-            }
+            ensure_block_bufs(K);
             // block CGS2 against the j accepted columns: one pass over V per
             // projection for all s vectors
             block_dots(K, j, W, ldw, s, K->hb);
@@ -2258,6 +2285,59 @@ int hymor_krylov_copy_column(hymor_krylov *K, int src, int dst) {
                                K->stream));
         HIP_TRY(hipStreamSynchronize(K->stream));
     });
+}
+
+int hymor_bench_krylov_projection(int64_t n, int j, int s, int reps, double *out) {
+    hymor_krylov *K = nullptr;
+    int rc = guarded([&] {
+        ARG_CHECK(out != nullptr, "null argument");
+        ARG_CHECK(n >= 16 && j >= 1 && j <= 1024 && s >= 1 && s <= 16 && reps >= 1,
+                  "need n >= 16, 1 <= j <= 1024, 1 <= s <= 16, reps >= 1");
+        ARG_CHECK((size_t)j * s * sizeof(double) <= 65536, "j * s too large");
+        hymor_krylov **pk = &K;
+        int e = hymor_krylov_create(n, j + s, HYMOR_F64, pk);
+        if (e != HYMOR_OK) throw HipError(g_err);
+        ensure_block_bufs(K);
+        // basis columns 0..j-1 and the block W (columns j..j+s-1) filled on
+        // the device; the values do not change what the kernels do
+        const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((n * (j + s) + 255) / 256,
+                                                                     16384));
+        hipLaunchKernelGGL(k_fill_uniform, dim3((unsigned)grid), dim3(256), 0, K->stream, K->V,
+                           n * (j + s), (uint64_t)12345);
+        HIP_TRY(hipGetLastError());
+        double *W = col(K, K->V, j);
+        const VtwShape v = vtw_shape(K, j);
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        auto timed = [&](auto &&launch) {
+            launch();
+            HIP_TRY(hipEventRecord(e0, K->stream));
+            for (int i = 0; i < reps; ++i) launch();
+            HIP_TRY(hipEventRecord(e1, K->stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            return (double)ms / reps;
+        };
+        // W -= V h with h = 0 leaves W unchanged, so repeated launches keep
+        // the same data and the same work
+        HIP_TRY(hipMemsetAsync(K->hb2, 0, (size_t)j * s * sizeof(double), K->stream));
+        out[0] = timed([&] { launch_vtw(K, v, j, W, n, s); });
+        out[1] = timed([&] { launch_reduce_vtw(K, v, j, s, K->hb); });
+        out[2] = timed([&] { block_sub(K, j, K->hb2, W, n, s); });
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        out[3] = (double)(j + s) * n * 8.0;
+        out[4] = 2.0 * n * j * s;
+        out[5] = 512.0 * n * v.ctiles;
+        out[6] = (double)(j + 2 * s) * n * 8.0;
+        out[7] = 2.0 * n * j * s;
+    });
+    const std::string msg = g_err;   // destroying K must not clear a failure's message
+    if (K) (void)hymor_krylov_destroy(K);
+    g_err = msg;
+    return rc;
 }
 
 }  // extern "C"
