@@ -166,6 +166,79 @@ def eigen_leg(k=10, tol=1e-9, block=4, seeds=(0, 1, 2)):
 FP64_MFMA_PEAK_TFS = 78.6
 
 
+def small_config_leg(steps=20, order=12, nsub=64, T=1.0):
+    """The launch-bound end: the same action T_12(A T/64)^64 v (768 Taylor
+    terms) on the 100 x 80-cell, b = 4 operator (N = 32 000, 1.13M
+    nonzeros, 9 MB of values: cache-resident), FP64, vectors resident on
+    the device, in actions/s: kernel by kernel, replayed from a captured
+    HIP graph, and through the no-alloc entry point (kernel by kernel on
+    the library's static scratch); plus the term kernel's own event-timed
+    duration, so that 768 x that duration against the action's time shows
+    the launch cost."""
+    import ctypes
+
+    from hymor_amd import _lib
+    from hymor_amd import operators as ops
+    from hymor_amd.core import _FORMS, DeviceOperator, DeviceVector, expm_taylor_dev
+
+    op = ops.config_operator("C1")
+    x = np.random.default_rng(0).standard_normal(op.n)
+    out = {"workload": ("100x80 cells, b=4 (4x4 blocks, 3x3 stencil), N=32000, nnz=1134784; "
+                        f"action T_{order}(A T/{nsub})^{nsub} v, T={T}, {FORM} form, FP64"),
+           "unit": "exp(AT)v actions/s", "steps": steps, "kernels_per_action": order * nsub}
+    lib = _lib.load()
+    with DeviceOperator.from_blocks(op, with_adjoint=False) as dev:
+        # at this size the launch variants differ more than at N = 1.44M:
+        # every form below runs the autotuner's pick
+        tune_ms, pick = dev.autotune(reps=50)
+        out["launch"] = {"split": pick[0], "waves": pick[1], "nontemporal": pick[2],
+                         "autotune_ms_powers_term": tune_ms}
+        dx, dy = DeviceVector.from_host(x), DeviceVector(op.n)
+        try:
+            def run_eager():
+                expm_taylor_dev(dev, dx, dy, order=order, nsub=nsub, dt=T / nsub, form=FORM)
+
+            def run_noalloc():
+                _lib.check(lib.hymor_expm_taylor_dev_noalloc(
+                    dev.handle, 0, order, nsub, T / nsub, _FORMS[FORM], dx.ptr, dy.ptr, _lib.F64))
+
+            def timed(run):
+                run()
+                dev.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    run()
+                dev.synchronize()
+                return (time.perf_counter() - t0) / steps
+
+            dev.set_graphs(False)
+            t_eager = timed(run_eager)
+            y_eager = dy.download()
+            dev.set_graphs(True)
+            t_graph = timed(run_eager)
+            y_graph = dy.download()
+            dev.set_graphs(False)
+            t_na = timed(run_noalloc)
+            y_na = dy.download()
+            k_ms, k_bytes = dev.bench_term_kernel(reps=200, form=FORM)
+        finally:
+            dx.free()
+            dy.free()
+    kern_s = order * nsub * k_ms * 1e-3
+    for key, t, y in (("eager", t_eager, y_eager), ("graph", t_graph, y_graph),
+                      ("noalloc", t_na, y_na)):
+        out[key] = {"value": 1.0 / t, "ms_per_action": t * 1e3,
+                    "kernel_time_fraction": kern_s / t,
+                    "bitwise_equal_to_eager": bool(np.array_equal(y, y_eager))}
+    out["term_kernel_us"] = k_ms * 1e3
+    out["term_kernel_GBps"] = k_bytes / (k_ms * 1e-3) / 1e9
+    out["term_kernel_algorithmic_bytes"] = k_bytes
+    log(f"small config: eager {1 / t_eager:.1f}, graph {1 / t_graph:.1f}, noalloc "
+        f"{1 / t_na:.1f} actions/s; term kernel {k_ms * 1e3:.2f} us "
+        f"({out['term_kernel_GBps']:.0f} GB/s)")
+    return out
+
+
 def krylov_projection_leg(n=1440000, j=32, widths=(4, 16), reps=20):
     """The dense Krylov projections of the batched eigen leg at its size:
     one block Gram-Schmidt pass of s new vectors against j basis vectors of
@@ -428,6 +501,8 @@ def main():
     ap.add_argument("--no-eigen", action="store_true", help="skip the eigenvalue leg")
     ap.add_argument("--no-receptivity", action="store_true",
                     help="skip the C4 receptivity leg (assembled vs factored)")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the launch-bound 100x80-cell leg (eager / graph / no-alloc)")
     ap.add_argument("--kernel-only",
                     choices=["plain", "batched", "block4", "receptivity", "krylov"],
                     help="time only the Taylor-term kernel of one form (for profiler runs): "
@@ -669,6 +744,8 @@ def main():
         },
         "batched": batched,
     }
+    if not args.no_small:
+        result["small_config"] = small_config_leg()
     if not args.no_receptivity:
         result["batched_receptivity"] = receptivity_leg()
     if not args.no_eigen:
