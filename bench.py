@@ -283,14 +283,14 @@ def krylov_projection_leg(n=1440000, j=32, widths=(4, 16), reps=20):
             "cases": cases}
 
 
-def receptivity_leg(steps=3, order=12, nsub=64, T=1.0, n_w=32, widths=(4, 16)):
+def receptivity_leg(steps=3, order=12, nsub=64, T=1.0, n_w=32, widths=(4, 8, 16)):
     """C4: the action T_12(L_ T/64)^64 on the extended receptivity operator
     L_ = [L repeat(B,1,32); 0 kron(diag(-i w), I)] (LINEARIZE_L_.jl:58-67) of
     a 500 x 250-cell, b = 4 operator with 32 wavenumbers w_k = 0.1 *
     100^((k-1)/31), in extended actions per second:
       * assembled L_, FP64 values, one complex vector per sweep (the
         reference's form);
-      * assembled L_, 4 and 16 complex vectors per sweep (8 and 32 doubles
+      * assembled L_, 4, 8 and 16 complex vectors per sweep (8, 16 and 32 doubles
         per row), FP32 and FP64 values -- the batches a block eigensolver
         over the extended space applies;
       * factored: the 32 per-wavenumber systems [L B; 0 -i w_k I] of ONE
@@ -504,13 +504,16 @@ def main():
     ap.add_argument("--no-small", action="store_true",
                     help="skip the launch-bound 100x80-cell leg (eager / graph / no-alloc)")
     ap.add_argument("--kernel-only",
-                    choices=["plain", "batched", "block4", "receptivity", "krylov"],
+                    choices=["plain", "batched", "block4", "receptivity", "receptivity16",
+                             "krylov"],
                     help="time only the Taylor-term kernel of one form (for profiler runs): "
                          "plain FP64 one vector, batched FP32 --nrhs vectors, block4 FP32 "
                          "four vectors (the eigen leg's FP32 phase), receptivity: the C4 "
                          "operator extended by 32 wavenumbers, FP64 values, one complex "
-                         "vector (block kernel + extra rows); krylov: the block Gram-Schmidt "
-                         "projection kernels at N=1.44M, 32 basis vectors, blocks of 4 and 16")
+                         "vector (block kernel + extra rows); receptivity16: the same "
+                         "operator on 16 complex vectors (its 32 copies of B kept once); "
+                         "krylov: the block Gram-Schmidt projection kernels at N=1.44M, "
+                         "32 basis vectors, blocks of 4 and 16")
     ap.add_argument("--batch-kernel", type=int, default=None,
                     help="--kernel-only batched: use this batch kernel (0..8) instead of "
                          "the default (6: the cell-pair kernel at b=8 and 16+ vectors)")
@@ -548,13 +551,14 @@ def main():
     op = ops.config_operator(args.config)
     if args.kernel_only:
         prec, nv = {"plain": ("f64", 1), "batched": ("f32", args.nrhs),
-                    "block4": ("f32", 4), "receptivity": ("f64", 2)}[args.kernel_only]
-        if args.kernel_only == "receptivity":
+                    "block4": ("f32", 4), "receptivity": ("f64", 2),
+                    "receptivity16": ("f64", 32)}[args.kernel_only]
+        if args.kernel_only.startswith("receptivity"):
             # the receptivity leg's extended operator, default launch tuning
             op = ops.receptivity_extension(ops.config_operator("C4"),
                                            0.1 * 100.0 ** (np.arange(32) / 31.0))
         devk = DeviceOperator.from_blocks(op, value_prec=prec, with_adjoint=False)
-        if args.kernel_only == "receptivity":
+        if args.kernel_only.startswith("receptivity"):
             pass  # default launch, as in the receptivity leg
         elif nv < 8:
             devk.set_tuning(*PLAIN_TUNING)

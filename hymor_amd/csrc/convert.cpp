@@ -358,4 +358,179 @@ void interleave_lanes(HostSliced &s, int vpl) {
     }
 }
 
+namespace {
+
+// Index of the listed row `row` in x, or -1.
+int64_t listed_index(const HostExtra &x, int64_t row) {
+    auto it = std::lower_bound(x.rows.begin(), x.rows.end(), row);
+    return (it != x.rows.end() && *it == row) ? (int64_t)(it - x.rows.begin()) : -1;
+}
+
+bool same_value(const HostExtra &x, int64_t e, int64_t f, int vs) {
+    for (int q = 0; q < vs; ++q)
+        if (x.val[(size_t)(e * vs + q)] != x.val[(size_t)(f * vs + q)]) return false;
+    return true;
+}
+
+// Copy of x keeping listed rows [0, keep_rows) only, with the entries each
+// row's `edit` callback emits (col, value index) in place of its own.
+template <typename Edit>
+HostExtra rebuild(const HostExtra &x, int64_t keep_rows, Edit &&edit) {
+    const int vs = x.is_complex ? 2 : 1;
+    HostExtra out;
+    out.is_complex = x.is_complex;
+    out.ptr.assign(1, 0);
+    for (int64_t i = 0; i < keep_rows; ++i) {
+        edit(i, [&](int32_t col, int64_t e) {
+            out.col.push_back(col);
+            for (int q = 0; q < vs; ++q) out.val.push_back(x.val[(size_t)(e * vs + q)]);
+        });
+        out.rows.push_back(x.rows[(size_t)i]);
+        out.ptr.push_back((int64_t)out.col.size());
+    }
+    return out;
+}
+
+}  // namespace
+
+bool compress_column_replicas(HostExtra &x, int64_t n, HostReplica &rep) {
+    const int vs = x.is_complex ? 2 : 1;
+    const int64_t nr = (int64_t)x.rows.size();
+    // trailing run of rows [c0, n), listed consecutively, each holding at
+    // most its own diagonal entry
+    int64_t i0 = nr;
+    while (i0 > 0) {
+        const int64_t i = i0 - 1, row = x.rows[(size_t)i];
+        const int64_t len = x.ptr[(size_t)i + 1] - x.ptr[(size_t)i];
+        if (row != n - (nr - i)) break;
+        if (len > 1 || (len == 1 && x.col[(size_t)x.ptr[(size_t)i]] != row)) break;
+        i0 = i;
+    }
+    if (i0 == nr) return false;
+    const int64_t c0 = x.rows[(size_t)i0], M = n - c0;
+    if (M < 2 || n + M >= (int64_t)INT32_MAX) return false;
+    // per earlier row, where its entries in columns >= c0 start (sorted rows)
+    std::vector<int64_t> tail((size_t)i0);
+    bool any = false;
+    for (int64_t i = 0; i < i0; ++i) {
+        const int64_t b0 = x.ptr[(size_t)i], b1 = x.ptr[(size_t)i + 1];
+        int64_t e = b0;
+        while (e < b1 && x.col[(size_t)e] < c0) ++e;
+        tail[(size_t)i] = e;
+        any = any || e < b1;
+    }
+    if (!any) return false;
+    // the largest number of copies nrep >= 2 (p = M / nrep) under which
+    // every row's tail is nrep copies of its first p columns' entries
+    int64_t nrep = 0, p = 0;
+    for (int64_t N = M; N >= 2 && !nrep; --N) {
+        if (M % N) continue;
+        const int64_t pp = M / N;
+        bool ok = true;
+        for (int64_t i = 0; i < i0 && ok; ++i) {
+            const int64_t t0 = tail[(size_t)i], len = x.ptr[(size_t)i + 1] - t0;
+            if (len == 0) continue;
+            if (len % N) { ok = false; break; }
+            const int64_t m = len / N;
+            for (int64_t j = 0; j < m && ok; ++j) {
+                const int64_t off = x.col[(size_t)(t0 + j)] - c0;
+                if (off >= pp) { ok = false; break; }
+                for (int64_t k = 1; k < N; ++k) {
+                    const int64_t e = t0 + k * m + j;
+                    if (x.col[(size_t)e] - c0 != off + k * pp || !same_value(x, e, t0 + j, vs)) {
+                        ok = false;
+                        break;
+                    }
+                }
+            }
+        }
+        if (ok) {
+            nrep = N;
+            p = pp;
+        }
+    }
+    if (!nrep) return false;
+    rep = HostReplica();
+    rep.kind = 1;
+    rep.c0 = c0;
+    rep.p = p;
+    rep.nrep = nrep;
+    rep.diag.assign((size_t)(M * vs), 0.0);
+    for (int64_t i = i0; i < nr; ++i)
+        if (x.ptr[(size_t)i + 1] > x.ptr[(size_t)i])
+            for (int q = 0; q < vs; ++q)
+                rep.diag[(size_t)((x.rows[(size_t)i] - c0) * vs + q)] =
+                    x.val[(size_t)(x.ptr[(size_t)i] * vs + q)];
+    x = rebuild(x, i0, [&](int64_t i, auto &&emit) {
+        const int64_t b0 = x.ptr[(size_t)i], t0 = tail[(size_t)i];
+        for (int64_t e = b0; e < t0; ++e) emit(x.col[(size_t)e], e);
+        const int64_t m = (x.ptr[(size_t)i + 1] - t0) / nrep;
+        for (int64_t j = 0; j < m; ++j)
+            emit((int32_t)(n + x.col[(size_t)(t0 + j)] - c0), t0 + j);
+    });
+    return true;
+}
+
+bool compress_row_replicas(HostExtra &x, int64_t n, int64_t c0, int64_t p, int64_t nrep,
+                           HostReplica &rep) {
+    const int vs = x.is_complex ? 2 : 1;
+    if (c0 < 0 || p < 1 || nrep < 2 || c0 + p * nrep != n) return false;
+    const int64_t i0 = listed_index(x, c0);
+    const int64_t nr = (int64_t)x.rows.size();
+    if (i0 < 0 || nr - i0 != n - c0) return false;   // rows c0..n-1 listed consecutively
+    // off-diagonal part of a row: entries before its diagonal (columns < c0
+    // only), and the diagonal entry itself if any
+    auto split = [&](int64_t i, int64_t &b0, int64_t &b1, int64_t &dg) {
+        b0 = x.ptr[(size_t)i];
+        const int64_t e1 = x.ptr[(size_t)i + 1], row = x.rows[(size_t)i];
+        int64_t e = b0;
+        while (e < e1 && x.col[(size_t)e] < c0) ++e;
+        b1 = e;
+        dg = -1;
+        if (e < e1) {
+            if (x.col[(size_t)e] != row || e + 1 != e1) return false;
+            dg = e;
+        }
+        return true;
+    };
+    for (int64_t q = 0; q < p; ++q) {
+        int64_t a0, a1, ad;
+        if (!split(i0 + q, a0, a1, ad)) return false;
+        for (int64_t k = 1; k < nrep; ++k) {
+            int64_t b0, b1, bd;
+            if (!split(i0 + k * p + q, b0, b1, bd)) return false;
+            if (b1 - b0 != a1 - a0) return false;
+            for (int64_t j = 0; j < a1 - a0; ++j)
+                if (x.col[(size_t)(b0 + j)] != x.col[(size_t)(a0 + j)] ||
+                    !same_value(x, b0 + j, a0 + j, vs))
+                    return false;
+        }
+    }
+    rep = HostReplica();
+    rep.kind = 2;
+    rep.c0 = c0;
+    rep.p = p;
+    rep.nrep = nrep;
+    rep.diag.assign((size_t)((n - c0) * vs), 0.0);
+    rep.vptr.assign(1, 0);
+    for (int64_t i = i0; i < nr; ++i) {
+        int64_t b0, b1, dg;
+        split(i, b0, b1, dg);
+        if (dg >= 0)
+            for (int q = 0; q < vs; ++q)
+                rep.diag[(size_t)((x.rows[(size_t)i] - c0) * vs + q)] = x.val[(size_t)(dg * vs + q)];
+        if (i - i0 < p) {
+            for (int64_t e = b0; e < b1; ++e) {
+                rep.vcol.push_back(x.col[(size_t)e]);
+                for (int q = 0; q < vs; ++q) rep.vval.push_back(x.val[(size_t)(e * vs + q)]);
+            }
+            rep.vptr.push_back((int64_t)rep.vcol.size());
+        }
+    }
+    x = rebuild(x, i0, [&](int64_t i, auto &&emit) {
+        for (int64_t e = x.ptr[(size_t)i]; e < x.ptr[(size_t)i + 1]; ++e) emit(x.col[(size_t)e], e);
+    });
+    return true;
+}
+
 }  // namespace hymor

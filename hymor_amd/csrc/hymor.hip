@@ -89,6 +89,10 @@ struct DevExtra {
     int64_t *rows = nullptr, *ptr = nullptr;
     int32_t *col = nullptr;
     double *val = nullptr;
+    // replicated freestream blocks (HostReplica), kept once
+    ReplicaDev rep;
+    double *rep_sum = nullptr;     // kind 1: p * 64 doubles of column sums
+    int64_t rep_nnz = 0;           // kind 2: entries of the shared rows
 };
 
 }  // namespace
@@ -99,6 +103,10 @@ struct hymor_op {
     bool is_complex = false, has_adj = false;
     DevSliced fwd, adj;
     DevExtra xf, xa;
+    // the same extra rows with the receptivity operator's replicated
+    // freestream blocks kept once (rep.kind != 0 when found), used at
+    // kRepMinWidth or more doubles per row
+    DevExtra xf_rep, xa_rep;
     hipStream_t stream = nullptr;
     int device = 0;
     // workspace: [0] stage (var-major), [1] in0, [2] va, [3] vb, [4] t0, [5] t1, [6] spare
@@ -320,6 +328,44 @@ void upload_extra(hymor_op *op, const HostExtra &h, DevExtra &d) {
     d.ptr = upload(op, ptr);
     d.col = upload(op, col);
     d.val = upload(op, val);
+}
+
+// The replicated blocks found by compress_column_replicas /
+// compress_row_replicas for the extra CSR d (uploaded by upload_extra).
+void upload_replica(hymor_op *op, const HostReplica &h, DevExtra &d) {
+    d.rep = ReplicaDev();
+    if (h.kind == 0) return;
+    d.rep.kind = h.kind;
+    d.rep.c0 = h.c0;
+    d.rep.p = h.p;
+    d.rep.nrep = h.nrep;
+    d.rep.diag = upload(op, h.diag);
+    if (h.kind == 1) {
+        d.rep.vbase = op->n;
+        d.rep_sum = (double *)dalloc(op, (size_t)h.p * 64 * sizeof(double));
+        d.rep.sum = d.rep_sum;
+    } else {
+        d.rep.vptr = upload(op, h.vptr);
+        d.rep.vcol = upload(op, h.vcol);
+        d.rep.vval = upload(op, h.vval);
+        d.rep_nnz = (int64_t)h.vcol.size();
+    }
+}
+
+// The compressed extra rows and their replicas (nothing when none found).
+void upload_replicated(hymor_op *op, const HostExtra &h, const HostReplica &rep, DevExtra &d) {
+    if (rep.kind == 0) return;
+    upload_extra(op, h, d);
+    upload_replica(op, rep, d);
+}
+
+// Compresses the replicated freestream blocks of a receptivity operator in
+// both directions' extra CSRs (xa may be null: no adjoint).
+void find_replicas(int64_t n, HostExtra &xf, HostExtra *xa, HostReplica &rf, HostReplica &ra) {
+    rf = HostReplica();
+    ra = HostReplica();
+    if (!compress_column_replicas(xf, n, rf)) return;
+    if (xa && !compress_row_replicas(*xa, n, rf.c0, rf.p, rf.nrep, ra)) ra = HostReplica();
 }
 
 void drop_graph(hymor_op *op) {
@@ -720,25 +766,56 @@ void launch_extra_nv(const DevExtra &xe, int64_t nflow, hipStream_t st, const do
                      double *tout, const double *vin, double *vout, double c1,
                      double c2, const ColDiag &cd, int tile_b) {
     // one launch: a workgroup of kLongWaves waves per long row, then
-    // workgroups of kLongWaves waves over the short rows
+    // workgroups of kLongWaves waves over the short rows, then (replicated
+    // rows of an adjoint receptivity operator) over the shared rows
     constexpr int NW = kLongWaves;
     const int lpr = xe.is_complex ? (NV >= 2 ? NV / 2 : 1) : NV;
     const int64_t rpw = 64 / lpr;
     const int64_t nshort = xe.nrows - xe.nlong;
     const int64_t short_waves = (nshort + rpw - 1) / rpw;
-    const int64_t grid = xe.nlong + (short_waves + NW - 1) / NW;
+    const int64_t nshort_wg = (short_waves + NW - 1) / NW;
+    int64_t nrep_wg = 0;
+    if (xe.rep.kind == 2) {
+        const int64_t vpw = xe.is_complex ? RepLanes<NV, true>::VPW : RepLanes<NV, false>::VPW;
+        nrep_wg = ((xe.rep.p + vpw - 1) / vpw + NW - 1) / NW;
+    }
+    if (xe.rep.kind && cd.d) throw ArgError("a per-column diagonal on replicated rows (internal)");
+    if (xe.rep.kind == 1) {
+        // column sums of the replicated blocks, and their diagonal rows
+        const int64_t vpw = xe.is_complex ? RepLanes<NV, true>::VPW : RepLanes<NV, false>::VPW;
+        const int64_t waves = (xe.rep.p + vpw - 1) / vpw;
+        const unsigned g = (unsigned)((waves + 3) / 4);
+        if (xe.is_complex) {
+            if constexpr (NV >= 2)
+                hipLaunchKernelGGL((k_rep_sum<NV, true>), dim3(g), dim3(256), 0, st, xe.rep, x,
+                                   xe.rep_sum, tout, vin, vout, c1, c2);
+        } else {
+            hipLaunchKernelGGL((k_rep_sum<NV, false>), dim3(g), dim3(256), 0, st, xe.rep, x,
+                               xe.rep_sum, tout, vin, vout, c1, c2);
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    const int64_t grid = xe.nlong + nshort_wg + nrep_wg;
     if (grid == 0) return;
+    // the replicated form is a variant of its own, so the other operators'
+    // extra rows run without its virtual-column test and shared-row section
+    auto go = [&](auto cplx, auto repl) {
+        constexpr bool C = decltype(cplx)::value, RP = decltype(repl)::value;
+        hipLaunchKernelGGL((k_extra<NV, C, NW, RP>), dim3((unsigned)grid), dim3(64 * NW), 0, st,
+                           xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, xe.nrows, nshort_wg, nflow,
+                           x, tout, vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b, xe.rep);
+    };
+    const bool rep = xe.rep.kind != 0;
     if (xe.is_complex) {
-        if constexpr (NV >= 2)
-            hipLaunchKernelGGL((k_extra<NV, true, NW>), dim3((unsigned)grid), dim3(64 * NW), 0, st,
-                               xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, xe.nrows, nflow, x,
-                               tout, vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
-        else
+        if constexpr (NV >= 2) {
+            if (rep) go(std::true_type{}, std::true_type{});
+            else go(std::true_type{}, std::false_type{});
+        } else {
             throw ArgError("complex extras need two doubles per row");
+        }
     } else {
-        hipLaunchKernelGGL((k_extra<NV, false, NW>), dim3((unsigned)grid), dim3(64 * NW), 0, st,
-                           xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, xe.nrows, nflow, x, tout,
-                           vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b);
+        if (rep) go(std::false_type{}, std::true_type{});
+        else go(std::false_type{}, std::false_type{});
     }
 }
 
@@ -757,19 +834,32 @@ void launch_extra(const DevExtra &xe, int nv, int64_t nflow, hipStream_t st, con
     }
 }
 
+// Doubles per row from which the replicated freestream blocks are applied
+// once (C4 receptivity operator, 32 copies: 16 vectors per sweep 11 %
+// faster, 1 and 4 vectors 6-14 % slower -- there the extra rows' cost is
+// launch and latency, not gathers, and the column sums take one more
+// launch per term).
+constexpr int kRepMinWidth = 16;
+
+const DevExtra &extra_of(const hymor_op *op, bool adj, int nv) {
+    const DevExtra &r = adj ? op->xa_rep : op->xf_rep;
+    if (nv >= kRepMinWidth && r.rep.kind) return r;
+    return adj ? op->xa : op->xf;
+}
+
 // One Taylor term (or a plain SpMV when vout == nullptr) of `op` on `st`;
 // nv doubles per row.
 void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, const double *x, double *tout,
                  const double *vin, double *vout, double c1, double c2) {
     const DevSliced &s = adj ? op->adj : op->fwd;
-    const DevExtra &xe = adj ? op->xa : op->xf;
+    const DevExtra &xe = extra_of(op, adj, nv);
     if (op->ncell > 0) {
         launch_block(s, nv, st, x, tout, vin, vout, c1, c2);
         HIP_TRY(hipGetLastError());
     }
     // then the extra rows: extra-DOF rows in full, flow rows adding their
     // extra entries' part to what the block kernel wrote
-    if (xe.nrows > 0) {
+    if (xe.nrows > 0 || xe.rep.kind) {
         ColDiag cd;
         if (op->cdiag_on) cd = ColDiag{adj ? op->cdiag_adj : op->cdiag, op->cd0, op->cd1};
         launch_extra(xe, nv, op->nflow, st, x, tout, vin, vout, c1, c2, cd,
@@ -1057,6 +1147,9 @@ int hymor_op_create_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
         HostExtra xf, xa;
         csc_to_blocks(n, colptr, rowval, (const double *)nzval, index_base, dtype == HYMOR_C128,
                       scale, vars_per_cell, nchi * neta, fwd, adj, xf, xa);
+        HostExtra xfc = xf, xac = xa;
+        HostReplica rf, ra;
+        find_replicas(n, xfc, &xac, rf, ra);
         op = new_op(nchi, neta, vars_per_cell);
         op->n = n;
         op->prec = value_prec;
@@ -1065,6 +1158,8 @@ int hymor_op_create_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
         upload_sliced(op, adj, value_prec, op->adj);
         upload_extra(op, xf, op->xf);
         upload_extra(op, xa, op->xa);
+        upload_replicated(op, xfc, rf, op->xf_rep);
+        upload_replicated(op, xac, ra, op->xa_rep);
         op->has_adj = true;
         *out = op;
     });
@@ -1116,6 +1211,10 @@ int hymor_op_create_bsr(int64_t nchi, int64_t neta, int vars_per_cell, int64_t n
         const int64_t nflow = (int64_t)b * ncell;
         HostExtra xf, xa;
         extra_from_coo(n, nflow, r, c, v, cplx, false, xf);
+        if (with_adjoint) extra_from_coo(n, nflow, r, c, v, cplx, true, xa);
+        HostExtra xfc = xf, xac = xa;
+        HostReplica rf, ra;
+        find_replicas(n, xfc, with_adjoint ? &xac : nullptr, rf, ra);
         op = new_op(nchi, neta, b);
         op->n = n;
         op->prec = value_prec;
@@ -1129,14 +1228,15 @@ int hymor_op_create_bsr(int64_t nchi, int64_t neta, int vars_per_cell, int64_t n
         build_sliced(op, b, ncell, fwd.ptr, fwd.col, nullptr, false, (const double *)raw.p, scale,
                      value_prec, op->fwd);
         upload_extra(op, xf, op->xf);
+        upload_replicated(op, xfc, rf, op->xf_rep);
         if (with_adjoint) {
             std::vector<int64_t> tptr, tsrc;
             std::vector<int32_t> tcol;
             bsr_transpose_map(ncell, fwd.ptr, fwd.col, tptr, tcol, tsrc);
             build_sliced(op, b, ncell, tptr, tcol, &tsrc, true, (const double *)raw.p, scale,
                          value_prec, op->adj);
-            extra_from_coo(n, nflow, r, c, v, cplx, true, xa);
             upload_extra(op, xa, op->xa);
+            upload_replicated(op, xac, ra, op->xa_rep);
             op->has_adj = true;
         }
         *out = op;
@@ -1663,8 +1763,12 @@ int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int form, int r
         // written)
         double bytes = (double)s.nnzb * (bb * sval + 4.0) + (double)(s.ncell + 1) * 4.0 +
                        (horner ? 3.0 : 4.0) * (double)op->n * nv * 8.0;
-        const DevExtra &xe = transpose ? op->xa : op->xf;
-        bytes += (double)xe.nnz * ((xe.is_complex ? 16.0 : 8.0) + 4.0);
+        const DevExtra &xe = extra_of(op, transpose != 0, nv);
+        const double vsz = xe.is_complex ? 16.0 : 8.0;
+        bytes += (double)xe.nnz * (vsz + 4.0);
+        // replicated blocks: their diagonal, and the shared rows (kind 2)
+        if (xe.rep.kind) bytes += (double)xe.rep.nrep * xe.rep.p * vsz;
+        if (xe.rep.kind == 2) bytes += (double)xe.rep_nnz * (vsz + 4.0);
         *bytes_per_launch = bytes;
     });
 }

@@ -48,6 +48,38 @@ struct HostSliced {
     int64_t nnzb = 0;                // unpadded
 };
 
+// Replicated freestream blocks: the receptivity operator
+//   L_ = [L  repeat(B,1,N);  0  kron(diag(-i w), I_p)]
+// (LINEARIZE_L_.jl:58-67) repeats B's columns N times, and its adjoint
+// repeats B''s rows N times.  Kept once, they are applied once per term:
+//   kind 1 (forward): rows [c0, n) carry only their diagonal; every other
+//     row's entries in columns [c0, n) repeat with period p over nrep
+//     copies, so sum_k B[r, q] x[c0 + k p + q] = B[r, q] S[q] with
+//     S[q] = sum_k x[c0 + k p + q].  Copy 0 stays, its columns renamed to
+//     virtual columns n + q that read S; rows [c0, n) leave the listing
+//     (their diagonal, in `diag`, is applied where S is formed).
+//   kind 2 (adjoint): rows c0 + k p + q share their off-diagonal entries
+//     across k: kept once as shared row q (vptr/vcol/vval), evaluated once
+//     and finished into the nrep rows with their own diagonal (`diag`);
+//     rows [c0, n) leave the listing.
+struct HostReplica {
+    int kind = 0;               // 0 none, 1 column replicas, 2 row replicas
+    int64_t c0 = 0, p = 0, nrep = 0;
+    std::vector<double> diag;   // nrep * p entries (1 or 2 doubles): rows c0 .. n-1
+    std::vector<int64_t> vptr;  // kind 2: p + 1
+    std::vector<int32_t> vcol;
+    std::vector<double> vval;
+};
+
+// Finds and applies kind 1 on a forward extra CSR; false (x unchanged) if
+// the operator has no such structure (nrep >= 2 required).
+bool compress_column_replicas(HostExtra &x, int64_t n, HostReplica &rep);
+
+// Applies kind 2 with the forward operator's (c0, p, nrep) on its adjoint
+// extra CSR; false (x unchanged) if the rows do not repeat.
+bool compress_row_replicas(HostExtra &x, int64_t n, int64_t c0, int64_t p, int64_t nrep,
+                           HostReplica &rep);
+
 void csc_to_blocks(int64_t n, const int64_t *colptr, const int64_t *rowval,
                    const double *nzval, int index_base, bool is_complex, double scale,
                    int b, int64_t ncell, HostBSR &fwd, HostBSR &adj,

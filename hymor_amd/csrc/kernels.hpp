@@ -163,6 +163,217 @@ __global__ __launch_bounds__(256) void k_term(
     }
 }
 
+// Replicated freestream blocks of the receptivity operator (see HostReplica):
+// kind 1 -- listed rows read virtual columns c >= vbase from sum[(c - vbase)
+// * NV + v], the column sums over the nrep copies, formed with the diagonal
+// rows [c0, c0 + nrep p) by k_rep_sum before k_extra runs; kind 2 -- nshared
+// shared rows (vptr/vcol/vval) each finished into rows c0 + k p + q with their
+// own diagonal, in k_extra's last workgroups.
+struct ReplicaDev {
+    int kind = 0;
+    int64_t c0 = 0, p = 0, nrep = 0;
+    int64_t vbase = INT64_MAX;      // kind 1: first virtual column (n)
+    const double *sum = nullptr;    // kind 1: p * NV column sums
+    const double *diag = nullptr;   // nrep * p diagonal values (1 or 2 doubles)
+    const int64_t *vptr = nullptr;  // kind 2: shared rows
+    const int32_t *vcol = nullptr;
+    const double *vval = nullptr;
+};
+
+// x value (row c, double v) as the extra rows read it: with replicated
+// blocks kept once (REP) a virtual column reads their column sum
+template <bool REP>
+__device__ __forceinline__ double xread(const double *__restrict__ x, const ReplicaDev &R,
+                                        int64_t c, int v, int nv, int tile_b, int64_t nflow) {
+    if constexpr (REP)
+        return c >= R.vbase ? R.sum[(c - R.vbase) * nv + v] : x[buf_pos(c, v, nv, tile_b, nflow)];
+    else
+        return x[buf_pos(c, v, nv, tile_b, nflow)];
+}
+
+// acc += row entries [e, e1) times x, for a lane owning NC doubles of the
+// row from v0: the loads of G entries are issued before their products, so
+// a short row costs ceil(len / G) memory round trips instead of len; the
+// products are added in entry order (the same sum as one entry at a time).
+template <int NV, bool CPLX, int G, bool REP>
+__device__ __forceinline__ void row_dot(double *acc, int64_t e, int64_t e1,
+                                        const int32_t *__restrict__ col,
+                                        const double *__restrict__ val,
+                                        const double *__restrict__ x, const ReplicaDev &R, int v0,
+                                        int tile_b, int64_t nflow) {
+    constexpr int NC = CPLX ? 2 : 1;
+    for (; e < e1; e += G) {
+        double a[G][NC], xv[G][NC];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) a[g][i] = xv[g][i] = 0.0;
+            if (e + g < e1) {       // no loads past the row's end
+                const int64_t c = col[e + g];
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    a[g][i] = val[(e + g) * NC + i];
+                    xv[g][i] = xread<REP>(x, R, c, v0 + i, NV, tile_b, nflow);
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (e + g < e1) {
+                if constexpr (CPLX) {
+                    acc[0] = fma(a[g][0], xv[g][0], fma(-a[g][1], xv[g][1], acc[0]));
+                    acc[1] = fma(a[g][0], xv[g][1], fma(a[g][1], xv[g][0], acc[1]));
+                } else {
+                    acc[0] = fma(a[g][0], xv[g][0], acc[0]);
+                }
+            }
+        }
+    }
+}
+
+// Lanes of a replicated row group: LPR lanes over the row's doubles (NC
+// each), KS lanes over the copies, VPW groups per wave.
+template <int NV, bool CPLX>
+struct RepLanes {
+    static constexpr int NC = CPLX ? 2 : 1;
+    static constexpr int LPR = CPLX ? (NV >= 2 ? NV / 2 : 1) : NV;
+    static constexpr int KS = (64 / LPR) >= 8 ? 8 : 64 / LPR;
+    static constexpr int VPW = 64 / (LPR * KS);
+};
+
+// Kind 1, before k_extra: for shared column q, S[q] = sum_k x[c0 + k p + q]
+// (copies split over KS lanes, then a fixed-order lane exchange), and each
+// row c0 + k p + q -- which holds only its diagonal -- finished as
+// extra_short_rows would: t = c1 * d * x, tout = t, vout = vin + c2 * t.
+template <int NV, bool CPLX>
+__global__ __launch_bounds__(256) void k_rep_sum(ReplicaDev R, const double *__restrict__ x,
+                                                 double *__restrict__ sum,
+                                                 double *__restrict__ tout, const double *vin,
+                                                 double *vout, double c1, double c2) {
+    using L = RepLanes<NV, CPLX>;
+    constexpr int NC = L::NC, LPR = L::LPR, KS = L::KS, VPW = L::VPW;
+    constexpr int VS = CPLX ? 2 : 1;
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t q = w * VPW + lane / (LPR * KS);
+    const bool live = q < R.p;
+    const int ks = (lane / LPR) % KS;
+    const int v0 = (lane % LPR) * NC;
+    double s[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) s[i] = 0.0;
+    if (live) {
+        // the copies in chunks of KU, loads first
+        constexpr int KU = 4;
+        for (int64_t k0 = ks; k0 < R.nrep; k0 += KS * KU) {
+            double xv[KU][NC], dv[KU][VS], vv[KU][NC];
+#pragma unroll
+            for (int u = 0; u < KU; ++u) {
+                const int64_t k = k0 + (int64_t)u * KS < R.nrep ? k0 + (int64_t)u * KS : k0;
+                const int64_t row = R.c0 + k * R.p + q;
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    xv[u][i] = x[row * NV + v0 + i];
+                    vv[u][i] = vout ? vin[row * NV + v0 + i] : 0.0;
+                }
+#pragma unroll
+                for (int i = 0; i < VS; ++i) dv[u][i] = R.diag[(k * R.p + q) * VS + i];
+            }
+#pragma unroll
+            for (int u = 0; u < KU; ++u) {
+                const int64_t k = k0 + (int64_t)u * KS;
+                if (k >= R.nrep) break;
+                const int64_t row = R.c0 + k * R.p + q;
+                double acc[NC];
+#pragma unroll
+                for (int i = 0; i < NC; ++i) s[i] += xv[u][i];
+                if constexpr (CPLX) {
+                    acc[0] = fma(dv[u][0], xv[u][0], fma(-dv[u][1], xv[u][1], 0.0));
+                    acc[1] = fma(dv[u][0], xv[u][1], fma(dv[u][1], xv[u][0], 0.0));
+                } else {
+                    acc[0] = fma(dv[u][0], xv[u][0], 0.0);
+                }
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    const int64_t o = row * NV + v0 + i;
+                    const double t = c1 * acc[i];
+                    if (tout) tout[o] = t;
+                    if (vout) vout[o] = fma(c2, t, vv[u][i]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+        for (int off = LPR; off < LPR * KS; off <<= 1) s[i] += __shfl_xor(s[i], off, 64);
+    if (live && ks == 0) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) sum[q * NV + v0 + i] = s[i];
+    }
+}
+
+// Kind 2, in k_extra: shared row q evaluated once per lane group (the KS
+// lanes of a group each evaluate it and finish every KS-th copy), then
+// finished into rows c0 + k p + q with each row's diagonal added last -- the
+// arithmetic extra_short_rows does on each unshared row.
+template <int NV, bool CPLX>
+__device__ __forceinline__ void extra_replica_rows(
+    int64_t w, const ReplicaDev &R, int64_t nflow, const double *__restrict__ x,
+    double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
+    int tile_b) {
+    using L = RepLanes<NV, CPLX>;
+    constexpr int NC = L::NC, LPR = L::LPR, KS = L::KS, VPW = L::VPW;
+    constexpr int VS = CPLX ? 2 : 1;
+    const int lane = threadIdx.x & 63;
+    const int64_t q = w * VPW + lane / (LPR * KS);
+    if (q >= R.p) return;
+    const int ks = (lane / LPR) % KS;
+    const int v0 = (lane % LPR) * NC;
+    double sh[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) sh[i] = 0.0;
+    row_dot<NV, CPLX, 4, false>(sh, R.vptr[q], R.vptr[q + 1], R.vcol, R.vval, x, R, v0, tile_b,
+                                nflow);
+    // the copies in chunks of KU, loads first
+    constexpr int KU = 2;
+    for (int64_t k0 = ks; k0 < R.nrep; k0 += KS * KU) {
+        double xv[KU][NC], dv[KU][VS], vv[KU][NC];
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+            const int64_t k = k0 + (int64_t)u * KS < R.nrep ? k0 + (int64_t)u * KS : k0;
+            const int64_t row = R.c0 + k * R.p + q;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                xv[u][i] = x[row * NV + v0 + i];
+                vv[u][i] = vout ? vin[row * NV + v0 + i] : 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < VS; ++i) dv[u][i] = R.diag[(k * R.p + q) * VS + i];
+        }
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+            const int64_t k = k0 + (int64_t)u * KS;
+            if (k >= R.nrep) break;
+            const int64_t row = R.c0 + k * R.p + q;
+            double acc[NC];
+            if constexpr (CPLX) {
+                acc[0] = fma(dv[u][0], xv[u][0], fma(-dv[u][1], xv[u][1], sh[0]));
+                acc[1] = fma(dv[u][0], xv[u][1], fma(dv[u][1], xv[u][0], sh[1]));
+            } else {
+                acc[0] = fma(dv[u][0], xv[u][0], sh[0]);
+            }
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const int64_t o = row * NV + v0 + i;
+                const double t = c1 * acc[i];
+                if (tout) tout[o] = t;
+                if (vout) vout[o] = fma(c2, t, vv[u][i]);
+            }
+        }
+    }
+}
+
 // One extra CSR row with many entries on the NW waves of a workgroup.  Flow
 // rows (row < nflow) add their extra entries' part to the block kernel's
 // output (which runs first); extra-DOF rows finish the term here.  CPLX: values
@@ -177,13 +388,14 @@ __global__ __launch_bounds__(256) void k_term(
 // cdiag (optional): a complex diagonal that differs per column (per pair
 // (re, im) of the NV doubles), added on rows [cd0, cd1) -- the -i*omega_k
 // block of the factored receptivity systems, one wavenumber per column.
-template <int NV, bool CPLX, int NW>
+template <int NV, bool CPLX, int NW, bool REP>
 __device__ __forceinline__ void extra_long_row(
     int64_t ri, const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
     const int32_t *__restrict__ col, const double *__restrict__ val,
     int64_t nflow, const double *__restrict__ x,
     double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
-    const double *__restrict__ cdiag, int64_t cd0, int64_t cd1, int tile_b) {
+    const double *__restrict__ cdiag, int64_t cd0, int64_t cd1, int tile_b,
+    const ReplicaDev &R) {
     constexpr int LPR = CPLX ? NV / 2 : NV;    // lanes per entry
     constexpr int EPW = 64 / LPR;              // entries per wave and step
     constexpr int NC = CPLX ? 2 : 1;           // doubles per lane
@@ -202,12 +414,12 @@ __device__ __forceinline__ void extra_long_row(
         const int64_t c = col[e];
         if constexpr (CPLX) {
             const double ar = val[2 * e], ai = val[2 * e + 1];
-            const double xr = x[buf_pos(c, v0, NV, tile_b, nflow)];
-            const double xi = x[buf_pos(c, v0 + 1, NV, tile_b, nflow)];
+            const double xr = xread<REP>(x, R, c, v0, NV, tile_b, nflow);
+            const double xi = xread<REP>(x, R, c, v0 + 1, NV, tile_b, nflow);
             acc[0] = fma(ar, xr, fma(-ai, xi, acc[0]));
             acc[1] = fma(ar, xi, fma(ai, xr, acc[1]));
         } else {
-            acc[0] = fma(val[e], x[buf_pos(c, v0, NV, tile_b, nflow)], acc[0]);
+            acc[0] = fma(val[e], xread<REP>(x, R, c, v0, NV, tile_b, nflow), acc[0]);
         }
     }
 #pragma unroll
@@ -264,13 +476,14 @@ __device__ __forceinline__ void extra_long_row(
 // loops over the row's entries; RPW = 64 / lanes-per-row rows per wave.
 // Wave w takes rows [r0 + w RPW, r0 + (w + 1) RPW) of [r0, nrows); same
 // outputs as extra_long_row.
-template <int NV, bool CPLX>
+template <int NV, bool CPLX, bool REP>
 __device__ __forceinline__ void extra_short_rows(
     int64_t w, const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
     const int32_t *__restrict__ col, const double *__restrict__ val, int64_t r0, int64_t nrows,
     int64_t nflow, const double *__restrict__ x,
     double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
-    const double *__restrict__ cdiag, int64_t cd0, int64_t cd1, int tile_b) {
+    const double *__restrict__ cdiag, int64_t cd0, int64_t cd1, int tile_b,
+    const ReplicaDev &R) {
     constexpr int LPR = CPLX ? (NV >= 2 ? NV / 2 : 1) : NV;   // lanes per row
     constexpr int RPW = 64 / LPR;                             // rows per wave
     constexpr int NC = CPLX ? 2 : 1;                          // doubles per lane
@@ -283,19 +496,11 @@ __device__ __forceinline__ void extra_short_rows(
     double acc[NC];
 #pragma unroll
     for (int q = 0; q < NC; ++q) acc[q] = 0.0;
-    const int64_t e1 = ptr[ri + 1];
-    for (int64_t e = ptr[ri]; e < e1; ++e) {
-        const int64_t c = col[e];
-        if constexpr (CPLX) {
-            const double ar = val[2 * e], ai = val[2 * e + 1];
-            const double xr = x[buf_pos(c, v0, NV, tile_b, nflow)];
-            const double xi = x[buf_pos(c, v0 + 1, NV, tile_b, nflow)];
-            acc[0] = fma(ar, xr, fma(-ai, xi, acc[0]));
-            acc[1] = fma(ar, xi, fma(ai, xr, acc[1]));
-        } else {
-            acc[0] = fma(val[e], x[buf_pos(c, v0, NV, tile_b, nflow)], acc[0]);
-        }
-    }
+    // one entry at a time (the unreplicated operator's short rows hold a
+    // single entry or a few); four loads at a time on the replicated form's
+    // rows of compressed B entries
+    row_dot<NV, CPLX, REP ? 4 : 1, REP>(acc, ptr[ri], ptr[ri + 1], col, val, x, R, v0, tile_b,
+                                        nflow);
     // the per-column complex diagonal acts on (re, im) pairs of the row
     // whether the operator's values are complex (a lane owns a pair) or real
     // (a lane owns one double of a pair)
@@ -332,22 +537,26 @@ __device__ __forceinline__ void extra_short_rows(
 // [0, nlong) take the long rows [0, nlong) (more than 32 entries), one each,
 // in contiguous ranges per XCD (neighbouring rows gather the same x rows --
 // the receptivity B rows of adjacent cells -- which then stay in one L2);
-// the rest take the short rows [nlong, nrows), RPW per wave.
-template <int NV, bool CPLX, int NW>
+// the next nshort_wg take the short rows [nlong, nrows), RPW per wave; the
+// rest (kind 2 replicas) the shared rows, VPW per wave.
+template <int NV, bool CPLX, int NW, bool REP>
 __global__ __launch_bounds__(64 * NW) void k_extra(
     const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
     const int32_t *__restrict__ col, const double *__restrict__ val, int64_t nlong,
-    int64_t nrows, int64_t nflow, const double *__restrict__ x,
+    int64_t nrows, int64_t nshort_wg, int64_t nflow, const double *__restrict__ x,
     double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
-    const double *__restrict__ cdiag, int64_t cd0, int64_t cd1, int tile_b) {
+    const double *__restrict__ cdiag, int64_t cd0, int64_t cd1, int tile_b, ReplicaDev R) {
     const int64_t bid = blockIdx.x;
     if (bid < nlong)
-        extra_long_row<NV, CPLX, NW>(xcd_tile(bid, nlong), rows, ptr, col, val, nflow, x, tout,
-                                     vin, vout, c1, c2, cdiag, cd0, cd1, tile_b);
-    else
-        extra_short_rows<NV, CPLX>((bid - nlong) * NW + (threadIdx.x >> 6), rows, ptr, col, val,
+        extra_long_row<NV, CPLX, NW, REP>(xcd_tile(bid, nlong), rows, ptr, col, val, nflow, x, tout,
+                                     vin, vout, c1, c2, cdiag, cd0, cd1, tile_b, R);
+    else if (bid < nlong + nshort_wg)
+        extra_short_rows<NV, CPLX, REP>((bid - nlong) * NW + (threadIdx.x >> 6), rows, ptr, col, val,
                                    nlong, nrows, nflow, x, tout, vin, vout, c1, c2, cdiag, cd0,
-                                   cd1, tile_b);
+                                   cd1, tile_b, R);
+    else if constexpr (REP)
+        extra_replica_rows<NV, CPLX>((bid - nlong - nshort_wg) * NW + (threadIdx.x >> 6), R,
+                                     nflow, x, tout, vin, vout, c1, c2, tile_b);
 }
 
 // var-major (ABI) -> cell-major (internal).  x_vm holds nrhs columns of n

@@ -107,6 +107,150 @@ extern "C" int emu_spmv_csc(int64_t n, const int64_t *colptr, const int64_t *row
     return g_oob;
 }
 
+// emu_spmv_csc with the replicated freestream blocks compressed as the
+// library does at upload (compress_column_replicas on the forward operator,
+// compress_row_replicas on the adjoint with the forward's period), and the
+// compressed form's arithmetic replayed: forward, S[q] = sum_k x[c0 + k p + q]
+// and the diagonal rows [c0, n) where S is formed, virtual columns n + q of
+// the remaining rows reading S; adjoint, each shared row q evaluated once and
+// finished into rows c0 + k p + q with their own diagonal.  info[0..3] =
+// kind (0 if nothing was compressed), c0, p, nrep.
+extern "C" int emu_spmv_csc_rep(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                                const double *nzval, int base, int is_complex, int b,
+                                int64_t ncell, int transpose, const double *x_vm, double *y_vm,
+                                int64_t *info) {
+    g_oob = 0;
+    HostBSR fwd, adj;
+    HostExtra xf, xa;
+    csc_to_blocks(n, colptr, rowval, nzval, base, is_complex != 0, 1.0, b, ncell, fwd, adj, xf, xa);
+    HostReplica rf, ra, *rep = nullptr;
+    const bool found = compress_column_replicas(xf, n, rf);
+    if (found && !transpose) rep = &rf;
+    if (found && transpose && compress_row_replicas(xa, n, rf.c0, rf.p, rf.nrep, ra)) rep = &ra;
+    info[0] = rep ? rep->kind : 0;
+    info[1] = rep ? rep->c0 : 0;
+    info[2] = rep ? rep->p : 0;
+    info[3] = rep ? rep->nrep : 0;
+    const HostBSR &A = transpose ? adj : fwd;
+    const HostExtra &X = transpose ? xa : xf;
+    HostSliced s;
+    bsr_to_sliced(A, s);
+    interleave_lanes(s, 2);
+    const int NV = is_complex ? 2 : 1;
+    const int vs = is_complex ? 2 : 1;
+    const int64_t nflow = (int64_t)b * ncell;
+    std::vector<double> x((size_t)n * NV), y((size_t)n * NV, 0.0), yx((size_t)nflow * NV, 0.0);
+    for (int64_t row = 0; row < n; ++row) {
+        const int64_t vm = row < nflow ? (row % b) * ncell + row / b : row;
+        for (int v = 0; v < NV; ++v) x[(size_t)(row * NV + v)] = x_vm[vm * NV + v];
+    }
+    auto cmac = [&](double *acc, const double *a, const double *xv) {
+        if (is_complex) {
+            acc[0] += a[0] * xv[0] - a[1] * xv[1];
+            acc[1] += a[0] * xv[1] + a[1] * xv[0];
+        } else {
+            acc[0] += a[0] * xv[0];
+        }
+    };
+    std::vector<double> S;
+    if (rep && rep->kind == 1) {
+        // the pre-pass: column sums, and the diagonal rows finished there
+        S.assign((size_t)(rep->p * NV), 0.0);
+        for (int64_t q = 0; q < rep->p; ++q)
+            for (int64_t k = 0; k < rep->nrep; ++k) {
+                const int64_t row = rep->c0 + k * rep->p + q;
+                double xv[2] = {at(x, row * NV), NV > 1 ? at(x, row * NV + 1) : 0.0};
+                for (int v = 0; v < NV; ++v) S[(size_t)(q * NV + v)] += xv[v];
+                double acc[2] = {0, 0};
+                const double dg[2] = {at(rep->diag, (row - rep->c0) * vs),
+                                      vs > 1 ? at(rep->diag, (row - rep->c0) * vs + 1) : 0.0};
+                cmac(acc, dg, xv);
+                for (int v = 0; v < NV; ++v) y[(size_t)(row * NV + v)] = acc[v];
+            }
+    }
+    // k_extra: listed rows (virtual columns read S)
+    for (int64_t w = 0; w < (int64_t)X.rows.size(); ++w) {
+        double acc[2] = {0, 0};
+        for (int64_t e = at(X.ptr, w); e < at(X.ptr, w + 1); ++e) {
+            const int64_t c = at(X.col, e);
+            double a[2] = {at(X.val, e * vs), vs > 1 ? at(X.val, e * vs + 1) : 0.0};
+            double xv[2];
+            if (c >= n) {
+                xv[0] = at(S, (c - n) * NV);
+                xv[1] = NV > 1 ? at(S, (c - n) * NV + 1) : 0.0;
+            } else {
+                xv[0] = at(x, c * NV);
+                xv[1] = NV > 1 ? at(x, c * NV + 1) : 0.0;
+            }
+            cmac(acc, a, xv);
+        }
+        const int64_t row = at(X.rows, w);
+        for (int v = 0; v < NV; ++v) {
+            if (row < nflow) {
+                if (row * NV + v >= (int64_t)yx.size()) ++g_oob;
+                else yx[(size_t)(row * NV + v)] = acc[v];
+            } else {
+                if (row * NV + v >= (int64_t)y.size()) ++g_oob;
+                else y[(size_t)(row * NV + v)] = acc[v];
+            }
+        }
+    }
+    if (rep && rep->kind == 2) {
+        // shared rows: evaluated once, finished into every replica
+        for (int64_t q = 0; q < rep->p; ++q) {
+            double sh[2] = {0, 0};
+            for (int64_t e = at(rep->vptr, q); e < at(rep->vptr, q + 1); ++e) {
+                const int64_t c = at(rep->vcol, e);
+                const double a[2] = {at(rep->vval, e * vs), vs > 1 ? at(rep->vval, e * vs + 1) : 0.0};
+                const double xv[2] = {at(x, c * NV), NV > 1 ? at(x, c * NV + 1) : 0.0};
+                cmac(sh, a, xv);
+            }
+            for (int64_t k = 0; k < rep->nrep; ++k) {
+                const int64_t row = rep->c0 + k * rep->p + q;
+                double acc[2] = {sh[0], sh[1]};
+                const double dg[2] = {at(rep->diag, (row - rep->c0) * vs),
+                                      vs > 1 ? at(rep->diag, (row - rep->c0) * vs + 1) : 0.0};
+                const double xv[2] = {at(x, row * NV), NV > 1 ? at(x, row * NV + 1) : 0.0};
+                cmac(acc, dg, xv);
+                for (int v = 0; v < NV; ++v) {
+                    if (row * NV + v >= (int64_t)y.size()) ++g_oob;
+                    else y[(size_t)(row * NV + v)] = acc[v];
+                }
+            }
+        }
+    }
+    // k_term on the blocks, adding the extra rows' part of flow rows
+    const int CPS = 64 / b, BB = b * b;
+    for (int64_t slice = 0; slice < s.nslice; ++slice) {
+        const int64_t p0 = at(s.slice_ptr, slice);
+        const int64_t width = (at(s.slice_ptr, slice + 1) - p0) / CPS;
+        for (int lane = 0; lane < 64; ++lane) {
+            const int q = lane / b, r = lane % b;
+            double acc[2] = {0, 0};
+            for (int64_t t = 0; t < width; ++t) {
+                const int64_t blk = p0 + q + t * CPS;
+                const int64_t c = at(s.col, blk);
+                if (c < 0 || c >= ncell) ++g_oob;
+                const int64_t sb = (p0 + t * CPS) * BB;
+                for (int k = 0; k < b; ++k) {
+                    const double a = at(s.val, sb + ((k / 2) * 64 + lane) * 2 + k % 2);
+                    for (int v = 0; v < NV; ++v) acc[v] += a * at(x, (c * b + k) * NV + v);
+                }
+            }
+            const int64_t cell = slice * CPS + q;
+            if (cell >= ncell) continue;
+            const int64_t row = cell * b + r;
+            for (int v = 0; v < NV; ++v)
+                y[(size_t)(row * NV + v)] = acc[v] + at(yx, row * NV + v);
+        }
+    }
+    for (int64_t vm = 0; vm < n; ++vm) {
+        const int64_t row = vm < nflow ? (vm % ncell) * b + vm / ncell : vm;
+        for (int v = 0; v < NV; ++v) y_vm[vm * NV + v] = y[(size_t)(row * NV + v)];
+    }
+    return g_oob;
+}
+
 // Batched form: k_perm_in (padded width) -> k_spmm's index math for R
 // doubles per row -> k_perm_out.  Real operators without extras.
 extern "C" int emu_spmm_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,

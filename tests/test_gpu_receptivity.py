@@ -158,3 +158,33 @@ def test_factored_real_upload_equals_complex_upload(orc, N, shock):
     finally:
         fr.close()
         fc.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,shock,nrhs", [(3, True, 8), (8, False, 16), (32, True, 8)])
+def test_assembled_wide_batches_replicated_blocks(orc, N, shock, nrhs):
+    """The assembled extended operator at 16 and 32 doubles per row, where
+    the library applies the N copies of B once (column sums of the
+    freestream blocks forward, shared rows in the adjoint): every column
+    equals the oracle's POWER_METHOD! on L_ (1e-12) and the single-vector
+    action, which keeps the copies (1e-13)."""
+    from hymor_amd.core import DeviceOperator
+    nchi, neta = 9, 6
+    say(f"[assembled receptivity, wide batches] N={N} shock={shock} nrhs={nrhs}")
+    base = ops.stencil_operator(nchi, neta, 4, seed=44, n_shock=nchi if shock else 0,
+                                row_sum=2.0)
+    L_ = ops.receptivity_extension(base, np.linspace(0.0, 5.0, N))
+    A_ = L_.to_scipy_varmajor()
+    Ao = orc.CSC.from_scipy(A_)
+    Aot = orc.csc_adjoint(Ao)
+    r = np.random.default_rng(60 + N)
+    X = r.standard_normal((L_.n, nrhs)) + 1j * r.standard_normal((L_.n, nrhs))
+    dt, nsub = 0.05, 3
+    with DeviceOperator.from_blocks(L_, scale=dt) as dev:
+        for tr, M in ((False, Ao), (True, Aot)):
+            Y = dev.expm_taylor_batched(X, order=5, nsub=nsub, dt=1.0, transpose=tr)
+            for j in range(nrhs):
+                ref = orc.power_method(M.scaled(dt), X[:, j], nsub)
+                assert relerr(Y[:, j], ref) <= 1e-12, (tr, j)
+                single = dev.expm_taylor(X[:, j], order=5, nsub=nsub, dt=1.0, transpose=tr)
+                assert relerr(Y[:, j], single) <= 1e-13, (tr, j)

@@ -79,6 +79,65 @@ def test_emulated_complex_receptivity(emu, transpose):
     assert np.allclose(y, ref, rtol=1e-13, atol=1e-13)
 
 
+def _run_rep(emu, op, transpose, x):
+    colptr, rowval, nzval = op.to_csc_varmajor(one_based=True)
+    cplx = np.iscomplexobj(nzval)
+    x = np.ascontiguousarray(x, dtype=np.complex128 if cplx else np.float64)
+    y = np.zeros_like(x)
+    info = np.zeros(4, dtype=np.int64)
+    emu.emu_spmv_csc_rep.restype = C.c_int
+    oob = emu.emu_spmv_csc_rep(C.c_int64(op.n), C.c_void_p(colptr.ctypes.data),
+                               C.c_void_p(rowval.ctypes.data),
+                               C.c_void_p(np.ascontiguousarray(nzval).ctypes.data), C.c_int(1),
+                               C.c_int(int(cplx)), C.c_int(op.b), C.c_int64(op.ncell),
+                               C.c_int(int(transpose)), C.c_void_p(x.ctypes.data),
+                               C.c_void_p(y.ctypes.data), C.c_void_p(info.ctypes.data))
+    return y, oob, info
+
+
+@pytest.mark.parametrize("nchi,neta,shock,w", [(6, 4, True, [0.0, 1.0, 2.5]),
+                                               (5, 3, False, [0.3, 0.7]),
+                                               (8, 5, True, list(np.linspace(0.1, 4.0, 9)))])
+@pytest.mark.parametrize("transpose", [False, True])
+def test_emulated_receptivity_replicas(emu, nchi, neta, shock, w, transpose):
+    """The receptivity operator's N copies of B are found at upload (forward:
+    N copies of p = 4 nchi freestream columns after the flow and shock DOFs;
+    adjoint: N copies of its rows) and the compressed arithmetic -- column
+    sums of the freestream blocks, shared rows finished with their own
+    diagonal -- gives scipy's SpMV of the extended operator."""
+    base = ops.stencil_operator(nchi, neta, 4, seed=9, n_shock=nchi if shock else 0)
+    L_ = ops.receptivity_extension(base, w)
+    A = L_.to_scipy_varmajor()
+    r = np.random.default_rng(2)
+    x = r.standard_normal(L_.n) + 1j * r.standard_normal(L_.n)
+    y, oob, info = _run_rep(emu, L_, transpose, x)
+    assert oob == 0
+    assert list(info) == [2 if transpose else 1, base.n, 4 * nchi, len(w)]
+    ref = (A.conj().T if transpose else A) @ x
+    assert np.allclose(y, ref, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("case", CASES[:3], ids=lambda c: f"{c['nchi']}x{c['neta']}b{c['b']}")
+def test_emulated_no_replicas_elsewhere(emu, case):
+    """Operators without repeated freestream blocks are left as they are."""
+    c = dict(case)
+    op = ops.stencil_operator(c.pop("nchi"), c.pop("neta"), c.pop("b"), c.pop("seed"), **c)
+    x = np.random.default_rng(3).standard_normal(op.n)
+    y, oob, info = _run_rep(emu, op, False, x)
+    assert oob == 0 and info[0] == 0
+    assert np.allclose(y, op.to_scipy_varmajor() @ x, rtol=1e-13, atol=1e-13)
+
+
+def test_emulated_single_wavenumber_is_not_compressed(emu):
+    """One freestream block (N = 1) has nothing to share."""
+    base = ops.stencil_operator(5, 4, 4, seed=10, n_shock=5)
+    L_ = ops.receptivity_extension(base, [1.3])
+    x = np.random.default_rng(4).standard_normal(L_.n) * (1 + 0j)
+    y, oob, info = _run_rep(emu, L_, False, x)
+    assert oob == 0 and info[0] == 0
+    assert np.allclose(y, L_.to_scipy_varmajor() @ x, rtol=1e-13, atol=1e-13)
+
+
 @pytest.mark.parametrize("b,R,nrhs", [(4, 16, 16), (4, 16, 11), (4, 32, 32), (4, 64, 40),
                                       (8, 8, 8), (8, 16, 13), (8, 32, 32), (8, 64, 64)])
 def test_emulated_batched_spmm(emu, b, R, nrhs):
