@@ -166,6 +166,62 @@ def eigen_leg(k=10, tol=1e-9, block=4, seeds=(0, 1, 2)):
 FP64_MFMA_PEAK_TFS = 78.6
 
 
+def transient_growth_leg(T=1.0, k=4, tol=1e-8, steps=5):
+    """Optimal transient growth at the 400 x 300-cell, b = 4 size with 400
+    shock DOFs (N = 480 400; Chu weights from a synthetic base flow of that
+    grid, dt = 1/64): TRANSIENT_GROWTH_DOWNSTREAM for horizon T with the
+    device Krylov solver (k gains), and the Chu sandwich it applies
+    (forward Taylor sweep, weight, adjoint sweep; T_5, n_t = T/dt
+    substeps each way) timed alone, with its weight stage timed as the
+    sandwich at n_t = 0 (permutations and the block-diagonal weight)."""
+    from hymor_amd import operators as ops
+    from hymor_amd import stability as st
+    from hymor_amd.baseflow import synthetic_solution
+    from hymor_amd.core import DeviceOperator, DeviceVector, chu_sandwich_ptr
+
+    nchi, neta = 400, 300
+    s = st.CFL_TIMESTEP(synthetic_solution(nchi, neta, seed=7, perturb_shock=True, cfl=100.0,
+                                           max_dt=1.0 / 64))
+    op = ops.config_operator("C3")
+    A = op.to_scipy_varmajor()
+    R = st.CONSTRUCT_R(s)
+    C = (R.T @ st.CONSTRUCT_M(s, [1, 1, 1, 1]) @ R).tocsc()
+    dt = s["time_integration"]["dt"]
+    n_t = int(round(T / dt))
+    out = {"workload": (f"400x300 cells, b=4, 400 shock DOFs, N={op.n}; Chu weights of a "
+                        f"synthetic base flow; T={T}, dt={dt}, n_t={n_t}, order 5"),
+           "k": k, "tol": tol}
+    t0 = time.perf_counter()
+    _, G, _ = st.TRANSIENT_GROWTH_DOWNSTREAM(A, s, k, T, tol=tol, krylov="device", ncv=24,
+                                             maxiter=400)
+    out["solve_seconds"] = time.perf_counter() - t0
+    out["gains"] = [float(g) for g in G]
+    x = np.random.default_rng(8).standard_normal(op.n)
+    with DeviceOperator.from_blocks(op, scale=dt) as dA, \
+            DeviceOperator.from_scipy(C, nchi=nchi, neta=neta) as dC:
+        dx, dy = DeviceVector.from_host(x), DeviceVector(op.n)
+        try:
+            def timed(nsub):
+                chu_sandwich_ptr(dA, dC, dx.ptr, dy.ptr, order=5, nsub=nsub)
+                dA.synchronize()
+                t1 = time.perf_counter()
+                for _ in range(steps):
+                    chu_sandwich_ptr(dA, dC, dx.ptr, dy.ptr, order=5, nsub=nsub)
+                dA.synchronize()
+                return (time.perf_counter() - t1) / steps
+            ts, tw = timed(n_t), timed(0)
+        finally:
+            dx.free()
+            dy.free()
+    out["sandwich_ms"] = ts * 1e3
+    out["sandwich_weight_stage_ms"] = tw * 1e3
+    out["sandwich_terms"] = 2 * 5 * n_t
+    out["sandwich_note"] = "device-resident vectors (hymor_chu_sandwich_dev)"
+    log(f"transient growth: solve {out['solve_seconds']:.2f} s, gains {out['gains']}, "
+        f"sandwich {ts * 1e3:.2f} ms, weight stage {tw * 1e3:.3f} ms")
+    return out
+
+
 def small_config_leg(steps=20, order=12, nsub=64, T=1.0):
     """The launch-bound end: the same action T_12(A T/64)^64 v (768 Taylor
     terms) on the 100 x 80-cell, b = 4 operator (N = 32 000, 1.13M
@@ -503,6 +559,8 @@ def main():
                     help="skip the C4 receptivity leg (assembled vs factored)")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the launch-bound 100x80-cell leg (eager / graph / no-alloc)")
+    ap.add_argument("--no-growth", action="store_true",
+                    help="skip the 400x300-cell transient-growth leg")
     ap.add_argument("--kernel-only",
                     choices=["plain", "batched", "block4", "receptivity", "receptivity16",
                              "krylov"],
@@ -750,6 +808,8 @@ def main():
     }
     if not args.no_small:
         result["small_config"] = small_config_leg()
+    if not args.no_growth:
+        result["transient_growth"] = transient_growth_leg()
     if not args.no_receptivity:
         result["batched_receptivity"] = receptivity_leg()
     if not args.no_eigen:
