@@ -324,7 +324,7 @@ def krylov_projection_leg(n=1440000, j=32, widths=(4, 16), reps=20):
                     "mfma_utilisation": fl_mfma / (ms_vtw * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFS,
                     "algorithmic_bytes": by_vtw, "flops": fl_vtw, "mfma_flops_issued": fl_mfma},
             "reduce": {"kernel": "k_reduce_vtw", "ms": ms_red},
-            "sub": {"kernel": f"k_sub_vh_blk<{8 if s <= 8 else 16}>", "ms": ms_sub,
+            "sub": {"kernel": "k_sub_vh_mfma<8>", "ms": ms_sub,
                     "GBps": by_sub / (ms_sub * 1e-3) / 1e9,
                     "frac_of_hbm_peak": by_sub / (ms_sub * 1e-3) / 1e9 / HBM_PEAK_GBS,
                     "TFps": fl_sub / (ms_sub * 1e-3) / 1e12, "algorithmic_bytes": by_sub},

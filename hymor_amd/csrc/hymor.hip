@@ -2060,10 +2060,10 @@ void launch_zero(hymor_krylov *K, double *p, int64_t len) {
     HIP_TRY(hipGetLastError());
 }
 
-constexpr int kVtwGroups = 512;   // row chunks of the block projection kernel
+constexpr int kVtwGroups = 1024;   // row chunks of the block projection kernel
 
-// Row split of the V^T W partial products: kVtwGroups workgroups of a
-// multiple of 16 rows each
+// Row split of the V^T W partial products: at most kVtwGroups workgroups of
+// a multiple of 64 rows each (16 per wave)
 struct VtwShape {
     int ctiles;
     int64_t rows_per, groups;
@@ -2072,7 +2072,7 @@ struct VtwShape {
 VtwShape vtw_shape(const hymor_krylov *K, int ncol) {
     VtwShape v;
     v.ctiles = (ncol + 15) / 16;
-    v.rows_per = ((K->n + kVtwGroups - 1) / kVtwGroups + 15) / 16 * 16;
+    v.rows_per = ((K->n + kVtwGroups - 1) / kVtwGroups + 63) / 64 * 64;
     v.groups = (K->n + v.rows_per - 1) / v.rows_per;
     return v;
 }
@@ -2082,12 +2082,23 @@ void launch_vtw(hymor_krylov *K, const VtwShape &v, int ncol, const double *W, i
                 int s) {
     const bool aligned = K->n % 2 == 0 && ldw % 2 == 0 &&
                          ((uintptr_t)K->V % 16) == 0 && ((uintptr_t)W % 16) == 0;
-    if (aligned)
-        hipLaunchKernelGGL(k_vtw_mfma<true>, dim3((unsigned)v.groups), dim3(256), 0, K->stream,
-                           K->V, K->n, ncol, W, ldw, s, K->n, v.rows_per, v.ctiles, K->bpartial);
-    else
-        hipLaunchKernelGGL(k_vtw_mfma<false>, dim3((unsigned)v.groups), dim3(256), 0, K->stream,
-                           K->V, K->n, ncol, W, ldw, s, K->n, v.rows_per, v.ctiles, K->bpartial);
+    auto go = [&](auto al, auto ct) {
+        constexpr bool AL = decltype(al)::value;
+        constexpr int CT = decltype(ct)::value;
+        hipLaunchKernelGGL((k_vtw_mfma<AL, CT>), dim3((unsigned)v.groups), dim3(256), 0,
+                           K->stream, K->V, K->n, ncol, W, ldw, s, K->n, v.rows_per, v.ctiles,
+                           K->bpartial);
+    };
+    // two column tiles per pass up to 32 basis columns, four beyond
+    using Two = std::integral_constant<int, 2>;
+    using Four = std::integral_constant<int, 4>;
+    if (aligned) {
+        if (v.ctiles <= 2) go(std::true_type{}, Two{});
+        else go(std::true_type{}, Four{});
+    } else {
+        if (v.ctiles <= 2) go(std::false_type{}, Two{});
+        else go(std::false_type{}, Four{});
+    }
     HIP_TRY(hipGetLastError());
 }
 
@@ -2116,14 +2127,12 @@ void ensure_block_bufs(hymor_krylov *K) {
 
 // W[:, :s] -= V[:, :ncol] h
 void block_sub(hymor_krylov *K, int ncol, const double *h, double *W, int64_t ldw, int s) {
-    const int64_t grid = std::min<int64_t>((K->n + 255) / 256, 4096);
+    // one wave per 16-row tile, four waves per workgroup
+    const int64_t tiles = (K->n + 15) / 16;
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 4096));
     const size_t lds = (size_t)ncol * s * sizeof(double);
-    if (s <= 8)
-        hipLaunchKernelGGL(k_sub_vh_blk<8>, dim3((unsigned)grid), dim3(256), lds, K->stream, K->V,
-                           K->n, ncol, h, W, ldw, s, K->n);
-    else
-        hipLaunchKernelGGL(k_sub_vh_blk<16>, dim3((unsigned)grid), dim3(256), lds, K->stream,
-                           K->V, K->n, ncol, h, W, ldw, s, K->n);
+    hipLaunchKernelGGL(k_sub_vh_mfma<8>, dim3((unsigned)grid), dim3(256), lds, K->stream, K->V,
+                       K->n, ncol, h, W, ldw, s, K->n);
     HIP_TRY(hipGetLastError());
 }
 

@@ -885,23 +885,52 @@ __global__ __launch_bounds__(256) void k_basis_combine(const double *__restrict_
                                                        int m, const double *__restrict__ Q, int k,
                                                        double *__restrict__ out, int64_t ldo,
                                                        int64_t n) {
+    // outputs in chunks of CK: each chunk reads the row's m basis values
+    // once (not once per output column), Q's entries are the same for all
+    // threads; every output is summed over j in order
+    constexpr int CK = 16;
     for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n;
          row += (int64_t)gridDim.x * blockDim.x) {
-        for (int c = 0; c < k; ++c) {
+        for (int c0 = 0; c0 < k; c0 += CK) {
+            const int nc = k - c0 < CK ? k - c0 : CK;
             if constexpr (CPLX) {
-                double sr = 0.0, si = 0.0;
+                double sr[CK], si[CK];
+#pragma unroll
+                for (int c = 0; c < CK; ++c) sr[c] = si[c] = 0.0;
                 for (int j = 0; j < m; ++j) {
                     const double *v = V + 2 * ((int64_t)j * ld + row);
-                    const double qr = Q[2 * ((int64_t)c * m + j)], qi = Q[2 * ((int64_t)c * m + j) + 1];
-                    sr = fma(v[0], qr, fma(-v[1], qi, sr));
-                    si = fma(v[0], qi, fma(v[1], qr, si));
+                    const double vr = v[0], vi = v[1];
+#pragma unroll
+                    for (int c = 0; c < CK; ++c) {
+                        if (c >= nc) break;
+                        const double *q = Q + 2 * ((int64_t)(c0 + c) * m + j);
+                        sr[c] = fma(vr, q[0], fma(-vi, q[1], sr[c]));
+                        si[c] = fma(vr, q[1], fma(vi, q[0], si[c]));
+                    }
                 }
-                out[2 * ((int64_t)c * ldo + row)] = sr;
-                out[2 * ((int64_t)c * ldo + row) + 1] = si;
+#pragma unroll
+                for (int c = 0; c < CK; ++c) {
+                    if (c >= nc) break;
+                    out[2 * ((int64_t)(c0 + c) * ldo + row)] = sr[c];
+                    out[2 * ((int64_t)(c0 + c) * ldo + row) + 1] = si[c];
+                }
             } else {
-                double s = 0.0;
-                for (int j = 0; j < m; ++j) s = fma(V[(int64_t)j * ld + row], Q[(int64_t)c * m + j], s);
-                out[(int64_t)c * ldo + row] = s;
+                double s[CK];
+#pragma unroll
+                for (int c = 0; c < CK; ++c) s[c] = 0.0;
+                for (int j = 0; j < m; ++j) {
+                    const double vj = V[(int64_t)j * ld + row];
+#pragma unroll
+                    for (int c = 0; c < CK; ++c) {
+                        if (c >= nc) break;
+                        s[c] = fma(vj, Q[(int64_t)(c0 + c) * m + j], s[c]);
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < CK; ++c) {
+                    if (c >= nc) break;
+                    out[(int64_t)(c0 + c) * ldo + row] = s[c];
+                }
             }
         }
     }
@@ -1877,63 +1906,101 @@ __global__ __launch_bounds__(256) void k_spmm_pair(
 // the 4-row index being a dummy summation index.  Lane (m = lane % 16,
 // kk = lane / 16) supplies V[row, c0 + m] and W[row, m] for rows
 // r + 4 kk + i (i = 0..3, adjacent: two 16-byte loads each when ALIGNED).
-// Workgroup g covers rows [g*rows_per, (g+1)*rows_per) (rows_per a
-// multiple of 16), wave w the 16-column tiles w, w+4, ...; partial
-// [(g*ctiles*16 + c)*16 + n] holds its sum for basis column c, block column n.
-template <bool ALIGNED>
+// Workgroup g covers rows [g*rows_per, (g+1)*rows_per) (rows_per a multiple
+// of 64), each of its four waves a quarter of them; a wave computes CT
+// 16-column tiles of V at a time against its rows of W, so W is read once
+// per CT tiles and every wave has work whatever ncol is.  The four waves'
+// products meet in LDS in wave order; partial [(g*ctiles*16 + c)*16 + n]
+// holds the workgroup's sum for basis column c, block column n.
+template <bool ALIGNED, int CT>
 __global__ __launch_bounds__(256) void k_vtw_mfma(const double *__restrict__ V, int64_t ldv,
                                                   int ncol, const double *__restrict__ W,
                                                   int64_t ldw, int s, int64_t n,
                                                   int64_t rows_per, int ctiles,
                                                   double *__restrict__ partial) {
+    __shared__ double red[3][CT][4][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m = lane & 15, kk = lane >> 4;
     const int64_t g = blockIdx.x;
-    const int64_t r0 = g * rows_per;
-    const int64_t r1 = r0 + rows_per < n ? r0 + rows_per : n;
-    const int64_t rfull = r0 + ((r1 > r0 ? r1 - r0 : 0) / 16) * 16;
+    const int64_t rw = rows_per / 4;
+    const int64_t r0 = g * rows_per + wave * rw;
+    const int64_t r1 = r0 + rw < n ? r0 + rw : n;
+    const int64_t rfull = r1 > r0 ? r0 + ((r1 - r0) / 16) * 16 : r0;
     const bool wv = m < s;
     const double *wp = W + (int64_t)(wv ? m : 0) * ldw;
-    for (int ct = wave; ct < ctiles; ct += 4) {
-        const int c = ct * 16 + m;
-        const bool cv = c < ncol;
-        const double *vp = V + (int64_t)(cv ? c : 0) * ldv;
-        v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int ct0 = 0; ct0 < ctiles; ct0 += CT) {
+        v4d acc[CT];
+        const double *vp[CT];
+        bool cv[CT];
+#pragma unroll
+        for (int u = 0; u < CT; ++u) {
+            acc[u] = v4d{0.0, 0.0, 0.0, 0.0};
+            const int c = (ct0 + u) * 16 + m;
+            cv[u] = ct0 + u < ctiles && c < ncol;
+            vp[u] = V + (int64_t)(cv[u] ? c : 0) * ldv;
+        }
         for (int64_t r = r0; r < rfull; r += 16) {
             const int64_t rr = r + 4 * kk;
-            double a[4], b[4];
+            double b[4];
             if constexpr (ALIGNED) {
-                const d2 a0 = *reinterpret_cast<const d2 *>(vp + rr);
-                const d2 a1 = *reinterpret_cast<const d2 *>(vp + rr + 2);
                 const d2 b0 = *reinterpret_cast<const d2 *>(wp + rr);
                 const d2 b1 = *reinterpret_cast<const d2 *>(wp + rr + 2);
-                a[0] = a0.x; a[1] = a0.y; a[2] = a1.x; a[3] = a1.y;
                 b[0] = b0.x; b[1] = b0.y; b[2] = b1.x; b[3] = b1.y;
             } else {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    a[i] = vp[rr + i];
-                    b[i] = wp[rr + i];
+                for (int i = 0; i < 4; ++i) b[i] = wp[rr + i];
+            }
+            double a[CT][4];
+#pragma unroll
+            for (int u = 0; u < CT; ++u) {
+                if constexpr (ALIGNED) {
+                    const d2 a0 = *reinterpret_cast<const d2 *>(vp[u] + rr);
+                    const d2 a1 = *reinterpret_cast<const d2 *>(vp[u] + rr + 2);
+                    a[u][0] = a0.x; a[u][1] = a0.y; a[u][2] = a1.x; a[u][3] = a1.y;
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) a[u][i] = vp[u][rr + i];
                 }
             }
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(cv ? a[i] : 0.0, wv ? b[i] : 0.0, acc,
-                                                           0, 0, 0);
+            for (int u = 0; u < CT; ++u)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(cv[u] ? a[u][i] : 0.0,
+                                                                  wv ? b[i] : 0.0, acc[u], 0, 0, 0);
         }
-        // tail rows (< 16, only in the last workgroup): clamped reads, zeroed
+        // tail rows (< 16, only in the last wave with rows): clamped reads, zeroed
         for (int64_t r = rfull; r < r1; r += 4) {
             const int64_t row = r + kk;
             const bool ok = row < r1;
             const int64_t rc = ok ? row : r1 - 1;
-            const double a = (cv && ok) ? vp[rc] : 0.0;
             const double b = (wv && ok) ? wp[rc] : 0.0;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-        }
-        // D[c][n] sits in lane n + 16 (c % 4), register c / 4
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-            partial[((g * ctiles + ct) * 16 + 4 * r + kk) * 16 + m] = acc[r];
+            for (int u = 0; u < CT; ++u) {
+                const double a = (cv[u] && ok) ? vp[u][rc] : 0.0;
+                acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[u], 0, 0, 0);
+            }
+        }
+        if (wave > 0) {
+#pragma unroll
+            for (int u = 0; u < CT; ++u)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) red[wave - 1][u][r][lane] = acc[u][r];
+        }
+        __syncthreads();
+        if (wave == 0) {
+            // D[c][n] sits in lane n + 16 (c % 4), register c / 4
+#pragma unroll
+            for (int u = 0; u < CT; ++u) {
+                if (ct0 + u >= ctiles) break;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    partial[((g * ctiles + ct0 + u) * 16 + 4 * r + kk) * 16 + m] =
+                        ((acc[u][r] + red[0][u][r][lane]) + red[1][u][r][lane]) +
+                        red[2][u][r][lane];
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -1954,29 +2021,50 @@ __global__ __launch_bounds__(256) void k_reduce_vtw(const double *__restrict__ p
     if (lane == 0) h[i] = acc;
 }
 
-// W[row, nn] -= sum_c V[row, c] h[nn*ncol + c]   (nn < S), one thread per row
-template <int S>
-__global__ __launch_bounds__(256) void k_sub_vh_blk(const double *__restrict__ V, int64_t ldv,
-                                                    int ncol, const double *__restrict__ h,
-                                                    double *__restrict__ W, int64_t ldw, int s,
-                                                    int64_t n) {
+// W[:, :s] -= V[:, :ncol] h on the FP64 matrix cores, formed transposed:
+// (V h)^T = h^T V^T, one v_mfma_f64_16x16x4f64 per 16 block columns x 16
+// rows x 4 basis columns with A[m][k] = h[c0 + k, m] (staged in LDS; h is
+// column-major, h[m * ncol + c]) and B[k][n] = V[r + n, c0 + k].  Lane
+// (n, kk) loads V[r + n] of basis column c0 + kk -- 16 consecutive rows per
+// column, 128-byte runs -- and afterwards holds (V h)[r + n, 4 reg + kk] in
+// register reg, so the read-modify-write of W runs along rows as well.  One
+// wave per 16-row tile (grid-stride); the loads of KU groups of 4 basis
+// columns are issued before their products.
+template <int KU>
+__global__ __launch_bounds__(256) void k_sub_vh_mfma(const double *__restrict__ V, int64_t ldv,
+                                                     int ncol, const double *__restrict__ h,
+                                                     double *__restrict__ W, int64_t ldw, int s,
+                                                     int64_t n) {
     extern __shared__ double hs[];
     for (int i = threadIdx.x; i < ncol * s; i += blockDim.x) hs[i] = h[i];
     __syncthreads();
-    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n;
-         row += (int64_t)gridDim.x * blockDim.x) {
-        double acc[S];
+    const int lane = threadIdx.x & 63;
+    const int nl = lane & 15, kk = lane >> 4;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int64_t ntiles = (n + 15) / 16;
+    for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+         tile < ntiles; tile += nwaves) {
+        const int64_t row = tile * 16 + nl;
+        const bool rok = row < n;
+        v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
+        for (int c0 = 0; c0 < ncol; c0 += 4 * KU) {
+            double a[KU], b[KU];
 #pragma unroll
-        for (int nn = 0; nn < S; ++nn) acc[nn] = 0.0;
-        for (int c = 0; c < ncol; ++c) {
-            const double v = V[(int64_t)c * ldv + row];
+            for (int u = 0; u < KU; ++u) {
+                const int c = c0 + 4 * u + kk;
+                const bool cok = c < ncol;
+                b[u] = (cok && rok) ? V[(int64_t)c * ldv + row] : 0.0;
+                a[u] = (cok && nl < s) ? hs[nl * ncol + c] : 0.0;
+            }
 #pragma unroll
-            for (int nn = 0; nn < S; ++nn)
-                if (nn < s) acc[nn] = fma(v, hs[nn * ncol + c], acc[nn]);
+            for (int u = 0; u < KU; ++u)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
         }
 #pragma unroll
-        for (int nn = 0; nn < S; ++nn)
-            if (nn < s) W[(int64_t)nn * ldw + row] -= acc[nn];
+        for (int reg = 0; reg < 4; ++reg) {
+            const int m = 4 * reg + kk;
+            if (m < s && rok) W[(int64_t)m * ldw + row] -= acc[reg];
+        }
     }
 }
 
