@@ -299,11 +299,12 @@ def krylov_projection_leg(n=1440000, j=32, widths=(4, 16), reps=20):
     """The dense Krylov projections of the batched eigen leg at its size:
     one block Gram-Schmidt pass of s new vectors against j basis vectors of
     length n (hymor_bench_krylov_projection): V^T W on the FP64 matrix
-    cores, the fixed-order reduction of its partials, and W -= V h.  Each
-    kernel's time comes from HIP events over `reps` launches; GB/s and
-    TF/s are algorithmic bytes and flops over that time, matrix-core
-    utilisation the flops the matrix cores execute (products always 16 x 16
-    wide) over time and the FP64 matrix-core peak."""
+    cores, the fixed-order reduction of its partials, and W -= V h, also on
+    the matrix cores.  Each kernel's time comes from HIP events over `reps`
+    launches; GB/s and TF/s are algorithmic bytes and flops over that time,
+    matrix-core utilisation the flops the matrix cores execute (products
+    always 16 x 16 wide, so 512 n ceil(j / 16) for either kernel) over time
+    and the FP64 matrix-core peak."""
     import ctypes
 
     from hymor_amd import _lib
@@ -316,7 +317,7 @@ def krylov_projection_leg(n=1440000, j=32, widths=(4, 16), reps=20):
         ms_vtw, ms_red, ms_sub, by_vtw, fl_vtw, fl_mfma, by_sub, fl_sub = list(out)
         cases.append({
             "s": s,
-            "vtw": {"kernel": "k_vtw_mfma<true>", "ms": ms_vtw,
+            "vtw": {"kernel": "k_vtw_mfma<true, 2>", "ms": ms_vtw,
                     "GBps": by_vtw / (ms_vtw * 1e-3) / 1e9,
                     "frac_of_hbm_peak": by_vtw / (ms_vtw * 1e-3) / 1e9 / HBM_PEAK_GBS,
                     "useful_TFps": fl_vtw / (ms_vtw * 1e-3) / 1e12,
@@ -327,12 +328,14 @@ def krylov_projection_leg(n=1440000, j=32, widths=(4, 16), reps=20):
             "sub": {"kernel": "k_sub_vh_mfma<8>", "ms": ms_sub,
                     "GBps": by_sub / (ms_sub * 1e-3) / 1e9,
                     "frac_of_hbm_peak": by_sub / (ms_sub * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                    "TFps": fl_sub / (ms_sub * 1e-3) / 1e12, "algorithmic_bytes": by_sub},
+                    "TFps": fl_sub / (ms_sub * 1e-3) / 1e12,
+                    "mfma_utilisation": fl_mfma / (ms_sub * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFS,
+                    "algorithmic_bytes": by_sub, "flops": fl_sub, "mfma_flops_issued": fl_mfma},
         })
         log(f"krylov projection s={s}: V^T W {ms_vtw * 1e3:.1f} us "
             f"({cases[-1]['vtw']['GBps']:.0f} GB/s, MFMA {cases[-1]['vtw']['mfma_utilisation']:.1%}), "
             f"reduce {ms_red * 1e3:.1f} us, W -= V h {ms_sub * 1e3:.1f} us "
-            f"({cases[-1]['sub']['GBps']:.0f} GB/s)")
+            f"({cases[-1]['sub']['GBps']:.0f} GB/s, MFMA {cases[-1]['sub']['mfma_utilisation']:.1%})")
     return {"n": n, "basis_columns": j, "fp64_mfma_peak_TFps": FP64_MFMA_PEAK_TFS,
             "bound": "hbm (V^T W at s = 16: 2 s j / ((j + s) 8) = 2.7 flop/B, below the "
                      "FP64 ridge of 9.8)",

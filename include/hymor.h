@@ -359,11 +359,12 @@ int hymor_krylov_copy_column(hymor_krylov *K, int src, int dst);
  * of j columns and a block W of s <= 16 columns, length n, filled on the
  * device; each kernel launched `reps` times between HIP events.
  *   out[0] ms per launch of the V^T W partial products (FP64 matrix cores),
- *   out[1] ms of their reduction, out[2] ms of W -= V h,
+ *   out[1] ms of their reduction, out[2] ms of W -= V h (FP64 matrix cores),
  *   out[3] V^T W algorithmic bytes ((j + s) n 8), out[4] its flops
- *   (2 n j s), out[5] its matrix-core flops as issued (512 n ceil(j / 16):
- *   the products always span 16 basis and 16 block columns),
- *   out[6] W -= V h algorithmic bytes ((j + 2 s) n 8), out[7] its flops. */
+ *   (2 n j s), out[5] the matrix-core flops either product kernel issues
+ *   (512 n ceil(j / 16): the products always span 16 basis and 16 block
+ *   columns), out[6] W -= V h algorithmic bytes ((j + 2 s) n 8), out[7] its
+ *   flops (2 n j s). */
 int hymor_bench_krylov_projection(int64_t n, int j, int s, int reps, double *out);
 
 #ifdef __cplusplus
