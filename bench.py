@@ -309,12 +309,18 @@ def krylov_projection_leg(n=1440000, j=32, widths=(4, 16), reps=20):
 
     from hymor_amd import _lib
 
+    from hymor_amd.calibrate import mixed_bandwidth
+
     lib = _lib.load()
     cases = []
     for s in widths:
         out = (ctypes.c_double * 8)()
         _lib.check(lib.hymor_bench_krylov_projection(n, j, s, reps, out))
         ms_vtw, ms_red, ms_sub, by_vtw, fl_vtw, fl_mfma, by_sub, fl_sub = list(out)
+        # streaming ceilings at each kernel's read/write mix (V^T W only
+        # reads; W -= V h writes its s columns)
+        mix_vtw = mixed_bandwidth(by_vtw, 0, 20)["GBps"]
+        mix_sub = mixed_bandwidth(by_sub - s * n * 8.0, s * n * 8.0, 20)["GBps"]
         cases.append({
             "s": s,
             "vtw": {"kernel": "k_vtw_mfma<true, 2>", "ms": ms_vtw,
@@ -323,14 +329,18 @@ def krylov_projection_leg(n=1440000, j=32, widths=(4, 16), reps=20):
                     "useful_TFps": fl_vtw / (ms_vtw * 1e-3) / 1e12,
                     "mfma_TFps_issued": fl_mfma / (ms_vtw * 1e-3) / 1e12,
                     "mfma_utilisation": fl_mfma / (ms_vtw * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFS,
-                    "algorithmic_bytes": by_vtw, "flops": fl_vtw, "mfma_flops_issued": fl_mfma},
+                    "algorithmic_bytes": by_vtw, "flops": fl_vtw, "mfma_flops_issued": fl_mfma,
+                    "same_mix_stream_GBps": mix_vtw,
+                    "frac_of_same_mix_stream": by_vtw / (ms_vtw * 1e-3) / 1e9 / mix_vtw},
             "reduce": {"kernel": "k_reduce_vtw", "ms": ms_red},
             "sub": {"kernel": "k_sub_vh_mfma<8>", "ms": ms_sub,
                     "GBps": by_sub / (ms_sub * 1e-3) / 1e9,
                     "frac_of_hbm_peak": by_sub / (ms_sub * 1e-3) / 1e9 / HBM_PEAK_GBS,
                     "TFps": fl_sub / (ms_sub * 1e-3) / 1e12,
                     "mfma_utilisation": fl_mfma / (ms_sub * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFS,
-                    "algorithmic_bytes": by_sub, "flops": fl_sub, "mfma_flops_issued": fl_mfma},
+                    "algorithmic_bytes": by_sub, "flops": fl_sub, "mfma_flops_issued": fl_mfma,
+                    "write_share": s * n * 8.0 / by_sub, "same_mix_stream_GBps": mix_sub,
+                    "frac_of_same_mix_stream": by_sub / (ms_sub * 1e-3) / 1e9 / mix_sub},
         })
         log(f"krylov projection s={s}: V^T W {ms_vtw * 1e3:.1f} us "
             f"({cases[-1]['vtw']['GBps']:.0f} GB/s, MFMA {cases[-1]['vtw']['mfma_utilisation']:.1%}), "
@@ -722,6 +732,13 @@ def main():
         devb.close()
         b_aps = nb * ksteps / tb
         b_gbs = bk_bytes / (bk_ms * 1e-3) / 1e9
+        # a Horner term writes its nb new vectors (n x nb doubles) beside what
+        # it reads: the streaming ceiling at that read/write mix
+        from hymor_amd.calibrate import mixed_bandwidth
+        b_write = float(op.n) * nb * 8
+        mix = mixed_bandwidth(bk_bytes - b_write, b_write, 20)
+        log(f"batched term: {b_gbs:.0f} GB/s; streaming reads + writes at its write share "
+            f"{b_write / bk_bytes:.1%}: {mix['GBps']:.0f} GB/s")
         batched = {
             "value": b_aps,
             "unit": "exp(AT)v actions/s",
@@ -757,6 +774,8 @@ def main():
             "algorithmic_bytes_per_launch": bk_bytes,
             "achieved_GBps": b_gbs,
             "frac_of_hbm_peak": b_gbs / HBM_PEAK_GBS,
+            "same_mix_stream": {"kernel": "k_stream_mix", "write_share": b_write / bk_bytes,
+                                "GBps": mix["GBps"], "frac": b_gbs / mix["GBps"]},
         }
         log(f"batched: {b_aps:.2f} actions/s ({b_aps / actions_per_s:.2f}x plain), err {err:.2e}")
 

@@ -69,6 +69,7 @@ SIGNATURES = {
     "hymor_op_set_tuning": (C.c_int, [_p, C.c_int, C.c_int, C.c_int]),
     "hymor_bench_copy": (C.c_int, [_i64, C.c_int, _pd]),
     "hymor_bench_read": (C.c_int, [_i64, C.c_int, _pd]),
+    "hymor_bench_stream_mix": (C.c_int, [_i64, _i64, C.c_int, _pd, _pd]),
     "hymor_krylov_create": (C.c_int, [_i64, C.c_int, C.c_int, _pp]),
     "hymor_krylov_destroy": (C.c_int, [_p]),
     "hymor_krylov_column": (C.c_int, [_p, C.c_int, _pp]),

@@ -37,6 +37,19 @@ def read_bandwidth(mib: int = 1024, reps: int = 20):
             "GBps": nbytes / (ms.value * 1e-3) / 1e9, "reps": reps}
 
 
+def mixed_bandwidth(read_bytes: int, write_bytes: int, reps: int = 20):
+    """Streaming reads and writes in the proportion read_bytes : write_bytes
+    (k_stream_mix): GB/s = bytes moved / time, the ceiling for a kernel
+    whose HBM traffic has that read/write mix."""
+    ms, moved = C.c_double(), C.c_double()
+    _lib.check(_lib.load().hymor_bench_stream_mix(int(read_bytes), int(write_bytes), reps,
+                                                  C.byref(ms), C.byref(moved)))
+    return {"kernel": "k_stream_mix", "read_bytes": int(read_bytes),
+            "write_bytes": int(write_bytes), "bytes_moved": moved.value,
+            "ms_per_launch": ms.value, "GBps": moved.value / (ms.value * 1e-3) / 1e9,
+            "reps": reps}
+
+
 if __name__ == "__main__":
     mib = int(sys.argv[1]) if len(sys.argv) > 1 else 1024
     reps = int(sys.argv[2]) if len(sys.argv) > 2 else 20

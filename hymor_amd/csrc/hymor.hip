@@ -1875,6 +1875,49 @@ int hymor_bench_read(int64_t bytes, int reps, double *ms_per_launch) {
     });
 }
 
+int hymor_bench_stream_mix(int64_t read_bytes, int64_t write_bytes, int reps,
+                           double *ms_per_launch, double *bytes_moved) {
+    return guarded([&] {
+        ARG_CHECK(ms_per_launch && bytes_moved && reps >= 1, "null argument or reps < 1");
+        // chunks of whole 1 KiB runs (64 items), so no cache line is shared
+        // between two chunks
+        constexpr int64_t kChunks = 16384, kRun = 64;
+        ARG_CHECK(read_bytes >= kChunks * kRun * 16 && write_bytes >= 0,
+                  "read_bytes must be >= 16 MiB, write_bytes >= 0");
+        const int64_t rin = read_bytes / 16 / kChunks / kRun * kRun;
+        const int64_t rout = write_bytes / 16 / kChunks / kRun * kRun;
+        std::vector<void *> allocs;
+        struct Free {
+            std::vector<void *> &a;
+            ~Free() {
+                for (void *p : a) (void)hipFree(p);
+            }
+        } guard{allocs};
+        d2 *in = (d2 *)dalloc_raw(allocs, (size_t)(kChunks * rin * 16));
+        d2 *out = rout ? (d2 *)dalloc_raw(allocs, (size_t)(kChunks * rout * 16)) : nullptr;
+        HIP_TRY(hipMemset(in, 0, (size_t)(kChunks * rin * 16)));
+        const int64_t grid = 2048;
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        hipLaunchKernelGGL(k_stream_mix, dim3((unsigned)grid), dim3(256), 0, nullptr, in, rin,
+                           out, rout, kChunks);
+        HIP_TRY(hipEventRecord(e0, nullptr));
+        for (int i = 0; i < reps; ++i)
+            hipLaunchKernelGGL(k_stream_mix, dim3((unsigned)grid), dim3(256), 0, nullptr, in,
+                               rin, out, rout, kChunks);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(e1, nullptr));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms_per_launch = (double)ms / reps;
+        *bytes_moved = (double)(kChunks * (rin + rout) * 16);
+    });
+}
+
 int hymor_weights_create_csc(int64_t n, int nmat, const int64_t *const *colptr,
                              const int64_t *const *rowval, const double *const *nzval,
                              int index_base, int vars_per_cell, int64_t nchi, int64_t neta,

@@ -735,6 +735,34 @@ __global__ __launch_bounds__(256) void k_read(const d2 *__restrict__ in, double 
     out[(int64_t)blockIdx.x * 256 + threadIdx.x] = acc0 + acc1;
 }
 
+// Streaming reads and writes in a fixed proportion: chunk c (workgroups
+// grid-strided over nchunk chunks) reads rin 16-byte items of `in` (eight
+// non-temporal loads in flight per thread) and then writes rout items of
+// `out` with each thread's partial sum, so that across the launch HBM
+// serves reads and writes in the ratio rin : rout -- the ceiling for a
+// kernel whose traffic has that mix (a Horner term writes its new vectors
+// beside the values and vectors it reads).
+__global__ __launch_bounds__(256) void k_stream_mix(const d2 *__restrict__ in, int64_t rin,
+                                                    d2 *__restrict__ out, int64_t rout,
+                                                    int64_t nchunk) {
+    constexpr int U = 8;
+    for (int64_t c = blockIdx.x; c < nchunk; c += gridDim.x) {
+        const d2 *src = in + c * rin;
+        d2 acc = d2{0.0, 0.0};
+        int64_t i = threadIdx.x;
+        for (; i + (U - 1) * 256 < rin; i += U * 256) {
+            d2 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(src + i + u * 256);
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc += v[u];
+        }
+        for (; i < rin; i += 256) acc += __builtin_nontemporal_load(src + i);
+        d2 *dst = out + c * rout;
+        for (int64_t k = threadIdx.x; k < rout; k += 256) dst[k] = acc;
+    }
+}
+
 }  // namespace hymor
 
 namespace hymor {

@@ -266,6 +266,15 @@ int hymor_bench_copy(int64_t bytes, int reps, double *ms_per_launch);
  * bandwidth ceiling for the value stream of the term kernels. */
 int hymor_bench_read(int64_t bytes, int reps, double *ms_per_launch);
 
+/* Timing probe, no reference counterpart: streaming reads and writes in the
+ * proportion read_bytes : write_bytes (16 384 chunks, each read then
+ * written by one workgroup; read_bytes >= 16 MiB), the ceiling for a
+ * kernel whose HBM traffic has that mix.  Returns ms per launch and the
+ * bytes one launch moves (the requested counts rounded down to whole
+ * 1 KiB runs per chunk). */
+int hymor_bench_stream_mix(int64_t read_bytes, int64_t write_bytes, int reps,
+                           double *ms_per_launch, double *bytes_moved);
+
 /* ------------------------------------------------------------------------
  * Energy-tracked integration (Freestream_receptivity/
  * LINEAR_INTEGRATION_AND_GAINS.jl:375-477, LINEAR_INTEGRATION_AND_ENERGY_GPU

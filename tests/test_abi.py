@@ -63,4 +63,6 @@ def test_new_entry_points_check_their_arguments():
     ms = ctypes.c_double()
     assert lib.hymor_bench_read(15, 1, ctypes.byref(ms)) == -1
     assert b"multiple of 16" in lib.hymor_last_error()
+    assert lib.hymor_bench_stream_mix(1024, 0, 1, ctypes.byref(ms), ctypes.byref(ms)) == -1
+    assert b"16 MiB" in lib.hymor_last_error()
     assert lib.hymor_bench_term_kernel(None, 0, 1, 2, 1, ctypes.byref(ms), ctypes.byref(ms)) == -1
