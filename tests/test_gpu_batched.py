@@ -133,7 +133,12 @@ def test_batch_kernels_wide_rows(orc, mfma, b, nrhs):
 def test_calibration_probes():
     """The streaming copy and read probes run and report plausible rates
     (between 0.5 TB/s and the 8 TB/s HBM peak) at 256 MiB."""
-    from hymor_amd.calibrate import copy_bandwidth, read_bandwidth
+    from hymor_amd.calibrate import copy_bandwidth, mixed_bandwidth, read_bandwidth
     c = copy_bandwidth(256, 3)
     r = read_bandwidth(256, 3)
     assert 500 < c["GBps"] < 8000 and 500 < r["GBps"] < 8000, (c, r)
+    # mixed stream: 16 384 chunks of whole 1 KiB runs; 200 MiB read and
+    # 50 MiB written round down to 12 and 3 runs per chunk
+    m = mixed_bandwidth(200 << 20, 50 << 20, 3)
+    assert m["bytes_moved"] == 16384 * (12 + 3) * 1024, m
+    assert 500 < m["GBps"] < 8000, m
