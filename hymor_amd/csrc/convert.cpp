@@ -471,6 +471,45 @@ bool compress_column_replicas(HostExtra &x, int64_t n, HostReplica &rep) {
     return true;
 }
 
+bool build_fused_tables(const HostExtra &xc, const HostReplica &rep, int64_t n, int64_t nflow,
+                        HostFusedRep &out) {
+    out = HostFusedRep();
+    if (rep.kind != 1 || rep.c0 != nflow || nflow <= 0 || rep.p < 1) return false;
+    const int vs = xc.is_complex ? 2 : 1;
+    const int64_t nslice = (nflow + 63) / 64, nr = (int64_t)xc.rows.size();
+    if (nslice >= (int64_t)INT32_MAX) return false;
+    for (int64_t i = 1; i < nr; ++i)
+        if (xc.rows[(size_t)i] <= xc.rows[(size_t)i - 1]) return false;
+    HostFusedRep f;
+    f.slice_tab.assign((size_t)nslice, -1);
+    int64_t i = 0;
+    while (i < nr) {
+        const int64_t s = xc.rows[(size_t)i] / 64;
+        if (xc.rows[(size_t)i] < 0 || xc.rows[(size_t)i] >= nflow) return false;
+        f.slice_tab[(size_t)s] = (int32_t)(f.lane_ptr.size() / 65);
+        const size_t base = f.lane_ptr.size();
+        f.lane_ptr.resize(base + 65);
+        for (int l = 0; l < 64; ++l) {
+            f.lane_ptr[base + (size_t)l] = (int32_t)f.ecol.size();
+            if (i < nr && xc.rows[(size_t)i] == 64 * s + l) {
+                for (int64_t e = xc.ptr[(size_t)i]; e < xc.ptr[(size_t)i + 1]; ++e) {
+                    const int64_t q = (int64_t)xc.col[(size_t)e] - n;
+                    if (q < 0 || q >= rep.p) return false;   // not a column sum
+                    f.ecol.push_back((int32_t)q);
+                    for (int v = 0; v < vs; ++v) f.eval.push_back(xc.val[(size_t)(e * vs + v)]);
+                }
+                ++i;
+                if ((int64_t)f.ecol.size() >= (int64_t)INT32_MAX) return false;
+            }
+        }
+        f.lane_ptr[base + 64] = (int32_t)f.ecol.size();
+        // rows ascending: anything left in this slice is out of order
+        if (i < nr && xc.rows[(size_t)i] / 64 == s) return false;
+    }
+    out = std::move(f);
+    return true;
+}
+
 bool compress_row_replicas(HostExtra &x, int64_t n, int64_t c0, int64_t p, int64_t nrep,
                            HostReplica &rep) {
     const int vs = x.is_complex ? 2 : 1;

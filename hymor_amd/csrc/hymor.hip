@@ -93,6 +93,14 @@ struct DevExtra {
     ReplicaDev rep;
     double *rep_sum = nullptr;     // kind 1: p * 64 doubles of column sums
     int64_t rep_nnz = 0;           // kind 2: entries of the shared rows
+    // kind 1 forward with every extra entry on a flow row (HostFusedRep):
+    // the tables of the fused Horner term and its two column-sum buffers
+    // (p * 2 doubles each), used at two doubles per row
+    bool fused = false;
+    int64_t fr_rot = 0;   // first slice with entries
+    int32_t *fr_slice = nullptr, *fr_lane = nullptr, *fr_col = nullptr;
+    double *fr_val = nullptr;
+    double *fr_sum[2] = {nullptr, nullptr};
 };
 
 }  // namespace
@@ -359,6 +367,25 @@ void upload_replicated(hymor_op *op, const HostExtra &h, const HostReplica &rep,
     upload_replica(op, rep, d);
 }
 
+// The fused Horner term's tables (build_fused_tables) for the compressed
+// forward extras d, when the operator has them.
+void upload_fused(hymor_op *op, const HostExtra &h, const HostReplica &rep, DevExtra &d) {
+    HostFusedRep f;
+    if (rep.kind != 1 || !build_fused_tables(h, rep, op->n, op->nflow, f)) return;
+    d.fr_rot = 0;
+    for (size_t s = 0; s < f.slice_tab.size(); ++s)
+        if (f.slice_tab[s] >= 0) {
+            d.fr_rot = (int64_t)s;
+            break;
+        }
+    d.fr_slice = upload(op, f.slice_tab);
+    d.fr_lane = upload(op, f.lane_ptr);
+    d.fr_col = upload(op, f.ecol);
+    d.fr_val = upload(op, f.eval);
+    for (double *&s : d.fr_sum) s = (double *)dalloc(op, (size_t)rep.p * 2 * sizeof(double));
+    d.fused = true;
+}
+
 // Compresses the replicated freestream blocks of a receptivity operator in
 // both directions' extra CSRs (xa may be null: no adjoint).
 void find_replicas(int64_t n, HostExtra &xf, HostExtra *xa, HostReplica &rf, HostReplica &ra) {
@@ -504,7 +531,7 @@ void launch_lds(const DevSliced &s, hipStream_t st, const double *x,
     constexpr int S = (9 + NW - 1) / NW;
     hipLaunchKernelGGL((k_term_lds<B, VT, NV, NW, S, NT>), dim3((unsigned)s.nslice),
                        dim3(64 * NW), 0, st, s.slice_ptr, s.col, (const VT *)s.val, s.nslice,
-                       s.ncell, x, tout, vin, vout, c1, c2);
+                       s.ncell, x, tout, vin, vout, c1, c2, FusedRepDev{});
 }
 
 template <int B, typename VT, int NV, bool NT>
@@ -868,6 +895,78 @@ void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, const double *x
     }
 }
 
+// The fused Horner term (k_term_lds with FR != 0) applies: forward, two
+// doubles per row, tables built for the operator (build_fused_tables), no
+// per-column diagonal, and the term kernel the default tuning runs
+// (k_term_lds, one wave per slice, non-temporal value loads).
+bool fused_term_ok(const hymor_op *op, bool adj, int nv) {
+    const TermTuning &t = op->fwd.tune;
+    return !adj && nv == 2 && op->xf_rep.fused && op->ncell > 0 && !op->cdiag_on &&
+           t.split == 3 && t.waves == 1 && t.nt == 1;
+}
+
+int64_t fused_rep_waves(const DevExtra &xe) {
+    const int64_t vpw = xe.is_complex ? RepLanes<2, true>::VPW : RepLanes<2, false>::VPW;
+    return (xe.rep.p + vpw - 1) / vpw;
+}
+
+// Column sums of x's replicated rows into fr_sum[slot], before the first
+// fused term of a sweep.
+void launch_fused_prologue(hymor_op *op, hipStream_t st, const double *x, int slot) {
+    const DevExtra &xe = op->xf_rep;
+    const unsigned g = (unsigned)((fused_rep_waves(xe) + 3) / 4);
+    if (xe.is_complex)
+        hipLaunchKernelGGL((k_rep_sum<2, true>), dim3(g), dim3(256), 0, st, xe.rep, x,
+                           xe.fr_sum[slot], nullptr, nullptr, nullptr, 0.0, 0.0);
+    else
+        hipLaunchKernelGGL((k_rep_sum<2, false>), dim3(g), dim3(256), 0, st, xe.rep, x,
+                           xe.fr_sum[slot], nullptr, nullptr, nullptr, 0.0, 0.0);
+    HIP_TRY(hipGetLastError());
+}
+
+template <int B, typename VT, int FR>
+void launch_fused_t(const DevSliced &s, const FusedRepDev &F, hipStream_t st, const double *x,
+                    const double *vin, double *vout, double c1, double c2) {
+    hipLaunchKernelGGL((k_term_lds<B, VT, 2, 1, 9, true, FR>),
+                       dim3((unsigned)(F.nfs_wg + s.nslice)), dim3(64), 0, st, s.slice_ptr,
+                       s.col, (const VT *)s.val, s.nslice, s.ncell, x, nullptr, vin, vout, c1,
+                       c2, F);
+}
+
+// One Horner term w_new = v + c1 L_ w (c2 = 1) in one launch (fused_term_ok):
+// the flow rows take the column sums of w from fr_sum[slot], the replicated
+// rows leave those of w_new in fr_sum[slot ^ 1].
+void launch_term_fused(hymor_op *op, hipStream_t st, const double *x, const double *vin,
+                       double *vout, double c1, double c2, int slot) {
+    const DevSliced &s = op->fwd;
+    const DevExtra &xe = op->xf_rep;
+    FusedRepDev F;
+    F.nfs_wg = (fused_rep_waves(xe) + 7) / 8 * 8;   // one wave per workgroup
+    F.rot = xe.fr_rot;
+    F.R = xe.rep;
+    F.s_in = xe.fr_sum[slot];
+    F.s_out = xe.fr_sum[slot ^ 1];
+    F.slice_tab = xe.fr_slice;
+    F.lane_ptr = xe.fr_lane;
+    F.ecol = xe.fr_col;
+    F.eval = xe.fr_val;
+    const bool f32 = s.prec == HYMOR_PREC_F32;
+#define HYMOR_FUSED(BV, VT)                                                          \
+    (xe.is_complex ? launch_fused_t<BV, VT, 2>(s, F, st, x, vin, vout, c1, c2)       \
+                   : launch_fused_t<BV, VT, 1>(s, F, st, x, vin, vout, c1, c2))
+    if (s.b == 4) {
+        if (f32) HYMOR_FUSED(4, float);
+        else HYMOR_FUSED(4, double);
+    } else if (s.b == 8) {
+        if (f32) HYMOR_FUSED(8, float);
+        else HYMOR_FUSED(8, double);
+    } else {
+        throw ArgError("vars_per_cell must be 4 or 8");
+    }
+#undef HYMOR_FUSED
+    HIP_TRY(hipGetLastError());
+}
+
 // width: doubles per internal row (>= nrhs*ncomp; the rest is zero padding)
 void launch_perm_in(hymor_op *op, hipStream_t st, const double *x_vm, int64_t ld, double *out,
                     int nrhs, int ncomp, int64_t nzero, int width) {
@@ -911,7 +1010,14 @@ const double *run_taylor_in(hymor_op *op, const SweepBufs &b, bool adj, int orde
     if (form == HYMOR_FORM_HORNER) {
         // w = v; w = v + (dt/j) A w for j = order, ..., 1.  Each term reads
         // the gathered w and v and writes the new w; v (the substep's input)
-        // is never written, w ping-pongs between two other buffers
+        // is never written, w ping-pongs between two other buffers.  With
+        // the replicated rows fused into the term (one launch per term), the
+        // column sums of each term's x alternate between two buffers: the
+        // prologue forms those of the sweep's input, each term those of the
+        // w it writes -- the next term's x, or the next substep's v = x.
+        const bool fused = fused_term_ok(op, adj, nv);
+        int slot = 0;
+        if (fused && nsub > 0) launch_fused_prologue(op, op->stream, cur, slot);
         double *const pool[4] = {va, vb, t0, t1};
         for (int64_t s = 0; s < nsub; ++s) {
             double *pp[2] = {nullptr, nullptr};
@@ -921,7 +1027,13 @@ const double *run_taylor_in(hymor_op *op, const SweepBufs &b, bool adj, int orde
             const double *w = cur;
             for (int j = order; j >= 1; --j) {
                 double *wn = pp[(order - j) & 1];
-                launch_term(op, adj, nv, op->stream, w, nullptr, cur, wn, dt / (double)j, 1.0);
+                if (fused) {
+                    launch_term_fused(op, op->stream, w, cur, wn, dt / (double)j, 1.0, slot);
+                    slot ^= 1;
+                } else {
+                    launch_term(op, adj, nv, op->stream, w, nullptr, cur, wn, dt / (double)j,
+                                1.0);
+                }
                 w = wn;
             }
             cur = w;
@@ -1159,6 +1271,7 @@ int hymor_op_create_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
         upload_extra(op, xf, op->xf);
         upload_extra(op, xa, op->xa);
         upload_replicated(op, xfc, rf, op->xf_rep);
+        upload_fused(op, xfc, rf, op->xf_rep);
         upload_replicated(op, xac, ra, op->xa_rep);
         op->has_adj = true;
         *out = op;
@@ -1229,6 +1342,7 @@ int hymor_op_create_bsr(int64_t nchi, int64_t neta, int vars_per_cell, int64_t n
                      value_prec, op->fwd);
         upload_extra(op, xf, op->xf);
         upload_replicated(op, xfc, rf, op->xf_rep);
+        upload_fused(op, xfc, rf, op->xf_rep);
         if (with_adjoint) {
             std::vector<int64_t> tptr, tsrc;
             std::vector<int32_t> tcol;
@@ -1737,9 +1851,15 @@ int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int form, int r
         HIP_TRY(hipEventCreate(&e1));
         // a middle term of the form: powers / normalised gather x, write the
         // term and update vout in place; Horner gathers w, reads v, writes
-        // the new w.  One warm launch, then the timed ones
+        // the new w (in one launch with the replicated rows fused in where a
+        // sweep does so: the sums of w formed once, into slot 0).  One warm
+        // launch, then the timed ones
+        const bool fused = horner && fused_term_ok(op, transpose != 0, nv);
+        if (fused) launch_fused_prologue(op, op->stream, op->ws[4], 0);
         auto term = [&] {
-            if (horner)
+            if (fused)
+                launch_term_fused(op, op->stream, op->ws[4], op->ws[5], op->ws[2], 0.5, 1.0, 0);
+            else if (horner)
                 launch_term(op, transpose != 0, nv, op->stream, op->ws[4], nullptr, op->ws[5],
                             op->ws[2], 0.5, 1.0);
             else
@@ -1763,7 +1883,8 @@ int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int form, int r
         // written)
         double bytes = (double)s.nnzb * (bb * sval + 4.0) + (double)(s.ncell + 1) * 4.0 +
                        (horner ? 3.0 : 4.0) * (double)op->n * nv * 8.0;
-        const DevExtra &xe = extra_of(op, transpose != 0, nv);
+        // (the fused term reads the compressed entries)
+        const DevExtra &xe = fused ? op->xf_rep : extra_of(op, transpose != 0, nv);
         const double vsz = xe.is_complex ? 16.0 : 8.0;
         bytes += (double)xe.nnz * (vsz + 4.0);
         // replicated blocks: their diagonal, and the shared rows (kind 2)

@@ -75,6 +75,24 @@ struct HostReplica {
 // the operator has no such structure (nrep >= 2 required).
 bool compress_column_replicas(HostExtra &x, int64_t n, HostReplica &rep);
 
+// The compressed flow-row entries of a kind 1 forward operator regrouped by
+// slice of 64 internal rows (a term kernel's workgroup), so that the term
+// kernel can add them in its epilogue from column sums formed by the
+// previous term: slice_tab[s] is the slice's block or -1; block k's lane l
+// (row 64 s + l) owns entries [lane_ptr[65 k + l], lane_ptr[65 k + l + 1])
+// of ecol (column q of the sums) and eval (1 or 2 doubles each).
+struct HostFusedRep {
+    std::vector<int32_t> slice_tab, lane_ptr, ecol;
+    std::vector<double> eval;
+};
+
+// Builds `out` from the compressed forward extras xc (compress_column_replicas)
+// when the replicated rows are exactly the extra DOFs (rep.c0 == nflow) and
+// every listed row is a flow row whose entries all reference column sums
+// (column >= n); false (out empty) otherwise.
+bool build_fused_tables(const HostExtra &xc, const HostReplica &rep, int64_t n, int64_t nflow,
+                        HostFusedRep &out);
+
 // Applies kind 2 with the forward operator's (c0, p, nrep) on its adjoint
 // extra CSR; false (x unchanged) if the rows do not repeat.
 bool compress_row_replicas(HostExtra &x, int64_t n, int64_t c0, int64_t p, int64_t nrep,

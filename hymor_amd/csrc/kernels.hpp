@@ -180,6 +180,28 @@ struct ReplicaDev {
     const double *vval = nullptr;
 };
 
+// A Horner term with the replicated rows of a kind 1 operator fused into the
+// term kernel's launch: workgroups [0, nfs_wg) finish the replicated rows
+// and form the column sums of the new rows into s_out (the sums the next
+// term needs); the others add each flow row's compressed entries times the
+// sums s_in of this term's x (formed by the previous term, or once before
+// the sweep) in their epilogue.  The flow rows' entries are grouped by
+// slice of 64 rows: slice_tab[s] is the slice's block or -1, block k's lane
+// l owns entries [lane_ptr[65 k + l], lane_ptr[65 k + l + 1]) of ecol (the
+// column q of the sums) and eval (1 or 2 doubles each).
+struct FusedRepDev {
+    int64_t nfs_wg = 0;
+    // slices are taken in the order rot, rot + 1, ..., wrapping: the first
+    // slice with entries leads, so the slices whose epilogue does more work
+    // are dispatched first instead of forming the launch's tail
+    int64_t rot = 0;
+    ReplicaDev R;
+    const double *s_in = nullptr;
+    double *s_out = nullptr;
+    const int32_t *slice_tab = nullptr, *lane_ptr = nullptr, *ecol = nullptr;
+    const double *eval = nullptr;
+};
+
 // x value (row c, double v) as the extra rows read it: with replicated
 // blocks kept once (REP) a virtual column reads their column sum
 template <bool REP>
@@ -245,16 +267,18 @@ struct RepLanes {
 // (copies split over KS lanes, then a fixed-order lane exchange), and each
 // row c0 + k p + q -- which holds only its diagonal -- finished as
 // extra_short_rows would: t = c1 * d * x, tout = t, vout = vin + c2 * t.
-template <int NV, bool CPLX>
-__global__ __launch_bounds__(256) void k_rep_sum(ReplicaDev R, const double *__restrict__ x,
-                                                 double *__restrict__ sum,
-                                                 double *__restrict__ tout, const double *vin,
-                                                 double *vout, double c1, double c2) {
+// With OUT_SUM the sums are of the new rows vout instead of x: the x of the
+// next Horner term (vout must then be set).  w: this wave's index.
+template <int NV, bool CPLX, bool OUT_SUM>
+__device__ __forceinline__ void rep_sum_wave(const ReplicaDev &R, int64_t w,
+                                             const double *__restrict__ x,
+                                             double *__restrict__ sum, double *__restrict__ tout,
+                                             const double *vin, double *vout, double c1,
+                                             double c2) {
     using L = RepLanes<NV, CPLX>;
     constexpr int NC = L::NC, LPR = L::LPR, KS = L::KS, VPW = L::VPW;
     constexpr int VS = CPLX ? 2 : 1;
     const int lane = threadIdx.x & 63;
-    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t q = w * VPW + lane / (LPR * KS);
     const bool live = q < R.p;
     const int ks = (lane / LPR) % KS;
@@ -285,8 +309,10 @@ __global__ __launch_bounds__(256) void k_rep_sum(ReplicaDev R, const double *__r
                 if (k >= R.nrep) break;
                 const int64_t row = R.c0 + k * R.p + q;
                 double acc[NC];
+                if constexpr (!OUT_SUM) {
 #pragma unroll
-                for (int i = 0; i < NC; ++i) s[i] += xv[u][i];
+                    for (int i = 0; i < NC; ++i) s[i] += xv[u][i];
+                }
                 if constexpr (CPLX) {
                     acc[0] = fma(dv[u][0], xv[u][0], fma(-dv[u][1], xv[u][1], 0.0));
                     acc[1] = fma(dv[u][0], xv[u][1], fma(dv[u][1], xv[u][0], 0.0));
@@ -298,7 +324,13 @@ __global__ __launch_bounds__(256) void k_rep_sum(ReplicaDev R, const double *__r
                     const int64_t o = row * NV + v0 + i;
                     const double t = c1 * acc[i];
                     if (tout) tout[o] = t;
-                    if (vout) vout[o] = fma(c2, t, vv[u][i]);
+                    if constexpr (OUT_SUM) {
+                        const double wn = fma(c2, t, vv[u][i]);
+                        vout[o] = wn;
+                        s[i] += wn;
+                    } else {
+                        if (vout) vout[o] = fma(c2, t, vv[u][i]);
+                    }
                 }
             }
         }
@@ -311,6 +343,15 @@ __global__ __launch_bounds__(256) void k_rep_sum(ReplicaDev R, const double *__r
 #pragma unroll
         for (int i = 0; i < NC; ++i) sum[q * NV + v0 + i] = s[i];
     }
+}
+
+template <int NV, bool CPLX>
+__global__ __launch_bounds__(256) void k_rep_sum(ReplicaDev R, const double *__restrict__ x,
+                                                 double *__restrict__ sum,
+                                                 double *__restrict__ tout, const double *vin,
+                                                 double *vout, double c1, double c2) {
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    rep_sum_wave<NV, CPLX, false>(R, w, x, sum, tout, vin, vout, c1, c2);
 }
 
 // Kind 2, in k_extra: shared row q evaluated once per lane group (the KS
@@ -1193,12 +1234,17 @@ __device__ __forceinline__ void wave_lds_sync() {
 // areas are padded by one chunk so the CPS cells of a read fall in
 // different banks.  Same slot order, FMA order and LDS combination as
 // k_term_split_p, hence the same bits.
-template <int B, typename VT, int NV, int NW, int S, bool NT>
+//
+// FR != 0: a Horner term of a kind 1 receptivity operator with its
+// replicated rows fused in (FusedRepDev; extra values real for FR = 1,
+// complex for FR = 2): no k_extra launch follows.
+template <int B, typename VT, int NV, int NW, int S, bool NT, int FR = 0>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_term_lds(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ val, int64_t nslice, int64_t ncell,
     const double *__restrict__ x,
-    double *__restrict__ tout, const double *vin, double *vout, double c1, double c2) {
+    double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
+    FusedRepDev F) {
     constexpr int CPS = 64 / B;
     constexpr int BB = B * B;
     constexpr int V = Vpl<VT>::value;
@@ -1208,11 +1254,28 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
     constexpr int XS = XC + 1;      // padded chunk stride of a cell's area
     static_assert(NV >= 2 && NV % 2 == 0, "x blocks move as pairs of doubles");
     typedef typename Chunk<VT>::type CT;
+    static_assert(FR != 2 || NV == 2, "complex extra values act on one complex vector");
     __shared__ double part[NW > 1 ? NW - 1 : 1][NV][64];
     __shared__ d2 xs[NW][S][CPS * XS];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    const int64_t slice = xcd_tile(blockIdx.x, gridDim.x);
+    int64_t bid = blockIdx.x, nwg = gridDim.x;
+    if constexpr (FR != 0) {
+        // the replicated rows first (nfs_wg is a multiple of 8, so the
+        // slices below keep the XCD mapping of an unfused launch)
+        if (bid < F.nfs_wg) {
+            rep_sum_wave<NV, FR == 2, true>(F.R, bid * NW + w, x, F.s_out, tout, vin, vout, c1,
+                                            c2);
+            return;
+        }
+        bid -= F.nfs_wg;
+        nwg -= F.nfs_wg;
+    }
+    int64_t slice = xcd_tile(bid, nwg);
+    if constexpr (FR != 0) {
+        slice += F.rot;   // rot < nslice == nwg
+        if (slice >= nwg) slice -= nwg;
+    }
     if (slice >= nslice) return;
     const int q = lane / B, r = lane % B;
     const int64_t p0 = slice_ptr[slice];
@@ -1226,6 +1289,16 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
         const double *pv = vout ? vin : x;
 #pragma unroll
         for (int v = 0; v < NV; ++v) pre_v[v] = w == 0 ? pv[row * NV + v] : 0.0;
+    }
+    // this lane's range of fused extra entries, fetched with the epilogue
+    // operands so that the epilogue starts with them at hand
+    int32_t fe0 = 0, fe1 = 0;
+    if constexpr (FR != 0) {
+        const int32_t blk = F.slice_tab[slice];
+        if (blk >= 0 && w == 0) {
+            fe0 = F.lane_ptr[(int64_t)blk * 65 + lane];
+            fe1 = F.lane_ptr[(int64_t)blk * 65 + lane + 1];
+        }
     }
     double acc[NV];
 #pragma unroll
@@ -1303,6 +1376,39 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
             for (int v = 0; v < NV; ++v) acc[v] += part[i][v][lane];
     }
     if (cell >= ncell) return;
+    if constexpr (FR != 0) {
+        // this row's compressed extra entries times the column sums of x, in
+        // groups of 8: the columns and values of a group, then its sums, are
+        // in flight together; products added in entry order
+        constexpr int G = 8, VS = FR == 2 ? 2 : 1;
+        for (int32_t e0 = fe0; e0 < fe1; e0 += G) {
+            int32_t qc[G];
+            double a[G][VS];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const int32_t e = e0 + u < fe1 ? e0 + u : fe1 - 1;
+                qc[u] = F.ecol[e];
+#pragma unroll
+                for (int i = 0; i < VS; ++i) a[u][i] = F.eval[(int64_t)e * VS + i];
+            }
+            double sv[G][NV];
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sv[u][v] = F.s_in[(int64_t)qc[u] * NV + v];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                if (e0 + u >= fe1) break;
+                if constexpr (FR == 2) {
+                    acc[0] = fma(a[u][0], sv[u][0], fma(-a[u][1], sv[u][1], acc[0]));
+                    acc[1] = fma(a[u][0], sv[u][1], fma(a[u][1], sv[u][0], acc[1]));
+                } else {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) acc[v] = fma(a[u][0], sv[u][v], acc[v]);
+                }
+            }
+        }
+    }
     // the vout stores first, behind one wait for the epilogue operands (a
     // store between two such waits would be waited for as well)
     if (vout) {

@@ -251,6 +251,178 @@ extern "C" int emu_spmv_csc_rep(int64_t n, const int64_t *colptr, const int64_t 
     return g_oob;
 }
 
+// A Horner sweep T_order(A dt)^nsub x with the replicated rows fused into
+// the term kernel, at two doubles per row (one complex vector, or two real
+// ones, x_vm[vm * 2 + v]): the prologue's column sums (k_rep_sum with no
+// outputs), then per term k_term_lds<FR != 0>'s index math -- the
+// replicated rows by rep_sum_wave's lane mapping summing the new rows, the
+// flow rows' entries from the slice tables in the epilogue -- with the two
+// sum buffers alternating and the work buffers picked as run_taylor_in
+// picks them.  info[0] = 1 when the tables were built (nothing is computed
+// otherwise), info[1] their blocks, info[2] their entries.
+extern "C" int emu_horner_fused(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                                const double *nzval, int base, int is_complex, int b,
+                                int64_t ncell, int order, int nsub, double dt, const double *x_vm,
+                                double *y_vm, int64_t *info) {
+    g_oob = 0;
+    info[0] = info[1] = info[2] = 0;
+    HostBSR fwd, adj;
+    HostExtra xf, xa;
+    csc_to_blocks(n, colptr, rowval, nzval, base, is_complex != 0, 1.0, b, ncell, fwd, adj, xf, xa);
+    HostReplica rep;
+    const int64_t nflow = (int64_t)b * ncell;
+    if (!compress_column_replicas(xf, n, rep)) return 0;
+    HostFusedRep F;
+    if (!build_fused_tables(xf, rep, n, nflow, F)) return 0;
+    info[0] = 1;
+    info[1] = (int64_t)F.lane_ptr.size() / 65;
+    info[2] = (int64_t)F.ecol.size();
+    HostSliced s;
+    bsr_to_sliced(fwd, s);
+    interleave_lanes(s, 2);
+    constexpr int NV = 2, KS = 8, KU = 4;
+    const bool cplx = is_complex != 0;
+    const int vs = cplx ? 2 : 1, NC = cplx ? 2 : 1, LPR = cplx ? 1 : 2;
+    const int VPW = 64 / (LPR * KS);
+    std::vector<double> in((size_t)n * NV), pool[4], S[2];
+    for (auto &p : pool) p.assign((size_t)n * NV, 0.0);
+    for (auto &v : S) v.assign((size_t)rep.p * NV, 0.0);
+    for (int64_t row = 0; row < n; ++row) {
+        const int64_t vm = row < nflow ? (row % b) * ncell + row / b : row;
+        for (int v = 0; v < NV; ++v) in[(size_t)(row * NV + v)] = x_vm[vm * NV + v];
+    }
+    auto put = [&](std::vector<double> &buf, int64_t i, double val) {
+        if (i < 0 || i >= (int64_t)buf.size()) ++g_oob;
+        else buf[(size_t)i] = val;
+    };
+    // rep_sum_wave over all waves; vout null: sums of x only (the prologue)
+    auto rep_waves = [&](bool out_sum, const std::vector<double> &x, const std::vector<double> *vin,
+                         std::vector<double> *vout, double c1, double c2, std::vector<double> &sum) {
+        const int64_t nw = (rep.p + VPW - 1) / VPW;
+        for (int64_t w = 0; w < nw; ++w) {
+            double sl[64][2] = {};
+            for (int lane = 0; lane < 64; ++lane) {
+                const int64_t q = w * VPW + lane / (LPR * KS);
+                const int ks = (lane / LPR) % KS, v0 = (lane % LPR) * NC;
+                if (q >= rep.p) continue;
+                for (int64_t k0 = ks; k0 < rep.nrep; k0 += KS * KU) {
+                    double xv[KU][2], dv[KU][2], vv[KU][2];
+                    for (int u = 0; u < KU; ++u) {
+                        const int64_t k = k0 + (int64_t)u * KS < rep.nrep ? k0 + (int64_t)u * KS : k0;
+                        const int64_t row = rep.c0 + k * rep.p + q;
+                        for (int i = 0; i < NC; ++i) {
+                            xv[u][i] = at(x, row * NV + v0 + i);
+                            vv[u][i] = vout ? at(*vin, row * NV + v0 + i) : 0.0;
+                        }
+                        for (int i = 0; i < vs; ++i) dv[u][i] = at(rep.diag, (k * rep.p + q) * vs + i);
+                    }
+                    for (int u = 0; u < KU; ++u) {
+                        const int64_t k = k0 + (int64_t)u * KS;
+                        if (k >= rep.nrep) break;
+                        const int64_t row = rep.c0 + k * rep.p + q;
+                        double acc[2];
+                        if (!out_sum)
+                            for (int i = 0; i < NC; ++i) sl[lane][i] += xv[u][i];
+                        if (cplx) {
+                            acc[0] = dv[u][0] * xv[u][0] - dv[u][1] * xv[u][1];
+                            acc[1] = dv[u][0] * xv[u][1] + dv[u][1] * xv[u][0];
+                        } else {
+                            acc[0] = dv[u][0] * xv[u][0];
+                        }
+                        for (int i = 0; i < NC; ++i) {
+                            const double wn = c2 * (c1 * acc[i]) + vv[u][i];
+                            if (vout) put(*vout, row * NV + v0 + i, wn);
+                            if (out_sum) sl[lane][i] += wn;
+                        }
+                    }
+                }
+            }
+            for (int off = LPR; off < LPR * KS; off <<= 1) {
+                double t[64][2];
+                for (int l = 0; l < 64; ++l)
+                    for (int i = 0; i < 2; ++i) t[l][i] = sl[l][i] + sl[l ^ off][i];
+                for (int l = 0; l < 64; ++l)
+                    for (int i = 0; i < 2; ++i) sl[l][i] = t[l][i];
+            }
+            for (int lane = 0; lane < 64; ++lane) {
+                const int64_t q = w * VPW + lane / (LPR * KS);
+                const int ks = (lane / LPR) % KS, v0 = (lane % LPR) * NC;
+                if (q < rep.p && ks == 0)
+                    for (int i = 0; i < NC; ++i) put(sum, q * NV + v0 + i, sl[lane][i]);
+            }
+        }
+    };
+    const int CPS = 64 / b, BB = b * b;
+    int slot = 0;
+    const std::vector<double> *cur = &in;
+    if (nsub > 0) rep_waves(false, *cur, nullptr, nullptr, 0.0, 0.0, S[slot]);
+    for (int sub = 0; sub < nsub; ++sub) {
+        std::vector<double> *pp[2] = {nullptr, nullptr};
+        int np = 0;
+        for (auto &p : pool)
+            if (&p != cur && np < 2) pp[np++] = &p;
+        const std::vector<double> *w = cur;
+        for (int j = order; j >= 1; --j) {
+            std::vector<double> &wn = *pp[(order - j) & 1];
+            const double c1 = dt / (double)j, c2 = 1.0;
+            const std::vector<double> &Sin = S[slot];
+            rep_waves(true, *w, cur, &wn, c1, c2, S[slot ^ 1]);
+            // slices from the first one with entries on, wrapping
+            int64_t rot = 0;
+            while (rot < s.nslice && at(F.slice_tab, rot) < 0) ++rot;
+            if (rot == s.nslice) rot = 0;
+            for (int64_t t = 0; t < s.nslice; ++t) {
+                int64_t slice = t + rot;
+                if (slice >= s.nslice) slice -= s.nslice;
+                const int64_t p0 = at(s.slice_ptr, slice);
+                const int64_t width = (at(s.slice_ptr, slice + 1) - p0) / CPS;
+                const int32_t blk = at(F.slice_tab, slice);
+                for (int lane = 0; lane < 64; ++lane) {
+                    const int q = lane / b, r = lane % b;
+                    double acc[2] = {0, 0};
+                    for (int64_t t = 0; t < width; ++t) {
+                        const int64_t c = at(s.col, p0 + q + t * CPS);
+                        if (c < 0 || c >= ncell) ++g_oob;
+                        const int64_t sb = (p0 + t * CPS) * BB;
+                        for (int k = 0; k < b; ++k) {
+                            const double a = at(s.val, sb + ((k / 2) * 64 + lane) * 2 + k % 2);
+                            for (int v = 0; v < NV; ++v) acc[v] += a * at(*w, (c * b + k) * NV + v);
+                        }
+                    }
+                    const int64_t cell = slice * CPS + q;
+                    if (cell >= ncell) continue;
+                    if (blk >= 0) {
+                        const int64_t e1 = at(F.lane_ptr, (int64_t)blk * 65 + lane + 1);
+                        for (int64_t e = at(F.lane_ptr, (int64_t)blk * 65 + lane); e < e1; ++e) {
+                            const int64_t qc = at(F.ecol, e);
+                            if (cplx) {
+                                const double a0 = at(F.eval, 2 * e), a1 = at(F.eval, 2 * e + 1);
+                                const double s0 = at(Sin, qc * 2), s1 = at(Sin, qc * 2 + 1);
+                                acc[0] += a0 * s0 - a1 * s1;
+                                acc[1] += a0 * s1 + a1 * s0;
+                            } else {
+                                const double a = at(F.eval, e);
+                                for (int v = 0; v < NV; ++v) acc[v] += a * at(Sin, qc * NV + v);
+                            }
+                        }
+                    }
+                    const int64_t row = cell * b + r;
+                    for (int v = 0; v < NV; ++v)
+                        put(wn, row * NV + v, c2 * (c1 * acc[v]) + at(*cur, row * NV + v));
+                }
+            }
+            slot ^= 1;
+            w = &wn;
+        }
+        cur = w;
+    }
+    for (int64_t vm = 0; vm < n; ++vm) {
+        const int64_t row = vm < nflow ? (vm % ncell) * b + vm / ncell : vm;
+        for (int v = 0; v < NV; ++v) y_vm[vm * NV + v] = at(*cur, row * NV + v);
+    }
+    return g_oob;
+}
+
 // Batched form: k_perm_in (padded width) -> k_spmm's index math for R
 // doubles per row -> k_perm_out.  Real operators without extras.
 extern "C" int emu_spmm_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,

@@ -188,3 +188,64 @@ def test_assembled_wide_batches_replicated_blocks(orc, N, shock, nrhs):
                 assert relerr(Y[:, j], ref) <= 1e-12, (tr, j)
                 single = dev.expm_taylor(X[:, j], order=5, nsub=nsub, dt=1.0, transpose=tr)
                 assert relerr(Y[:, j], single) <= 1e-13, (tr, j)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,nsub,prec", [(2, 3, "f64"), (9, 4, "f64"), (32, 2, "f64"),
+                                         (9, 3, "f32")])
+def test_assembled_single_vector_fused_horner(orc, N, nsub, prec):
+    """One complex vector in Horner form on the assembled extended operator
+    without shock DOFs: the library fuses the replicated freestream rows into
+    the term kernel (one launch per term; the column sums of each new w
+    carried to the next term and substep).  Equal to the oracle's
+    POWER_METHOD! (1e-12 with FP64 values, 2e-6 with FP32), identical on
+    three replays of the captured sweep, and equal to the two-launch term
+    that a two-wave tuning runs (1e-13); the fused term reports the bytes of
+    the compressed entries it reads."""
+    from hymor_amd.core import DeviceOperator
+    nchi, neta = 11, 7
+    say(f"[assembled receptivity, fused Horner] N={N} nsub={nsub} values={prec}")
+    base = ops.stencil_operator(nchi, neta, 4, seed=45, row_sum=2.0)
+    L_ = ops.receptivity_extension(base, np.linspace(0.1, 6.0, N))
+    Ao = orc.CSC.from_scipy(L_.to_scipy_varmajor())
+    r = np.random.default_rng(70 + N)
+    x = r.standard_normal(L_.n) + 1j * r.standard_normal(L_.n)
+    dt, order = 0.05, 6
+    ref = orc.power_method(Ao.scaled(dt), x, nsub, order)
+    tol = 1e-12 if prec == "f64" else 2e-6
+    with DeviceOperator.from_blocks(L_, scale=dt, value_prec=prec) as dev:
+        assert dev.tuning == (3, 1, 1)
+        fused = dev.expm_taylor(x, order=order, nsub=nsub, dt=1.0, form="horner")
+        assert relerr(fused, ref) <= tol
+        _, by_fused = dev.bench_term_kernel(reps=2, nv=2, form="horner")
+        dev.set_graphs(True)
+        for _ in range(3):
+            y = dev.expm_taylor(x, order=order, nsub=nsub, dt=1.0, form="horner")
+            assert np.array_equal(y, fused)
+        dev.set_graphs(False)
+        dev.set_tuning(3, 2, 1)
+        two = dev.expm_taylor(x, order=order, nsub=nsub, dt=1.0, form="horner")
+        _, by_two = dev.bench_term_kernel(reps=2, nv=2, form="horner")
+        assert relerr(two, ref) <= tol
+        assert relerr(fused, two) <= 1e-13
+        # the copies of B are read once per term instead of N times
+        assert by_fused < by_two
+
+
+@pytest.mark.gpu
+def test_fused_horner_real_values(orc):
+    """Real extra values (the freestream diagonal -w made real) with two real
+    vectors per sweep: the fused term's real-value form against the
+    oracle, column by column."""
+    from hymor_amd.core import DeviceOperator
+    nchi, neta = 9, 6
+    say("[fused Horner, real values, two vectors]")
+    base = ops.stencil_operator(nchi, neta, 4, seed=46, row_sum=2.0)
+    A = ops.receptivity_extension(base, [0.4, 1.1, 2.0, 3.3, 4.1]).to_scipy_varmajor()
+    A = (A.real + A.imag).tocsc()
+    Ao = orc.CSC.from_scipy(A).scaled(0.05)
+    X = np.random.default_rng(71).standard_normal((A.shape[0], 2))
+    with DeviceOperator.from_scipy(A, nchi=nchi, neta=neta, vars_per_cell=4, scale=0.05) as dev:
+        Y = dev.expm_taylor_batched(X, order=5, nsub=3, dt=1.0, form="horner")
+    for v in range(2):
+        assert relerr(Y[:, v], orc.power_method(Ao, X[:, v], 3, 5)) <= 1e-12, v

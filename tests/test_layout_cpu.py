@@ -138,6 +138,79 @@ def test_emulated_single_wavenumber_is_not_compressed(emu):
     assert np.allclose(y, L_.to_scipy_varmajor() @ x, rtol=1e-13, atol=1e-13)
 
 
+def _run_fused(emu, A, b, ncell, X2, order, nsub, dt):
+    """emu_horner_fused on scipy matrix A (var-major) with two doubles per
+    row: X2 is n complex values or an n x 2 real array."""
+    A = A.tocsc()
+    A.sort_indices()
+    cplx = np.iscomplexobj(A.data)
+    colptr = (A.indptr.astype(np.int64) + 1)
+    rowval = (A.indices.astype(np.int64) + 1)
+    nzval = np.ascontiguousarray(A.data, dtype=np.complex128 if cplx else np.float64)
+    x = np.ascontiguousarray(X2)
+    y = np.zeros_like(x)
+    info = np.zeros(3, dtype=np.int64)
+    emu.emu_horner_fused.restype = C.c_int
+    oob = emu.emu_horner_fused(
+        C.c_int64(A.shape[0]), C.c_void_p(colptr.ctypes.data), C.c_void_p(rowval.ctypes.data),
+        C.c_void_p(nzval.ctypes.data), C.c_int(1), C.c_int(int(cplx)), C.c_int(b),
+        C.c_int64(ncell), C.c_int(order), C.c_int(nsub), C.c_double(dt),
+        C.c_void_p(x.ctypes.data), C.c_void_p(y.ctypes.data), C.c_void_p(info.ctypes.data))
+    return y, oob, info
+
+
+@pytest.mark.parametrize("nchi,neta,w", [(6, 4, [0.0, 1.0, 2.5]), (5, 3, [0.3, 0.7]),
+                                         (9, 7, list(np.linspace(0.1, 4.0, 9)))])
+def test_emulated_fused_horner_sweep(emu, orc, nchi, neta, w):
+    """The Horner sweep with the replicated rows fused into the term kernel
+    (slice tables in the epilogue, column sums of each new w for the next
+    term, two sum buffers alternating across terms and substeps) stays in
+    bounds and gives the oracle's POWER_METHOD! on the extended operator."""
+    base = ops.stencil_operator(nchi, neta, 4, seed=9)
+    L_ = ops.receptivity_extension(base, w)
+    A = L_.to_scipy_varmajor()
+    r = np.random.default_rng(6)
+    x = r.standard_normal(L_.n) + 1j * r.standard_normal(L_.n)
+    order, nsub, dt = 5, 3, 0.05
+    y, oob, info = _run_fused(emu, A, 4, base.ncell, x, order, nsub, dt)
+    assert oob == 0
+    # one table entry per entry of B (4 nchi freestream columns)
+    nnz_B = A[:base.n, base.n:base.n + 4 * nchi].nnz
+    assert info[0] == 1 and info[2] == nnz_B
+    ref = orc.power_method(orc.CSC.from_scipy(A).scaled(dt), x, nsub, order)
+    assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_emulated_fused_horner_real_values(emu, orc):
+    """Real extra values (two real vectors per row): the same sweep with the
+    freestream diagonal -w made real."""
+    base = ops.stencil_operator(7, 5, 4, seed=11)
+    L_ = ops.receptivity_extension(base, [0.4, 1.1, 2.0, 3.3])
+    A = L_.to_scipy_varmajor()
+    A = (A.real + A.imag).tocsc()
+    X = np.random.default_rng(7).standard_normal((L_.n, 2))
+    y, oob, info = _run_fused(emu, A, 4, base.ncell, X, 4, 2, 0.1)
+    assert oob == 0 and info[0] == 1
+    Ao = orc.CSC.from_scipy(A).scaled(0.1)
+    for v in range(2):
+        ref = orc.power_method(Ao, X[:, v], 2, 4)
+        assert np.linalg.norm(y[:, v] - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_emulated_fused_declines_other_extras(emu):
+    """Shock DOFs between the flow and freestream rows, or no replicated
+    blocks at all: no tables (the library keeps its two-launch term)."""
+    base = ops.stencil_operator(6, 4, 4, seed=9, n_shock=6)
+    L_ = ops.receptivity_extension(base, [0.2, 0.9, 1.7])
+    xs = np.zeros(L_.n, dtype=np.complex128)
+    _, oob, info = _run_fused(emu, L_.to_scipy_varmajor(), 4, base.ncell, xs, 2, 1, 0.1)
+    assert oob == 0 and info[0] == 0
+    op = ops.stencil_operator(6, 4, 4, seed=9)
+    x = np.zeros((op.n, 2))
+    _, oob, info = _run_fused(emu, op.to_scipy_varmajor(), 4, op.ncell, x, 2, 1, 0.1)
+    assert oob == 0 and info[0] == 0
+
+
 @pytest.mark.parametrize("b,R,nrhs", [(4, 16, 16), (4, 16, 11), (4, 32, 32), (4, 64, 40),
                                       (8, 8, 8), (8, 16, 13), (8, 32, 32), (8, 64, 64)])
 def test_emulated_batched_spmm(emu, b, R, nrhs):
