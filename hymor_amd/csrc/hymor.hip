@@ -95,7 +95,7 @@ struct DevExtra {
     int64_t rep_nnz = 0;           // kind 2: entries of the shared rows
     // kind 1 forward with every extra entry on a flow row (HostFusedRep):
     // the tables of the fused Horner term and its two column-sum buffers
-    // (p * 2 doubles each), used at two doubles per row
+    // (p * 64 doubles each: up to 64 doubles per row)
     bool fused = false;
     int64_t fr_rot = 0;   // first slice with entries
     int32_t *fr_slice = nullptr, *fr_lane = nullptr, *fr_col = nullptr;
@@ -382,7 +382,7 @@ void upload_fused(hymor_op *op, const HostExtra &h, const HostReplica &rep, DevE
     d.fr_lane = upload(op, f.lane_ptr);
     d.fr_col = upload(op, f.ecol);
     d.fr_val = upload(op, f.eval);
-    for (double *&s : d.fr_sum) s = (double *)dalloc(op, (size_t)rep.p * 2 * sizeof(double));
+    for (double *&s : d.fr_sum) s = (double *)dalloc(op, (size_t)rep.p * 64 * sizeof(double));
     d.fused = true;
 }
 
@@ -688,7 +688,7 @@ void launch_quad4c_t(const DevSliced &s, hipStream_t st, const double *x,
     const int64_t grid = (ngrp + wpb - 1) / wpb;
     hipLaunchKernelGGL((k_spmm_quad4c<VT, R, W>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
                        s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, tout, vin, vout,
-                       c1, c2);
+                       c1, c2, FusedRepDev{});
 }
 
 template <int B, typename VT>
@@ -895,53 +895,109 @@ void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, const double *x
     }
 }
 
-// The fused Horner term (k_term_lds with FR != 0) applies: forward, two
-// doubles per row, tables built for the operator (build_fused_tables), no
-// per-column diagonal, and the term kernel the default tuning runs
-// (k_term_lds, one wave per slice, non-temporal value loads).
-bool fused_term_ok(const hymor_op *op, bool adj, int nv) {
+// Which fused Horner term applies (0: none -- the block kernel and the
+// extra rows run as separate launches): forward, tables built for the
+// operator (build_fused_tables), no per-column diagonal, and
+//   1: two doubles per row under the default tuning (k_term_lds, one wave
+//      per slice, non-temporal value loads);
+//   2: 8-64 doubles per row at b = 4 under the default batch kernel
+//      (k_spmm_quad4c).
+int fused_kind(const hymor_op *op, bool adj, int nv) {
+    if (adj || !op->xf_rep.fused || op->ncell == 0 || op->cdiag_on) return 0;
     const TermTuning &t = op->fwd.tune;
-    return !adj && nv == 2 && op->xf_rep.fused && op->ncell > 0 && !op->cdiag_on &&
-           t.split == 3 && t.waves == 1 && t.nt == 1;
+    if (nv == 2) return (t.split == 3 && t.waves == 1 && t.nt == 1) ? 1 : 0;
+    if (op->b == 4 && nv >= 8 && nv <= 64 && (t.spmm_mfma == 6 || t.spmm_mfma == 8)) return 2;
+    return 0;
 }
 
-int64_t fused_rep_waves(const DevExtra &xe) {
-    const int64_t vpw = xe.is_complex ? RepLanes<2, true>::VPW : RepLanes<2, false>::VPW;
+// Waves of rep_sum_wave over the replicated rows at NV doubles per row.
+template <int NV>
+int64_t fused_rep_waves_t(const DevExtra &xe) {
+    const int64_t vpw = xe.is_complex ? RepLanes<NV, true>::VPW : RepLanes<NV, false>::VPW;
     return (xe.rep.p + vpw - 1) / vpw;
 }
 
-// Column sums of x's replicated rows into fr_sum[slot], before the first
-// fused term of a sweep.
-void launch_fused_prologue(hymor_op *op, hipStream_t st, const double *x, int slot) {
-    const DevExtra &xe = op->xf_rep;
-    const unsigned g = (unsigned)((fused_rep_waves(xe) + 3) / 4);
+int64_t fused_rep_waves(const DevExtra &xe, int nv) {
+    switch (nv) {
+    case 2: return fused_rep_waves_t<2>(xe);
+    case 8: return fused_rep_waves_t<8>(xe);
+    case 16: return fused_rep_waves_t<16>(xe);
+    case 32: return fused_rep_waves_t<32>(xe);
+    case 64: return fused_rep_waves_t<64>(xe);
+    default: throw ArgError("fused term: unsupported vector width (internal)");
+    }
+}
+
+template <int NV>
+void launch_fused_prologue_t(const DevExtra &xe, hipStream_t st, const double *x, int slot) {
+    const unsigned g = (unsigned)((fused_rep_waves_t<NV>(xe) + 3) / 4);
     if (xe.is_complex)
-        hipLaunchKernelGGL((k_rep_sum<2, true>), dim3(g), dim3(256), 0, st, xe.rep, x,
+        hipLaunchKernelGGL((k_rep_sum<NV, true>), dim3(g), dim3(256), 0, st, xe.rep, x,
                            xe.fr_sum[slot], nullptr, nullptr, nullptr, 0.0, 0.0);
     else
-        hipLaunchKernelGGL((k_rep_sum<2, false>), dim3(g), dim3(256), 0, st, xe.rep, x,
+        hipLaunchKernelGGL((k_rep_sum<NV, false>), dim3(g), dim3(256), 0, st, xe.rep, x,
                            xe.fr_sum[slot], nullptr, nullptr, nullptr, 0.0, 0.0);
+}
+
+// Column sums of x's replicated rows (nv doubles per row) into
+// fr_sum[slot], before the first fused term of a sweep.
+void launch_fused_prologue(hymor_op *op, hipStream_t st, const double *x, int slot, int nv) {
+    const DevExtra &xe = op->xf_rep;
+    switch (nv) {
+    case 2: launch_fused_prologue_t<2>(xe, st, x, slot); break;
+    case 8: launch_fused_prologue_t<8>(xe, st, x, slot); break;
+    case 16: launch_fused_prologue_t<16>(xe, st, x, slot); break;
+    case 32: launch_fused_prologue_t<32>(xe, st, x, slot); break;
+    case 64: launch_fused_prologue_t<64>(xe, st, x, slot); break;
+    default: throw ArgError("fused term: unsupported vector width (internal)");
+    }
     HIP_TRY(hipGetLastError());
 }
 
 template <int B, typename VT, int FR>
-void launch_fused_t(const DevSliced &s, const FusedRepDev &F, hipStream_t st, const double *x,
-                    const double *vin, double *vout, double c1, double c2) {
+void launch_fused_lds_t(const DevSliced &s, const FusedRepDev &F, hipStream_t st,
+                        const double *x, const double *vin, double *vout, double c1, double c2) {
     hipLaunchKernelGGL((k_term_lds<B, VT, 2, 1, 9, true, FR>),
                        dim3((unsigned)(F.nfs_wg + s.nslice)), dim3(64), 0, st, s.slice_ptr,
                        s.col, (const VT *)s.val, s.nslice, s.ncell, x, nullptr, vin, vout, c1,
                        c2, F);
 }
 
-// One Horner term w_new = v + c1 L_ w (c2 = 1) in one launch (fused_term_ok):
-// the flow rows take the column sums of w from fr_sum[slot], the replicated
-// rows leave those of w_new in fr_sum[slot ^ 1].
+template <typename VT, int R, int W, int FR>
+void launch_fused_quad4c_t(const DevSliced &s, const FusedRepDev &F, hipStream_t st,
+                           const double *x, const double *vin, double *vout, double c1,
+                           double c2) {
+    const int wpb = 4;
+    const int64_t ngrp = (s.ncell + 3) / 4;
+    const int64_t grid = (ngrp + wpb - 1) / wpb;
+    hipLaunchKernelGGL((k_spmm_quad4c<VT, R, W, FR>), dim3((unsigned)(F.nfs_wg + grid)),
+                       dim3(64 * wpb), 0, st, s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x,
+                       nullptr, vin, vout, c1, c2, F);
+}
+
+// the slot groups launch_block_part uses at each width
+template <typename VT, int FR>
+void launch_fused_quad4c(const DevSliced &s, const FusedRepDev &F, int nv, hipStream_t st,
+                         const double *x, const double *vin, double *vout, double c1,
+                         double c2) {
+    switch (nv) {
+    case 8: launch_fused_quad4c_t<VT, 8, 9, FR>(s, F, st, x, vin, vout, c1, c2); break;
+    case 16: launch_fused_quad4c_t<VT, 16, 9, FR>(s, F, st, x, vin, vout, c1, c2); break;
+    case 32: launch_fused_quad4c_t<VT, 32, 3, FR>(s, F, st, x, vin, vout, c1, c2); break;
+    case 64: launch_fused_quad4c_t<VT, 64, 3, FR>(s, F, st, x, vin, vout, c1, c2); break;
+    default: throw ArgError("fused term: unsupported vector width (internal)");
+    }
+}
+
+// One Horner term w_new = v + c1 L_ w (c2 = 1) in one launch (fused_kind
+// != 0) at nv doubles per row: the flow rows take the column sums of w from
+// fr_sum[slot], the replicated rows leave those of w_new in fr_sum[slot ^ 1].
 void launch_term_fused(hymor_op *op, hipStream_t st, const double *x, const double *vin,
-                       double *vout, double c1, double c2, int slot) {
+                       double *vout, double c1, double c2, int slot, int nv) {
     const DevSliced &s = op->fwd;
     const DevExtra &xe = op->xf_rep;
+    const int kind = fused_kind(op, false, nv);
     FusedRepDev F;
-    F.nfs_wg = (fused_rep_waves(xe) + 7) / 8 * 8;   // one wave per workgroup
     F.rot = xe.fr_rot;
     F.R = xe.rep;
     F.s_in = xe.fr_sum[slot];
@@ -951,19 +1007,34 @@ void launch_term_fused(hymor_op *op, hipStream_t st, const double *x, const doub
     F.ecol = xe.fr_col;
     F.eval = xe.fr_val;
     const bool f32 = s.prec == HYMOR_PREC_F32;
-#define HYMOR_FUSED(BV, VT)                                                          \
-    (xe.is_complex ? launch_fused_t<BV, VT, 2>(s, F, st, x, vin, vout, c1, c2)       \
-                   : launch_fused_t<BV, VT, 1>(s, F, st, x, vin, vout, c1, c2))
-    if (s.b == 4) {
-        if (f32) HYMOR_FUSED(4, float);
-        else HYMOR_FUSED(4, double);
-    } else if (s.b == 8) {
-        if (f32) HYMOR_FUSED(8, float);
-        else HYMOR_FUSED(8, double);
-    } else {
-        throw ArgError("vars_per_cell must be 4 or 8");
-    }
+    if (kind == 1) {
+        F.nfs_wg = (fused_rep_waves(xe, 2) + 7) / 8 * 8;   // one wave per workgroup
+#define HYMOR_FUSED(BV, VT)                                                              \
+    (xe.is_complex ? launch_fused_lds_t<BV, VT, 2>(s, F, st, x, vin, vout, c1, c2)       \
+                   : launch_fused_lds_t<BV, VT, 1>(s, F, st, x, vin, vout, c1, c2))
+        if (s.b == 4) {
+            if (f32) HYMOR_FUSED(4, float);
+            else HYMOR_FUSED(4, double);
+        } else if (s.b == 8) {
+            if (f32) HYMOR_FUSED(8, float);
+            else HYMOR_FUSED(8, double);
+        } else {
+            throw ArgError("vars_per_cell must be 4 or 8");
+        }
 #undef HYMOR_FUSED
+    } else if (kind == 2) {
+        // four waves per workgroup
+        F.nfs_wg = ((fused_rep_waves(xe, nv) + 3) / 4 + 7) / 8 * 8;
+        if (f32) {
+            if (xe.is_complex) launch_fused_quad4c<float, 2>(s, F, nv, st, x, vin, vout, c1, c2);
+            else launch_fused_quad4c<float, 1>(s, F, nv, st, x, vin, vout, c1, c2);
+        } else {
+            if (xe.is_complex) launch_fused_quad4c<double, 2>(s, F, nv, st, x, vin, vout, c1, c2);
+            else launch_fused_quad4c<double, 1>(s, F, nv, st, x, vin, vout, c1, c2);
+        }
+    } else {
+        throw ArgError("fused term does not apply (internal)");
+    }
     HIP_TRY(hipGetLastError());
 }
 
@@ -1015,9 +1086,9 @@ const double *run_taylor_in(hymor_op *op, const SweepBufs &b, bool adj, int orde
         // column sums of each term's x alternate between two buffers: the
         // prologue forms those of the sweep's input, each term those of the
         // w it writes -- the next term's x, or the next substep's v = x.
-        const bool fused = fused_term_ok(op, adj, nv);
+        const bool fused = fused_kind(op, adj, nv) != 0;
         int slot = 0;
-        if (fused && nsub > 0) launch_fused_prologue(op, op->stream, cur, slot);
+        if (fused && nsub > 0) launch_fused_prologue(op, op->stream, cur, slot, nv);
         double *const pool[4] = {va, vb, t0, t1};
         for (int64_t s = 0; s < nsub; ++s) {
             double *pp[2] = {nullptr, nullptr};
@@ -1028,7 +1099,8 @@ const double *run_taylor_in(hymor_op *op, const SweepBufs &b, bool adj, int orde
             for (int j = order; j >= 1; --j) {
                 double *wn = pp[(order - j) & 1];
                 if (fused) {
-                    launch_term_fused(op, op->stream, w, cur, wn, dt / (double)j, 1.0, slot);
+                    launch_term_fused(op, op->stream, w, cur, wn, dt / (double)j, 1.0, slot,
+                                      nv);
                     slot ^= 1;
                 } else {
                     launch_term(op, adj, nv, op->stream, w, nullptr, cur, wn, dt / (double)j,
@@ -1854,11 +1926,12 @@ int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int form, int r
         // the new w (in one launch with the replicated rows fused in where a
         // sweep does so: the sums of w formed once, into slot 0).  One warm
         // launch, then the timed ones
-        const bool fused = horner && fused_term_ok(op, transpose != 0, nv);
-        if (fused) launch_fused_prologue(op, op->stream, op->ws[4], 0);
+        const bool fused = horner && fused_kind(op, transpose != 0, nv) != 0;
+        if (fused) launch_fused_prologue(op, op->stream, op->ws[4], 0, nv);
         auto term = [&] {
             if (fused)
-                launch_term_fused(op, op->stream, op->ws[4], op->ws[5], op->ws[2], 0.5, 1.0, 0);
+                launch_term_fused(op, op->stream, op->ws[4], op->ws[5], op->ws[2], 0.5, 1.0, 0,
+                                  nv);
             else if (horner)
                 launch_term(op, transpose != 0, nv, op->stream, op->ws[4], nullptr, op->ws[5],
                             op->ws[2], 0.5, 1.0);

@@ -2469,12 +2469,19 @@ __global__ __launch_bounds__(256) void k_spmm_quad4(
 // doubles feed its VPL products and its VPL results are adjacent in the row.
 // Slots in groups of W, ascending; the group's 4 W block columns come from
 // one vector load and reach their lanes by ds_bpermute.
-template <typename VT, int R, int W>
+//
+// FR != 0: a Horner term of a kind 1 receptivity operator with its
+// replicated rows fused in (FusedRepDev, as k_term_lds; extra values real
+// for FR = 1, complex for FR = 2, the R doubles of a row then being R / 2
+// complex values): workgroups [0, nfs_wg) finish the replicated rows, the
+// others (one 16-cell slice per 4-wave workgroup) add their rows' entries in
+// the epilogue.
+template <typename VT, int R, int W, int FR = 0>
 __global__ __launch_bounds__(256) void k_spmm_quad4c(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ val, int64_t ncell, const double *__restrict__ x,
     double *__restrict__ tout, const double *vin, double *vout,
-    double c1, double c2) {
+    double c1, double c2, FusedRepDev F) {
     constexpr int B = 4, CPS = 16, BB = 16;
     constexpr int V = Vpl<VT>::value;
     constexpr int VPL = R >= 16 ? 4 : 2;          // vectors of a lane's x chunk
@@ -2485,8 +2492,25 @@ __global__ __launch_bounds__(256) void k_spmm_quad4c(
     static_assert(4 * W <= 64, "four columns per slot");
     const int lane = threadIdx.x & 63;
     const int64_t wpb = blockDim.x >> 6;
+    int64_t bid = blockIdx.x, nwg = gridDim.x;
+    if constexpr (FR != 0) {
+        // the replicated rows first (nfs_wg a multiple of 8: the slices keep
+        // the XCD mapping of an unfused launch)
+        if (bid < F.nfs_wg) {
+            rep_sum_wave<R, FR == 2, true>(F.R, bid * wpb + (threadIdx.x >> 6), x, F.s_out,
+                                           tout, vin, vout, c1, c2);
+            return;
+        }
+        bid -= F.nfs_wg;
+        nwg -= F.nfs_wg;
+    }
+    int64_t tile = xcd_tile(bid, nwg);
+    if constexpr (FR != 0) {
+        tile += F.rot;   // in slices (one per workgroup); rot < nwg
+        if (tile >= nwg) tile -= nwg;
+    }
     const int64_t cell0 = 4 * (int64_t)__builtin_amdgcn_readfirstlane(
-        (int)(xcd_tile(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6)));
+        (int)(tile * wpb + (threadIdx.x >> 6)));
     if (cell0 >= ncell) return;
     const int lo = lane & 3, blk = (lane >> 2) & 3, hi = lane >> 4;
     const int64_t slice = cell0 / CPS;
@@ -2543,6 +2567,40 @@ __global__ __launch_bounds__(256) void k_spmm_quad4c(
         }
     }
     if (cell0 + blk >= ncell) return;
+    if constexpr (FR != 0) {
+        // row (cell0 + blk) * 4 + hi: its compressed extra entries times the
+        // column sums of x, for this lane's doubles of the row
+        const int64_t row = (cell0 + blk) * B + hi;
+        const int32_t fb = F.slice_tab[row >> 6];
+        if (fb >= 0) {
+            const int32_t *lp = F.lane_ptr + (int64_t)fb * 65 + (row & 63);
+            const int32_t e1 = lp[1];
+            for (int32_t e = lp[0]; e < e1; ++e) {
+                const double *sp = F.s_in + (int64_t)F.ecol[e] * R + VPL * lo;
+                if constexpr (FR == 2) {
+                    const double a0 = F.eval[2 * (int64_t)e], a1 = F.eval[2 * (int64_t)e + 1];
+#pragma unroll
+                    for (int i = 0; i < NG; ++i)
+#pragma unroll
+                        for (int h = 0; h < H; ++h) {
+                            const d2 s = *reinterpret_cast<const d2 *>(sp + GW * i + 2 * h);
+                            acc[i][2 * h] = fma(a0, s[0], fma(-a1, s[1], acc[i][2 * h]));
+                            acc[i][2 * h + 1] = fma(a0, s[1], fma(a1, s[0], acc[i][2 * h + 1]));
+                        }
+                } else {
+                    const double a = F.eval[e];
+#pragma unroll
+                    for (int i = 0; i < NG; ++i)
+#pragma unroll
+                        for (int h = 0; h < H; ++h) {
+                            const d2 s = *reinterpret_cast<const d2 *>(sp + GW * i + 2 * h);
+                            acc[i][2 * h] = fma(a, s[0], acc[i][2 * h]);
+                            acc[i][2 * h + 1] = fma(a, s[1], acc[i][2 * h + 1]);
+                        }
+                }
+            }
+        }
+    }
     // the vout stores first, behind one wait for the epilogue operands (a
     // store between two such waits would be waited for as well)
     if (vout) {

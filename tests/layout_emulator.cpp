@@ -252,20 +252,24 @@ extern "C" int emu_spmv_csc_rep(int64_t n, const int64_t *colptr, const int64_t 
 }
 
 // A Horner sweep T_order(A dt)^nsub x with the replicated rows fused into
-// the term kernel, at two doubles per row (one complex vector, or two real
-// ones, x_vm[vm * 2 + v]): the prologue's column sums (k_rep_sum with no
-// outputs), then per term k_term_lds<FR != 0>'s index math -- the
+// the term kernel, at nv doubles per row (x_vm[vm * nv + d]; complex
+// operators: nv / 2 complex values): the prologue's column sums (k_rep_sum
+// with no outputs), then per term the fused kernel's index math -- the
 // replicated rows by rep_sum_wave's lane mapping summing the new rows, the
-// flow rows' entries from the slice tables in the epilogue -- with the two
-// sum buffers alternating and the work buffers picked as run_taylor_in
-// picks them.  info[0] = 1 when the tables were built (nothing is computed
-// otherwise), info[1] their blocks, info[2] their entries.
+// flow rows' entries from the slice tables in the epilogue, the slices in
+// rotated order -- with the two sum buffers alternating and the work
+// buffers picked as run_taylor_in picks them.  nv = 2: k_term_lds<FR>'s
+// lanes (one per row); nv = 8..64 (b = 4): k_spmm_quad4c<FR>'s lanes (row
+// (cell0 + blk) * 4 + hi, doubles GW i + VPL lo + g), its block part
+// computed from the BSR.  info[0] = 1 when the tables were built (nothing
+// is computed otherwise), info[1] their blocks, info[2] their entries.
 extern "C" int emu_horner_fused(int64_t n, const int64_t *colptr, const int64_t *rowval,
                                 const double *nzval, int base, int is_complex, int b,
-                                int64_t ncell, int order, int nsub, double dt, const double *x_vm,
-                                double *y_vm, int64_t *info) {
+                                int64_t ncell, int order, int nsub, double dt, int nv,
+                                const double *x_vm, double *y_vm, int64_t *info) {
     g_oob = 0;
     info[0] = info[1] = info[2] = 0;
+    if (!(nv == 2 || (b == 4 && (nv == 8 || nv == 16 || nv == 32 || nv == 64)))) return -1;
     HostBSR fwd, adj;
     HostExtra xf, xa;
     csc_to_blocks(n, colptr, rowval, nzval, base, is_complex != 0, 1.0, b, ncell, fwd, adj, xf, xa);
@@ -280,9 +284,11 @@ extern "C" int emu_horner_fused(int64_t n, const int64_t *colptr, const int64_t 
     HostSliced s;
     bsr_to_sliced(fwd, s);
     interleave_lanes(s, 2);
-    constexpr int NV = 2, KS = 8, KU = 4;
+    constexpr int KU = 4;
+    const int NV = nv;
     const bool cplx = is_complex != 0;
-    const int vs = cplx ? 2 : 1, NC = cplx ? 2 : 1, LPR = cplx ? 1 : 2;
+    const int vs = cplx ? 2 : 1, NC = cplx ? 2 : 1, LPR = cplx ? NV / 2 : NV;
+    const int KS = (64 / LPR) >= 8 ? 8 : 64 / LPR;
     const int VPW = 64 / (LPR * KS);
     std::vector<double> in((size_t)n * NV), pool[4], S[2];
     for (auto &p : pool) p.assign((size_t)n * NV, 0.0);
@@ -352,7 +358,29 @@ extern "C" int emu_horner_fused(int64_t n, const int64_t *colptr, const int64_t 
             }
         }
     };
+    // entries of `row` times the sums, added to the doubles d0 + 0, 1 of acc
+    auto add_entries = [&](int64_t row, const std::vector<double> &Sin, int64_t d0, double *a2) {
+        const int32_t fb = at(F.slice_tab, row >> 6);
+        if (fb < 0) return;
+        const int64_t e1 = at(F.lane_ptr, (int64_t)fb * 65 + (row & 63) + 1);
+        for (int64_t e = at(F.lane_ptr, (int64_t)fb * 65 + (row & 63)); e < e1; ++e) {
+            const int64_t qc = at(F.ecol, e);
+            const double s0 = at(Sin, qc * NV + d0), s1 = at(Sin, qc * NV + d0 + 1);
+            if (cplx) {
+                const double a0 = at(F.eval, 2 * e), a1 = at(F.eval, 2 * e + 1);
+                a2[0] += a0 * s0 - a1 * s1;
+                a2[1] += a0 * s1 + a1 * s0;
+            } else {
+                const double a = at(F.eval, e);
+                a2[0] += a * s0;
+                a2[1] += a * s1;
+            }
+        }
+    };
     const int CPS = 64 / b, BB = b * b;
+    int64_t rot = 0;   // slices from the first one with entries on, wrapping
+    while (rot < s.nslice && at(F.slice_tab, rot) < 0) ++rot;
+    if (rot == s.nslice) rot = 0;
     int slot = 0;
     const std::vector<double> *cur = &in;
     if (nsub > 0) rep_waves(false, *cur, nullptr, nullptr, 0.0, 0.0, S[slot]);
@@ -367,48 +395,62 @@ extern "C" int emu_horner_fused(int64_t n, const int64_t *colptr, const int64_t 
             const double c1 = dt / (double)j, c2 = 1.0;
             const std::vector<double> &Sin = S[slot];
             rep_waves(true, *w, cur, &wn, c1, c2, S[slot ^ 1]);
-            // slices from the first one with entries on, wrapping
-            int64_t rot = 0;
-            while (rot < s.nslice && at(F.slice_tab, rot) < 0) ++rot;
-            if (rot == s.nslice) rot = 0;
             for (int64_t t = 0; t < s.nslice; ++t) {
                 int64_t slice = t + rot;
                 if (slice >= s.nslice) slice -= s.nslice;
-                const int64_t p0 = at(s.slice_ptr, slice);
-                const int64_t width = (at(s.slice_ptr, slice + 1) - p0) / CPS;
-                const int32_t blk = at(F.slice_tab, slice);
-                for (int lane = 0; lane < 64; ++lane) {
-                    const int q = lane / b, r = lane % b;
-                    double acc[2] = {0, 0};
-                    for (int64_t t = 0; t < width; ++t) {
-                        const int64_t c = at(s.col, p0 + q + t * CPS);
-                        if (c < 0 || c >= ncell) ++g_oob;
-                        const int64_t sb = (p0 + t * CPS) * BB;
-                        for (int k = 0; k < b; ++k) {
-                            const double a = at(s.val, sb + ((k / 2) * 64 + lane) * 2 + k % 2);
-                            for (int v = 0; v < NV; ++v) acc[v] += a * at(*w, (c * b + k) * NV + v);
-                        }
-                    }
-                    const int64_t cell = slice * CPS + q;
-                    if (cell >= ncell) continue;
-                    if (blk >= 0) {
-                        const int64_t e1 = at(F.lane_ptr, (int64_t)blk * 65 + lane + 1);
-                        for (int64_t e = at(F.lane_ptr, (int64_t)blk * 65 + lane); e < e1; ++e) {
-                            const int64_t qc = at(F.ecol, e);
-                            if (cplx) {
-                                const double a0 = at(F.eval, 2 * e), a1 = at(F.eval, 2 * e + 1);
-                                const double s0 = at(Sin, qc * 2), s1 = at(Sin, qc * 2 + 1);
-                                acc[0] += a0 * s0 - a1 * s1;
-                                acc[1] += a0 * s1 + a1 * s0;
-                            } else {
-                                const double a = at(F.eval, e);
-                                for (int v = 0; v < NV; ++v) acc[v] += a * at(Sin, qc * NV + v);
+                if (NV == 2) {
+                    // k_term_lds: lane (q, r) of the slice owns row slice * 64 + lane
+                    const int64_t p0 = at(s.slice_ptr, slice);
+                    const int64_t width = (at(s.slice_ptr, slice + 1) - p0) / CPS;
+                    for (int lane = 0; lane < 64; ++lane) {
+                        const int q = lane / b, r = lane % b;
+                        double acc[2] = {0, 0};
+                        for (int64_t tt = 0; tt < width; ++tt) {
+                            const int64_t c = at(s.col, p0 + q + tt * CPS);
+                            if (c < 0 || c >= ncell) ++g_oob;
+                            const int64_t sb = (p0 + tt * CPS) * BB;
+                            for (int k = 0; k < b; ++k) {
+                                const double a = at(s.val, sb + ((k / 2) * 64 + lane) * 2 + k % 2);
+                                for (int v = 0; v < 2; ++v) acc[v] += a * at(*w, (c * b + k) * 2 + v);
                             }
                         }
+                        const int64_t cell = slice * CPS + q;
+                        if (cell >= ncell) continue;
+                        const int64_t row = cell * b + r;
+                        add_entries(row, Sin, 0, acc);
+                        for (int v = 0; v < 2; ++v)
+                            put(wn, row * 2 + v, c2 * (c1 * acc[v]) + at(*cur, row * 2 + v));
                     }
-                    const int64_t row = cell * b + r;
-                    for (int v = 0; v < NV; ++v)
-                        put(wn, row * NV + v, c2 * (c1 * acc[v]) + at(*cur, row * NV + v));
+                } else {
+                    // k_spmm_quad4c: workgroup `slice`, four waves of four cells
+                    const int VPL = NV >= 16 ? 4 : 2, GW = 4 * VPL, NG = NV / GW, H = VPL / 2;
+                    for (int wv = 0; wv < 4; ++wv) {
+                        const int64_t cell0 = 4 * (slice * 4 + wv);
+                        if (cell0 >= ncell) continue;
+                        for (int lane = 0; lane < 64; ++lane) {
+                            const int lo = lane & 3, blk = (lane >> 2) & 3, hi = lane >> 4;
+                            const int64_t cell = cell0 + blk;
+                            if (cell >= ncell) continue;
+                            const int64_t row = cell * 4 + hi;
+                            for (int i = 0; i < NG; ++i)
+                                for (int h = 0; h < H; ++h) {
+                                    const int64_t d0 = GW * i + VPL * lo + 2 * h;
+                                    double acc[2] = {0, 0};
+                                    for (int64_t k = at(fwd.ptr, cell); k < at(fwd.ptr, cell + 1); ++k) {
+                                        const int64_t c = at(fwd.col, k);
+                                        for (int jj = 0; jj < 4; ++jj) {
+                                            const double a = at(fwd.val, k * 16 + hi * 4 + jj);
+                                            for (int e = 0; e < 2; ++e)
+                                                acc[e] += a * at(*w, (c * 4 + jj) * NV + d0 + e);
+                                        }
+                                    }
+                                    add_entries(row, Sin, d0, acc);
+                                    for (int e = 0; e < 2; ++e)
+                                        put(wn, row * NV + d0 + e,
+                                            c2 * (c1 * acc[e]) + at(*cur, row * NV + d0 + e));
+                                }
+                        }
+                    }
                 }
             }
             slot ^= 1;

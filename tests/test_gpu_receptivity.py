@@ -249,3 +249,44 @@ def test_fused_horner_real_values(orc):
         Y = dev.expm_taylor_batched(X, order=5, nsub=3, dt=1.0, form="horner")
     for v in range(2):
         assert relerr(Y[:, v], orc.power_method(Ao, X[:, v], 3, 5)) <= 1e-12, v
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nrhs,prec", [(4, "f64"), (8, "f64"), (16, "f64"), (32, "f64"),
+                                       (8, "f32")])
+def test_assembled_batches_fused_horner(orc, nrhs, prec):
+    """nrhs complex vectors (8-64 doubles per row) in Horner form on the
+    assembled extended operator: the b = 4 batch kernel runs with the
+    replicated rows fused in (one launch per term).  Every column equals the
+    oracle's POWER_METHOD! (1e-12 with FP64 values, 2e-6 with FP32) and the
+    powers form, which keeps separate launches (1e-13); three graph replays
+    give the same bits; at 8 doubles per row the fused term reports the
+    compressed entries' bytes (the separate launches read B's copies)."""
+    from hymor_amd.core import DeviceOperator
+    nchi, neta = 10, 7
+    say(f"[assembled receptivity, fused Horner batches] nrhs={nrhs} values={prec}")
+    base = ops.stencil_operator(nchi, neta, 4, seed=47, row_sum=2.0)
+    L_ = ops.receptivity_extension(base, np.linspace(0.1, 6.0, 12))
+    Ao = orc.CSC.from_scipy(L_.to_scipy_varmajor())
+    r = np.random.default_rng(80 + nrhs)
+    X = r.standard_normal((L_.n, nrhs)) + 1j * r.standard_normal((L_.n, nrhs))
+    dt, nsub, order = 0.05, 3, 5
+    tol = 1e-12 if prec == "f64" else 2e-6
+    with DeviceOperator.from_blocks(L_, scale=dt, value_prec=prec) as dev:
+        Y = dev.expm_taylor_batched(X, order=order, nsub=nsub, dt=1.0, form="horner")
+        P = dev.expm_taylor_batched(X, order=order, nsub=nsub, dt=1.0, form="powers")
+        for j in range(nrhs):
+            ref = orc.power_method(Ao.scaled(dt), X[:, j], nsub, order)
+            assert relerr(Y[:, j], ref) <= tol, j
+            assert relerr(Y[:, j], P[:, j]) <= 1e-13, j
+        dev.set_graphs(True)
+        for _ in range(3):
+            assert np.array_equal(
+                dev.expm_taylor_batched(X, order=order, nsub=nsub, dt=1.0, form="horner"), Y)
+        dev.set_graphs(False)
+        if nrhs == 4:
+            _, by_h = dev.bench_term_kernel(reps=2, nv=8, form="horner")
+            _, by_p = dev.bench_term_kernel(reps=2, nv=8, form="powers")
+            # Horner moves 3 vectors, powers 4; beyond that, B's copies
+            n8 = L_.n * 8 * 8.0
+            assert by_h - 3 * n8 < by_p - 4 * n8

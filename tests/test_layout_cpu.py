@@ -138,9 +138,10 @@ def test_emulated_single_wavenumber_is_not_compressed(emu):
     assert np.allclose(y, L_.to_scipy_varmajor() @ x, rtol=1e-13, atol=1e-13)
 
 
-def _run_fused(emu, A, b, ncell, X2, order, nsub, dt):
-    """emu_horner_fused on scipy matrix A (var-major) with two doubles per
-    row: X2 is n complex values or an n x 2 real array."""
+def _run_fused(emu, A, b, ncell, X2, order, nsub, dt, nv=2):
+    """emu_horner_fused on scipy matrix A (var-major) with nv doubles per
+    row: X2 is n x nv/2 complex values (or n complex values at nv = 2) or
+    an n x nv real array."""
     A = A.tocsc()
     A.sort_indices()
     cplx = np.iscomplexobj(A.data)
@@ -154,7 +155,7 @@ def _run_fused(emu, A, b, ncell, X2, order, nsub, dt):
     oob = emu.emu_horner_fused(
         C.c_int64(A.shape[0]), C.c_void_p(colptr.ctypes.data), C.c_void_p(rowval.ctypes.data),
         C.c_void_p(nzval.ctypes.data), C.c_int(1), C.c_int(int(cplx)), C.c_int(b),
-        C.c_int64(ncell), C.c_int(order), C.c_int(nsub), C.c_double(dt),
+        C.c_int64(ncell), C.c_int(order), C.c_int(nsub), C.c_double(dt), C.c_int(nv),
         C.c_void_p(x.ctypes.data), C.c_void_p(y.ctypes.data), C.c_void_p(info.ctypes.data))
     return y, oob, info
 
@@ -179,6 +180,25 @@ def test_emulated_fused_horner_sweep(emu, orc, nchi, neta, w):
     assert info[0] == 1 and info[2] == nnz_B
     ref = orc.power_method(orc.CSC.from_scipy(A).scaled(dt), x, nsub, order)
     assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("nv", [8, 16, 32, 64])
+def test_emulated_fused_horner_batched(emu, orc, nv):
+    """The fused sweep in the b = 4 batch kernel's lane layout (4 cells per
+    wave, each lane a row and a run of its doubles): nv / 2 complex vectors,
+    every column against the oracle's POWER_METHOD!, in bounds."""
+    base = ops.stencil_operator(9, 7, 4, seed=12)
+    L_ = ops.receptivity_extension(base, list(np.linspace(0.2, 5.0, 6)))
+    A = L_.to_scipy_varmajor()
+    r = np.random.default_rng(nv)
+    X = r.standard_normal((L_.n, nv // 2)) + 1j * r.standard_normal((L_.n, nv // 2))
+    order, nsub, dt = 4, 2, 0.05
+    Y, oob, info = _run_fused(emu, A, 4, base.ncell, X, order, nsub, dt, nv=nv)
+    assert oob == 0 and info[0] == 1
+    Ao = orc.CSC.from_scipy(A).scaled(dt)
+    for v in (0, nv // 2 - 1):
+        ref = orc.power_method(Ao, X[:, v], nsub, order)
+        assert np.linalg.norm(Y[:, v] - ref) <= 1e-12 * np.linalg.norm(ref), v
 
 
 def test_emulated_fused_horner_real_values(emu, orc):
