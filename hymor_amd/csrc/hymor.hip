@@ -874,10 +874,20 @@ const DevExtra &extra_of(const hymor_op *op, bool adj, int nv) {
     return adj ? op->xa : op->xf;
 }
 
+int fused_kind(const hymor_op *op, bool adj, int nv);
+void launch_term_fused(hymor_op *op, bool adj, hipStream_t st, const double *x, double *tout,
+                       const double *vin, double *vout, double c1, double c2, int slot, int nv);
+
 // One Taylor term (or a plain SpMV when vout == nullptr) of `op` on `st`;
 // nv doubles per row.
 void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, const double *x, double *tout,
                  const double *vin, double *vout, double c1, double c2) {
+    // an adjoint whose only extra rows are its replicated blocks' shared
+    // rows: one launch
+    if (adj && fused_kind(op, true, nv)) {
+        launch_term_fused(op, true, st, x, tout, vin, vout, c1, c2, 0, nv);
+        return;
+    }
     const DevSliced &s = adj ? op->adj : op->fwd;
     const DevExtra &xe = extra_of(op, adj, nv);
     if (op->ncell > 0) {
@@ -895,15 +905,24 @@ void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, const double *x
     }
 }
 
-// Which fused Horner term applies (0: none -- the block kernel and the
-// extra rows run as separate launches): forward, tables built for the
-// operator (build_fused_tables), no per-column diagonal, and
+// Which fused term applies (0: none -- the block kernel and the extra rows
+// run as separate launches).  Forward: Horner terms of an operator with the
+// fused tables (build_fused_tables).  Adjoint: any term of an operator whose
+// only extra rows are the shared rows of its replicated blocks (kind 2,
+// starting at the first extra DOF).  Both: no per-column diagonal, and
 //   1: two doubles per row under the default tuning (k_term_lds, one wave
 //      per slice, non-temporal value loads);
 //   2: 8-64 doubles per row at b = 4 under the default batch kernel
 //      (k_spmm_quad4c).
 int fused_kind(const hymor_op *op, bool adj, int nv) {
-    if (adj || !op->xf_rep.fused || op->ncell == 0 || op->cdiag_on) return 0;
+    if (op->ncell == 0 || op->cdiag_on) return 0;
+    if (adj) {
+        const DevExtra &xa = op->xa_rep;
+        if (!op->has_adj || xa.rep.kind != 2 || xa.nrows != 0 || xa.rep.c0 != op->nflow)
+            return 0;
+    } else if (!op->xf_rep.fused) {
+        return 0;
+    }
     const TermTuning &t = op->fwd.tune;
     if (nv == 2) return (t.split == 3 && t.waves == 1 && t.nt == 1) ? 1 : 0;
     if (op->b == 4 && nv >= 8 && nv <= 64 && (t.spmm_mfma == 6 || t.spmm_mfma == 8)) return 2;
@@ -956,84 +975,97 @@ void launch_fused_prologue(hymor_op *op, hipStream_t st, const double *x, int sl
 
 template <int B, typename VT, int FR>
 void launch_fused_lds_t(const DevSliced &s, const FusedRepDev &F, hipStream_t st,
-                        const double *x, const double *vin, double *vout, double c1, double c2) {
+                        const double *x, double *tout, const double *vin, double *vout,
+                        double c1, double c2) {
     hipLaunchKernelGGL((k_term_lds<B, VT, 2, 1, 9, true, FR>),
                        dim3((unsigned)(F.nfs_wg + s.nslice)), dim3(64), 0, st, s.slice_ptr,
-                       s.col, (const VT *)s.val, s.nslice, s.ncell, x, nullptr, vin, vout, c1,
-                       c2, F);
+                       s.col, (const VT *)s.val, s.nslice, s.ncell, x, tout, vin, vout, c1, c2,
+                       F);
 }
 
 template <typename VT, int R, int W, int FR>
 void launch_fused_quad4c_t(const DevSliced &s, const FusedRepDev &F, hipStream_t st,
-                           const double *x, const double *vin, double *vout, double c1,
-                           double c2) {
+                           const double *x, double *tout, const double *vin, double *vout,
+                           double c1, double c2) {
     const int wpb = 4;
     const int64_t ngrp = (s.ncell + 3) / 4;
     const int64_t grid = (ngrp + wpb - 1) / wpb;
     hipLaunchKernelGGL((k_spmm_quad4c<VT, R, W, FR>), dim3((unsigned)(F.nfs_wg + grid)),
                        dim3(64 * wpb), 0, st, s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x,
-                       nullptr, vin, vout, c1, c2, F);
+                       tout, vin, vout, c1, c2, F);
 }
 
 // the slot groups launch_block_part uses at each width
 template <typename VT, int FR>
 void launch_fused_quad4c(const DevSliced &s, const FusedRepDev &F, int nv, hipStream_t st,
-                         const double *x, const double *vin, double *vout, double c1,
-                         double c2) {
+                         const double *x, double *tout, const double *vin, double *vout,
+                         double c1, double c2) {
     switch (nv) {
-    case 8: launch_fused_quad4c_t<VT, 8, 9, FR>(s, F, st, x, vin, vout, c1, c2); break;
-    case 16: launch_fused_quad4c_t<VT, 16, 9, FR>(s, F, st, x, vin, vout, c1, c2); break;
-    case 32: launch_fused_quad4c_t<VT, 32, 3, FR>(s, F, st, x, vin, vout, c1, c2); break;
-    case 64: launch_fused_quad4c_t<VT, 64, 3, FR>(s, F, st, x, vin, vout, c1, c2); break;
+    case 8: launch_fused_quad4c_t<VT, 8, 9, FR>(s, F, st, x, tout, vin, vout, c1, c2); break;
+    case 16: launch_fused_quad4c_t<VT, 16, 9, FR>(s, F, st, x, tout, vin, vout, c1, c2); break;
+    case 32: launch_fused_quad4c_t<VT, 32, 3, FR>(s, F, st, x, tout, vin, vout, c1, c2); break;
+    case 64: launch_fused_quad4c_t<VT, 64, 3, FR>(s, F, st, x, tout, vin, vout, c1, c2); break;
     default: throw ArgError("fused term: unsupported vector width (internal)");
     }
 }
 
-// One Horner term w_new = v + c1 L_ w (c2 = 1) in one launch (fused_kind
-// != 0) at nv doubles per row: the flow rows take the column sums of w from
-// fr_sum[slot], the replicated rows leave those of w_new in fr_sum[slot ^ 1].
-void launch_term_fused(hymor_op *op, hipStream_t st, const double *x, const double *vin,
-                       double *vout, double c1, double c2, int slot, int nv) {
-    const DevSliced &s = op->fwd;
-    const DevExtra &xe = op->xf_rep;
-    const int kind = fused_kind(op, false, nv);
-    FusedRepDev F;
-    F.rot = xe.fr_rot;
-    F.R = xe.rep;
-    F.s_in = xe.fr_sum[slot];
-    F.s_out = xe.fr_sum[slot ^ 1];
-    F.slice_tab = xe.fr_slice;
-    F.lane_ptr = xe.fr_lane;
-    F.ecol = xe.fr_col;
-    F.eval = xe.fr_val;
+template <int FR>
+void launch_fused_fr(const DevSliced &s, const FusedRepDev &F, int kind, int nv, hipStream_t st,
+                     const double *x, double *tout, const double *vin, double *vout, double c1,
+                     double c2) {
     const bool f32 = s.prec == HYMOR_PREC_F32;
     if (kind == 1) {
-        F.nfs_wg = (fused_rep_waves(xe, 2) + 7) / 8 * 8;   // one wave per workgroup
-#define HYMOR_FUSED(BV, VT)                                                              \
-    (xe.is_complex ? launch_fused_lds_t<BV, VT, 2>(s, F, st, x, vin, vout, c1, c2)       \
-                   : launch_fused_lds_t<BV, VT, 1>(s, F, st, x, vin, vout, c1, c2))
         if (s.b == 4) {
-            if (f32) HYMOR_FUSED(4, float);
-            else HYMOR_FUSED(4, double);
+            if (f32) launch_fused_lds_t<4, float, FR>(s, F, st, x, tout, vin, vout, c1, c2);
+            else launch_fused_lds_t<4, double, FR>(s, F, st, x, tout, vin, vout, c1, c2);
         } else if (s.b == 8) {
-            if (f32) HYMOR_FUSED(8, float);
-            else HYMOR_FUSED(8, double);
+            if (f32) launch_fused_lds_t<8, float, FR>(s, F, st, x, tout, vin, vout, c1, c2);
+            else launch_fused_lds_t<8, double, FR>(s, F, st, x, tout, vin, vout, c1, c2);
         } else {
             throw ArgError("vars_per_cell must be 4 or 8");
         }
-#undef HYMOR_FUSED
-    } else if (kind == 2) {
-        // four waves per workgroup
-        F.nfs_wg = ((fused_rep_waves(xe, nv) + 3) / 4 + 7) / 8 * 8;
-        if (f32) {
-            if (xe.is_complex) launch_fused_quad4c<float, 2>(s, F, nv, st, x, vin, vout, c1, c2);
-            else launch_fused_quad4c<float, 1>(s, F, nv, st, x, vin, vout, c1, c2);
-        } else {
-            if (xe.is_complex) launch_fused_quad4c<double, 2>(s, F, nv, st, x, vin, vout, c1, c2);
-            else launch_fused_quad4c<double, 1>(s, F, nv, st, x, vin, vout, c1, c2);
-        }
     } else {
-        throw ArgError("fused term does not apply (internal)");
+        if (f32) launch_fused_quad4c<float, FR>(s, F, nv, st, x, tout, vin, vout, c1, c2);
+        else launch_fused_quad4c<double, FR>(s, F, nv, st, x, tout, vin, vout, c1, c2);
+    }
+}
+
+// One term in one launch (fused_kind != 0) at nv doubles per row.  Forward
+// (a Horner term w_new = v + c1 L_ w, c2 = 1, tout null): the flow rows take
+// the column sums of w from fr_sum[slot], the replicated rows leave those of
+// w_new in fr_sum[slot ^ 1].  Adjoint (any term, as launch_term): the
+// shared rows in the leading workgroups.
+void launch_term_fused(hymor_op *op, bool adj, hipStream_t st, const double *x, double *tout,
+                       const double *vin, double *vout, double c1, double c2, int slot,
+                       int nv) {
+    const DevSliced &s = adj ? op->adj : op->fwd;
+    const DevExtra &xe = adj ? op->xa_rep : op->xf_rep;
+    const int kind = fused_kind(op, adj, nv);
+    if (kind == 0) throw ArgError("fused term does not apply (internal)");
+    FusedRepDev F;
+    F.R = xe.rep;
+    if (adj) {
+        F.nflow = op->nflow;
+    } else {
+        F.rot = xe.fr_rot;
+        F.s_in = xe.fr_sum[slot];
+        F.s_out = xe.fr_sum[slot ^ 1];
+        F.slice_tab = xe.fr_slice;
+        F.lane_ptr = xe.fr_lane;
+        F.ecol = xe.fr_col;
+        F.eval = xe.fr_val;
+    }
+    // replicated-row waves (rep_sum_wave and extra_replica_rows share their
+    // lane layout): one wave per workgroup for k_term_lds, four for
+    // k_spmm_quad4c; a multiple of 8 workgroups
+    const int64_t waves = fused_rep_waves(xe, nv);
+    F.nfs_wg = ((kind == 1 ? waves : (waves + 3) / 4) + 7) / 8 * 8;
+    const int fr = (xe.is_complex ? 2 : 1) + (adj ? 4 : 0);
+    switch (fr) {
+    case 1: launch_fused_fr<1>(s, F, kind, nv, st, x, tout, vin, vout, c1, c2); break;
+    case 2: launch_fused_fr<2>(s, F, kind, nv, st, x, tout, vin, vout, c1, c2); break;
+    case 5: launch_fused_fr<5>(s, F, kind, nv, st, x, tout, vin, vout, c1, c2); break;
+    default: launch_fused_fr<6>(s, F, kind, nv, st, x, tout, vin, vout, c1, c2); break;
     }
     HIP_TRY(hipGetLastError());
 }
@@ -1086,7 +1118,8 @@ const double *run_taylor_in(hymor_op *op, const SweepBufs &b, bool adj, int orde
         // column sums of each term's x alternate between two buffers: the
         // prologue forms those of the sweep's input, each term those of the
         // w it writes -- the next term's x, or the next substep's v = x.
-        const bool fused = fused_kind(op, adj, nv) != 0;
+        // (the adjoint's fused term needs no sums: launch_term takes it)
+        const bool fused = !adj && fused_kind(op, adj, nv) != 0;
         int slot = 0;
         if (fused && nsub > 0) launch_fused_prologue(op, op->stream, cur, slot, nv);
         double *const pool[4] = {va, vb, t0, t1};
@@ -1099,8 +1132,8 @@ const double *run_taylor_in(hymor_op *op, const SweepBufs &b, bool adj, int orde
             for (int j = order; j >= 1; --j) {
                 double *wn = pp[(order - j) & 1];
                 if (fused) {
-                    launch_term_fused(op, op->stream, w, cur, wn, dt / (double)j, 1.0, slot,
-                                      nv);
+                    launch_term_fused(op, adj, op->stream, w, nullptr, cur, wn, dt / (double)j,
+                                      1.0, slot, nv);
                     slot ^= 1;
                 } else {
                     launch_term(op, adj, nv, op->stream, w, nullptr, cur, wn, dt / (double)j,
@@ -1926,12 +1959,12 @@ int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int form, int r
         // the new w (in one launch with the replicated rows fused in where a
         // sweep does so: the sums of w formed once, into slot 0).  One warm
         // launch, then the timed ones
-        const bool fused = horner && fused_kind(op, transpose != 0, nv) != 0;
+        const bool fused = horner && !transpose && fused_kind(op, false, nv) != 0;
         if (fused) launch_fused_prologue(op, op->stream, op->ws[4], 0, nv);
         auto term = [&] {
             if (fused)
-                launch_term_fused(op, op->stream, op->ws[4], op->ws[5], op->ws[2], 0.5, 1.0, 0,
-                                  nv);
+                launch_term_fused(op, false, op->stream, op->ws[4], nullptr, op->ws[5],
+                                  op->ws[2], 0.5, 1.0, 0, nv);
             else if (horner)
                 launch_term(op, transpose != 0, nv, op->stream, op->ws[4], nullptr, op->ws[5],
                             op->ws[2], 0.5, 1.0);
@@ -1956,8 +1989,10 @@ int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int form, int r
         // written)
         double bytes = (double)s.nnzb * (bb * sval + 4.0) + (double)(s.ncell + 1) * 4.0 +
                        (horner ? 3.0 : 4.0) * (double)op->n * nv * 8.0;
-        // (the fused term reads the compressed entries)
-        const DevExtra &xe = fused ? op->xf_rep : extra_of(op, transpose != 0, nv);
+        // (the fused terms read the compressed entries)
+        const bool fused_adj = transpose && fused_kind(op, true, nv) != 0;
+        const DevExtra &xe = fused ? op->xf_rep
+                                   : (fused_adj ? op->xa_rep : extra_of(op, transpose != 0, nv));
         const double vsz = xe.is_complex ? 16.0 : 8.0;
         bytes += (double)xe.nnz * (vsz + 4.0);
         // replicated blocks: their diagonal, and the shared rows (kind 2)

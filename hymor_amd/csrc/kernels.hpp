@@ -180,8 +180,22 @@ struct ReplicaDev {
     const double *vval = nullptr;
 };
 
-// A Horner term with the replicated rows of a kind 1 operator fused into the
-// term kernel's launch: workgroups [0, nfs_wg) finish the replicated rows
+// The term kernels' fused replicated rows, FR = (complex extra values ? 2 : 1)
+// + (adjoint ? 4 : 0); 0: none.
+template <int FR>
+struct FusedKind {
+    static constexpr bool on = FR != 0;
+    static constexpr bool cplx = (FR & 3) == 2;
+    static constexpr bool adj = (FR & 4) != 0;
+    static constexpr bool fwd = on && !adj;
+};
+
+// Adjoint (kind 2): workgroups [0, nfs_wg) evaluate the shared rows and
+// finish their copies (extra_replica_rows, as k_extra would after the block
+// kernel; they read only x's flow rows), the others run the block term; no
+// column sums, no epilogue entries.
+// Forward: a Horner term with the replicated rows of a kind 1 operator fused
+// into the term kernel's launch: workgroups [0, nfs_wg) finish the replicated rows
 // and form the column sums of the new rows into s_out (the sums the next
 // term needs); the others add each flow row's compressed entries times the
 // sums s_in of this term's x (formed by the previous term, or once before
@@ -191,6 +205,7 @@ struct ReplicaDev {
 // column q of the sums) and eval (1 or 2 doubles each).
 struct FusedRepDev {
     int64_t nfs_wg = 0;
+    int64_t nflow = 0;   // adjoint: the shared rows read x's flow rows
     // slices are taken in the order rot, rot + 1, ..., wrapping: the first
     // slice with entries leads, so the slices whose epilogue does more work
     // are dispatched first instead of forming the launch's tail
@@ -1235,9 +1250,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 // different banks.  Same slot order, FMA order and LDS combination as
 // k_term_split_p, hence the same bits.
 //
-// FR != 0: a Horner term of a kind 1 receptivity operator with its
-// replicated rows fused in (FusedRepDev; extra values real for FR = 1,
-// complex for FR = 2): no k_extra launch follows.
+// FR != 0: the receptivity operator's replicated rows fused in (FusedRepDev,
+// FusedKind: forward Horner terms of a kind 1 operator, or any term of a
+// kind 2 adjoint): no k_extra launch follows.
 template <int B, typename VT, int NV, int NW, int S, bool NT, int FR = 0>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_term_lds(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
@@ -1254,7 +1269,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
     constexpr int XS = XC + 1;      // padded chunk stride of a cell's area
     static_assert(NV >= 2 && NV % 2 == 0, "x blocks move as pairs of doubles");
     typedef typename Chunk<VT>::type CT;
-    static_assert(FR != 2 || NV == 2, "complex extra values act on one complex vector");
+    using FK = FusedKind<FR>;
+    static_assert(!(FK::fwd && FK::cplx) || NV == 2, "complex entries act on one complex vector");
     __shared__ double part[NW > 1 ? NW - 1 : 1][NV][64];
     __shared__ d2 xs[NW][S][CPS * XS];
     const int lane = threadIdx.x & 63;
@@ -1264,15 +1280,19 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
         // the replicated rows first (nfs_wg is a multiple of 8, so the
         // slices below keep the XCD mapping of an unfused launch)
         if (bid < F.nfs_wg) {
-            rep_sum_wave<NV, FR == 2, true>(F.R, bid * NW + w, x, F.s_out, tout, vin, vout, c1,
-                                            c2);
+            if constexpr (FK::adj)
+                extra_replica_rows<NV, FK::cplx>(bid * NW + w, F.R, F.nflow, x, tout, vin, vout,
+                                                 c1, c2, 0);
+            else
+                rep_sum_wave<NV, FK::cplx, true>(F.R, bid * NW + w, x, F.s_out, tout, vin, vout,
+                                                 c1, c2);
             return;
         }
         bid -= F.nfs_wg;
         nwg -= F.nfs_wg;
     }
     int64_t slice = xcd_tile(bid, nwg);
-    if constexpr (FR != 0) {
+    if constexpr (FK::fwd) {
         slice += F.rot;   // rot < nslice == nwg
         if (slice >= nwg) slice -= nwg;
     }
@@ -1293,7 +1313,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
     // this lane's range of fused extra entries, fetched with the epilogue
     // operands so that the epilogue starts with them at hand
     int32_t fe0 = 0, fe1 = 0;
-    if constexpr (FR != 0) {
+    if constexpr (FK::fwd) {
         const int32_t blk = F.slice_tab[slice];
         if (blk >= 0 && w == 0) {
             fe0 = F.lane_ptr[(int64_t)blk * 65 + lane];
@@ -1376,11 +1396,11 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
             for (int v = 0; v < NV; ++v) acc[v] += part[i][v][lane];
     }
     if (cell >= ncell) return;
-    if constexpr (FR != 0) {
+    if constexpr (FK::fwd) {
         // this row's compressed extra entries times the column sums of x, in
         // groups of 8: the columns and values of a group, then its sums, are
         // in flight together; products added in entry order
-        constexpr int G = 8, VS = FR == 2 ? 2 : 1;
+        constexpr int G = 8, VS = FK::cplx ? 2 : 1;
         for (int32_t e0 = fe0; e0 < fe1; e0 += G) {
             int32_t qc[G];
             double a[G][VS];
@@ -1399,7 +1419,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
 #pragma unroll
             for (int u = 0; u < G; ++u) {
                 if (e0 + u >= fe1) break;
-                if constexpr (FR == 2) {
+                if constexpr (FK::cplx) {
                     acc[0] = fma(a[u][0], sv[u][0], fma(-a[u][1], sv[u][1], acc[0]));
                     acc[1] = fma(a[u][0], sv[u][1], fma(a[u][1], sv[u][0], acc[1]));
                 } else {
@@ -2470,12 +2490,11 @@ __global__ __launch_bounds__(256) void k_spmm_quad4(
 // Slots in groups of W, ascending; the group's 4 W block columns come from
 // one vector load and reach their lanes by ds_bpermute.
 //
-// FR != 0: a Horner term of a kind 1 receptivity operator with its
-// replicated rows fused in (FusedRepDev, as k_term_lds; extra values real
-// for FR = 1, complex for FR = 2, the R doubles of a row then being R / 2
-// complex values): workgroups [0, nfs_wg) finish the replicated rows, the
-// others (one 16-cell slice per 4-wave workgroup) add their rows' entries in
-// the epilogue.
+// FR != 0: the receptivity operator's replicated rows fused in (FusedRepDev,
+// FusedKind, as k_term_lds; with complex extra values the R doubles of a
+// row are R / 2 complex values): workgroups [0, nfs_wg) take the replicated
+// rows, the others (one 16-cell slice per 4-wave workgroup) the block term,
+// forward adding their rows' entries in the epilogue.
 template <typename VT, int R, int W, int FR = 0>
 __global__ __launch_bounds__(256) void k_spmm_quad4c(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
@@ -2492,20 +2511,24 @@ __global__ __launch_bounds__(256) void k_spmm_quad4c(
     static_assert(4 * W <= 64, "four columns per slot");
     const int lane = threadIdx.x & 63;
     const int64_t wpb = blockDim.x >> 6;
+    using FK = FusedKind<FR>;
     int64_t bid = blockIdx.x, nwg = gridDim.x;
-    if constexpr (FR != 0) {
+    if constexpr (FK::on) {
         // the replicated rows first (nfs_wg a multiple of 8: the slices keep
         // the XCD mapping of an unfused launch)
         if (bid < F.nfs_wg) {
-            rep_sum_wave<R, FR == 2, true>(F.R, bid * wpb + (threadIdx.x >> 6), x, F.s_out,
-                                           tout, vin, vout, c1, c2);
+            const int64_t wv = bid * wpb + (threadIdx.x >> 6);
+            if constexpr (FK::adj)
+                extra_replica_rows<R, FK::cplx>(wv, F.R, F.nflow, x, tout, vin, vout, c1, c2, 0);
+            else
+                rep_sum_wave<R, FK::cplx, true>(F.R, wv, x, F.s_out, tout, vin, vout, c1, c2);
             return;
         }
         bid -= F.nfs_wg;
         nwg -= F.nfs_wg;
     }
     int64_t tile = xcd_tile(bid, nwg);
-    if constexpr (FR != 0) {
+    if constexpr (FK::fwd) {
         tile += F.rot;   // in slices (one per workgroup); rot < nwg
         if (tile >= nwg) tile -= nwg;
     }
@@ -2567,7 +2590,7 @@ __global__ __launch_bounds__(256) void k_spmm_quad4c(
         }
     }
     if (cell0 + blk >= ncell) return;
-    if constexpr (FR != 0) {
+    if constexpr (FK::fwd) {
         // row (cell0 + blk) * 4 + hi: its compressed extra entries times the
         // column sums of x, for this lane's doubles of the row
         const int64_t row = (cell0 + blk) * B + hi;
@@ -2577,7 +2600,7 @@ __global__ __launch_bounds__(256) void k_spmm_quad4c(
             const int32_t e1 = lp[1];
             for (int32_t e = lp[0]; e < e1; ++e) {
                 const double *sp = F.s_in + (int64_t)F.ecol[e] * R + VPL * lo;
-                if constexpr (FR == 2) {
+                if constexpr (FK::cplx) {
                     const double a0 = F.eval[2 * (int64_t)e], a1 = F.eval[2 * (int64_t)e + 1];
 #pragma unroll
                     for (int i = 0; i < NG; ++i)

@@ -290,3 +290,46 @@ def test_assembled_batches_fused_horner(orc, nrhs, prec):
             # Horner moves 3 vectors, powers 4; beyond that, B's copies
             n8 = L_.n * 8 * 8.0
             assert by_h - 3 * n8 < by_p - 4 * n8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nrhs", [1, 4, 16])
+def test_adjoint_fused_term(orc, nrhs):
+    """The adjoint of the assembled extended operator without shock DOFs:
+    its only extra rows are the shared rows B^H x (+ the conjugate
+    diagonal) of the freestream copies, evaluated in the term kernel's own
+    launch for every Taylor form.  Horner and powers forms against the
+    oracle's POWER_METHOD! on L_^H (1e-12); the single vector also against
+    the two-launch term another tuning runs (1e-13), and the adjoint SpMV
+    against scipy (1e-13)."""
+    from hymor_amd.core import DeviceOperator
+    nchi, neta = 10, 6
+    say(f"[assembled receptivity, fused adjoint] nrhs={nrhs}")
+    base = ops.stencil_operator(nchi, neta, 4, seed=48, row_sum=2.0)
+    L_ = ops.receptivity_extension(base, np.linspace(0.1, 6.0, 10))
+    A_ = L_.to_scipy_varmajor()
+    Aot = orc.csc_adjoint(orc.CSC.from_scipy(A_))
+    r = np.random.default_rng(90 + nrhs)
+    X = r.standard_normal((L_.n, nrhs)) + 1j * r.standard_normal((L_.n, nrhs))
+    dt, nsub, order = 0.05, 3, 5
+    with DeviceOperator.from_blocks(L_, scale=dt) as dev:
+        for form in ("horner", "powers"):
+            if nrhs == 1:
+                Y = dev.expm_taylor(X[:, 0], order=order, nsub=nsub, dt=1.0, form=form,
+                                    transpose=True)[:, None]
+            else:
+                Y = dev.expm_taylor_batched(X, order=order, nsub=nsub, dt=1.0, form=form,
+                                            transpose=True)
+            for j in range(nrhs):
+                ref = orc.power_method(Aot.scaled(dt), X[:, j], nsub, order)
+                assert relerr(Y[:, j], ref) <= 1e-12, (form, j)
+        if nrhs == 1:
+            y = dev.spmv(X[:, 0], transpose=True)
+            ref = dt * (A_.conj().T @ X[:, 0])
+            assert relerr(y, ref) <= 1e-13
+            fused = dev.expm_taylor(X[:, 0], order=order, nsub=nsub, dt=1.0, form="horner",
+                                    transpose=True)
+            dev.set_tuning(3, 2, 1)
+            two = dev.expm_taylor(X[:, 0], order=order, nsub=nsub, dt=1.0, form="horner",
+                                  transpose=True)
+            assert relerr(fused, two) <= 1e-13
