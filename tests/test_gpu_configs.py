@@ -10,7 +10,9 @@ oracle SpMVs.
   w_k = 0.1 * 100^((k-1)/31) (N_ext = 564 000 complex): the factored
   receptivity action equals the assembled extended operator's action on the
   same vector, forward and adjoint, and one assembled SpMV equals the
-  oracle's.
+  oracle's; the Horner terms with the replicated rows fused into one launch
+  (one vector and 16-vector batches) equal the powers form's separate
+  launches, and the fused adjoint satisfies the dot-product identity.
 """
 import sys
 
@@ -49,12 +51,17 @@ def test_c3_properties(orc):
         assert relerr(T(a * x + b * y), a * Tx + b * T(y)) <= 1e-13
 
 
-def test_c4_factored_receptivity_full_size(orc):
+@pytest.fixture(scope="module")
+def c4_extended():
     say("[C4 500x250 b4, 32 wavenumbers] building the extended operator")
     base = ops.config_operator("C4")
     w = 0.1 * 100.0 ** (np.arange(32) / 31.0)
     L_ = ops.receptivity_extension(base, w)
-    A_ = L_.to_scipy_varmajor()
+    return base, w, L_, L_.to_scipy_varmajor()
+
+
+def test_c4_factored_receptivity_full_size(orc, c4_extended):
+    base, w, L_, A_ = c4_extended
     n_L, p = base.n, 4 * base.nchi
     L, B = split_extended(A_, n_L, p, w)
     r = np.random.default_rng(32)
@@ -73,3 +80,25 @@ def test_c4_factored_receptivity_full_size(orc):
         assert relerr(fr.expm(x, order=5, nsub=2, transpose=True), ref_t) <= 1e-12
     finally:
         fr.close()
+
+
+def test_c4_fused_terms_full_size(c4_extended):
+    base, w, L_, A_ = c4_extended
+    r = np.random.default_rng(33)
+    n = A_.shape[0]
+    x = r.standard_normal(n) + 1j * r.standard_normal(n)
+    y = r.standard_normal(n) + 1j * r.standard_normal(n)
+    X = r.standard_normal((n, 16)) + 1j * r.standard_normal((n, 16))
+    say(f"[C4] n_ext={n}: fused Horner terms vs the powers form; fused adjoint identity")
+    with DeviceOperator.from_blocks(L_, scale=1 / 64) as dev:
+        T = lambda v, form, tr=False: dev.expm_taylor(v, order=6, nsub=3, dt=1.0, form=form,
+                                                      transpose=tr)
+        assert relerr(T(x, "horner"), T(x, "powers")) <= 1e-13
+        Y = dev.expm_taylor_batched(X, order=6, nsub=3, dt=1.0, form="horner")
+        P = dev.expm_taylor_batched(X, order=6, nsub=3, dt=1.0, form="powers")
+        for j in range(16):
+            assert relerr(Y[:, j], P[:, j]) <= 1e-13, j
+        # <T x, y> = <x, T^H y>
+        lhs = np.vdot(y, T(x, "horner"))
+        rhs = np.vdot(T(y, "horner", tr=True), x)
+        assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
