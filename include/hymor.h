@@ -207,7 +207,11 @@ int hymor_dev_fill_uniform(void *d_x, int64_t len, uint64_t seed);
  * write the new term and update vout; Horner gathers w, reads v and writes
  * the new w) between two HIP events; returns the mean duration per launch
  * in ms and the kernel's algorithmic bytes per launch.  nv = doubles per
- * row (1 real, 2 complex, up to 64 for batches; a power of two). */
+ * row (1 real, 2 complex, up to 64 for batches; a power of two).  A term
+ * that a sweep runs as one launch with an assembled receptivity operator's
+ * replicated rows fused in is timed as such (the forward Horner term after
+ * one pass forming the column sums of its x), its bytes counting the
+ * replicated rows' compressed entries. */
 int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int form, int reps,
                             double *ms_per_launch, double *bytes_per_launch);
 
