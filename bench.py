@@ -352,15 +352,16 @@ def krylov_projection_leg(n=1440000, j=32, widths=(4, 16), reps=20):
             "cases": cases}
 
 
-def receptivity_leg(steps=3, order=12, nsub=64, T=1.0, n_w=32, widths=(4, 8, 16)):
+def receptivity_leg(steps=3, order=12, nsub=64, T=1.0, n_w=32, widths=(4, 8, 16, 32)):
     """C4: the action T_12(L_ T/64)^64 on the extended receptivity operator
     L_ = [L repeat(B,1,32); 0 kron(diag(-i w), I)] (LINEARIZE_L_.jl:58-67) of
     a 500 x 250-cell, b = 4 operator with 32 wavenumbers w_k = 0.1 *
     100^((k-1)/31), in extended actions per second:
       * assembled L_, FP64 values, one complex vector per sweep (the
         reference's form);
-      * assembled L_, 4, 8 and 16 complex vectors per sweep (8, 16 and 32 doubles
-        per row), FP32 and FP64 values -- the batches a block eigensolver
+      * assembled L_, 4, 8, 16 and 32 complex vectors per sweep (8-64 doubles
+        per row; 32 is the widest sweep, two of which apply a 64-vector
+        batch), FP32 and FP64 values -- the batches a block eigensolver
         over the extended space applies;
       * factored: the 32 per-wavenumber systems [L B; 0 -i w_k I] of ONE
         extended vector as 32 complex columns of one sweep (B stored once).
