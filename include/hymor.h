@@ -221,7 +221,10 @@ int hymor_bench_term_kernel(hymor_op *op, int transpose, int nv, int form, int r
  * waves per slice; at nv >= 2 also grouped loads with the x blocks staged
  * through LDS; plain or non-temporal block loads) on this operator, `reps`
  * launches each, and keeps the fastest for both directions.  nv: 1, 2 or 4
- * doubles per row. */
+ * doubles per row.  The timed terms are the block kernel alone: on an
+ * assembled receptivity operator whose replicated rows the default tuning
+ * fuses into the term kernel (one launch per term), a pick other than the
+ * default runs them as launches of their own. */
 int hymor_op_autotune(hymor_op *op, int nv, int reps, double *best_ms);
 
 /* enable != 0: the Taylor sweep of hymor_expm_taylor(_dev) and the batched
@@ -250,7 +253,9 @@ int hymor_op_set_graphs(hymor_op *op, int enable);
 int hymor_op_set_batch_kernel(hymor_op *op, int use_mfma);
 
 /* Times the nine batch kernels at width nv (8, 16, 32 or 64) and keeps the
- * fastest; returns its id and ms per launch. */
+ * fastest; returns its id and ms per launch.  As hymor_op_autotune, the
+ * timed terms leave out a receptivity operator's fused replicated rows,
+ * which only batch kernels 6 and 8 at b = 4 take into their launch. */
 int hymor_op_autotune_batch(hymor_op *op, int nv, int reps, int *best_kernel, double *best_ms);
 
 /* Read / force the chosen variant (split 0 k_term | 1 k_term_split |
