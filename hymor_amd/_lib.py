@@ -29,6 +29,9 @@ SIGNATURES = {
     "hymor_last_error": (C.c_char_p, []),
     "hymor_op_create_csc": (C.c_int, [_i64, _p, _p, _p, C.c_int, C.c_int, C.c_double, C.c_int,
                                       _i64, _i64, C.c_int, _pp]),
+    "hymor_op_create_receptivity": (C.c_int, [_i64, _p, _p, _p, C.c_int, C.c_int, C.c_double,
+                                              C.c_int, _i64, _i64, _i64, _i64, _p, C.c_int,
+                                              _pp]),
     "hymor_op_create_bsr": (C.c_int, [_i64, _i64, C.c_int, _i64, _p, _p, _p, _i64, _p, _p, _p,
                                       C.c_int, C.c_double, C.c_int, C.c_int, _pp]),
     "hymor_op_destroy": (C.c_int, [_p]),

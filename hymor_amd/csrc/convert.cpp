@@ -572,4 +572,69 @@ bool compress_row_replicas(HostExtra &x, int64_t n, int64_t c0, int64_t p, int64
     return true;
 }
 
+
+void receptivity_from_parts(int64_t n_l, int64_t p, int64_t nwave, const double *w, double scale,
+                            HostExtra &xf, HostExtra &xa, HostReplica &rf, HostReplica &ra) {
+    if (!xf.is_complex || !xa.is_complex)
+        throw std::invalid_argument("the receptivity operator is complex");
+    if (n_l < 1 || p < 1 || nwave < 1) throw std::invalid_argument("bad receptivity sizes");
+    const int64_t n = n_l + nwave * p;
+    // forward: S's rows n_l .. n_l + p - 1 are the empty lower block; the
+    // listed rows below n_l keep their entries, B's columns renamed to the
+    // virtual columns n + q that read the column sums
+    int64_t keep = 0;
+    while (keep < (int64_t)xf.rows.size() && xf.rows[(size_t)keep] < n_l) ++keep;
+    for (int64_t i = keep; i < (int64_t)xf.rows.size(); ++i)
+        if (xf.ptr[(size_t)i + 1] > xf.ptr[(size_t)i])
+            throw std::invalid_argument(
+                "rows n_l.. of the per-wavenumber system must be empty (its freestream "
+                "diagonal is given by w)");
+    xf = rebuild(xf, keep, [&](int64_t i, auto &&emit) {
+        for (int64_t e = xf.ptr[(size_t)i]; e < xf.ptr[(size_t)i + 1]; ++e) {
+            const int64_t c = xf.col[(size_t)e];
+            emit((int32_t)(c < n_l ? c : n + c - n_l), e);
+        }
+    });
+    rf = HostReplica();
+    rf.kind = 1;
+    rf.c0 = n_l;
+    rf.p = p;
+    rf.nrep = nwave;
+    rf.diag.assign((size_t)(2 * nwave * p), 0.0);
+    for (int64_t k = 0; k < nwave; ++k)
+        for (int64_t q = 0; q < p; ++q) rf.diag[(size_t)(2 * (k * p + q) + 1)] = -w[k] * scale;
+    // adjoint: S''s rows n_l + q (every extra-DOF row is listed) are B''s
+    // rows, kept once as the shared rows of the N copies
+    const int64_t i0 = listed_index(xa, n_l);
+    if (i0 < 0 || (int64_t)xa.rows.size() - i0 != p)
+        throw std::invalid_argument("adjoint extras do not list the freestream rows");
+    ra = HostReplica();
+    ra.kind = 2;
+    ra.c0 = n_l;
+    ra.p = p;
+    ra.nrep = nwave;
+    ra.vptr.assign(1, 0);
+    for (int64_t q = 0; q < p; ++q) {
+        const int64_t i = i0 + q;
+        if (xa.rows[(size_t)i] != n_l + q)
+            throw std::invalid_argument("adjoint extras do not list the freestream rows");
+        for (int64_t e = xa.ptr[(size_t)i]; e < xa.ptr[(size_t)i + 1]; ++e) {
+            if (xa.col[(size_t)e] >= n_l)
+                throw std::invalid_argument(
+                    "columns n_l.. of the per-wavenumber system must be empty below row n_l");
+            ra.vcol.push_back(xa.col[(size_t)e]);
+            ra.vval.push_back(xa.val[(size_t)(2 * e)]);
+            ra.vval.push_back(xa.val[(size_t)(2 * e + 1)]);
+        }
+        ra.vptr.push_back((int64_t)ra.vcol.size());
+    }
+    ra.diag.assign((size_t)(2 * nwave * p), 0.0);
+    for (int64_t k = 0; k < nwave; ++k)
+        for (int64_t q = 0; q < p; ++q) ra.diag[(size_t)(2 * (k * p + q) + 1)] = w[k] * scale;
+    xa = rebuild(xa, i0, [&](int64_t i, auto &&emit) {
+        for (int64_t e = xa.ptr[(size_t)i]; e < xa.ptr[(size_t)i + 1]; ++e)
+            emit(xa.col[(size_t)e], e);
+    });
+}
+
 }  // namespace hymor

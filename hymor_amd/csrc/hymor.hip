@@ -115,6 +115,9 @@ struct hymor_op {
     // freestream blocks kept once (rep.kind != 0 when found), used at
     // kRepMinWidth or more doubles per row
     DevExtra xf_rep, xa_rep;
+    // built from one per-wavenumber system (hymor_op_create_receptivity):
+    // only the compressed extras exist, used at every width
+    bool rep_only = false;
     hipStream_t stream = nullptr;
     int device = 0;
     // workspace: [0] stage (var-major), [1] in0, [2] va, [3] vb, [4] t0, [5] t1, [6] spare
@@ -870,7 +873,7 @@ constexpr int kRepMinWidth = 16;
 
 const DevExtra &extra_of(const hymor_op *op, bool adj, int nv) {
     const DevExtra &r = adj ? op->xa_rep : op->xf_rep;
-    if (nv >= kRepMinWidth && r.rep.kind) return r;
+    if ((nv >= kRepMinWidth || op->rep_only) && r.rep.kind) return r;
     return adj ? op->xa : op->xf;
 }
 
@@ -1388,6 +1391,64 @@ int hymor_op_create_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
     return rc;
 }
 
+int hymor_op_create_receptivity(int64_t n_sys, const int64_t *colptr, const int64_t *rowval,
+                                const void *nzval, int index_base, int dtype, double scale,
+                                int vars_per_cell, int64_t nchi, int64_t neta, int64_t n_l,
+                                int64_t nwave, const double *w, int value_prec, hymor_op **out) {
+    hymor_op *op = nullptr;
+    int rc = guarded([&] {
+        ARG_CHECK(out != nullptr, "out is null");
+        *out = nullptr;
+        ARG_CHECK(colptr && rowval && nzval && w, "null argument");
+        ARG_CHECK(index_base == 0 || index_base == 1, "index_base must be 0 or 1");
+        ARG_CHECK(dtype == HYMOR_F64 || dtype == HYMOR_C128, "bad dtype");
+        ARG_CHECK(vars_per_cell == 4 || vars_per_cell == 8, "vars_per_cell must be 4 or 8");
+        ARG_CHECK(value_prec == HYMOR_PREC_F64 || value_prec == HYMOR_PREC_F32, "bad value_prec");
+        ARG_CHECK(nchi > 0 && neta > 0, "grid must be non-empty");
+        ARG_CHECK(n_l >= vars_per_cell * nchi * neta, "n_l smaller than vars_per_cell*nchi*neta");
+        ARG_CHECK(n_sys > n_l, "the system needs freestream DOFs: n_sys > n_l");
+        ARG_CHECK(nwave >= 1, "nwave must be positive");
+        const int64_t p = n_sys - n_l;
+        ARG_CHECK(nwave <= ((int64_t)INT32_MAX - n_l) / p - 1,
+                  "n_l + (nwave + 1) * p too large for int32 column indices");
+        for (int64_t k = 0; k < nwave; ++k) ARG_CHECK(std::isfinite(w[k]), "w must be finite");
+        // the operator is complex (its -i w diagonal): real values get a zero
+        // imaginary part
+        const int64_t nnz = colptr[n_sys] - index_base;
+        ARG_CHECK(nnz >= 0, "colptr is not monotone");
+        std::vector<double> cval;
+        const double *vals = (const double *)nzval;
+        if (dtype == HYMOR_F64) {
+            cval.assign((size_t)(2 * nnz), 0.0);
+            for (int64_t e = 0; e < nnz; ++e) cval[(size_t)(2 * e)] = vals[e];
+            vals = cval.data();
+        }
+        HostBSR fwd, adj;
+        HostExtra xf, xa;
+        csc_to_blocks(n_sys, colptr, rowval, vals, index_base, true, scale, vars_per_cell,
+                      nchi * neta, fwd, adj, xf, xa);
+        HostReplica rf, ra;
+        receptivity_from_parts(n_l, p, nwave, w, scale, xf, xa, rf, ra);
+        op = new_op(nchi, neta, vars_per_cell);
+        op->n = n_l + nwave * p;
+        op->prec = value_prec;
+        op->is_complex = true;
+        op->rep_only = true;
+        upload_sliced(op, fwd, value_prec, op->fwd);
+        upload_sliced(op, adj, value_prec, op->adj);
+        upload_replicated(op, xf, rf, op->xf_rep);
+        upload_fused(op, xf, rf, op->xf_rep);
+        upload_replicated(op, xa, ra, op->xa_rep);
+        op->has_adj = true;
+        *out = op;
+    });
+    if (rc != HYMOR_OK && op) {
+        release(op);
+        delete op;
+    }
+    return rc;
+}
+
 int hymor_op_create_bsr(int64_t nchi, int64_t neta, int vars_per_cell, int64_t n_extra,
                         const int64_t *brow_ptr, const int32_t *bcol, const double *bval,
                         int64_t nx, const int64_t *xrow, const int64_t *xcol, const void *xval,
@@ -1516,8 +1577,9 @@ int hymor_op_set_value_prec(hymor_op *op, int value_prec) {
 int hymor_op_info(const hymor_op *op, int64_t *out, int nout) {
     return guarded([&] {
         ARG_CHECK(op && out, "null argument");
+        const DevExtra &xe = op->rep_only ? op->xf_rep : op->xf;
         const int64_t v[10] = {op->n, op->b, op->ncell, op->fwd.nnzb, op->fwd.nnzb_pad,
-                               op->xf.nnz, op->xf.nrows, op->is_complex ? 1 : 0, op->prec,
+                               xe.nnz, xe.nrows, op->is_complex ? 1 : 0, op->prec,
                                op->dev_bytes};
         for (int i = 0; i < nout && i < 10; ++i) out[i] = v[i];
     });
@@ -1584,6 +1646,8 @@ int hymor_expm_taylor_batched_coldiag_dev(hymor_op *op, int transpose, int order
         check_batch(op, nrhs, ldx, ldy);
         ARG_CHECK(row0 >= op->nflow && row0 <= row1 && row1 <= op->n,
                   "the diagonal block must lie on extra DOFs: nflow <= row0 <= row1 <= n");
+        ARG_CHECK(!op->rep_only,
+                  "a receptivity operator carries its wavenumbers: no per-column diagonal");
         const int ncomp = 2;
         const int w = pad_width(nrhs * ncomp);
         ensure_ws(op, w);

@@ -98,6 +98,20 @@ bool build_fused_tables(const HostExtra &xc, const HostReplica &rep, int64_t n, 
 bool compress_row_replicas(HostExtra &x, int64_t n, int64_t c0, int64_t p, int64_t nrep,
                            HostReplica &rep);
 
+// The receptivity operator L_ = [L  repeat(B,1,N);  0  kron(diag(-i w), I_p)]
+// (LINEARIZE_L_.jl:58-67) in the compressed form that compress_column_replicas
+// and compress_row_replicas give for its assembled upload, made from the
+// extras of ONE per-wavenumber system S = [L B; 0 0] (size n_l + p, from
+// csc_to_blocks with the same scale, complex) and the N = nwave wavenumbers
+// w, so that no copy of B is ever formed.  In: xf, xa are S's forward and
+// adjoint extras; out: L_'s compressed extras (rows below n_l; B's columns
+// renamed to the virtual columns n + q, n = n_l + N p), rf (kind 1) and ra
+// (kind 2) with c0 = n_l, the period p, N copies and the diagonal
+// -i w_k scale (+i w_k scale in the adjoint).  Throws std::invalid_argument if
+// S's lower rows or its adjoint's freestream columns hold entries.
+void receptivity_from_parts(int64_t n_l, int64_t p, int64_t nwave, const double *w, double scale,
+                            HostExtra &xf, HostExtra &xa, HostReplica &rf, HostReplica &ra);
+
 void csc_to_blocks(int64_t n, const int64_t *colptr, const int64_t *rowval,
                    const double *nzval, int index_base, bool is_complex, double scale,
                    int b, int64_t ncell, HostBSR &fwd, HostBSR &adj,

@@ -452,35 +452,18 @@ def TRANSIENT_GROWTH_DOWNSTREAM(A, s, n_modes, T, *, tol=1e-4, maxiter=None, ncv
                 d.close()
 
 
-def _factored_construct_c(A_, C_, s, w_infty, dt, n_t, n_set_0):
-    """CONSTRUCT_C on the extended operator through its factored form: the N
-    wavenumber systems [L B; 0 -i w_k I] in one batched sweep per direction
-    (hymor_amd.receptivity), the Chu weight on the flow part.  Returns
-    (matvec, close)."""
-    from .receptivity import FactoredReceptivity, split_extended
+def _upload_extended_from_parts(A_, s, w_infty, dt):
+    """The extended operator A_ * dt on the device from its parts: A_ is
+    checked to have the reference structure (split_extended) and rebuilt by
+    hymor_op_create_receptivity from [L B; 0 0] and the wavenumbers, with B
+    kept once."""
+    from .receptivity import receptivity_operator, split_extended
     nx, ny = _grid(s)
     N = len(w_infty)
     p = B * nx
     n_L = A_.shape[0] - N * p
     L, Bm = split_extended(A_, n_L, p, w_infty)
-    C_ = sp.csr_matrix(C_)
-    tail = C_[n_L:, :]
-    if tail.nnz and abs(tail).max() != 0 or (C_[:, n_L:].nnz and abs(C_[:, n_L:]).max() != 0):
-        raise ValueError("the Chu weight must vanish on the freestream DOFs (M_ = blkdiag(M, 0))")
-    C_L = C_[:n_L, :n_L].astype(np.complex128)
-    fr = FactoredReceptivity(L, Bm, w_infty, nchi=nx, neta=ny, vars_per_cell=B, scale=dt)
-
-    def matvec(x):
-        x = np.array(x, dtype=np.complex128)
-        x[:n_set_0] = 0.0                                     # CONSTRUCT_C.jl:39
-        y = fr.expm(x, order=ORDER, nsub=n_t, form="normalised")
-        z = np.zeros_like(y)
-        z[:n_L] = C_L @ y[:n_L]
-        out = fr.expm(z, order=ORDER, nsub=n_t, form="normalised", transpose=True)
-        out[:n_set_0] = 0.0                                   # CONSTRUCT_C.jl:55
-        return out
-
-    return matvec, fr.close
+    return receptivity_operator(L, Bm, w_infty, nchi=nx, neta=ny, vars_per_cell=B, scale=dt)
 
 
 def FREESTREAM_RECEPTIVITY(A_, s, n_modes, T, w_infty, *, tol=1e-4, maxiter=None, ncv=None,
@@ -491,9 +474,9 @@ def FREESTREAM_RECEPTIVITY(A_, s, n_modes, T, w_infty, *, tol=1e-4, maxiter=None
     Hermitian generalized ARPACK on CONSTRUCT_C with the first N_set_0 =
     4*Nchi*Neta entries zeroed; gains are real(lambda)/2.
 
-    factored=True applies CONSTRUCT_C through the factored form of A_ (one
-    upload of [L B; 0 0], the N wavenumbers as columns of one batched sweep,
-    B never repeated), with ARPACK as the eigensolver.
+    factored=True uploads A_ from its parts (one per-wavenumber system
+    [L B; 0 0] and the wavenumbers, hymor_op_create_receptivity): B is never
+    repeated on the device, and each action costs one flow SpMV per term.
     """
     k = n_modes
     s = CFL_TIMESTEP(s)
@@ -508,19 +491,11 @@ def FREESTREAM_RECEPTIVITY(A_, s, n_modes, T, w_infty, *, tol=1e-4, maxiter=None
     n_set_0 = 4 * nx * ny
     n = A_.shape[0]
     devA = devC = None
-    close_fr = None
     try:
         if factored:
-            if krylov != "arpack":
-                raise ValueError("the factored form runs with ARPACK")
-            matvec, close_fr = _factored_construct_c(A_, C_, s, w_infty, dt, n_t, n_set_0)
-            op = spla.LinearOperator((n, n), dtype=np.complex128, matvec=matvec)
-            vals, vecs = spla.eigsh(op, k=k, M=D_.astype(np.complex128), which="LM", tol=tol,
-                                    maxiter=maxiter or 10 * k, ncv=ncv, v0=v0)
-            D = np.real(vals) / 2
-            order = np.argsort(-np.abs(D), kind="stable")
-            return vecs[:, order], D[order], T_opt
-        devA = upload(sp.csc_matrix(A_, dtype=np.complex128), s, scale=dt)
+            devA = _upload_extended_from_parts(A_, s, w_infty, dt)
+        else:
+            devA = upload(sp.csc_matrix(A_, dtype=np.complex128), s, scale=dt)
         devC = upload(C_, s)
         if krylov == "device":
             vals, vecs = _device_generalized(devA, devC, D_, s, n, k, n_t, n_set_0, True, tol,
@@ -541,8 +516,6 @@ def FREESTREAM_RECEPTIVITY(A_, s, n_modes, T, w_infty, *, tol=1e-4, maxiter=None
         for d in (devA, devC):
             if d is not None:
                 d.close()
-        if close_fr is not None:
-            close_fr()
 
 
 # ====================================================================== energy tracking

@@ -84,6 +84,28 @@ int hymor_op_create_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
                         int vars_per_cell, int64_t nchi, int64_t neta,
                         int value_prec, hymor_op **out);
 
+/* Build the extended receptivity operator
+ *     L_ = [L  repeat(B,1,N);  0  kron(diag(-i*w), I_p)]
+ * of LINEARIZE_L_.jl:58-67 -- what `CuSparseMatrixCSC(A_ * dt)` uploads in
+ * FREESTREAM_RECEPTIVITY.jl:83-85 -- from ONE per-wavenumber system and the
+ * wavenumbers, with B kept once on the device at every vector width: the
+ * forward operator applies B to the column sums of the N freestream blocks,
+ * the adjoint evaluates B' once for the N blocks (the same arithmetic the
+ * library uses for an assembled L_ uploaded by hymor_op_create_csc).
+ *   n_sys, colptr, rowval, nzval, index_base, dtype: the CSC of
+ *       S = [L B; 0 0] (n_sys = n_l + p, reference var-major order, flow
+ *       DOFs then shock DOFs, then the p freestream DOFs of one wavenumber);
+ *       rows n_l.. must be empty.  Real (HYMOR_F64) or complex values.
+ *   n_l: flow plus shock DOFs; nwave = N, w[N]: the wavenumbers.
+ *   scale multiplies every value and the -i*w diagonal (the reference's
+ *   A_*dt).  The handle is complex and of size n_l + N p; it serves every
+ *   entry point that takes an operator, except the per-column diagonal of
+ *   hymor_expm_taylor_batched_coldiag_dev. */
+int hymor_op_create_receptivity(int64_t n_sys, const int64_t *colptr, const int64_t *rowval,
+                                const void *nzval, int index_base, int dtype, double scale,
+                                int vars_per_cell, int64_t nchi, int64_t neta, int64_t n_l,
+                                int64_t nwave, const double *w, int value_prec, hymor_op **out);
+
 /* Build a device operator from cell-major block-CSR arrays (0-based), the
  * library's native layout: brow_ptr[ncell+1], bcol[nnzb] (int32),
  * bval[nnzb*b*b] (row-major blocks, real), plus nx extra entries in

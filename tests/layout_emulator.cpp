@@ -1239,3 +1239,54 @@ extern "C" int emu_spmm_pair_csc(int64_t n, const int64_t *colptr, const int64_t
     }
     return g_oob;
 }
+
+// The receptivity operator built from its parts (receptivity_from_parts on
+// one per-wavenumber system S = [L B; 0 0] and the wavenumbers w) against
+// the compressed form the library finds in the assembled L_ (both CSCs
+// 1-based, complex, var-major): returns 0 when the forward and adjoint extra
+// CSRs, both replica descriptions and the fused Horner tables are equal
+// entry for entry, else the number of the first part that differs (1
+// forward extras, 2 forward replica, 3 adjoint extras, 4 adjoint replica, 5
+// fused tables, 6 the assembled operator was not compressed, 7 the block
+// parts differ).
+namespace {
+bool same_extra(const HostExtra &a, const HostExtra &b) {
+    return a.is_complex == b.is_complex && a.rows == b.rows && a.ptr == b.ptr &&
+           a.col == b.col && a.val == b.val;
+}
+bool same_rep(const HostReplica &a, const HostReplica &b) {
+    return a.kind == b.kind && a.c0 == b.c0 && a.p == b.p && a.nrep == b.nrep &&
+           a.diag == b.diag && a.vptr == b.vptr && a.vcol == b.vcol && a.vval == b.vval;
+}
+}  // namespace
+
+extern "C" int emu_receptivity_parts_equal(int64_t n_ext, const int64_t *colptr,
+                                           const int64_t *rowval, const double *nzval,
+                                           int64_t n_sys, const int64_t *s_colptr,
+                                           const int64_t *s_rowval, const double *s_nzval,
+                                           int b, int64_t ncell, int64_t n_l, int64_t nwave,
+                                           const double *w, double scale) {
+    HostBSR fwd, adj, sfwd, sadj;
+    HostExtra xf, xa, sxf, sxa;
+    csc_to_blocks(n_ext, colptr, rowval, nzval, 1, true, scale, b, ncell, fwd, adj, xf, xa);
+    csc_to_blocks(n_sys, s_colptr, s_rowval, s_nzval, 1, true, scale, b, ncell, sfwd, sadj, sxf,
+                  sxa);
+    if (fwd.ptr != sfwd.ptr || fwd.col != sfwd.col || fwd.val != sfwd.val ||
+        adj.ptr != sadj.ptr || adj.col != sadj.col || adj.val != sadj.val)
+        return 7;
+    HostReplica rf, ra, pf, pa;
+    if (!compress_column_replicas(xf, n_ext, rf)) return 6;
+    if (!compress_row_replicas(xa, n_ext, rf.c0, rf.p, rf.nrep, ra)) return 6;
+    receptivity_from_parts(n_l, n_sys - n_l, nwave, w, scale, sxf, sxa, pf, pa);
+    if (!same_extra(xf, sxf)) return 1;
+    if (!same_rep(rf, pf)) return 2;
+    if (!same_extra(xa, sxa)) return 3;
+    if (!same_rep(ra, pa)) return 4;
+    HostFusedRep fa, fp;
+    const bool ba = build_fused_tables(xf, rf, n_ext, (int64_t)b * ncell, fa);
+    const bool bp = build_fused_tables(sxf, pf, n_ext, (int64_t)b * ncell, fp);
+    if (ba != bp || fa.slice_tab != fp.slice_tab || fa.lane_ptr != fp.lane_ptr ||
+        fa.ecol != fp.ecol || fa.eval != fp.eval)
+        return 5;
+    return 0;
+}

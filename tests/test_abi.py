@@ -66,3 +66,29 @@ def test_new_entry_points_check_their_arguments():
     assert lib.hymor_bench_stream_mix(1024, 0, 1, ctypes.byref(ms), ctypes.byref(ms)) == -1
     assert b"16 MiB" in lib.hymor_last_error()
     assert lib.hymor_bench_term_kernel(None, 0, 1, 2, 1, ctypes.byref(ms), ctypes.byref(ms)) == -1
+
+
+def test_receptivity_constructor_checks_its_arguments():
+    """hymor_op_create_receptivity rejects a system without freestream DOFs,
+    a wavenumber count below one and non-finite wavenumbers before it
+    touches a device."""
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    n = 4 * 2 * 2 + 8
+    colptr = np.ones(n + 1, dtype=np.int64)
+    rowval = np.zeros(0, dtype=np.int64)
+    nz = np.zeros(0)
+    w = np.array([1.0, np.nan])
+
+    def create(n_l, nwave, wv):
+        return lib.hymor_op_create_receptivity(n, _lib.ptr(colptr), _lib.ptr(rowval),
+                                               _lib.ptr(nz), 1, 0, 1.0, 4, 2, 2, n_l, nwave,
+                                               _lib.ptr(wv), 0, ctypes.byref(h))
+
+    assert create(n, 2, w) == -1
+    assert b"freestream DOFs" in lib.hymor_last_error()
+    assert create(16, 0, w) == -1
+    assert b"nwave" in lib.hymor_last_error()
+    assert create(16, 2, w) == -1
+    assert b"finite" in lib.hymor_last_error()
+    assert not h.value

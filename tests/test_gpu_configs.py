@@ -8,8 +8,9 @@ oracle SpMVs.
   this size are in test_gpu_fullsize.py.)
 * C4: 500 x 250 cells, b = 4, 32 freestream wavenumbers
   w_k = 0.1 * 100^((k-1)/31) (N_ext = 564 000 complex): the factored
-  receptivity action equals the assembled extended operator's action on the
-  same vector, forward and adjoint, and one assembled SpMV equals the
+  receptivity actions (the extended operator from its parts, and the 32
+  per-wavenumber systems) equal the assembled extended operator's action on
+  the same vector, forward and adjoint, and one assembled SpMV equals the
   oracle's; the Horner terms with the replicated rows fused into one launch
   (one vector and 16-vector batches) equal the powers form's separate
   launches, and the fused adjoint satisfies the dot-product identity.
@@ -73,13 +74,15 @@ def test_c4_factored_receptivity_full_size(orc, c4_extended):
         ref_t = dev.expm_taylor(x, order=5, nsub=2, dt=1.0, transpose=True)
         Ao = orc.CSC.from_scipy(A_)
         assert relerr(dev.spmv(x) / dt, orc.csc_mul(Ao, x)) <= 1e-13
-    fr = FactoredReceptivity(L, B, w, nchi=base.nchi, neta=base.neta, vars_per_cell=4,
-                             scale=dt)
-    try:
-        assert relerr(fr.expm(x, order=5, nsub=2), ref) <= 1e-12
-        assert relerr(fr.expm(x, order=5, nsub=2, transpose=True), ref_t) <= 1e-12
-    finally:
-        fr.close()
+    for per_wavenumber in (False, True):
+        fr = FactoredReceptivity(L, B, w, nchi=base.nchi, neta=base.neta, vars_per_cell=4,
+                                 scale=dt, per_wavenumber=per_wavenumber)
+        try:
+            assert relerr(fr.expm(x, order=5, nsub=2), ref) <= 1e-12, per_wavenumber
+            assert relerr(fr.expm(x, order=5, nsub=2, transpose=True), ref_t) <= 1e-12, \
+                per_wavenumber
+        finally:
+            fr.close()
 
 
 def test_c4_fused_terms_full_size(c4_extended):

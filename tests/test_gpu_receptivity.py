@@ -1,7 +1,8 @@
-"""Factored receptivity action (N per-wavenumber systems in one batched
-sweep) against the oracle's POWER_METHOD! restatement on the reference's
-extended operator L_ = [L repeat(B,1,N); 0 kron(diag(-i w), I)], forward and
-adjoint, at FP64 tolerance (1e-12)."""
+"""Receptivity actions against the oracle's POWER_METHOD! restatement on the
+reference's extended operator L_ = [L repeat(B,1,N); 0 kron(diag(-i w), I)],
+forward and adjoint, at FP64 tolerance (1e-12): the extended operator built
+from its parts (B kept once), the N per-wavenumber systems in one batched
+sweep, and the assembled upload with its copies of B found and kept once."""
 import sys
 
 import numpy as np
@@ -9,7 +10,8 @@ import pytest
 import scipy.sparse as sp
 
 from hymor_amd import operators as ops
-from hymor_amd.receptivity import FactoredReceptivity, split_extended
+from hymor_amd.receptivity import (FactoredReceptivity, per_omega_system, receptivity_operator,
+                                   split_extended)
 
 def say(msg):
     print(msg, flush=True)
@@ -21,10 +23,11 @@ def relerr(a, b):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("per_wavenumber", [False, True])
 @pytest.mark.parametrize("N,shock", [(3, True), (8, False), (32, True)])
-def test_factored_matches_extended_operator(orc, N, shock):
+def test_factored_matches_extended_operator(orc, N, shock, per_wavenumber):
     nchi, neta = 10, 7
-    say(f"[factored receptivity] N={N} shock={shock}")
+    say(f"[factored receptivity] N={N} shock={shock} per_wavenumber={per_wavenumber}")
     base = ops.stencil_operator(nchi, neta, 4, seed=40, n_shock=nchi if shock else 0,
                                 row_sum=2.0)
     w = np.linspace(0.0, 6.0, N)
@@ -36,7 +39,8 @@ def test_factored_matches_extended_operator(orc, N, shock):
     x = r.standard_normal(A_.shape[0]) + 1j * r.standard_normal(A_.shape[0])
     Ao = orc.CSC.from_scipy(A_)
     dt, nsub = 0.05, 5
-    fr = FactoredReceptivity(L, B, w, nchi=nchi, neta=neta, vars_per_cell=4, scale=dt)
+    fr = FactoredReceptivity(L, B, w, nchi=nchi, neta=neta, vars_per_cell=4, scale=dt,
+                             per_wavenumber=per_wavenumber)
     try:
         got = fr.expm(x, order=5, nsub=nsub)
         ref = orc.power_method(Ao.scaled(dt), x, nsub)
@@ -107,7 +111,8 @@ def test_factorisation_on_host(orc, shock):
     r = np.random.default_rng(3)
     x = r.standard_normal(A_.shape[0]) + 1j * r.standard_normal(A_.shape[0])
     Ao = orc.CSC.from_scipy(A_)
-    fr = _HostSystems(L, B, w, nchi=nchi, neta=neta, vars_per_cell=4, scale=0.05)
+    fr = _HostSystems(L, B, w, nchi=nchi, neta=neta, vars_per_cell=4, scale=0.05,
+                      per_wavenumber=True)
     got = fr.expm(x, order=5, nsub=3)
     assert relerr(got, orc.power_method(Ao.scaled(0.05), x, 3)) <= 1e-13
     got_t = fr.expm(x, order=5, nsub=3, transpose=True)
@@ -145,8 +150,10 @@ def test_factored_real_upload_equals_complex_upload(orc, N, shock):
     x = r.standard_normal(A_.shape[0]) + 1j * r.standard_normal(A_.shape[0])
     Ao = orc.CSC.from_scipy(A_)
     dt, nsub = 0.05, 4
-    fc = FactoredReceptivity(L, B, w, nchi=nchi, neta=neta, vars_per_cell=4, scale=dt)
-    fr = _RealValues(L, B, w, nchi=nchi, neta=neta, vars_per_cell=4, scale=dt)
+    fc = FactoredReceptivity(L, B, w, nchi=nchi, neta=neta, vars_per_cell=4, scale=dt,
+                             per_wavenumber=True)
+    fr = _RealValues(L, B, w, nchi=nchi, neta=neta, vars_per_cell=4, scale=dt,
+                     per_wavenumber=True)
     try:
         for tr in (False, True):
             got_r = fr.expm(x, order=5, nsub=nsub, transpose=tr)
@@ -333,3 +340,105 @@ def test_adjoint_fused_term(orc, nrhs):
             two = dev.expm_taylor(X[:, 0], order=order, nsub=nsub, dt=1.0, form="horner",
                                   transpose=True)
             assert relerr(fused, two) <= 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,shock,prec", [(3, True, "f64"), (8, False, "f64"), (32, False, "f64"),
+                                          (32, True, "f64"), (12, False, "f32")])
+def test_receptivity_operator_from_parts(orc, N, shock, prec):
+    """The extended operator built from [L B; 0 0] and the wavenumbers
+    (hymor_op_create_receptivity, B kept once at every width) against the
+    oracle's POWER_METHOD! on the assembled L_ and its adjoint: SpMV (1e-14),
+    one complex vector in the powers, normalised and Horner forms, and 2-32
+    complex vectors per sweep in the Horner and powers forms (1e-12 with FP64
+    values, 2e-6 with FP32), each equal to the assembled upload (1e-13 FP64);
+    three graph replays give the same bits; a real-valued upload of the
+    system gives the same bits as a complex one."""
+    from hymor_amd.core import DeviceOperator
+    nchi, neta = 9, 6
+    say(f"[receptivity operator from parts] N={N} shock={shock} values={prec}")
+    base = ops.stencil_operator(nchi, neta, 4, seed=49, n_shock=nchi if shock else 0,
+                                row_sum=2.0)
+    w = np.linspace(0.0, 6.0, N)
+    L_ = ops.receptivity_extension(base, w)
+    A_ = L_.to_scipy_varmajor()
+    Ao = orc.CSC.from_scipy(A_)
+    Aot = orc.csc_adjoint(Ao)
+    L, B = split_extended(A_, base.n, 4 * nchi, w)
+    r = np.random.default_rng(100 + N)
+    n = A_.shape[0]
+    X = r.standard_normal((n, 16)) + 1j * r.standard_normal((n, 16))
+    dt, nsub, order = 0.05, 3, 5
+    tol = 1e-12 if prec == "f64" else 2e-6
+    same = 1e-13 if prec == "f64" else 1e-6
+    kw = dict(nchi=nchi, neta=neta, vars_per_cell=4, scale=dt, value_prec=prec)
+    with receptivity_operator(L, B, w, **kw) as dev, \
+            DeviceOperator.from_blocks(L_, scale=dt, value_prec=prec) as asm:
+        assert dev.n == n and dev.is_complex
+        # B once: the device holds one copy of its entries, the assembly N
+        assert dev.nnz_extra < asm.nnz_extra or N == 1
+        for tr, M in ((False, Ao), (True, Aot)):
+            y = dev.spmv(X[:, 0], transpose=tr)
+            if prec == "f64":
+                assert relerr(y / dt, orc.csc_mul(M, X[:, 0])) <= 1e-14, tr
+            for form in ("powers", "normalised", "horner"):
+                got = dev.expm_taylor(X[:, 0], order=order, nsub=nsub, dt=1.0, form=form,
+                                      transpose=tr)
+                ref = orc.power_method(M.scaled(dt), X[:, 0], nsub, order)
+                assert relerr(got, ref) <= tol, (tr, form)
+                a = asm.expm_taylor(X[:, 0], order=order, nsub=nsub, dt=1.0, form=form,
+                                    transpose=tr)
+                assert relerr(got, a) <= same, (tr, form)
+            for m in (2, 4, 8, 16):
+                for form in ("horner", "powers"):
+                    Y = dev.expm_taylor_batched(X[:, :m], order=order, nsub=nsub, dt=1.0,
+                                                form=form, transpose=tr)
+                    Ya = asm.expm_taylor_batched(X[:, :m], order=order, nsub=nsub, dt=1.0,
+                                                 form=form, transpose=tr)
+                    for j in range(m):
+                        ref = orc.power_method(M.scaled(dt), X[:, j], nsub, order)
+                        assert relerr(Y[:, j], ref) <= tol, (tr, m, form, j)
+                        assert relerr(Y[:, j], Ya[:, j]) <= same, (tr, m, form, j)
+        base_y = dev.expm_taylor(X[:, 1], order=order, nsub=nsub, dt=1.0, form="horner")
+        dev.set_graphs(True)
+        for _ in range(3):
+            assert np.array_equal(
+                dev.expm_taylor(X[:, 1], order=order, nsub=nsub, dt=1.0, form="horner"), base_y)
+        dev.set_graphs(False)
+    if prec == "f64":
+        with receptivity_operator(sp.csc_matrix(L.real), sp.csc_matrix(B.real), w,
+                                  **kw) as real:
+            assert not np.iscomplexobj(sp.csc_matrix(L.real).data)
+            for tr in (False, True):
+                with receptivity_operator(L, B, w, **kw) as cplx:
+                    want = cplx.expm_taylor_batched(X[:, :4], order=order, nsub=nsub, dt=1.0,
+                                                    form="horner", transpose=tr)
+                got = real.expm_taylor_batched(X[:, :4], order=order, nsub=nsub, dt=1.0,
+                                               form="horner", transpose=tr)
+                assert np.array_equal(got, want), tr
+
+
+@pytest.mark.gpu
+def test_receptivity_operator_rejects_a_per_column_diagonal():
+    """The operator from parts carries its wavenumbers: the factored entry
+    point's per-column diagonal is refused with a clear error."""
+    from hymor_amd import _lib
+    from hymor_amd.core import DeviceMatrix
+    nchi, neta = 5, 4
+    base = ops.stencil_operator(nchi, neta, 4, seed=50)
+    w = [0.5, 1.5]
+    A_ = ops.receptivity_extension(base, w).to_scipy_varmajor()
+    L, B = split_extended(A_, base.n, 4 * nchi, w)
+    with receptivity_operator(L, B, w, nchi=nchi, neta=neta) as dev:
+        X = DeviceMatrix.from_host(np.zeros((dev.n, 2), dtype=np.complex128))
+        Y = DeviceMatrix(dev.n, 2, np.complex128)
+        d = np.zeros(4)
+        try:
+            rc = _lib.load().hymor_expm_taylor_batched_coldiag_dev(
+                dev.handle, 0, 5, 1, 1.0, 1, 2, d.ctypes.data, base.n, dev.n, X.ptr, dev.n,
+                Y.ptr, dev.n)
+            assert rc == _lib.ERR_ARG
+            assert b"carries its wavenumbers" in _lib.load().hymor_last_error()
+        finally:
+            X.free()
+            Y.free()

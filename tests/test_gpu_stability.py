@@ -199,10 +199,12 @@ def test_freestream_receptivity_vs_dense():
     assert np.all(np.abs(G - ref) <= 1e-8 * np.abs(ref)), (G, ref)
 
 
-def test_freestream_receptivity_factored_matches_assembled():
-    """FREESTREAM_RECEPTIVITY through the factored extended operator (one
-    batched sweep over the N wavenumber systems per direction) gives the
-    gains of the assembled path and of the dense reference."""
+@pytest.mark.parametrize("krylov", ["arpack", "device"])
+def test_freestream_receptivity_factored_matches_assembled(krylov):
+    """FREESTREAM_RECEPTIVITY with the extended operator uploaded from its
+    parts ([L B; 0 0] and the wavenumbers, B kept once) gives the gains of
+    the assembled path and of the dense reference, with ARPACK and with the
+    device Krylov solver."""
     nchi, neta = 6, 5
     w = [0.0, 1.0, 2.5, 4.0]
     say("[FREESTREAM_RECEPTIVITY factored] vs assembled and dense")
@@ -211,7 +213,12 @@ def test_freestream_receptivity_factored_matches_assembled():
                                 row_sum=0.5 / s["time_integration"]["dt"])
     A = ops.receptivity_extension(base, w).to_scipy_varmajor()
     v0 = np.ones(A.shape[0], dtype=complex)
-    _, Gf, _ = st.FREESTREAM_RECEPTIVITY(A, s, 3, 1.0, w, tol=1e-12, ncv=24, v0=v0, factored=True)
+    if krylov == "device":
+        _, Gf, _ = st.FREESTREAM_RECEPTIVITY(A, s, 3, 1.0, w, tol=1e-12, ncv=24,
+                                             krylov="device", factored=True)
+    else:
+        _, Gf, _ = st.FREESTREAM_RECEPTIVITY(A, s, 3, 1.0, w, tol=1e-12, ncv=24, v0=v0,
+                                             factored=True)
     _, Ga, _ = st.FREESTREAM_RECEPTIVITY(A, s, 3, 1.0, w, tol=1e-12, ncv=24, v0=v0)
     assert np.all(np.abs(Gf - Ga) <= 1e-9 * np.abs(Ga)), (Gf, Ga)
     n_t = int(round(1.0 / s["time_integration"]["dt"]))

@@ -10,6 +10,7 @@ import subprocess
 from pathlib import Path
 
 import numpy as np
+import scipy.sparse as sp
 import pytest
 
 from hymor_amd import operators as ops
@@ -136,6 +137,43 @@ def test_emulated_single_wavenumber_is_not_compressed(emu):
     y, oob, info = _run_rep(emu, L_, False, x)
     assert oob == 0 and info[0] == 0
     assert np.allclose(y, L_.to_scipy_varmajor() @ x, rtol=1e-13, atol=1e-13)
+
+
+def _csc1(A, dtype=np.complex128):
+    A = sp.csc_matrix(A, dtype=dtype)
+    A.sort_indices()
+    return (np.ascontiguousarray(A.indptr + 1, dtype=np.int64),
+            np.ascontiguousarray(A.indices + 1, dtype=np.int64),
+            np.ascontiguousarray(A.data, dtype=dtype))
+
+
+@pytest.mark.parametrize("nchi,neta,shock,N", [(6, 4, True, 3), (5, 3, False, 2),
+                                               (8, 5, False, 32), (7, 4, True, 9)])
+def test_receptivity_from_parts_equals_compressed_assembly(emu, nchi, neta, shock, N):
+    """hymor_op_create_receptivity's host conversion (receptivity_from_parts,
+    from ONE per-wavenumber system [L B; 0 0] and the wavenumbers) yields
+    exactly the tables the library derives from the reference's assembled
+    L_ = [L repeat(B,1,N); 0 kron(diag(-i w), I)]: block parts, forward and
+    adjoint extra CSRs, both replica descriptions (period, copies, -i w
+    diagonal, shared rows) and the fused Horner tables, entry for entry.  The
+    assembled form's device arithmetic is pinned against the oracle by the
+    GPU tests, so the form from parts runs the same arithmetic."""
+    from hymor_amd.receptivity import per_omega_system, split_extended
+    base = ops.stencil_operator(nchi, neta, 4, seed=70 + N, n_shock=nchi if shock else 0)
+    w = np.linspace(0.0, 6.0, N)
+    A_ = ops.receptivity_extension(base, w, seed=N).to_scipy_varmajor()
+    n_L, p = base.n, 4 * nchi
+    L, B = split_extended(A_, n_L, p, w)
+    S = per_omega_system(L, B)
+    a = _csc1(A_)
+    s_ = _csc1(S)
+    emu.emu_receptivity_parts_equal.restype = C.c_int
+    rc = emu.emu_receptivity_parts_equal(
+        C.c_int64(A_.shape[0]), *(C.c_void_p(v.ctypes.data) for v in a),
+        C.c_int64(S.shape[0]), *(C.c_void_p(v.ctypes.data) for v in s_),
+        C.c_int(4), C.c_int64(base.ncell), C.c_int64(n_L), C.c_int64(N),
+        C.c_void_p(np.ascontiguousarray(w).ctypes.data), C.c_double(0.05))
+    assert rc == 0
 
 
 def _run_fused(emu, A, b, ncell, X2, order, nsub, dt, nv=2):
