@@ -637,4 +637,37 @@ void receptivity_from_parts(int64_t n_l, int64_t p, int64_t nwave, const double 
     });
 }
 
+
+std::vector<int64_t> xcd_balanced_order(const std::vector<int64_t> &len) {
+    const int64_t m = (int64_t)len.size();
+    std::vector<int64_t> order((size_t)m);
+    std::iota(order.begin(), order.end(), 0);
+    if (m < 16) return order;
+    const auto mm = std::minmax_element(len.begin(), len.end());
+    if (*mm.second <= *mm.first + *mm.first / 8) return order;   // contiguous split is balanced
+    // xcd_tile's ranges: the first m % 8 hold m / 8 + 1 rows, the rest m / 8
+    const int64_t per = m / 8, rem = m % 8;
+    std::vector<int64_t> by_len = order;
+    std::stable_sort(by_len.begin(), by_len.end(),
+                     [&](int64_t a, int64_t b) { return len[(size_t)a] > len[(size_t)b]; });
+    std::vector<std::vector<int64_t>> bin(8);
+    int64_t load[8] = {};
+    for (const int64_t r : by_len) {
+        int best = -1;
+        for (int k = 0; k < 8; ++k) {
+            const int64_t cap = per + (k < rem ? 1 : 0);
+            if ((int64_t)bin[(size_t)k].size() < cap && (best < 0 || load[k] < load[best]))
+                best = k;
+        }
+        bin[(size_t)best].push_back(r);
+        load[best] += len[(size_t)r];
+    }
+    int64_t o = 0;
+    for (auto &bk : bin) {
+        std::sort(bk.begin(), bk.end());
+        for (const int64_t r : bk) order[(size_t)o++] = r;
+    }
+    return order;
+}
+
 }  // namespace hymor

@@ -318,6 +318,15 @@ void upload_extra(hymor_op *op, const HostExtra &h, DevExtra &d) {
     for (int pass = 0; pass < 2; ++pass)
         for (int64_t r = 0; r < nr; ++r)
             if ((h.ptr[r + 1] - h.ptr[r] > kShortRow) == (pass == 0)) order.push_back(r);
+    {
+        // the long rows spread over the XCDs by entry count
+        std::vector<int64_t> lens;
+        for (const int64_t r : order)
+            if (h.ptr[r + 1] - h.ptr[r] > kShortRow) lens.push_back(h.ptr[r + 1] - h.ptr[r]);
+        const std::vector<int64_t> perm = xcd_balanced_order(lens);
+        std::vector<int64_t> head(order.begin(), order.begin() + (long)lens.size());
+        for (size_t i = 0; i < perm.size(); ++i) order[i] = head[(size_t)perm[i]];
+    }
     std::vector<int64_t> rows((size_t)nr), ptr((size_t)nr + 1, 0);
     std::vector<int32_t> col(h.col.size());
     std::vector<double> val(h.val.size());

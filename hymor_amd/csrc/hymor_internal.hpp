@@ -112,6 +112,16 @@ bool compress_row_replicas(HostExtra &x, int64_t n, int64_t c0, int64_t p, int64
 void receptivity_from_parts(int64_t n_l, int64_t p, int64_t nwave, const double *w, double scale,
                             HostExtra &xf, HostExtra &xa, HostReplica &rf, HostReplica &ra);
 
+// Order of the m long extra rows (one k_extra workgroup each; len[i]
+// entries) such that the 8 contiguous ranges xcd_tile hands to the 8 XCDs
+// (sizes m / 8 + 1 for the first m % 8, m / 8 for the rest) hold about the
+// same number of entries: when the lengths differ by more than an eighth,
+// the rows are dealt longest first to the range with the fewest entries that
+// has room, and keep their relative order inside a range; otherwise (and
+// below 16 rows) the order is unchanged, so neighbouring rows stay on one
+// XCD.  Returns new position -> old index.
+std::vector<int64_t> xcd_balanced_order(const std::vector<int64_t> &len);
+
 void csc_to_blocks(int64_t n, const int64_t *colptr, const int64_t *rowval,
                    const double *nzval, int index_base, bool is_complex, double scale,
                    int b, int64_t ncell, HostBSR &fwd, HostBSR &adj,

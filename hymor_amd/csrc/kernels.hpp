@@ -218,21 +218,26 @@ struct FusedRepDev {
 };
 
 // x value (row c, double v) as the extra rows read it: with replicated
-// blocks kept once (REP) a virtual column reads their column sum
-template <bool REP>
+// blocks kept once (REP) a virtual column reads their column sum.  The
+// address is selected, not branched on, so that a lane's loads of several
+// entries can be in flight together; TILE = false (no tile-layout rows in x,
+// tile_b == 0) leaves out buf_pos's test.
+template <bool REP, bool TILE = true>
 __device__ __forceinline__ double xread(const double *__restrict__ x, const ReplicaDev &R,
                                         int64_t c, int v, int nv, int tile_b, int64_t nflow) {
-    if constexpr (REP)
-        return c >= R.vbase ? R.sum[(c - R.vbase) * nv + v] : x[buf_pos(c, v, nv, tile_b, nflow)];
-    else
-        return x[buf_pos(c, v, nv, tile_b, nflow)];
+    const double *px = x + (TILE ? buf_pos(c, v, nv, tile_b, nflow) : c * nv + v);
+    if constexpr (REP) {
+        const double *ps = R.sum + (c - R.vbase) * nv + v;
+        px = c >= R.vbase ? ps : px;
+    }
+    return *px;
 }
 
 // acc += row entries [e, e1) times x, for a lane owning NC doubles of the
 // row from v0: the loads of G entries are issued before their products, so
 // a short row costs ceil(len / G) memory round trips instead of len; the
 // products are added in entry order (the same sum as one entry at a time).
-template <int NV, bool CPLX, int G, bool REP>
+template <int NV, bool CPLX, int G, bool REP, bool TILE = true>
 __device__ __forceinline__ void row_dot(double *acc, int64_t e, int64_t e1,
                                         const int32_t *__restrict__ col,
                                         const double *__restrict__ val,
@@ -241,19 +246,21 @@ __device__ __forceinline__ void row_dot(double *acc, int64_t e, int64_t e1,
     constexpr int NC = CPLX ? 2 : 1;
     for (; e < e1; e += G) {
         double a[G][NC], xv[G][NC];
+        int32_t cg[G];
+        // columns and values first, then the x entries (one wait for the
+        // columns); past the row's end the loads repeat entry e
 #pragma unroll
         for (int g = 0; g < G; ++g) {
+            const int64_t eg = e + g < e1 ? e + g : e;
+            cg[g] = col[eg];
 #pragma unroll
-            for (int i = 0; i < NC; ++i) a[g][i] = xv[g][i] = 0.0;
-            if (e + g < e1) {       // no loads past the row's end
-                const int64_t c = col[e + g];
-#pragma unroll
-                for (int i = 0; i < NC; ++i) {
-                    a[g][i] = val[(e + g) * NC + i];
-                    xv[g][i] = xread<REP>(x, R, c, v0 + i, NV, tile_b, nflow);
-                }
-            }
+            for (int i = 0; i < NC; ++i) a[g][i] = val[eg * NC + i];
         }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int i = 0; i < NC; ++i)
+                xv[g][i] = xread<REP, TILE>(x, R, cg[g], v0 + i, NV, tile_b, nflow);
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             if (e + g < e1) {
@@ -373,7 +380,7 @@ __global__ __launch_bounds__(256) void k_rep_sum(ReplicaDev R, const double *__r
 // lanes of a group each evaluate it and finish every KS-th copy), then
 // finished into rows c0 + k p + q with each row's diagonal added last -- the
 // arithmetic extra_short_rows does on each unshared row.
-template <int NV, bool CPLX>
+template <int NV, bool CPLX, bool TILE = true>
 __device__ __forceinline__ void extra_replica_rows(
     int64_t w, const ReplicaDev &R, int64_t nflow, const double *__restrict__ x,
     double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
@@ -389,8 +396,8 @@ __device__ __forceinline__ void extra_replica_rows(
     double sh[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) sh[i] = 0.0;
-    row_dot<NV, CPLX, 4, false>(sh, R.vptr[q], R.vptr[q + 1], R.vcol, R.vval, x, R, v0, tile_b,
-                                nflow);
+    row_dot<NV, CPLX, 4, false, TILE>(sh, R.vptr[q], R.vptr[q + 1], R.vcol, R.vval, x, R, v0,
+                                      tile_b, nflow);
     // the copies in chunks of KU, loads first
     constexpr int KU = 2;
     for (int64_t k0 = ks; k0 < R.nrep; k0 += KS * KU) {
@@ -444,7 +451,7 @@ __device__ __forceinline__ void extra_replica_rows(
 // cdiag (optional): a complex diagonal that differs per column (per pair
 // (re, im) of the NV doubles), added on rows [cd0, cd1) -- the -i*omega_k
 // block of the factored receptivity systems, one wavenumber per column.
-template <int NV, bool CPLX, int NW, bool REP>
+template <int NV, bool CPLX, int NW, bool REP, bool TILE>
 __device__ __forceinline__ void extra_long_row(
     int64_t ri, const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
     const int32_t *__restrict__ col, const double *__restrict__ val,
@@ -464,18 +471,60 @@ __device__ __forceinline__ void extra_long_row(
     double acc[NC];
 #pragma unroll
     for (int q = 0; q < NC; ++q) acc[q] = 0.0;
+    // the operands of the epilogue's update, loaded before the row's loop
+    // (only this workgroup writes the row in this launch)
+    const int64_t row = rows[ri];
+    double old_t[NC], old_v[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        const int v = v0 + q;
+        old_t[q] = old_v[q] = 0.0;
+        if (j == 0 && wv == 0) {
+            if (row < nflow) {
+                const int64_t o = buf_pos(row, v, NV, tile_b, nflow);
+                if (tout) old_t[q] = tout[o];
+                if (vout) old_v[q] = vout[o];
+            } else if (vout) {
+                old_v[q] = vin[row * NV + v];
+            }
+        }
+    }
+    // A lane's entries lie S apart; G of them are in flight at a time: their
+    // column, value and x loads are issued before the products, which are
+    // added in entry order (the same sum as one entry at a time).  A row of
+    // the shock bands (3 600 entries at 400 chi stations) then costs one
+    // round of dependent loads per lane instead of fourteen.
+    constexpr int G = (NC == 1 && LPR <= 4) ? 8 : 4;
+    constexpr int64_t S = (int64_t)NW * EPW;
     const int64_t e1 = ptr[ri + 1];
-#pragma unroll 4
-    for (int64_t e = ptr[ri] + wv * EPW + j; e < e1; e += NW * EPW) {
-        const int64_t c = col[e];
-        if constexpr (CPLX) {
-            const double ar = val[2 * e], ai = val[2 * e + 1];
-            const double xr = xread<REP>(x, R, c, v0, NV, tile_b, nflow);
-            const double xi = xread<REP>(x, R, c, v0 + 1, NV, tile_b, nflow);
-            acc[0] = fma(ar, xr, fma(-ai, xi, acc[0]));
-            acc[1] = fma(ar, xi, fma(ai, xr, acc[1]));
-        } else {
-            acc[0] = fma(val[e], xread<REP>(x, R, c, v0, NV, tile_b, nflow), acc[0]);
+    for (int64_t e = ptr[ri] + wv * EPW + j; e < e1; e += G * S) {
+        double a[G][NC], xv[G][NC];
+        int32_t cg[G];
+        // columns and values of the G entries first, then their x entries:
+        // one wait for all the columns (past the row's end the loads repeat
+        // entry e, so nothing outside the row is read)
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int64_t eg = e + g * S < e1 ? e + g * S : e;
+            cg[g] = col[eg];
+#pragma unroll
+            for (int i = 0; i < NC; ++i) a[g][i] = val[eg * NC + i];
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int i = 0; i < NC; ++i)
+                xv[g][i] = xread<REP, TILE>(x, R, cg[g], v0 + i, NV, tile_b, nflow);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (e + g * S < e1) {
+                if constexpr (CPLX) {
+                    acc[0] = fma(a[g][0], xv[g][0], fma(-a[g][1], xv[g][1], acc[0]));
+                    acc[1] = fma(a[g][0], xv[g][1], fma(a[g][1], xv[g][0], acc[1]));
+                } else {
+                    acc[0] = fma(a[g][0], xv[g][0], acc[0]);
+                }
+            }
         }
     }
 #pragma unroll
@@ -495,7 +544,6 @@ __device__ __forceinline__ void extra_long_row(
             for (int q = 0; q < NC; ++q) acc[q] += part[i][v0 + q];
     }
     if (j != 0) return;
-    const int64_t row = rows[ri];
     if constexpr (NV % 2 == 0) {
         if (cdiag && row >= cd0 && row < cd1) {
 #pragma unroll
@@ -516,12 +564,12 @@ __device__ __forceinline__ void extra_long_row(
             // flow row: the block kernel has written its part already
             const int64_t o = buf_pos(row, v, NV, tile_b, nflow);
             const double t = c1 * acc[q];
-            if (tout) tout[o] += t;
-            if (vout) vout[o] = fma(c2, t, vout[o]);
+            if (tout) tout[o] = old_t[q] + t;
+            if (vout) vout[o] = fma(c2, t, old_v[q]);
         } else {
             const double t = c1 * acc[q];
             if (tout) tout[row * NV + v] = t;
-            if (vout) vout[row * NV + v] = fma(c2, t, vin[row * NV + v]);
+            if (vout) vout[row * NV + v] = fma(c2, t, old_v[q]);
         }
     }
 }
@@ -532,7 +580,7 @@ __device__ __forceinline__ void extra_long_row(
 // loops over the row's entries; RPW = 64 / lanes-per-row rows per wave.
 // Wave w takes rows [r0 + w RPW, r0 + (w + 1) RPW) of [r0, nrows); same
 // outputs as extra_long_row.
-template <int NV, bool CPLX, bool REP>
+template <int NV, bool CPLX, bool REP, bool TILE>
 __device__ __forceinline__ void extra_short_rows(
     int64_t w, const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
     const int32_t *__restrict__ col, const double *__restrict__ val, int64_t r0, int64_t nrows,
@@ -555,8 +603,8 @@ __device__ __forceinline__ void extra_short_rows(
     // one entry at a time (the unreplicated operator's short rows hold a
     // single entry or a few); four loads at a time on the replicated form's
     // rows of compressed B entries
-    row_dot<NV, CPLX, REP ? 4 : 1, REP>(acc, ptr[ri], ptr[ri + 1], col, val, x, R, v0, tile_b,
-                                        nflow);
+    row_dot<NV, CPLX, REP ? 4 : 1, REP, TILE>(acc, ptr[ri], ptr[ri + 1], col, val, x, R, v0,
+                                              tile_b, nflow);
     // the per-column complex diagonal acts on (re, im) pairs of the row
     // whether the operator's values are complex (a lane owns a pair) or real
     // (a lane owns one double of a pair)
@@ -603,16 +651,27 @@ __global__ __launch_bounds__(64 * NW) void k_extra(
     double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
     const double *__restrict__ cdiag, int64_t cd0, int64_t cd1, int tile_b, ReplicaDev R) {
     const int64_t bid = blockIdx.x;
-    if (bid < nlong)
-        extra_long_row<NV, CPLX, NW, REP>(xcd_tile(bid, nlong), rows, ptr, col, val, nflow, x, tout,
-                                     vin, vout, c1, c2, cdiag, cd0, cd1, tile_b, R);
-    else if (bid < nlong + nshort_wg)
-        extra_short_rows<NV, CPLX, REP>((bid - nlong) * NW + (threadIdx.x >> 6), rows, ptr, col, val,
-                                   nlong, nrows, nflow, x, tout, vin, vout, c1, c2, cdiag, cd0,
-                                   cd1, tile_b, R);
-    else if constexpr (REP)
-        extra_replica_rows<NV, CPLX>((bid - nlong - nshort_wg) * NW + (threadIdx.x >> 6), R,
-                                     nflow, x, tout, vin, vout, c1, c2, tile_b);
+    // tile_b is uniform: one specialised body per layout, so the x reads of
+    // the row-major case carry no layout test
+    auto body = [&](auto tile) {
+        constexpr bool TILE = decltype(tile)::value;
+        if (bid < nlong)
+            extra_long_row<NV, CPLX, NW, REP, TILE>(xcd_tile(bid, nlong), rows, ptr, col, val,
+                                                    nflow, x, tout, vin, vout, c1, c2, cdiag,
+                                                    cd0, cd1, tile_b, R);
+        else if (bid < nlong + nshort_wg)
+            extra_short_rows<NV, CPLX, REP, TILE>((bid - nlong) * NW + (threadIdx.x >> 6), rows,
+                                                  ptr, col, val, nlong, nrows, nflow, x, tout,
+                                                  vin, vout, c1, c2, cdiag, cd0, cd1, tile_b, R);
+        else if constexpr (REP)
+            extra_replica_rows<NV, CPLX, TILE>((bid - nlong - nshort_wg) * NW +
+                                                   (threadIdx.x >> 6),
+                                               R, nflow, x, tout, vin, vout, c1, c2, tile_b);
+    };
+    if (tile_b)
+        body(std::true_type{});
+    else
+        body(std::false_type{});
 }
 
 // var-major (ABI) -> cell-major (internal).  x_vm holds nrhs columns of n
@@ -1281,8 +1340,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
         // slices below keep the XCD mapping of an unfused launch)
         if (bid < F.nfs_wg) {
             if constexpr (FK::adj)
-                extra_replica_rows<NV, FK::cplx>(bid * NW + w, F.R, F.nflow, x, tout, vin, vout,
-                                                 c1, c2, 0);
+                extra_replica_rows<NV, FK::cplx, false>(bid * NW + w, F.R, F.nflow, x, tout,
+                                                        vin, vout, c1, c2, 0);
             else
                 rep_sum_wave<NV, FK::cplx, true>(F.R, bid * NW + w, x, F.s_out, tout, vin, vout,
                                                  c1, c2);
@@ -2519,7 +2578,8 @@ __global__ __launch_bounds__(256) void k_spmm_quad4c(
         if (bid < F.nfs_wg) {
             const int64_t wv = bid * wpb + (threadIdx.x >> 6);
             if constexpr (FK::adj)
-                extra_replica_rows<R, FK::cplx>(wv, F.R, F.nflow, x, tout, vin, vout, c1, c2, 0);
+                extra_replica_rows<R, FK::cplx, false>(wv, F.R, F.nflow, x, tout, vin, vout, c1,
+                                                       c2, 0);
             else
                 rep_sum_wave<R, FK::cplx, true>(F.R, wv, x, F.s_out, tout, vin, vout, c1, c2);
             return;

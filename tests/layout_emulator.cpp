@@ -1290,3 +1290,11 @@ extern "C" int emu_receptivity_parts_equal(int64_t n_ext, const int64_t *colptr,
         return 5;
     return 0;
 }
+
+// xcd_balanced_order (the long extra rows' order at upload) for m rows of
+// len[i] entries: out[new position] = old index.
+extern "C" void emu_xcd_balanced_order(int64_t m, const int64_t *len, int64_t *out) {
+    const std::vector<int64_t> l(len, len + m);
+    const std::vector<int64_t> o = xcd_balanced_order(l);
+    for (int64_t i = 0; i < m; ++i) out[i] = o[(size_t)i];
+}

@@ -176,6 +176,49 @@ def test_receptivity_from_parts_equals_compressed_assembly(emu, nchi, neta, shoc
     assert rc == 0
 
 
+def _xcd_loads(order, lens):
+    """Entries per XCD when k_extra's workgroup bid takes long row
+    order[xcd_tile(bid, m)] and the hardware deals workgroups round-robin
+    over the 8 XCDs (xcd_tile's contiguous ranges replayed)."""
+    m = len(order)
+    per, rem = m // 8, m % 8
+    loads = np.zeros(8)
+    for bid in range(m):
+        xcd, idx = bid & 7, bid >> 3
+        tile = xcd * (per + 1) + idx if xcd < rem else rem * (per + 1) + (xcd - rem) * per + idx
+        assert 0 <= tile < m
+        loads[xcd] += lens[order[tile]]
+    return loads
+
+
+@pytest.mark.parametrize("case", ["c3_forward", "c3_adjoint", "c2_forward", "ragged",
+                                  "receptivity", "few"])
+def test_long_extra_rows_spread_over_xcds(emu, case):
+    """The long extra rows (one k_extra workgroup each) are ordered at upload
+    so that the 8 XCDs get about the same number of entries: at 400 x 300
+    cells with 400 shock DOFs the 400 shock rows (3 600 entries) would
+    otherwise all fall on the last XCDs' ranges.  Rows of equal length keep
+    their order (neighbouring rows on one XCD)."""
+    lens = {"c3_forward": [400] * 1600 + [3600] * 400,
+            "c3_adjoint": [400] * 3200 + [2000] * 400,
+            "c2_forward": [200] * 800 + [1800] * 200,
+            "ragged": list(np.random.default_rng(5).integers(33, 5000, 1003)),
+            "receptivity": [384] * 2000,
+            "few": [40, 5000, 60, 7000, 33]}[case]
+    lens = np.asarray(lens, dtype=np.int64)
+    out = np.zeros(lens.shape[0], dtype=np.int64)
+    emu.emu_xcd_balanced_order(C.c_int64(lens.shape[0]), C.c_void_p(lens.ctypes.data),
+                               C.c_void_p(out.ctypes.data))
+    assert sorted(out) == list(range(lens.shape[0]))
+    if case in ("receptivity", "few"):
+        assert list(out) == list(range(lens.shape[0]))
+        return
+    loads = _xcd_loads(out, lens)
+    assert loads.max() <= loads.mean() + lens.max(), loads
+    before = _xcd_loads(np.arange(lens.shape[0]), lens)
+    assert loads.max() < before.max()
+
+
 def _run_fused(emu, A, b, ncell, X2, order, nsub, dt, nv=2):
     """emu_horner_fused on scipy matrix A (var-major) with nv doubles per
     row: X2 is n x nv/2 complex values (or n complex values at nv = 2) or
