@@ -27,6 +27,7 @@ import time
 from pathlib import Path
 
 import numpy as np
+import scipy.sparse as sp
 
 METRIC = "exp(AT)v actions/s + SpMV HBM GB/s (% peak), batched vs plain FP64 at 1e-8"
 HBM_PEAK_GBS = 8000.0
@@ -173,7 +174,11 @@ def transient_growth_leg(T=1.0, k=4, tol=1e-8, steps=5):
     device Krylov solver (k gains), and the Chu sandwich it applies
     (forward Taylor sweep, weight, adjoint sweep; T_5, n_t = T/dt
     substeps each way) timed alone, with its weight stage timed as the
-    sandwich at n_t = 0 (permutations and the block-diagonal weight)."""
+    sandwich at n_t = 0 (permutations and the block-diagonal weight); and
+    the gain's gradient with respect to the operator's entries
+    (GainFunctional.operator_gradient: one forward action, the weight, and
+    the checkpointed pullback, every 8 substeps) on the operator's own
+    pattern, host vectors in and out."""
     from hymor_amd import operators as ops
     from hymor_amd import stability as st
     from hymor_amd.baseflow import synthetic_solution
@@ -219,6 +224,27 @@ def transient_growth_leg(T=1.0, k=4, tol=1e-8, steps=5):
     out["sandwich_note"] = "device-resident vectors (hymor_chu_sandwich_dev)"
     log(f"transient growth: solve {out['solve_seconds']:.2f} s, gains {out['gains']}, "
         f"sandwich {ts * 1e3:.2f} ms, weight stage {tw * 1e3:.3f} ms")
+    # the gain's sensitivity to the operator's entries
+    from hymor_amd.adjoint import GainFunctional
+    D = (R.T @ st.CONSTRUCT_M_HAT(s, [1, 1, 1, 1]) @ R).tocsc()
+    As = sp.csc_matrix(A * dt)
+    As.sort_indices()
+    colptr, rowval = As.indptr.astype(np.int64), As.indices.astype(np.int64)
+    ckpt = 8
+    with DeviceOperator.from_scipy(As, nchi=nchi, neta=neta) as dA, \
+            DeviceOperator.from_scipy(C, nchi=nchi, neta=neta) as dC:
+        F = GainFunctional(dA, dC, D, n_t)
+        F.operator_gradient(x, colptr, rowval, index_base=0, ckpt_every=ckpt)
+        t1 = time.perf_counter()
+        G, g = F.operator_gradient(x, colptr, rowval, index_base=0, ckpt_every=ckpt)
+        tg = time.perf_counter() - t1
+    out["operator_gradient"] = {
+        "ms": tg * 1e3, "entries": int(As.nnz), "ckpt_every": ckpt,
+        "device_vectors": int(-(-n_t // ckpt) + (2 * 5 + 1) * ckpt + 5),
+        "note": ("dG/dA on the operator's CSC pattern: forward action, Chu weight, checkpointed "
+                 "pullback (hymor_expm_taylor_vjp_values); host vectors and gradient")}
+    log(f"transient growth: operator gradient on {As.nnz} entries {tg * 1e3:.1f} ms "
+        f"(checkpoint every {ckpt} substeps)")
     return out
 
 

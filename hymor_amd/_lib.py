@@ -52,6 +52,8 @@ SIGNATURES = {
     "hymor_expm_taylor_batched_coldiag_dev": (C.c_int, [_p, C.c_int, C.c_int, _i64, C.c_double,
                                                         C.c_int, _i64, _p, _i64, _i64, _p, _i64,
                                                         _p, _i64]),
+    "hymor_expm_taylor_vjp_values": (C.c_int, [_p, C.c_int, _i64, C.c_double, _i64, _p, _p, _p,
+                                               _p, C.c_int, _p]),
     "hymor_chu_sandwich": (C.c_int, [_p, _p, C.c_int, _i64, C.c_double, _i64, _p, _p, C.c_int]),
     "hymor_chu_sandwich_dev": (C.c_int, [_p, _p, C.c_int, _i64, C.c_double, _i64, _p, _p,
                                          C.c_int]),

@@ -19,6 +19,17 @@ vector.
 ``GainFunctional`` exposes value / value_and_grad; ``gain_autograd`` wraps it
 as a torch.autograd.Function (PyTorch's counterpart of a ChainRules rrule)
 when torch is importable.
+
+The gradient with respect to the operator's entries (the gain's
+sensitivity to the Jacobian, ``GainFunctional.operator_gradient``) does need
+the forward states: with w = T(A)^n P v,
+
+    dG/dA = d<2 C w / (v' D v), T(A)^n P v>/dA    (C, D fixed),
+
+the pullback of the Taylor sweep for the cotangent 2 C w / (v' D v), which
+hymor_expm_taylor_vjp_values evaluates with checkpointed substeps (a state
+kept every ckpt_every substeps, each group recomputed from its checkpoint
+in the backward sweep).
 """
 from __future__ import annotations
 
@@ -55,6 +66,22 @@ class GainFunctional:
         den = np.vdot(v, Dv).real
         G = np.vdot(v, Sv).real / den
         return float(G), 2.0 * (Sv - G * Dv) / den
+
+
+    def operator_gradient(self, v, colptr, rowval, *, index_base=1, ckpt_every=8):
+        """(G(v), dG/dA) with dG/dA on the CSC pattern (colptr, rowval) of
+        the stored operator (A*dt as uploaded; real operators)."""
+        v = np.array(v, dtype=np.float64)
+        v[:self.n_set0] = 0.0
+        w = self.devA.expm_taylor(v, order=self.order, nsub=self.n_t, dt=1.0, form="horner")
+        Cw = self.devC.spmv(w)
+        vo = np.asarray(v)
+        den = float(vo @ (self.D @ vo))
+        G = float(w @ Cw) / den
+        grad = self.devA.expm_taylor_vjp_values(v, 2.0 * Cw / den, colptr, rowval,
+                                                order=self.order, nsub=self.n_t, dt=1.0,
+                                                ckpt_every=ckpt_every, index_base=index_base)
+        return G, grad
 
 
 def gain_autograd(functional: GainFunctional):

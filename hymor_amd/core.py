@@ -149,6 +149,26 @@ class DeviceOperator:
             X.shape[1], ptr(X), self.n, ptr(Y), self.n, C128 if cplx else F64))
         return Y
 
+    def expm_taylor_vjp_values(self, x, wbar, colptr, rowval, *, order=5, nsub=1, dt=1.0,
+                               ckpt_every=8, index_base=1):
+        """Gradient of <wbar, T_order(dt A)^nsub x> with respect to the
+        stored operator's entries on the CSC pattern (colptr, rowval) (real
+        operators; hymor_expm_taylor_vjp_values, checkpointed every
+        ckpt_every substeps)."""
+        if self.is_complex:
+            raise ValueError("the operator gradient is for real operators")
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        wbar = np.ascontiguousarray(wbar, dtype=np.float64)
+        colptr = np.ascontiguousarray(colptr, dtype=np.int64)
+        rowval = np.ascontiguousarray(rowval, dtype=np.int64)
+        if x.shape != (self.n,) or wbar.shape != (self.n,) or colptr.shape != (self.n + 1,):
+            raise ValueError("x, wbar need n entries and colptr n + 1")
+        grad = np.empty(rowval.shape[0], dtype=np.float64)
+        check(_lib.load().hymor_expm_taylor_vjp_values(
+            self._h, int(order), int(nsub), float(dt), int(ckpt_every), ptr(x), ptr(wbar),
+            ptr(colptr), ptr(rowval), int(index_base), ptr(grad)))
+        return grad
+
     def chu_sandwich(self, weight: "DeviceOperator", x, *, order=5, nsub=1, dt=1.0, n_set0=0):
         """T(A'dt)^n * C * T(A dt)^n * x with CONSTRUCT_C's zeroing."""
         x, dty = _vec(x, self.is_complex)

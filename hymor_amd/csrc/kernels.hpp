@@ -2712,3 +2712,76 @@ __global__ __launch_bounds__(256) void k_spmm_quad4c(
 }
 
 }  // namespace hymor
+
+namespace hymor {
+
+// ------------------------------------------------------ operator gradient
+// The gradient of J = <lambda, T_p(dt A)^n x> with respect to A's entries.
+// One substep x_{k+1} = S x_k, S = sum_{j<=p} (dt A)^j / j!, contributes
+//     dt * sum_{m<p} u_m z_m'   with u_m = (dt A')^m lambda_{k+1},
+//     z_m = sum_{r<p-m} y_r / (m+r+1)!,  y_r = (dt A)^r x_k,
+// (d(A^j) = sum_m A^m dA A^(j-1-m)), and lambda_k = S' lambda_{k+1}.
+
+// Up to kMaxVjp vectors of n doubles with one coefficient each.
+constexpr int kMaxVjp = 21;
+struct VecList {
+    const double *v[kMaxVjp];
+    double c[kMaxVjp];
+    int cnt;
+};
+
+// Z column m (at Z + m n) = sum_{r < p - m} Y.c[m + r + 1] * y_r, m = 0..p-1,
+// for y_r = Y.v[r] and Y.c[j] = 1 / j!: the right factors of one substep's
+// contribution (terms added in ascending r).
+__global__ void k_vjp_z(VecList Y, int p, int64_t n, double *__restrict__ Z) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        for (int m = 0; m < p; ++m) {
+            double z = 0.0;
+            for (int r = 0; r + m < p; ++r) z = fma(Y.c[m + r + 1], Y.v[r][i], z);
+            Z[m * n + i] = z;
+        }
+    }
+}
+
+// out = sum_{j < L.cnt} L.c[j] * L.v[j] (terms added in ascending j); out is
+// none of the inputs.
+__global__ void k_lincomb(VecList L, int64_t n, double *__restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int j = 0; j < L.cnt; ++j) s = fma(L.c[j], L.v[j][i], s);
+        out[i] = s;
+    }
+}
+
+// Sampled product on a pattern: for its nnz entries (row er[e], column
+// ec[e], cell-major), grad[e] = (init ? 0 : grad[e]) + alpha * sum_{k < kc}
+// U[k n + er[e]] * Z[k n + ec[e]] (U, Z column-major n x kc; the products
+// added in ascending k).  A thread per entry; the workgroups take
+// contiguous ranges of entries per XCD, so the U and Z rows that
+// neighbouring entries share stay in one L2.
+__global__ __launch_bounds__(256) void k_sddmm_pattern(
+    const int32_t *__restrict__ er, const int32_t *__restrict__ ec, int64_t nnz,
+    const double *__restrict__ U, const double *__restrict__ Z, int64_t n, int kc, double alpha,
+    int init, double *__restrict__ grad) {
+    const int64_t e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    const int64_t r = er[e], c = ec[e];
+    double acc = 0.0;
+    int k = 0;
+    for (; k + 4 <= kc; k += 4) {
+        double u[4], z[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            u[q] = U[(int64_t)(k + q) * n + r];
+            z[q] = Z[(int64_t)(k + q) * n + c];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc = fma(u[q], z[q], acc);
+    }
+    for (; k < kc; ++k) acc = fma(U[(int64_t)k * n + r], Z[(int64_t)k * n + c], acc);
+    grad[e] = (init ? 0.0 : grad[e]) + alpha * acc;
+}
+
+}  // namespace hymor

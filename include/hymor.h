@@ -198,6 +198,27 @@ int hymor_expm_taylor_batched_coldiag_dev(hymor_op *op, int transpose, int order
                                           int64_t row0, int64_t row1, const void *d_X,
                                           int64_t ldx, void *d_Y, int64_t ldy);
 
+/* Gradient of J = <wbar, T_order(dt*A)^nsub x> with respect to the
+ * operator's entries -- the pullback a ChainRules rrule needs for the
+ * operator argument, e.g. the sensitivity of a transient-growth gain to the
+ * Jacobian (no reference counterpart: the reference takes no derivatives;
+ * the gain's gradient with respect to the vector needs only
+ * hymor_chu_sandwich).  Real operators.  x, wbar: host vectors (var-major,
+ * length n).  The gradient is returned for every entry of the pattern
+ * colptr[n+1], rowval (CSC, index_base 0 or 1: the upload's own arrays give
+ * it on A's pattern; entries outside it are allowed): grad[e] = dJ/dA_ij for
+ * entry e = (rowval[e], its column), A the operator as stored (the uploaded
+ * values times `scale`).  Checkpointed: the forward sweep keeps the state
+ * every ckpt_every substeps and the backward sweep recomputes one group of
+ * ckpt_every states at a time from its checkpoint, so beside the handle's
+ * workspace the call holds ceil(nsub/ckpt_every) + (2*order + 1)*ckpt_every
+ * + order vectors of n doubles and 16 bytes per pattern entry.
+ * Synchronous. */
+int hymor_expm_taylor_vjp_values(hymor_op *op, int order, int64_t nsub, double dt,
+                                 int64_t ckpt_every, const double *x, const double *wbar,
+                                 const int64_t *colptr, const int64_t *rowval, int index_base,
+                                 double *grad);
+
 /* Chu-norm sandwich y = T_p(A'dt)^n * C * T_p(A dt)^n * x with the zeroing
  * of CONSTRUCT_C (CONSTRUCT_C.jl:33-58 around EXPONENTIAL! :78-111):
  * x[0:n_set0) is zeroed before and y[0:n_set0) after (var-major indices).

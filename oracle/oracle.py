@@ -15,6 +15,9 @@ Two layers:
       - linear_integration_energy
                        LINEAR_INTEGRATION_AND_ENERGY_GPU_NO_SAMPLES
                        Freestream_receptivity/LINEAR_INTEGRATION_AND_GAINS.jl:436-477
+  * taylor_vjp_values: the gradient of <wbar, P(A)^n x> (POWER_METHOD!'s
+    polynomial) with respect to A's entries, every forward state kept (no
+    reference counterpart: the reference has no derivatives);
   * closed forms that pin those restatements independently of them
     (block-circulant operators diagonalised by the 2-D DFT, dense matrix
     polynomials, diagonal and nilpotent operators).
@@ -151,6 +154,50 @@ def power_method(A: CSC, v, timesteps: int, order: int = 5):
     lib.oc_power_method_f64(A.n, _p(A.colptr), _p(A.rowval), _p(A.nzval), order, timesteps, _p(v),
                             _p(vout), _p(term))
     return v
+
+
+def taylor_vjp_values(A: CSC, x, wbar, timesteps: int, order: int, colptr, rowval):
+    """Gradient of J = <wbar, P(A)^timesteps x>, P(A) = sum_{j <= order} A^j / j!
+    (POWER_METHOD!'s polynomial, ARNOLDI_POWER_GPU.jl:70-79, with A the
+    scaled operator A*dt), with respect to the entries (rowval[e], column of
+    e) of a 1-based CSC pattern (colptr, rowval), real operators.  The
+    reverse sweep of the polynomial: with lambda_n = wbar and x_{k+1} =
+    P(A) x_k,
+        d<l, A^j x> = sum_{m < j} <(A')^m l, dA A^(j-1-m) x>,
+        grad += sum_{m < order} u_m[row] z_m[col],  u_m = (A')^m lambda_{k+1},
+        z_m = sum_{r < order-m} A^r x_k / (m+r+1)!,   lambda_k = P(A') lambda_{k+1},
+    every forward state kept, the SpMVs those of csc_mul on A and its
+    adjoint (pinned by central differences of power_method and by closed
+    forms in tests/test_oracle_pins.py)."""
+    At = csc_adjoint(A)
+    p = int(order)
+    colptr = np.asarray(colptr, dtype=np.int64)
+    rows = np.asarray(rowval, dtype=np.int64) - 1
+    cols = np.repeat(np.arange(colptr.shape[0] - 1), np.diff(colptr))
+    fact = [1.0]
+    for j in range(1, p + 1):
+        fact.append(fact[-1] / j)
+    xs = [np.array(x, dtype=np.float64)]
+    for _ in range(int(timesteps)):
+        xs.append(power_method(A, xs[-1], 1, p))
+    lam = np.array(wbar, dtype=np.float64)
+    grad = np.zeros(rows.shape[0])
+    for k in reversed(range(int(timesteps))):
+        y = [xs[k]]
+        for _ in range(1, p):
+            y.append(csc_mul(A, y[-1]))
+        u = [lam]
+        for _ in range(p):
+            u.append(csc_mul(At, u[-1]))
+        for m in range(p):
+            z = np.zeros_like(lam)
+            for r in range(p - m):
+                z = z + fact[m + r + 1] * y[r]
+            grad += u[m][rows] * z[cols]
+        lam = np.zeros_like(lam)
+        for m in range(p + 1):
+            lam = lam + fact[m] * u[m]
+    return grad
 
 
 def exponential(A: CSC, At: CSC, Cw: CSC, v, timesteps: int, order: int = 5):

@@ -293,3 +293,82 @@ def test_taylor_csr_omp_matches_power_method(orc):
     ref = orc.power_method(orc.CSC.from_scipy(A.tocsc()), v, 3)
     assert used >= 1
     assert np.allclose(got, ref, rtol=1e-13, atol=1e-13)
+
+
+# ---------------------------------------------------------------- operator gradient
+
+def _pattern(A):
+    """1-based CSC arrays of A's pattern plus a few entries outside it."""
+    B = sp.csc_matrix(A, dtype=np.float64).tolil()
+    n = A.shape[0]
+    rng = np.random.default_rng(11)
+    for i, j in zip(rng.integers(0, n, 6), rng.integers(0, n, 6)):
+        if B[i, j] == 0:
+            B[i, j] = 1e-300        # marks the slot; the value is not used
+    B = sp.csc_matrix(B)
+    B.sort_indices()
+    return B.indptr.astype(np.int64) + 1, B.indices.astype(np.int64) + 1
+
+
+def _j(orc, A, x, wbar, n_t, p):
+    return float(wbar @ orc.power_method(orc.CSC.from_scipy(A), x, n_t, p))
+
+
+@pytest.mark.parametrize("n_t,p", [(1, 1), (3, 5), (4, 12)])
+def test_taylor_vjp_central_differences(orc, n_t, p):
+    """taylor_vjp_values against central differences of power_method's
+    J = <wbar, P(A)^n x> in 16 random directions over A's pattern and six
+    entries outside it (a transposed operand, a dropped term or a wrong
+    factorial moves the gradient far outside the 1e-7 band)."""
+    A = rand_sparse(40, 0.12, 21) * 0.2
+    A = sp.csc_matrix(A)
+    rng = np.random.default_rng(22)
+    x, wbar = rng.standard_normal(40), rng.standard_normal(40)
+    colptr, rowval = _pattern(A)
+    g = orc.taylor_vjp_values(orc.CSC.from_scipy(A), x, wbar, n_t, p, colptr, rowval)
+    rows = rowval - 1
+    cols = np.repeat(np.arange(40), np.diff(colptr))
+    eps = 1e-5
+    for d in range(16):
+        delta = rng.standard_normal(rows.shape[0])
+        D = sp.csc_matrix((delta, (rows, cols)), shape=A.shape)
+        fd = (_j(orc, A + eps * D, x, wbar, n_t, p) - _j(orc, A - eps * D, x, wbar, n_t, p)) / (2 * eps)
+        assert abs(g @ delta - fd) <= 1e-7 * max(1.0, abs(fd)), (d, g @ delta, fd)
+
+
+def test_taylor_vjp_closed_forms(orc):
+    """Diagonal A = diag(a): the derivative of the matrix polynomial
+    phi(A) = P(A)^n at a diagonal matrix has entries given by divided
+    differences (Daleckii-Krein), so dJ/dA_ij = wbar_i x_j (phi(a_i) -
+    phi(a_j)) / (a_i - a_j) off the diagonal and wbar_i x_i phi'(a_i) =
+    wbar_i x_i n P(a_i)^(n-1) P'(a_i) on it, P'(a) = sum_{j<p} a^j / j!;
+    one step of order 1 (P(A) = I + A): dJ/dA_ij = wbar_i x_j on any
+    pattern."""
+    rng = np.random.default_rng(23)
+    n, n_t, p = 30, 6, 5
+    a = rng.uniform(-0.5, 0.3, n)
+    A = sp.diags(a, format="csc")
+    A = A + sp.csc_matrix(([0.0], ([0], [1])), shape=(n, n))   # an empty slot in (0, 1)
+    x, wbar = rng.standard_normal(n), rng.standard_normal(n)
+    colptr, rowval = _pattern(A)
+    g = orc.taylor_vjp_values(orc.CSC.from_scipy(sp.diags(a, format="csc")), x, wbar, n_t, p,
+                              colptr, rowval)
+    rows = rowval - 1
+    cols = np.repeat(np.arange(n), np.diff(colptr))
+    P = sum(a ** j / math.factorial(j) for j in range(p + 1))
+    dP = sum(a ** j / math.factorial(j) for j in range(p))
+    phi = P ** n_t
+    diag = rows == cols
+    assert diag.sum() == n and (~diag).sum() >= 3
+    with np.errstate(invalid="ignore", divide="ignore"):
+        dd = np.where(diag, n_t * P[rows] ** (n_t - 1) * dP[rows],
+                      (phi[rows] - phi[cols]) / (a[rows] - a[cols]))
+    want = wbar[rows] * x[cols] * dd
+    assert np.allclose(g, want, rtol=1e-11, atol=1e-13)
+    B = rand_sparse(25, 0.2, 24)
+    colptr, rowval = _pattern(B)
+    x, wbar = rng.standard_normal(25), rng.standard_normal(25)
+    g1 = orc.taylor_vjp_values(orc.CSC.from_scipy(B), x, wbar, 1, 1, colptr, rowval)
+    rows = rowval - 1
+    cols = np.repeat(np.arange(25), np.diff(colptr))
+    assert np.allclose(g1, wbar[rows] * x[cols], rtol=1e-15, atol=0)
