@@ -248,6 +248,90 @@ def transient_growth_leg(T=1.0, k=4, tol=1e-8, steps=5):
     return out
 
 
+def shock_config_leg(steps=5, order=12, nsub=64, T=1.0, k=20, ncv=80, cpu=True):
+    """The 200 x 300-cell, b = 4 operator with 200 shock DOFs (N = 240 200,
+    the shock bands of the reference's shock fitting):
+      * the plain FP64 action T_12(A T/64)^64 on the GPU as the drivers run
+        it (one HIP graph replay per action) and on the host (the oracle's
+        OpenMP CSR form: one substep, scaled x64);
+      * EIGENVALUES_GPU_ARNOLDI for 20 modes with Krylov dimension 80
+        (T_ref = 0.5, dt = 1/64, order 5) through the device Krylov-Schur
+        (residual tolerance 1e-9) and through ARPACK at the reference's
+        tolerance 1e-4 (EIGENVALUES.jl:159), both applying the GPU action,
+        on an operator of that shape and storage whose spectrum is exact
+        (one-way coupling; its diagonal blocks' eigenvalues), with the
+        worst relative eigenvalue error of each."""
+    import math
+    from hymor_amd import operators as ops
+    from hymor_amd import stability as st
+    from hymor_amd.core import DeviceOperator, DeviceVector, expm_taylor_dev
+
+    nchi, neta = 200, 300
+    op = ops.config_operator("C2")
+    x = np.random.default_rng(9).standard_normal(op.n)
+    out = {"workload": (f"200x300 cells, b=4, 200 shock DOFs, N={op.n}, "
+                        f"nnz={op.nnzb * 16 + op.xrow.shape[0]}; action T_{order}(A T/{nsub})^"
+                        f"{nsub} v, T={T}, {FORM} form, FP64"), "unit": "exp(AT)v actions/s"}
+    with DeviceOperator.from_blocks(op, with_adjoint=False) as dev:
+        dev.set_graphs(True)
+        dx, dy = DeviceVector.from_host(x), DeviceVector(op.n)
+        try:
+            run = lambda: expm_taylor_dev(dev, dx, dy, order=order, nsub=nsub, dt=T / nsub,
+                                          form=FORM)
+            run()
+            dev.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                run()
+            dev.synchronize()
+            ta = (time.perf_counter() - t0) / steps
+            k_ms, k_bytes = dev.bench_term_kernel(reps=20, form=FORM)
+        finally:
+            dx.free()
+            dy.free()
+    out["gpu"] = {"value": 1.0 / ta, "ms_per_action": ta * 1e3,
+                  "term_ms": k_ms, "term_GBps": k_bytes / (k_ms * 1e-3) / 1e9,
+                  "note": "term: block kernel + the shock bands' extra rows (HIP events)"}
+    if cpu:
+        from oracle import oracle
+        csr = oracle.CsrArrays(op.to_csr_cellmajor() * (T / nsub))
+        nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        oracle.taylor_csr_omp(csr, x, 1, order, nthreads)
+        t_omp = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            _, used = oracle.taylor_csr_omp(csr, x, 1, order, nthreads)
+            t_omp.append(time.perf_counter() - t0)
+        cpu_aps = 1.0 / (float(np.median(t_omp)) * nsub)
+        out["cpu"] = {"value": cpu_aps, "cores": int(used), "kind": "port",
+                      "sample": "median of 3 single substeps of the OpenMP CSR form, x64"}
+        out["gpu_vs_cpu"] = out["gpu"]["value"] / cpu_aps
+    dt, T_ref = 1.0 / 64, 0.5
+    n_t = int(round(T_ref / dt))
+    eop, lam = ops.marching_operator(nchi, neta, 4, seed=12, n_lead=24, bulk=(-6.0, -3.0),
+                                     n_shock=nchi)
+    mu = sum((lam * dt) ** j / math.factorial(j) for j in range(6)) ** n_t
+    exact = np.log(mu[np.argsort(-np.abs(mu), kind="stable")][: 3 * k].astype(complex)) / (
+        n_t * dt)
+    out["eigen"] = {"k": k, "ncv": ncv, "T_ref": T_ref, "dt": dt}
+    for name, kw in (("device_krylov_schur", dict(krylov="device", tol=1e-9)),
+                     ("arpack", dict(krylov="arpack", tol=1e-4, v0=np.ones(eop.n)))):
+        sd = {"mesh": {"Nchi": nchi, "Neta": neta}, "time_integration": {"dt": dt},
+              "stability_analysis": {"perturb_shock": False,
+                                     "eigenvalue_solver": "GPU_TIMESTEPPER_ARNOLDI"}}
+        V, D = st.EIGENVALUES_GPU_ARNOLDI(eop, sd, k, ncv=ncv, **kw)
+        err = max(float(np.min(np.abs(exact - d)) / abs(d)) for d in D)
+        res = {"seconds": sd["timings"]["eigenvalues"], "tol": kw["tol"],
+               "max_rel_eig_err": err}
+        if "eigenvalues_krylov" in sd["timings"]:
+            res["applications"] = sd["timings"]["eigenvalues_krylov"].get("applications")
+        out["eigen"][name] = res
+        log(f"shock config: EIGENVALUES ({name}) {res}")
+    log(f"shock config: GPU {out['gpu']['value']:.1f} actions/s"
+        + (f", CPU {out['cpu']['value']:.3f} ({out['gpu_vs_cpu']:.0f}x)" if cpu else ""))
+    return out
+
+
 def small_config_leg(steps=20, order=12, nsub=64, T=1.0):
     """The launch-bound end: the same action T_12(A T/64)^64 v (768 Taylor
     terms) on the 100 x 80-cell, b = 4 operator (N = 32 000, 1.13M
@@ -635,6 +719,8 @@ def main():
                     help="skip the launch-bound 100x80-cell leg (eager / graph / no-alloc)")
     ap.add_argument("--no-growth", action="store_true",
                     help="skip the 400x300-cell transient-growth leg")
+    ap.add_argument("--no-shock", action="store_true",
+                    help="skip the 200x300-cell shock-band leg (action, 20-mode eigensolve)")
     ap.add_argument("--kernel-only",
                     choices=["plain", "batched", "block4", "receptivity", "receptivity16",
                              "krylov"],
@@ -791,6 +877,26 @@ def main():
         bk_ms, bk_bytes = devb.bench_term_kernel(reps=max(5, args.kernel_reps // 5), nv=nb,
                                                  form=FORM)
         devb.close()
+        # the same sweep with FP64 values: the batched form at FP64 accuracy
+        log(f"batched leg: FP64 values, {nb} vectors per sweep")
+        dev64 = DeviceOperator.from_blocks(op, value_prec="f64", with_adjoint=False)
+        dev64.set_batch_kernel(bkern)
+
+        def bstep64():
+            expm_taylor_batched_dev(dev64, dX, dY, order=order, nsub=nsub, dt=T / nsub,
+                                    form=FORM)
+
+        bstep64()
+        dev64.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(ksteps):
+            bstep64()
+        dev64.synchronize()
+        tb64 = time.perf_counter() - t0
+        y64 = dY.download()[:, 0]
+        err64 = float(np.linalg.norm(y64 - y_plain) / np.linalg.norm(y_plain))
+        dev64.close()
+        b64_aps = nb * ksteps / tb64
         b_aps = nb * ksteps / tb
         b_gbs = bk_bytes / (bk_ms * 1e-3) / 1e9
         # a Horner term writes its nb new vectors (n x nb doubles) beside what
@@ -837,8 +943,12 @@ def main():
             "frac_of_hbm_peak": b_gbs / HBM_PEAK_GBS,
             "same_mix_stream": {"kernel": "k_stream_mix", "write_share": b_write / bk_bytes,
                                 "GBps": mix["GBps"], "frac": b_gbs / mix["GBps"]},
+            "f64_values": {"value": b64_aps, "ms_per_sweep": tb64 / ksteps * 1e3,
+                           "speedup_vs_plain_fp64": b64_aps / actions_per_s,
+                           "rel_err_vs_plain_fp64_action": err64},
         }
-        log(f"batched: {b_aps:.2f} actions/s ({b_aps / actions_per_s:.2f}x plain), err {err:.2e}")
+        log(f"batched: {b_aps:.2f} actions/s ({b_aps / actions_per_s:.2f}x plain), err {err:.2e}; "
+            f"FP64 values {b64_aps:.2f} ({b64_aps / actions_per_s:.2f}x), err {err64:.2e}")
 
     result = {
         "metric": METRIC,
@@ -893,6 +1003,8 @@ def main():
         result["small_config"] = small_config_leg()
     if not args.no_growth:
         result["transient_growth"] = transient_growth_leg()
+    if not args.no_shock:
+        result["shock_config"] = shock_config_leg(cpu=not args.no_cpu)
     if not args.no_receptivity:
         result["batched_receptivity"] = receptivity_leg()
     if not args.no_eigen:
