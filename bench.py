@@ -261,9 +261,7 @@ def shock_config_leg(steps=5, order=12, nsub=64, T=1.0, k=20, ncv=80, cpu=True):
         on an operator of that shape and storage whose spectrum is exact
         (one-way coupling; its diagonal blocks' eigenvalues), with the
         worst relative eigenvalue error of each."""
-    import math
     from hymor_amd import operators as ops
-    from hymor_amd import stability as st
     from hymor_amd.core import DeviceOperator, DeviceVector, expm_taylor_dev
 
     nchi, neta = 200, 300
@@ -293,27 +291,51 @@ def shock_config_leg(steps=5, order=12, nsub=64, T=1.0, k=20, ncv=80, cpu=True):
                   "term_ms": k_ms, "term_GBps": k_bytes / (k_ms * 1e-3) / 1e9,
                   "note": "term: block kernel + the shock bands' extra rows (HIP events)"}
     if cpu:
-        from oracle import oracle
-        csr = oracle.CsrArrays(op.to_csr_cellmajor() * (T / nsub))
-        nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
-        oracle.taylor_csr_omp(csr, x, 1, order, nthreads)
-        t_omp = []
-        for _ in range(3):
-            t0 = time.perf_counter()
-            _, used = oracle.taylor_csr_omp(csr, x, 1, order, nthreads)
-            t_omp.append(time.perf_counter() - t0)
-        cpu_aps = 1.0 / (float(np.median(t_omp)) * nsub)
-        out["cpu"] = {"value": cpu_aps, "cores": int(used), "kind": "port",
-                      "sample": "median of 3 single substeps of the OpenMP CSR form, x64"}
-        out["gpu_vs_cpu"] = out["gpu"]["value"] / cpu_aps
+        out["cpu"] = _cpu_action_sample(op, x, order, nsub, T)
+        out["gpu_vs_cpu"] = out["gpu"]["value"] / out["cpu"]["value"]
+    out["eigen"] = _eigen_timings(nchi, neta, k, ncv, seed=12, n_lead=24, bulk=(-6.0, -3.0),
+                                  n_shock=nchi, tag="shock config")
+    log(f"shock config: GPU {out['gpu']['value']:.1f} actions/s"
+        + (f", CPU {out['cpu']['value']:.3f} ({out['gpu_vs_cpu']:.0f}x)" if cpu else ""))
+    return out
+
+
+def _cpu_action_sample(op, x, order, nsub, T, samples=3):
+    """The oracle's OpenMP CSR form of the action on the host: the median of
+    `samples` single substeps, scaled to actions/s."""
+    from oracle import oracle
+    csr = oracle.CsrArrays(op.to_csr_cellmajor() * (T / nsub))
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    oracle.taylor_csr_omp(csr, x, 1, order, nthreads)
+    t_omp = []
+    for _ in range(samples):
+        t0 = time.perf_counter()
+        _, used = oracle.taylor_csr_omp(csr, x, 1, order, nthreads)
+        t_omp.append(time.perf_counter() - t0)
+    return {"value": 1.0 / (float(np.median(t_omp)) * nsub), "cores": int(used), "kind": "port",
+            "sample": f"median of {samples} single substeps of the OpenMP CSR form, x{nsub}"}
+
+
+def _eigen_timings(nchi, neta, k, ncv, *, seed, n_lead, bulk, n_shock, tag):
+    """EIGENVALUES_GPU_ARNOLDI (T_ref = 0.5, dt = 1/64, order 5) for k modes
+    with Krylov dimension ncv through the device Krylov-Schur (tolerance
+    1e-9) and through ARPACK at the reference's tolerance 1e-4
+    (EIGENVALUES.jl:159), both applying the GPU action, on a b = 4 operator
+    of the given shape with an exactly known spectrum (one-way coupling:
+    its diagonal blocks' eigenvalues); seconds and the worst relative
+    eigenvalue error of each."""
+    import math
+    from hymor_amd import operators as ops
+    from hymor_amd import stability as st
+
     dt, T_ref = 1.0 / 64, 0.5
     n_t = int(round(T_ref / dt))
-    eop, lam = ops.marching_operator(nchi, neta, 4, seed=12, n_lead=24, bulk=(-6.0, -3.0),
-                                     n_shock=nchi)
+    eop, lam = ops.marching_operator(nchi, neta, 4, seed=seed, n_lead=n_lead, bulk=bulk,
+                                     n_shock=n_shock)
     mu = sum((lam * dt) ** j / math.factorial(j) for j in range(6)) ** n_t
     exact = np.log(mu[np.argsort(-np.abs(mu), kind="stable")][: 3 * k].astype(complex)) / (
         n_t * dt)
-    out["eigen"] = {"k": k, "ncv": ncv, "T_ref": T_ref, "dt": dt}
+    out = {"k": k, "ncv": ncv, "T_ref": T_ref, "dt": dt}
     for name, kw in (("device_krylov_schur", dict(krylov="device", tol=1e-9)),
                      ("arpack", dict(krylov="arpack", tol=1e-4, v0=np.ones(eop.n)))):
         sd = {"mesh": {"Nchi": nchi, "Neta": neta}, "time_integration": {"dt": dt},
@@ -325,14 +347,12 @@ def shock_config_leg(steps=5, order=12, nsub=64, T=1.0, k=20, ncv=80, cpu=True):
                "max_rel_eig_err": err}
         if "eigenvalues_krylov" in sd["timings"]:
             res["applications"] = sd["timings"]["eigenvalues_krylov"].get("applications")
-        out["eigen"][name] = res
-        log(f"shock config: EIGENVALUES ({name}) {res}")
-    log(f"shock config: GPU {out['gpu']['value']:.1f} actions/s"
-        + (f", CPU {out['cpu']['value']:.3f} ({out['gpu_vs_cpu']:.0f}x)" if cpu else ""))
+        out[name] = res
+        log(f"{tag}: EIGENVALUES ({name}) {res}")
     return out
 
 
-def small_config_leg(steps=20, order=12, nsub=64, T=1.0):
+def small_config_leg(steps=20, order=12, nsub=64, T=1.0, cpu=True):
     """The launch-bound end: the same action T_12(A T/64)^64 v (768 Taylor
     terms) on the 100 x 80-cell, b = 4 operator (N = 32 000, 1.13M
     nonzeros, 9 MB of values: cache-resident), FP64, vectors resident on
@@ -340,7 +360,10 @@ def small_config_leg(steps=20, order=12, nsub=64, T=1.0):
     HIP graph, and through the no-alloc entry point (kernel by kernel on
     the library's static scratch); plus the term kernel's own event-timed
     duration, so that 768 x that duration against the action's time shows
-    the launch cost."""
+    the launch cost; the host's OpenMP CSR form of the same action (the
+    size the reference runs on the CPU); and EIGENVALUES_GPU_ARNOLDI for 10
+    modes with Krylov dimension 60 on an operator of that shape with an
+    exact spectrum."""
     import ctypes
 
     from hymor_amd import _lib
@@ -402,6 +425,13 @@ def small_config_leg(steps=20, order=12, nsub=64, T=1.0):
     log(f"small config: eager {1 / t_eager:.1f}, graph {1 / t_graph:.1f}, noalloc "
         f"{1 / t_na:.1f} actions/s; term kernel {k_ms * 1e3:.2f} us "
         f"({out['term_kernel_GBps']:.0f} GB/s)")
+    if cpu:
+        out["cpu"] = _cpu_action_sample(op, x, order, nsub, T, samples=5)
+        out["graph_vs_cpu"] = out["graph"]["value"] / out["cpu"]["value"]
+        log(f"small config: CPU {out['cpu']['value']:.2f} actions/s "
+            f"(graph replay {out['graph_vs_cpu']:.0f}x)")
+    out["eigen"] = _eigen_timings(100, 80, 10, 60, seed=11, n_lead=12, bulk=(-3.0, -1.0),
+                                  n_shock=0, tag="small config")
     return out
 
 
@@ -1000,7 +1030,7 @@ def main():
         "batched": batched,
     }
     if not args.no_small:
-        result["small_config"] = small_config_leg()
+        result["small_config"] = small_config_leg(cpu=not args.no_cpu)
     if not args.no_growth:
         result["transient_growth"] = transient_growth_leg()
     if not args.no_shock:
