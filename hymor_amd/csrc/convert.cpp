@@ -670,4 +670,75 @@ std::vector<int64_t> xcd_balanced_order(const std::vector<int64_t> &len) {
     return order;
 }
 
+
+bool build_shock_fuse(const HostExtra &x, int64_t nflow, HostExtra &rows_out,
+                      HostShockPanels &pan) {
+    rows_out = HostExtra();
+    pan = HostShockPanels();
+    if (x.is_complex || nflow <= 0) return false;
+    const int64_t nr = (int64_t)x.rows.size();
+    int64_t nflow_entries = 0;
+    for (int64_t i = 0; i < nr; ++i) {
+        const int64_t row = x.rows[(size_t)i];
+        if (row >= nflow) continue;
+        for (int64_t e = x.ptr[(size_t)i]; e < x.ptr[(size_t)i + 1]; ++e)
+            if (x.col[(size_t)e] < nflow) return false;
+        nflow_entries += x.ptr[(size_t)i + 1] - x.ptr[(size_t)i];
+    }
+    // the extra-DOF rows, in their order
+    HostExtra r;
+    r.is_complex = false;
+    r.ptr.assign(1, 0);
+    for (int64_t i = 0; i < nr; ++i) {
+        const int64_t row = x.rows[(size_t)i];
+        if (row < nflow) continue;
+        for (int64_t e = x.ptr[(size_t)i]; e < x.ptr[(size_t)i + 1]; ++e) {
+            r.col.push_back(x.col[(size_t)e]);
+            r.val.push_back(x.val[(size_t)e]);
+        }
+        r.rows.push_back(row);
+        r.ptr.push_back((int64_t)r.col.size());
+    }
+    if (nflow_entries == 0 && r.rows.empty()) return false;
+    // panels: the listed flow rows grouped by slice of 64 rows (ascending)
+    const int64_t nslice = (nflow + 63) / 64;
+    HostShockPanels pp;
+    pp.slice_tab.assign((size_t)nslice, -1);
+    pp.cptr.assign(1, 0);
+    int64_t i = 0;
+    while (i < nr && x.rows[(size_t)i] < nflow) {
+        const int64_t sl = x.rows[(size_t)i] / 64;
+        int64_t j = i;
+        std::vector<int32_t> cols;
+        while (j < nr && x.rows[(size_t)j] < nflow && x.rows[(size_t)j] / 64 == sl) {
+            for (int64_t e = x.ptr[(size_t)j]; e < x.ptr[(size_t)j + 1]; ++e)
+                cols.push_back(x.col[(size_t)e]);
+            ++j;
+        }
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+        if (!cols.empty()) {
+            const int64_t base = (int64_t)pp.pcol.size();
+            pp.slice_tab[(size_t)sl] = (int32_t)(pp.cptr.size() - 1);
+            pp.pcol.insert(pp.pcol.end(), cols.begin(), cols.end());
+            pp.pval.resize(pp.pcol.size() * 64, 0.0);
+            for (int64_t k = i; k < j; ++k) {
+                const int lane = (int)(x.rows[(size_t)k] - 64 * sl);
+                for (int64_t e = x.ptr[(size_t)k]; e < x.ptr[(size_t)k + 1]; ++e) {
+                    const int64_t pos =
+                        std::lower_bound(cols.begin(), cols.end(), x.col[(size_t)e]) - cols.begin();
+                    pp.pval[(size_t)((base + pos) * 64 + lane)] += x.val[(size_t)e];
+                }
+            }
+            pp.cptr.push_back((int64_t)pp.pcol.size());
+            if ((int64_t)pp.pcol.size() >= (int64_t)INT32_MAX) return false;
+        }
+        i = j;
+    }
+    if ((int64_t)pp.pval.size() > 8 * nflow_entries) return false;
+    rows_out = std::move(r);
+    pan = std::move(pp);
+    return true;
+}
+
 }  // namespace hymor

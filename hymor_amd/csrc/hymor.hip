@@ -103,6 +103,19 @@ struct DevExtra {
     double *fr_sum[2] = {nullptr, nullptr};
 };
 
+// The shock bands in the single-vector term kernel's launch
+// (build_shock_fuse / ShockFuseDev): the extra-DOF rows and the slice panels.
+struct DevShock {
+    bool on = false;
+    DevExtra rows;
+    int64_t npan = 0;
+    int32_t *pslices = nullptr;   // the slices with a panel, ascending
+    int32_t *slice_tab = nullptr, *pcol = nullptr;
+    int64_t *cptr = nullptr;
+    double *pval = nullptr;
+    int64_t npcol = 0;
+};
+
 }  // namespace
 
 struct hymor_op {
@@ -118,6 +131,10 @@ struct hymor_op {
     // built from one per-wavenumber system (hymor_op_create_receptivity):
     // only the compressed extras exist, used at every width
     bool rep_only = false;
+    // the extra rows of a real operator whose flow rows couple only to
+    // extra DOFs, in the single-vector term kernel's launch (forward,
+    // adjoint); DevShock::on when found
+    DevShock shf, sha;
     hipStream_t stream = nullptr;
     int device = 0;
     // workspace: [0] stage (var-major), [1] in0, [2] va, [3] vb, [4] t0, [5] t1, [6] spare
@@ -398,6 +415,27 @@ void upload_fused(hymor_op *op, const HostExtra &h, const HostReplica &rep, DevE
     d.fused = true;
 }
 
+// The shock bands' fused form (build_shock_fuse) of the extra CSR h, when
+// the operator has it.
+void upload_shock(hymor_op *op, const HostExtra &h, DevShock &d) {
+    HostExtra rows;
+    HostShockPanels pan;
+    d = DevShock();
+    if (!build_shock_fuse(h, op->nflow, rows, pan)) return;
+    upload_extra(op, rows, d.rows);
+    d.slice_tab = upload(op, pan.slice_tab);
+    d.cptr = upload(op, pan.cptr);
+    d.pcol = upload(op, pan.pcol);
+    d.pval = upload(op, pan.pval);
+    d.npcol = (int64_t)pan.pcol.size();
+    std::vector<int32_t> ps;
+    for (size_t k = 0; k < pan.slice_tab.size(); ++k)
+        if (pan.slice_tab[k] >= 0) ps.push_back((int32_t)k);
+    d.npan = (int64_t)ps.size();
+    d.pslices = upload(op, ps);
+    d.on = true;
+}
+
 // Compresses the replicated freestream blocks of a receptivity operator in
 // both directions' extra CSRs (xa may be null: no adjoint).
 void find_replicas(int64_t n, HostExtra &xf, HostExtra *xa, HostReplica &rf, HostReplica &ra) {
@@ -533,7 +571,64 @@ void launch_split_p(const DevSliced &s, hipStream_t st, const double *x,
     constexpr int S = (9 + NW - 1) / NW;
     hipLaunchKernelGGL((k_term_split_p<B, VT, NV, NW, S, NT>), dim3((unsigned)s.nslice),
                        dim3(64 * NW), 0, st, s.slice_ptr, s.col, (const VT *)s.val, s.nslice,
-                       s.ncell, x, tout, vin, vout, c1, c2);
+                       s.ncell, x, tout, vin, vout, c1, c2, ShockFuseDev{});
+}
+
+template <typename T>
+struct TypeTag {
+    using type = T;
+};
+
+// One real vector's term with the shock bands in the same launch
+// (k_term_split_p with SH): the default single-vector tuning's kernel, three
+// waves per slice, plus one leading workgroup per extra-DOF row.
+template <int B, typename VT, bool NT>
+void launch_shock_t(const DevSliced &s, const DevShock &sh, int64_t nflow, hipStream_t st,
+                    const double *x, double *tout, const double *vin, double *vout, double c1,
+                    double c2) {
+    ShockFuseDev F;
+    F.nrows = sh.rows.nrows;
+    F.nsh_wg = (F.nrows + 7) / 8 * 8;
+    F.nflow = nflow;
+    F.rows = sh.rows.rows;
+    F.ptr = sh.rows.ptr;
+    F.col = sh.rows.col;
+    F.val = sh.rows.val;
+    F.slice_tab = sh.slice_tab;
+    F.cptr = sh.cptr;
+    F.pcol = sh.pcol;
+    F.pval = sh.pval;
+    F.npan = sh.npan;
+    F.npan_wg = (sh.npan + 7) / 8 * 8;
+    F.pslices = sh.pslices;
+    hipLaunchKernelGGL((k_term_split_p<B, VT, 1, 3, 3, NT, true>),
+                       dim3((unsigned)(F.nsh_wg + F.npan_wg + s.nslice)), dim3(64 * 3), 0, st,
+                       s.slice_ptr,
+                       s.col, (const VT *)s.val, s.nslice, s.ncell, x, tout, vin, vout, c1, c2,
+                       F);
+}
+
+void launch_shock(const DevSliced &s, const DevShock &sh, int64_t nflow, hipStream_t st,
+                  const double *x, double *tout, const double *vin, double *vout, double c1,
+                  double c2) {
+    const bool f32 = s.prec == HYMOR_PREC_F32, nt = s.tune.nt != 0;
+    auto go = [&](auto b, auto vt, auto ntc) {
+        constexpr int B = decltype(b)::value;
+        using VT = typename decltype(vt)::type;
+        constexpr bool NT = decltype(ntc)::value;
+        launch_shock_t<B, VT, NT>(s, sh, nflow, st, x, tout, vin, vout, c1, c2);
+    };
+    auto by_nt = [&](auto b, auto vt) {
+        if (nt) go(b, vt, std::true_type{});
+        else go(b, vt, std::false_type{});
+    };
+    auto by_prec = [&](auto b) {
+        if (f32) by_nt(b, TypeTag<float>{});
+        else by_nt(b, TypeTag<double>{});
+    };
+    if (s.b == 4) by_prec(std::integral_constant<int, 4>{});
+    else if (s.b == 8) by_prec(std::integral_constant<int, 8>{});
+    else throw ArgError("vars_per_cell must be 4 or 8");
 }
 
 // split == 3: k_term_lds, as split 2 with the x blocks staged through LDS
@@ -901,6 +996,15 @@ void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, const double *x
         return;
     }
     const DevSliced &s = adj ? op->adj : op->fwd;
+    // one real vector under the default tuning (three waves per slice):
+    // the shock bands in the term kernel's own launch
+    const DevShock &sh = adj ? op->sha : op->shf;
+    if (nv == 1 && sh.on && !op->cdiag_on && op->ncell > 0 && s.tune.split >= 2 &&
+        s.tune.waves != 9) {
+        launch_shock(s, sh, op->nflow, st, x, tout, vin, vout, c1, c2);
+        HIP_TRY(hipGetLastError());
+        return;
+    }
     const DevExtra &xe = extra_of(op, adj, nv);
     if (op->ncell > 0) {
         launch_block(s, nv, st, x, tout, vin, vout, c1, c2);
@@ -1390,6 +1494,10 @@ int hymor_op_create_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
         upload_replicated(op, xfc, rf, op->xf_rep);
         upload_fused(op, xfc, rf, op->xf_rep);
         upload_replicated(op, xac, ra, op->xa_rep);
+        if (rf.kind == 0) {
+            upload_shock(op, xf, op->shf);
+            upload_shock(op, xa, op->sha);
+        }
         op->has_adj = true;
         *out = op;
     });
@@ -1518,6 +1626,7 @@ int hymor_op_create_bsr(int64_t nchi, int64_t neta, int vars_per_cell, int64_t n
         upload_extra(op, xf, op->xf);
         upload_replicated(op, xfc, rf, op->xf_rep);
         upload_fused(op, xfc, rf, op->xf_rep);
+        if (rf.kind == 0) upload_shock(op, xf, op->shf);
         if (with_adjoint) {
             std::vector<int64_t> tptr, tsrc;
             std::vector<int32_t> tcol;
@@ -1526,6 +1635,7 @@ int hymor_op_create_bsr(int64_t nchi, int64_t neta, int vars_per_cell, int64_t n
                          value_prec, op->adj);
             upload_extra(op, xa, op->xa);
             upload_replicated(op, xac, ra, op->xa_rep);
+            if (rf.kind == 0) upload_shock(op, xa, op->sha);
             op->has_adj = true;
         }
         *out = op;

@@ -122,6 +122,31 @@ void receptivity_from_parts(int64_t n_l, int64_t p, int64_t nwave, const double 
 // XCD.  Returns new position -> old index.
 std::vector<int64_t> xcd_balanced_order(const std::vector<int64_t> &len);
 
+// The extra entries of a real operator whose flow rows couple only to
+// extra DOFs (the shock bands: A12 in the forward operator, A21' in the
+// adjoint) in the form the single-vector term kernel takes them in its own
+// launch: the extra-DOF rows (rows >= nflow) as their own CSR, finished by
+// leading workgroups, and, per slice of 64 rows holding flow-row entries, a
+// dense panel: the slice's extra columns (sorted) with 64 values each, one
+// per row of the slice (zero where a row has no entry), added by the
+// slice's workgroup.  slice_tab[s] is the panel of slice s or -1; panel k
+// owns columns [cptr[k], cptr[k+1]) of pcol and values pval[64 c + lane]
+// for those columns c.
+struct HostShockPanels {
+    std::vector<int32_t> slice_tab;
+    std::vector<int64_t> cptr;
+    std::vector<int32_t> pcol;
+    std::vector<double> pval;
+};
+
+// Builds the panels and the extra-DOF rows from the extra CSR x of an
+// operator with nflow flow rows; false (outputs empty) when x is complex,
+// when a flow row has an extra entry in a flow column, when there is nothing
+// to fuse, or when the panels would hold more than 8 values per entry they
+// replace (rows of a slice with little overlap in their columns).
+bool build_shock_fuse(const HostExtra &x, int64_t nflow, HostExtra &rows_out,
+                      HostShockPanels &pan);
+
 void csc_to_blocks(int64_t n, const int64_t *colptr, const int64_t *rowval,
                    const double *nzval, int index_base, bool is_complex, double scale,
                    int b, int64_t ncell, HostBSR &fwd, HostBSR &adj,

@@ -1173,6 +1173,32 @@ __global__ __launch_bounds__(64 * NW) void k_term_split(
     }
 }
 
+// The extra entries of a real operator whose flow rows couple only to
+// extra DOFs (the shock bands; HostShockPanels) fused into the
+// single-vector term kernel.  Workgroups in launch order:
+//   [0, nsh_wg): the nrows extra-DOF rows, one workgroup per row
+//     (extra_long_row);
+//   the next npan_wg: the npan slices with a panel (pslices), one each,
+//     dealt round-robin over the XCDs: their block term plus the panel, for
+//     every column c of it pval[64 c + lane] * x[pcol[c]] on row lane;
+//   the rest: the block term of every other slice, in xcd_tile order (a
+//     slice with a panel returns at once).
+// nsh_wg and npan_wg are multiples of 8, so the last group keeps the XCD
+// mapping of an unfused launch.  The leading groups' work lasts longer per
+// workgroup and starts first, under the bulk of the block term.
+struct ShockFuseDev {
+    int64_t nsh_wg = 0, nrows = 0, nflow = 0;
+    int64_t npan_wg = 0, npan = 0;
+    const int32_t *pslices = nullptr;
+    const int64_t *rows = nullptr, *ptr = nullptr;
+    const int32_t *col = nullptr;
+    const double *val = nullptr;
+    const int32_t *slice_tab = nullptr;
+    const int64_t *cptr = nullptr;
+    const int32_t *pcol = nullptr;
+    const double *pval = nullptr;
+};
+
 // k_term_split with the loads of each wave grouped: the block columns of
 // the wave's next S slots and their values are issued together, then the x
 // entries they select, and wave 0 issues its epilogue operand (vin)
@@ -1191,21 +1217,51 @@ struct Chunk<float> {
     typedef f4 type;
 };
 
-template <int B, typename VT, int NV, int NW, int S, bool NT>
+//
+// SH: the shock bands fused in (ShockFuseDev; one real vector): no k_extra
+// launch follows.
+template <int B, typename VT, int NV, int NW, int S, bool NT, bool SH = false>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_term_split_p(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ val, int64_t nslice, int64_t ncell,
     const double *__restrict__ x,
-    double *__restrict__ tout, const double *vin, double *vout, double c1, double c2) {
+    double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
+    ShockFuseDev F) {
     constexpr int CPS = 64 / B;
     constexpr int BB = B * B;
     constexpr int V = Vpl<VT>::value;
     constexpr int NC = B / V;  // 16-byte chunks per lane and slot
     typedef typename Chunk<VT>::type CT;
+    static_assert(!SH || NV == 1, "the shock bands act on one real vector");
     __shared__ double part[NW > 1 ? NW - 1 : 1][NV][64];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    const int64_t slice = xcd_tile(blockIdx.x, gridDim.x);
+    int64_t bid = blockIdx.x, nwg = gridDim.x;
+    if constexpr (SH) {
+        // the extra-DOF rows first: they read only x, so they overlap the
+        // block term instead of following it
+        if (bid < F.nsh_wg) {
+            if (bid < F.nrows)
+                extra_long_row<NV, false, NW, false, false>(bid, F.rows, F.ptr, F.col, F.val,
+                                                           F.nflow, x, tout, vin, vout, c1, c2,
+                                                           nullptr, 0, 0, 0, ReplicaDev{});
+            return;
+        }
+        bid -= F.nsh_wg;
+        nwg -= F.nsh_wg;
+    }
+    int64_t slice;
+    if constexpr (SH) {
+        if (bid < F.npan_wg) {
+            if (bid >= F.npan) return;
+            slice = F.pslices[bid];
+        } else {
+            slice = xcd_tile(bid - F.npan_wg, nwg - F.npan_wg);
+            if (slice < nslice && F.slice_tab[slice] >= 0) return;   // a leading workgroup's
+        }
+    } else {
+        slice = xcd_tile(bid, nwg);
+    }
     if (slice >= nslice) return;
     const int q = lane / B, r = lane % B;
     const int64_t p0 = slice_ptr[slice];
@@ -1263,6 +1319,28 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
 #pragma unroll
                     for (int v = 0; v < NV; ++v) acc[v] = fma(ak, xv[u][k * NV + v], acc[v]);
                 }
+        }
+    }
+    if constexpr (SH) {
+        // the slice's panel: wave w takes columns w, w + NW, ..., PG at a
+        // time (values and x loads first; the column and its x entry are
+        // the same for every lane)
+        const int32_t pk = F.slice_tab[slice];
+        if (pk >= 0) {
+            constexpr int PG = 16;
+            const int64_t ce = F.cptr[pk + 1];
+            for (int64_t cb = F.cptr[pk] + w; cb < ce; cb += (int64_t)NW * PG) {
+                double a[PG], xe[PG];
+#pragma unroll
+                for (int g = 0; g < PG; ++g) {
+                    const int64_t c = cb + (int64_t)g * NW < ce ? cb + (int64_t)g * NW : cb;
+                    a[g] = F.pval[c * 64 + lane];
+                    xe[g] = x[F.pcol[c]];
+                }
+#pragma unroll
+                for (int g = 0; g < PG; ++g)
+                    if (cb + (int64_t)g * NW < ce) acc[0] = fma(a[g], xe[g], acc[0]);
+            }
         }
     }
     if constexpr (NW > 1) {

@@ -1298,3 +1298,75 @@ extern "C" void emu_xcd_balanced_order(int64_t m, const int64_t *len, int64_t *o
     const std::vector<int64_t> o = xcd_balanced_order(l);
     for (int64_t i = 0; i < m; ++i) out[i] = o[(size_t)i];
 }
+
+// One real SpMV through the shock bands' fused form (build_shock_fuse, as
+// k_term_split_p<..., SH> takes it): the block term per slice lane, plus the
+// slice's panel (columns pcol[cptr[k] .. cptr[k+1]), values pval[64 c +
+// lane]), and the extra-DOF rows (their CSR).  info[0] = 1 if the fused
+// form was built, info[1] = panels, info[2] = panel columns, info[3] =
+// extra-DOF rows.  Returns the out-of-bounds access count.
+extern "C" int emu_spmv_shock(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                              const double *nzval, int base, int b, int64_t ncell, int transpose,
+                              const double *x_vm, double *y_vm, int64_t *info) {
+    g_oob = 0;
+    HostBSR fwd, adj;
+    HostExtra xf, xa;
+    csc_to_blocks(n, colptr, rowval, nzval, base, false, 1.0, b, ncell, fwd, adj, xf, xa);
+    const HostBSR &A = transpose ? adj : fwd;
+    const HostExtra &X = transpose ? xa : xf;
+    const int64_t nflow = (int64_t)b * ncell;
+    HostExtra rows;
+    HostShockPanels pan;
+    const bool ok = build_shock_fuse(X, nflow, rows, pan);
+    info[0] = ok ? 1 : 0;
+    info[1] = ok ? (int64_t)pan.cptr.size() - 1 : 0;
+    info[2] = ok ? (int64_t)pan.pcol.size() : 0;
+    info[3] = ok ? (int64_t)rows.rows.size() : 0;
+    if (!ok) return 0;
+    HostSliced s;
+    bsr_to_sliced(A, s);
+    interleave_lanes(s, 2);
+    std::vector<double> x((size_t)n), y((size_t)n, 0.0);
+    for (int64_t row = 0; row < n; ++row) {
+        const int64_t vm = row < nflow ? (row % b) * ncell + row / b : row;
+        x[(size_t)row] = x_vm[vm];
+    }
+    // extra-DOF rows (leading workgroups)
+    for (int64_t i = 0; i < (int64_t)rows.rows.size(); ++i) {
+        double acc = 0.0;
+        for (int64_t e = at(rows.ptr, i); e < at(rows.ptr, i + 1); ++e)
+            acc += at(rows.val, e) * at(x, at(rows.col, e));
+        const int64_t row = at(rows.rows, i);
+        if (row < nflow || row >= n) ++g_oob;
+        else y[(size_t)row] = acc;
+    }
+    // slices: block term, then the panel
+    const int CPS = 64 / b, BB = b * b;
+    if ((int64_t)pan.slice_tab.size() != s.nslice) ++g_oob;
+    for (int64_t slice = 0; slice < s.nslice; ++slice) {
+        const int64_t p0 = at(s.slice_ptr, slice);
+        const int64_t width = (at(s.slice_ptr, slice + 1) - p0) / CPS;
+        const int32_t pk = at(pan.slice_tab, slice);
+        for (int lane = 0; lane < 64; ++lane) {
+            const int q = lane / b, r = lane % b;
+            double acc = 0.0;
+            for (int64_t t = 0; t < width; ++t) {
+                const int64_t c = at(s.col, p0 + q + t * CPS);
+                const int64_t sb = (p0 + t * CPS) * BB;
+                for (int k = 0; k < b; ++k)
+                    acc += at(s.val, sb + ((k / 2) * 64 + lane) * 2 + k % 2) * at(x, c * b + k);
+            }
+            if (pk >= 0)
+                for (int64_t c = at(pan.cptr, pk); c < at(pan.cptr, pk + 1); ++c)
+                    acc += at(pan.pval, c * 64 + lane) * at(x, at(pan.pcol, c));
+            const int64_t cell = slice * CPS + q;
+            if (cell >= ncell) continue;
+            y[(size_t)(cell * b + r)] = acc;
+        }
+    }
+    for (int64_t vm = 0; vm < n; ++vm) {
+        const int64_t row = vm < nflow ? (vm % ncell) * b + vm / ncell : vm;
+        y_vm[vm] = y[(size_t)row];
+    }
+    return g_oob;
+}

@@ -219,6 +219,51 @@ def test_long_extra_rows_spread_over_xcds(emu, case):
     assert loads.max() < before.max()
 
 
+@pytest.mark.parametrize("nchi,neta,b", [(10, 7, 4), (6, 5, 8), (9, 4, 4), (40, 5, 4)])
+@pytest.mark.parametrize("transpose", [False, True])
+def test_emulated_shock_bands_fused(emu, nchi, neta, b, transpose):
+    """The shock bands in the single-vector term kernel's launch: the
+    extra-DOF rows as their own CSR and, per slice of 64 rows, a dense panel
+    of the flow rows' extra entries over the slice's extra columns.  Built
+    for operators with shock bands (forward: A12 rows; adjoint: A21' rows),
+    and the replayed SpMV (block term + panel per slice lane + extra-DOF
+    rows, every access bounds-checked) equals scipy's."""
+    op = ops.stencil_operator(nchi, neta, b, seed=80 + nchi, n_shock=nchi)
+    colptr, rowval, nzval = op.to_csc_varmajor(one_based=True)
+    A = op.to_scipy_varmajor()
+    x = np.random.default_rng(nchi).standard_normal(op.n)
+    y = np.zeros_like(x)
+    info = np.zeros(4, dtype=np.int64)
+    emu.emu_spmv_shock.restype = C.c_int
+    oob = emu.emu_spmv_shock(C.c_int64(op.n), C.c_void_p(colptr.ctypes.data),
+                             C.c_void_p(rowval.ctypes.data),
+                             C.c_void_p(np.ascontiguousarray(nzval).ctypes.data), C.c_int(1),
+                             C.c_int(b), C.c_int64(op.ncell), C.c_int(int(transpose)),
+                             C.c_void_p(x.ctypes.data), C.c_void_p(y.ctypes.data),
+                             C.c_void_p(info.ctypes.data))
+    assert oob == 0
+    assert info[0] == 1 and info[3] == nchi and info[1] >= 1
+    ref = (A.T if transpose else A) @ x
+    assert np.allclose(y, ref, rtol=1e-13, atol=1e-13)
+
+
+def test_emulated_shock_fuse_declines_without_extras(emu):
+    """An operator without extra DOFs has nothing to fuse: no fused form."""
+    op = ops.stencil_operator(6, 4, 4, seed=90)
+    colptr, rowval, nzval = op.to_csc_varmajor(one_based=True)
+    x = np.random.default_rng(1).standard_normal(op.n)
+    y = np.zeros_like(x)
+    info = np.zeros(4, dtype=np.int64)
+    emu.emu_spmv_shock.restype = C.c_int
+    assert emu.emu_spmv_shock(C.c_int64(op.n), C.c_void_p(colptr.ctypes.data),
+                              C.c_void_p(rowval.ctypes.data),
+                              C.c_void_p(np.ascontiguousarray(nzval).ctypes.data), C.c_int(1),
+                              C.c_int(4), C.c_int64(op.ncell), C.c_int(0),
+                              C.c_void_p(x.ctypes.data), C.c_void_p(y.ctypes.data),
+                              C.c_void_p(info.ctypes.data)) == 0
+    assert info[0] == 0
+
+
 def _run_fused(emu, A, b, ncell, X2, order, nsub, dt, nv=2):
     """emu_horner_fused on scipy matrix A (var-major) with nv doubles per
     row: X2 is n x nv/2 complex values (or n complex values at nv = 2) or
