@@ -550,6 +550,81 @@ def LINEAR_INTEGRATION_AND_ENERGY_GPU(x, n_t, N_samples, N_skip, dt, t_0, orderT
     return (t, *E, Vf)
 
 
+def LINEAR_INTEGRATION_AND_GAINS(freestream_disturbances, v, lam, T_opt, A_, s, chemistry, T_f,
+                                 get_amplification_only, w_infty):
+    """Energy gains of a perturbation integrated forward to T_f
+    (Freestream_receptivity/LINEAR_INTEGRATION_AND_GAINS.jl:45-366): the Chu
+    weights of the total, acoustic, kinetic and entropic energies (:66-114),
+    one energy-tracked integration of v under T_5(A_ dt) with dt = T_f /
+    round(T_f / dt_CFL) (:53-58, :141-145), the initial energies (:169-174),
+    the scalings (:180-187: E(0), or for freestream disturbances the inflow
+    energy rate v' D v times GET_T_REF and times t) and the returned
+    max_gain dict (:341-363): "temporal" -- max over t >= t_10 of
+    E(t) / scaling(t) -- and "non_temporal" -- max of E / scaling_ref --
+    for "all", "acoustic", "kinetic", "entropic".
+
+    Only the amplification path (get_amplification_only = True) is
+    provided: the energy budgets of the other path (COMPUTE_BUDGETS_KINETIC,
+    _ENTROPY, _SHOCK, ENERGY_INFLOW, :204-339) are flux and chemistry
+    post-processing of the nonlinear solver, outside this engine; the
+    figures are not drawn.  lam, T_opt and chemistry are accepted for the
+    reference's signature (they feed only those parts)."""
+    if not get_amplification_only:
+        raise NotImplementedError(
+            "LINEAR_INTEGRATION_AND_GAINS: the energy budgets (get_amplification_only=false) need "
+            "the nonlinear solver's flux and chemistry post-processing (COMPUTE_BUDGETS_*), "
+            "which this engine does not include; call it with get_amplification_only=true")
+    orderTaylor = ORDER
+    s = CFL_TIMESTEP(s)
+    n_t = int(round(T_f / s["time_integration"]["dt"]))
+    dt = T_f / n_t
+    one, p_only, k_only, s_only = [1, 1, 1, 1], [1, 0, 0, 0], [0, 1, 1, 0], [0, 0, 0, 1]
+    if freestream_disturbances:
+        R_ = CONSTRUCT_R_(s, w_infty)
+        quad = lambda M: (R_.T @ M @ R_).tocsc()
+        C_all = quad(CONSTRUCT_M_(s, one, w_infty))
+        D_mat = quad(CONSTRUCT_M_INFTY_(s, one, 1, w_infty, False))
+        C_p = quad(CONSTRUCT_M_(s, p_only, w_infty))
+        C_k = quad(CONSTRUCT_M_(s, k_only, w_infty))
+        C_S = quad(CONSTRUCT_M_(s, s_only, w_infty))
+    else:
+        R = CONSTRUCT_R(s)
+        quad = lambda M: (R.T @ M @ R).tocsc()
+        C_all = quad(CONSTRUCT_M(s, one))
+        D_mat = quad(CONSTRUCT_M_HAT(s, one))
+        C_p = quad(CONSTRUCT_M(s, p_only))
+        C_k = quad(CONSTRUCT_M(s, k_only))
+        C_S = quad(CONSTRUCT_M(s, s_only))
+    cplx = freestream_disturbances or np.iscomplexobj(v) or np.iscomplexobj(getattr(A_, "data", 0))
+    devA = upload(sp.csc_matrix(A_, dtype=np.complex128 if cplx else np.float64), s, scale=dt)
+    try:
+        t, E_all, E_p, E_k, E_S = LINEAR_INTEGRATION_AND_ENERGY_GPU_NO_SAMPLES(
+            np.asarray(v, dtype=np.complex128 if cplx else np.float64), n_t, dt, 0.0, orderTaylor,
+            devA, C_all, C_p, C_k, C_S)
+    finally:
+        devA.close()
+    v_real = np.real(np.asarray(v))
+    for E, C in ((E_all, C_all), (E_p, C_p), (E_k, C_k), (E_S, C_S)):   # :169-174
+        E[0] = float(v_real @ (C @ v_real))
+    if freestream_disturbances:                                           # :180-183
+        vv = np.asarray(v)
+        dE_dt_inflow_time = float(np.real(np.vdot(vv, D_mat @ vv)))
+        scaling_E_ref = dE_dt_inflow_time * GET_T_REF(s)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            scaling_E = dE_dt_inflow_time * t
+    else:                                                                 # :184-186
+        scaling_E_ref = E_all[0]
+        scaling_E = E_all[0]
+    series = {"all": E_all, "acoustic": E_p, "kinetic": E_k, "entropic": E_S}
+    start = 9                                          # start_index = 10 (1-based), :342
+    sc_t = scaling_E[start:] if np.ndim(scaling_E) else scaling_E
+    max_gain = {"temporal": {k: float(np.max(np.real(E[start:]) / sc_t)) for k, E in series.items()},
+                "non_temporal": {k: float(np.max(np.real(E) / scaling_E_ref))
+                                 for k, E in series.items()}}
+    s.setdefault("timings", {})["linear_integration_steps"] = n_t
+    return max_gain
+
+
 def _energy_run(x, n_t, dt, t_0, order, Adt, mats, weights, n_skip, n_samples):
     own = weights is None
     if own:

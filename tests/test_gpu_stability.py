@@ -326,3 +326,55 @@ def test_batched_fp32_eigs_with_fp64_refinement(orc):
     assert err(D1) > 1e-10            # FP32-stored values alone do not reach FP64 accuracy
     V2, D2, _ = st.batched_exp_eigs(A, s, k, block=16, tol=1e-12, value_prec="f64")
     assert err(D2) <= 1e-8
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_linear_integration_and_gains(orc, fs):
+    """LINEAR_INTEGRATION_AND_GAINS' max_gain dict (temporal and
+    non-temporal maxima of the total, acoustic, kinetic and entropic
+    energies) against the same quantities from the oracle's POWER_METHOD!
+    stepped one substep at a time and the Chu weights applied on the host
+    (1e-12), with and without freestream disturbances; the budget path
+    reports that it is not provided."""
+    nchi, neta = 6, 5
+    say(f"[LINEAR_INTEGRATION_AND_GAINS] freestream={fs}")
+    s = st.CFL_TIMESTEP(synthetic_solution(nchi, neta, seed=5, perturb_shock=True))
+    dt_cfl = s["time_integration"]["dt"]
+    base = ops.stencil_operator(nchi, neta, 4, seed=9, n_shock=nchi, row_sum=0.5 / dt_cfl)
+    w = [0.0, 1.0, 2.5]
+    A = (ops.receptivity_extension(base, w) if fs else base).to_scipy_varmajor()
+    n = A.shape[0]
+    r = np.random.default_rng(12)
+    v = r.standard_normal(n) + (1j * r.standard_normal(n) if fs else 0)
+    T_f = 37.3 * dt_cfl
+    mg = st.LINEAR_INTEGRATION_AND_GAINS(fs, v, 1.0, 1.0, A, s, {}, T_f, True, w)
+    n_t = int(round(T_f / dt_cfl))
+    dt = T_f / n_t
+    one = [1, 1, 1, 1]
+    parts = {"all": one, "acoustic": [1, 0, 0, 0], "kinetic": [0, 1, 1, 0],
+             "entropic": [0, 0, 0, 1]}
+    if fs:
+        R = st.CONSTRUCT_R_(s, w)
+        C = {k: R.T @ st.CONSTRUCT_M_(s, nm, w) @ R for k, nm in parts.items()}
+        Dm = R.T @ st.CONSTRUCT_M_INFTY_(s, one, 1, w, False) @ R
+    else:
+        R = st.CONSTRUCT_R(s)
+        C = {k: R.T @ st.CONSTRUCT_M(s, nm) @ R for k, nm in parts.items()}
+    Ao = orc.CSC.from_scipy(A * dt)
+    xs = [np.asarray(v)]
+    for _ in range(n_t):
+        xs.append(orc.power_method(Ao, xs[-1], 1))
+    E = {k: np.array([np.real(x) @ (Ck @ np.real(x)) for x in xs]) for k, Ck in C.items()}
+    t = dt * np.arange(n_t + 1)
+    if fs:
+        rate = float(np.real(np.vdot(v, Dm @ v)))
+        ref_scale, scale = rate * st.GET_T_REF(s), rate * t
+    else:
+        ref_scale = scale = E["all"][0]
+    for k, Ek in E.items():
+        want_t = np.max(Ek[9:] / (scale[9:] if fs else scale))
+        want_nt = np.max(Ek / ref_scale)
+        assert abs(mg["temporal"][k] - want_t) <= 1e-12 * abs(want_t), (k, mg, want_t)
+        assert abs(mg["non_temporal"][k] - want_nt) <= 1e-12 * abs(want_nt), (k, mg, want_nt)
+    with pytest.raises(NotImplementedError, match="budgets"):
+        st.LINEAR_INTEGRATION_AND_GAINS(fs, v, 1.0, 1.0, A, s, {}, T_f, False, w)
