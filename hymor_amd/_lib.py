@@ -57,6 +57,8 @@ SIGNATURES = {
     "hymor_chu_sandwich": (C.c_int, [_p, _p, C.c_int, _i64, C.c_double, _i64, _p, _p, C.c_int]),
     "hymor_chu_sandwich_dev": (C.c_int, [_p, _p, C.c_int, _i64, C.c_double, _i64, _p, _p,
                                          C.c_int]),
+    "hymor_chu_sandwich_batched_dev": (C.c_int, [_p, _p, _p, _p, C.c_int, _i64, C.c_double, _i64,
+                                                 _i64, _p, _i64, _p, _i64, C.c_int]),
     "hymor_op_synchronize": (C.c_int, [_p]),
     "hymor_dev_alloc": (C.c_int, [_i64, _pp]),
     "hymor_dev_free": (C.c_int, [_p]),

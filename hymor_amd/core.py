@@ -434,6 +434,19 @@ def chu_sandwich_ptr(op: DeviceOperator, weight: DeviceOperator, d_x, d_y, *, or
                                              C128 if cplx else F64))
 
 
+def chu_sandwich_batched_ptr(op: DeviceOperator, weight: DeviceOperator, d_X, ldx: int,
+                             nrhs: int, d_Y, ldy: int, *, order=5, nsub=1, dt=1.0, n_set0=0,
+                             pre: DeviceOperator = None, post: DeviceOperator = None,
+                             complex_=None):
+    """hymor_chu_sandwich_batched_dev: Y = post Z T(A')^n C T(A)^n Z pre X on
+    nrhs device columns (pre / post None: identity)."""
+    cplx = op.is_complex if complex_ is None else complex_
+    check(_lib.load().hymor_chu_sandwich_batched_dev(
+        op.handle, weight.handle, pre.handle if pre is not None else None,
+        post.handle if post is not None else None, int(order), int(nsub), float(dt),
+        int(n_set0), int(nrhs), d_X, int(ldx), d_Y, int(ldy), C128 if cplx else F64))
+
+
 def expm_taylor_batched_ptr(op: DeviceOperator, d_X, ldx: int, nrhs: int, d_Y, ldy: int, *,
                             order=5, nsub=1, dt=1.0, form="powers", transpose=False,
                             complex_=None):

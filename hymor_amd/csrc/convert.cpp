@@ -741,4 +741,21 @@ bool build_shock_fuse(const HostExtra &x, int64_t nflow, HostExtra &rows_out,
     return true;
 }
 
+HostExtra extra_flow_rows(const HostExtra &x, int64_t nflow) {
+    HostExtra f;
+    f.is_complex = x.is_complex;
+    f.ptr.assign(1, 0);
+    const int vs = x.is_complex ? 2 : 1;
+    for (size_t i = 0; i < x.rows.size(); ++i) {
+        if (x.rows[i] >= nflow) continue;
+        for (int64_t e = x.ptr[i]; e < x.ptr[i + 1]; ++e) {
+            f.col.push_back(x.col[(size_t)e]);
+            for (int q = 0; q < vs; ++q) f.val.push_back(x.val[(size_t)e * vs + q]);
+        }
+        f.rows.push_back(x.rows[i]);
+        f.ptr.push_back((int64_t)f.col.size());
+    }
+    return f;
+}
+
 }  // namespace hymor

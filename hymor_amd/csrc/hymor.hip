@@ -114,7 +114,11 @@ struct DevShock {
     int64_t *cptr = nullptr;
     double *pval = nullptr;
     int64_t npcol = 0;
+    // batched terms: the flow rows' extra entries, for k_extra after the
+    // batch kernel has finished the extra-DOF rows in its launch
+    DevExtra flows;
 };
+
 
 }  // namespace
 
@@ -433,6 +437,7 @@ void upload_shock(hymor_op *op, const HostExtra &h, DevShock &d) {
         if (pan.slice_tab[k] >= 0) ps.push_back((int32_t)k);
     d.npan = (int64_t)ps.size();
     d.pslices = upload(op, ps);
+    upload_extra(op, extra_flow_rows(h, op->nflow), d.flows);
     d.on = true;
 }
 
@@ -608,6 +613,95 @@ void launch_shock_t(const DevSliced &s, const DevShock &sh, int64_t nflow, hipSt
                        F);
 }
 
+ShockBatchDev shock_batch_dev(const DevShock &sh, int64_t nflow) {
+    ShockBatchDev H;
+    H.nrows = sh.rows.nrows;
+    H.nsh_wg = (H.nrows + 7) / 8 * 8;   // multiples of 8: the slices keep their XCDs
+    H.nflow = nflow;
+    H.rows = sh.rows.rows;
+    H.ptr = sh.rows.ptr;
+    H.col = sh.rows.col;
+    H.val = sh.rows.val;
+    return H;
+}
+
+// Whether a batched term of nv doubles per row runs a shock operator's
+// extra-DOF rows in its batch kernel's launch: k_term_lds with one wave per
+// slice at 2 and 4, the default b = 4 batch kernel (k_spmm_quad4c) at 8..64.
+bool shock_batched_ok(const hymor_op *op, const DevSliced &s, int nv) {
+    if (nv == 2 || nv == 4) return s.tune.split == 3 && s.tune.waves == 1;
+    if (nv >= 8 && nv <= 64)
+        return op->b == 4 && (s.tune.spmm_mfma == 6 || s.tune.spmm_mfma == 8);
+    return false;
+}
+
+template <int B, typename VT, int NV, bool NT>
+void launch_shock_lds_t(const DevSliced &s, const ShockBatchDev &H, hipStream_t st,
+                        const double *x, double *tout, const double *vin, double *vout,
+                        double c1, double c2) {
+    hipLaunchKernelGGL((k_term_lds<B, VT, NV, 1, 9, NT, 0, true>),
+                       dim3((unsigned)(H.nsh_wg + s.nslice)), dim3(64), 0, st,
+                       s.slice_ptr, s.col, (const VT *)s.val, s.nslice, s.ncell, x, tout, vin,
+                       vout, c1, c2, FusedRepDev{}, H);
+}
+
+template <typename VT, int R, int W>
+void launch_shock_quad4c_t(const DevSliced &s, const ShockBatchDev &H, hipStream_t st,
+                           const double *x, double *tout, const double *vin, double *vout,
+                           double c1, double c2) {
+    const int64_t grid = ((s.ncell + 3) / 4 + 3) / 4;
+    hipLaunchKernelGGL((k_spmm_quad4c<VT, R, W, 0, true>),
+                       dim3((unsigned)(H.nsh_wg + grid)), dim3(256), 0, st,
+                       s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, tout, vin, vout, c1,
+                       c2, FusedRepDev{}, H);
+}
+
+template <int B, typename VT, bool NT>
+void launch_shock_batched_t(const DevSliced &s, const ShockBatchDev &H, int nv, hipStream_t st,
+                            const double *x, double *tout, const double *vin, double *vout,
+                            double c1, double c2) {
+    switch (nv) {
+    case 2: launch_shock_lds_t<B, VT, 2, NT>(s, H, st, x, tout, vin, vout, c1, c2); return;
+    case 4: launch_shock_lds_t<B, VT, 4, NT>(s, H, st, x, tout, vin, vout, c1, c2); return;
+    default: break;
+    }
+    if constexpr (B == 4) {
+        // the slot groups launch_block_part uses at each width
+        switch (nv) {
+        case 8: launch_shock_quad4c_t<VT, 8, 9>(s, H, st, x, tout, vin, vout, c1, c2); return;
+        case 16: launch_shock_quad4c_t<VT, 16, 9>(s, H, st, x, tout, vin, vout, c1, c2); return;
+        case 32: launch_shock_quad4c_t<VT, 32, 3>(s, H, st, x, tout, vin, vout, c1, c2); return;
+        case 64: launch_shock_quad4c_t<VT, 64, 3>(s, H, st, x, tout, vin, vout, c1, c2); return;
+        default: break;
+        }
+    }
+    throw ArgError("shock rows: unsupported vector width (internal)");
+}
+
+void launch_shock_batched(const DevSliced &s, const DevShock &sh, int64_t nflow, int nv,
+                          hipStream_t st, const double *x, double *tout, const double *vin,
+                          double *vout, double c1, double c2) {
+    const ShockBatchDev H = shock_batch_dev(sh, nflow);
+    const bool f32 = s.prec == HYMOR_PREC_F32, nt = s.tune.nt != 0;
+    auto go = [&](auto b, auto vt, auto ntc) {
+        constexpr int B = decltype(b)::value;
+        using VT = typename decltype(vt)::type;
+        constexpr bool NT = decltype(ntc)::value;
+        launch_shock_batched_t<B, VT, NT>(s, H, nv, st, x, tout, vin, vout, c1, c2);
+    };
+    auto by_nt = [&](auto b, auto vt) {
+        if (nt) go(b, vt, std::true_type{});
+        else go(b, vt, std::false_type{});
+    };
+    auto by_prec = [&](auto b) {
+        if (f32) by_nt(b, TypeTag<float>{});
+        else by_nt(b, TypeTag<double>{});
+    };
+    if (s.b == 4) by_prec(std::integral_constant<int, 4>{});
+    else if (s.b == 8) by_prec(std::integral_constant<int, 8>{});
+    else throw ArgError("vars_per_cell must be 4 or 8");
+}
+
 void launch_shock(const DevSliced &s, const DevShock &sh, int64_t nflow, hipStream_t st,
                   const double *x, double *tout, const double *vin, double *vout, double c1,
                   double c2) {
@@ -638,7 +732,7 @@ void launch_lds(const DevSliced &s, hipStream_t st, const double *x,
     constexpr int S = (9 + NW - 1) / NW;
     hipLaunchKernelGGL((k_term_lds<B, VT, NV, NW, S, NT>), dim3((unsigned)s.nslice),
                        dim3(64 * NW), 0, st, s.slice_ptr, s.col, (const VT *)s.val, s.nslice,
-                       s.ncell, x, tout, vin, vout, c1, c2, FusedRepDev{});
+                       s.ncell, x, tout, vin, vout, c1, c2, FusedRepDev{}, ShockBatchDev{});
 }
 
 template <int B, typename VT, int NV, bool NT>
@@ -795,7 +889,7 @@ void launch_quad4c_t(const DevSliced &s, hipStream_t st, const double *x,
     const int64_t grid = (ngrp + wpb - 1) / wpb;
     hipLaunchKernelGGL((k_spmm_quad4c<VT, R, W>), dim3((unsigned)grid), dim3(64 * wpb), 0, st,
                        s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, tout, vin, vout,
-                       c1, c2, FusedRepDev{});
+                       c1, c2, FusedRepDev{}, ShockBatchDev{});
 }
 
 template <int B, typename VT>
@@ -933,23 +1027,36 @@ void launch_extra_nv(const DevExtra &xe, int64_t nflow, hipStream_t st, const do
     if (grid == 0) return;
     // the replicated form is a variant of its own, so the other operators'
     // extra rows run without its virtual-column test and shared-row section
-    auto go = [&](auto cplx, auto repl) {
+    auto go = [&](auto cplx, auto repl, auto lpe) {
         constexpr bool C = decltype(cplx)::value, RP = decltype(repl)::value;
-        hipLaunchKernelGGL((k_extra<NV, C, NW, RP>), dim3((unsigned)grid), dim3(64 * NW), 0, st,
-                           xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, xe.nrows, nshort_wg, nflow,
-                           x, tout, vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b, xe.rep);
+        constexpr int L = decltype(lpe)::value;
+        hipLaunchKernelGGL((k_extra<NV, C, NW, RP, L>), dim3((unsigned)grid), dim3(64 * NW), 0,
+                           st, xe.rows, xe.ptr, xe.col, xe.val, xe.nlong, xe.nrows, nshort_wg,
+                           nflow, x, tout, vin, vout, c1, c2, cd.d, cd.r0, cd.r1, tile_b, xe.rep);
     };
     const bool rep = xe.rep.kind != 0;
     if (xe.is_complex) {
         if constexpr (NV >= 2) {
-            if (rep) go(std::true_type{}, std::true_type{});
-            else go(std::true_type{}, std::false_type{});
+            if (rep) go(std::true_type{}, std::true_type{}, std::integral_constant<int, 0>{});
+            else go(std::true_type{}, std::false_type{}, std::integral_constant<int, 0>{});
         } else {
             throw ArgError("complex extras need two doubles per row");
         }
+    } else if (rep) {
+        go(std::false_type{}, std::true_type{}, std::integral_constant<int, 0>{});
     } else {
-        if (rep) go(std::false_type{}, std::true_type{});
-        else go(std::false_type{}, std::false_type{});
+        // real long rows at 2..8 doubles per row: NV/2 lanes per entry,
+        // one 16-byte load of x per lane and entry.  On the 400x300-cell
+        // shock operator the Chu sandwich at 2, 4 and 8 vectors runs 3, 6
+        // and 12 % faster than with NV lanes per entry; one lane per entry
+        // is 11 % slower at 8 (16 bytes per cache-line access)
+        if constexpr (NV >= 2 && NV <= 8) {
+            if (!cd.d) {
+                go(std::false_type{}, std::false_type{}, std::integral_constant<int, NV / 2>{});
+                return;
+            }
+        }
+        go(std::false_type{}, std::false_type{}, std::integral_constant<int, 0>{});
     }
 }
 
@@ -1003,6 +1110,18 @@ void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, const double *x
         s.tune.waves != 9) {
         launch_shock(s, sh, op->nflow, st, x, tout, vin, vout, c1, c2);
         HIP_TRY(hipGetLastError());
+        return;
+    }
+    // batches: the extra-DOF rows in the batch kernel's launch, then the
+    // flow rows' extra entries
+    if (nv >= 2 && sh.on && !op->cdiag_on && op->ncell > 0 && shock_batched_ok(op, s, nv)) {
+        launch_shock_batched(s, sh, op->nflow, nv, st, x, tout, vin, vout, c1, c2);
+        HIP_TRY(hipGetLastError());
+        if (sh.flows.nrows > 0) {
+            launch_extra(sh.flows, nv, op->nflow, st, x, tout, vin, vout, c1, c2, ColDiag{},
+                         tile_b_of(op, nv));
+            HIP_TRY(hipGetLastError());
+        }
         return;
     }
     const DevExtra &xe = extra_of(op, adj, nv);
@@ -1096,7 +1215,7 @@ void launch_fused_lds_t(const DevSliced &s, const FusedRepDev &F, hipStream_t st
     hipLaunchKernelGGL((k_term_lds<B, VT, 2, 1, 9, true, FR>),
                        dim3((unsigned)(F.nfs_wg + s.nslice)), dim3(64), 0, st, s.slice_ptr,
                        s.col, (const VT *)s.val, s.nslice, s.ncell, x, tout, vin, vout, c1, c2,
-                       F);
+                       F, ShockBatchDev{});
 }
 
 template <typename VT, int R, int W, int FR>
@@ -1108,7 +1227,7 @@ void launch_fused_quad4c_t(const DevSliced &s, const FusedRepDev &F, hipStream_t
     const int64_t grid = (ngrp + wpb - 1) / wpb;
     hipLaunchKernelGGL((k_spmm_quad4c<VT, R, W, FR>), dim3((unsigned)(F.nfs_wg + grid)),
                        dim3(64 * wpb), 0, st, s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x,
-                       tout, vin, vout, c1, c2, F);
+                       tout, vin, vout, c1, c2, F, ShockBatchDev{});
 }
 
 // the slot groups launch_block_part uses at each width
@@ -2039,6 +2158,62 @@ int hymor_chu_sandwich_dev(hymor_op *A, hymor_op *C, int order, int64_t nsub, do
         ensure_ws(A, nv);
         ensure_ws(C, nv);
         sandwich_core(A, C, order, nsub, dt, n_set0, (const double *)d_x, (double *)d_y, nv);
+    });
+}
+
+int hymor_chu_sandwich_batched_dev(hymor_op *A, hymor_op *C, hymor_op *P_in, hymor_op *P_out,
+                                   int order, int64_t nsub, double dt, int64_t n_set0,
+                                   int64_t nrhs, const void *d_X, int64_t ldx, void *d_Y,
+                                   int64_t ldy, int dtype) {
+    return guarded([&] {
+        check_sandwich(A, C, order, nsub, n_set0);
+        ARG_CHECK(d_X && d_Y, "null matrix");
+        check_batch(A, nrhs, ldx, ldy);
+        const int ncomp = nv_of(A, dtype);
+        ARG_CHECK(!(C->is_complex && ncomp == 1), "complex weight needs complex vectors");
+        const int w = pad_width(nrhs * ncomp);
+        hymor_op *ops[3] = {C, P_in, P_out};
+        for (hymor_op *o : ops) {
+            if (!o) continue;
+            ARG_CHECK(o->n == A->n && o->b == A->b && o->ncell == A->ncell,
+                      "weight or factor operator shape differs from A");
+            ARG_CHECK(!(o->is_complex && ncomp == 1), "complex operator needs complex vectors");
+            ARG_CHECK(tile_b_of(o, w) == tile_b_of(A, w),
+                      "weight or factor operator uses another batch row layout than A");
+        }
+        ensure_ws(A, w);
+        for (hymor_op *o : ops)
+            if (o) ensure_ws(o, w);
+        hipStream_t st = A->stream;
+        const int tile_b = tile_b_of(A, w);
+        auto zero_prefix = [&](double *buf) {
+            if (n_set0 == 0) return;
+            const int64_t total = A->n * w;
+            const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 65536));
+            hipLaunchKernelGGL(k_zero_vm_prefix, dim3((unsigned)grid), dim3(256), 0, st, buf, A->n,
+                               A->ncell, A->b, n_set0, w, tile_b);
+            HIP_TRY(hipGetLastError());
+        };
+        // x -> P_in x, zeroed (ws[1] -> ws[0]; the zeroing rides on the
+        // permutation when there is no P_in)
+        launch_perm_in(A, st, (const double *)d_X, ldx, A->ws[1], (int)nrhs, ncomp,
+                       P_in ? 0 : n_set0, w);
+        const double *in = A->ws[1];
+        if (P_in) {
+            launch_term(P_in, false, w, st, A->ws[1], A->ws[0], nullptr, nullptr, 1.0, 0.0);
+            zero_prefix(A->ws[0]);
+            in = A->ws[0];
+        }
+        const double *r1 = run_taylor(A, false, order, nsub, dt, HYMOR_FORM_HORNER, w, in);
+        launch_term(C, false, w, st, r1, A->ws[6], nullptr, nullptr, 1.0, 0.0);
+        const double *r2 = run_taylor(A, true, order, nsub, dt, HYMOR_FORM_HORNER, w, A->ws[6]);
+        if (P_out) {
+            // r2 is one of A's own buffers (ws[2..6]): zeroed in place
+            zero_prefix(const_cast<double *>(r2));
+            launch_term(P_out, false, w, st, r2, A->ws[1], nullptr, nullptr, 1.0, 0.0);
+            r2 = A->ws[1];
+        }
+        launch_perm_out(A, st, r2, (double *)d_Y, ldy, (int)nrhs, ncomp, P_out ? 0 : n_set0, w);
     });
 }
 

@@ -147,6 +147,11 @@ struct HostShockPanels {
 bool build_shock_fuse(const HostExtra &x, int64_t nflow, HostExtra &rows_out,
                       HostShockPanels &pan);
 
+// The flow rows of an extra CSR (rows < nflow, in order): what a batched
+// term over the shock bands leaves to k_extra once its launch has finished
+// the extra-DOF rows.
+HostExtra extra_flow_rows(const HostExtra &x, int64_t nflow);
+
 void csc_to_blocks(int64_t n, const int64_t *colptr, const int64_t *rowval,
                    const double *nzval, int index_base, bool is_complex, double scale,
                    int b, int64_t ncell, HostBSR &fwd, HostBSR &adj,

@@ -637,13 +637,121 @@ __device__ __forceinline__ void extra_short_rows(
     }
 }
 
+// A long real row at NV = 2..8 doubles per row with LPR lanes per entry,
+// each lane loading a contiguous chunk of CH = NV / LPR doubles of the
+// entry's x row (16-byte loads from CH = 2 up).  The LPR lanes of an entry
+// cover its x row in one cache line, and G entries per lane are in flight
+// (G * CH about 16 doubles): with extra_long_row's NV lanes per entry and
+// four entries in flight, a 3 600-entry shock row at NV = 8 takes 28
+// rounds of dependent loads per 4-wave workgroup.  A lane's entries lie
+// 64*NW/LPR apart.  The chunk sums are reduced across the wave by
+// butterfly shuffles over the lanes of one chunk position and across waves
+// through LDS in wave order.  No per-column diagonal.
+template <int NV, int NW, bool TILE, int LPR>
+__device__ __forceinline__ void extra_long_row_lpe(
+    int64_t ri, const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
+    const int32_t *__restrict__ col, const double *__restrict__ val, int64_t nflow,
+    const double *__restrict__ x, double *__restrict__ tout, const double *vin, double *vout,
+    double c1, double c2, int tile_b) {
+    static_assert(NV >= 2 && NV <= 8 && NV % LPR == 0, "LPR divides NV <= 8");
+    constexpr int CH = NV / LPR;
+    constexpr int G = CH == 1 ? 16 : (CH == 2 ? 8 : (CH == 4 ? 4 : 2));
+    constexpr int EPW = 64 / LPR;
+    constexpr int64_t S = (int64_t)NW * EPW;
+    __shared__ double part[NW > 1 ? NW - 1 : 1][NV];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int j = lane / LPR, v0 = (lane % LPR) * CH;
+    const int64_t row = rows[ri];
+    // lanes [0, LPR) of wave 0 write doubles [v0, v0 + CH): old values first
+    double old_t[CH], old_v[CH];
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+        old_t[i] = old_v[i] = 0.0;
+        if (wv == 0 && j == 0) {
+            if (row < nflow) {
+                const int64_t o = buf_pos(row, v0 + i, NV, tile_b, nflow);
+                if (tout) old_t[i] = tout[o];
+                if (vout) old_v[i] = vout[o];
+            } else if (vout) {
+                old_v[i] = vin[row * NV + v0 + i];
+            }
+        }
+    }
+    double acc[CH];
+#pragma unroll
+    for (int i = 0; i < CH; ++i) acc[i] = 0.0;
+    const int64_t e1 = ptr[ri + 1];
+    for (int64_t e = ptr[ri] + wv * EPW + j; e < e1; e += G * S) {
+        int32_t cg[G];
+        double a[G];
+        // past the row's end the loads repeat entry e (not added)
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int64_t eg = e + g * S < e1 ? e + g * S : e;
+            cg[g] = col[eg];
+            a[g] = val[eg];
+        }
+        double xv[G][CH];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if constexpr (TILE || CH == 1) {
+#pragma unroll
+                for (int i = 0; i < CH; ++i)
+                    xv[g][i] = x[TILE ? buf_pos(cg[g], v0 + i, NV, tile_b, nflow)
+                                      : (int64_t)cg[g] * NV + v0 + i];
+            } else {
+                load_f64<CH>(x + (int64_t)cg[g] * NV + v0, xv[g]);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (e + g * S < e1) {
+#pragma unroll
+                for (int i = 0; i < CH; ++i) acc[i] = fma(a[g], xv[g][i], acc[i]);
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < CH; ++i)
+#pragma unroll
+        for (int off = 32; off >= LPR; off >>= 1) acc[i] += __shfl_xor(acc[i], off, 64);
+    if constexpr (NW > 1) {
+        if (wv > 0 && j == 0) {
+#pragma unroll
+            for (int i = 0; i < CH; ++i) part[wv - 1][v0 + i] = acc[i];
+        }
+        __syncthreads();
+        if (wv > 0) return;
+#pragma unroll
+        for (int k = 0; k < NW - 1; ++k)
+#pragma unroll
+            for (int i = 0; i < CH; ++i) acc[i] += part[k][v0 + i];
+    }
+    if (j != 0) return;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+        const double t = c1 * acc[i];
+        if (row < nflow) {
+            // flow row: the block kernel has written its part already
+            const int64_t o = buf_pos(row, v0 + i, NV, tile_b, nflow);
+            if (tout) tout[o] = old_t[i] + t;
+            if (vout) vout[o] = fma(c2, t, old_v[i]);
+        } else {
+            if (tout) tout[row * NV + v0 + i] = t;
+            if (vout) vout[row * NV + v0 + i] = fma(c2, t, old_v[i]);
+        }
+    }
+}
+
 // The extra CSR rows in one launch of NW-wave workgroups: workgroups
 // [0, nlong) take the long rows [0, nlong) (more than 32 entries), one each,
 // in contiguous ranges per XCD (neighbouring rows gather the same x rows --
 // the receptivity B rows of adjacent cells -- which then stay in one L2);
 // the next nshort_wg take the short rows [nlong, nrows), RPW per wave; the
 // rest (kind 2 replicas) the shared rows, VPW per wave.
-template <int NV, bool CPLX, int NW, bool REP>
+// LPE > 0: real rows at 2..8 doubles per row, no per-column diagonal: the
+// long rows with LPE lanes per entry (extra_long_row_lpe).
+template <int NV, bool CPLX, int NW, bool REP, int LPE = 0>
 __global__ __launch_bounds__(64 * NW) void k_extra(
     const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
     const int32_t *__restrict__ col, const double *__restrict__ val, int64_t nlong,
@@ -655,11 +763,15 @@ __global__ __launch_bounds__(64 * NW) void k_extra(
     // the row-major case carry no layout test
     auto body = [&](auto tile) {
         constexpr bool TILE = decltype(tile)::value;
-        if (bid < nlong)
-            extra_long_row<NV, CPLX, NW, REP, TILE>(xcd_tile(bid, nlong), rows, ptr, col, val,
-                                                    nflow, x, tout, vin, vout, c1, c2, cdiag,
-                                                    cd0, cd1, tile_b, R);
-        else if (bid < nlong + nshort_wg)
+        if (bid < nlong) {
+            if constexpr (LPE > 0 && !CPLX && !REP && NV >= 2 && NV <= 8)
+                extra_long_row_lpe<NV, NW, TILE, (LPE > 0 ? LPE : 1)>(xcd_tile(bid, nlong), rows, ptr, col, val, nflow,
+                                                 x, tout, vin, vout, c1, c2, tile_b);
+            else
+                extra_long_row<NV, CPLX, NW, REP, TILE>(xcd_tile(bid, nlong), rows, ptr, col,
+                                                        val, nflow, x, tout, vin, vout, c1, c2,
+                                                        cdiag, cd0, cd1, tile_b, R);
+        } else if (bid < nlong + nshort_wg)
             extra_short_rows<NV, CPLX, REP, TILE>((bid - nlong) * NW + (threadIdx.x >> 6), rows,
                                                   ptr, col, val, nlong, nrows, nflow, x, tout,
                                                   vin, vout, c1, c2, cdiag, cd0, cd1, tile_b, R);
@@ -714,6 +826,23 @@ __global__ void k_perm_out(const double *__restrict__ in, double *__restrict__ y
         const int64_t row = vm < nflow ? (vm % ncell) * b + vm / ncell : vm;
         y_vm[(rhs * ld + vm) * ncomp + comp] =
             vm < nzero ? 0.0 : in[buf_pos(row, (int)(rhs * ncomp + comp), width, tile_b, nflow)];
+    }
+}
+
+// Zero, in an internal (cell-major) buffer of `width` doubles per row, the
+// rows whose var-major index is below nzero: CONSTRUCT_C's zeroing applied
+// between two internal operators.  Only flow rows can map below nzero when
+// nzero <= nflow; the loop covers every row so any nzero <= n works.
+__global__ void k_zero_vm_prefix(double *__restrict__ buf, int64_t n, int64_t ncell, int b,
+                                 int64_t nzero, int width, int tile_b) {
+    const int64_t total = n * width;
+    const int64_t nflow = ncell * b;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = i / width;
+        const int j = (int)(i - row * width);
+        const int64_t vm = row < nflow ? (row % b) * ncell + row / b : row;
+        if (vm < nzero) buf[buf_pos(row, j, width, tile_b, nflow)] = 0.0;
     }
 }
 
@@ -1199,6 +1328,18 @@ struct ShockFuseDev {
     const double *pval = nullptr;
 };
 
+// The extra-DOF rows of a shock operator inside a batched term's launch
+// (k_term_lds at 2..4 and k_spmm_quad4c at 8..64 doubles per row):
+// workgroups [0, nsh_wg) finish them -- they read only x, so they overlap
+// the block term -- and the flow rows' shock entries follow in a k_extra
+// launch of their own.
+struct ShockBatchDev {
+    int64_t nsh_wg = 0, nrows = 0, nflow = 0;
+    const int64_t *rows = nullptr, *ptr = nullptr;
+    const int32_t *col = nullptr;
+    const double *val = nullptr;
+};
+
 // k_term_split with the loads of each wave grouped: the block columns of
 // the wave's next S slots and their values are issued together, then the x
 // entries they select, and wave 0 issues its epilogue operand (vin)
@@ -1390,13 +1531,16 @@ __device__ __forceinline__ void wave_lds_sync() {
 // FR != 0: the receptivity operator's replicated rows fused in (FusedRepDev,
 // FusedKind: forward Horner terms of a kind 1 operator, or any term of a
 // kind 2 adjoint): no k_extra launch follows.
-template <int B, typename VT, int NV, int NW, int S, bool NT, int FR = 0>
+//
+// SH: a shock operator's extra-DOF rows in leading workgroups
+// (ShockBatchDev; one wave per slice).
+template <int B, typename VT, int NV, int NW, int S, bool NT, int FR = 0, bool SH = false>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_term_lds(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ val, int64_t nslice, int64_t ncell,
     const double *__restrict__ x,
     double *__restrict__ tout, const double *vin, double *vout, double c1, double c2,
-    FusedRepDev F) {
+    FusedRepDev F, ShockBatchDev H) {
     constexpr int CPS = 64 / B;
     constexpr int BB = B * B;
     constexpr int V = Vpl<VT>::value;
@@ -1408,11 +1552,23 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
     typedef typename Chunk<VT>::type CT;
     using FK = FusedKind<FR>;
     static_assert(!(FK::fwd && FK::cplx) || NV == 2, "complex entries act on one complex vector");
+    static_assert(!SH || (NW == 1 && FR == 0), "shock rows: one wave per slice, no replicas");
     __shared__ double part[NW > 1 ? NW - 1 : 1][NV][64];
     __shared__ d2 xs[NW][S][CPS * XS];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     int64_t bid = blockIdx.x, nwg = gridDim.x;
+    if constexpr (SH) {
+        // the extra-DOF rows first: they read only x
+        if (bid < H.nsh_wg) {
+            if (bid < H.nrows)
+                extra_long_row_lpe<NV, 1, false, NV / 2>(bid, H.rows, H.ptr, H.col, H.val,
+                                                         H.nflow, x, tout, vin, vout, c1, c2, 0);
+            return;
+        }
+        bid -= H.nsh_wg;
+        nwg -= H.nsh_wg;
+    }
     if constexpr (FR != 0) {
         // the replicated rows first (nfs_wg is a multiple of 8, so the
         // slices below keep the XCD mapping of an unfused launch)
@@ -2632,12 +2788,15 @@ __global__ __launch_bounds__(256) void k_spmm_quad4(
 // row are R / 2 complex values): workgroups [0, nfs_wg) take the replicated
 // rows, the others (one 16-cell slice per 4-wave workgroup) the block term,
 // forward adding their rows' entries in the epilogue.
-template <typename VT, int R, int W, int FR = 0>
+//
+// SH: a shock operator's extra-DOF rows in leading workgroups
+// (ShockBatchDev).
+template <typename VT, int R, int W, int FR = 0, bool SH = false>
 __global__ __launch_bounds__(256) void k_spmm_quad4c(
     const int64_t *__restrict__ slice_ptr, const int32_t *__restrict__ col,
     const VT *__restrict__ val, int64_t ncell, const double *__restrict__ x,
     double *__restrict__ tout, const double *vin, double *vout,
-    double c1, double c2, FusedRepDev F) {
+    double c1, double c2, FusedRepDev F, ShockBatchDev SB) {
     constexpr int B = 4, CPS = 16, BB = 16;
     constexpr int V = Vpl<VT>::value;
     constexpr int VPL = R >= 16 ? 4 : 2;          // vectors of a lane's x chunk
@@ -2649,7 +2808,27 @@ __global__ __launch_bounds__(256) void k_spmm_quad4c(
     const int lane = threadIdx.x & 63;
     const int64_t wpb = blockDim.x >> 6;
     using FK = FusedKind<FR>;
+    static_assert(!SH || FR == 0, "shock bands and replicas do not mix");
     int64_t bid = blockIdx.x, nwg = gridDim.x;
+    if constexpr (SH) {
+        // the extra-DOF rows first: they read only x
+        if (bid < SB.nsh_wg) {
+            if (bid < SB.nrows) {
+                if constexpr (R <= 8)
+                    extra_long_row_lpe<R, 4, false, R / 2>(bid, SB.rows, SB.ptr, SB.col, SB.val,
+                                                           SB.nflow, x, tout, vin, vout, c1, c2,
+                                                           0);
+                else
+                    extra_long_row<R, false, 4, false, false>(bid, SB.rows, SB.ptr, SB.col, SB.val,
+                                                              SB.nflow, x, tout, vin, vout, c1,
+                                                              c2, nullptr, 0, 0, 0,
+                                                              ReplicaDev{});
+            }
+            return;
+        }
+        bid -= SB.nsh_wg;
+        nwg -= SB.nsh_wg;
+    }
     if constexpr (FK::on) {
         // the replicated rows first (nfs_wg a multiple of 8: the slices keep
         // the XCD mapping of an unfused launch)

@@ -23,8 +23,9 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from . import _lib
-from .core import (DeviceOperator, DeviceVector, DeviceWeights, chu_sandwich_ptr, energy_track,
-                   expm_taylor_batched_ptr, expm_taylor_ptr, spmv_dev)
+from .core import (DeviceOperator, DeviceVector, DeviceWeights, chu_sandwich_batched_ptr,
+                   chu_sandwich_ptr, energy_track, expm_taylor_batched_ptr, expm_taylor_ptr,
+                   spmv_dev)
 from .krylov import block_krylov_schur, krylov_schur
 
 ORDER = 5            # Taylor order hard-coded by the reference
@@ -410,12 +411,16 @@ def EIGENVALUES_GPU_ARNOLDI(A, s, n_modes, *, T_ref=0.5, tol=1e-4, maxiter=None,
 
 
 def TRANSIENT_GROWTH_DOWNSTREAM(A, s, n_modes, T, *, tol=1e-4, maxiter=None, ncv=None, v0=None,
-                                krylov="arpack"):
+                                krylov="arpack", block=2):
     """Optimal transient growth (TRANSIENT_GROWTH_DOWNSTREAM.jl:34-153).
 
     Returns (V, D, T_opt): D are the gains, eigenvalues of
     exp(A'T) R'MR exp(AT) x = lambda R'M_hat R x (ARPACK, symmetric
     generalized mode, which=LM, tol=1e-4, maxiter=10k).
+
+    krylov="device": Krylov-Schur on the Cholesky-reduced operator with a
+    device basis; block > 1 expands it `block` vectors at a time through the
+    batched sandwich (block=1: one sandwich per matvec).
     """
     k = n_modes
     s = CFL_TIMESTEP(s)
@@ -429,11 +434,15 @@ def TRANSIENT_GROWTH_DOWNSTREAM(A, s, n_modes, T, *, tol=1e-4, maxiter=None, ncv
     n = A.shape[0]
     devA = devC = None
     try:
+        t0 = time.perf_counter()
         devA = upload(A, s, scale=dt)
         devC = upload(C_no_exp, s)
+        s.setdefault("timings", {})["transient_growth_upload"] = time.perf_counter() - t0
         if krylov == "device":
             vals, vecs = _device_generalized(devA, devC, D_mat, s, n, k, n_t, 0, False, tol,
-                                             maxiter or 10 * k, ncv, v0)
+                                             maxiter or 10 * k, ncv, v0, block,
+                                             s.setdefault("timings", {}).setdefault(
+                                                 "transient_growth_krylov", {}))
             vals = np.real(vals)
         else:
             op = spla.LinearOperator((n, n), dtype=np.float64,
@@ -467,7 +476,7 @@ def _upload_extended_from_parts(A_, s, w_infty, dt):
 
 
 def FREESTREAM_RECEPTIVITY(A_, s, n_modes, T, w_infty, *, tol=1e-4, maxiter=None, ncv=None,
-                           v0=None, krylov="arpack", factored=False):
+                           v0=None, krylov="arpack", factored=False, block=2):
     """Freestream receptivity (FREESTREAM_RECEPTIVITY.jl:37-162).
 
     A_ is the complex extended operator [L B..B; 0 kron(diag(-i w), I)].
@@ -477,6 +486,7 @@ def FREESTREAM_RECEPTIVITY(A_, s, n_modes, T, w_infty, *, tol=1e-4, maxiter=None
     factored=True uploads A_ from its parts (one per-wavenumber system
     [L B; 0 0] and the wavenumbers, hymor_op_create_receptivity): B is never
     repeated on the device, and each action costs one flow SpMV per term.
+    krylov="device" and block as in TRANSIENT_GROWTH_DOWNSTREAM.
     """
     k = n_modes
     s = CFL_TIMESTEP(s)
@@ -499,7 +509,9 @@ def FREESTREAM_RECEPTIVITY(A_, s, n_modes, T, w_infty, *, tol=1e-4, maxiter=None
         devC = upload(C_, s)
         if krylov == "device":
             vals, vecs = _device_generalized(devA, devC, D_, s, n, k, n_t, n_set_0, True, tol,
-                                             maxiter or 10 * k, ncv, v0)
+                                             maxiter or 10 * k, ncv, v0, block,
+                                             s.setdefault("timings", {}).setdefault(
+                                                 "receptivity_krylov", {}))
         else:
             op = spla.LinearOperator((n, n), dtype=np.complex128,
                                      matvec=lambda x: CONSTRUCT_C(x, devA, None, devC, n_t, True,
@@ -678,29 +690,51 @@ def block_cholesky_inverse(D):
     return Linv, Linv.conj().T.tocsc()
 
 
-def _device_generalized(devA, devC, D, s, n, k, n_t, n_set0, cplx, tol, maxiter, ncv, v0):
+def _device_generalized(devA, devC, D, s, n, k, n_t, n_set0, cplx, tol, maxiter, ncv, v0,
+                        block=1, info=None):
     """Largest eigenpairs of T(A')^n C T(A)^n x = lambda D x (with the
     freestream zeroing) through the exact block-Cholesky reduction
     L^-1 (.) L^-H and Krylov-Schur on a device basis.  Returns eigenvectors
-    x = L^-H y, D-orthonormal as ARPACK's generalized mode returns them."""
+    x = L^-H y, D-orthonormal as ARPACK's generalized mode returns them.
+
+    block > 1: block Krylov-Schur whose every expansion applies the reduced
+    operator to `block` basis vectors in one batched sandwich
+    (hymor_chu_sandwich_batched_dev, the factors applied inside); v0, if
+    given, seeds the first column of the start block."""
+    t0 = time.perf_counter()
     Linv, LinvH = block_cholesky_inverse(D)
     dLi = upload(Linv, s)
     dLiH = upload(LinvH, s)
+    t_reduce = time.perf_counter() - t0
     dtype = np.complex128 if cplx else np.float64
     t1, t2 = DeviceVector(n, dtype), DeviceVector(n, dtype)
     try:
-        def apply(din, dout):
-            spmv_dev(dLiH, din, t1.ptr, complex_=cplx)
-            dLiH.synchronize()
-            chu_sandwich_ptr(devA, devC, t1.ptr, t2.ptr, order=ORDER, nsub=n_t, n_set0=n_set0,
-                             complex_=cplx)
-            devA.synchronize()
-            spmv_dev(dLi, t2.ptr, dout, complex_=cplx)
-            dLi.synchronize()
         if v0 is not None:
             v0 = np.asarray(v0, dtype=dtype)
-        vals, Y, _ = krylov_schur(apply, n, k, ncv or max(20, 2 * k + 1), complex_=cplx, tol=tol,
-                                  maxiter=maxiter, v0=v0)
+        if block > 1:
+            def apply_block(d_in, ld, nvec, d_out):
+                chu_sandwich_batched_ptr(devA, devC, d_in, ld, nvec, d_out, n, order=ORDER,
+                                         nsub=n_t, n_set0=n_set0, pre=dLiH, post=dLi,
+                                         complex_=cplx)
+                devA.synchronize()
+            m = ncv or max(2 * k + 4 * block, 8 * block)
+            vals, Y, kinfo = block_krylov_schur(apply_block, n, k, m, block, complex_=cplx,
+                                                tol=tol, maxiter=maxiter,
+                                                V0=None if v0 is None else v0[:, None])
+        else:
+            def apply(din, dout):
+                spmv_dev(dLiH, din, t1.ptr, complex_=cplx)
+                dLiH.synchronize()
+                chu_sandwich_ptr(devA, devC, t1.ptr, t2.ptr, order=ORDER, nsub=n_t,
+                                 n_set0=n_set0, complex_=cplx)
+                devA.synchronize()
+                spmv_dev(dLi, t2.ptr, dout, complex_=cplx)
+                dLi.synchronize()
+            vals, Y, kinfo = krylov_schur(apply, n, k, ncv or max(20, 2 * k + 1), complex_=cplx,
+                                          tol=tol, maxiter=maxiter, v0=v0)
+        if info is not None:
+            info.update(kinfo, block=block, reduce_seconds=t_reduce,
+                        solve_seconds=time.perf_counter() - t0 - t_reduce)
         X = LinvH @ Y
         if not cplx:
             X = np.real(X)

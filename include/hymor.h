@@ -231,6 +231,22 @@ int hymor_chu_sandwich(hymor_op *A, hymor_op *C, int order, int64_t nsub, double
 int hymor_chu_sandwich_dev(hymor_op *A, hymor_op *C, int order, int64_t nsub, double dt,
                            int64_t n_set0, const void *d_x, void *d_y, int dtype);
 
+/* The sandwich on nrhs device columns at once (one batched sweep each way),
+ * optionally between two operators of A's grid:
+ *   Y = P_out * Z * T_p(A'dt)^n * C * T_p(A dt)^n * Z * P_in * X,
+ * Z the zeroing of var-major rows [0, n_set0); P_in / P_out NULL for the
+ * identity.  With P_in = L^-H and P_out = L^-1 for the Cholesky factor L of
+ * the right-hand matrix (R'M_hat R, TRANSIENT_GROWTH_DOWNSTREAM.jl:52), the
+ * generalized problem the reference hands to eigsh (:63-110, and
+ * FREESTREAM_RECEPTIVITY.jl:88-112) becomes a standard symmetric one whose
+ * blocks a block Krylov method can expand.  nrhs * (2 if complex) <= 64;
+ * C and the factors must use A's batch row layout (the default).
+ * Asynchronous on A's stream. */
+int hymor_chu_sandwich_batched_dev(hymor_op *A, hymor_op *C, hymor_op *P_in, hymor_op *P_out,
+                                   int order, int64_t nsub, double dt, int64_t n_set0,
+                                   int64_t nrhs, const void *d_X, int64_t ldx, void *d_Y,
+                                   int64_t ldy, int dtype);
+
 /* Wait for all work on the handle's stream; reports asynchronous errors. */
 int hymor_op_synchronize(hymor_op *op);
 

@@ -6,6 +6,7 @@
 // index math on the CPU, checking every access against the buffer it reads,
 // so that a layout or indexing mistake shows up here rather than as a GPU
 // memory fault.
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -1367,6 +1368,96 @@ extern "C" int emu_spmv_shock(int64_t n, const int64_t *colptr, const int64_t *r
     for (int64_t vm = 0; vm < n; ++vm) {
         const int64_t row = vm < nflow ? (vm % ncell) * b + vm / ncell : vm;
         y_vm[vm] = y[(size_t)row];
+    }
+    return g_oob;
+}
+
+namespace {
+template <typename T>
+T &atw(std::vector<T> &v, int64_t i) {
+    if (i < 0 || i >= (int64_t)v.size()) {
+        ++g_oob;
+        static T zero{};
+        return zero;
+    }
+    return v[(size_t)i];
+}
+}  // namespace
+
+// A batched term over a shock operator at R doubles per row: the extra-DOF
+// rows (build_shock_fuse's CSR) finished in the batch kernel's leading
+// workgroups, the block part, then the flow rows' extra entries
+// (extra_flow_rows) added by k_extra -- every access bounds-checked, every
+// row written by exactly one of the three.  x_vm, y_vm: R columns of n
+// (var-major).  info: [fused?, extra-DOF rows, flow rows].  Returns the
+// out-of-bounds count.
+extern "C" int emu_spmv_shock_batched(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                                      const double *nzval, int base, int b, int64_t ncell,
+                                      int transpose, int R, const double *x_vm, double *y_vm,
+                                      int64_t *info) {
+    g_oob = 0;
+    HostBSR fwd, adj;
+    HostExtra xf, xa;
+    csc_to_blocks(n, colptr, rowval, nzval, base, false, 1.0, b, ncell, fwd, adj, xf, xa);
+    const HostBSR &A = transpose ? adj : fwd;
+    const HostExtra &X = transpose ? xa : xf;
+    const int64_t nflow = (int64_t)b * ncell;
+    HostExtra rows;
+    HostShockPanels pan;
+    const bool ok = build_shock_fuse(X, nflow, rows, pan);
+    for (int i = 0; i < 3; ++i) info[i] = 0;
+    info[0] = ok ? 1 : 0;
+    if (!ok) return 0;
+    const HostExtra flows = extra_flow_rows(X, nflow);
+    info[1] = (int64_t)rows.rows.size();
+    info[2] = (int64_t)flows.rows.size();
+    std::vector<double> x((size_t)(n * R)), y((size_t)(n * R), 0.0);
+    std::vector<int> writes((size_t)n, 0);
+    for (int64_t row = 0; row < n; ++row) {
+        const int64_t vm = row < nflow ? (row % b) * ncell + row / b : row;
+        for (int v = 0; v < R; ++v) x[(size_t)(row * R + v)] = x_vm[(int64_t)v * n + vm];
+    }
+    // leading workgroups: the extra-DOF rows
+    for (int64_t i = 0; i < (int64_t)rows.rows.size(); ++i) {
+        const int64_t row = at(rows.rows, i);
+        if (row < nflow || row >= n) ++g_oob;
+        atw(writes, row) += 1;
+        for (int v = 0; v < R; ++v) {
+            double acc = 0.0;
+            for (int64_t e = at(rows.ptr, i); e < at(rows.ptr, i + 1); ++e)
+                acc += at(rows.val, e) * at(x, (int64_t)at(rows.col, e) * R + v);
+            atw(y, row * R + v) = acc;
+        }
+    }
+    // the block part of every flow row
+    for (int64_t row = 0; row < nflow; ++row) {
+        const int64_t cell = row / b;
+        const int r = (int)(row % b);
+        atw(writes, row) += 1;
+        for (int64_t k = at(A.ptr, cell); k < at(A.ptr, cell + 1); ++k) {
+            const int64_t c = at(A.col, k);
+            for (int j = 0; j < b; ++j)
+                for (int v = 0; v < R; ++v)
+                    atw(y, row * R + v) +=
+                        at(A.val, k * b * b + r * b + j) * at(x, (c * b + j) * R + v);
+        }
+    }
+    // k_extra over the flow rows: adds to what the block kernel wrote
+    for (int64_t i = 0; i < (int64_t)flows.rows.size(); ++i) {
+        const int64_t row = at(flows.rows, i);
+        if (row >= nflow) ++g_oob;
+        for (int v = 0; v < R; ++v) {
+            double acc = 0.0;
+            for (int64_t e = at(flows.ptr, i); e < at(flows.ptr, i + 1); ++e)
+                acc += at(flows.val, e) * at(x, (int64_t)at(flows.col, e) * R + v);
+            atw(y, row * R + v) += acc;
+        }
+    }
+    for (int64_t row = 0; row < n; ++row)
+        if (writes[(size_t)row] != 1) ++g_oob;
+    for (int64_t vm = 0; vm < n; ++vm) {
+        const int64_t row = vm < nflow ? (vm % ncell) * b + vm / ncell : vm;
+        for (int v = 0; v < R; ++v) y_vm[(int64_t)v * n + vm] = y[(size_t)(row * R + v)];
     }
     return g_oob;
 }

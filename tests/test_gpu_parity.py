@@ -15,7 +15,8 @@ import numpy as np
 import pytest
 
 from hymor_amd import operators as ops
-from hymor_amd.core import DeviceOperator, DeviceVector, expm_taylor_dev
+from hymor_amd.core import (DeviceMatrix, DeviceOperator, DeviceVector, chu_sandwich_batched_ptr,
+                            expm_taylor_dev)
 
 pytestmark = pytest.mark.gpu
 
@@ -137,6 +138,55 @@ def test_chu_sandwich_parity(case, orc):
         got0 = dev.chu_sandwich(dW, x, order=5, nsub=nt, dt=dt, n_set0=n0)
         assert not np.any(got0[:n0])
         assert relerr(got0, ref0) <= 1e-12
+
+
+def _batched_sandwich_ref(orc, A, dt, nt, Cw, X, n0, Pin=None, Pout=None):
+    At = orc.csc_adjoint(A)
+    out = []
+    for j in range(X.shape[1]):
+        x = X[:, j] if Pin is None else orc.csc_mul(Pin, X[:, j])
+        y = orc.construct_c(x, A.scaled(dt), At.scaled(dt), Cw, nt, n0 > 0, n0)
+        out.append(y if Pout is None else orc.csc_mul(Pout, y))
+    return np.stack(out, axis=1)
+
+
+@pytest.mark.parametrize("nrhs", [3, 8])
+def test_chu_sandwich_batched_parity(case, orc, nrhs):
+    """hymor_chu_sandwich_batched_dev against the oracle's CONSTRUCT_C per
+    column: with and without the two factor operators, the zeroing on a
+    var-major prefix that splits cells (so it lands inside cell-major rows),
+    3 columns (padded width 4) and 8 (the tile row layout at b = 8)."""
+    name, op, A, dev = case
+    say(f"[{name}] batched sandwich nrhs={nrhs}")
+    W = _chu_weight(op, 5)
+    Cw = orc.CSC.from_arrays(*W.to_csc_varmajor())
+    P1 = orc.CSC.from_arrays(*_chu_weight(op, 13).to_csc_varmajor())
+    P2 = orc.CSC.from_arrays(*_chu_weight(op, 14).to_csc_varmajor())
+    dt, nt = 0.1, 3
+    X = np.asfortranarray(np.random.default_rng(15).standard_normal((op.n, nrhs)))
+    n0 = op.b * op.ncell // 2 + 1
+    mk = lambda M: DeviceOperator.from_csc(op.n, M.colptr, M.rowval, M.nzval, vars_per_cell=op.b,
+                                           nchi=op.nchi, neta=op.neta)
+    with mk(Cw) as dW, mk(P1) as dP1, mk(P2) as dP2:
+        dX = DeviceMatrix.from_host(X)
+        dY = DeviceMatrix(op.n, nrhs)
+        try:
+            for pre, post, z in ((None, None, 0), (None, None, n0), (dP1, dP2, n0),
+                                 (dP1, None, 0)):
+                chu_sandwich_batched_ptr(dev, dW, dX.ptr, op.n, nrhs, dY.ptr, op.n, order=5,
+                                         nsub=nt, dt=dt, n_set0=z, pre=pre, post=post)
+                dev.synchronize()
+                got = dY.download()
+                ref = _batched_sandwich_ref(orc, A, dt, nt, Cw, X, z,
+                                            P1 if pre is not None else None,
+                                            P2 if post is not None else None)
+                for j in range(nrhs):
+                    assert relerr(got[:, j], ref[:, j]) <= 1e-12, (pre, post, z, j)
+                if z and post is None:
+                    assert not np.any(got[:z])
+        finally:
+            dX.free()
+            dY.free()
 
 
 def test_f32_storage_error_bound(case, orc):
@@ -278,6 +328,37 @@ def test_complex_sandwich_with_freestream_zeroing(recept, orc):
         got = dev.chu_sandwich(dW, x, order=5, nsub=4, dt=0.05, n_set0=n0)
     assert not np.any(got[:n0])
     assert relerr(got, ref) <= 1e-12
+
+
+def test_complex_batched_sandwich_with_factors(recept, orc):
+    """The batched sandwich on 4 complex columns of the receptivity operator
+    (width 8) with real factor operators and the freestream zeroing, against
+    the oracle's CONSTRUCT_C per column."""
+    base, L_, A, dev = recept
+    say("[receptivity] batched CONSTRUCT_C with factors")
+    shape = ops.BlockOperator(L_.nchi, L_.neta, 4, L_.n_extra, base.brow_ptr, base.bcol,
+                              base.bval)
+    Cw = orc.CSC.from_arrays(*_chu_weight(shape, 10).to_csc_varmajor())
+    P1 = orc.CSC.from_arrays(*_chu_weight(shape, 16).to_csc_varmajor())
+    P2 = orc.CSC.from_arrays(*_chu_weight(shape, 17).to_csc_varmajor())
+    n0 = 4 * base.ncell
+    X = np.asfortranarray(np.stack([_cvec(L_.n, 20 + j) for j in range(4)], axis=1))
+    mk = lambda M: DeviceOperator.from_csc(L_.n, M.colptr, M.rowval, M.nzval, vars_per_cell=4,
+                                           nchi=L_.nchi, neta=L_.neta)
+    with mk(Cw) as dW, mk(P1) as dP1, mk(P2) as dP2:
+        dX = DeviceMatrix.from_host(X)
+        dY = DeviceMatrix(L_.n, 4, np.complex128)
+        try:
+            chu_sandwich_batched_ptr(dev, dW, dX.ptr, L_.n, 4, dY.ptr, L_.n, order=5, nsub=4,
+                                     dt=0.05, n_set0=n0, pre=dP1, post=dP2)
+            dev.synchronize()
+            got = dY.download()
+        finally:
+            dX.free()
+            dY.free()
+    ref = _batched_sandwich_ref(orc, A, 0.05, 4, Cw, X, n0, P1, P2)
+    for j in range(4):
+        assert relerr(got[:, j], ref[:, j]) <= 1e-12, j
 
 
 # ------------------------------------------------------------------ closed forms

@@ -65,3 +65,70 @@ def test_shock_bands_in_the_term_launch(orc, case, prec):
             for form in ("powers", "horner", "normalised"):
                 two = dev.expm_taylor(x, order=5, nsub=nsub, dt=dt, form=form, transpose=tr)
                 assert relerr(two, fused[(tr, form)]) <= 1e-13, (tr, form)
+
+
+@pytest.mark.parametrize("nrhs", [2, 3, 4, 8, 16, 32])
+@pytest.mark.parametrize("case", [(24, 30, 4), (70, 6, 4), (70, 3, 8)],
+                         ids=lambda c: f"{c[0]}x{c[1]}b{c[2]}")
+def test_shock_bands_in_batched_terms(orc, case, nrhs):
+    """Batched sweeps over shock operators with the extra-DOF rows in the
+    batch kernel's launch (k_term_lds at 2 and 4 doubles per row,
+    k_spmm_quad4c at 8+ at b = 4; 216-1190 entries per row) and the flow
+    rows' shock entries in a k_extra launch after it.  Every column against
+    the oracle, forward and adjoint; the two-launch term (a nine-wave tuning
+    at 2 and 4 doubles, batch kernel 7 at 8+) to 1e-13; graph replays give
+    the eager bits."""
+    nchi, neta, b = case
+    say(f"[shock bands batched] {nchi}x{neta} b={b} nrhs={nrhs}")
+    op = ops.stencil_operator(nchi, neta, b, 105 + nchi + b, n_shock=nchi, row_sum=2.0)
+    A = orc.CSC.from_arrays(*op.to_csc_varmajor())
+    At = orc.csc_adjoint(A)
+    X = np.asfortranarray(np.random.default_rng(nrhs).standard_normal((op.n, nrhs)))
+    kw = dict(order=5, nsub=3, dt=0.05, form="horner")
+    with DeviceOperator.from_blocks(op) as dev:
+        got = {}
+        for tr, M in ((False, A), (True, At)):
+            Y = dev.expm_taylor_batched(X, transpose=tr, **kw)
+            Y0 = dev.expm_taylor_batched(X, order=5, nsub=0, dt=0.05, transpose=tr)
+            np.testing.assert_array_equal(Y0, X)
+            for j in range(nrhs):
+                ref = orc.power_method(M.scaled(0.05), X[:, j], 3)
+                assert relerr(Y[:, j], ref) <= 1e-12, (tr, j)
+            got[tr] = Y
+        dev.set_graphs(True)
+        for _ in range(3):
+            for tr in (False, True):
+                np.testing.assert_array_equal(dev.expm_taylor_batched(X, transpose=tr, **kw),
+                                              got[tr])
+        dev.set_graphs(False)
+        if nrhs <= 4:
+            dev.set_tuning(3, 9, 1)
+        else:
+            dev.set_batch_kernel(7)
+        for tr in (False, True):
+            two = dev.expm_taylor_batched(X, transpose=tr, **kw)
+            assert relerr(two, got[tr]) <= 1e-13, tr
+
+
+@pytest.mark.parametrize("nrhs", [2, 3, 4, 8])
+@pytest.mark.parametrize("b", [4, 8])
+def test_shock_rows_at_batched_widths(orc, b, nrhs):
+    """Batched sweeps over a shock operator whose shock rows are long (30
+    eta stations: 216 entries at b = 4): the extra rows at 2..8 doubles per
+    row take one lane per entry (8 doubles at b = 8: the tile row layout);
+    every column against the oracle, forward and adjoint, SpMV and Horner
+    action."""
+    nchi, neta = 24, 30
+    say(f"[shock rows batched] b={b} nrhs={nrhs}")
+    op = ops.stencil_operator(nchi, neta, b, 105 + b, n_shock=nchi, row_sum=2.0)
+    A = orc.CSC.from_arrays(*op.to_csc_varmajor())
+    At = orc.csc_adjoint(A)
+    X = np.asfortranarray(np.random.default_rng(nrhs).standard_normal((op.n, nrhs)))
+    with DeviceOperator.from_blocks(op) as dev:
+        for tr, M in ((False, A), (True, At)):
+            Y = dev.expm_taylor_batched(X, order=5, nsub=3, dt=0.05, form="horner", transpose=tr)
+            Y0 = dev.expm_taylor_batched(X, order=5, nsub=0, dt=0.05, transpose=tr)
+            np.testing.assert_array_equal(Y0, X)
+            for j in range(nrhs):
+                ref = orc.power_method(M.scaled(0.05), X[:, j], 3)
+                assert relerr(Y[:, j], ref) <= 1e-12, (tr, j)

@@ -264,14 +264,17 @@ def test_eigenvalues_device_krylov_matches_closed_form(orc):
     dev.close()
 
 
-def test_transient_growth_device_krylov_vs_dense():
+@pytest.mark.parametrize("block", [1, 2, 4])
+def test_transient_growth_device_krylov_vs_dense(block):
     nchi, neta = 8, 6
-    say("[TRANSIENT_GROWTH_DOWNSTREAM krylov=device] vs dense")
+    say(f"[TRANSIENT_GROWTH_DOWNSTREAM krylov=device block={block}] vs dense")
     s = st.CFL_TIMESTEP(synthetic_solution(nchi, neta, seed=3, perturb_shock=True))
     op = ops.stencil_operator(nchi, neta, 4, seed=7, n_shock=nchi,
                               row_sum=0.5 / s["time_integration"]["dt"])
     A = op.to_scipy_varmajor()
-    V, G, T_opt = st.TRANSIENT_GROWTH_DOWNSTREAM(A, s, 4, 1.0, tol=1e-12, ncv=30, krylov="device")
+    V, G, T_opt = st.TRANSIENT_GROWTH_DOWNSTREAM(A, s, 4, 1.0, tol=1e-12, ncv=30, krylov="device",
+                                                 block=block)
+    assert s["timings"]["transient_growth_krylov"]["block"] == block
     n_t = int(round(1.0 / s["time_integration"]["dt"]))
     R = st.CONSTRUCT_R(s)
     C = R.T @ st.CONSTRUCT_M(s, [1, 1, 1, 1]) @ R
@@ -282,15 +285,17 @@ def test_transient_growth_device_krylov_vs_dense():
     assert np.allclose(V.T @ (D @ V), np.eye(4), atol=1e-8)
 
 
-def test_freestream_receptivity_device_krylov_vs_dense():
+@pytest.mark.parametrize("block", [1, 2, 4])
+def test_freestream_receptivity_device_krylov_vs_dense(block):
     nchi, neta = 6, 5
     w = [0.0, 2.0]
-    say("[FREESTREAM_RECEPTIVITY krylov=device] vs dense")
+    say(f"[FREESTREAM_RECEPTIVITY krylov=device block={block}] vs dense")
     s = st.CFL_TIMESTEP(synthetic_solution(nchi, neta, seed=4, perturb_shock=True))
     base = ops.stencil_operator(nchi, neta, 4, seed=8, n_shock=nchi,
                                 row_sum=0.5 / s["time_integration"]["dt"])
     A = ops.receptivity_extension(base, w).to_scipy_varmajor()
-    V, G, T_opt = st.FREESTREAM_RECEPTIVITY(A, s, 3, 1.0, w, tol=1e-12, ncv=24, krylov="device")
+    V, G, T_opt = st.FREESTREAM_RECEPTIVITY(A, s, 3, 1.0, w, tol=1e-12, ncv=24, krylov="device",
+                                            block=block)
     n_t = int(round(1.0 / s["time_integration"]["dt"]))
     R_ = st.CONSTRUCT_R_(s, w)
     C_ = R_.T @ st.CONSTRUCT_M_(s, [1, 1, 1, 1], w) @ R_
