@@ -741,21 +741,49 @@ bool build_shock_fuse(const HostExtra &x, int64_t nflow, HostExtra &rows_out,
     return true;
 }
 
-HostExtra extra_flow_rows(const HostExtra &x, int64_t nflow) {
-    HostExtra f;
-    f.is_complex = x.is_complex;
-    f.ptr.assign(1, 0);
-    const int vs = x.is_complex ? 2 : 1;
-    for (size_t i = 0; i < x.rows.size(); ++i) {
-        if (x.rows[i] >= nflow) continue;
-        for (int64_t e = x.ptr[i]; e < x.ptr[i + 1]; ++e) {
-            f.col.push_back(x.col[(size_t)e]);
-            for (int q = 0; q < vs; ++q) f.val.push_back(x.val[(size_t)e * vs + q]);
+void build_shock_lead(int b, const std::vector<int64_t> &bptr, const std::vector<int32_t> &bcol,
+                      const double *bval, const std::vector<int64_t> *bsrc, bool transpose,
+                      double scale, const HostExtra &x, const HostExtra &rows_x,
+                      const HostShockPanels &pan, int64_t nflow, HostShockLead &out) {
+    out = HostShockLead();
+    out.sh = rows_x;
+    HostExtra &r64 = out.fr, &r32 = out.fr32;
+    r64.ptr.assign(1, 0);
+    r32.ptr.assign(1, 0);
+    auto push = [&](int32_t c, double v64, double v32) {
+        r64.col.push_back(c);
+        r64.val.push_back(v64);
+        r32.col.push_back(c);
+        r32.val.push_back(v32);
+    };
+    const int bb = b * b;
+    for (size_t sl = 0; sl < pan.slice_tab.size(); ++sl) {
+        if (pan.slice_tab[sl] < 0) continue;
+        for (int64_t row = (int64_t)sl * 64; row < (int64_t)sl * 64 + 64 && row < nflow; ++row) {
+            const int64_t cell = row / b;
+            const int r = (int)(row % b);
+            for (int64_t k = bptr[(size_t)cell]; k < bptr[(size_t)cell + 1]; ++k) {
+                const int64_t src = bsrc ? (*bsrc)[(size_t)k] : k;
+                const int64_t c = bcol[(size_t)k];
+                for (int j = 0; j < b; ++j) {
+                    const double v =
+                        bval[src * bb + (transpose ? j * b + r : r * b + j)] * scale;
+                    push((int32_t)(c * b + j), v, (double)(float)v);
+                }
+            }
+            // the row's extra entries (x lists rows ascending)
+            const auto it = std::lower_bound(x.rows.begin(), x.rows.end(), row);
+            if (it != x.rows.end() && *it == row) {
+                const size_t i = (size_t)(it - x.rows.begin());
+                for (int64_t e = x.ptr[i]; e < x.ptr[i + 1]; ++e)
+                    push(x.col[(size_t)e], x.val[(size_t)e], x.val[(size_t)e]);
+            }
+            r64.rows.push_back(row);
+            r32.rows.push_back(row);
+            r64.ptr.push_back((int64_t)r64.col.size());
+            r32.ptr.push_back((int64_t)r32.col.size());
         }
-        f.rows.push_back(x.rows[i]);
-        f.ptr.push_back((int64_t)f.col.size());
     }
-    return f;
 }
 
 }  // namespace hymor

@@ -114,9 +114,20 @@ struct DevShock {
     int64_t *cptr = nullptr;
     double *pval = nullptr;
     int64_t npcol = 0;
-    // batched terms: the flow rows' extra entries, for k_extra after the
-    // batch kernel has finished the extra-DOF rows in its launch
-    DevExtra flows;
+    // batched terms: the rows the batch kernel's leading workgroups finish
+    // (build_shock_lead): extra-DOF rows, whole flow rows in FP64 and with
+    // FP32-rounded block entries
+    DevExtra lead_sh, lead_fr, lead_fr32;
+};
+
+// Where the block values of the rows in build_shock_lead come from.
+struct BlockSrc {
+    const std::vector<int64_t> *ptr = nullptr;
+    const std::vector<int32_t> *col = nullptr;
+    const double *val = nullptr;
+    const std::vector<int64_t> *src = nullptr;
+    bool transpose = false;
+    double scale = 1.0;
 };
 
 
@@ -421,11 +432,17 @@ void upload_fused(hymor_op *op, const HostExtra &h, const HostReplica &rep, DevE
 
 // The shock bands' fused form (build_shock_fuse) of the extra CSR h, when
 // the operator has it.
-void upload_shock(hymor_op *op, const HostExtra &h, DevShock &d) {
+void upload_shock(hymor_op *op, const HostExtra &h, const BlockSrc &bs, DevShock &d) {
     HostExtra rows;
     HostShockPanels pan;
     d = DevShock();
     if (!build_shock_fuse(h, op->nflow, rows, pan)) return;
+    HostShockLead lead;
+    build_shock_lead(op->b, *bs.ptr, *bs.col, bs.val, bs.src, bs.transpose, bs.scale, h, rows,
+                     pan, op->nflow, lead);
+    upload_extra(op, lead.sh, d.lead_sh);
+    upload_extra(op, lead.fr, d.lead_fr);
+    upload_extra(op, lead.fr32, d.lead_fr32);
     upload_extra(op, rows, d.rows);
     d.slice_tab = upload(op, pan.slice_tab);
     d.cptr = upload(op, pan.cptr);
@@ -437,7 +454,6 @@ void upload_shock(hymor_op *op, const HostExtra &h, DevShock &d) {
         if (pan.slice_tab[k] >= 0) ps.push_back((int32_t)k);
     d.npan = (int64_t)ps.size();
     d.pslices = upload(op, ps);
-    upload_extra(op, extra_flow_rows(h, op->nflow), d.flows);
     d.on = true;
 }
 
@@ -613,15 +629,23 @@ void launch_shock_t(const DevSliced &s, const DevShock &sh, int64_t nflow, hipSt
                        F);
 }
 
-ShockBatchDev shock_batch_dev(const DevShock &sh, int64_t nflow) {
+ShockBatchDev shock_batch_dev(const DevShock &sh, int64_t nflow, bool f32) {
+    const DevExtra &F = f32 ? sh.lead_fr32 : sh.lead_fr;
     ShockBatchDev H;
-    H.nrows = sh.rows.nrows;
-    H.nsh_wg = (H.nrows + 7) / 8 * 8;   // multiples of 8: the slices keep their XCDs
+    H.nsh = sh.lead_sh.nrows;
+    H.nfr = F.nrows;
+    // a multiple of 8: the slices keep their XCDs
+    H.nlead_wg = (H.nsh + H.nfr + 7) / 8 * 8;
     H.nflow = nflow;
-    H.rows = sh.rows.rows;
-    H.ptr = sh.rows.ptr;
-    H.col = sh.rows.col;
-    H.val = sh.rows.val;
+    H.sh_rows = sh.lead_sh.rows;
+    H.sh_ptr = sh.lead_sh.ptr;
+    H.sh_col = sh.lead_sh.col;
+    H.sh_val = sh.lead_sh.val;
+    H.fr_rows = F.rows;
+    H.fr_ptr = F.ptr;
+    H.fr_col = F.col;
+    H.fr_val = F.val;
+    H.skip = sh.slice_tab;
     return H;
 }
 
@@ -640,7 +664,7 @@ void launch_shock_lds_t(const DevSliced &s, const ShockBatchDev &H, hipStream_t 
                         const double *x, double *tout, const double *vin, double *vout,
                         double c1, double c2) {
     hipLaunchKernelGGL((k_term_lds<B, VT, NV, 1, 9, NT, 0, true>),
-                       dim3((unsigned)(H.nsh_wg + s.nslice)), dim3(64), 0, st,
+                       dim3((unsigned)(H.nlead_wg + s.nslice)), dim3(64), 0, st,
                        s.slice_ptr, s.col, (const VT *)s.val, s.nslice, s.ncell, x, tout, vin,
                        vout, c1, c2, FusedRepDev{}, H);
 }
@@ -651,20 +675,18 @@ void launch_shock_quad4c_t(const DevSliced &s, const ShockBatchDev &H, hipStream
                            double c1, double c2) {
     const int64_t grid = ((s.ncell + 3) / 4 + 3) / 4;
     hipLaunchKernelGGL((k_spmm_quad4c<VT, R, W, 0, true>),
-                       dim3((unsigned)(H.nsh_wg + grid)), dim3(256), 0, st,
+                       dim3((unsigned)(H.nlead_wg + grid)), dim3(256), 0, st,
                        s.slice_ptr, s.col, (const VT *)s.val, s.ncell, x, tout, vin, vout, c1,
                        c2, FusedRepDev{}, H);
 }
 
 template <int B, typename VT, bool NT>
-void launch_shock_batched_t(const DevSliced &s, const ShockBatchDev &H, int nv, hipStream_t st,
-                            const double *x, double *tout, const double *vin, double *vout,
-                            double c1, double c2) {
-    switch (nv) {
-    case 2: launch_shock_lds_t<B, VT, 2, NT>(s, H, st, x, tout, vin, vout, c1, c2); return;
-    case 4: launch_shock_lds_t<B, VT, 4, NT>(s, H, st, x, tout, vin, vout, c1, c2); return;
-    default: break;
-    }
+void launch_shock_batched_t(const DevSliced &s, const DevShock &sh, int64_t nflow, int nv,
+                            hipStream_t st, const double *x, double *tout, const double *vin,
+                            double *vout, double c1, double c2) {
+    const ShockBatchDev H = shock_batch_dev(sh, nflow, s.prec == HYMOR_PREC_F32);
+    if (nv == 2) return launch_shock_lds_t<B, VT, 2, NT>(s, H, st, x, tout, vin, vout, c1, c2);
+    if (nv == 4) return launch_shock_lds_t<B, VT, 4, NT>(s, H, st, x, tout, vin, vout, c1, c2);
     if constexpr (B == 4) {
         // the slot groups launch_block_part uses at each width
         switch (nv) {
@@ -681,13 +703,12 @@ void launch_shock_batched_t(const DevSliced &s, const ShockBatchDev &H, int nv, 
 void launch_shock_batched(const DevSliced &s, const DevShock &sh, int64_t nflow, int nv,
                           hipStream_t st, const double *x, double *tout, const double *vin,
                           double *vout, double c1, double c2) {
-    const ShockBatchDev H = shock_batch_dev(sh, nflow);
     const bool f32 = s.prec == HYMOR_PREC_F32, nt = s.tune.nt != 0;
     auto go = [&](auto b, auto vt, auto ntc) {
         constexpr int B = decltype(b)::value;
         using VT = typename decltype(vt)::type;
         constexpr bool NT = decltype(ntc)::value;
-        launch_shock_batched_t<B, VT, NT>(s, H, nv, st, x, tout, vin, vout, c1, c2);
+        launch_shock_batched_t<B, VT, NT>(s, sh, nflow, nv, st, x, tout, vin, vout, c1, c2);
     };
     auto by_nt = [&](auto b, auto vt) {
         if (nt) go(b, vt, std::true_type{});
@@ -1112,16 +1133,11 @@ void launch_term(hymor_op *op, bool adj, int nv, hipStream_t st, const double *x
         HIP_TRY(hipGetLastError());
         return;
     }
-    // batches: the extra-DOF rows in the batch kernel's launch, then the
-    // flow rows' extra entries
+    // batches: the extra-DOF rows and the panel slices' rows in the batch
+    // kernel's leading workgroups, one launch
     if (nv >= 2 && sh.on && !op->cdiag_on && op->ncell > 0 && shock_batched_ok(op, s, nv)) {
         launch_shock_batched(s, sh, op->nflow, nv, st, x, tout, vin, vout, c1, c2);
         HIP_TRY(hipGetLastError());
-        if (sh.flows.nrows > 0) {
-            launch_extra(sh.flows, nv, op->nflow, st, x, tout, vin, vout, c1, c2, ColDiag{},
-                         tile_b_of(op, nv));
-            HIP_TRY(hipGetLastError());
-        }
         return;
     }
     const DevExtra &xe = extra_of(op, adj, nv);
@@ -1614,8 +1630,15 @@ int hymor_op_create_csc(int64_t n, const int64_t *colptr, const int64_t *rowval,
         upload_fused(op, xfc, rf, op->xf_rep);
         upload_replicated(op, xac, ra, op->xa_rep);
         if (rf.kind == 0) {
-            upload_shock(op, xf, op->shf);
-            upload_shock(op, xa, op->sha);
+            BlockSrc bf, ba;
+            bf.ptr = &fwd.ptr;
+            bf.col = &fwd.col;
+            bf.val = fwd.val.data();
+            ba.ptr = &adj.ptr;
+            ba.col = &adj.col;
+            ba.val = adj.val.data();
+            upload_shock(op, xf, bf, op->shf);
+            upload_shock(op, xa, ba, op->sha);
         }
         op->has_adj = true;
         *out = op;
@@ -1745,7 +1768,12 @@ int hymor_op_create_bsr(int64_t nchi, int64_t neta, int vars_per_cell, int64_t n
         upload_extra(op, xf, op->xf);
         upload_replicated(op, xfc, rf, op->xf_rep);
         upload_fused(op, xfc, rf, op->xf_rep);
-        if (rf.kind == 0) upload_shock(op, xf, op->shf);
+        BlockSrc bf;
+        bf.ptr = &fwd.ptr;
+        bf.col = &fwd.col;
+        bf.val = bval;
+        bf.scale = scale;
+        if (rf.kind == 0) upload_shock(op, xf, bf, op->shf);
         if (with_adjoint) {
             std::vector<int64_t> tptr, tsrc;
             std::vector<int32_t> tcol;
@@ -1754,7 +1782,14 @@ int hymor_op_create_bsr(int64_t nchi, int64_t neta, int vars_per_cell, int64_t n
                          value_prec, op->adj);
             upload_extra(op, xa, op->xa);
             upload_replicated(op, xac, ra, op->xa_rep);
-            if (rf.kind == 0) upload_shock(op, xa, op->sha);
+            BlockSrc ba;
+            ba.ptr = &tptr;
+            ba.col = &tcol;
+            ba.val = bval;
+            ba.src = &tsrc;
+            ba.transpose = true;
+            ba.scale = scale;
+            if (rf.kind == 0) upload_shock(op, xa, ba, op->sha);
             op->has_adj = true;
         }
         *out = op;

@@ -147,10 +147,24 @@ struct HostShockPanels {
 bool build_shock_fuse(const HostExtra &x, int64_t nflow, HostExtra &rows_out,
                       HostShockPanels &pan);
 
-// The flow rows of an extra CSR (rows < nflow, in order): what a batched
-// term over the shock bands leaves to k_extra once its launch has finished
-// the extra-DOF rows.
-HostExtra extra_flow_rows(const HostExtra &x, int64_t nflow);
+// The rows a batched term over the shock bands finishes in its batch
+// kernel's leading workgroups (cell-major rows and columns): `sh`, the
+// extra-DOF rows (rows_x, from build_shock_fuse), and `fr`, every flow row
+// of every slice that holds a panel, whole -- its block entries followed by
+// its extra entries -- so that the batch kernel skips those slices and no
+// launch follows it.  Block values are read as bval[src * b * b + r * b + j]
+// for block row r and column j (transpose: bval[src * b * b + j * b + r]),
+// src = bsrc[k] for block k when bsrc is given (else k), times scale.  `fr`
+// holds them in FP64, `fr32` the same with the block entries rounded to FP32
+// (what the batch kernel multiplies when the operator stores FP32).
+struct HostShockLead {
+    HostExtra sh, fr, fr32;
+};
+
+void build_shock_lead(int b, const std::vector<int64_t> &bptr, const std::vector<int32_t> &bcol,
+                      const double *bval, const std::vector<int64_t> *bsrc, bool transpose,
+                      double scale, const HostExtra &x, const HostExtra &rows_x,
+                      const HostShockPanels &pan, int64_t nflow, HostShockLead &out);
 
 void csc_to_blocks(int64_t n, const int64_t *colptr, const int64_t *rowval,
                    const double *nzval, int index_base, bool is_complex, double scale,

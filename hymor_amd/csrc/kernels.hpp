@@ -451,7 +451,7 @@ __device__ __forceinline__ void extra_replica_rows(
 // cdiag (optional): a complex diagonal that differs per column (per pair
 // (re, im) of the NV doubles), added on rows [cd0, cd1) -- the -i*omega_k
 // block of the factored receptivity systems, one wavenumber per column.
-template <int NV, bool CPLX, int NW, bool REP, bool TILE>
+template <int NV, bool CPLX, int NW, bool REP, bool TILE, bool OVW = false>
 __device__ __forceinline__ void extra_long_row(
     int64_t ri, const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
     const int32_t *__restrict__ col, const double *__restrict__ val,
@@ -465,7 +465,9 @@ __device__ __forceinline__ void extra_long_row(
     static_assert(!CPLX || NV >= 2, "complex rows need two doubles");
     static_assert(64 % LPR == 0, "lanes per entry divide the wave");
     __shared__ double part[NW > 1 ? NW - 1 : 1][64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // one wave per row (NW = 1) may run inside a larger workgroup: its
+    // rows are the waves' own
+    const int lane = threadIdx.x & 63, wv = NW > 1 ? (int)(threadIdx.x >> 6) : 0;
     const int j = lane / LPR;                  // entry of the step
     const int v0 = (lane % LPR) * NC;          // first of this lane's doubles
     double acc[NC];
@@ -482,8 +484,12 @@ __device__ __forceinline__ void extra_long_row(
         if (j == 0 && wv == 0) {
             if (row < nflow) {
                 const int64_t o = buf_pos(row, v, NV, tile_b, nflow);
-                if (tout) old_t[q] = tout[o];
-                if (vout) old_v[q] = vout[o];
+                if (OVW) {   // the whole row is here: start from vin
+                    if (vout) old_v[q] = vin[o];
+                } else {
+                    if (tout) old_t[q] = tout[o];
+                    if (vout) old_v[q] = vout[o];
+                }
             } else if (vout) {
                 old_v[q] = vin[row * NV + v];
             }
@@ -646,8 +652,10 @@ __device__ __forceinline__ void extra_short_rows(
 // rounds of dependent loads per 4-wave workgroup.  A lane's entries lie
 // 64*NW/LPR apart.  The chunk sums are reduced across the wave by
 // butterfly shuffles over the lanes of one chunk position and across waves
-// through LDS in wave order.  No per-column diagonal.
-template <int NV, int NW, bool TILE, int LPR>
+// through LDS in wave order.  No per-column diagonal.  OVW: a flow row is
+// written whole (its block entries are in the row), not added to what the
+// block kernel wrote.
+template <int NV, int NW, bool TILE, int LPR, bool OVW = false>
 __device__ __forceinline__ void extra_long_row_lpe(
     int64_t ri, const int64_t *__restrict__ rows, const int64_t *__restrict__ ptr,
     const int32_t *__restrict__ col, const double *__restrict__ val, int64_t nflow,
@@ -659,7 +667,9 @@ __device__ __forceinline__ void extra_long_row_lpe(
     constexpr int EPW = 64 / LPR;
     constexpr int64_t S = (int64_t)NW * EPW;
     __shared__ double part[NW > 1 ? NW - 1 : 1][NV];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // one wave per row (NW = 1) may run inside a larger workgroup: its
+    // rows are the waves' own
+    const int lane = threadIdx.x & 63, wv = NW > 1 ? (int)(threadIdx.x >> 6) : 0;
     const int j = lane / LPR, v0 = (lane % LPR) * CH;
     const int64_t row = rows[ri];
     // lanes [0, LPR) of wave 0 write doubles [v0, v0 + CH): old values first
@@ -670,8 +680,12 @@ __device__ __forceinline__ void extra_long_row_lpe(
         if (wv == 0 && j == 0) {
             if (row < nflow) {
                 const int64_t o = buf_pos(row, v0 + i, NV, tile_b, nflow);
-                if (tout) old_t[i] = tout[o];
-                if (vout) old_v[i] = vout[o];
+                if (OVW) {
+                    if (vout) old_v[i] = vin[o];
+                } else {
+                    if (tout) old_t[i] = tout[o];
+                    if (vout) old_v[i] = vout[o];
+                }
             } else if (vout) {
                 old_v[i] = vin[row * NV + v0 + i];
             }
@@ -1328,16 +1342,22 @@ struct ShockFuseDev {
     const double *pval = nullptr;
 };
 
-// The extra-DOF rows of a shock operator inside a batched term's launch
-// (k_term_lds at 2..4 and k_spmm_quad4c at 8..64 doubles per row):
-// workgroups [0, nsh_wg) finish them -- they read only x, so they overlap
-// the block term -- and the flow rows' shock entries follow in a k_extra
-// launch of their own.
+// A shock operator's batched term in one launch (k_term_lds at 2..4 and
+// k_spmm_quad4c at 8..64 doubles per row): leading workgroups, one row
+// each, finish the extra-DOF rows (CSR sh_*: [0, nsh)) and, whole (block and
+// shock entries), every flow row of the slices with shock columns (CSR fr_*:
+// [nsh, nsh + nfr)) -- they read only x, so they overlap the block term --
+// and the block kernel skips those slices (skip[slice] >= 0).  nlead_wg (a
+// multiple of 8) workgroups lead.
 struct ShockBatchDev {
-    int64_t nsh_wg = 0, nrows = 0, nflow = 0;
-    const int64_t *rows = nullptr, *ptr = nullptr;
-    const int32_t *col = nullptr;
-    const double *val = nullptr;
+    int64_t nlead_wg = 0, nsh = 0, nfr = 0, nflow = 0;
+    const int64_t *sh_rows = nullptr, *sh_ptr = nullptr;
+    const int32_t *sh_col = nullptr;
+    const double *sh_val = nullptr;
+    const int64_t *fr_rows = nullptr, *fr_ptr = nullptr;
+    const int32_t *fr_col = nullptr;
+    const double *fr_val = nullptr;
+    const int32_t *skip = nullptr;
 };
 
 // k_term_split with the loads of each wave grouped: the block columns of
@@ -1559,15 +1579,20 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
     const int w = threadIdx.x >> 6;
     int64_t bid = blockIdx.x, nwg = gridDim.x;
     if constexpr (SH) {
-        // the extra-DOF rows first: they read only x
-        if (bid < H.nsh_wg) {
-            if (bid < H.nrows)
-                extra_long_row_lpe<NV, 1, false, NV / 2>(bid, H.rows, H.ptr, H.col, H.val,
-                                                         H.nflow, x, tout, vin, vout, c1, c2, 0);
+        // the leading rows first: they read only x
+        if (bid < H.nlead_wg) {
+            if (bid < H.nsh)
+                extra_long_row_lpe<NV, 1, false, NV / 2, true>(
+                    bid, H.sh_rows, H.sh_ptr, H.sh_col, H.sh_val, H.nflow, x, tout, vin, vout,
+                    c1, c2, 0);
+            else if (bid < H.nsh + H.nfr)
+                extra_long_row_lpe<NV, 1, false, NV / 2, true>(
+                    bid - H.nsh, H.fr_rows, H.fr_ptr, H.fr_col, H.fr_val, H.nflow, x, tout, vin,
+                    vout, c1, c2, 0);
             return;
         }
-        bid -= H.nsh_wg;
-        nwg -= H.nsh_wg;
+        bid -= H.nlead_wg;
+        nwg -= H.nlead_wg;
     }
     if constexpr (FR != 0) {
         // the replicated rows first (nfs_wg is a multiple of 8, so the
@@ -1590,6 +1615,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 4)))
         if (slice >= nwg) slice -= nwg;
     }
     if (slice >= nslice) return;
+    if constexpr (SH) {
+        if (H.skip[slice] >= 0) return;   // its rows are leading rows
+    }
     const int q = lane / B, r = lane % B;
     const int64_t p0 = slice_ptr[slice];
     const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);
@@ -2811,23 +2839,27 @@ __global__ __launch_bounds__(256) void k_spmm_quad4c(
     static_assert(!SH || FR == 0, "shock bands and replicas do not mix");
     int64_t bid = blockIdx.x, nwg = gridDim.x;
     if constexpr (SH) {
-        // the extra-DOF rows first: they read only x
-        if (bid < SB.nsh_wg) {
-            if (bid < SB.nrows) {
+        // the leading rows first, one per workgroup: they read only x
+        if (bid < SB.nlead_wg) {
+            const bool shr = bid < SB.nsh;
+            const int64_t ri = shr ? bid : bid - SB.nsh;
+            const int64_t *rows = shr ? SB.sh_rows : SB.fr_rows;
+            const int64_t *ptr = shr ? SB.sh_ptr : SB.fr_ptr;
+            const int32_t *lc = shr ? SB.sh_col : SB.fr_col;
+            const double *lv = shr ? SB.sh_val : SB.fr_val;
+            if (shr || ri < SB.nfr) {
                 if constexpr (R <= 8)
-                    extra_long_row_lpe<R, 4, false, R / 2>(bid, SB.rows, SB.ptr, SB.col, SB.val,
-                                                           SB.nflow, x, tout, vin, vout, c1, c2,
-                                                           0);
+                    extra_long_row_lpe<R, 4, false, R / 2, true>(ri, rows, ptr, lc, lv, SB.nflow,
+                                                                 x, tout, vin, vout, c1, c2, 0);
                 else
-                    extra_long_row<R, false, 4, false, false>(bid, SB.rows, SB.ptr, SB.col, SB.val,
-                                                              SB.nflow, x, tout, vin, vout, c1,
-                                                              c2, nullptr, 0, 0, 0,
-                                                              ReplicaDev{});
+                    extra_long_row<R, false, 4, false, false, true>(
+                        ri, rows, ptr, lc, lv, SB.nflow, x, tout, vin, vout, c1, c2, nullptr, 0,
+                        0, 0, ReplicaDev{});
             }
             return;
         }
-        bid -= SB.nsh_wg;
-        nwg -= SB.nsh_wg;
+        bid -= SB.nlead_wg;
+        nwg -= SB.nlead_wg;
     }
     if constexpr (FK::on) {
         // the replicated rows first (nfs_wg a multiple of 8: the slices keep
@@ -2854,6 +2886,9 @@ __global__ __launch_bounds__(256) void k_spmm_quad4c(
     if (cell0 >= ncell) return;
     const int lo = lane & 3, blk = (lane >> 2) & 3, hi = lane >> 4;
     const int64_t slice = cell0 / CPS;
+    if constexpr (SH) {
+        if (SB.skip[slice] >= 0) return;   // its rows are leading rows
+    }
     const int q0 = (int)(cell0 - slice * CPS);
     const int64_t p0 = slice_ptr[slice];
     const int width = (int)((slice_ptr[slice + 1] - p0) / CPS);

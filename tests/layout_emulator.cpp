@@ -1384,17 +1384,22 @@ T &atw(std::vector<T> &v, int64_t i) {
 }
 }  // namespace
 
-// A batched term over a shock operator at R doubles per row: the extra-DOF
-// rows (build_shock_fuse's CSR) finished in the batch kernel's leading
-// workgroups, the block part, then the flow rows' extra entries
-// (extra_flow_rows) added by k_extra -- every access bounds-checked, every
-// row written by exactly one of the three.  x_vm, y_vm: R columns of n
-// (var-major).  info: [fused?, extra-DOF rows, flow rows].  Returns the
-// out-of-bounds count.
+// A batched term over a shock operator at R doubles per row, in one launch:
+// the leading workgroups finish the rows of build_shock_lead (the extra-DOF
+// rows, then every flow row of the slices with shock columns, whole) and
+// the block kernel runs the other slices.  Every row is written exactly
+// once, every access is bounds-checked.  bsr_form = 1 builds the leading
+// rows as the BSR upload does for the adjoint (the caller's raw blocks
+// through bsr_transpose_map, read transposed) and checks them against the
+// CSC upload's (blocks transposed on the host): same columns, same values.
+// The FP32 copy must hold the block entries rounded to FP32 and the extra
+// entries unchanged.  x_vm, y_vm: R columns of n (var-major).  info:
+// [fused?, leading rows, skipped slices, FP32 mismatches, BSR-form
+// mismatches].  Returns the out-of-bounds count.
 extern "C" int emu_spmv_shock_batched(int64_t n, const int64_t *colptr, const int64_t *rowval,
                                       const double *nzval, int base, int b, int64_t ncell,
-                                      int transpose, int R, const double *x_vm, double *y_vm,
-                                      int64_t *info) {
+                                      int transpose, int R, int bsr_form, const double *x_vm,
+                                      double *y_vm, int64_t *info) {
     g_oob = 0;
     HostBSR fwd, adj;
     HostExtra xf, xa;
@@ -1405,52 +1410,89 @@ extern "C" int emu_spmv_shock_batched(int64_t n, const int64_t *colptr, const in
     HostExtra rows;
     HostShockPanels pan;
     const bool ok = build_shock_fuse(X, nflow, rows, pan);
-    for (int i = 0; i < 3; ++i) info[i] = 0;
+    for (int i = 0; i < 5; ++i) info[i] = 0;
     info[0] = ok ? 1 : 0;
     if (!ok) return 0;
-    const HostExtra flows = extra_flow_rows(X, nflow);
-    info[1] = (int64_t)rows.rows.size();
-    info[2] = (int64_t)flows.rows.size();
+    HostShockLead lead;
+    build_shock_lead(b, A.ptr, A.col, A.val.data(), nullptr, false, 1.0, X, rows, pan, nflow,
+                     lead);
+    const HostExtra &SH = lead.sh, &L = lead.fr, &L32 = lead.fr32;
+    info[1] = (int64_t)(SH.rows.size() + L.rows.size());
+    if (L32.rows != L.rows || L32.ptr != L.ptr || L32.col != L.col) ++info[3];
+    for (size_t i = 0; i < L.rows.size(); ++i)
+        for (int64_t e = at(L.ptr, (int64_t)i); e < at(L.ptr, (int64_t)i + 1); ++e) {
+            const double v = at(L.val, e), v32 = at(L32.val, e);
+            // block entries (flow row, flow column) rounded, extra entries unchanged
+            const bool block = L.rows[i] < nflow && at(L.col, e) < nflow;
+            if (v32 != (block ? (double)(float)v : v)) ++info[3];
+        }
+    if (bsr_form && transpose) {
+        // the BSR upload's adjoint: raw forward blocks through the transpose map
+        std::vector<int64_t> tptr, tsrc;
+        std::vector<int32_t> tcol;
+        bsr_transpose_map(ncell, fwd.ptr, fwd.col, tptr, tcol, tsrc);
+        HostShockLead lb;
+        build_shock_lead(b, tptr, tcol, fwd.val.data(), &tsrc, true, 1.0, X, rows, pan, nflow, lb);
+        if (lb.fr.rows != L.rows || lb.fr.ptr != L.ptr || lb.fr.col != L.col ||
+            lb.fr.val != L.val || lb.sh.rows != SH.rows || lb.sh.val != SH.val)
+            ++info[4];
+    }
     std::vector<double> x((size_t)(n * R)), y((size_t)(n * R), 0.0);
     std::vector<int> writes((size_t)n, 0);
     for (int64_t row = 0; row < n; ++row) {
         const int64_t vm = row < nflow ? (row % b) * ncell + row / b : row;
         for (int v = 0; v < R; ++v) x[(size_t)(row * R + v)] = x_vm[(int64_t)v * n + vm];
     }
-    // leading workgroups: the extra-DOF rows
-    for (int64_t i = 0; i < (int64_t)rows.rows.size(); ++i) {
-        const int64_t row = at(rows.rows, i);
-        if (row < nflow || row >= n) ++g_oob;
+    // leading workgroups, one row each, as both kernels map them: the
+    // extra-DOF rows, then the whole flow rows
+    const int64_t nsh = (int64_t)SH.rows.size(), nfr = (int64_t)L.rows.size();
+    const int64_t nlead_wg = (nsh + nfr + 7) / 8 * 8;
+    std::vector<int> done_sh((size_t)nsh, 0), done_fr((size_t)nfr, 0);
+    auto run_row = [&](const HostExtra &C, int64_t i) {
+        const int64_t row = at(C.rows, i);
+        if (row < 0 || row >= n) ++g_oob;
         atw(writes, row) += 1;
         for (int v = 0; v < R; ++v) {
             double acc = 0.0;
-            for (int64_t e = at(rows.ptr, i); e < at(rows.ptr, i + 1); ++e)
-                acc += at(rows.val, e) * at(x, (int64_t)at(rows.col, e) * R + v);
+            for (int64_t e = at(C.ptr, i); e < at(C.ptr, i + 1); ++e)
+                acc += at(C.val, e) * at(x, (int64_t)at(C.col, e) * R + v);
             atw(y, row * R + v) = acc;
         }
-    }
-    // the block part of every flow row
-    for (int64_t row = 0; row < nflow; ++row) {
-        const int64_t cell = row / b;
-        const int r = (int)(row % b);
-        atw(writes, row) += 1;
-        for (int64_t k = at(A.ptr, cell); k < at(A.ptr, cell + 1); ++k) {
-            const int64_t c = at(A.col, k);
-            for (int j = 0; j < b; ++j)
-                for (int v = 0; v < R; ++v)
-                    atw(y, row * R + v) +=
-                        at(A.val, k * b * b + r * b + j) * at(x, (c * b + j) * R + v);
+    };
+    for (int64_t bid = 0; bid < nlead_wg; ++bid) {
+        if (bid < nsh) {
+            atw(done_sh, bid) += 1;
+            run_row(SH, bid);
+        } else if (bid - nsh < nfr) {
+            atw(done_fr, bid - nsh) += 1;
+            run_row(L, bid - nsh);
         }
     }
-    // k_extra over the flow rows: adds to what the block kernel wrote
-    for (int64_t i = 0; i < (int64_t)flows.rows.size(); ++i) {
-        const int64_t row = at(flows.rows, i);
-        if (row >= nflow) ++g_oob;
-        for (int v = 0; v < R; ++v) {
-            double acc = 0.0;
-            for (int64_t e = at(flows.ptr, i); e < at(flows.ptr, i + 1); ++e)
-                acc += at(flows.val, e) * at(x, (int64_t)at(flows.col, e) * R + v);
-            atw(y, row * R + v) += acc;
+    for (int d : done_sh)
+        if (d != 1) ++g_oob;
+    for (int d : done_fr)
+        if (d != 1) ++g_oob;
+    // the block kernel over the slices it does not skip
+    const int64_t nslice = (nflow + 63) / 64;
+    if ((int64_t)pan.slice_tab.size() != nslice) ++g_oob;
+    for (int64_t slice = 0; slice < nslice; ++slice) {
+        if (at(pan.slice_tab, slice) >= 0) {
+            ++info[2];
+            continue;
+        }
+        for (int64_t row = slice * 64; row < slice * 64 + 64 && row < nflow; ++row) {
+            const int64_t cell = row / b;
+            const int r = (int)(row % b);
+            atw(writes, row) += 1;
+            for (int v = 0; v < R; ++v) {
+                double acc = 0.0;
+                for (int64_t k = at(A.ptr, cell); k < at(A.ptr, cell + 1); ++k) {
+                    const int64_t c = at(A.col, k);
+                    for (int j = 0; j < b; ++j)
+                        acc += at(A.val, k * b * b + r * b + j) * at(x, (c * b + j) * R + v);
+                }
+                atw(y, row * R + v) = acc;
+            }
         }
     }
     for (int64_t row = 0; row < n; ++row)
