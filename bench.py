@@ -171,9 +171,12 @@ def transient_growth_leg(T=1.0, k=4, tol=1e-8, steps=5):
     """Optimal transient growth at the 400 x 300-cell, b = 4 size with 400
     shock DOFs (N = 480 400; Chu weights from a synthetic base flow of that
     grid, dt = 1/64): TRANSIENT_GROWTH_DOWNSTREAM for horizon T with the
-    device Krylov solver (k gains), and the Chu sandwich it applies
+    device Krylov solver (k gains; block Krylov-Schur on blocks of 2, and
+    the single-vector solver for comparison, each split into upload,
+    Cholesky reduction and Krylov seconds), and the Chu sandwich it applies
     (forward Taylor sweep, weight, adjoint sweep; T_5, n_t = T/dt
-    substeps each way) timed alone, with its weight stage timed as the
+    substeps each way) timed alone, on 1 and on 2, 4 and 8 columns, with
+    its weight stage timed as the
     sandwich at n_t = 0 (permutations and the block-diagonal weight); and
     the gain's gradient with respect to the operator's entries
     (GainFunctional.operator_gradient: one forward action, the weight, and
@@ -182,7 +185,8 @@ def transient_growth_leg(T=1.0, k=4, tol=1e-8, steps=5):
     from hymor_amd import operators as ops
     from hymor_amd import stability as st
     from hymor_amd.baseflow import synthetic_solution
-    from hymor_amd.core import DeviceOperator, DeviceVector, chu_sandwich_ptr
+    from hymor_amd.core import (DeviceMatrix, DeviceOperator, DeviceVector,
+                                chu_sandwich_batched_ptr, chu_sandwich_ptr)
 
     nchi, neta = 400, 300
     s = st.CFL_TIMESTEP(synthetic_solution(nchi, neta, seed=7, perturb_shock=True, cfl=100.0,
@@ -196,11 +200,20 @@ def transient_growth_leg(T=1.0, k=4, tol=1e-8, steps=5):
     out = {"workload": (f"400x300 cells, b=4, 400 shock DOFs, N={op.n}; Chu weights of a "
                         f"synthetic base flow; T={T}, dt={dt}, n_t={n_t}, order 5"),
            "k": k, "tol": tol}
-    t0 = time.perf_counter()
-    _, G, _ = st.TRANSIENT_GROWTH_DOWNSTREAM(A, s, k, T, tol=tol, krylov="device", ncv=24,
-                                             maxiter=400)
-    out["solve_seconds"] = time.perf_counter() - t0
+    def solve(block):
+        t0 = time.perf_counter()
+        _, G, _ = st.TRANSIENT_GROWTH_DOWNSTREAM(A, s, k, T, tol=tol, krylov="device", ncv=24,
+                                                 maxiter=400, block=block)
+        secs = time.perf_counter() - t0
+        tk = s["timings"]["transient_growth_krylov"]
+        return G, {"seconds": secs, "upload_seconds": s["timings"]["transient_growth_upload"],
+                   "reduce_seconds": tk["reduce_seconds"], "krylov_seconds": tk["solve_seconds"],
+                   "applications": tk.get("applications"), "block": block}
+    G, out["solve"] = solve(2)
+    G1, out["solve_single_vector"] = solve(1)
+    out["solve_seconds"] = out["solve"]["seconds"]
     out["gains"] = [float(g) for g in G]
+    out["solve"]["max_rel_diff_vs_single_vector"] = float(np.max(np.abs(G - G1) / np.abs(G1)))
     x = np.random.default_rng(8).standard_normal(op.n)
     with DeviceOperator.from_blocks(op, scale=dt) as dA, \
             DeviceOperator.from_scipy(C, nchi=nchi, neta=neta) as dC:
@@ -218,12 +231,35 @@ def transient_growth_leg(T=1.0, k=4, tol=1e-8, steps=5):
         finally:
             dx.free()
             dy.free()
+        # the sandwich on blocks of columns (hymor_chu_sandwich_batched_dev)
+        batched = {}
+        for nrhs in (2, 4, 8):
+            X = DeviceMatrix.from_host(
+                np.asfortranarray(np.random.default_rng(nrhs).standard_normal((op.n, nrhs))))
+            Y = DeviceMatrix(op.n, nrhs)
+            try:
+                chu_sandwich_batched_ptr(dA, dC, X.ptr, op.n, nrhs, Y.ptr, op.n, order=5,
+                                         nsub=n_t)
+                dA.synchronize()
+                t1 = time.perf_counter()
+                for _ in range(steps):
+                    chu_sandwich_batched_ptr(dA, dC, X.ptr, op.n, nrhs, Y.ptr, op.n, order=5,
+                                             nsub=n_t)
+                dA.synchronize()
+                tb = (time.perf_counter() - t1) / steps
+            finally:
+                X.free()
+                Y.free()
+            batched[str(nrhs)] = {"ms": tb * 1e3, "ms_per_vector": tb * 1e3 / nrhs}
     out["sandwich_ms"] = ts * 1e3
     out["sandwich_weight_stage_ms"] = tw * 1e3
     out["sandwich_terms"] = 2 * 5 * n_t
     out["sandwich_note"] = "device-resident vectors (hymor_chu_sandwich_dev)"
-    log(f"transient growth: solve {out['solve_seconds']:.2f} s, gains {out['gains']}, "
-        f"sandwich {ts * 1e3:.2f} ms, weight stage {tw * 1e3:.3f} ms")
+    out["sandwich_batched"] = batched
+    log(f"transient growth: solve {out['solve_seconds']:.2f} s (blocks of 2; single vector "
+        f"{out['solve_single_vector']['seconds']:.2f} s), gains {out['gains']}, "
+        f"sandwich {ts * 1e3:.2f} ms, weight stage {tw * 1e3:.3f} ms, batched per vector "
+        + ", ".join(f"{k_}: {v['ms_per_vector']:.2f} ms" for k_, v in batched.items()))
     # the gain's sensitivity to the operator's entries
     from hymor_amd.adjoint import GainFunctional
     D = (R.T @ st.CONSTRUCT_M_HAT(s, [1, 1, 1, 1]) @ R).tocsc()
@@ -262,7 +298,8 @@ def shock_config_leg(steps=5, order=12, nsub=64, T=1.0, k=20, ncv=80, cpu=True):
         (one-way coupling; its diagonal blocks' eigenvalues), with the
         worst relative eigenvalue error of each."""
     from hymor_amd import operators as ops
-    from hymor_amd.core import DeviceOperator, DeviceVector, expm_taylor_dev
+    from hymor_amd.core import (DeviceMatrix, DeviceOperator, DeviceVector,
+                                expm_taylor_batched_ptr, expm_taylor_dev)
 
     nchi, neta = 200, 300
     op = ops.config_operator("C2")
@@ -284,18 +321,42 @@ def shock_config_leg(steps=5, order=12, nsub=64, T=1.0, k=20, ncv=80, cpu=True):
             dev.synchronize()
             ta = (time.perf_counter() - t0) / steps
             k_ms, k_bytes = dev.bench_term_kernel(reps=20, form=FORM)
+            # 16 vectors per sweep (FP64 values): the shock rows and the
+            # shock-column slices in the batch kernel's leading workgroups
+            nb = 16
+            X = DeviceMatrix.from_host(
+                np.asfortranarray(np.random.default_rng(10).standard_normal((op.n, nb))))
+            Y = DeviceMatrix(op.n, nb)
+            try:
+                runb = lambda: expm_taylor_batched_ptr(dev, X.ptr, op.n, nb, Y.ptr, op.n,
+                                                       order=order, nsub=nsub, dt=T / nsub,
+                                                       form=FORM)
+                runb()
+                dev.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    runb()
+                dev.synchronize()
+                tb = (time.perf_counter() - t0) / steps
+            finally:
+                X.free()
+                Y.free()
         finally:
             dx.free()
             dy.free()
     out["gpu"] = {"value": 1.0 / ta, "ms_per_action": ta * 1e3,
                   "term_ms": k_ms, "term_GBps": k_bytes / (k_ms * 1e-3) / 1e9,
                   "note": "term: block kernel + the shock bands' extra rows (HIP events)"}
+    out["gpu_batched16"] = {"value": nb / tb, "ms_per_sweep": tb * 1e3,
+                            "vs_single": (nb / tb) * ta,
+                            "note": "16 vectors per sweep, FP64 values"}
     if cpu:
         out["cpu"] = _cpu_action_sample(op, x, order, nsub, T)
         out["gpu_vs_cpu"] = out["gpu"]["value"] / out["cpu"]["value"]
     out["eigen"] = _eigen_timings(nchi, neta, k, ncv, seed=12, n_lead=24, bulk=(-6.0, -3.0),
                                   n_shock=nchi, tag="shock config")
-    log(f"shock config: GPU {out['gpu']['value']:.1f} actions/s"
+    log(f"shock config: GPU {out['gpu']['value']:.1f} actions/s, "
+        f"{out['gpu_batched16']['value']:.1f} in 16-vector sweeps"
         + (f", CPU {out['cpu']['value']:.3f} ({out['gpu_vs_cpu']:.0f}x)" if cpu else ""))
     return out
 

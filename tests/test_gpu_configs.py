@@ -1,7 +1,10 @@
-"""Full-size configurations C3 and C4 (sizes the oracle cannot run a full
+"""Full-size configurations C2, C3 and C4 (sizes the oracle cannot run a full
 action on in seconds) through size-independent properties and single
 oracle SpMVs.
 
+* C2: 200 x 300 cells, b = 4, 200 shock DOFs (N = 240 200): 16-vector
+  sweeps (FP64 and FP32 values) against the plain action, one batched SpMV
+  column against the oracle.
 * C3: 400 x 300 cells, b = 4, 400 shock DOFs (N = 480 400): one SpMV
   against the oracle, the adjoint dot-product identity of the Taylor action,
   linearity.  (The Chu sandwich, transient growth and the gain gradient at
@@ -50,6 +53,30 @@ def test_c3_properties(orc):
         assert abs(Tx @ y - x @ Ty_t) <= 1e-12 * abs(Tx @ y)
         a, b = 0.3, -1.7
         assert relerr(T(a * x + b * y), a * Tx + b * T(y)) <= 1e-13
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-13), ("f32", 2e-6)])
+def test_c2_batched_columns(orc, prec, tol):
+    """C2 (200 x 300 cells, b = 4, 200 shock DOFs, N = 240 200): 16 vectors
+    per Horner sweep -- the shock rows and the shock-column slices in the
+    batch kernel's leading workgroups -- equal the plain FP64 action column
+    by column, forward and adjoint (FP32-stored values: within their
+    rounding band); one SpMV column against the oracle."""
+    say(f"[C2 200x300 b4 + 200 shock DOFs] 16-vector sweeps, values {prec}")
+    op = ops.config_operator("C2")
+    X = np.asfortranarray(np.random.default_rng(41).standard_normal((op.n, 16)))
+    kw = dict(order=12, nsub=4, dt=1 / 64, form="horner")
+    with DeviceOperator.from_blocks(op) as d64, \
+            DeviceOperator.from_blocks(op, value_prec=prec) as dev:
+        for tr in (False, True):
+            Y = dev.expm_taylor_batched(X, transpose=tr, **kw)
+            for j in (0, 7, 15):
+                want = d64.expm_taylor(X[:, j], transpose=tr, **kw)
+                assert relerr(Y[:, j], want) <= tol, (tr, j)
+        if prec == "f64":
+            A = orc.CSC.from_arrays(*op.to_csc_varmajor())
+            Y1 = dev.expm_taylor_batched(X, order=1, nsub=1, dt=1.0) - X
+            assert relerr(Y1[:, 3], orc.csc_mul(A, X[:, 3])) <= 1e-13
 
 
 @pytest.fixture(scope="module")

@@ -37,7 +37,8 @@ from hymor_amd import operators as ops
 from hymor_amd import stability as st
 from hymor_amd.adjoint import GainFunctional
 from hymor_amd.baseflow import synthetic_solution
-from hymor_amd.core import DeviceMatrix, DeviceOperator, expm_taylor_batched_ptr
+from hymor_amd.core import (DeviceMatrix, DeviceOperator, chu_sandwich_batched_ptr,
+                            expm_taylor_batched_ptr)
 
 pytestmark = pytest.mark.gpu
 
@@ -274,6 +275,49 @@ def test_c3_transient_growth(c3, T):
             # L^-T y, |y| = 1) to residual tol * G: ||L^-1 (S v - G D v)||
             res = Linv @ (Sv - G[j] * (D @ v))
             assert np.linalg.norm(res) <= 10 * tol * G[j], j
+
+
+def test_c3_batched_sandwich_columns(c3):
+    """The sandwich on 4 and 8 columns at once between the Cholesky factors
+    of R'M_hat R (hymor_chu_sandwich_batched_dev: the shock rows and the
+    shock-column slices in the batch kernel's launch) equals, column by
+    column, L^-1 times the single-vector sandwich of L^-T x, to 1e-12."""
+    s, op, A, C, D = c3
+    dt = s["time_integration"]["dt"]
+    n_t = 8
+    nchi, neta = s["mesh"]["Nchi"], s["mesh"]["Neta"]
+    Linv, LinvH = st.block_cholesky_inverse(D)
+    say(f"[400x300 b4] batched Chu sandwich with factors, n_t={n_t}")
+    with DeviceOperator.from_blocks(op, scale=dt) as dA, \
+            DeviceOperator.from_scipy(C, nchi=nchi, neta=neta) as dC, \
+            DeviceOperator.from_scipy(Linv, nchi=nchi, neta=neta) as dL, \
+            DeviceOperator.from_scipy(LinvH, nchi=nchi, neta=neta) as dLH:
+        for nrhs in (4, 8):
+            X = np.asfortranarray(np.random.default_rng(nrhs).standard_normal((op.n, nrhs)))
+            dX, dY = DeviceMatrix.from_host(X), DeviceMatrix(op.n, nrhs)
+            try:
+                chu_sandwich_batched_ptr(dA, dC, dX.ptr, op.n, nrhs, dY.ptr, op.n, order=5,
+                                         nsub=n_t, pre=dLH, post=dL)
+                dA.synchronize()
+                Y = dY.download()
+            finally:
+                dX.free()
+                dY.free()
+            for j in (0, nrhs - 1):
+                want = Linv @ dA.chu_sandwich(dC, LinvH @ X[:, j], order=5, nsub=n_t, dt=1.0)
+                assert relerr(Y[:, j], want) <= 1e-12, (nrhs, j)
+
+
+def test_c3_transient_growth_block_sizes_agree(c3):
+    """Block Krylov-Schur on blocks of 4 (batched sandwiches) and the
+    single-vector solver find the same 4 gains at T = 1 (tolerance 1e-10)."""
+    s, op, A, C, D = c3
+    say("[400x300 b4] TRANSIENT_GROWTH_DOWNSTREAM T=1, blocks of 1 and 4")
+    _, G1, _ = st.TRANSIENT_GROWTH_DOWNSTREAM(A, s, 4, 1.0, tol=1e-10, krylov="device", ncv=24,
+                                              maxiter=400, block=1)
+    _, G4, _ = st.TRANSIENT_GROWTH_DOWNSTREAM(A, s, 4, 1.0, tol=1e-10, krylov="device", ncv=24,
+                                              maxiter=400, block=4)
+    assert np.all(np.abs(G4 - G1) <= 1e-9 * np.abs(G1)), (G1, G4)
 
 
 def test_c3_gain_gradient_fd(c3):
