@@ -923,11 +923,12 @@ def main():
     dev.close()
     nt_s = str(bool(tuning[2])).lower()
     # split 3 (x staged through LDS) runs k_term_split_p at one double per row,
-    # with 9 waves per slice if so tuned, else 3
+    # with 9 waves per slice if so tuned, else 3 (its last template argument:
+    # no shock bands on this operator)
     nw_p = 9 if tuning[1] == 9 else 3
     kname = {0: f"k_term<{cfg['b']}, double, 1, {nt_s}>",
              1: f"k_term_split<{cfg['b']}, double, 1, {tuning[1]}, {nt_s}>",
-             2: f"k_term_split_p<{cfg['b']}, double, 1, {nw_p}, {9 // nw_p}, {nt_s}>"
+             2: f"k_term_split_p<{cfg['b']}, double, 1, {nw_p}, {9 // nw_p}, {nt_s}, false>"
              }[min(tuning[0], 2)]
     traffic, factors = None, None
     if passes is not None:
