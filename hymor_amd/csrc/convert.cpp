@@ -9,6 +9,9 @@
 #include "hymor_internal.hpp"
 
 #include <algorithm>
+#include <exception>
+#include <functional>
+#include <thread>
 #include <numeric>
 #include <stdexcept>
 
@@ -24,6 +27,41 @@ struct BEnt {
 
 inline int64_t vm_to_cm(int64_t vm, int64_t ncell, int b, int64_t nflow) {
     return vm < nflow ? (vm % ncell) * b + vm / ncell : vm;
+}
+
+}  // namespace
+
+namespace {
+
+// Host threads for the upload's conversion: the hardware's, at most 16,
+// and at most one per 4 096 cell columns.
+int host_threads(int64_t ncell) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int64_t by_size = std::max<int64_t>(1, ncell / 4096);
+    return (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)(hw ? hw : 1), 16, by_size}));
+}
+
+// f(t) for t = 0 .. nth-1, each on a thread of its own (t = 0 on the
+// caller's); an exception in any is rethrown here.
+void parallel_for(int nth, const std::function<void(int)> &f) {
+    std::vector<std::thread> pool;
+    std::vector<std::exception_ptr> err((size_t)nth);
+    for (int t = 1; t < nth; ++t)
+        pool.emplace_back([&, t] {
+            try {
+                f(t);
+            } catch (...) {
+                err[(size_t)t] = std::current_exception();
+            }
+        });
+    try {
+        f(0);
+    } catch (...) {
+        err[0] = std::current_exception();
+    }
+    for (std::thread &th : pool) th.join();
+    for (const std::exception_ptr &e : err)
+        if (e) std::rethrow_exception(e);
 }
 
 }  // namespace
@@ -44,51 +82,80 @@ void csc_to_blocks(int64_t n, const int64_t *colptr, const int64_t *rowval,
         if (rowval[k] < base || rowval[k] >= n + base)
             throw std::invalid_argument("rowval out of range");
 
-    std::vector<int64_t> xr, xc;
-    std::vector<double> xv;
     const int vstride = is_complex ? 2 : 1;
 
-    // block-compressed columns (BSC) over cell columns
-    std::vector<int64_t> bsc_ptr(ncell + 1, 0);
-    std::vector<int64_t> bsc_row;
-    std::vector<double> bsc_val;
-    std::vector<BEnt> tmp;
-    for (int64_t cc = 0; cc < ncell; ++cc) {
-        tmp.clear();
-        for (int k = 0; k < b; ++k) {
-            const int64_t vmc = (int64_t)k * ncell + cc;
-            const int64_t cmc = cc * b + k;
-            for (int64_t e = colptr[vmc] - base; e < colptr[vmc + 1] - base; ++e) {
-                const int64_t vmr = rowval[e] - base;
-                const int64_t cmr = vm_to_cm(vmr, ncell, b, nflow);
-                const double re = nzval[e * vstride] * scale;
-                const double im = is_complex ? nzval[e * vstride + 1] * scale : 0.0;
-                if (cmr < nflow) {
-                    tmp.push_back({cmr / b, (int32_t)(cmr % b), k, re});
-                    if (is_complex && im != 0.0) {
-                        xr.push_back(cmr); xc.push_back(cmc);
-                        xv.push_back(0.0); xv.push_back(im);
+    // block-compressed columns (BSC) over cell columns, contiguous ranges of
+    // cell columns per host thread, concatenated in order (the same arrays
+    // as one pass over the columns)
+    const int nth = host_threads(ncell);
+    struct Part {
+        std::vector<int64_t> cnt;        // blocks per cell column of the range
+        std::vector<int64_t> row;        // block rows
+        std::vector<double> val;         // b*b per block
+        std::vector<int64_t> xr, xc;     // extra entries
+        std::vector<double> xv;
+    };
+    std::vector<Part> parts((size_t)nth);
+    auto c_lo = [&](int t) { return ncell * t / nth; };
+    parallel_for(nth, [&](int t) {
+        Part &P = parts[(size_t)t];
+        std::vector<BEnt> tmp;
+        const int64_t c0 = c_lo(t), c1 = c_lo(t + 1);
+        P.cnt.assign((size_t)(c1 - c0), 0);
+        for (int64_t cc = c0; cc < c1; ++cc) {
+            tmp.clear();
+            for (int k = 0; k < b; ++k) {
+                const int64_t vmc = (int64_t)k * ncell + cc;
+                const int64_t cmc = cc * b + k;
+                for (int64_t e = colptr[vmc] - base; e < colptr[vmc + 1] - base; ++e) {
+                    const int64_t vmr = rowval[e] - base;
+                    const int64_t cmr = vm_to_cm(vmr, ncell, b, nflow);
+                    const double re = nzval[e * vstride] * scale;
+                    const double im = is_complex ? nzval[e * vstride + 1] * scale : 0.0;
+                    if (cmr < nflow) {
+                        tmp.push_back({cmr / b, (int32_t)(cmr % b), k, re});
+                        if (is_complex && im != 0.0) {
+                            P.xr.push_back(cmr); P.xc.push_back(cmc);
+                            P.xv.push_back(0.0); P.xv.push_back(im);
+                        }
+                    } else {
+                        P.xr.push_back(cmr); P.xc.push_back(cmc);
+                        P.xv.push_back(re);
+                        if (is_complex) P.xv.push_back(im);
                     }
-                } else {
-                    xr.push_back(cmr); xc.push_back(cmc);
-                    xv.push_back(re);
-                    if (is_complex) xv.push_back(im);
                 }
             }
-        }
-        std::stable_sort(tmp.begin(), tmp.end(),
-                         [](const BEnt &a, const BEnt &c) { return a.cell_r < c.cell_r; });
-        int64_t last = -1;
-        for (const BEnt &e : tmp) {
-            if (e.cell_r != last) {
-                bsc_row.push_back(e.cell_r);
-                bsc_val.resize(bsc_val.size() + bb, 0.0);
-                last = e.cell_r;
+            std::stable_sort(tmp.begin(), tmp.end(),
+                             [](const BEnt &a, const BEnt &c) { return a.cell_r < c.cell_r; });
+            int64_t last = -1, nb = 0;
+            for (const BEnt &e : tmp) {
+                if (e.cell_r != last) {
+                    P.row.push_back(e.cell_r);
+                    P.val.resize(P.val.size() + bb, 0.0);
+                    last = e.cell_r;
+                    ++nb;
+                }
+                // duplicates are summed, as sparse(I,J,V) does
+                P.val[P.val.size() - bb + e.i * b + e.j] += e.v;
             }
-            // duplicates are summed, as sparse(I,J,V) does
-            bsc_val[bsc_val.size() - bb + e.i * b + e.j] += e.v;
+            P.cnt[(size_t)(cc - c0)] = nb;
         }
-        bsc_ptr[cc + 1] = (int64_t)bsc_row.size();
+    });
+    std::vector<int64_t> bsc_ptr(ncell + 1, 0);
+    std::vector<int64_t> part_blk((size_t)nth + 1, 0);   // first block of each range
+    for (int t = 0; t < nth; ++t) {
+        const Part &P = parts[(size_t)t];
+        for (int64_t i = 0; i < (int64_t)P.cnt.size(); ++i)
+            bsc_ptr[c_lo(t) + i + 1] = bsc_ptr[c_lo(t) + i] + P.cnt[(size_t)i];
+        part_blk[(size_t)t + 1] = part_blk[(size_t)t] + (int64_t)P.row.size();
+    }
+    const int64_t nnzb = bsc_ptr[ncell];
+    std::vector<int64_t> xr, xc;
+    std::vector<double> xv;
+    for (const Part &P : parts) {
+        xr.insert(xr.end(), P.xr.begin(), P.xr.end());
+        xc.insert(xc.end(), P.xc.begin(), P.xc.end());
+        xv.insert(xv.end(), P.xv.begin(), P.xv.end());
     }
     // columns of extra DOFs: every entry is extra
     for (int64_t vmc = nflow; vmc < n; ++vmc) {
@@ -102,35 +169,58 @@ void csc_to_blocks(int64_t n, const int64_t *colptr, const int64_t *rowval,
     }
 
     // adjoint BSR = BSC with each block transposed (real parts: no conjugate)
-    const int64_t nnzb = (int64_t)bsc_row.size();
     adj.b = b;
     adj.ncell = ncell;
     adj.ptr = bsc_ptr;
     adj.col.resize(nnzb);
     adj.val.resize((size_t)nnzb * bb);
-    for (int64_t k = 0; k < nnzb; ++k) {
-        adj.col[k] = (int32_t)bsc_row[k];
-        for (int i = 0; i < b; ++i)
-            for (int j = 0; j < b; ++j)
-                adj.val[(size_t)k * bb + j * b + i] = bsc_val[(size_t)k * bb + i * b + j];
-    }
-    // forward BSR = BSC regrouped by block row (stable => columns ascending)
+    parallel_for(nth, [&](int t) {
+        const Part &P = parts[(size_t)t];
+        const int64_t k0 = part_blk[(size_t)t];
+        for (int64_t q = 0; q < (int64_t)P.row.size(); ++q) {
+            const int64_t k = k0 + q;
+            adj.col[k] = (int32_t)P.row[(size_t)q];
+            for (int i = 0; i < b; ++i)
+                for (int j = 0; j < b; ++j)
+                    adj.val[(size_t)k * bb + j * b + i] = P.val[(size_t)q * bb + i * b + j];
+        }
+    });
+    // forward BSR = BSC regrouped by block row (columns ascending): each
+    // range counts its blocks per block row; a block row's blocks are placed
+    // range by range, so the order is that of one pass over the columns
+    std::vector<std::vector<int64_t>> rc((size_t)nth);
+    parallel_for(nth, [&](int t) {
+        std::vector<int64_t> &c = rc[(size_t)t];
+        c.assign((size_t)ncell, 0);
+        for (const int64_t r : parts[(size_t)t].row) c[(size_t)r]++;
+    });
     fwd.b = b;
     fwd.ncell = ncell;
     fwd.ptr.assign(ncell + 1, 0);
-    for (int64_t k = 0; k < nnzb; ++k) fwd.ptr[bsc_row[k] + 1]++;
-    for (int64_t c = 0; c < ncell; ++c) fwd.ptr[c + 1] += fwd.ptr[c];
+    for (int64_t r = 0; r < ncell; ++r) {
+        int64_t acc = fwd.ptr[r];
+        for (int t = 0; t < nth; ++t) {
+            const int64_t c = rc[(size_t)t][(size_t)r];
+            rc[(size_t)t][(size_t)r] = acc;   // this range's first slot in row r
+            acc += c;
+        }
+        fwd.ptr[r + 1] = acc;
+    }
     fwd.col.resize(nnzb);
     fwd.val.resize((size_t)nnzb * bb);
-    std::vector<int64_t> pos(fwd.ptr.begin(), fwd.ptr.end() - 1);
-    for (int64_t cc = 0; cc < ncell; ++cc) {
-        for (int64_t k = bsc_ptr[cc]; k < bsc_ptr[cc + 1]; ++k) {
-            const int64_t d = pos[bsc_row[k]]++;
-            fwd.col[d] = (int32_t)cc;
-            std::copy(bsc_val.begin() + (size_t)k * bb, bsc_val.begin() + (size_t)(k + 1) * bb,
-                      fwd.val.begin() + (size_t)d * bb);
+    parallel_for(nth, [&](int t) {
+        const Part &P = parts[(size_t)t];
+        std::vector<int64_t> &pos = rc[(size_t)t];
+        int64_t q = 0;
+        for (int64_t cc = c_lo(t); cc < c_lo(t + 1); ++cc) {
+            for (int64_t k = bsc_ptr[cc]; k < bsc_ptr[cc + 1]; ++k, ++q) {
+                const int64_t d = pos[(size_t)P.row[(size_t)q]]++;
+                fwd.col[d] = (int32_t)cc;
+                std::copy(P.val.begin() + (size_t)q * bb, P.val.begin() + (size_t)(q + 1) * bb,
+                          fwd.val.begin() + (size_t)d * bb);
+            }
         }
-    }
+    });
     extra_from_coo(n, nflow, xr, xc, xv, is_complex, false, xf);
     extra_from_coo(n, nflow, xr, xc, xv, is_complex, true, xa);
 }
@@ -266,11 +356,22 @@ void extra_from_coo(int64_t n, int64_t nflow, const std::vector<int64_t> &r,
     const int64_t m = (int64_t)r.size();
     const std::vector<int64_t> &rr = adjoint ? c : r;
     const std::vector<int64_t> &cc = adjoint ? r : c;
-    std::vector<int64_t> idx(m);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b2) {
-        return rr[a] != rr[b2] ? rr[a] < rr[b2] : cc[a] < cc[b2];
-    });
+    // entries in (row, column) order, ties in input order: a stable counting
+    // sort by row, then a stable sort by column inside each row
+    std::vector<int64_t> idx(m), start((size_t)n + 1, 0);
+    for (int64_t k = 0; k < m; ++k) start[(size_t)rr[k] + 1]++;
+    for (int64_t i = 0; i < n; ++i) start[(size_t)i + 1] += start[(size_t)i];
+    {
+        std::vector<int64_t> fill(start.begin(), start.end() - 1);
+        for (int64_t k = 0; k < m; ++k) idx[(size_t)fill[(size_t)rr[k]]++] = k;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        const auto b0 = idx.begin() + start[(size_t)i], b1 = idx.begin() + start[(size_t)i + 1];
+        if (b1 - b0 > 1 && !std::is_sorted(b0, b1, [&](int64_t a, int64_t b2) {
+                return cc[a] < cc[b2];
+            }))
+            std::stable_sort(b0, b1, [&](int64_t a, int64_t b2) { return cc[a] < cc[b2]; });
+    }
     out.is_complex = is_complex;
     out.rows.clear();
     out.ptr.assign(1, 0);

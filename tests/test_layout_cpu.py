@@ -54,6 +54,31 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("transpose", [False, True])
+@pytest.mark.parametrize("kind", ["shock_b4", "shock_b8", "complex"])
+def test_conversion_split_over_host_threads(emu, kind, transpose):
+    """Grids large enough that the upload's conversion splits its cell
+    columns over host threads (one per 4 096: 19 200 and 9 600 cells): the
+    blocks, their transposes, the regrouping by block row and the extra
+    rows (shock DOFs; a complex receptivity extension) give scipy's product."""
+    if kind == "shock_b4":
+        op = ops.stencil_operator(160, 120, 4, seed=8, n_shock=160)
+    elif kind == "shock_b8":
+        op = ops.stencil_operator(120, 80, 8, seed=9, n_shock=120)
+    else:
+        op = ops.receptivity_extension(ops.stencil_operator(100, 90, 4, seed=10, n_shock=100),
+                                       [0.0, 1.5])
+    A = op.to_scipy_varmajor()
+    r = np.random.default_rng(3)
+    x = r.standard_normal(op.n)
+    if np.iscomplexobj(A.data):
+        x = x + 1j * r.standard_normal(op.n)
+    y, oob = _run(emu, op, transpose, x)
+    assert oob == 0
+    ref = (A.conj().T if transpose else A) @ x     # the adjoint: conjugate transpose
+    assert np.allclose(y, ref, rtol=1e-13, atol=1e-12)
+
+
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['nchi']}x{c['neta']}b{c['b']}")
 @pytest.mark.parametrize("transpose", [False, True])
 def test_emulated_spmv_matches_scipy(emu, case, transpose):
