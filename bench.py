@@ -267,6 +267,7 @@ def transient_growth_leg(T=1.0, k=4, tol=1e-8, steps=5):
     As.sort_indices()
     colptr, rowval = As.indptr.astype(np.int64), As.indices.astype(np.int64)
     ckpt = 8
+    from hymor_amd.core import DevicePattern
     with DeviceOperator.from_scipy(As, nchi=nchi, neta=neta) as dA, \
             DeviceOperator.from_scipy(C, nchi=nchi, neta=neta) as dC:
         F = GainFunctional(dA, dC, D, n_t)
@@ -274,13 +275,28 @@ def transient_growth_leg(T=1.0, k=4, tol=1e-8, steps=5):
         t1 = time.perf_counter()
         G, g = F.operator_gradient(x, colptr, rowval, index_base=0, ckpt_every=ckpt)
         tg = time.perf_counter() - t1
+        # the same on a pattern kept on the device, gradient left there
+        with DevicePattern(dA, colptr, rowval, index_base=0) as pat:
+            dg = DeviceVector(pat.nnz)
+            try:
+                F.operator_gradient_dev(x, pat, dg, ckpt_every=ckpt)
+                t1 = time.perf_counter()
+                F.operator_gradient_dev(x, pat, dg, ckpt_every=ckpt)
+                tgd = time.perf_counter() - t1
+                same = bool(np.array_equal(dg.download(), g))
+            finally:
+                dg.free()
     out["operator_gradient"] = {
         "ms": tg * 1e3, "entries": int(As.nnz), "ckpt_every": ckpt,
         "device_vectors": int(-(-n_t // ckpt) + (2 * 5 + 1) * ckpt + 5),
         "note": ("dG/dA on the operator's CSC pattern: forward action, Chu weight, checkpointed "
-                 "pullback (hymor_expm_taylor_vjp_values); host vectors and gradient")}
-    log(f"transient growth: operator gradient on {As.nnz} entries {tg * 1e3:.1f} ms "
-        f"(checkpoint every {ckpt} substeps)")
+                 "pullback (hymor_expm_taylor_vjp_values); host vectors and gradient"),
+        "device_pattern_ms": tgd * 1e3, "device_pattern_same_bits": same,
+        "device_pattern_note": ("the pattern mapped and kept on the device once "
+                                "(hymor_pattern_create), gradient left on the device "
+                                "(hymor_expm_taylor_vjp_values_dev)")}
+    log(f"transient growth: operator gradient on {As.nnz} entries {tg * 1e3:.1f} ms, "
+        f"{tgd * 1e3:.1f} ms on a device pattern (checkpoint every {ckpt} substeps)")
     return out
 
 

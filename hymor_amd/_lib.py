@@ -54,6 +54,11 @@ SIGNATURES = {
                                                         _p, _i64]),
     "hymor_expm_taylor_vjp_values": (C.c_int, [_p, C.c_int, _i64, C.c_double, _i64, _p, _p, _p,
                                                _p, C.c_int, _p]),
+    "hymor_pattern_create": (C.c_int, [_p, _p, _p, C.c_int, _pp]),
+    "hymor_pattern_destroy": (C.c_int, [_p]),
+    "hymor_pattern_nnz": (C.c_int, [_p, _pi64]),
+    "hymor_expm_taylor_vjp_values_dev": (C.c_int, [_p, _p, C.c_int, _i64, C.c_double, _i64, _p,
+                                                   _p, _p]),
     "hymor_chu_sandwich": (C.c_int, [_p, _p, C.c_int, _i64, C.c_double, _i64, _p, _p, C.c_int]),
     "hymor_chu_sandwich_dev": (C.c_int, [_p, _p, C.c_int, _i64, C.c_double, _i64, _p, _p,
                                          C.c_int]),

@@ -83,6 +83,26 @@ class GainFunctional:
                                                 ckpt_every=ckpt_every, index_base=index_base)
         return G, grad
 
+    def operator_gradient_dev(self, v, pattern, d_grad, *, ckpt_every=8):
+        """operator_gradient on a DevicePattern kept between calls: dG/dA is
+        written to the device array d_grad (pattern.nnz doubles, in the
+        pattern's CSC order) and never crosses to the host; returns G(v)."""
+        from .core import DeviceVector
+        v = np.array(v, dtype=np.float64)
+        v[:self.n_set0] = 0.0
+        w = self.devA.expm_taylor(v, order=self.order, nsub=self.n_t, dt=1.0, form="horner")
+        Cw = self.devC.spmv(w)
+        den = float(v @ (self.D @ v))
+        G = float(w @ Cw) / den
+        dx, dw = DeviceVector.from_host(v), DeviceVector.from_host(2.0 * Cw / den)
+        try:
+            self.devA.expm_taylor_vjp_values_dev(pattern, dx, dw, d_grad, order=self.order,
+                                                 nsub=self.n_t, dt=1.0, ckpt_every=ckpt_every)
+        finally:
+            dx.free()
+            dw.free()
+        return G
+
 
 def gain_autograd(functional: GainFunctional):
     """torch.autograd.Function computing G(v) for a torch float64 / complex128

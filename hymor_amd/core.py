@@ -169,6 +169,16 @@ class DeviceOperator:
             ptr(colptr), ptr(rowval), int(index_base), ptr(grad)))
         return grad
 
+    def expm_taylor_vjp_values_dev(self, pattern: "DevicePattern", d_x, d_wbar, d_grad, *,
+                                   order=5, nsub=1, dt=1.0, ckpt_every=8):
+        """expm_taylor_vjp_values on a DevicePattern with device vectors
+        (raw pointers or DeviceVector) and a device gradient of
+        pattern.nnz doubles (hymor_expm_taylor_vjp_values_dev)."""
+        ptr_of = lambda d: d.ptr if hasattr(d, "ptr") else d
+        check(_lib.load().hymor_expm_taylor_vjp_values_dev(
+            self._h, pattern.handle, int(order), int(nsub), float(dt), int(ckpt_every),
+            ptr_of(d_x), ptr_of(d_wbar), ptr_of(d_grad)))
+
     def chu_sandwich(self, weight: "DeviceOperator", x, *, order=5, nsub=1, dt=1.0, n_set0=0):
         """T(A'dt)^n * C * T(A dt)^n * x with CONSTRUCT_C's zeroing."""
         x, dty = _vec(x, self.is_complex)
@@ -407,6 +417,38 @@ def energy_track(op: DeviceOperator, weights: DeviceWeights, x, *, n_t, order=5,
                                          float(dt), _FORMS[form], ptr(x), dty, ptr(E), ptr(xf),
                                          int(max(n_skip, 1)), int(n_samples), sp_ptr))
     return E, xf, (samp.T if samp is not None else None)
+
+
+class DevicePattern:
+    """A CSC pattern of an operator of `op`'s shape kept on the device for
+    repeated operator gradients (hymor_pattern_create)."""
+
+    def __init__(self, op: DeviceOperator, colptr, rowval, *, index_base=1):
+        colptr = np.ascontiguousarray(colptr, dtype=np.int64)
+        rowval = np.ascontiguousarray(rowval, dtype=np.int64)
+        if colptr.shape != (op.n + 1,):
+            raise ValueError("colptr needs n + 1 entries")
+        self._h = C.c_void_p()
+        check(_lib.load().hymor_pattern_create(op.handle, ptr(colptr), ptr(rowval),
+                                               int(index_base), C.byref(self._h)))
+        nnz = C.c_int64()
+        check(_lib.load().hymor_pattern_nnz(self._h, C.byref(nnz)))
+        self.nnz = nnz.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            check(_lib.load().hymor_pattern_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def spmv_dev(op: DeviceOperator, d_x, d_y, *, transpose=False, complex_=None):

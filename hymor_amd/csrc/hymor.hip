@@ -2036,130 +2036,226 @@ int hymor_expm_taylor(hymor_op *op, int transpose, int order, int64_t nsub, doub
     });
 }
 
+// The caller's CSC pattern (var-major rows and columns, as
+// hymor_op_create_csc takes A) in the operator's cell-major indices, one
+// (row, column) pair per entry in the pattern's order.
+static void pattern_cellmajor(const hymor_op *op, const int64_t *colptr, const int64_t *rowval,
+                              int index_base, std::vector<int32_t> &er,
+                              std::vector<int32_t> &ec) {
+    ARG_CHECK(colptr && rowval, "null pattern");
+    ARG_CHECK(index_base == 0 || index_base == 1, "index_base must be 0 or 1");
+    const int64_t n = op->n, base = index_base;
+    ARG_CHECK(colptr[0] == base, "colptr[0] must equal the index base");
+    for (int64_t j = 0; j < n; ++j)
+        ARG_CHECK(colptr[j + 1] >= colptr[j], "colptr is not monotone");
+    const int64_t nnz = colptr[n] - base;
+    // var-major index vm = var * ncell + cell -> cell * b + var, extras unchanged
+    std::vector<int32_t> cm((size_t)n);
+    for (int64_t vm = 0; vm < n; ++vm)
+        cm[(size_t)vm] =
+            (int32_t)(vm < op->nflow ? (vm % op->ncell) * op->b + vm / op->ncell : vm);
+    er.resize((size_t)nnz);
+    ec.resize((size_t)nnz);
+    for (int64_t j = 0; j < n; ++j) {
+        const int32_t cj = cm[(size_t)j];
+        for (int64_t e = colptr[j] - base; e < colptr[j + 1] - base; ++e) {
+            const int64_t r = rowval[e] - base;
+            ARG_CHECK(r >= 0 && r < n, "rowval out of range");
+            er[(size_t)e] = cm[(size_t)r];
+            ec[(size_t)e] = cj;
+        }
+    }
+}
+
+static void check_vjp_args(const hymor_op *op, int order, int64_t nsub, int64_t ckpt_every) {
+    ARG_CHECK(op != nullptr, "null operator");
+    ARG_CHECK(!op->is_complex, "the operator gradient is for real operators");
+    ARG_CHECK(order >= 1 && order <= 20, "order must be in [1, 20]");
+    ARG_CHECK(nsub >= 0, "nsub must be >= 0");
+    ARG_CHECK(ckpt_every >= 1, "ckpt_every must be >= 1");
+    ARG_CHECK(op->has_adj, "operator was built without its adjoint");
+}
+
+// The gradient of <wbar, T_p(dt A)^nsub x> on the pattern (d_er, d_ec: nnz
+// cell-major pairs) into d_grad; x and wbar are var-major device vectors.
+// Synchronous (its buffers are released on return).
+static void vjp_core(hymor_op *op, int order, int64_t nsub, double dt, int64_t ckpt_every,
+                     const double *d_x, const double *d_wbar, const int32_t *d_er,
+                     const int32_t *d_ec, int64_t nnz, double *d_grad) {
+    hipStream_t st = op->stream;
+    if (nnz == 0) return;
+    if (nsub == 0) {   // w = x does not depend on A
+        HIP_TRY(hipMemsetAsync(d_grad, 0, (size_t)nnz * 8, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return;
+    }
+    ensure_ws(op, 1);
+    const int64_t n = op->n;
+    const int p = order;
+    const int64_t K = std::min(ckpt_every, nsub);    // substeps per checkpoint
+    const int64_t ng = (nsub + K - 1) / K;
+    const size_t vb = (size_t)n * sizeof(double);
+    DevTemp d_ck((size_t)ng * vb), d_grp((size_t)std::max<int64_t>(K - 1, 1) * vb);
+    DevTemp d_U((size_t)p * K * vb), d_Z((size_t)p * K * vb);
+    DevTemp d_Y((size_t)std::max(p - 1, 1) * vb), d_lam(vb), d_up(vb);
+    double *ck = (double *)d_ck.p, *grp = (double *)d_grp.p, *U = (double *)d_U.p,
+           *Z = (double *)d_Z.p, *Y = (double *)d_Y.p, *lam = (double *)d_lam.p,
+           *up = (double *)d_up.p;
+    double invf[kMaxVjp];
+    invf[0] = 1.0;
+    for (int j = 1; j <= p; ++j) invf[j] = invf[j - 1] / (double)j;
+    const unsigned gv = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256,
+                                                                          65536));
+    // forward: the state at the start of every K-substep group
+    launch_perm_in(op, st, d_x, n, ck, 1, 1, 0, 1);
+    for (int64_t g = 0; g + 1 < ng; ++g) {
+        const double *res = run_taylor(op, false, p, K, dt, HYMOR_FORM_HORNER, 1, ck + g * n);
+        HIP_TRY(hipMemcpyAsync(ck + (g + 1) * n, res, vb, hipMemcpyDeviceToDevice, st));
+    }
+    // backward, group by group from the last: the group's states are
+    // recomputed from its checkpoint, then lambda steps back through them
+    launch_perm_in(op, st, d_wbar, n, lam, 1, 1, 0, 1);
+    int init = 1;
+    for (int64_t g = ng - 1; g >= 0; --g) {
+        const int64_t cnt = std::min(K, nsub - g * K);
+        std::vector<const double *> xs((size_t)cnt);
+        xs[0] = ck + g * n;
+        for (int64_t i = 1; i < cnt; ++i) {
+            const double *res =
+                run_taylor(op, false, p, 1, dt, HYMOR_FORM_HORNER, 1, xs[(size_t)i - 1]);
+            double *dst = grp + (i - 1) * n;
+            HIP_TRY(hipMemcpyAsync(dst, res, vb, hipMemcpyDeviceToDevice, st));
+            xs[(size_t)i] = dst;
+        }
+        for (int64_t i = cnt - 1; i >= 0; --i) {
+            double *Uk = U + (size_t)i * p * n, *Zk = Z + (size_t)i * p * n;
+            // y_r = (dt A)^r x_k, then the z_m
+            VecList ys{};
+            ys.v[0] = xs[(size_t)i];
+            for (int r = 1; r < p; ++r) {
+                double *yr = Y + (size_t)(r - 1) * n;
+                launch_term(op, false, 1, st, ys.v[r - 1], yr, nullptr, nullptr, dt, 0.0);
+                ys.v[r] = yr;
+            }
+            for (int j = 0; j <= p; ++j) ys.c[j] = invf[j];
+            ys.cnt = p;
+            hipLaunchKernelGGL(k_vjp_z, dim3(gv), dim3(256), 0, st, ys, p, n, Zk);
+            HIP_TRY(hipGetLastError());
+            // u_m = (dt A')^m lambda, m < p; u_p into up
+            HIP_TRY(hipMemcpyAsync(Uk, lam, vb, hipMemcpyDeviceToDevice, st));
+            for (int m = 1; m <= p; ++m) {
+                double *um = m < p ? Uk + (size_t)m * n : up;
+                launch_term(op, true, 1, st, Uk + (size_t)(m - 1) * n, um, nullptr, nullptr,
+                            dt, 0.0);
+            }
+            // lambda_k = S' lambda_{k+1} = sum_{m <= p} u_m / m!
+            VecList us{};
+            for (int m = 0; m <= p; ++m) {
+                us.v[m] = m < p ? Uk + (size_t)m * n : up;
+                us.c[m] = invf[m];
+            }
+            us.cnt = p + 1;
+            hipLaunchKernelGGL(k_lincomb, dim3(gv), dim3(256), 0, st, us, n, lam);
+            HIP_TRY(hipGetLastError());
+        }
+        const unsigned ge = (unsigned)((nnz + 255) / 256);
+        hipLaunchKernelGGL(k_sddmm_pattern, dim3(ge), dim3(256), 0, st,
+                           d_er, d_ec, nnz,
+                           (const double *)U, (const double *)Z, n, (int)(cnt * p), dt,
+                           init, d_grad);
+        HIP_TRY(hipGetLastError());
+        init = 0;
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+}
+
 int hymor_expm_taylor_vjp_values(hymor_op *op, int order, int64_t nsub, double dt,
                                  int64_t ckpt_every, const double *x, const double *wbar,
                                  const int64_t *colptr, const int64_t *rowval, int index_base,
                                  double *grad) {
     return guarded([&] {
-        ARG_CHECK(op != nullptr, "null operator");
-        ARG_CHECK(!op->is_complex, "the operator gradient is for real operators");
-        ARG_CHECK(order >= 1 && order <= 20, "order must be in [1, 20]");
-        ARG_CHECK(nsub >= 0, "nsub must be >= 0");
-        ARG_CHECK(ckpt_every >= 1, "ckpt_every must be >= 1");
-        ARG_CHECK(x && wbar && colptr && rowval && grad, "null argument");
-        ARG_CHECK(index_base == 0 || index_base == 1, "index_base must be 0 or 1");
-        ARG_CHECK(op->has_adj, "operator was built without its adjoint");
-        const int64_t n = op->n, base = index_base;
-        ARG_CHECK(colptr[0] == base, "colptr[0] must equal the index base");
-        for (int64_t j = 0; j < n; ++j)
-            ARG_CHECK(colptr[j + 1] >= colptr[j], "colptr is not monotone");
-        const int64_t nnz = colptr[n] - base;
-        // the pattern's entries in cell-major rows and columns (var-major
-        // index vm = var * ncell + cell -> cell * b + var, extras unchanged)
-        std::vector<int32_t> cm((size_t)n);
-        for (int64_t vm = 0; vm < n; ++vm)
-            cm[(size_t)vm] = (int32_t)(vm < op->nflow ? (vm % op->ncell) * op->b + vm / op->ncell
-                                                      : vm);
-        std::vector<int32_t> er((size_t)nnz), ec((size_t)nnz);
-        for (int64_t j = 0; j < n; ++j) {
-            const int32_t cj = cm[(size_t)j];
-            for (int64_t e = colptr[j] - base; e < colptr[j + 1] - base; ++e) {
-                const int64_t r = rowval[e] - base;
-                ARG_CHECK(r >= 0 && r < n, "rowval out of range");
-                er[(size_t)e] = cm[(size_t)r];
-                ec[(size_t)e] = cj;
-            }
-        }
+        check_vjp_args(op, order, nsub, ckpt_every);
+        ARG_CHECK(x && wbar && grad, "null argument");
+        std::vector<int32_t> er, ec;
+        pattern_cellmajor(op, colptr, rowval, index_base, er, ec);
+        const int64_t nnz = (int64_t)er.size();
         if (nnz == 0) return;
-        if (nsub == 0) {   // w = x does not depend on A
-            std::fill(grad, grad + nnz, 0.0);
-            return;
-        }
-        ensure_ws(op, 1);
-        hipStream_t st = op->stream;
-        const int p = order;
-        const int64_t K = std::min(ckpt_every, nsub);    // substeps per checkpoint
-        const int64_t ng = (nsub + K - 1) / K;
-        const size_t vb = (size_t)n * sizeof(double);
+        const size_t vb = (size_t)op->n * sizeof(double);
         DevTemp d_er((size_t)nnz * 4), d_ec((size_t)nnz * 4), d_grad((size_t)nnz * 8);
-        DevTemp d_ck((size_t)ng * vb), d_grp((size_t)std::max<int64_t>(K - 1, 1) * vb);
-        DevTemp d_U((size_t)p * K * vb), d_Z((size_t)p * K * vb);
-        DevTemp d_Y((size_t)std::max(p - 1, 1) * vb), d_lam(vb), d_up(vb);
-        double *ck = (double *)d_ck.p, *grp = (double *)d_grp.p, *U = (double *)d_U.p,
-               *Z = (double *)d_Z.p, *Y = (double *)d_Y.p, *lam = (double *)d_lam.p,
-               *up = (double *)d_up.p;
+        DevTemp d_x(vb), d_w(vb);
+        hipStream_t st = op->stream;
         HIP_TRY(hipMemcpyAsync(d_er.p, er.data(), (size_t)nnz * 4, hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(d_ec.p, ec.data(), (size_t)nnz * 4, hipMemcpyHostToDevice, st));
-        double invf[kMaxVjp];
-        invf[0] = 1.0;
-        for (int j = 1; j <= p; ++j) invf[j] = invf[j - 1] / (double)j;
-        const unsigned gv = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256,
-                                                                              65536));
-        // forward: the state at the start of every K-substep group
-        HIP_TRY(hipMemcpyAsync(op->ws[0], x, vb, hipMemcpyHostToDevice, st));
-        launch_perm_in(op, st, op->ws[0], n, ck, 1, 1, 0, 1);
-        for (int64_t g = 0; g + 1 < ng; ++g) {
-            const double *res = run_taylor(op, false, p, K, dt, HYMOR_FORM_HORNER, 1,
-                                           ck + g * n);
-            HIP_TRY(hipMemcpyAsync(ck + (g + 1) * n, res, vb, hipMemcpyDeviceToDevice, st));
-        }
-        // backward, group by group from the last: the group's states are
-        // recomputed from its checkpoint, then lambda steps back through them
-        HIP_TRY(hipMemcpyAsync(op->ws[0], wbar, vb, hipMemcpyHostToDevice, st));
-        launch_perm_in(op, st, op->ws[0], n, lam, 1, 1, 0, 1);
-        int init = 1;
-        for (int64_t g = ng - 1; g >= 0; --g) {
-            const int64_t cnt = std::min(K, nsub - g * K);
-            std::vector<const double *> xs((size_t)cnt);
-            xs[0] = ck + g * n;
-            for (int64_t i = 1; i < cnt; ++i) {
-                const double *res =
-                    run_taylor(op, false, p, 1, dt, HYMOR_FORM_HORNER, 1, xs[(size_t)i - 1]);
-                double *dst = grp + (i - 1) * n;
-                HIP_TRY(hipMemcpyAsync(dst, res, vb, hipMemcpyDeviceToDevice, st));
-                xs[(size_t)i] = dst;
-            }
-            for (int64_t i = cnt - 1; i >= 0; --i) {
-                double *Uk = U + (size_t)i * p * n, *Zk = Z + (size_t)i * p * n;
-                // y_r = (dt A)^r x_k, then the z_m
-                VecList ys{};
-                ys.v[0] = xs[(size_t)i];
-                for (int r = 1; r < p; ++r) {
-                    double *yr = Y + (size_t)(r - 1) * n;
-                    launch_term(op, false, 1, st, ys.v[r - 1], yr, nullptr, nullptr, dt, 0.0);
-                    ys.v[r] = yr;
-                }
-                for (int j = 0; j <= p; ++j) ys.c[j] = invf[j];
-                ys.cnt = p;
-                hipLaunchKernelGGL(k_vjp_z, dim3(gv), dim3(256), 0, st, ys, p, n, Zk);
-                HIP_TRY(hipGetLastError());
-                // u_m = (dt A')^m lambda, m < p; u_p into up
-                HIP_TRY(hipMemcpyAsync(Uk, lam, vb, hipMemcpyDeviceToDevice, st));
-                for (int m = 1; m <= p; ++m) {
-                    double *um = m < p ? Uk + (size_t)m * n : up;
-                    launch_term(op, true, 1, st, Uk + (size_t)(m - 1) * n, um, nullptr, nullptr,
-                                dt, 0.0);
-                }
-                // lambda_k = S' lambda_{k+1} = sum_{m <= p} u_m / m!
-                VecList us{};
-                for (int m = 0; m <= p; ++m) {
-                    us.v[m] = m < p ? Uk + (size_t)m * n : up;
-                    us.c[m] = invf[m];
-                }
-                us.cnt = p + 1;
-                hipLaunchKernelGGL(k_lincomb, dim3(gv), dim3(256), 0, st, us, n, lam);
-                HIP_TRY(hipGetLastError());
-            }
-            const unsigned ge = (unsigned)((nnz + 255) / 256);
-            hipLaunchKernelGGL(k_sddmm_pattern, dim3(ge), dim3(256), 0, st,
-                               (const int32_t *)d_er.p, (const int32_t *)d_ec.p, nnz,
-                               (const double *)U, (const double *)Z, n, (int)(cnt * p), dt,
-                               init, (double *)d_grad.p);
-            HIP_TRY(hipGetLastError());
-            init = 0;
-        }
+        HIP_TRY(hipMemcpyAsync(d_x.p, x, vb, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d_w.p, wbar, vb, hipMemcpyHostToDevice, st));
+        vjp_core(op, order, nsub, dt, ckpt_every, (const double *)d_x.p, (const double *)d_w.p,
+                 (const int32_t *)d_er.p, (const int32_t *)d_ec.p, nnz, (double *)d_grad.p);
         HIP_TRY(hipMemcpyAsync(grad, d_grad.p, (size_t)nnz * 8, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
+    });
+}
+
+struct hymor_pattern {
+    int64_t n = 0, b = 0, ncell = 0, nnz = 0;
+    int32_t *er = nullptr, *ec = nullptr;
+};
+
+int hymor_pattern_create(const hymor_op *op, const int64_t *colptr, const int64_t *rowval,
+                         int index_base, hymor_pattern **out) {
+    hymor_pattern *pat = nullptr;
+    int rc = guarded([&] {
+        ARG_CHECK(op && out, "null argument");
+        *out = nullptr;
+        std::vector<int32_t> er, ec;
+        pattern_cellmajor(op, colptr, rowval, index_base, er, ec);
+        pat = new hymor_pattern();
+        pat->n = op->n;
+        pat->b = op->b;
+        pat->ncell = op->ncell;
+        pat->nnz = (int64_t)er.size();
+        if (pat->nnz) {
+            HIP_TRY(hipMalloc((void **)&pat->er, (size_t)pat->nnz * 4));
+            HIP_TRY(hipMalloc((void **)&pat->ec, (size_t)pat->nnz * 4));
+            HIP_TRY(hipMemcpy(pat->er, er.data(), (size_t)pat->nnz * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(pat->ec, ec.data(), (size_t)pat->nnz * 4, hipMemcpyHostToDevice));
+        }
+        *out = pat;
+    });
+    if (rc != HYMOR_OK && pat) {
+        (void)hipFree(pat->er);
+        (void)hipFree(pat->ec);
+        delete pat;
+    }
+    return rc;
+}
+
+int hymor_pattern_destroy(hymor_pattern *pat) {
+    return guarded([&] {
+        if (!pat) return;
+        if (pat->er) HIP_TRY(hipFree(pat->er));
+        if (pat->ec) HIP_TRY(hipFree(pat->ec));
+        delete pat;
+    });
+}
+
+int hymor_pattern_nnz(const hymor_pattern *pat, int64_t *nnz) {
+    return guarded([&] {
+        ARG_CHECK(pat && nnz, "null argument");
+        *nnz = pat->nnz;
+    });
+}
+
+int hymor_expm_taylor_vjp_values_dev(hymor_op *op, const hymor_pattern *pat, int order,
+                                     int64_t nsub, double dt, int64_t ckpt_every,
+                                     const double *d_x, const double *d_wbar, double *d_grad) {
+    return guarded([&] {
+        check_vjp_args(op, order, nsub, ckpt_every);
+        ARG_CHECK(pat != nullptr, "null pattern");
+        ARG_CHECK(pat->n == op->n && pat->b == op->b && pat->ncell == op->ncell,
+                  "the pattern was made for an operator of another shape");
+        ARG_CHECK(d_x && d_wbar && (d_grad || pat->nnz == 0), "null argument");
+        vjp_core(op, order, nsub, dt, ckpt_every, d_x, d_wbar, pat->er, pat->ec, pat->nnz,
+                 d_grad);
     });
 }
 

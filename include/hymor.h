@@ -58,6 +58,7 @@ extern "C" {
                                    reference counterpart) */
 
 typedef struct hymor_op hymor_op;
+typedef struct hymor_pattern hymor_pattern;
 
 /* 1 if a GPU the library can use is present, else 0.  Replaces the
  * reference's `use_gpu = CUDA.functional()` (EIGENVALUES.jl:119). */
@@ -218,6 +219,24 @@ int hymor_expm_taylor_vjp_values(hymor_op *op, int order, int64_t nsub, double d
                                  int64_t ckpt_every, const double *x, const double *wbar,
                                  const int64_t *colptr, const int64_t *rowval, int index_base,
                                  double *grad);
+
+/* A gradient pattern kept on the device: the CSC pattern (colptr, rowval,
+ * index_base as above) of an operator of op's shape, mapped to the device
+ * layout once (8 bytes per entry), for callers that differentiate on the
+ * same pattern repeatedly (an optimisation loop over the Jacobian).  No
+ * reference counterpart. */
+int hymor_pattern_create(const hymor_op *op, const int64_t *colptr, const int64_t *rowval,
+                         int index_base, hymor_pattern **out);
+int hymor_pattern_destroy(hymor_pattern *pat);
+int hymor_pattern_nnz(const hymor_pattern *pat, int64_t *nnz);
+
+/* hymor_expm_taylor_vjp_values on a device-resident pattern with device
+ * vectors: d_x, d_wbar (var-major, n doubles), d_grad (the pattern's nnz
+ * doubles, in its CSC order) -- no pattern, vector or gradient crosses to
+ * the host.  Synchronous. */
+int hymor_expm_taylor_vjp_values_dev(hymor_op *op, const hymor_pattern *pat, int order,
+                                     int64_t nsub, double dt, int64_t ckpt_every,
+                                     const double *d_x, const double *d_wbar, double *d_grad);
 
 /* Chu-norm sandwich y = T_p(A'dt)^n * C * T_p(A dt)^n * x with the zeroing
  * of CONSTRUCT_C (CONSTRUCT_C.jl:33-58 around EXPONENTIAL! :78-111):

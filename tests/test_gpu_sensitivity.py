@@ -153,3 +153,68 @@ def test_gain_operator_gradient_fd(orc):
         Dm = sp.csc_matrix((d, (rows, cols)), shape=As.shape)
         fd = (gain(As + h * Dm) - gain(As - h * Dm)) / (2 * h)
         assert abs(fd - g @ d) <= 1e-6 * gn, (i, fd, g @ d)
+
+
+def test_vjp_values_on_a_device_pattern(orc):
+    """hymor_pattern_create + hymor_expm_taylor_vjp_values_dev (pattern,
+    vectors and gradient on the device) give the host entry point's bits, on
+    repeated calls with the same pattern and different vectors (the gradient
+    array overwritten each time); GainFunctional.operator_gradient_dev gives
+    operator_gradient's G and dG/dA; a pattern of another shape is refused."""
+    from hymor_amd.core import DevicePattern, DeviceVector
+    op = ops.stencil_operator(10, 7, 4, 67, n_shock=10, row_sum=2.0)
+    A = op.to_scipy_varmajor()
+    colptr, rowval = _pattern(A, 9, 10)
+    say("[operator gradient] device-resident pattern")
+    with DeviceOperator.from_blocks(op) as dev, DevicePattern(dev, colptr, rowval) as pat:
+        assert pat.nnz == rowval.shape[0]
+        dg = DeviceVector(pat.nnz)
+        try:
+            for seed in (1, 2):
+                r = np.random.default_rng(seed)
+                x, wbar = r.standard_normal(op.n), r.standard_normal(op.n)
+                dx, dw = DeviceVector.from_host(x), DeviceVector.from_host(wbar)
+                try:
+                    dev.expm_taylor_vjp_values_dev(pat, dx, dw, dg, order=5, nsub=7, dt=0.05,
+                                                   ckpt_every=3)
+                finally:
+                    dx.free()
+                    dw.free()
+                want = dev.expm_taylor_vjp_values(x, wbar, colptr, rowval, order=5, nsub=7,
+                                                  dt=0.05, ckpt_every=3)
+                np.testing.assert_array_equal(dg.download(), want)
+        finally:
+            dg.free()
+    other = ops.stencil_operator(9, 7, 4, 68, n_shock=9)
+    with DeviceOperator.from_blocks(op) as dev, DeviceOperator.from_blocks(other) as dev2, \
+            DevicePattern(dev, colptr, rowval) as pat:
+        v = DeviceVector(dev2.n)
+        try:
+            with pytest.raises(_lib.HymorError, match="another shape"):
+                dev2.expm_taylor_vjp_values_dev(pat, v, v, v, order=5, nsub=2)
+        finally:
+            v.free()
+    # the gain's dG/dA through the device pattern
+    nchi, neta = 9, 7
+    s = st.CFL_TIMESTEP(synthetic_solution(nchi, neta, seed=5, perturb_shock=True))
+    dt = s["time_integration"]["dt"]
+    base = ops.stencil_operator(nchi, neta, 4, seed=21, n_shock=nchi, row_sum=0.5 / dt)
+    R = st.CONSTRUCT_R(s)
+    C = (R.T @ st.CONSTRUCT_M(s, [1, 1, 1, 1]) @ R).tocsc()
+    D = (R.T @ st.CONSTRUCT_M_HAT(s, [1, 1, 1, 1]) @ R).tocsc()
+    As = sp.csc_matrix(base.to_scipy_varmajor() * dt)
+    As.sort_indices()
+    cp, rv = As.indptr.astype(np.int64) + 1, As.indices.astype(np.int64) + 1
+    v = np.random.default_rng(66).standard_normal(As.shape[0])
+    with DeviceOperator.from_scipy(As, nchi=nchi, neta=neta) as dA, \
+            DeviceOperator.from_scipy(C, nchi=nchi, neta=neta) as dC, \
+            DevicePattern(dA, cp, rv) as pat:
+        F = GainFunctional(dA, dC, D, 6)
+        G, g = F.operator_gradient(v, cp, rv, ckpt_every=4)
+        dg = DeviceVector(pat.nnz)
+        try:
+            G2 = F.operator_gradient_dev(v, pat, dg, ckpt_every=4)
+            assert G2 == G
+            np.testing.assert_array_equal(dg.download(), g)
+        finally:
+            dg.free()
