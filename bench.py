@@ -830,7 +830,7 @@ def main():
                     help="skip the 200x300-cell shock-band leg (action, 20-mode eigensolve)")
     ap.add_argument("--kernel-only",
                     choices=["plain", "batched", "block4", "receptivity", "receptivity16",
-                             "krylov"],
+                             "krylov", "shock_batch"],
                     help="time only the Taylor-term kernel of one form (for profiler runs): "
                          "plain FP64 one vector, batched FP32 --nrhs vectors, block4 FP32 "
                          "four vectors (the eigen leg's FP32 phase), receptivity: the C4 "
@@ -838,7 +838,9 @@ def main():
                          "vector (block kernel + extra rows); receptivity16: the same "
                          "operator on 16 complex vectors (its 32 copies of B kept once); "
                          "krylov: the block Gram-Schmidt projection kernels at N=1.44M, "
-                         "32 basis vectors, blocks of 4 and 16")
+                         "32 basis vectors, blocks of 4 and 16; shock_batch: the 400x300-cell "
+                         "operator with 400 shock DOFs, FP64 values, --nrhs vectors (the "
+                         "shock rows in the batch kernel's launch)")
     ap.add_argument("--batch-kernel", type=int, default=None,
                     help="--kernel-only batched: use this batch kernel (0..8) instead of "
                          "the default (6: the cell-pair kernel at b=8 and 16+ vectors)")
@@ -877,7 +879,10 @@ def main():
     if args.kernel_only:
         prec, nv = {"plain": ("f64", 1), "batched": ("f32", args.nrhs),
                     "block4": ("f32", 4), "receptivity": ("f64", 2),
-                    "receptivity16": ("f64", 32)}[args.kernel_only]
+                    "receptivity16": ("f64", 32),
+                    "shock_batch": ("f64", args.nrhs)}[args.kernel_only]
+        if args.kernel_only == "shock_batch":
+            op = ops.config_operator("C3")
         if args.kernel_only.startswith("receptivity"):
             # the receptivity leg's extended operator, default launch tuning
             op = ops.receptivity_extension(ops.config_operator("C4"),
