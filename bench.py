@@ -397,10 +397,11 @@ def _eigen_timings(nchi, neta, k, ncv, *, seed, n_lead, bulk, n_shock, tag):
     """EIGENVALUES_GPU_ARNOLDI (T_ref = 0.5, dt = 1/64, order 5) for k modes
     with Krylov dimension ncv through the device Krylov-Schur (tolerance
     1e-9) and through ARPACK at the reference's tolerance 1e-4
-    (EIGENVALUES.jl:159), both applying the GPU action, on a b = 4 operator
-    of the given shape with an exactly known spectrum (one-way coupling:
-    its diagonal blocks' eigenvalues); seconds and the worst relative
-    eigenvalue error of each."""
+    (EIGENVALUES.jl:159), both applying the GPU action, and batched_exp_eigs
+    (block Krylov-Schur on 4-vector sweeps, FP64 values, tolerance 1e-9), on
+    a b = 4 operator of the given shape with an exactly known spectrum
+    (one-way coupling: its diagonal blocks' eigenvalues); seconds and the
+    worst relative eigenvalue error of each."""
     import math
     from hymor_amd import operators as ops
     from hymor_amd import stability as st
@@ -426,6 +427,18 @@ def _eigen_timings(nchi, neta, k, ncv, *, seed, n_lead, bulk, n_shock, tag):
             res["applications"] = sd["timings"]["eigenvalues_krylov"].get("applications")
         out[name] = res
         log(f"{tag}: EIGENVALUES ({name}) {res}")
+    # the same modes by block Krylov-Schur on batched sweeps (blocks of 4,
+    # FP64 values: every expansion one 4-column sweep)
+    sd = {"mesh": {"Nchi": nchi, "Neta": neta}, "time_integration": {"dt": dt},
+          "stability_analysis": {"perturb_shock": False,
+                                 "eigenvalue_solver": "GPU_TIMESTEPPER_ARNOLDI"}}
+    _, D, info = st.batched_exp_eigs(eop, sd, k, block=4, value_prec="f64", tol=1e-9,
+                                     maxiter=300)
+    err = max(float(np.min(np.abs(exact - d)) / abs(d)) for d in D)
+    res = {"seconds": info["seconds"], "tol": 1e-9, "max_rel_eig_err": err,
+           "applications": info["phase1"].get("applications"), "block": 4}
+    out["batched_block4_fp64"] = res
+    log(f"{tag}: batched_exp_eigs (blocks of 4, FP64) {res}")
     return out
 
 

@@ -208,6 +208,22 @@ def test_c2_eigenvalues_k20_m80():
     _check_eigs(op, lam, D, V, k, dt, 1e-8, 1e-9)
 
 
+def test_c2_batched_eigenvalues_k20():
+    """The same 20 modes by block Krylov-Schur on 4-vector sweeps with FP64
+    values (batched_exp_eigs): every expansion one batched sweep over the
+    shock operator, its shock rows in the batch kernel's launch; the same
+    eigenvalue and Ritz-residual checks."""
+    nchi, neta, k, dt = 200, 300, 20, 1.0 / 64
+    say("[200x300 b4 + 200 shock DOFs] batched_exp_eigs k=20, blocks of 4, FP64 values")
+    op, lam = ops.marching_operator(nchi, neta, 4, seed=12, n_lead=24, bulk=(-6.0, -3.0),
+                                    n_shock=nchi)
+    s = _s_dict(nchi, neta, dt)
+    V, D, info = st.batched_exp_eigs(op, s, k, block=4, value_prec="f64", tol=1e-11,
+                                     maxiter=300)
+    say(f"[200x300 b4] {info['phase1']['applications']} actions, {info['seconds']:.2f} s")
+    _check_eigs(op, lam, D, V, k, dt, 1e-8, 1e-9)
+
+
 # ---------------------------------------------------------------- transient growth
 
 @pytest.fixture(scope="module")
