@@ -815,8 +815,11 @@ def batched_exp_eigs(A, s, n_modes, *, T_ref=0.5, block=4, ncv=None, tol=1e-10,
             dev.set_value_prec("f64")
             prepare(dev, refine_block)
             try:
-                vals, Y, i2 = block_krylov_schur(make_apply(dev), n, k,
-                                                 max(refine_ncv, k + refine_block), refine_block,
+                # room for at least two blocks past the wanted set between
+                # restarts (with one, every restart follows one expansion)
+                m2 = (refine_ncv if k + refine_block <= refine_ncv
+                      else k + 2 * refine_block)
+                vals, Y, i2 = block_krylov_schur(make_apply(dev), n, k, m2, refine_block,
                                                  tol=tol, maxiter=maxiter, V0=seed)
             finally:
                 seed.free()
