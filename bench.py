@@ -723,6 +723,28 @@ def receptivity_leg(steps=3, order=12, nsub=64, T=1.0, n_w=32, widths=(4, 8, 16,
         "rel_err_vs_assembled_fp64_single": float(np.linalg.norm(y - y_asm) /
                                                   np.linalg.norm(y_asm))}
     log(f"receptivity leg: per-wavenumber: {1.0 / tf:.2f} actions/s ({t1 / tf:.2f}x)")
+    # FREESTREAM_RECEPTIVITY on this operator: 4 gains at T = 1 (Chu weights
+    # of a synthetic base flow of the grid), the extended operator from its
+    # parts, device Krylov-Schur one vector and blocks of 4
+    from hymor_amd import stability as st
+    from hymor_amd.baseflow import synthetic_solution
+    s = st.CFL_TIMESTEP(synthetic_solution(500, 250, seed=11, perturb_shock=False, cfl=100.0,
+                                           max_dt=1.0 / 64))
+    solves = {}
+    for block in (1, 4):
+        t0 = time.perf_counter()
+        _, G, _ = st.FREESTREAM_RECEPTIVITY(A_, s, 4, 1.0, w, tol=1e-8, krylov="device", ncv=24,
+                                            maxiter=400, factored=True, block=block)
+        tk = s["timings"]["receptivity_krylov"]
+        solves[f"block{block}"] = {"seconds": time.perf_counter() - t0,
+                                   "reduce_seconds": tk["reduce_seconds"],
+                                   "krylov_seconds": tk["solve_seconds"],
+                                   "applications": tk.get("applications"),
+                                   "gains": [float(g) for g in G]}
+    out["solve"] = dict(solves, note=("FREESTREAM_RECEPTIVITY(factored=True, krylov='device'), "
+                                      "k=4, T=1, tol 1e-8"))
+    log(f"receptivity leg: FREESTREAM_RECEPTIVITY {solves['block1']['seconds']:.2f} s one "
+        f"vector, {solves['block4']['seconds']:.2f} s blocks of 4")
     return out
 
 

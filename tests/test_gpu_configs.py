@@ -16,7 +16,9 @@ oracle SpMVs.
   the same vector, forward and adjoint, and one assembled SpMV equals the
   oracle's; the Horner terms with the replicated rows fused into one launch
   (one vector and 16-vector batches) equal the powers form's separate
-  launches, and the fused adjoint satisfies the dot-product identity.
+  launches, and the fused adjoint satisfies the dot-product identity;
+  FREESTREAM_RECEPTIVITY at this size (device Krylov, one vector and
+  blocks of 4) finds the same gains with D-orthonormal modes.
 """
 import sys
 
@@ -110,6 +112,34 @@ def test_c4_factored_receptivity_full_size(orc, c4_extended):
                 per_wavenumber
         finally:
             fr.close()
+
+
+def test_c4_freestream_receptivity_solve(c4_extended):
+    """FREESTREAM_RECEPTIVITY at full size (564 000 complex unknowns; the
+    extended operator uploaded from its parts, B kept once): the device
+    Krylov-Schur on the Cholesky-reduced Hermitian problem, one vector and
+    blocks of 4 (batched complex sandwiches with the freestream zeroing),
+    finds the same 4 gains (1e-9); the modes are D-orthonormal and satisfy
+    the reduced problem to the solver's tolerance."""
+    from hymor_amd import stability as st
+    from hymor_amd.baseflow import synthetic_solution
+    base, w, L_, A_ = c4_extended
+    say("[C4] FREESTREAM_RECEPTIVITY k=4 T=1, blocks of 1 and 4")
+    s = st.CFL_TIMESTEP(synthetic_solution(500, 250, seed=11, perturb_shock=False, cfl=100.0,
+                                           max_dt=1.0 / 64))
+    tol = 1e-10
+    _, G1, _ = st.FREESTREAM_RECEPTIVITY(A_, s, 4, 1.0, w, tol=tol, krylov="device", ncv=24,
+                                         maxiter=400, factored=True, block=1)
+    V, G4, T_opt = st.FREESTREAM_RECEPTIVITY(A_, s, 4, 1.0, w, tol=tol, krylov="device",
+                                             ncv=24, maxiter=400, factored=True, block=4)
+    say(f"[C4] gains {G4}")
+    assert np.all(G4 > 0) and np.all(np.diff(G4) <= 0)
+    assert np.all(np.abs(G4 - G1) <= 1e-9 * np.abs(G1)), (G1, G4)
+    R_ = st.CONSTRUCT_R_(s, w)
+    D_ = (R_.T @ st.CONSTRUCT_M_INFTY_(s, [1, 1, 1, 1], 1.0, w, True) @ R_).tocsc()
+    Gm = V.conj().T @ (D_ @ V)
+    assert np.abs(np.diag(Gm) - 1.0).max() <= 1e-10
+    assert np.abs(Gm - np.diag(np.diag(Gm))).max() <= 1e-6
 
 
 def test_c4_fused_terms_full_size(c4_extended):
