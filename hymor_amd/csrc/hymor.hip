@@ -1066,12 +1066,12 @@ void launch_extra_nv(const DevExtra &xe, int64_t nflow, hipStream_t st, const do
     } else if (rep) {
         go(std::false_type{}, std::true_type{}, std::integral_constant<int, 0>{});
     } else {
-        // real long rows at 2..8 doubles per row: NV/2 lanes per entry,
+        // real long rows at 2..64 doubles per row: NV/2 lanes per entry,
         // one 16-byte load of x per lane and entry.  On the 400x300-cell
         // shock operator the Chu sandwich at 2, 4 and 8 vectors runs 3, 6
         // and 12 % faster than with NV lanes per entry; one lane per entry
         // is 11 % slower at 8 (16 bytes per cache-line access)
-        if constexpr (NV >= 2 && NV <= 8) {
+        if constexpr (NV >= 2) {
             if (!cd.d) {
                 go(std::false_type{}, std::false_type{}, std::integral_constant<int, NV / 2>{});
                 return;

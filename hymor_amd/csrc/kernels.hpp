@@ -643,7 +643,7 @@ __device__ __forceinline__ void extra_short_rows(
     }
 }
 
-// A long real row at NV = 2..8 doubles per row with LPR lanes per entry,
+// A long real row at NV = 2..64 doubles per row with LPR lanes per entry,
 // each lane loading a contiguous chunk of CH = NV / LPR doubles of the
 // entry's x row (16-byte loads from CH = 2 up).  The LPR lanes of an entry
 // cover its x row in one cache line, and G entries per lane are in flight
@@ -661,7 +661,7 @@ __device__ __forceinline__ void extra_long_row_lpe(
     const int32_t *__restrict__ col, const double *__restrict__ val, int64_t nflow,
     const double *__restrict__ x, double *__restrict__ tout, const double *vin, double *vout,
     double c1, double c2, int tile_b) {
-    static_assert(NV >= 2 && NV <= 8 && NV % LPR == 0, "LPR divides NV <= 8");
+    static_assert(NV >= 2 && NV <= 64 && NV % LPR == 0 && LPR <= 64, "LPR divides NV <= 64");
     constexpr int CH = NV / LPR;
     constexpr int G = CH == 1 ? 16 : (CH == 2 ? 8 : (CH == 4 ? 4 : 2));
     constexpr int EPW = 64 / LPR;
@@ -763,7 +763,7 @@ __device__ __forceinline__ void extra_long_row_lpe(
 // the receptivity B rows of adjacent cells -- which then stay in one L2);
 // the next nshort_wg take the short rows [nlong, nrows), RPW per wave; the
 // rest (kind 2 replicas) the shared rows, VPW per wave.
-// LPE > 0: real rows at 2..8 doubles per row, no per-column diagonal: the
+// LPE > 0: real rows at 2..64 doubles per row, no per-column diagonal: the
 // long rows with LPE lanes per entry (extra_long_row_lpe).
 template <int NV, bool CPLX, int NW, bool REP, int LPE = 0>
 __global__ __launch_bounds__(64 * NW) void k_extra(
@@ -778,7 +778,7 @@ __global__ __launch_bounds__(64 * NW) void k_extra(
     auto body = [&](auto tile) {
         constexpr bool TILE = decltype(tile)::value;
         if (bid < nlong) {
-            if constexpr (LPE > 0 && !CPLX && !REP && NV >= 2 && NV <= 8)
+            if constexpr (LPE > 0 && !CPLX && !REP && NV >= 2)
                 extra_long_row_lpe<NV, NW, TILE, (LPE > 0 ? LPE : 1)>(xcd_tile(bid, nlong), rows, ptr, col, val, nflow,
                                                  x, tout, vin, vout, c1, c2, tile_b);
             else
@@ -2847,15 +2847,9 @@ __global__ __launch_bounds__(256) void k_spmm_quad4c(
             const int64_t *ptr = shr ? SB.sh_ptr : SB.fr_ptr;
             const int32_t *lc = shr ? SB.sh_col : SB.fr_col;
             const double *lv = shr ? SB.sh_val : SB.fr_val;
-            if (shr || ri < SB.nfr) {
-                if constexpr (R <= 8)
-                    extra_long_row_lpe<R, 4, false, R / 2, true>(ri, rows, ptr, lc, lv, SB.nflow,
-                                                                 x, tout, vin, vout, c1, c2, 0);
-                else
-                    extra_long_row<R, false, 4, false, false, true>(
-                        ri, rows, ptr, lc, lv, SB.nflow, x, tout, vin, vout, c1, c2, nullptr, 0,
-                        0, 0, ReplicaDev{});
-            }
+            if (shr || ri < SB.nfr)
+                extra_long_row_lpe<R, 4, false, R / 2, true>(ri, rows, ptr, lc, lv, SB.nflow, x,
+                                                             tout, vin, vout, c1, c2, 0);
             return;
         }
         bid -= SB.nlead_wg;
