@@ -649,9 +649,11 @@ ShockBatchDev shock_batch_dev(const DevShock &sh, int64_t nflow, bool f32) {
     return H;
 }
 
-// Whether a batched term of nv doubles per row runs a shock operator's
-// extra-DOF rows in its batch kernel's launch: k_term_lds with one wave per
-// slice at 2 and 4, the default b = 4 batch kernel (k_spmm_quad4c) at 8..64.
+// Whether a batched term of nv doubles per row runs in one launch over a
+// shock operator (the extra-DOF rows and the shock-column slices' whole rows
+// in the batch kernel's leading workgroups, those slices skipped): k_term_lds
+// with one wave per slice at 2 and 4, the default b = 4 batch kernel
+// (k_spmm_quad4c) at 8..64, whose row-major buffers the leading rows write.
 bool shock_batched_ok(const hymor_op *op, const DevSliced &s, int nv) {
     if (nv == 2 || nv == 4) return s.tune.split == 3 && s.tune.waves == 1;
     if (nv >= 8 && nv <= 64)
