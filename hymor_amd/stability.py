@@ -476,7 +476,7 @@ def _upload_extended_from_parts(A_, s, w_infty, dt):
 
 
 def FREESTREAM_RECEPTIVITY(A_, s, n_modes, T, w_infty, *, tol=1e-4, maxiter=None, ncv=None,
-                           v0=None, krylov="arpack", factored=False, block=2):
+                           v0=None, krylov="arpack", factored=False, block=4):
     """Freestream receptivity (FREESTREAM_RECEPTIVITY.jl:37-162).
 
     A_ is the complex extended operator [L B..B; 0 kron(diag(-i w), I)].
@@ -486,7 +486,9 @@ def FREESTREAM_RECEPTIVITY(A_, s, n_modes, T, w_infty, *, tol=1e-4, maxiter=None
     factored=True uploads A_ from its parts (one per-wavenumber system
     [L B; 0 0] and the wavenumbers, hymor_op_create_receptivity): B is never
     repeated on the device, and each action costs one flow SpMV per term.
-    krylov="device" and block as in TRANSIENT_GROWTH_DOWNSTREAM.
+    krylov="device" and block as in TRANSIENT_GROWTH_DOWNSTREAM (blocks of 4
+    by default: on the 500 x 250-cell operator extended by 32 wavenumbers
+    the Krylov stage takes 0.50 s against 0.69 s at 2 and 0.87 s at 1).
     """
     k = n_modes
     s = CFL_TIMESTEP(s)
