@@ -13,6 +13,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "hymor.h"
@@ -2091,12 +2092,21 @@ static void vjp_core(hymor_op *op, int order, int64_t nsub, double dt, int64_t c
         HIP_TRY(hipStreamSynchronize(st));
         return;
     }
-    ensure_ws(op, 1);
     const int64_t n = op->n;
     const int p = order;
     const int64_t K = std::min(ckpt_every, nsub);    // substeps per checkpoint
     const int64_t ng = (nsub + K - 1) / K;
     const size_t vb = (size_t)n * sizeof(double);
+    // a group's forward powers y_r = (dt A)^r x_i, r < p, run as p - 1
+    // batched SpMVs over its states (2..kVjpBatch of them; one launch per
+    // power instead of one per power and state); the workspace for that
+    // width is sized before any work is queued
+    constexpr int64_t kVjpBatch = 16;
+    const int wb = (K >= 2 && p >= 2) ? pad_width(std::min(K, kVjpBatch)) : 1;
+    ensure_ws(op, wb);
+    const int tile_b = tile_b_of(op, wb);
+    std::unique_ptr<DevTemp> d_yb;
+    if (wb > 1) d_yb.reset(new DevTemp((size_t)p * n * wb * sizeof(double)));
     DevTemp d_ck((size_t)ng * vb), d_grp((size_t)std::max<int64_t>(K - 1, 1) * vb);
     DevTemp d_U((size_t)p * K * vb), d_Z((size_t)p * K * vb);
     DevTemp d_Y((size_t)std::max(p - 1, 1) * vb), d_lam(vb), d_up(vb);
@@ -2129,20 +2139,47 @@ static void vjp_core(hymor_op *op, int order, int64_t nsub, double dt, int64_t c
             HIP_TRY(hipMemcpyAsync(dst, res, vb, hipMemcpyDeviceToDevice, st));
             xs[(size_t)i] = dst;
         }
+        const bool batched = wb > 1 && cnt >= 2 && cnt <= wb;
+        if (batched) {
+            // the states as columns, p - 1 batched SpMVs, then every z_m
+            double *yb = (double *)d_yb->p;
+            VecList xl{};
+            for (int64_t i = 0; i < cnt; ++i) xl.v[i] = xs[(size_t)i];
+            xl.cnt = (int)cnt;
+            const int64_t tot = n * wb;
+            const unsigned gb =
+                (unsigned)std::max<int64_t>(1, std::min<int64_t>((tot + 255) / 256, 65536));
+            hipLaunchKernelGGL(k_cols_to_batch, dim3(gb), dim3(256), 0, st, xl, (int)cnt, n, wb,
+                               tile_b, op->nflow, yb);
+            HIP_TRY(hipGetLastError());
+            BatchList yl{};
+            yl.v[0] = yb;
+            for (int r = 1; r < p; ++r) {
+                double *yr = yb + (size_t)r * n * wb;
+                launch_term(op, false, wb, st, yl.v[r - 1], yr, nullptr, nullptr, dt, 0.0);
+                yl.v[r] = yr;
+            }
+            for (int j = 0; j <= p; ++j) yl.c[j] = invf[j];
+            hipLaunchKernelGGL(k_vjp_z_batch, dim3(gb), dim3(256), 0, st, yl, p, (int)cnt, n, wb,
+                               tile_b, op->nflow, Z);
+            HIP_TRY(hipGetLastError());
+        }
         for (int64_t i = cnt - 1; i >= 0; --i) {
             double *Uk = U + (size_t)i * p * n, *Zk = Z + (size_t)i * p * n;
-            // y_r = (dt A)^r x_k, then the z_m
-            VecList ys{};
-            ys.v[0] = xs[(size_t)i];
-            for (int r = 1; r < p; ++r) {
-                double *yr = Y + (size_t)(r - 1) * n;
-                launch_term(op, false, 1, st, ys.v[r - 1], yr, nullptr, nullptr, dt, 0.0);
-                ys.v[r] = yr;
+            if (!batched) {
+                // y_r = (dt A)^r x_k, then the z_m
+                VecList ys{};
+                ys.v[0] = xs[(size_t)i];
+                for (int r = 1; r < p; ++r) {
+                    double *yr = Y + (size_t)(r - 1) * n;
+                    launch_term(op, false, 1, st, ys.v[r - 1], yr, nullptr, nullptr, dt, 0.0);
+                    ys.v[r] = yr;
+                }
+                for (int j = 0; j <= p; ++j) ys.c[j] = invf[j];
+                ys.cnt = p;
+                hipLaunchKernelGGL(k_vjp_z, dim3(gv), dim3(256), 0, st, ys, p, n, Zk);
+                HIP_TRY(hipGetLastError());
             }
-            for (int j = 0; j <= p; ++j) ys.c[j] = invf[j];
-            ys.cnt = p;
-            hipLaunchKernelGGL(k_vjp_z, dim3(gv), dim3(256), 0, st, ys, p, n, Zk);
-            HIP_TRY(hipGetLastError());
             // u_m = (dt A')^m lambda, m < p; u_p into up
             HIP_TRY(hipMemcpyAsync(Uk, lam, vb, hipMemcpyDeviceToDevice, st));
             for (int m = 1; m <= p; ++m) {

@@ -3030,6 +3030,44 @@ __global__ void k_vjp_z(VecList Y, int p, int64_t n, double *__restrict__ Z) {
     }
 }
 
+// Columns of a batched buffer (width w, layout buf_pos with tile_b) from the
+// cnt single vectors X.v[0..cnt) (cell-major, n doubles each); columns from
+// cnt on are zero padding.
+__global__ void k_cols_to_batch(VecList X, int cnt, int64_t n, int w, int tile_b,
+                                int64_t nflow, double *__restrict__ out) {
+    const int64_t total = n * w;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = i / w;
+        const int j = (int)(i - row * w);
+        out[buf_pos(row, j, w, tile_b, nflow)] = j < cnt ? X.v[j][row] : 0.0;
+    }
+}
+
+// k_vjp_z for cnt states at once: state i's y_r is column i of the batched
+// buffer Yb[r] (r < p, width w, layout buf_pos), and its z_m, m < p, go to
+// Z + (i p + m) n.  Coefficients c[j] = 1 / j! (terms added in ascending r,
+// as k_vjp_z adds them).
+struct BatchList {
+    const double *v[kMaxVjp];
+    double c[kMaxVjp];
+};
+__global__ void k_vjp_z_batch(BatchList Yb, int p, int cnt, int64_t n, int w, int tile_b,
+                              int64_t nflow, double *__restrict__ Z) {
+    const int64_t total = n * cnt;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = t / cnt;
+        const int i = (int)(t - row * cnt);
+        const int64_t o = buf_pos(row, i, w, tile_b, nflow);
+        for (int m = 0; m < p; ++m) {
+            double z = 0.0;
+            for (int r = 0; r + m < p; ++r) z = fma(Yb.c[m + r + 1], Yb.v[r][o], z);
+            Z[((int64_t)i * p + m) * n + row] = z;
+        }
+    }
+}
+
 // out = sum_{j < L.cnt} L.c[j] * L.v[j] (terms added in ascending j); out is
 // none of the inputs.
 __global__ void k_lincomb(VecList L, int64_t n, double *__restrict__ out) {
