@@ -29,7 +29,8 @@ the forward states: with w = T(A)^n P v,
 the pullback of the Taylor sweep for the cotangent 2 C w / (v' D v), which
 hymor_expm_taylor_vjp_values evaluates with checkpointed substeps (a state
 kept every ckpt_every substeps, each group recomputed from its checkpoint
-in the backward sweep).
+in the backward sweep; ckpt_every=1 keeps every state and recomputes none,
+the fastest form where nsub vectors fit in device memory).
 """
 from __future__ import annotations
 

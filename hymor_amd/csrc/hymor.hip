@@ -2097,18 +2097,24 @@ static void vjp_core(hymor_op *op, int order, int64_t nsub, double dt, int64_t c
     const int64_t K = std::min(ckpt_every, nsub);    // substeps per checkpoint
     const int64_t ng = (nsub + K - 1) / K;
     const size_t vb = (size_t)n * sizeof(double);
-    // a group's forward powers y_r = (dt A)^r x_i, r < p, run as p - 1
-    // batched SpMVs over its states (2..kVjpBatch of them; one launch per
-    // power instead of one per power and state); the workspace for that
-    // width is sized before any work is queued
+    // the backward sweep takes the states in segments: a checkpoint group,
+    // recomputed from its checkpoint, or, when every state is kept
+    // (ckpt_every 1), up to kVjpBatch consecutive kept states, recomputing
+    // nothing.  A segment's forward powers y_r = (dt A)^r x_i, r < p, run as
+    // p - 1 batched SpMVs over its states (one launch per power instead of
+    // one per power and state); the workspace for that width is sized
+    // before any work is queued
     constexpr int64_t kVjpBatch = 16;
-    const int wb = (K >= 2 && p >= 2) ? pad_width(std::min(K, kVjpBatch)) : 1;
+    const bool keep_all = K == 1 && nsub > 1;
+    const int64_t S = keep_all ? std::min(nsub, kVjpBatch) : K;   // states per segment
+    const int64_t nseg = (nsub + S - 1) / S;
+    const int wb = (S >= 2 && p >= 2) ? pad_width(std::min(S, kVjpBatch)) : 1;
     ensure_ws(op, wb);
     const int tile_b = tile_b_of(op, wb);
     std::unique_ptr<DevTemp> d_yb;
     if (wb > 1) d_yb.reset(new DevTemp((size_t)p * n * wb * sizeof(double)));
     DevTemp d_ck((size_t)ng * vb), d_grp((size_t)std::max<int64_t>(K - 1, 1) * vb);
-    DevTemp d_U((size_t)p * K * vb), d_Z((size_t)p * K * vb);
+    DevTemp d_U((size_t)p * S * vb), d_Z((size_t)p * S * vb);
     DevTemp d_Y((size_t)std::max(p - 1, 1) * vb), d_lam(vb), d_up(vb);
     double *ck = (double *)d_ck.p, *grp = (double *)d_grp.p, *U = (double *)d_U.p,
            *Z = (double *)d_Z.p, *Y = (double *)d_Y.p, *lam = (double *)d_lam.p,
@@ -2128,16 +2134,20 @@ static void vjp_core(hymor_op *op, int order, int64_t nsub, double dt, int64_t c
     // recomputed from its checkpoint, then lambda steps back through them
     launch_perm_in(op, st, d_wbar, n, lam, 1, 1, 0, 1);
     int init = 1;
-    for (int64_t g = ng - 1; g >= 0; --g) {
-        const int64_t cnt = std::min(K, nsub - g * K);
+    for (int64_t g = nseg - 1; g >= 0; --g) {
+        const int64_t cnt = std::min(S, nsub - g * S);
         std::vector<const double *> xs((size_t)cnt);
-        xs[0] = ck + g * n;
-        for (int64_t i = 1; i < cnt; ++i) {
-            const double *res =
-                run_taylor(op, false, p, 1, dt, HYMOR_FORM_HORNER, 1, xs[(size_t)i - 1]);
-            double *dst = grp + (i - 1) * n;
-            HIP_TRY(hipMemcpyAsync(dst, res, vb, hipMemcpyDeviceToDevice, st));
-            xs[(size_t)i] = dst;
+        if (keep_all) {
+            for (int64_t i = 0; i < cnt; ++i) xs[(size_t)i] = ck + (g * S + i) * n;
+        } else {
+            xs[0] = ck + g * n;
+            for (int64_t i = 1; i < cnt; ++i) {
+                const double *res =
+                    run_taylor(op, false, p, 1, dt, HYMOR_FORM_HORNER, 1, xs[(size_t)i - 1]);
+                double *dst = grp + (i - 1) * n;
+                HIP_TRY(hipMemcpyAsync(dst, res, vb, hipMemcpyDeviceToDevice, st));
+                xs[(size_t)i] = dst;
+            }
         }
         const bool batched = wb > 1 && cnt >= 2 && cnt <= wb;
         if (batched) {

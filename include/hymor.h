@@ -213,7 +213,10 @@ int hymor_expm_taylor_batched_coldiag_dev(hymor_op *op, int transpose, int order
  * every ckpt_every substeps and the backward sweep recomputes one group of
  * ckpt_every states at a time from its checkpoint, so beside the handle's
  * workspace the call holds ceil(nsub/ckpt_every) + (2*order + 1)*ckpt_every
- * + order vectors of n doubles and 16 bytes per pattern entry.
+ * + order vectors of n doubles and 16 bytes per pattern entry.  ckpt_every 1
+ * keeps every state and recomputes none (nsub + (2*order + 1)*16 + order
+ * vectors).  The entries' sums are grouped per segment of states, so
+ * gradients for different ckpt_every agree to rounding, not bit for bit.
  * Synchronous. */
 int hymor_expm_taylor_vjp_values(hymor_op *op, int order, int64_t nsub, double dt,
                                  int64_t ckpt_every, const double *x, const double *wbar,
