@@ -13,6 +13,8 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <algorithm>
+#include <climits>
 #include <memory>
 #include <vector>
 
@@ -2070,6 +2072,91 @@ static void pattern_cellmajor(const hymor_op *op, const int64_t *colptr, const i
     }
 }
 
+// A pattern mapped for the device: its entries as cell-major (row, column)
+// pairs and, for operators of block size 4 or 8, the same entries grouped by
+// b x b block for k_sddmm_blocks (nblk blocks: bcr, bslot), with the entries
+// no block takes (rer, rec, ridx) left to the per-entry kernel.
+struct hymor_pattern {
+    int64_t n = 0, b = 0, ncell = 0, nnz = 0;
+    int32_t *er = nullptr, *ec = nullptr;
+    int64_t nblk = 0, nres = 0;
+    int32_t *bcr = nullptr, *bslot = nullptr, *rer = nullptr, *rec = nullptr, *ridx = nullptr;
+};
+
+// Groups a cell-major pattern's flow entries by b x b block: blocks ordered
+// by the cell of their columns, then of their rows (bcr: the two cells;
+// slot: b*b entry positions, -1 for none).  A block is kept when it holds at
+// least b entries (reading its 2b values per column then costs no more than
+// the 2 per entry of the per-entry kernel); the entries of every other block,
+// those in extra-DOF rows or columns and a repeated (row, column) stay in
+// the per-entry list, in pattern order (rer, rec; ridx their positions).
+static void build_pattern_blocks(int b, int64_t ncell, int64_t nflow,
+                                 const std::vector<int32_t> &er, const std::vector<int32_t> &ec,
+                                 std::vector<int32_t> &bcr, std::vector<int32_t> &slot,
+                                 std::vector<int32_t> &rer, std::vector<int32_t> &rec,
+                                 std::vector<int32_t> &ridx) {
+    const int64_t nnz = (int64_t)er.size();
+    std::vector<int64_t> cptr((size_t)ncell + 1, 0);
+    std::vector<int32_t> res;
+    for (int64_t e = 0; e < nnz; ++e) {
+        if (er[(size_t)e] < nflow && ec[(size_t)e] < nflow)
+            ++cptr[(size_t)(ec[(size_t)e] / b) + 1];
+        else
+            res.push_back((int32_t)e);
+    }
+    for (int64_t c = 0; c < ncell; ++c) cptr[(size_t)c + 1] += cptr[(size_t)c];
+    std::vector<int32_t> ord((size_t)cptr[(size_t)ncell]);
+    {
+        std::vector<int64_t> pos(cptr.begin(), cptr.end() - 1);
+        for (int64_t e = 0; e < nnz; ++e)
+            if (er[(size_t)e] < nflow && ec[(size_t)e] < nflow)
+                ord[(size_t)pos[(size_t)(ec[(size_t)e] / b)]++] = (int32_t)e;
+    }
+    std::vector<std::pair<int32_t, int32_t>> buf;
+    std::vector<int32_t> tile((size_t)(b * b));
+    for (int64_t c = 0; c < ncell; ++c) {
+        buf.clear();
+        for (int64_t q = cptr[(size_t)c]; q < cptr[(size_t)c + 1]; ++q) {
+            const int32_t e = ord[(size_t)q];
+            buf.emplace_back(er[(size_t)e] / b, e);
+        }
+        std::sort(buf.begin(), buf.end());
+        for (size_t q0 = 0; q0 < buf.size();) {
+            size_t q1 = q0;
+            while (q1 < buf.size() && buf[q1].first == buf[q0].first) ++q1;
+            std::fill(tile.begin(), tile.end(), -1);
+            int held = 0;
+            for (size_t q = q0; q < q1; ++q) {
+                const int32_t e = buf[q].second;
+                int32_t &t = tile[(size_t)((er[(size_t)e] % b) * b + ec[(size_t)e] % b)];
+                if (t < 0) {
+                    t = e;
+                    ++held;
+                } else {
+                    res.push_back(e);
+                }
+            }
+            if (held >= b) {
+                bcr.push_back(buf[q0].first);
+                bcr.push_back((int32_t)c);
+                slot.insert(slot.end(), tile.begin(), tile.end());
+            } else {
+                for (int32_t e : tile)
+                    if (e >= 0) res.push_back(e);
+            }
+            q0 = q1;
+        }
+    }
+    std::sort(res.begin(), res.end());
+    rer.resize(res.size());
+    rec.resize(res.size());
+    ridx = res;
+    for (size_t q = 0; q < res.size(); ++q) {
+        rer[q] = er[(size_t)res[q]];
+        rec[q] = ec[(size_t)res[q]];
+    }
+}
+
 static void check_vjp_args(const hymor_op *op, int order, int64_t nsub, int64_t ckpt_every) {
     ARG_CHECK(op != nullptr, "null operator");
     ARG_CHECK(!op->is_complex, "the operator gradient is for real operators");
@@ -2081,10 +2168,13 @@ static void check_vjp_args(const hymor_op *op, int order, int64_t nsub, int64_t 
 
 // The gradient of <wbar, T_p(dt A)^nsub x> on the pattern (d_er, d_ec: nnz
 // cell-major pairs) into d_grad; x and wbar are var-major device vectors.
-// Synchronous (its buffers are released on return).
+// With `blocks` (a device pattern grouped by block), its blocks go through
+// k_sddmm_blocks and the rest through the per-entry kernel: the same sums,
+// the same bits.  Synchronous (its buffers are released on return).
 static void vjp_core(hymor_op *op, int order, int64_t nsub, double dt, int64_t ckpt_every,
                      const double *d_x, const double *d_wbar, const int32_t *d_er,
-                     const int32_t *d_ec, int64_t nnz, double *d_grad) {
+                     const int32_t *d_ec, int64_t nnz, double *d_grad,
+                     const hymor_pattern *blocks = nullptr) {
     hipStream_t st = op->stream;
     if (nnz == 0) return;
     if (nsub == 0) {   // w = x does not depend on A
@@ -2207,12 +2297,33 @@ static void vjp_core(hymor_op *op, int order, int64_t nsub, double dt, int64_t c
             hipLaunchKernelGGL(k_lincomb, dim3(gv), dim3(256), 0, st, us, n, lam);
             HIP_TRY(hipGetLastError());
         }
-        const unsigned ge = (unsigned)((nnz + 255) / 256);
-        hipLaunchKernelGGL(k_sddmm_pattern, dim3(ge), dim3(256), 0, st,
-                           d_er, d_ec, nnz,
-                           (const double *)U, (const double *)Z, n, (int)(cnt * p), dt,
-                           init, d_grad);
-        HIP_TRY(hipGetLastError());
+        const int kc = (int)(cnt * p);
+        if (blocks && blocks->nblk > 0) {
+            const unsigned gbk = (unsigned)((blocks->nblk + 255) / 256);
+            if (op->b == 4)
+                hipLaunchKernelGGL(k_sddmm_blocks<4>, dim3(gbk), dim3(256), 0, st, blocks->bcr,
+                                   blocks->bslot, blocks->nblk, (const double *)U,
+                                   (const double *)Z, n, kc, dt, init, d_grad);
+            else
+                hipLaunchKernelGGL(k_sddmm_blocks<8>, dim3(gbk), dim3(256), 0, st, blocks->bcr,
+                                   blocks->bslot, blocks->nblk, (const double *)U,
+                                   (const double *)Z, n, kc, dt, init, d_grad);
+            HIP_TRY(hipGetLastError());
+            if (blocks->nres > 0) {
+                const unsigned gr = (unsigned)((blocks->nres + 255) / 256);
+                hipLaunchKernelGGL(k_sddmm_pattern, dim3(gr), dim3(256), 0, st, blocks->rer,
+                                   blocks->rec, blocks->nres, (const double *)U,
+                                   (const double *)Z, n, kc, dt, init,
+                                   (const int32_t *)blocks->ridx, d_grad);
+                HIP_TRY(hipGetLastError());
+            }
+        } else {
+            const unsigned ge = (unsigned)((nnz + 255) / 256);
+            hipLaunchKernelGGL(k_sddmm_pattern, dim3(ge), dim3(256), 0, st, d_er, d_ec, nnz,
+                               (const double *)U, (const double *)Z, n, kc, dt, init,
+                               (const int32_t *)nullptr, d_grad);
+            HIP_TRY(hipGetLastError());
+        }
         init = 0;
     }
     HIP_TRY(hipStreamSynchronize(st));
@@ -2244,11 +2355,6 @@ int hymor_expm_taylor_vjp_values(hymor_op *op, int order, int64_t nsub, double d
     });
 }
 
-struct hymor_pattern {
-    int64_t n = 0, b = 0, ncell = 0, nnz = 0;
-    int32_t *er = nullptr, *ec = nullptr;
-};
-
 int hymor_pattern_create(const hymor_op *op, const int64_t *colptr, const int64_t *rowval,
                          int index_base, hymor_pattern **out) {
     hymor_pattern *pat = nullptr;
@@ -2262,17 +2368,30 @@ int hymor_pattern_create(const hymor_op *op, const int64_t *colptr, const int64_
         pat->b = op->b;
         pat->ncell = op->ncell;
         pat->nnz = (int64_t)er.size();
-        if (pat->nnz) {
-            HIP_TRY(hipMalloc((void **)&pat->er, (size_t)pat->nnz * 4));
-            HIP_TRY(hipMalloc((void **)&pat->ec, (size_t)pat->nnz * 4));
-            HIP_TRY(hipMemcpy(pat->er, er.data(), (size_t)pat->nnz * 4, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(pat->ec, ec.data(), (size_t)pat->nnz * 4, hipMemcpyHostToDevice));
+        auto up = [](int32_t **d, const std::vector<int32_t> &h) {
+            if (h.empty()) return;
+            HIP_TRY(hipMalloc((void **)d, h.size() * 4));
+            HIP_TRY(hipMemcpy(*d, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+        };
+        up(&pat->er, er);
+        up(&pat->ec, ec);
+        if ((op->b == 4 || op->b == 8) && pat->nnz > 0 && pat->nnz <= INT32_MAX) {
+            std::vector<int32_t> bcr, bslot, rer, rec, ridx;
+            build_pattern_blocks(op->b, op->ncell, op->nflow, er, ec, bcr, bslot, rer, rec,
+                                 ridx);
+            pat->nblk = (int64_t)bcr.size() / 2;
+            pat->nres = (int64_t)ridx.size();
+            up(&pat->bcr, bcr);
+            up(&pat->bslot, bslot);
+            up(&pat->rer, rer);
+            up(&pat->rec, rec);
+            up(&pat->ridx, ridx);
         }
         *out = pat;
     });
     if (rc != HYMOR_OK && pat) {
-        (void)hipFree(pat->er);
-        (void)hipFree(pat->ec);
+        for (int32_t *d : {pat->er, pat->ec, pat->bcr, pat->bslot, pat->rer, pat->rec, pat->ridx})
+            (void)hipFree(d);
         delete pat;
     }
     return rc;
@@ -2281,8 +2400,8 @@ int hymor_pattern_create(const hymor_op *op, const int64_t *colptr, const int64_
 int hymor_pattern_destroy(hymor_pattern *pat) {
     return guarded([&] {
         if (!pat) return;
-        if (pat->er) HIP_TRY(hipFree(pat->er));
-        if (pat->ec) HIP_TRY(hipFree(pat->ec));
+        for (int32_t *d : {pat->er, pat->ec, pat->bcr, pat->bslot, pat->rer, pat->rec, pat->ridx})
+            if (d) HIP_TRY(hipFree(d));
         delete pat;
     });
 }
@@ -2304,7 +2423,7 @@ int hymor_expm_taylor_vjp_values_dev(hymor_op *op, const hymor_pattern *pat, int
                   "the pattern was made for an operator of another shape");
         ARG_CHECK(d_x && d_wbar && (d_grad || pat->nnz == 0), "null argument");
         vjp_core(op, order, nsub, dt, ckpt_every, d_x, d_wbar, pat->er, pat->ec, pat->nnz,
-                 d_grad);
+                 d_grad, pat);
     });
 }
 

@@ -3084,14 +3084,16 @@ __global__ void k_lincomb(VecList L, int64_t n, double *__restrict__ out) {
 // U[k n + er[e]] * Z[k n + ec[e]] (U, Z column-major n x kc; the products
 // added in ascending k).  A thread per entry; the workgroups take
 // contiguous ranges of entries per XCD, so the U and Z rows that
-// neighbouring entries share stay in one L2.
+// neighbouring entries share stay in one L2.  With eidx, list entry i is
+// the pattern's entry eidx[i] (the entries k_sddmm_blocks leaves).
 __global__ __launch_bounds__(256) void k_sddmm_pattern(
     const int32_t *__restrict__ er, const int32_t *__restrict__ ec, int64_t nnz,
     const double *__restrict__ U, const double *__restrict__ Z, int64_t n, int kc, double alpha,
-    int init, double *__restrict__ grad) {
-    const int64_t e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (e >= nnz) return;
-    const int64_t r = er[e], c = ec[e];
+    int init, const int32_t *__restrict__ eidx, double *__restrict__ grad) {
+    const int64_t i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    const int64_t r = er[i], c = ec[i];
+    const int64_t e = eidx ? (int64_t)eidx[i] : i;
     double acc = 0.0;
     int k = 0;
     for (; k + 4 <= kc; k += 4) {
@@ -3106,6 +3108,50 @@ __global__ __launch_bounds__(256) void k_sddmm_pattern(
     }
     for (; k < kc; ++k) acc = fma(U[(int64_t)k * n + r], Z[(int64_t)k * n + c], acc);
     grad[e] = (init ? 0.0 : grad[e]) + alpha * acc;
+}
+
+// The sampled product over the b x b blocks of a pattern (B = b): thread t
+// takes block t, whose rows are those of cell bcr[2t] and columns those of
+// cell bcr[2t + 1] (cell-major: B consecutive indices each), and whose
+// entries are slot[t B^2 + i B + j] for row i, column j of the block (-1:
+// none).  Each slot gets the sum k_sddmm_pattern forms for it -- the same
+// products, added in ascending k -- with the block's B values of U and B of
+// Z per column k read once for its up to B^2 entries instead of once per
+// entry.
+template <int B>
+__global__ __launch_bounds__(256) void k_sddmm_blocks(
+    const int32_t *__restrict__ bcr, const int32_t *__restrict__ slot, int64_t nblk,
+    const double *__restrict__ U, const double *__restrict__ Z, int64_t n, int kc, double alpha,
+    int init, double *__restrict__ grad) {
+    const int64_t t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= nblk) return;
+    const int64_t rb = (int64_t)bcr[2 * t] * B, cb = (int64_t)bcr[2 * t + 1] * B;
+    double acc[B][B];
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+        for (int j = 0; j < B; ++j) acc[i][j] = 0.0;
+    for (int k = 0; k < kc; ++k) {
+        const double *u = U + (int64_t)k * n + rb, *z = Z + (int64_t)k * n + cb;
+        double uv[B], zv[B];
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            uv[i] = u[i];
+            zv[i] = z[i];
+        }
+#pragma unroll
+        for (int i = 0; i < B; ++i)
+#pragma unroll
+            for (int j = 0; j < B; ++j) acc[i][j] = fma(uv[i], zv[j], acc[i][j]);
+    }
+    const int32_t *sl = slot + t * (B * B);
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const int32_t e = sl[i * B + j];
+            if (e >= 0) grad[e] = (init ? 0.0 : grad[e]) + alpha * acc[i][j];
+        }
 }
 
 }  // namespace hymor

@@ -225,7 +225,9 @@ int hymor_expm_taylor_vjp_values(hymor_op *op, int order, int64_t nsub, double d
 
 /* A gradient pattern kept on the device: the CSC pattern (colptr, rowval,
  * index_base as above) of an operator of op's shape, mapped to the device
- * layout once (8 bytes per entry), for callers that differentiate on the
+ * layout once (8 bytes per entry; for block sizes 4 and 8 also grouped by
+ * block for the sampled product: 8 + 4*b*b bytes per block of at least b
+ * entries, 12 per other entry), for callers that differentiate on the
  * same pattern repeatedly (an optimisation loop over the Jacobian).  No
  * reference counterpart. */
 int hymor_pattern_create(const hymor_op *op, const int64_t *colptr, const int64_t *rowval,

@@ -222,3 +222,40 @@ def test_vjp_values_on_a_device_pattern(orc):
             assert relerr(dg.download(), g) <= 1e-12
         finally:
             dg.free()
+
+@pytest.mark.parametrize("b,n_shock", [(8, 0), (4, 9)])
+def test_device_pattern_blocks_give_the_per_entry_bits(orc, b, n_shock):
+    """A device pattern groups its flow entries by b x b block (b 4 and 8);
+    entries of blocks holding fewer than b, those touching extra DOFs and a
+    repeated (row, column) stay per entry.  The gradient equals the host
+    entry point's (every entry per entry) bit for bit, and the oracle's."""
+    from hymor_amd.core import DevicePattern, DeviceVector
+    op = ops.stencil_operator(9, 6, b, 71, n_shock=n_shock, row_sum=2.0)
+    A = op.to_scipy_varmajor()
+    colptr, rowval = _pattern(A, 40, 11)
+    # repeat the first entry of column 3
+    j = 3
+    e0 = colptr[j] - 1
+    rowval = np.insert(rowval, e0, rowval[e0])
+    colptr = colptr.copy()
+    colptr[j + 1:] += 1
+    say(f"[operator gradient] block map b={b} n_shock={n_shock}")
+    r = np.random.default_rng(b + n_shock)
+    x, wbar = r.standard_normal(op.n), r.standard_normal(op.n)
+    with DeviceOperator.from_blocks(op) as dev, DevicePattern(dev, colptr, rowval) as pat:
+        want = dev.expm_taylor_vjp_values(x, wbar, colptr, rowval, order=4, nsub=5, dt=0.05,
+                                          ckpt_every=2)
+        dx, dw = DeviceVector.from_host(x), DeviceVector.from_host(wbar)
+        dg = DeviceVector(pat.nnz)
+        try:
+            dev.expm_taylor_vjp_values_dev(pat, dx, dw, dg, order=4, nsub=5, dt=0.05,
+                                           ckpt_every=2)
+            got = dg.download()
+        finally:
+            for v in (dx, dw, dg):
+                v.free()
+    np.testing.assert_array_equal(got, want)
+    assert got[e0] == got[e0 + 1]
+    Ao = orc.CSC.from_scipy(A).scaled(0.05)
+    ref = 0.05 * orc.taylor_vjp_values(Ao, x, wbar, 5, 4, colptr, rowval)
+    assert relerr(got, ref) <= 1e-12
